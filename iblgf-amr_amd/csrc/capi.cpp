@@ -1,0 +1,351 @@
+// C ABI of libiblgf_gpu.so: status codes around the host octree, the kernel
+// tables and the device solver.  Declarations and reference citations are in
+// include/iblgf_gpu.h.
+#include "../../include/iblgf_gpu.h"
+
+#include "gpu_solver.hpp"
+#include "host/octree.hpp"
+#include "host/tables.hpp"
+
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
+using namespace iblgf_amd;
+
+struct iblgf_tree
+{
+    Octree tree;
+    bool   masks_built = false;
+};
+
+struct iblgf_gpu_ctx
+{
+    std::unique_ptr<GpuSolver> solver;
+    std::string                error;
+};
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg, iblgf_gpu_ctx* ctx = nullptr)
+{
+    g_last_error = msg;
+    if (ctx) ctx->error = msg;
+    return code;
+}
+
+template<class F>
+int guarded(iblgf_gpu_ctx* ctx, F&& f)
+{
+    try
+    {
+        f();
+        return IBLGF_OK;
+    }
+    catch (const std::invalid_argument& e)
+    {
+        return fail(IBLGF_ERR_ARG, e.what(), ctx);
+    }
+    catch (const std::runtime_error& e)
+    {
+        const std::string m = e.what();
+        if (m.rfind("HIP error", 0) == 0) return fail(IBLGF_ERR_DEVICE, m, ctx);
+        if (m.find("unsupported") != std::string::npos) return fail(IBLGF_ERR_UNSUPPORTED, m, ctx);
+        return fail(IBLGF_ERR_STATE, m, ctx);
+    }
+    catch (const std::exception& e)
+    {
+        return fail(IBLGF_ERR_INTERNAL, e.what(), ctx);
+    }
+    catch (...)
+    {
+        return fail(IBLGF_ERR_INTERNAL, "unknown error", ctx);
+    }
+}
+
+const LaplaceLgfTable& shared_table()
+{
+    static std::once_flag  once;
+    static LaplaceLgfTable tab;
+    std::call_once(once, [] { tab.build(100); });
+    return tab;
+}
+}  // namespace
+
+extern "C" {
+
+const char* iblgf_last_error(void) { return g_last_error.c_str(); }
+
+int iblgf_tree_create(const int bd_base[3], const int bd_extent[3], int block_extent, double dx_base, int nregions,
+    const int* regions, iblgf_tree** out)
+{
+    if (!bd_base || !bd_extent || !out || (nregions > 0 && !regions) || block_extent <= 0)
+        return fail(IBLGF_ERR_ARG, "iblgf_tree_create: invalid argument");
+    return guarded(nullptr, [&] {
+        DomainSpec spec;
+        for (int d = 0; d < 3; ++d)
+        {
+            spec.bd_base[d] = bd_base[d];
+            spec.bd_extent[d] = bd_extent[d];
+            if (bd_extent[d] % block_extent) throw std::invalid_argument("bounding box not divisible by block extent");
+        }
+        spec.block_extent = block_extent;
+        spec.dx_base = dx_base;
+        for (int r = 0; r < nregions; ++r)
+        {
+            std::array<int, 6> a;
+            for (int k = 0; k < 6; ++k) a[k] = regions[6 * r + k];
+            spec.regions.push_back(a);
+        }
+        auto t = std::make_unique<iblgf_tree>();
+        t->tree.build_base_mesh(spec);
+        t->tree.construct_level_maps();
+        t->tree.construct_leaf_maps(false);
+        t->tree.construct_lists();
+        *out = t.release();
+    });
+}
+
+int iblgf_tree_init_refine(iblgf_tree* t, int nref, int level_up_max, iblgf_refine_fn fn, void* user)
+{
+    if (!t || nref < 0 || level_up_max < 0 || (nref > 0 && !fn)) return fail(IBLGF_ERR_ARG, "iblgf_tree_init_refine: invalid argument");
+    return guarded(nullptr, [&] {
+        RefineCond cond = [&](const Octree& tr, int o, int diff) {
+            int b[3];
+            tr.block_base(o, b);
+            return fn(user, b, tr.block_extent, tr.refinement_level(o), diff) != 0;
+        };
+        t->tree.init_refine(nref, level_up_max, cond);
+        t->masks_built = false;
+    });
+}
+
+int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf)
+{
+    if (!t) return fail(IBLGF_ERR_ARG, "iblgf_tree_build_masks: null tree");
+    return guarded(nullptr, [&] {
+        t->tree.fmm_lgf_mask_build(subtract_non_leaf != 0);
+        t->tree.fmm_ib_masks();
+        t->masks_built = true;
+    });
+}
+
+int iblgf_tree_info(const iblgf_tree* t, int* nblocks, int* base_level, int* depth, int* block_extent,
+    double* dx_base, int* nleaves)
+{
+    if (!t) return fail(IBLGF_ERR_ARG, "iblgf_tree_info: null tree");
+    int nb = 0, nl = 0;
+    for (const auto& lm : t->tree.level_map)
+        for (int o : lm)
+        {
+            ++nb;
+            if (t->tree.oct[o].leaf) ++nl;
+        }
+    if (nblocks) *nblocks = nb;
+    if (base_level) *base_level = t->tree.base_level;
+    if (depth) *depth = t->tree.depth;
+    if (block_extent) *block_extent = t->tree.block_extent;
+    if (dx_base) *dx_base = t->tree.dx_base;
+    if (nleaves) *nleaves = nl;
+    return IBLGF_OK;
+}
+
+int iblgf_tree_export(const iblgf_tree* t, uint64_t* key, int* level, int* coord, int* bbase, uint32_t* flags,
+    int* parent, int* children, int* neighbor, int* influence, int* influence_num, uint64_t* mask_src,
+    uint64_t* mask_tgt)
+{
+    if (!t) return fail(IBLGF_ERR_ARG, "iblgf_tree_export: null tree");
+    return guarded(nullptr, [&] {
+        const Octree&    tr = t->tree;
+        std::vector<int> slot(tr.oct.size(), -1), order;
+        for (const auto& lm : tr.level_map)
+            for (int o : lm)
+            {
+                slot[o] = static_cast<int>(order.size());
+                order.push_back(o);
+            }
+        auto sl = [&](int o) { return o < 0 ? -1 : slot[o]; };
+        for (size_t s = 0; s < order.size(); ++s)
+        {
+            const Octant& o = tr.oct[order[s]];
+            if (key) key[s] = o.key;
+            if (level) level[s] = o.level;
+            int b[3];
+            tr.block_base(order[s], b);
+            for (int d = 0; d < 3; ++d)
+            {
+                if (coord) coord[3 * s + d] = o.coord[d];
+                if (bbase) bbase[3 * s + d] = b[d];
+            }
+            if (flags)
+                flags[s] = (o.leaf ? 1u : 0u) | (o.correction ? 2u : 0u) | (o.leaf_boundary ? 4u : 0u) |
+                           (o.physical ? 8u : 0u);
+            if (parent) parent[s] = (o.parent >= 0 && tr.oct[o.parent].has_data) ? sl(o.parent) : -1;
+            if (children)
+                for (int c = 0; c < 8; ++c)
+                    children[8 * s + c] = (o.children[c] >= 0 && tr.oct[o.children[c]].has_data) ? sl(o.children[c]) : -1;
+            if (neighbor)
+                for (int i = 0; i < 27; ++i) neighbor[27 * s + i] = sl(o.neighbor[i]);
+            if (influence)
+                for (int i = 0; i < 189; ++i) influence[189 * s + i] = i < o.influence_num ? sl(o.influence[i]) : -1;
+            if (influence_num) influence_num[s] = o.influence_num;
+            if (mask_src) mask_src[s] = o.mask_src;
+            if (mask_tgt) mask_tgt[s] = o.mask_tgt;
+        }
+    });
+}
+
+void iblgf_tree_destroy(iblgf_tree* t) { delete t; }
+
+uint64_t iblgf_key_make(int x, int y, int z, int level) { return Morton::make(x, y, z, level); }
+int      iblgf_key_coord(uint64_t key, int* xyz)
+{
+    if (!xyz) return -1;
+    Morton::coord(key, xyz);
+    return Morton::level(key);
+}
+uint64_t iblgf_key_parent(uint64_t key) { return Morton::parent(key); }
+uint64_t iblgf_key_child(uint64_t key, int i) { return Morton::child(key, i); }
+int      iblgf_key_influence(uint64_t key, uint64_t* out) { return out ? Morton::influence_keys(key, out) : -1; }
+
+int iblgf_lgf_table(int nmax, double* g)
+{
+    if (nmax < 0 || !g) return fail(IBLGF_ERR_ARG, "iblgf_lgf_table: invalid argument");
+    return guarded(nullptr, [&] {
+        LaplaceLgfTable tab;
+        tab.build(nmax);
+        std::copy(tab.g.begin(), tab.g.end(), g);
+    });
+}
+
+double iblgf_lgf_get(int x, int y, int z) { return shared_table().get(x, y, z); }
+double iblgf_if_kernel(int x, int y, int z, double alpha) { return if_kernel(x, y, z, alpha); }
+
+int iblgf_lagrange_matrix(int n, int pts_cap, int relative_pos, double* out)
+{
+    if (n < 2 || pts_cap < 1 || !out || (relative_pos != 0 && relative_pos != 1))
+        return fail(IBLGF_ERR_ARG, "iblgf_lagrange_matrix: invalid argument");
+    const auto A = relative_pos == 0 && pts_cap == 10 ? nli_matrix(n, 10) : cc_interp_matrix(n, pts_cap, relative_pos);
+    std::copy(A.begin(), A.end(), out);
+    return IBLGF_OK;
+}
+
+int iblgf_gpu_create(int device, iblgf_gpu_ctx** out)
+{
+    if (!out) return fail(IBLGF_ERR_ARG, "iblgf_gpu_create: null output");
+    auto ctx = std::make_unique<iblgf_gpu_ctx>();
+    const int rc = guarded(ctx.get(), [&] { ctx->solver = std::make_unique<GpuSolver>(device); });
+    if (rc == IBLGF_OK) *out = ctx.release();
+    return rc;
+}
+
+int iblgf_gpu_destroy(iblgf_gpu_ctx* ctx)
+{
+    if (!ctx) return fail(IBLGF_ERR_ARG, "iblgf_gpu_destroy: null context");
+    const int rc = guarded(ctx, [&] { ctx->solver.reset(); });
+    delete ctx;
+    return rc;
+}
+
+const char* iblgf_gpu_last_error(const iblgf_gpu_ctx* ctx) { return ctx ? ctx->error.c_str() : g_last_error.c_str(); }
+
+#define NEED_CTX(name)                                                         \
+    if (!ctx || !ctx->solver) return fail(IBLGF_ERR_ARG, name ": null context")
+
+int iblgf_gpu_tree_upload(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
+{
+    NEED_CTX("iblgf_gpu_tree_upload");
+    if (!t) return fail(IBLGF_ERR_ARG, "iblgf_gpu_tree_upload: null tree", ctx);
+    if (!t->masks_built) return fail(IBLGF_ERR_STATE, "iblgf_gpu_tree_upload: masks not built", ctx);
+    return guarded(ctx, [&] { ctx->solver->upload_tree(t->tree); });
+}
+
+int iblgf_gpu_field_create(iblgf_gpu_ctx* ctx, int* field_id)
+{
+    NEED_CTX("iblgf_gpu_field_create");
+    if (!field_id) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_create: null output", ctx);
+    return guarded(ctx, [&] {
+        if (ctx->solver->nblocks() == 0) throw std::runtime_error("tree not uploaded");
+        *field_id = ctx->solver->field_create();
+    });
+}
+
+int iblgf_gpu_field_device_ptr(iblgf_gpu_ctx* ctx, int field_id, void** dptr)
+{
+    NEED_CTX("iblgf_gpu_field_device_ptr");
+    if (!dptr) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_device_ptr: null output", ctx);
+    return guarded(ctx, [&] { *dptr = ctx->solver->field_ptr(field_id); });
+}
+
+int iblgf_gpu_field_upload(iblgf_gpu_ctx* ctx, int field_id, const double* host)
+{
+    NEED_CTX("iblgf_gpu_field_upload");
+    if (!host) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_upload: null host pointer", ctx);
+    return guarded(ctx, [&] { ctx->solver->field_upload(field_id, host); });
+}
+
+int iblgf_gpu_field_download(iblgf_gpu_ctx* ctx, int field_id, double* host)
+{
+    NEED_CTX("iblgf_gpu_field_download");
+    if (!host) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_download: null host pointer", ctx);
+    return guarded(ctx, [&] { ctx->solver->field_download(field_id, host); });
+}
+
+int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base)
+{
+    NEED_CTX("iblgf_gpu_apply_lgf");
+    if (kernel != IBLGF_KERNEL_LAPLACE && kernel != IBLGF_KERNEL_IF)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf: unknown kernel", ctx);
+    if (src_field == dst_field) return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf: src and dst must differ", ctx);
+    return guarded(ctx, [&] {
+        ctx->solver->field_ptr(src_field);
+        ctx->solver->field_ptr(dst_field);
+        if (kernel == IBLGF_KERNEL_LAPLACE) ctx->solver->apply_lgf(src_field, dst_field);
+        else
+            ctx->solver->apply_if(src_field, dst_field, alpha_base);
+    });
+}
+
+int iblgf_gpu_prepare(iblgf_gpu_ctx* ctx, int kernel, double alpha_base)
+{
+    NEED_CTX("iblgf_gpu_prepare");
+    return guarded(ctx, [&] {
+        if (kernel == IBLGF_KERNEL_LAPLACE) ctx->solver->prepare_laplace();
+        else if (kernel == IBLGF_KERNEL_IF)
+            ctx->solver->prepare_if(alpha_base);
+        else
+            throw std::invalid_argument("unknown kernel");
+    });
+}
+
+int iblgf_gpu_sync(iblgf_gpu_ctx* ctx)
+{
+    NEED_CTX("iblgf_gpu_sync");
+    return guarded(ctx, [&] { ctx->solver->sync(); });
+}
+
+int iblgf_gpu_stream(iblgf_gpu_ctx* ctx, void** stream)
+{
+    NEED_CTX("iblgf_gpu_stream");
+    if (!stream) return fail(IBLGF_ERR_ARG, "iblgf_gpu_stream: null output", ctx);
+    *stream = reinterpret_cast<void*>(ctx->solver->stream());
+    return IBLGF_OK;
+}
+
+int iblgf_gpu_counts(iblgf_gpu_ctx* ctx, long* c)
+{
+    NEED_CTX("iblgf_gpu_counts");
+    if (!c) return fail(IBLGF_ERR_ARG, "iblgf_gpu_counts: null output", ctx);
+    const SolveCounts s = ctx->solver->lgf_counts();
+    c[0] = s.passes;
+    c[1] = s.n_src;
+    c[2] = s.n_tgt;
+    c[3] = s.n_pair;
+    c[4] = s.n_transfer;
+    c[5] = s.n_leaf;
+    return IBLGF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,166 @@
+// Device mirror of the block octree and the driver of the LGF solves.
+//
+// GpuSolver keeps the reference's solver entry points, on one GPU with the
+// tree resident:
+//   apply_lgf  PoissonSolver::apply_lgf (include/iblgf/solver/poisson/poisson.hpp:587-894)
+//   apply_if   PoissonSolver::apply_lgf_IF / apply_if (poisson.hpp:427-494)
+//   fmm_pass   Fmm::apply (include/iblgf/fmm/fmm.hpp:729-870)
+// The work of one FMM pass (block lists, sibling groups of the spectral
+// multiply-accumulate) is built once from the octree at upload and stays on
+// the device.
+#pragma once
+
+#include "host/octree.hpp"
+#include "kernels.hpp"
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace iblgf_amd {
+
+template<class T>
+struct DevArray
+{
+    T*     p = nullptr;
+    size_t n = 0;
+    DevArray() = default;
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    DevArray(DevArray&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevArray& operator=(DevArray&& o) noexcept
+    {
+        if (this != &o)
+        {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DevArray() { release(); }
+    void release();
+    void alloc(size_t count);
+    void upload(const std::vector<T>& v);
+    int  size() const { return static_cast<int>(n); }
+};
+
+struct MacLevelPlan
+{
+    int                  lambda = 0;  // level of the targets
+    int                  d = 0;       // level difference of the kernel
+    DevArray<MacGroup>   groups;
+    DevArray<MacSrc>     srcs;
+    long                 npairs = 0;
+};
+
+struct PassPlan
+{
+    int                              L = 0, nl = 0, idx = 0;
+    bool                             neighbor_only = false;
+    DevArray<int>                    zero_s, zero_t, copy_s, fwd, inv, add;
+    std::vector<DevArray<int>>       antrp;  // by level (parents)
+    std::vector<DevArray<int>>       intrp;  // by level (parents at level)
+    std::vector<MacLevelPlan>        mac;
+    long                             npairs = 0;
+    long                             ntransfer = 0;  // parent-child transfers (anterp + interp)
+};
+
+struct SolveCounts
+{
+    long passes = 0, n_src = 0, n_tgt = 0, n_pair = 0, n_transfer = 0, n_leaf = 0;
+};
+
+class GpuSolver
+{
+  public:
+    explicit GpuSolver(int device);
+    ~GpuSolver();
+
+    void upload_tree(const Octree& t);
+
+    int     field_create();
+    double* field_ptr(int id);
+    void    field_upload(int id, const double* host);
+    void    field_download(int id, double* host);
+    size_t  field_size() const { return static_cast<size_t>(nb_) * n3_; }
+
+    // Laplace solve of one scalar component (AMR2AMR, use_correction,
+    // subtract_non_leaf = false).  dst is written on leaf blocks only.
+    void apply_lgf(int src, int dst);
+    // Integrating factor e^{alpha Lap} of one scalar component; dst is
+    // written on every block with data.
+    void apply_if(int src, int dst, double alpha_base);
+    // Build the kernel spectra that a solve will use (cold-path work).
+    void prepare_laplace();
+    void prepare_if(double alpha_base);
+
+    void        sync();
+    hipStream_t stream() const { return stream_; }
+    int         nblocks() const { return nb_; }
+    int         block_n() const { return n_; }
+    SolveCounts lgf_counts() const;
+    int         num_leaves() const { return n_leaf_; }
+
+  private:
+    void      build_pass(PassPlan& p, int L, int nl, bool neighbor_only);
+    PassPlan& pass(int L, int nl, bool neighbor_only);
+    void      run_pass(PassPlan& p, int src_field, int dst_field, double scale, bool laplace, double alpha);
+    double2*  ghat_laplace(int d);
+    double2*  ghat_if(double alpha);
+    void      fill_zero(double* f);
+    double*   F(int id) { return fields_.at(id).p; }
+
+    int         device_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool        uploaded_ = false;
+
+    // tree mirror (host copies)
+    Octree                  tree_;
+    int                     nb_ = 0, n_ = 0, e_ = 0, n3_ = 0, nf_ = 0;
+    int                     base_level_ = 0, depth_ = 0, n_leaf_ = 0;
+    double                  dx_base_ = 1.0;
+    std::vector<int>        slot_of_;   // octant index -> slot
+    std::vector<int>        oct_of_;    // slot -> octant index
+    std::vector<int>        lvl_begin_;
+    std::vector<uint64_t>   key_;
+    std::vector<int>        level_, coord_, bbase_, parent_, children_, neighbor_;
+    std::vector<uint32_t>   flags_;
+    std::vector<uint64_t>   msrc_, mtgt_;
+    std::vector<int>        infl_, infl_num_;
+    DevArray<int>           d_neighbor_;
+
+    // transfer matrices [2][n*n]
+    DevArray<double> m_nli_anterp_, m_nli_interp_, m_cc_restrict_, m_cc_interp_;
+    int              intrp_order_ = 5;
+
+    // fields
+    std::vector<DevArray<double>> fields_;
+    int f_src_tmp_ = -1, f_tgt_tmp_ = -1, f_corr_tmp_ = -1, f_fmm_s_ = -1, f_fmm_t_ = -1;
+    DevArray<double2> sspec_, tspec_;
+
+    // kernel spectra
+    DevArray<double>                         d_lgf_table_;
+    int                                      lgf_nmax_ = 100;
+    std::map<int, DevArray<double2>>         ghat_lap_;
+    std::map<double, DevArray<double2>>      ghat_if_;
+    DevArray<double>                         gen_sample_;
+    DevArray<double2>                        gen_a_, gen_b_;
+    DevArray<double>                         d_iv_;
+
+    // plans
+    std::map<int, std::unique_ptr<PassPlan>> passes_;
+    DevArray<int>                            p_leaves_;
+    std::vector<DevArray<int>>               p_coarsify_;    // by level: parent + 8 children
+    std::vector<DevArray<int>>               p_level_slots_; // by level
+    std::vector<DevArray<int>>               p_leaf_level_;  // leaves by level
+    std::vector<DevArray<int>>               p_ccint_;       // children of level-l octants
+    std::vector<DevArray<int>>               p_ccint_corr_;  // correction children only
+    std::vector<DevArray<int>>               p_kids_;        // child slots of level-l octants
+    DevArray<int>                            p_all_;
+};
+
+}  // namespace iblgf_amd

@@ -1,0 +1,703 @@
+// Host octree: construction, AMR refinement, lists and FMM masks.
+// See octree.hpp for the reference file:line map.
+#include "octree.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+
+namespace iblgf_amd {
+
+int fmm_mask_idx_gen(int type, int refinement_level, int non_leaf_as_source)
+{
+    // include/iblgf/domain/octree/octant.hpp:94-131
+    switch (type)
+    {
+        case MASK_STREAM: return 0;
+        case MASK_IB2xIB: return 1;
+        case MASK_xIB2IB: return 2;
+        case MASK_Laplacian_BC: return 3;
+        case MASK_IB2AMR: return refinement_level + 4;
+        case MASK_AMR2AMR: return refinement_level * 2 + non_leaf_as_source + 15;
+        case MASK_RefFourierStream: return refinement_level + 30;
+        case MASK_RefFourierLGF: return refinement_level + 40;
+        default: throw std::runtime_error("Wrong mask Idx");
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Construction
+// ---------------------------------------------------------------------------
+int Octree::insert(uint64_t k)
+{
+    if (oct.empty())
+    {
+        Octant root;
+        root.key = Morton::make(0, 0, 0, 0);
+        root.level = 0;
+        std::fill(root.neighbor, root.neighbor + 27, -1);
+        std::fill(root.influence, root.influence + 189, -1);
+        oct.push_back(root);
+        index[root.key] = 0;
+    }
+    int       n = 0;
+    const int L = Morton::level(k);
+    for (int l = 1; l <= L; ++l)
+    {
+        const int cn = Morton::child_number(k, l);
+        int       c = oct[n].children[cn];
+        if (c < 0) c = refine_child(n, cn);
+        n = c;
+    }
+    if (L + 1 > depth) depth = L + 1;
+    return n;
+}
+
+int Octree::refine_child(int o, int i)
+{
+    if (oct[o].children[i] >= 0) return oct[o].children[i];
+    Octant c;
+    c.key = Morton::child(oct[o].key, i);
+    c.level = oct[o].level + 1;
+    Morton::coord(c.key, c.coord);
+    c.parent = o;
+    std::fill(c.neighbor, c.neighbor + 27, -1);
+    std::fill(c.influence, c.influence + 189, -1);
+    const int id = static_cast<int>(oct.size());
+    oct.push_back(c);
+    oct[o].children[i] = id;
+    index[c.key] = id;
+    return id;
+}
+
+void Octree::build_base_mesh(const DomainSpec& spec)
+{
+    // include/iblgf/domain/domain.hpp:166-211, 406-487
+    block_extent = spec.block_extent;
+    dx_base = spec.dx_base;
+    const int e = spec.block_extent;
+    std::vector<std::array<int, 3>> bases;
+    for (const auto& r : spec.regions)
+    {
+        for (int d = 0; d < 3; ++d)
+        {
+            if (r[3 + d] % e) throw std::runtime_error("Domain: Extent of blocks are not evenly divisible");
+            if (std::abs(r[d]) % e) throw std::runtime_error("Domain: Base of blocks are not evenly divisible");
+        }
+        for (int k = 0; k < r[5] / e; ++k)
+            for (int j = 0; j < r[4] / e; ++j)
+                for (int i = 0; i < r[3] / e; ++i)
+                    bases.push_back({(r[0] + i * e) / e, (r[1] + j * e) / e, (r[2] + k * e) / e});
+    }
+    int maxext = 0;
+    for (int d = 0; d < 3; ++d)
+    {
+        base_shift[d] = spec.bd_base[d] / e;
+        maxext = std::max(maxext, spec.bd_extent[d] / e);
+        key_box_base[d] = 2;  // baseBlockBufferNumber
+        key_box_extent[d] = spec.bd_extent[d] / e - 4;
+    }
+    base_level = Morton::minimum_level(maxext);
+    depth = base_level + 1;
+    oct.clear();
+    index.clear();
+    insert(Morton::make(0, 0, 0, 0));
+    for (auto& b : bases)
+        insert(Morton::make(b[0] - base_shift[0], b[1] - base_shift[1], b[2] - base_shift[2], base_level));
+    // The server instantiates a block on every octant of the initial tree.
+    for (auto& o : oct) o.has_data = true;
+}
+
+// Pre-order DFS snapshot: root first, children in child-index order.
+std::vector<int> Octree::dfs_snapshot() const
+{
+    std::vector<int> out;
+    if (oct.empty()) return out;
+    std::vector<int> st{0};
+    while (!st.empty())
+    {
+        const int o = st.back();
+        st.pop_back();
+        out.push_back(o);
+        for (int i = 7; i >= 0; --i)
+            if (oct[o].children[i] >= 0) st.push_back(oct[o].children[i]);
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// AMR: include/iblgf/domain/octree/tree.hpp:272-453
+// ---------------------------------------------------------------------------
+std::vector<uint64_t> Octree::unfound_neighbors(int o, bool correction_as_neighbors) const
+{
+    std::vector<uint64_t> keys;
+    uint64_t              nk[27];
+    Morton::neighbor_keys(oct[o].key, nk);
+    for (int i = 0; i < 27; ++i)
+    {
+        if (Morton::is_end(nk[i])) continue;
+        if (nk[i] == oct[o].key) continue;
+        const int n = find(nk[i]);
+        if (n < 0 || !oct[n].has_data) keys.push_back(nk[i]);
+        else if (!correction_as_neighbors && oct[n].correction)
+            keys.push_back(nk[i]);
+    }
+    return keys;
+}
+
+bool Octree::try_2to1(uint64_t k, std::unordered_map<uint64_t, bool>& checklist) const
+{
+    auto it = checklist.find(k);
+    if (it != checklist.end()) return it->second;
+    const int rf_l = Morton::level(k) - base_level;
+    uint64_t  nk[27];
+    Morton::neighbor_keys(k, nk);
+    if (rf_l == 0)
+    {
+        for (int i = 0; i < 27; ++i)
+        {
+            if (nk[i] == k) continue;
+            if (!key_inside_box(nk[i]))
+            {
+                checklist.emplace(k, false);
+                return false;
+            }
+        }
+        checklist.emplace(k, true);
+        return true;
+    }
+    else if (rf_l > 0)
+    {
+        for (int i = 0; i < 27; ++i)
+        {
+            if (nk[i] == k) continue;
+            if (!try_2to1(Morton::parent(nk[i]), checklist))
+            {
+                checklist.emplace(k, false);
+                return false;
+            }
+        }
+        checklist.emplace(k, true);
+        return true;
+    }
+    return false;
+}
+
+void Octree::refine(int o, bool ratio_2to1)
+{
+    if (ratio_2to1)
+    {
+        for (int i = 0; i < 27; ++i)
+        {
+            const int n = oct[o].neighbor[i];
+            if (n >= 0 && oct[n].has_data) oct[n].aim_deletion = false;
+        }
+        if (refinement_level(o) >= 0)
+        {
+            const auto keys = unfound_neighbors(o, false);
+            for (uint64_t k : keys)
+            {
+                if (Morton::is_end(k)) continue;
+                if (refinement_level(o) == 0)
+                {
+                    const int n = insert(k);
+                    if (!oct[n].has_data) give_data(n);
+                    oct[n].leaf = true;
+                    oct[n].correction = false;
+                    oct[n].leaf_boundary = false;
+                    oct[n].aim_deletion = false;
+                }
+                else
+                {
+                    const uint64_t pk = Morton::parent(k);
+                    int            pa = find(pk);
+                    if (pa < 0)
+                    {
+                        pa = insert(pk);
+                        give_data(pa);
+                    }
+                    refine(pa, ratio_2to1);
+                }
+            }
+        }
+    }
+    for (int i = 0; i < 8; ++i)
+    {
+        const int c0 = oct[o].children[i];
+        if (c0 >= 0 && oct[c0].has_data) continue;
+        const int c = refine_child(o, i);
+        if (!oct[c].has_data) give_data(c);
+    }
+    for (int i = 0; i < 8; ++i)
+    {
+        const int c = oct[o].children[i];
+        oct[c].leaf = true;
+        oct[c].correction = false;
+        oct[c].leaf_boundary = false;
+        oct[c].aim_deletion = false;
+    }
+    oct[o].leaf = false;
+    oct[o].correction = false;
+    oct[o].leaf_boundary = false;
+    oct[o].aim_deletion = false;
+    if (oct[o].level + 2 > depth) depth = oct[o].level + 2;
+}
+
+void Octree::init_refine(int nRef, int level_up_max, const RefineCond& cond)
+{
+    // include/iblgf/domain/domain.hpp:878-943
+    construct_leaf_maps(false);
+    std::unordered_map<uint64_t, bool> checklist;
+    for (int l = 0; l < nRef; ++l)
+    {
+        // Live pre-order traversal: children of a node are pushed when the
+        // node is popped, so blocks created under already-visited nodes are
+        // not revisited (iterators.hpp:271-287).
+        auto visit = [&](int cur) {
+            if (refinement_level(cur) != l) return;
+            if (!oct[cur].has_data) return;
+            if (cond(*this, cur, nRef - l))
+            {
+                if (try_2to1(oct[cur].key, checklist)) refine(cur, true);
+            }
+        };
+        visit(0);  // the root is the first element, visited before it is popped
+        std::vector<int> st{0};
+        while (!st.empty())
+        {
+            const int cur = st.back();
+            st.pop_back();
+            for (int i = 7; i >= 0; --i)
+                if (oct[cur].children[i] >= 0) st.push_back(oct[cur].children[i]);
+            if (oct[cur].level == 0) continue;
+            visit(cur);
+        }
+    }
+    construct_level_maps();
+    for (int g = 0; g < level_up_max; ++g)
+    {
+        for (int l = depth - 1; l >= base_level; --l)
+        {
+            const std::vector<int> lm = l < (int)level_map.size() ? level_map[l] : std::vector<int>();
+            for (int o : lm)
+                if (oct[o].leaf) refine(o, false);
+        }
+        construct_leaf_maps(true);
+        construct_level_maps();
+        construct_lists();
+    }
+    mark_correction();
+    mark_leaf_boundary();
+}
+
+void Octree::mark_correction()
+{
+    // include/iblgf/domain/domain.hpp:617-735
+    for (auto& o : oct) o.physical = false;
+    for (int l = depth - 1; l >= base_level; --l)
+    {
+        if (l >= (int)level_map.size()) continue;
+        for (int o : level_map[l])
+        {
+            if (!oct[o].has_data) continue;
+            oct[o].physical = oct[o].leaf && !oct[o].aim_deletion;
+            if (!oct[o].physical)
+            {
+                for (int i = 0; i < 8; ++i)
+                {
+                    const int c = oct[o].children[i];
+                    if (c >= 0 && oct[c].has_data) oct[o].physical = oct[o].physical || oct[c].physical;
+                }
+            }
+        }
+    }
+    if (use_correction_buffer)
+    {
+        const int d0 = depth;
+        for (int l = base_level + 1; l < d0; ++l)
+        {
+            if (l >= (int)level_map.size()) continue;
+            const std::vector<int> lm = level_map[l];
+            for (int o : lm)
+            {
+                if (!oct[o].has_data) continue;
+                if (!oct[o].physical) continue;
+                const auto keys = unfound_neighbors(o, true);
+                for (uint64_t k : keys)
+                {
+                    const int n = insert(k);
+                    if (!oct[n].has_data) give_data(n);
+                }
+            }
+        }
+        construct_lists();
+        construct_level_maps();
+        construct_leaf_maps(true);
+        for (int l = base_level + 1; l < depth; ++l)
+        {
+            if (l >= (int)level_map.size()) continue;
+            for (int o : level_map[l])
+            {
+                if (!oct[o].physical) continue;
+                for (int i = 0; i < 27; ++i)
+                {
+                    const int n = oct[o].neighbor[i];
+                    if (n < 0 || !oct[n].has_data || oct[n].physical) continue;
+                    oct[n].aim_deletion = false;
+                    oct[n].correction = true;
+                }
+            }
+        }
+        construct_level_maps();
+        construct_leaf_maps(true);
+        construct_lists();
+    }
+    if (base_level < (int)level_map.size())
+    {
+        for (int o : level_map[base_level])
+        {
+            if (!oct[o].has_data) continue;
+            if (!oct[o].physical) continue;
+            bool all = true;
+            for (int i = 0; i < 27; ++i)
+            {
+                const int n = oct[o].neighbor[i];
+                if (n < 0 || !oct[n].has_data) all = false;
+            }
+            if (!all)
+            {
+                oct[o].correction = true;
+                oct[o].aim_deletion = false;
+                oct[o].leaf = true;
+            }
+            else
+            {
+                oct[o].correction = false;
+                oct[o].aim_deletion = false;
+            }
+        }
+    }
+    construct_level_maps();
+    construct_lists();
+    construct_leaf_maps(true);
+}
+
+void Octree::mark_leaf_boundary()
+{
+    // include/iblgf/domain/domain.hpp:580-615
+    for (auto& o : oct) o.leaf_boundary = false;
+    const auto order = dfs_snapshot();
+    for (int o : order)
+    {
+        if (!oct[o].has_data) continue;
+        if (oct[o].leaf)
+        {
+            for (int i = 0; i < 27; ++i)
+            {
+                const int n = oct[o].neighbor[i];
+                if (n < 0 || !oct[n].has_data || oct[n].correction || oct[n].leaf) continue;
+                oct[o].leaf_boundary = true;
+            }
+        }
+        else
+        {
+            if (oct[o].correction) continue;
+            for (int i = 0; i < 27; ++i)
+            {
+                const int n = oct[o].neighbor[i];
+                if (n < 0 || !oct[n].has_data || !oct[n].leaf) continue;
+                oct[o].leaf_boundary = true;
+            }
+        }
+        if (oct[o].leaf_boundary)
+        {
+            for (int i = 0; i < 8; ++i)
+            {
+                const int c = oct[o].children[i];
+                if (c >= 0 && oct[c].has_data) oct[c].leaf_boundary = true;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Maps and lists: include/iblgf/domain/octree/tree.hpp:484-495, 559-603, 809-824
+// ---------------------------------------------------------------------------
+void Octree::construct_leaf_maps(bool from_existing_flag)
+{
+    if (from_existing_flag) return;  // leaf flags are the map
+    for (int o : dfs_snapshot())
+    {
+        if (!oct[o].has_data) continue;
+        bool leaf = true;
+        for (int i = 0; i < 8; ++i)
+            if (oct[o].children[i] >= 0) leaf = false;
+        oct[o].leaf = leaf;
+    }
+}
+
+void Octree::construct_level_maps()
+{
+    level_map.assign(Morton::max_level, {});
+    for (int o = 0; o < (int)oct.size(); ++o)
+        if (oct[o].has_data) level_map[oct[o].level].push_back(o);
+    for (auto& lm : level_map)
+        std::sort(lm.begin(), lm.end(), [&](int a, int b) { return oct[a].key < oct[b].key; });
+}
+
+void Octree::neighbor_list_build(int o)
+{
+    uint64_t nk[27];
+    Morton::neighbor_keys(oct[o].key, nk);
+    for (int i = 0; i < 27; ++i)
+    {
+        const int n = find(nk[i]);
+        oct[o].neighbor[i] = (n < 0 || !oct[n].has_data) ? -1 : n;
+    }
+}
+
+void Octree::influence_list_build(int o)
+{
+    std::fill(oct[o].influence, oct[o].influence + 189, -1);
+    oct[o].influence_num = 0;
+    if (oct[o].parent < 0) return;
+    uint64_t  keys[216];
+    const int nk = Morton::influence_keys(oct[o].key, keys);
+    int       id = 0;
+    for (int i = 0; i < nk; ++i)
+    {
+        const int n = find(keys[i]);
+        if (n >= 0 && oct[n].has_data)
+        {
+            if (id >= 189) throw std::runtime_error("influence list overflow (>189)");
+            oct[o].influence[id++] = n;
+        }
+    }
+    oct[o].influence_num = id;
+}
+
+void Octree::construct_lists()
+{
+    for (int o : dfs_snapshot())
+    {
+        if (!oct[o].has_data) continue;
+        neighbor_list_build(o);
+        influence_list_build(o);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// AMR2AMR masks: include/iblgf/fmm/fmm.hpp:392-680
+// ---------------------------------------------------------------------------
+void Octree::fmm_lgf_mask_build(bool subtract_non_leaf)
+{
+    for (int l = base_level; l < depth; ++l)
+    {
+        fmm_dry(l, false, subtract_non_leaf);
+        fmm_dry(l, true, subtract_non_leaf);
+    }
+}
+
+void Octree::fmm_dry(int base, bool nl, bool subtract_non_leaf)
+{
+    const int idx = fmm_mask_idx_gen(MASK_AMR2AMR, base - base_level, nl ? 1 : 0);
+    fmm_dry_init_base_level_masks(base, nl, idx, subtract_non_leaf);
+    fmm_upward_pass_masks(base, idx);
+    fmm_clean_no_inf_masks(base, idx);
+}
+
+void Octree::fmm_dry_init_base_level_masks(
+    int base, bool nl, int idx, bool subtract_non_leaf, int load_factor, bool neighbor_only)
+{
+    if (base >= (int)level_map.size()) return;
+    for (int o : level_map[base])
+    {
+        oct[o].set_mask(idx, 0, false);
+        oct[o].set_mask(idx, 1, false);
+    }
+    for (int o : level_map[base])
+    {
+        Octant& it = oct[o];
+        if (!it.has_data) continue;
+        if (nl)
+        {
+            bool correction_parent = false;
+            for (int i = 0; i < 8; ++i)
+            {
+                const int c = it.children[i];
+                if (c >= 0 && oct[c].has_data && oct[c].correction) correction_parent = true;
+            }
+            if (subtract_non_leaf)
+            {
+                if (it.leaf || it.correction)
+                {
+                    it.set_mask(idx, 0, false);
+                    it.set_mask(idx, 1, correction_parent);
+                }
+                else
+                {
+                    it.set_mask(idx, 0, true);
+                    it.set_mask(idx, 1, true);
+                    if (!neighbor_only) it.load += it.influence_num * load_factor;
+                    it.load += 27 * load_factor;
+                }
+            }
+            else
+            {
+                if (it.leaf || it.correction)
+                {
+                    it.set_mask(idx, 0, true);
+                    it.set_mask(idx, 1, it.leaf);
+                }
+                else
+                {
+                    it.set_mask(idx, 0, false);
+                    it.set_mask(idx, 1, true);
+                    if (!neighbor_only) it.load += it.influence_num * load_factor;
+                    it.load += 27 * load_factor;
+                }
+            }
+        }
+        else
+        {
+            if (subtract_non_leaf)
+            {
+                it.set_mask(idx, 0, true);
+                it.set_mask(idx, 1, true);
+            }
+            else
+            {
+                if (it.leaf || it.correction)
+                {
+                    it.set_mask(idx, 0, false);
+                    it.set_mask(idx, 1, it.leaf);
+                }
+                else
+                {
+                    it.set_mask(idx, 0, true);
+                    it.set_mask(idx, 1, false);
+                }
+            }
+            if (!neighbor_only) it.load += it.influence_num * load_factor;
+            it.load += 27 * load_factor;
+        }
+    }
+}
+
+void Octree::fmm_upward_pass_masks(int base, int idx, double base_factor)
+{
+    for (int level = base - 1; level >= 0; --level)
+    {
+        if (level >= (int)level_map.size()) continue;
+        for (int o : level_map[level])
+        {
+            oct[o].set_mask(idx, 0, false);
+            for (int c = 0; c < 8; ++c)
+            {
+                const int ch = oct[o].children[c];
+                if (ch >= 0 && oct[ch].has_data && oct[ch].mask(idx, 0))
+                {
+                    oct[o].set_mask(idx, 0, true);
+                    break;
+                }
+            }
+        }
+        for (int o : level_map[level])
+        {
+            oct[o].set_mask(idx, 1, false);
+            for (int c = 0; c < 8; ++c)
+            {
+                const int ch = oct[o].children[c];
+                if (ch >= 0 && oct[ch].has_data && oct[ch].mask(idx, 1))
+                {
+                    oct[o].set_mask(idx, 1, true);
+                    break;
+                }
+            }
+            if (oct[o].mask(idx, 1))
+                for (int i = 0; i < oct[o].influence_num; ++i)
+                {
+                    const int inf = oct[o].influence[i];
+                    if (inf >= 0 && oct[inf].mask(idx, 0)) oct[o].load += 1.0 * base_factor;
+                }
+        }
+    }
+}
+
+void Octree::fmm_clean_no_inf_masks(int base, int idx)
+{
+    for (int level = 0; level < base; ++level)
+    {
+        if (level >= (int)level_map.size()) continue;
+        for (int o : level_map[level])
+        {
+            Octant& it = oct[o];
+            const int p = it.parent;
+            if (it.mask(idx, 0))
+            {
+                // A source whose parent is a source is kept, and the target
+                // check below is skipped for it as well (fmm.hpp:425-427).
+                if (p >= 0 && oct[p].has_data && oct[p].mask(idx, 0)) continue;
+                bool has_target = false;
+                for (int i = 0; i < it.influence_num; ++i)
+                {
+                    const int n = it.influence[i];
+                    if (n >= 0 && oct[n].has_data && oct[n].mask(idx, 1))
+                    {
+                        has_target = true;
+                        break;
+                    }
+                }
+                if (!has_target) it.set_mask(idx, 0, false);
+            }
+            if (it.mask(idx, 1))
+            {
+                if (p >= 0 && oct[p].has_data && oct[p].mask(idx, 1)) continue;
+                bool has_source = false;
+                for (int i = 0; i < it.influence_num; ++i)
+                {
+                    const int n = it.influence[i];
+                    if (n >= 0 && oct[n].has_data && oct[n].mask(idx, 0))
+                    {
+                        has_source = true;
+                        break;
+                    }
+                }
+                if (!has_source) it.set_mask(idx, 1, false);
+            }
+        }
+    }
+}
+
+void Octree::fmm_ib_masks()
+{
+    // IB2xIB / xIB2IB: include/iblgf/fmm/fmm.hpp:70-144.  Finest level only.
+    const int l = depth - 1;
+    for (int type : {MASK_IB2xIB, MASK_xIB2IB})
+    {
+        const int idx = fmm_mask_idx_gen(type);
+        for (auto& o : oct)
+        {
+            o.set_mask(idx, 0, false);
+            o.set_mask(idx, 1, false);
+        }
+        if (l < (int)level_map.size())
+            for (int o : level_map[l])
+            {
+                if (oct[o].is_ib)
+                {
+                    oct[o].set_mask(idx, 0, true);
+                    oct[o].set_mask(idx, 1, true);
+                }
+                if (oct[o].extended_ib)
+                {
+                    if (type == MASK_IB2xIB) oct[o].set_mask(idx, 1, true);
+                    else
+                        oct[o].set_mask(idx, 0, true);
+                }
+            }
+        fmm_upward_pass_masks(l, idx);
+        fmm_clean_no_inf_masks(l, idx);
+    }
+}
+
+}  // namespace iblgf_amd

@@ -1,0 +1,761 @@
+// gfx950 kernels of the LGF fast-multipole path (FP64 plain path).
+//
+// Reference operations these kernels carry out on the device (IBLGF-AMR):
+//   forward_dft / inverse_dft   Convolution::execute_fwrd_field / apply_backward,
+//                               include/iblgf/utilities/convolution.hpp:226-304
+//   spectral_mac                simd_prod_complex_add over all (source, target)
+//                               pairs of Fmm::compute_influence_field,
+//                               include/iblgf/fmm/fmm.hpp:988-1027, 1167-1195
+//   transfer_gather / _scatter  Nli::nli_antrp_node / nli_intrp_node,
+//                               include/iblgf/fmm/fmm_nli.hpp:101-247; cell_center_nli
+//                               include/iblgf/interpolation/cell_center_nli_intrp.hpp:337-420, 591-674
+//   buffer_exchange             halo fill, include/iblgf/domain/mpi/haloCommunicator.hpp:67-128
+//   correction_stencil          cell_center_nli::add_source_correction,
+//                               include/iblgf/interpolation/cell_center_nli_intrp.hpp:171-238
+//   lgf_spectra                 LGF_Base::get_subblock + dft,
+//                               include/iblgf/lgf/lgf.hpp:77-98, 221-262
+#include "kernels.hpp"
+
+#include <stdexcept>
+
+namespace iblgf_amd {
+namespace {
+
+constexpr int kThreads = 256;
+
+template<int N>
+struct Geo
+{
+    static constexpr int n = N;
+    static constexpr int E = N - 2;
+    static constexpr int P = 2 * N - 1;
+    static constexpr int H = P / 2 + 1;
+    static constexpr int NF = H * P * P;
+    static constexpr int N3 = N * N * N;
+};
+
+__device__ inline void twiddles(double2* W, int P)
+{
+    for (int m = threadIdx.x; m < P; m += blockDim.x)
+    {
+        double s, c;
+        sincospi(2.0 * m / P, &s, &c);
+        W[m] = make_double2(c, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise block kernels
+// ---------------------------------------------------------------------------
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_zero(double* f, const int* slots, int count)
+{
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    double* p = f + (size_t)slots[b] * Geo<N>::N3;
+    for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads) p[q] = 0.0;
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_copy(
+    double* dst, const double* src, const int* slots, int count, int interior)
+{
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const size_t  off = (size_t)slots[b] * Geo<N>::N3;
+    double*       d = dst + off;
+    const double* s = src + off;
+    for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads)
+    {
+        if (interior)
+        {
+            const int i = q % N, j = (q / N) % N, k = q / (N * N);
+            if (i == 0 || j == 0 || k == 0 || i == N - 1 || j == N - 1 || k == N - 1) continue;
+        }
+        d[q] = s[q];
+    }
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_axpy(
+    double* dst, const double* src, const int* slots, int count, double a)
+{
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const size_t off = (size_t)slots[b] * Geo<N>::N3;
+    for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads) dst[off + q] += src[off + q] * a;
+}
+
+// ---------------------------------------------------------------------------
+// Separable 3-leg transfer: out[k][j][i] = sum Kz[k][c] Ky[j][b] Kx[i][a] in[c][b][a]
+// Stages x, y in LDS; the z stage lands in per-thread registers.
+// ---------------------------------------------------------------------------
+template<int N>
+struct TransferLds
+{
+    double in[Geo<N>::N3];
+    double t1[Geo<N>::N3];
+    double m[2][N * N];
+};
+
+template<int N>
+__device__ inline void contract_xy(TransferLds<N>& L, int bx, int by)
+{
+    constexpr int N3 = Geo<N>::N3;
+    const double* Kx = L.m[bx];
+    const double* Ky = L.m[by];
+    for (int q = threadIdx.x; q < N3; q += kThreads)
+    {
+        const int i = q % N, cb = q / N;
+        double    s = 0.0;
+#pragma unroll
+        for (int a = 0; a < N; ++a) s += Kx[i * N + a] * L.in[cb * N + a];
+        L.t1[q] = s;
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < N3; q += kThreads)
+    {
+        const int i = q % N, j = (q / N) % N, c = q / (N * N);
+        double    s = 0.0;
+#pragma unroll
+        for (int b = 0; b < N; ++b) s += Ky[j * N + b] * L.t1[(c * N + b) * N + i];
+        L.in[q] = s;  // t2 reuses the input buffer
+    }
+    __syncthreads();
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_transfer_gather(
+    const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
+    const double* __restrict__ mats)
+{
+    constexpr int N3 = Geo<N>::N3;
+    __shared__ TransferLds<N> L;
+    const int e = blockIdx.x;
+    if (e >= count) return;
+    const int* ent = entries + 9 * e;
+    for (int q = threadIdx.x; q < 2 * N * N; q += kThreads) (&L.m[0][0])[q] = mats[q];
+    // One workgroup owns the parent block, so the per-child contributions are
+    // added to it in place; each thread touches the same points every child.
+    double* out = dst + (size_t)ent[0] * N3;
+    for (int c = 0; c < 8; ++c)
+    {
+        const int ch = ent[1 + c];
+        if (ch < 0) continue;  // uniform across the workgroup
+        __syncthreads();
+        const double* in = src + (size_t)ch * N3;
+        for (int q = threadIdx.x; q < N3; q += kThreads) L.in[q] = in[q];
+        __syncthreads();
+        contract_xy<N>(L, c & 1, (c >> 1) & 1);
+        const double* Kz = L.m[(c >> 2) & 1];
+        for (int q = threadIdx.x; q < N3; q += kThreads)
+        {
+            const int ji = q % (N * N), k = q / (N * N);
+            double    s = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < N; ++cc) s += Kz[k * N + cc] * L.in[cc * N * N + ji];
+            out[q] += s;
+        }
+    }
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_transfer_scatter(
+    const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
+    const double* __restrict__ mats, int accumulate)
+{
+    constexpr int N3 = Geo<N>::N3;
+    __shared__ TransferLds<N> L;
+    const int e = blockIdx.x;
+    if (e >= count) return;
+    const int child = entries[3 * e], parent = entries[3 * e + 1], c = entries[3 * e + 2];
+    for (int q = threadIdx.x; q < 2 * N * N; q += kThreads) (&L.m[0][0])[q] = mats[q];
+    const double* in = src + (size_t)parent * N3;
+    for (int q = threadIdx.x; q < N3; q += kThreads) L.in[q] = in[q];
+    __syncthreads();
+    contract_xy<N>(L, c & 1, (c >> 1) & 1);
+    const double* Kz = L.m[(c >> 2) & 1];
+    double*       out = dst + (size_t)child * N3;
+    for (int q = threadIdx.x; q < N3; q += kThreads)
+    {
+        const int ji = q % (N * N), k = q / (N * N);
+        double    s = 0.0;
+#pragma unroll
+        for (int cc = 0; cc < N; ++cc) s += Kz[k * N + cc] * L.in[cc * N * N + ji];
+        if (accumulate) out[q] += s;
+        else
+            out[q] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Halo fill and correction stencil
+// ---------------------------------------------------------------------------
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_buffer_exchange(
+    double* f, const int* slots, int count, const int* __restrict__ neighbor)
+{
+    constexpr int N3 = Geo<N>::N3, E = Geo<N>::E;
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const int s = slots[b];
+    for (int q = threadIdx.x; q < N3; q += kThreads)
+    {
+        const int i = q % N, j = (q / N) % N, k = q / (N * N);
+        const int dx = i == 0 ? -1 : (i == N - 1 ? 1 : 0);
+        const int dy = j == 0 ? -1 : (j == N - 1 ? 1 : 0);
+        const int dz = k == 0 ? -1 : (k == N - 1 ? 1 : 0);
+        if (!dx && !dy && !dz) continue;
+        const int nb = neighbor[27 * s + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+        if (nb < 0) continue;
+        f[(size_t)s * N3 + q] = f[(size_t)nb * N3 + q - (dz * E * N * N + dy * E * N + dx * E)];
+    }
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_correction_stencil(
+    const double* __restrict__ tt, double* __restrict__ cc, const int* slots, int count, double inv, double om2)
+{
+    constexpr int N3 = Geo<N>::N3;
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const size_t  off = (size_t)slots[b] * N3;
+    const double* t = tt + off;
+    double*       c = cc + off;
+    for (int q = threadIdx.x; q < N3; q += kThreads)
+    {
+        const int i = q % N, j = (q / N) % N, k = q / (N * N);
+        if (i == 0 || j == 0 || k == 0 || i == N - 1 || j == N - 1 || k == N - 1) continue;
+        double v = c[q];
+        v += 6.0 * t[q] * inv;
+        v -= t[q - N * N] * inv;
+        v -= t[q + N * N] * inv;
+        v -= t[q - N] * inv;
+        v -= t[q + N] * inv;
+        v -= t[q + 1] * inv;
+        v -= t[q - 1] * inv;
+        v += om2 * t[q];
+        c[q] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward r2c of an n^3 block zero-padded to P^3 (pruned legs), one block per
+// workgroup: x leg into LDS, then per kx plane the y and z legs.
+// ---------------------------------------------------------------------------
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_forward_dft(
+    const double* __restrict__ field, const int* __restrict__ slots, int count, double2* __restrict__ spec)
+{
+    using G = Geo<N>;
+    constexpr int P = G::P, H = G::H;
+    __shared__ double2 X1[N * N * H];  // [z][y][kx]
+    __shared__ double2 T[N * P];       // [z][ky]
+    __shared__ double2 W[P];
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const int     slot = slots[b];
+    const double* in = field + (size_t)slot * G::N3;
+    twiddles(W, P);
+    __syncthreads();
+    for (int q = threadIdx.x; q < N * N * H; q += kThreads)
+    {
+        const int     zy = q / H, kx = q - zy * H;
+        const double* line = in + zy * N;
+        double        sr = 0.0, si = 0.0;
+        int           m = 0;
+#pragma unroll
+        for (int x = 0; x < N; ++x)
+        {
+            const double  v = line[x];
+            const double2 w = W[m];
+            sr += v * w.x;
+            si -= v * w.y;
+            m += kx;
+            if (m >= P) m -= P;
+        }
+        X1[q] = make_double2(sr, si);
+    }
+    __syncthreads();
+    double2* out = spec + (size_t)slot * G::NF;
+    for (int kx = 0; kx < H; ++kx)
+    {
+        for (int q = threadIdx.x; q < N * P; q += kThreads)
+        {
+            const int z = q / P, ky = q - z * P;
+            double    sr = 0.0, si = 0.0;
+            int       m = 0;
+#pragma unroll
+            for (int y = 0; y < N; ++y)
+            {
+                const double2 a = X1[(z * N + y) * H + kx];
+                const double2 w = W[m];
+                sr += a.x * w.x + a.y * w.y;
+                si += a.y * w.x - a.x * w.y;
+                m += ky;
+                if (m >= P) m -= P;
+            }
+            T[q] = make_double2(sr, si);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < P * P; q += kThreads)
+        {
+            const int ky = q / P, kz = q - ky * P;
+            double    sr = 0.0, si = 0.0;
+            int       m = 0;
+#pragma unroll
+            for (int z = 0; z < N; ++z)
+            {
+                const double2 a = T[z * P + ky];
+                const double2 w = W[m];
+                sr += a.x * w.x + a.y * w.y;
+                si += a.y * w.x - a.x * w.y;
+                m += kz;
+                if (m >= P) m -= P;
+            }
+            out[(size_t)(kx * P + ky) * P + kz] = make_double2(sr, si);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
+// added into the target block.
+// ---------------------------------------------------------------------------
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restrict__ spec,
+    const int* __restrict__ slots, int count, double* __restrict__ field, double scale)
+{
+    using G = Geo<N>;
+    constexpr int P = G::P, H = G::H;
+    __shared__ double2 V[H * N * N];  // [kx][z][y]
+    __shared__ double2 U[P * N];      // [ky][z]
+    __shared__ double2 W[P];
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const int      slot = slots[b];
+    const double2* in = spec + (size_t)slot * G::NF;
+    twiddles(W, P);
+    __syncthreads();
+    for (int kx = 0; kx < H; ++kx)
+    {
+        for (int q = threadIdx.x; q < P * N; q += kThreads)
+        {
+            const int      ky = q / N, z = q - ky * N;
+            const int      step = (N - 1 + z) % P;
+            const double2* row = in + (size_t)(kx * P + ky) * P;
+            double         sr = 0.0, si = 0.0;
+            int            m = 0;
+#pragma unroll 5
+            for (int kz = 0; kz < P; ++kz)
+            {
+                const double2 a = row[kz];
+                const double2 w = W[m];
+                sr += a.x * w.x - a.y * w.y;
+                si += a.x * w.y + a.y * w.x;
+                m += step;
+                if (m >= P) m -= P;
+            }
+            U[q] = make_double2(sr, si);
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < N * N; q += kThreads)
+        {
+            const int z = q / N, y = q - z * N;
+            const int step = (N - 1 + y) % P;
+            double    sr = 0.0, si = 0.0;
+            int       m = 0;
+#pragma unroll 5
+            for (int ky = 0; ky < P; ++ky)
+            {
+                const double2 a = U[ky * N + z];
+                const double2 w = W[m];
+                sr += a.x * w.x - a.y * w.y;
+                si += a.x * w.y + a.y * w.x;
+                m += step;
+                if (m >= P) m -= P;
+            }
+            V[(kx * N + z) * N + y] = make_double2(sr, si);
+        }
+        __syncthreads();
+    }
+    double* out = field + (size_t)slot * G::N3;
+    for (int q = threadIdx.x; q < G::N3; q += kThreads)
+    {
+        const int x = q % N, y = (q / N) % N, z = q / (N * N);
+        const int step = (N - 1 + x) % P;
+        double    s = 0.0;
+        int       m = 0;
+#pragma unroll
+        for (int kx = 0; kx < H; ++kx)
+        {
+            const double2 a = V[(kx * N + z) * N + y];
+            const double2 w = W[m];
+            const double  c = (kx == 0) ? 1.0 : 2.0;  // P odd: no Nyquist plane
+            s += c * (a.x * w.x - a.y * w.y);
+            m += step;
+            if (m >= P) m -= P;
+        }
+        out[q] += scale * s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral multiply-accumulate over sibling groups.  A workgroup owns KF
+// frequencies and walks every group; the kernel spectra of the 343 offsets
+// for its frequencies sit in LDS.  Each thread keeps the 8 target
+// accumulators of one group at one frequency.
+// ---------------------------------------------------------------------------
+template<int N, int KF>
+__global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
+    const MacSrc* __restrict__ srcs, const double2* __restrict__ ghat, const double2* __restrict__ sspec,
+    double2* __restrict__ tspec)
+{
+    constexpr int NF = Geo<N>::NF;
+    constexpr int GPW = kThreads / KF;
+    __shared__ double2 Gs[343 * KF];
+    const int f0 = blockIdx.x * KF;
+    for (int q = threadIdx.x; q < 343 * KF; q += kThreads)
+    {
+        const int o = q / KF, f = q - o * KF;
+        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    const int  f = threadIdx.x % KF, gl = threadIdx.x / KF;
+    const int  fg = f0 + f;
+    const bool fok = fg < NF;
+    for (int g = gl; g < ngroups; g += GPW)
+    {
+        const MacGroup grp = groups[g];
+        double2        acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
+        for (int e = grp.begin; e < grp.begin + grp.count; ++e)
+        {
+            const MacSrc  sr = srcs[e];
+            const double2 S = fok ? sspec[(size_t)sr.slot * NF + fg] : make_double2(0.0, 0.0);
+            const int     r = sr.meta & 255, tm = sr.meta >> 8;
+            const int     rx = r % 6, ry = (r / 6) % 6, rz = r / 36;
+            const int     ob = (rx + 1) + 7 * (ry + 1) + 49 * (rz + 1);
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+            {
+                if (tm & (1 << t))
+                {
+                    const int     oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
+                    const double2 gv = Gs[oi * KF + f];
+                    acc[t].x += gv.x * S.x - gv.y * S.y;
+                    acc[t].y += gv.x * S.y + gv.y * S.x;
+                }
+            }
+        }
+        if (fok)
+        {
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel spectra generation: sample, then three DFT legs through HBM scratch.
+// ---------------------------------------------------------------------------
+__device__ double lgf_asym(int n1, int n2, int n3)
+{
+    const double a = (double)n1 * n1, b = (double)n2 * n2, c = (double)n3 * n3;
+    const double r = sqrt(a + b + c);
+    const double r6 = r * r * r * r * r * r, r7 = r6 * r, r13 = r6 * r6 * r;
+    const double pi = 3.14159265358979323846;
+    double       v = -1.0 / 4.0 / pi / r;
+    v = v - (a * a + b * b + c * c - 3.0 * a * b - 3.0 * b * c - 3.0 * c * a) / 16.0 / pi / r7;
+    if (n1 < 600 || n2 < 600 || n3 < 600)
+    {
+        const double a3 = a * a * a, b3 = b * b * b, c3 = c * c * c;
+        double t2 = -3.0 * (23 * (a3 * a + b3 * b + c3 * c) -
+                               244 * (b3 * c + c3 * b + a3 * b + b3 * a + a3 * c + c3 * a) +
+                               621 * ((a * a) * (b * b) + (b * b) * (c * c) + (c * c) * (a * a)) -
+                               228 * ((a * a) * b * c + a * (b * b) * c + a * b * (c * c)));
+        t2 = t2 / r13 / 4.0 * (8.0 / (768.0 * pi));
+        v += t2;
+    }
+    return v;
+}
+
+__device__ double lgf_value(const double* g, int nmax, int x, int y, int z)
+{
+    x = abs(x);
+    y = abs(y);
+    z = abs(z);
+    if (x <= nmax && y <= nmax && z <= nmax)
+    {
+        int t;
+        if (x < y) { t = x; x = y; y = t; }
+        if (x < z) { t = x; x = z; z = t; }
+        if (y < z) { t = y; y = z; z = t; }
+        return g[(long)x * (x + 1) * (x + 2) / 6 + (long)y * (y + 1) / 2 + z];
+    }
+    return lgf_asym(x, y, z);
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_lgf_sample(int kind, int d, const double* __restrict__ g, int nmax,
+    const double* __restrict__ iv, int kmax, double* __restrict__ out)
+{
+    constexpr int P = Geo<N>::P, E = Geo<N>::E;
+    constexpr long total = 343L * P * P * P;
+    const long     q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= total) return;
+    const int x = q % P, y = (q / P) % P, z = (q / (P * P)) % P;
+    const int o = (int)(q / ((long)P * P * P));
+    const int ox = o % 7 - 3, oy = (o / 7) % 7 - 3, oz = o / 49 - 3;
+    const int step = 1 << d;
+    // shift = t_base - s_base = -o * e; block base = shift - (n-1)
+    const int gx = (-ox * E - (N - 1) + x) * step;
+    const int gy = (-oy * E - (N - 1) + y) * step;
+    const int gz = (-oz * E - (N - 1) + z) * step;
+    double    v;
+    if (kind == 0) v = lgf_value(g, nmax, gx, gy, gz);
+    else
+    {
+        const int ax = abs(gx), ay = abs(gy), az = abs(gz);
+        v = (ax <= kmax && ay <= kmax && az <= kmax) ? iv[ax] * iv[ay] * iv[az] : 0.0;
+    }
+    out[q] = v;
+}
+
+// x leg: in[o][z][y][x] real -> out[o][z][y][kx]
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_leg_x(const double* __restrict__ in, double2* __restrict__ out)
+{
+    constexpr int  P = Geo<N>::P, H = Geo<N>::H;
+    constexpr long total = 343L * P * P * H;
+    const long     q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= total) return;
+    const int     kx = q % H;
+    const long    line = q / H;
+    const double* src = in + line * P;
+    double        sr = 0.0, si = 0.0;
+    for (int x = 0; x < P; ++x)
+    {
+        double s, c;
+        sincospi(2.0 * ((kx * x) % P) / P, &s, &c);
+        sr += src[x] * c;
+        si -= src[x] * s;
+    }
+    out[q] = make_double2(sr, si);
+}
+
+// y leg: in[o][z][y][kx] -> out[o][z][ky][kx]
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_leg_y(const double2* __restrict__ in, double2* __restrict__ out)
+{
+    constexpr int  P = Geo<N>::P, H = Geo<N>::H;
+    constexpr long total = 343L * P * P * H;
+    const long     q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= total) return;
+    const int      kx = q % H, ky = (q / H) % P;
+    const long     oz = q / ((long)H * P);
+    const double2* src = in + oz * P * H;
+    double         sr = 0.0, si = 0.0;
+    for (int y = 0; y < P; ++y)
+    {
+        double s, c;
+        sincospi(2.0 * ((ky * y) % P) / P, &s, &c);
+        const double2 a = src[y * H + kx];
+        sr += a.x * c + a.y * s;
+        si += a.y * c - a.x * s;
+    }
+    out[q] = make_double2(sr, si);
+}
+
+// z leg: in[o][z][ky][kx] -> ghat[o][kx][ky][kz]
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_leg_z(const double2* __restrict__ in, double2* __restrict__ out)
+{
+    constexpr int  P = Geo<N>::P, H = Geo<N>::H;
+    constexpr long total = 343L * P * P * H;
+    const long     q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= total) return;
+    const int      kz = q % P, ky = (q / P) % P, kx = (q / (P * P)) % H;
+    const long     o = q / ((long)P * P * H);
+    const double2* src = in + o * P * P * H;
+    double         sr = 0.0, si = 0.0;
+    for (int z = 0; z < P; ++z)
+    {
+        double s, c;
+        sincospi(2.0 * ((kz * z) % P) / P, &s, &c);
+        const double2 a = src[((long)z * P + ky) * H + kx];
+        sr += a.x * c + a.y * s;
+        si += a.y * c - a.x * s;
+    }
+    out[q] = make_double2(sr, si);
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch on the block size n
+// ---------------------------------------------------------------------------
+template<template<int> class F, typename... Args>
+void dispatch_n(int n, Args&&... args)
+{
+    switch (n)
+    {
+        case 8: F<8>::run(args...); break;
+        case 10: F<10>::run(args...); break;
+        case 12: F<12>::run(args...); break;
+        case 16: F<16>::run(args...); break;
+        case 18: F<18>::run(args...); break;
+        default: throw std::runtime_error("unsupported block size n (supported: 8, 10, 12, 16, 18)");
+    }
+}
+
+inline unsigned grid1(long total) { return (unsigned)((total + kThreads - 1) / kThreads); }
+
+template<int N> struct RunZero
+{
+    static void run(hipStream_t s, double* f, const int* slots, int count)
+    {
+        hipLaunchKernelGGL(k_zero<N>, dim3(count), dim3(kThreads), 0, s, f, slots, count);
+    }
+};
+template<int N> struct RunCopy
+{
+    static void run(hipStream_t s, double* d, const double* src, const int* slots, int count, int interior)
+    {
+        hipLaunchKernelGGL(k_copy<N>, dim3(count), dim3(kThreads), 0, s, d, src, slots, count, interior);
+    }
+};
+template<int N> struct RunAxpy
+{
+    static void run(hipStream_t s, double* d, const double* src, const int* slots, int count, double a)
+    {
+        hipLaunchKernelGGL(k_axpy<N>, dim3(count), dim3(kThreads), 0, s, d, src, slots, count, a);
+    }
+};
+template<int N> struct RunGather
+{
+    static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m)
+    {
+        hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count), dim3(kThreads), 0, s, src, dst, e, count, m);
+    }
+};
+template<int N> struct RunScatter
+{
+    static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m, int acc)
+    {
+        hipLaunchKernelGGL(k_transfer_scatter<N>, dim3(count), dim3(kThreads), 0, s, src, dst, e, count, m, acc);
+    }
+};
+template<int N> struct RunHalo
+{
+    static void run(hipStream_t s, double* f, const int* slots, int count, const int* nb)
+    {
+        hipLaunchKernelGGL(k_buffer_exchange<N>, dim3(count), dim3(kThreads), 0, s, f, slots, count, nb);
+    }
+};
+template<int N> struct RunStencil
+{
+    static void run(hipStream_t s, const double* t, double* c, const int* slots, int count, double inv, double om2)
+    {
+        hipLaunchKernelGGL(k_correction_stencil<N>, dim3(count), dim3(kThreads), 0, s, t, c, slots, count, inv, om2);
+    }
+};
+template<int N> struct RunFwd
+{
+    static void run(hipStream_t s, const double* f, const int* slots, int count, double2* spec)
+    {
+        hipLaunchKernelGGL(k_forward_dft<N>, dim3(count), dim3(kThreads), 0, s, f, slots, count, spec);
+    }
+};
+template<int N> struct RunInv
+{
+    static void run(hipStream_t s, const double2* spec, const int* slots, int count, double* f, double scale)
+    {
+        hipLaunchKernelGGL(k_inverse_dft<N>, dim3(count), dim3(kThreads), 0, s, spec, slots, count, f, scale);
+    }
+};
+template<int N> struct RunMac
+{
+    static void run(hipStream_t s, const MacGroup* g, int ng, const MacSrc* src, const double2* ghat,
+        const double2* ss, double2* ts)
+    {
+        constexpr int KF = 16;
+        const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
+        hipLaunchKernelGGL((k_spectral_mac<N, KF>), dim3(nblk), dim3(kThreads), 0, s, g, ng, src, ghat, ss, ts);
+    }
+};
+template<int N> struct RunLgf
+{
+    static void run(hipStream_t s, int kind, int d, const double* g, int nmax, const double* iv, int kmax,
+        double* sample, double2* a, double2* b, double2* ghat)
+    {
+        constexpr int P = Geo<N>::P, H = Geo<N>::H;
+        hipLaunchKernelGGL(k_lgf_sample<N>, dim3(grid1(343L * P * P * P)), dim3(kThreads), 0, s, kind, d, g, nmax,
+            iv, kmax, sample);
+        hipLaunchKernelGGL(k_leg_x<N>, dim3(grid1(343L * P * P * H)), dim3(kThreads), 0, s, sample, a);
+        hipLaunchKernelGGL(k_leg_y<N>, dim3(grid1(343L * P * P * H)), dim3(kThreads), 0, s, a, b);
+        hipLaunchKernelGGL(k_leg_z<N>, dim3(grid1(343L * P * P * H)), dim3(kThreads), 0, s, b, ghat);
+    }
+};
+
+}  // namespace
+
+namespace launch {
+
+bool supported_n(int n) { return n == 8 || n == 10 || n == 12 || n == 16 || n == 18; }
+
+void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count)
+{
+    if (count > 0) dispatch_n<RunZero>(n, s, f, slots, count);
+}
+void copy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
+    bool interior_only)
+{
+    if (count > 0) dispatch_n<RunCopy>(n, s, dst, src, slots, count, interior_only ? 1 : 0);
+}
+void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a)
+{
+    if (count > 0) dispatch_n<RunAxpy>(n, s, dst, src, slots, count, a);
+}
+void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
+    const double* mats)
+{
+    if (count > 0) dispatch_n<RunGather>(n, s, src, dst, entries, count, mats);
+}
+void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
+    const double* mats, bool accumulate)
+{
+    if (count > 0) dispatch_n<RunScatter>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0);
+}
+void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor)
+{
+    if (count > 0) dispatch_n<RunHalo>(n, s, f, slots, count, neighbor);
+}
+void correction_stencil(hipStream_t s, int n, const double* t, double* c, const int* slots, int count,
+    double inv_dx2, double omega2)
+{
+    if (count > 0) dispatch_n<RunStencil>(n, s, t, c, slots, count, inv_dx2, omega2);
+}
+void forward_dft(hipStream_t s, int n, const double* field, const int* slots, int count, double2* spec)
+{
+    if (count > 0) dispatch_n<RunFwd>(n, s, field, slots, count, spec);
+}
+void inverse_dft(hipStream_t s, int n, const double2* spec, const int* slots, int count, double* field,
+    double scale)
+{
+    if (count > 0) dispatch_n<RunInv>(n, s, spec, slots, count, field, scale);
+}
+void spectral_mac(hipStream_t s, int n, const MacGroup* groups, int ngroups, const MacSrc* srcs,
+    const double2* ghat, const double2* sspec, double2* tspec)
+{
+    if (ngroups > 0) dispatch_n<RunMac>(n, s, groups, ngroups, srcs, ghat, sspec, tspec);
+}
+void lgf_spectra(hipStream_t s, int n, int kind, int d, const double* g, int nmax, const double* iv, int kmax,
+    double* sample, double2* a, double2* b, double2* ghat)
+{
+    dispatch_n<RunLgf>(n, s, kind, d, g, nmax, iv, kmax, sample, a, b, ghat);
+}
+
+}  // namespace launch
+}  // namespace iblgf_amd

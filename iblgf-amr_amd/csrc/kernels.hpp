@@ -1,0 +1,69 @@
+// Launch interface of the gfx950 kernels of the LGF fast-multipole path.
+// Every field is a contiguous array of blocks: field[slot][n^3] FP64, x
+// fastest, slots ordered by level then Morton key.  Every spectrum is
+// spec[slot][NF] complex FP64 with NF = H*P*P, P = 2n-1, H = n, index
+// (kx*P + ky)*P + kz.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace iblgf_amd {
+
+// One sibling group of the spectral multiply-accumulate: the (up to) eight
+// children of one parent that are targets at this level, and the range of
+// source entries shared by them.
+struct MacGroup
+{
+    int tslot[8];  // target slot per child id, -1 if not a target
+    int begin;     // first source entry
+    int count;     // number of source entries
+};
+
+// A source entry: block slot, position r = rx + 6 ry + 36 rz of the source in
+// the 6^3 block region spanned by the children of the parent's 27 neighbours,
+// and the 8-bit mask of the group's targets that it acts on.
+struct MacSrc
+{
+    int slot;
+    int meta;  // r | (tmask << 8)
+};
+
+namespace launch {
+
+bool supported_n(int n);
+
+void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count);
+void copy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
+    bool interior_only);
+void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
+    double a);
+
+// dst[entry.parent] += sum_c K(child) for entries {parent, child[8]}; K built
+// from the side matrices mats[2][n*n] (row = output point, col = input point).
+void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
+    const double* mats);
+// dst[entry.child] (=|+=) K(src[entry.parent]) for entries {child, parent, cidx}.
+void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
+    const double* mats, bool accumulate);
+
+void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor);
+void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* corr, const int* slots,
+    int count, double inv_dx2, double omega2);
+
+void forward_dft(hipStream_t s, int n, const double* field, const int* slots, int count, double2* spec);
+void inverse_dft(hipStream_t s, int n, const double2* spec, const int* slots, int count, double* field,
+    double scale);
+void spectral_mac(hipStream_t s, int n, const MacGroup* groups, int ngroups, const MacSrc* srcs,
+    const double2* ghat, const double2* sspec, double2* tspec);
+
+// Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:
+// block[a] = K(step*(-o*e - (n-1) + a)), a in [0,P)^3, step = 2^d, then r2c.
+// kind 0: Laplace LGF (table g up to nmax, asymptotics beyond);
+// kind 1: integrating factor, iv[k] = e^{-2a} I_k(2a) for k <= kmax.
+void lgf_spectra(hipStream_t s, int n, int kind, int d, const double* g, int nmax, const double* iv,
+    int kmax, double* sample_scratch, double2* scratch_a, double2* scratch_b, double2* ghat);
+
+}  // namespace launch
+}  // namespace iblgf_amd

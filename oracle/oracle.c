@@ -1,0 +1,1026 @@
+/* CPU FP64 oracle of the LGF fast-multipole path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker, never the product).  Loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ *
+ * Restates, in plain C and in the reference's own loop order:
+ *   LGF table / asymptotics  include/iblgf/lgf/lgf_gl_lookup.hpp:33-52, 74-119
+ *                            include/iblgf/lgf/lgf_table_gen/lgf_integrator.py:7-25
+ *   IF kernel                include/iblgf/lgf/lgf_ge.hpp:83, 158-168
+ *   spectrum cache           include/iblgf/lgf/lgf.hpp:77-98, 221-262
+ *   block convolution        include/iblgf/utilities/convolution.hpp:226-304,
+ *                            src/utilities/convolution.cpp:357-570 (pruned legs)
+ *   Nli                      include/iblgf/fmm/fmm_nli.hpp:101-247, 319-374
+ *   Fmm::apply               include/iblgf/fmm/fmm.hpp:729-870, 988-1195
+ *   cell_center_nli          include/iblgf/interpolation/cell_center_nli_intrp.hpp:171-238, 310-420, 557-787
+ *   halo exchange            include/iblgf/domain/mpi/haloCommunicator.hpp:67-128
+ *   PoissonSolver            include/iblgf/solver/poisson/poisson.hpp:427-494, 587-894, 896-1194
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define O_PI 3.14159265358979323846264338327950288
+
+/* ------------------------------------------------------------------------ */
+/* Scaled modified Bessel functions                                          */
+/* ------------------------------------------------------------------------ */
+void oracle_ive(double x, int kmax, double* out)
+{
+    int k;
+    if (x <= 0.0)
+    {
+        out[0] = 1.0;
+        for (k = 1; k <= kmax; ++k) out[k] = 0.0;
+        return;
+    }
+    if (x > 1.0e4)
+    {
+        /* Hankel expansion e^{-x} I_k(x) ~ (2 pi x)^{-1/2} sum_j (-1)^j a_j(k) / x^j */
+        double pre = 1.0 / sqrt(2.0 * O_PI * x);
+        for (k = 0; k <= kmax; ++k)
+        {
+            double mu = 4.0 * (double)k * (double)k, term = 1.0, s = 1.0;
+            int    j;
+            for (j = 1; j < 100; ++j)
+            {
+                double a = 2.0 * j - 1.0;
+                term *= -(mu - a * a) / (8.0 * x * (double)j);
+                s += term;
+                if (fabs(term) < 1e-18 * fabs(s)) break;
+            }
+            out[k] = pre * s;
+        }
+        return;
+    }
+    {
+        /* backward recurrence from N well past max(k, x) */
+        int    N = kmax + (int)x + (int)(15.0 * sqrt(x)) + 50;
+        double ip = 0.0, ic = 1e-250, norm = 0.0;
+        for (k = 0; k <= kmax; ++k) out[k] = 0.0;
+        norm = 2.0 * ic;
+        for (k = N; k > 0; --k)
+        {
+            double im = ip + (2.0 * (double)k / x) * ic;
+            ip = ic;
+            ic = im; /* ic = I_{k-1} */
+            if (k - 1 <= kmax) out[k - 1] = ic;
+            norm += (k - 1 == 0) ? ic : 2.0 * ic;
+            if (ic > 1e200)
+            {
+                int j;
+                ic *= 1e-200;
+                ip *= 1e-200;
+                norm *= 1e-200;
+                for (j = (k - 1 < 0 ? 0 : k - 1); j <= kmax; ++j) out[j] *= 1e-200;
+            }
+        }
+        for (k = 0; k <= kmax; ++k) out[k] /= norm;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Laplace LGF                                                                */
+/* ------------------------------------------------------------------------ */
+static long lgf_idx(int x, int y, int z)
+{
+    return (long)x * (x + 1) * (x + 2) / 6 + (long)y * (y + 1) / 2 + z;
+}
+
+void oracle_lgf_table(int nmax, double* g)
+{
+    /* G(n) = -int_0^inf prod_i ive(n_i, 2t) dt, on t = e^s, trapezoid with
+     * step 1/32 on s in [-44, ln 1e11], end corrections and analytic tail. */
+    const double h = 1.0 / 32.0, s0 = -44.0;
+    const int    ns = (int)ceil((log(1e11) - s0) / h);
+    const double s1 = s0 + ns * h;
+    const long   size = lgf_idx(nmax, nmax, nmax) + 1;
+    double*      iv = (double*)malloc(sizeof(double) * (nmax + 1));
+    long         i;
+    int          q, x, y, z;
+    for (i = 0; i < size; ++i) g[i] = 0.0;
+    for (q = 0; q <= ns; ++q)
+    {
+        double t = exp(s0 + q * h);
+        double w = h * t * (q == ns ? 0.5 : 1.0);
+        long   id = 0;
+        oracle_ive(2.0 * t, nmax, iv);
+        for (x = 0; x <= nmax; ++x)
+            for (y = 0; y <= x; ++y)
+                for (z = 0; z <= y; ++z) g[id++] += w * iv[x] * iv[y] * iv[z];
+    }
+    {
+        double t = exp(s1), T = t;
+        double ce = (h * h / 24.0 - h * h * h * h / 5760.0) * t;
+        double c0 = pow(4.0 * O_PI, -1.5);
+        long   id = 0;
+        oracle_ive(2.0 * t, nmax, iv);
+        for (x = 0; x <= nmax; ++x)
+            for (y = 0; y <= x; ++y)
+                for (z = 0; z <= y; ++z)
+                {
+                    double m = 4.0 * ((double)x * x + (double)y * y + (double)z * z) - 3.0;
+                    double tail = c0 * (2.0 / sqrt(T) - m / 24.0 * pow(T, -1.5));
+                    g[id] = -(g[id] + ce * iv[x] * iv[y] * iv[z] + tail);
+                    ++id;
+                }
+    }
+    free(iv);
+}
+
+double oracle_lgf_asym(int n1, int n2, int n3)
+{
+    double a = (double)n1 * n1, b = (double)n2 * n2, c = (double)n3 * n3;
+    double r = sqrt(a + b + c);
+    double r2 = r * r, r6 = r2 * r2 * r2, r7 = r6 * r, r13 = r6 * r6 * r;
+    double v = -1.0 / (4.0 * O_PI * r);
+    v -= (a * a + b * b + c * c - 3.0 * a * b - 3.0 * b * c - 3.0 * c * a) / (16.0 * O_PI * r7);
+    if (n1 < 600 || n2 < 600 || n3 < 600)
+    {
+        double a3 = a * a * a, b3 = b * b * b, c3 = c * c * c;
+        double p = 23.0 * (a3 * a + b3 * b + c3 * c) -
+                   244.0 * (b3 * c + c3 * b + a3 * b + b3 * a + a3 * c + c3 * a) +
+                   621.0 * (a * a * b * b + b * b * c * c + c * c * a * a) -
+                   228.0 * (a * a * b * c + a * b * b * c + a * b * c * c);
+        v += -3.0 * p / r13 / 4.0 * (8.0 / (768.0 * O_PI));
+    }
+    return v;
+}
+
+double oracle_lgf_get(const double* g, int nmax, int x, int y, int z)
+{
+    int t;
+    x = abs(x);
+    y = abs(y);
+    z = abs(z);
+    if (x <= nmax && y <= nmax && z <= nmax)
+    {
+        if (x < y) { t = x; x = y; y = t; }
+        if (x < z) { t = x; x = z; z = t; }
+        if (y < z) { t = y; y = z; z = t; }
+        return g[lgf_idx(x, y, z)];
+    }
+    return oracle_lgf_asym(x, y, z);
+}
+
+void oracle_direct_sum(const double* g, int nmax, int ns, const int* sx, const double* q, int nt,
+    const int* tx, double scale, double* phi)
+{
+    int i, j;
+    for (i = 0; i < nt; ++i)
+    {
+        double acc = 0.0;
+        for (j = 0; j < ns; ++j)
+            acc += oracle_lgf_get(g, nmax, tx[3 * i] - sx[3 * j], tx[3 * i + 1] - sx[3 * j + 1],
+                       tx[3 * i + 2] - sx[3 * j + 2]) *
+                   q[j];
+        phi[i] = scale * acc;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Lagrange transfer matrices                                                 */
+/* ------------------------------------------------------------------------ */
+void oracle_lagrange_matrix(int n, int cap, int rp, double* A)
+{
+    int c, p, l;
+    memset(A, 0, sizeof(double) * (size_t)n * 2 * n);
+    for (c = 1; c < 2 * n - 1; ++c)
+    {
+        double cpos = rp ? (c - n + 0.5) : (double)(c - n + 1); /* doubled, centred position */
+        int    bl, br;
+        long double sum = 0.0L;
+        if (!rp && c % 2 == 0)
+        {
+            A[(size_t)(c / 2) * 2 * n + c - 1] = 1.0;
+            continue;
+        }
+        if ((rp ? (c - n + 0.5) : (double)(c - n)) < -1.0)
+        {
+            bl = (c + 1) / 2 - cap / 2;
+            if (bl < 0) bl = 0;
+            br = bl + cap;
+            if (br > n) br = n;
+        }
+        else
+        {
+            br = (c + 1) / 2 + cap / 2;
+            if (br > n) br = n;
+            bl = br - cap;
+            if (bl < 0) bl = 0;
+        }
+        for (p = bl; p < br; ++p)
+        {
+            long double m = 1.0L;
+            int         pp = 2 * p - n + 1;
+            for (l = bl; l < br; ++l)
+                if (l != p) m /= -(long double)(2 * l - n + 1 - pp);
+            m /= (long double)(cpos - pp);
+            A[(size_t)p * 2 * n + c - 1] = (double)m;
+            sum += m;
+        }
+        for (p = 0; p < n; ++p) A[(size_t)p * 2 * n + c - 1] /= (double)sum;
+    }
+    for (c = 2 * n - 1; c > n - 1; --c)
+        for (p = 0; p < n; ++p) A[(size_t)p * 2 * n + c] = A[(size_t)p * 2 * n + c - 2];
+}
+
+/* n x n sub-block of the n x 2n matrix for child side b: S[o][i] = A[o][b n + i] */
+static void sub_matrix(const double* A, int n, int b, double* S)
+{
+    int o, i;
+    for (o = 0; o < n; ++o)
+        for (i = 0; i < n; ++i) S[o * n + i] = A[(size_t)o * 2 * n + b * n + i];
+}
+static void transpose(const double* S, int n, double* T)
+{
+    int o, i;
+    for (o = 0; o < n; ++o)
+        for (i = 0; i < n; ++i) T[i * n + o] = S[o * n + i];
+}
+
+/* restriction sub-matrix (cell_center_nli simple coarsening), rp 0/1 */
+static void restrict_matrix(int n, int b, int rp, double* M)
+{
+    int i, j;
+    memset(M, 0, sizeof(double) * n * n);
+    for (i = 1; i < n - 1; ++i)
+    {
+        /* simple matrix columns 2i-1 (and 2i for rp 1), then shift by off */
+        int off = (b == 0) ? 1 : n - 1;
+        for (j = 1; j < n - 1; ++j)
+        {
+            int col = off + j - 1;
+            double v = 0.0;
+            if (col == 2 * i - 1) v = rp ? 0.5 : 1.0;
+            if (rp && col == 2 * i) v = 0.5;
+            M[i * n + j] = v;
+        }
+    }
+}
+
+/* out[k][j][i] (=|+=) sum Mz[k][c] My[j][b] Mx[i][a] in[c][b][a] */
+static void transfer3(int n, const double* Mx, const double* My, const double* Mz, const double* in,
+    double* out, int accumulate)
+{
+    size_t  n3 = (size_t)n * n * n;
+    double* t1 = (double*)calloc(n3, sizeof(double));
+    double* t2 = (double*)calloc(n3, sizeof(double));
+    int     a, b, c, i, j, k;
+    for (c = 0; c < n; ++c)
+        for (b = 0; b < n; ++b)
+            for (i = 0; i < n; ++i)
+            {
+                double s = 0.0;
+                for (a = 0; a < n; ++a) s += Mx[i * n + a] * in[((size_t)c * n + b) * n + a];
+                t1[((size_t)c * n + b) * n + i] = s;
+            }
+    for (c = 0; c < n; ++c)
+        for (j = 0; j < n; ++j)
+            for (i = 0; i < n; ++i)
+            {
+                double s = 0.0;
+                for (b = 0; b < n; ++b) s += My[j * n + b] * t1[((size_t)c * n + b) * n + i];
+                t2[((size_t)c * n + j) * n + i] = s;
+            }
+    for (k = 0; k < n; ++k)
+        for (j = 0; j < n; ++j)
+            for (i = 0; i < n; ++i)
+            {
+                double s = 0.0;
+                for (c = 0; c < n; ++c) s += Mz[k * n + c] * t2[((size_t)c * n + j) * n + i];
+                if (accumulate) out[((size_t)k * n + j) * n + i] += s;
+                else
+                    out[((size_t)k * n + j) * n + i] = s;
+            }
+    free(t1);
+    free(t2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Block convolution by zero-padded DFT of size P = 2n-1                     */
+/* ------------------------------------------------------------------------ */
+typedef struct
+{
+    int     n, P, H;
+    double* cw; /* cos(2 pi m / P) */
+    double* sw; /* sin(2 pi m / P) */
+} dft_t;
+
+static void dft_init(dft_t* d, int n)
+{
+    int m;
+    d->n = n;
+    d->P = 2 * n - 1;
+    d->H = d->P / 2 + 1;
+    d->cw = (double*)malloc(sizeof(double) * d->P);
+    d->sw = (double*)malloc(sizeof(double) * d->P);
+    for (m = 0; m < d->P; ++m)
+    {
+        d->cw[m] = cos(2.0 * O_PI * m / d->P);
+        d->sw[m] = sin(2.0 * O_PI * m / d->P);
+    }
+}
+static void dft_free(dft_t* d)
+{
+    free(d->cw);
+    free(d->sw);
+}
+static size_t spec_size(const dft_t* d) { return (size_t)d->H * d->P * d->P; }
+
+/* r2c of an nin^3 block (x fastest) zero-padded to P^3.  Output (re, im) at
+ * [(kz*P + ky)*H + kx].  Legs pruned to the nonzero input lines. */
+static void dft_forward(const dft_t* d, const double* in, int nin, double* ore, double* oim)
+{
+    const int P = d->P, H = d->H;
+    size_t    sa = (size_t)nin * nin * H, sb = (size_t)nin * P * H;
+    double *  are = (double*)malloc(sizeof(double) * sa), *aim = (double*)malloc(sizeof(double) * sa);
+    double *  bre = (double*)malloc(sizeof(double) * sb), *bim = (double*)malloc(sizeof(double) * sb);
+    int       x, y, z, kx, ky, kz;
+    for (z = 0; z < nin; ++z)
+        for (y = 0; y < nin; ++y)
+        {
+            const double* line = in + ((size_t)z * nin + y) * nin;
+            for (kx = 0; kx < H; ++kx)
+            {
+                double sr = 0.0, si = 0.0;
+                for (x = 0; x < nin; ++x)
+                {
+                    int m = (kx * x) % P;
+                    sr += line[x] * d->cw[m];
+                    si -= line[x] * d->sw[m];
+                }
+                are[((size_t)z * nin + y) * H + kx] = sr;
+                aim[((size_t)z * nin + y) * H + kx] = si;
+            }
+        }
+    for (z = 0; z < nin; ++z)
+        for (ky = 0; ky < P; ++ky)
+            for (kx = 0; kx < H; ++kx)
+            {
+                double sr = 0.0, si = 0.0;
+                for (y = 0; y < nin; ++y)
+                {
+                    int    m = (ky * y) % P;
+                    double c = d->cw[m], s = -d->sw[m];
+                    double ar = are[((size_t)z * nin + y) * H + kx], ai = aim[((size_t)z * nin + y) * H + kx];
+                    sr += ar * c - ai * s;
+                    si += ar * s + ai * c;
+                }
+                bre[((size_t)z * P + ky) * H + kx] = sr;
+                bim[((size_t)z * P + ky) * H + kx] = si;
+            }
+    for (kz = 0; kz < P; ++kz)
+        for (ky = 0; ky < P; ++ky)
+            for (kx = 0; kx < H; ++kx)
+            {
+                double sr = 0.0, si = 0.0;
+                for (z = 0; z < nin; ++z)
+                {
+                    int    m = (kz * z) % P;
+                    double c = d->cw[m], s = -d->sw[m];
+                    double br = bre[((size_t)z * P + ky) * H + kx], bi = bim[((size_t)z * P + ky) * H + kx];
+                    sr += br * c - bi * s;
+                    si += br * s + bi * c;
+                }
+                ore[((size_t)kz * P + ky) * H + kx] = sr;
+                oim[((size_t)kz * P + ky) * H + kx] = si;
+            }
+    free(are);
+    free(aim);
+    free(bre);
+    free(bim);
+}
+
+/* c2r restricted to the output window [n-1, 2n-2]^3; out += scale * result / P^3 */
+static void dft_inverse_window(const dft_t* d, const double* cre, const double* cim, double scale, double* out)
+{
+    const int n = d->n, P = d->P, H = d->H;
+    size_t    sd = (size_t)n * P * H, se = (size_t)n * n * H;
+    double *  dre = (double*)malloc(sizeof(double) * sd), *dim = (double*)malloc(sizeof(double) * sd);
+    double *  ere = (double*)malloc(sizeof(double) * se), *eim = (double*)malloc(sizeof(double) * se);
+    const double f = scale / ((double)P * P * P);
+    int       x, y, z, kx, ky, kz;
+    for (z = 0; z < n; ++z)
+        for (ky = 0; ky < P; ++ky)
+            for (kx = 0; kx < H; ++kx)
+            {
+                double sr = 0.0, si = 0.0;
+                for (kz = 0; kz < P; ++kz)
+                {
+                    int    m = (kz * (n - 1 + z)) % P;
+                    double c = d->cw[m], s = d->sw[m];
+                    double ar = cre[((size_t)kz * P + ky) * H + kx], ai = cim[((size_t)kz * P + ky) * H + kx];
+                    sr += ar * c - ai * s;
+                    si += ar * s + ai * c;
+                }
+                dre[((size_t)z * P + ky) * H + kx] = sr;
+                dim[((size_t)z * P + ky) * H + kx] = si;
+            }
+    for (z = 0; z < n; ++z)
+        for (y = 0; y < n; ++y)
+            for (kx = 0; kx < H; ++kx)
+            {
+                double sr = 0.0, si = 0.0;
+                for (ky = 0; ky < P; ++ky)
+                {
+                    int    m = (ky * (n - 1 + y)) % P;
+                    double c = d->cw[m], s = d->sw[m];
+                    double ar = dre[((size_t)z * P + ky) * H + kx], ai = dim[((size_t)z * P + ky) * H + kx];
+                    sr += ar * c - ai * s;
+                    si += ar * s + ai * c;
+                }
+                ere[((size_t)z * n + y) * H + kx] = sr;
+                eim[((size_t)z * n + y) * H + kx] = si;
+            }
+    for (z = 0; z < n; ++z)
+        for (y = 0; y < n; ++y)
+            for (x = 0; x < n; ++x)
+            {
+                double s = 0.0;
+                for (kx = 0; kx < H; ++kx)
+                {
+                    int    m = (kx * (n - 1 + x)) % P;
+                    double w = (kx == 0 || (P % 2 == 0 && 2 * kx == P)) ? 1.0 : 2.0;
+                    double ar = ere[((size_t)z * n + y) * H + kx], ai = eim[((size_t)z * n + y) * H + kx];
+                    s += w * (ar * d->cw[m] - ai * d->sw[m]);
+                }
+                out[((size_t)z * n + y) * n + x] += f * s;
+            }
+    free(dre);
+    free(dim);
+    free(ere);
+    free(eim);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kernels: Laplace LGF or integrating factor                                 */
+/* ------------------------------------------------------------------------ */
+typedef struct
+{
+    int           is_if;
+    const double* g;
+    int           nmax;
+    double        alpha; /* current level alpha (IF) */
+} kernel_t;
+
+static double kernel_value(const kernel_t* k, int x, int y, int z, double* ivbuf, int ivmax)
+{
+    if (!k->is_if) return oracle_lgf_get(k->g, k->nmax, x, y, z);
+    (void)ivbuf;
+    (void)ivmax;
+    {
+        int     m = abs(x), t;
+        double* iv;
+        double  r;
+        if (abs(y) > m) m = abs(y);
+        if (abs(z) > m) m = abs(z);
+        iv = (double*)malloc(sizeof(double) * (m + 1));
+        oracle_ive(2.0 * k->alpha, m, iv);
+        r = iv[abs(x)] * iv[abs(y)] * iv[abs(z)];
+        (void)t;
+        free(iv);
+        return r;
+    }
+}
+
+/* Spectrum cache keyed by (level_diff, lgf block base) as LGF_Base::dft. */
+typedef struct
+{
+    int     d, bx, by, bz;
+    double *re, *im;
+} spec_entry;
+typedef struct
+{
+    spec_entry* e;
+    int         n, cap;
+} spec_cache;
+
+static void cache_clear(spec_cache* c)
+{
+    int i;
+    for (i = 0; i < c->n; ++i)
+    {
+        free(c->e[i].re);
+        free(c->e[i].im);
+    }
+    c->n = 0;
+}
+
+static const spec_entry* lgf_spectrum(spec_cache* c, const dft_t* d, const kernel_t* k, int ld,
+    const int base[3])
+{
+    int i;
+    for (i = 0; i < c->n; ++i)
+        if (c->e[i].d == ld && c->e[i].bx == base[0] && c->e[i].by == base[1] && c->e[i].bz == base[2])
+            return &c->e[i];
+    if (c->n == c->cap)
+    {
+        c->cap = c->cap ? 2 * c->cap : 64;
+        c->e = (spec_entry*)realloc(c->e, sizeof(spec_entry) * c->cap);
+    }
+    {
+        spec_entry* s = &c->e[c->n++];
+        const int   P = d->P, step = 1 << ld;
+        size_t      P3 = (size_t)P * P * P;
+        double*     blk = (double*)malloc(sizeof(double) * P3);
+        int         x, y, z;
+        s->d = ld;
+        s->bx = base[0];
+        s->by = base[1];
+        s->bz = base[2];
+        s->re = (double*)malloc(sizeof(double) * spec_size(d));
+        s->im = (double*)malloc(sizeof(double) * spec_size(d));
+        for (z = 0; z < P; ++z)
+            for (y = 0; y < P; ++y)
+                for (x = 0; x < P; ++x)
+                    blk[((size_t)z * P + y) * P + x] = kernel_value(
+                        k, (base[0] + x) * step, (base[1] + y) * step, (base[2] + z) * step, NULL, 0);
+        dft_forward(d, blk, P, s->re, s->im);
+        free(blk);
+        return s;
+    }
+}
+
+void oracle_block_conv(const double* g, int nmax, int n, const double* src, const int shift[3],
+    int step, double scale, double* tgt)
+{
+    dft_t      d;
+    spec_cache c = {0, 0, 0};
+    kernel_t   k = {0, g, nmax, 0.0};
+    int        ld = 0, base[3], i;
+    size_t     S;
+    double *   sre, *sim, *are, *aim;
+    const spec_entry* e;
+    while ((1 << ld) < step) ++ld;
+    dft_init(&d, n);
+    S = spec_size(&d);
+    for (i = 0; i < 3; ++i) base[i] = shift[i] - (n - 1);
+    e = lgf_spectrum(&c, &d, &k, ld, base);
+    sre = (double*)malloc(sizeof(double) * S);
+    sim = (double*)malloc(sizeof(double) * S);
+    are = (double*)calloc(S, sizeof(double));
+    aim = (double*)calloc(S, sizeof(double));
+    dft_forward(&d, src, n, sre, sim);
+    for (i = 0; i < (int)S; ++i)
+    {
+        are[i] += e->re[i] * sre[i] - e->im[i] * sim[i];
+        aim[i] += e->re[i] * sim[i] + e->im[i] * sre[i];
+    }
+    dft_inverse_window(&d, are, aim, scale, tgt);
+    free(sre);
+    free(sim);
+    free(are);
+    free(aim);
+    cache_clear(&c);
+    free(c.e);
+    dft_free(&d);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FMM and Poisson solver                                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct
+{
+    const oracle_tree* t;
+    int                n;
+    size_t             n3;
+    dft_t              dft;
+    spec_cache         cache;
+    kernel_t           ker;
+    int                faithful;
+    int                intrp_order;
+    /* transfer matrices: nli (anterp/interp) and cell-centre (restrict/interp) */
+    double *nli_sub[2], *nli_subT[2], *cc_restrict[2], *cc_interpT[2];
+    /* working fields [nb][n^3] */
+    double *source_tmp, *target_tmp, *correction_tmp, *fmm_s, *fmm_t;
+    /* per-pass source spectra (non-faithful mode) */
+    double **src_re, **src_im;
+    int*    lvl_begin; /* slots of level l: [lvl_begin[l], lvl_begin[l+1]) */
+} ostate;
+
+#define BLK(f, s) ((f) + (size_t)(s) * o->n3)
+#define IS_LEAF(s) (o->t->flags[s] & 1u)
+#define IS_CORR(s) (o->t->flags[s] & 2u)
+#define MS(s, idx) ((o->t->mask_src[s] >> (idx)) & 1ull)
+#define MT(s, idx) ((o->t->mask_tgt[s] >> (idx)) & 1ull)
+
+static void ostate_init(ostate* o, const oracle_tree* t, int faithful, int intrp_order)
+{
+    int     n = t->n, b, l;
+    double* A = (double*)malloc(sizeof(double) * n * 2 * n);
+    double* S = (double*)malloc(sizeof(double) * n * n);
+    memset(o, 0, sizeof(*o));
+    o->t = t;
+    o->n = n;
+    o->n3 = (size_t)n * n * n;
+    o->faithful = faithful;
+    o->intrp_order = intrp_order;
+    dft_init(&o->dft, n);
+    oracle_lagrange_matrix(n, 10, 0, A);
+    for (b = 0; b < 2; ++b)
+    {
+        o->nli_sub[b] = (double*)malloc(sizeof(double) * n * n);
+        o->nli_subT[b] = (double*)malloc(sizeof(double) * n * n);
+        sub_matrix(A, n, b, o->nli_sub[b]);
+        transpose(o->nli_sub[b], n, o->nli_subT[b]);
+    }
+    oracle_lagrange_matrix(n, intrp_order, 1, A);
+    for (b = 0; b < 2; ++b)
+    {
+        o->cc_restrict[b] = (double*)malloc(sizeof(double) * n * n);
+        o->cc_interpT[b] = (double*)malloc(sizeof(double) * n * n);
+        restrict_matrix(n, b, 1, o->cc_restrict[b]);
+        sub_matrix(A, n, b, S);
+        transpose(S, n, o->cc_interpT[b]);
+    }
+    free(A);
+    free(S);
+    o->source_tmp = (double*)calloc(o->n3 * t->nb, sizeof(double));
+    o->target_tmp = (double*)calloc(o->n3 * t->nb, sizeof(double));
+    o->correction_tmp = (double*)calloc(o->n3 * t->nb, sizeof(double));
+    o->fmm_s = (double*)calloc(o->n3 * t->nb, sizeof(double));
+    o->fmm_t = (double*)calloc(o->n3 * t->nb, sizeof(double));
+    o->src_re = (double**)calloc(t->nb, sizeof(double*));
+    o->src_im = (double**)calloc(t->nb, sizeof(double*));
+    o->lvl_begin = (int*)calloc(t->depth + 2, sizeof(int));
+    for (l = 0; l <= t->depth; ++l)
+    {
+        int s = 0;
+        while (s < t->nb && t->level[s] < l) ++s;
+        o->lvl_begin[l] = s;
+    }
+}
+
+static void ostate_free(ostate* o)
+{
+    int b, s;
+    for (b = 0; b < 2; ++b)
+    {
+        free(o->nli_sub[b]);
+        free(o->nli_subT[b]);
+        free(o->cc_restrict[b]);
+        free(o->cc_interpT[b]);
+    }
+    for (s = 0; s < o->t->nb; ++s)
+    {
+        free(o->src_re[s]);
+        free(o->src_im[s]);
+    }
+    free(o->src_re);
+    free(o->src_im);
+    free(o->source_tmp);
+    free(o->target_tmp);
+    free(o->correction_tmp);
+    free(o->fmm_s);
+    free(o->fmm_t);
+    free(o->lvl_begin);
+    cache_clear(&o->cache);
+    free(o->cache.e);
+    dft_free(&o->dft);
+}
+
+static void child_bits(int c, int* bx, int* by, int* bz)
+{
+    *bx = c & 1;
+    *by = (c >> 1) & 1;
+    *bz = (c >> 2) & 1;
+}
+
+/* fmm_tt + execute_fwrd_field: accumulate G_hat * S_hat into (are, aim) */
+static void fmm_tt(ostate* o, int s, int tg, int ld, double* are, double* aim)
+{
+    const int*  bb = o->t->bbase;
+    int         base[3], i;
+    size_t      S = spec_size(&o->dft), k;
+    const spec_entry* e;
+    double *    sre, *sim;
+    int         own = 0;
+    for (i = 0; i < 3; ++i) base[i] = (bb[3 * tg + i] - bb[3 * s + i]) - (o->n - 1);
+    e = lgf_spectrum(&o->cache, &o->dft, &o->ker, ld, base);
+    if (o->faithful || !o->src_re[s])
+    {
+        sre = (double*)malloc(sizeof(double) * S);
+        sim = (double*)malloc(sizeof(double) * S);
+        dft_forward(&o->dft, BLK(o->fmm_s, s), o->n, sre, sim);
+        if (o->faithful) own = 1;
+        else
+        {
+            o->src_re[s] = sre;
+            o->src_im[s] = sim;
+        }
+    }
+    else
+    {
+        sre = o->src_re[s];
+        sim = o->src_im[s];
+    }
+    for (k = 0; k < S; ++k)
+    {
+        are[k] += e->re[k] * sre[k] - e->im[k] * sim[k];
+        aim[k] += e->re[k] * sim[k] + e->im[k] * sre[k];
+    }
+    if (own)
+    {
+        free(sre);
+        free(sim);
+    }
+}
+
+/* compute_influence_field (fmm.hpp:988-1027) for one target */
+static long influence_field(ostate* o, int tg, int L, int idx, int neighbor_only, double scale)
+{
+    const oracle_tree* t = o->t;
+    size_t             S = spec_size(&o->dft);
+    double *           are = (double*)calloc(S, sizeof(double)), *aim = (double*)calloc(S, sizeof(double));
+    long               count = 0;
+    int                i;
+    const int          lvl = t->level[tg];
+    if (lvl == L)
+        for (i = 0; i < 27; ++i)
+        {
+            int s = t->neighbor[27 * tg + i];
+            if (s >= 0 && MS(s, idx))
+            {
+                fmm_tt(o, s, tg, 0, are, aim);
+                ++count;
+            }
+        }
+    if (!neighbor_only)
+        for (i = 0; i < t->influence_num[tg]; ++i)
+        {
+            int s = t->influence[189 * tg + i];
+            if (s >= 0 && MS(s, idx))
+            {
+                fmm_tt(o, s, tg, L - lvl, are, aim);
+                ++count;
+            }
+        }
+    if (count) dft_inverse_window(&o->dft, are, aim, scale, BLK(o->fmm_t, tg));
+    free(are);
+    free(aim);
+    return count;
+}
+
+/* Fmm::apply (fmm.hpp:729-870) */
+static void fmm_apply(ostate* o, const double* Source, double* Target, int L, int nl, double add_scale)
+{
+    const oracle_tree* t = o->t;
+    const int          rl = L - t->base_level;
+    const int          idx = 15 + 2 * rl + nl;
+    const int          neighbor_only = o->ker.is_if;
+    const double       dx_level = t->dx_base / pow(2.0, rl);
+    int                l, s, c, i, j, k;
+    const int          n = o->n;
+    /* init zero (levels L..0) */
+    for (l = L; l >= 0; --l)
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+        {
+            if (MS(s, idx)) memset(BLK(o->fmm_s, s), 0, sizeof(double) * o->n3);
+            if (MT(s, idx)) memset(BLK(o->fmm_t, s), 0, sizeof(double) * o->n3);
+        }
+    /* init copy: interior only */
+    for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
+        if (MS(s, idx))
+            for (k = 1; k < n - 1; ++k)
+                for (j = 1; j < n - 1; ++j)
+                    for (i = 1; i < n - 1; ++i)
+                    {
+                        size_t q = ((size_t)k * n + j) * n + i;
+                        BLK(o->fmm_s, s)[q] = BLK(Source, s)[q];
+                    }
+    for (s = 0; s < t->nb; ++s)
+    {
+        free(o->src_re[s]);
+        free(o->src_im[s]);
+        o->src_re[s] = o->src_im[s] = NULL;
+    }
+    if (!neighbor_only)
+    {
+        /* anterpolation (fmm.hpp:1128-1165) */
+        for (l = L - 1; l >= 0; --l)
+            for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+            {
+                if (!MS(s, idx)) continue;
+                for (c = 0; c < 8; ++c)
+                {
+                    int ch = t->children[8 * s + c], bx, by, bz;
+                    if (ch < 0 || !MS(ch, idx)) continue;
+                    child_bits(c, &bx, &by, &bz);
+                    transfer3(n, o->nli_sub[bx], o->nli_sub[by], o->nli_sub[bz], BLK(o->fmm_s, ch),
+                        BLK(o->fmm_s, s), 1);
+                }
+            }
+    }
+    /* Bx over targets at levels L..0 (L only for neighbour-only kernels) */
+    for (l = L; l >= (neighbor_only ? L : 0); --l)
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+        {
+            if (!MT(s, idx)) continue;
+            influence_field(o, s, L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
+        }
+    if (!neighbor_only)
+    {
+        /* interpolation (fmm.hpp:1108-1126) */
+        for (l = 1; l < L; ++l)
+            for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+            {
+                if (!MT(s, idx)) continue;
+                for (c = 0; c < 8; ++c)
+                {
+                    int ch = t->children[8 * s + c], bx, by, bz;
+                    if (ch < 0 || !MT(ch, idx)) continue;
+                    child_bits(c, &bx, &by, &bz);
+                    transfer3(n, o->nli_subT[bx], o->nli_subT[by], o->nli_subT[bz], BLK(o->fmm_t, s),
+                        BLK(o->fmm_t, ch), 1);
+                }
+            }
+    }
+    /* fmm_add_equal */
+    for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
+        if (MT(s, idx))
+        {
+            size_t q;
+            for (q = 0; q < o->n3; ++q) BLK(Target, s)[q] += BLK(o->fmm_t, s)[q] * add_scale;
+        }
+}
+
+/* halo exchange at one level: buffer cells <- neighbour interiors */
+static void buffer_exchange(ostate* o, double* f, int l)
+{
+    const oracle_tree* t = o->t;
+    const int          n = o->n, e = n - 2;
+    int                s, i, j, k;
+    for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+        for (k = 0; k < n; ++k)
+            for (j = 0; j < n; ++j)
+                for (i = 0; i < n; ++i)
+                {
+                    int dx = (i == 0) ? -1 : (i == n - 1 ? 1 : 0);
+                    int dy = (j == 0) ? -1 : (j == n - 1 ? 1 : 0);
+                    int dz = (k == 0) ? -1 : (k == n - 1 ? 1 : 0);
+                    int nb;
+                    if (!dx && !dy && !dz) continue;
+                    nb = t->neighbor[27 * s + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+                    if (nb < 0) continue;
+                    BLK(f, s)[((size_t)k * n + j) * n + i] =
+                        BLK(f, nb)[((size_t)(k - dz * e) * n + (j - dy * e)) * n + (i - dx * e)];
+                }
+}
+
+/* cell-centred restriction of all children of octants at level ls into the parent */
+static void source_coarsify(ostate* o, double* f)
+{
+    const oracle_tree* t = o->t;
+    int                ls, s, c;
+    for (ls = t->depth - 2; ls >= t->base_level; --ls)
+        for (s = o->lvl_begin[ls]; s < o->lvl_begin[ls + 1]; ++s)
+            for (c = 0; c < 8; ++c)
+            {
+                int ch = t->children[8 * s + c], bx, by, bz;
+                if (ch < 0) continue;
+                child_bits(c, &bx, &by, &bz);
+                transfer3(o->n, o->cc_restrict[bx], o->cc_restrict[by], o->cc_restrict[bz], BLK(f, ch),
+                    BLK(f, s), 1);
+            }
+}
+
+/* cell-centred interpolation parent -> children (assignment) */
+static void cc_intrp_node(ostate* o, const double* from, double* to, int s, int correction_only)
+{
+    const oracle_tree* t = o->t;
+    int                c;
+    for (c = 0; c < 8; ++c)
+    {
+        int ch = t->children[8 * s + c], bx, by, bz;
+        if (ch < 0) continue;
+        if (correction_only && !IS_CORR(ch)) continue;
+        child_bits(c, &bx, &by, &bz);
+        transfer3(o->n, o->cc_interpT[bx], o->cc_interpT[by], o->cc_interpT[bz], BLK(from, s), BLK(to, ch), 0);
+    }
+}
+
+/* add_source_correction (cell_center_nli_intrp.hpp:171-238) */
+static void add_source_correction(ostate* o, int s, double dx, double omega)
+{
+    const oracle_tree* t = o->t;
+    const int          n = o->n;
+    int                c, i, j, k;
+    const double       inv = 1.0 / (dx * dx);
+    for (c = 0; c < 8; ++c)
+    {
+        int           ch = t->children[8 * s + c];
+        const double* tt;
+        double*       cc;
+        if (ch < 0) continue;
+        tt = BLK(o->target_tmp, ch);
+        cc = BLK(o->correction_tmp, ch);
+        for (k = 1; k < n - 1; ++k)
+            for (j = 1; j < n - 1; ++j)
+                for (i = 1; i < n - 1; ++i)
+                {
+                    size_t q = ((size_t)k * n + j) * n + i;
+                    cc[q] += 6.0 * tt[q] * inv;
+                    cc[q] -= tt[q - (size_t)n * n] * inv;
+                    cc[q] -= tt[q + (size_t)n * n] * inv;
+                    cc[q] -= tt[q - n] * inv;
+                    cc[q] -= tt[q + n] * inv;
+                    cc[q] -= tt[q + 1] * inv;
+                    cc[q] -= tt[q - 1] * inv;
+                    cc[q] += omega * omega * tt[q];
+                }
+    }
+}
+
+int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
+    int faithful, int intrp_order)
+{
+    ostate o_;
+    ostate* o = &o_;
+    int     l, s;
+    size_t  q;
+    ostate_init(o, t, faithful, intrp_order);
+    o->ker.is_if = 0;
+    o->ker.g = g;
+    o->ker.nmax = nmax;
+    /* copy_leaf with buffer */
+    for (s = 0; s < t->nb; ++s)
+        if (IS_LEAF(s)) memcpy(BLK(o->source_tmp, s), BLK(src, s), sizeof(double) * o->n3);
+    source_coarsify(o, o->source_tmp);
+    for (l = t->base_level; l < t->depth; ++l)
+    {
+        const double dx = t->dx_base / pow(2.0, l - t->base_level);
+        fmm_apply(o, o->source_tmp, o->target_tmp, l, 1, 1.0);
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+            cc_intrp_node(o, o->target_tmp, o->target_tmp, s, 0);
+        fmm_apply(o, o->source_tmp, o->target_tmp, l, 0, 1.0);
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+            if (IS_LEAF(s)) memcpy(BLK(dst, s), BLK(o->target_tmp, s), sizeof(double) * o->n3);
+        if (l == t->depth - 1) continue;
+        buffer_exchange(o, o->source_tmp, l);
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+            cc_intrp_node(o, o->source_tmp, o->correction_tmp, s, 1);
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s) add_source_correction(o, s, dx / 2.0, 0.0);
+        for (s = o->lvl_begin[l + 1]; s < o->lvl_begin[l + 2]; ++s)
+            for (q = 0; q < o->n3; ++q) BLK(o->source_tmp, s)[q] += BLK(o->correction_tmp, s)[q];
+    }
+    ostate_free(o);
+    return 0;
+}
+
+int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double alpha_base, int faithful,
+    int intrp_order)
+{
+    ostate o_;
+    ostate* o = &o_;
+    int     l, s;
+    ostate_init(o, t, faithful, intrp_order);
+    o->ker.is_if = 1;
+    for (s = 0; s < t->nb; ++s)
+        if (IS_LEAF(s)) memcpy(BLK(o->source_tmp, s), BLK(src, s), sizeof(double) * o->n3);
+    source_coarsify(o, o->source_tmp);
+    for (l = t->depth - 2; l >= t->base_level; --l)
+    {
+        buffer_exchange(o, o->source_tmp, l);
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+            cc_intrp_node(o, o->source_tmp, o->source_tmp, s, 1);
+    }
+    for (l = t->base_level; l < t->depth; ++l)
+    {
+        o->ker.alpha = alpha_base * pow(4.0, l - t->base_level);
+        cache_clear(&o->cache);
+        fmm_apply(o, o->source_tmp, o->target_tmp, l, 0, 1.0);
+        fmm_apply(o, o->source_tmp, o->target_tmp, l, 1, 1.0);
+        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+            memcpy(BLK(dst, s), BLK(o->target_tmp, s), sizeof(double) * o->n3);
+    }
+    ostate_free(o);
+    return 0;
+}
+
+long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const double* fmm_s, int L, int nl,
+    int ntargets, const int* targets, int faithful, double* fmm_t)
+{
+    ostate o_;
+    ostate* o = &o_;
+    long    pairs = 0;
+    int     i;
+    const int rl = L - t->base_level, idx = 15 + 2 * rl + nl;
+    const double dx_level = t->dx_base / pow(2.0, rl);
+    ostate_init(o, t, faithful, 5);
+    o->ker.is_if = 0;
+    o->ker.g = g;
+    o->ker.nmax = nmax;
+    memcpy(o->fmm_s, fmm_s, sizeof(double) * o->n3 * t->nb);
+    for (i = 0; i < ntargets; ++i)
+    {
+        int s = targets[i];
+        memset(BLK(o->fmm_t, s), 0, sizeof(double) * o->n3);
+        pairs += influence_field(o, s, L, idx, 0, dx_level * dx_level);
+        memcpy(fmm_t + (size_t)i * o->n3, BLK(o->fmm_t, s), sizeof(double) * o->n3);
+    }
+    ostate_free(o);
+    return pairs;
+}

@@ -1,0 +1,86 @@
+/* CPU FP64 oracle of the LGF fast-multipole path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against; only tests/, __graft_entry__.smoke() and the cpu_baseline
+ * leg of bench.py may load it.  The product (iblgf-amr_amd/) never links or
+ * calls it.
+ *
+ * Each function restates the reference algorithm it names (IBLGF-AMR file:line).
+ * The tree arrays it consumes are the flat block table exported by the host
+ * octree (slots ordered by level, then Morton key).
+ */
+#ifndef IBLGF_ORACLE_H
+#define IBLGF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_tree
+{
+    int             nb;          /* blocks with data */
+    int             n;           /* block extent + 2 buffer cells */
+    int             base_level;  /* tree base level */
+    int             depth;       /* tree depth (max level + 1) */
+    double          dx_base;
+    const int*      level;       /* [nb] */
+    const int*      coord;       /* [3nb] key coordinates */
+    const int*      bbase;       /* [3nb] block base, cells of its level */
+    const uint32_t* flags;       /* [nb] bit0 leaf, bit1 correction */
+    const int*      parent;      /* [nb] slot or -1 */
+    const int*      children;    /* [8nb] slot or -1 */
+    const int*      neighbor;    /* [27nb] slot or -1 */
+    const int*      influence;   /* [189nb] */
+    const int*      influence_num; /* [nb] */
+    const uint64_t* mask_src;    /* [nb] bit i = source mask of fmm mask idx i */
+    const uint64_t* mask_tgt;    /* [nb] */
+} oracle_tree;
+
+/* Laplace LGF table G(x>=y>=z>=0), size (N+1)(N+2)(N+3)/6
+ * (include/iblgf/lgf/lgf_table_gen/lgf_integrator.py:7-25 as the definition). */
+void   oracle_lgf_table(int nmax, double* g);
+double oracle_lgf_get(const double* g, int nmax, int x, int y, int z);
+double oracle_lgf_asym(int n1, int n2, int n3);
+/* e^{-x} I_k(x), k = 0..kmax */
+void   oracle_ive(double x, int kmax, double* out);
+
+/* Transfer matrices (row-major n x 2n), include/iblgf/fmm/fmm_nli.hpp:319-374 and
+ * include/iblgf/interpolation/cell_center_nli_intrp.hpp:677-787. */
+void oracle_lagrange_matrix(int n, int cap, int relative_pos, double* A);
+
+/* Brute-force direct LGF sum on a set of cells (for the single-level case):
+ * phi(x_t) = scale * sum_s G(x_t - x_s) q_s. */
+void oracle_direct_sum(const double* g, int nmax, int ns, const int* src_xyz, const double* q,
+    int nt, const int* tgt_xyz, double scale, double* phi);
+
+/* One block convolution (include/iblgf/utilities/convolution.hpp:226-304):
+ * tgt[n^3] += scale * window_{[n-1,2n-2]^3}( LGF_block (*) src ), where the LGF
+ * block spans [shift-(n-1), shift+(n-1)] sampled as G(step * x). */
+void oracle_block_conv(const double* g, int nmax, int n, const double* src, const int shift[3],
+    int step, double scale, double* tgt);
+
+/* PoissonSolver::apply_lgf for one scalar component
+ * (include/iblgf/solver/poisson/poisson.hpp:587-894), AMR2AMR masks,
+ * use_correction = true, subtract_non_leaf = false.
+ * src/dst: [nb][n^3].  faithful = 1 re-transforms the source for every pair
+ * as the reference does (convolution.hpp:259-260); 0 transforms each source
+ * once per FMM pass. */
+int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const double* src,
+    double* dst, int faithful, int intrp_order);
+
+/* PoissonSolver::apply_lgf_IF (poisson.hpp:427-494), integrating-factor kernel
+ * E_a with a_l = a_base * 4^(l-base). */
+int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double alpha_base,
+    int faithful, int intrp_order);
+
+/* Restricted single-target evaluation used for the CPU baseline: one FMM pass
+ * Bx for the given target slots only (counts pairs).  Returns pairs done. */
+long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const double* fmm_s,
+    int base_level_fmm, int nl, int ntargets, const int* targets, int faithful, double* fmm_t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,147 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker the HIP path is compared against.
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
+never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_lib = None
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+
+
+class OracleTree(ctypes.Structure):
+    _fields_ = [
+        ("nb", _I), ("n", _I), ("base_level", _I), ("depth", _I), ("dx_base", _D),
+        ("level", _P), ("coord", _P), ("bbase", _P), ("flags", _P), ("parent", _P), ("children", _P),
+        ("neighbor", _P), ("influence", _P), ("influence_num", _P), ("mask_src", _P), ("mask_tgt", _P),
+    ]
+
+
+def build():
+    """Compile liboracle.so with gcc (oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_lgf_table.argtypes = [_I, _P]
+        L.oracle_lgf_get.argtypes = [_P, _I, _I, _I, _I]
+        L.oracle_lgf_get.restype = _D
+        L.oracle_lgf_asym.argtypes = [_I, _I, _I]
+        L.oracle_lgf_asym.restype = _D
+        L.oracle_ive.argtypes = [_D, _I, _P]
+        L.oracle_lagrange_matrix.argtypes = [_I, _I, _I, _P]
+        L.oracle_direct_sum.argtypes = [_P, _I, _I, _P, _P, _I, _P, _D, _P]
+        L.oracle_block_conv.argtypes = [_P, _I, _I, _P, _P, _I, _D, _P]
+        L.oracle_apply_lgf.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _P, _I, _I]
+        L.oracle_apply_if.argtypes = [ctypes.POINTER(OracleTree), _P, _P, _D, _I, _I]
+        L.oracle_bx_sample.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _I, _I, _I, _P, _I, _P]
+        L.oracle_bx_sample.restype = ctypes.c_long
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+_TABLES = {}
+
+
+def lgf_table(nmax=100):
+    if nmax not in _TABLES:
+        g = np.zeros((nmax + 1) * (nmax + 2) * (nmax + 3) // 6, np.float64)
+        lib().oracle_lgf_table(nmax, _p(g))
+        _TABLES[nmax] = g
+    return _TABLES[nmax]
+
+
+def lgf_get(x, y, z, nmax=100):
+    return lib().oracle_lgf_get(_p(lgf_table(nmax)), nmax, int(x), int(y), int(z))
+
+
+def lgf_asym(x, y, z):
+    return lib().oracle_lgf_asym(int(x), int(y), int(z))
+
+
+def ive(x, kmax):
+    out = np.zeros(kmax + 1)
+    lib().oracle_ive(float(x), int(kmax), _p(out))
+    return out
+
+
+def lagrange_matrix(n, cap, rp):
+    A = np.zeros((n, 2 * n))
+    lib().oracle_lagrange_matrix(int(n), int(cap), int(rp), _p(A))
+    return A
+
+
+def direct_sum(src_xyz, q, tgt_xyz, scale, nmax=100):
+    sx = np.ascontiguousarray(src_xyz, np.int32)
+    tx = np.ascontiguousarray(tgt_xyz, np.int32)
+    qq = np.ascontiguousarray(q, np.float64)
+    phi = np.zeros(len(tx))
+    lib().oracle_direct_sum(_p(lgf_table(nmax)), nmax, len(sx), _p(sx), _p(qq), len(tx), _p(tx), float(scale), _p(phi))
+    return phi
+
+
+def block_conv(src, shift, step=1, scale=1.0, nmax=100):
+    n = src.shape[0]
+    s = np.ascontiguousarray(src, np.float64)
+    t = np.zeros_like(s)
+    sh = np.ascontiguousarray(shift, np.int32)
+    lib().oracle_block_conv(_p(lgf_table(nmax)), nmax, n, _p(s), _p(sh), int(step), float(scale), _p(t))
+    return t
+
+
+class Tree:
+    """Oracle view of an exported block table (dict of numpy arrays from iblgf_amd.Tree.export)."""
+
+    def __init__(self, arrays, info, n):
+        self.a = {k: np.ascontiguousarray(v) for k, v in arrays.items()}
+        self.info = info
+        self.n = n
+        a = self.a
+        self.s = OracleTree(
+            info["nblocks"], n, info["base_level"], info["depth"], info["dx_base"],
+            _p(a["level"]), _p(a["coord"]), _p(a["bbase"]), _p(a["flags"]), _p(a["parent"]), _p(a["children"]),
+            _p(a["neighbor"]), _p(a["influence"]), _p(a["influence_num"]), _p(a["mask_src"]), _p(a["mask_tgt"]),
+        )
+
+    def apply_lgf(self, src, faithful=False, intrp_order=5, nmax=100):
+        s = np.ascontiguousarray(src, np.float64)
+        dst = np.zeros_like(s)
+        rc = lib().oracle_apply_lgf(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(s), _p(dst),
+                                    1 if faithful else 0, int(intrp_order))
+        assert rc == 0
+        return dst
+
+    def apply_if(self, src, alpha_base, faithful=False, intrp_order=5):
+        s = np.ascontiguousarray(src, np.float64)
+        dst = np.zeros_like(s)
+        rc = lib().oracle_apply_if(ctypes.byref(self.s), _p(s), _p(dst), float(alpha_base), 1 if faithful else 0,
+                                   int(intrp_order))
+        assert rc == 0
+        return dst
+
+    def bx_sample(self, fmm_s, L, nl, targets, faithful=True, nmax=100):
+        fs = np.ascontiguousarray(fmm_s, np.float64)
+        tg = np.ascontiguousarray(targets, np.int32)
+        out = np.zeros((len(tg),) + fs.shape[1:])
+        pairs = lib().oracle_bx_sample(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(fs), int(L), int(nl),
+                                       len(tg), _p(tg), 1 if faithful else 0, _p(out))
+        return pairs, out
