@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X LGF fast-multipole Poisson solve.
+
+Workload: the adaptive vortex-ring octree (8^3 base blocks of 16^3 cells plus
+3 refinement levels, 16 seeded Gaussian-core rings, seed 1001).  One step is
+one vector Poisson solve: PoissonSolver::apply_lgf over the three vorticity
+components, i.e. three scalar LGF solves over the whole tree, all levels.
+Inputs are resident in HBM before the timed region; kernel spectra are built
+during warm-up (the reference times its second, warm solve).
+
+Prints one JSON line with the throughput (solves/s), the roofline of the
+dominant kernel class (timed live with HIP events on the solver's stream
+inside the timed region) and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "iblgf-amr_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "LGF solves/s + HBM/MFMA roofline %, batched-mixed vs plain FP64 at relL2≤1e-6"
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64, vector and matrix
+PEAK_HBM_GBS = 8000.0
+
+
+def unit_flops(n):
+    """Implementation-independent work per unit at the reference transform size P = 2n-1."""
+    P = 2 * n - 1
+    H = P // 2 + 1
+    f_fwd = 4 * n ** 3 * H + 8 * n ** 2 * H * P + 8 * n * H * P ** 2
+    return dict(forward_dft=f_fwd, inverse_dft=f_fwd, spectral_mac=8 * H * P ** 2, transfer=6 * n ** 4)
+
+
+def cpu_baseline(wl, counts, budget_s):
+    """FP64 oracle in the reference's faithful mode (per-pair source transform)
+    on a bounded sample of the finest-level FMM pass, scaled by pair count."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    a, info, n = wl.arrays, wl.info, wl.tree.n
+    ot = O.Tree(a, info, n)
+    L = info["depth"] - 1
+    nl = 1
+    idx = 15 + 2 * (L - info["base_level"]) + nl
+    lvl = a["level"]
+    tmask = (a["mask_tgt"] >> np.uint64(idx)) & np.uint64(1)
+    targets = [int(s) for s in np.nonzero((lvl == L) & (tmask == 1))[0]]
+    rng = np.random.default_rng(0)
+    fmm_s = rng.standard_normal((info["nblocks"], n, n, n))
+    pairs1, sec1, _ = ot.bx_sample(fmm_s, L, nl, targets[:2], faithful=True)
+    per_target = sec1 / 2.0
+    k = max(2, min(len(targets), int(budget_s / max(per_target, 1e-6))))
+    pairs, sec, _ = ot.bx_sample(fmm_s, L, nl, targets[:k], faithful=True)
+    t_solve = sec / pairs * counts["n_pair"]
+    return {
+        "value": 1.0 / t_solve,
+        "unit": "solves/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"FP64 oracle, faithful mode (source re-transformed per pair as convolution.hpp:259-260), "
+                   f"Bx of the finest-level pass over {k} targets = {pairs} pairs in {sec:.1f} s (warm spectra), "
+                   f"scaled by pair count to one full solve ({counts['n_pair']} pairs); 1 thread"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design in DESIGN.md, not built)", file=sys.stderr)
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+
+    import numpy as np
+
+    from iblgf_amd import PoissonSolver, hip_events, setups
+
+    wl = setups.VortexRings(seed=1001)
+    srcs = [wl.source(c) for c in range(3)]
+    solver = PoissonSolver(wl.tree)
+    fin = [solver.field() for _ in range(3)]
+    fout = [solver.field() for _ in range(3)]
+    for f, s in zip(fin, srcs):
+        solver.upload(f, s)
+    solver.prepare()
+
+    def step():
+        for c in range(3):
+            solver.apply_lgf(fin[c], fout[c])
+
+    for _ in range(args.warmup):
+        step()
+    solver.sync()
+
+    stream = solver.stream()
+    ev0, ev1 = hip_events.Event(), hip_events.Event()
+    solver.timing(True)
+    solver.sync()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    solver.sync()
+    t1 = time.perf_counter()
+    gpu_ms = ev0.elapsed_ms(ev1)
+    kt = solver.timing_read()
+    solver.timing(False)
+    counts = solver.counts()
+
+    wall = t1 - t0
+    solves = 3 * args.steps
+    value = solves / wall
+    n = wl.tree.n
+    uf = unit_flops(n)
+    kernels = {}
+    for k, (ms, launches, work) in kt.items():
+        if k == "other":
+            kernels[k] = {"ms_per_solve": ms / solves, "launches_per_solve": launches / solves}
+            continue
+        fl = work * uf[k]
+        kernels[k] = {
+            "ms_per_solve": ms / solves,
+            "launches_per_solve": launches / solves,
+            "work_per_solve": work / solves,
+            "achieved_tflops": (fl / (ms * 1e-3) / 1e12) if ms > 0 else 0.0,
+        }
+    dom = max((k for k in kernels if k != "other"), key=lambda k: kernels[k]["ms_per_solve"])
+    ms, launches, work = kt[dom]
+    achieved = work * uf[dom] / (ms * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+    roofline = {
+        "bound": "mfma",
+        "kernel": dom,
+        "achieved": achieved,
+        "peak": PEAK_FP64_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": achieved / PEAK_FP64_TFLOPS,
+        "traffic": traffic,
+        "avg_launch_ms": ms / max(launches, 1),
+        "flops_per_launch": work * uf[dom] / max(launches, 1),
+    }
+    flops_per_solve = sum(counts[c] * uf[k] for c, k in (("n_src", "forward_dft"), ("n_tgt", "inverse_dft"),
+                                                             ("n_pair", "spectral_mac"), ("n_transfer", "transfer")))
+    leaf_cells = counts["n_leaf"] * wl.block_extent ** 3
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": ("adaptive vortex-ring octree Poisson: 8^3 base blocks of 16^3 cells + 3 refinement levels, "
+                         "16 seeded Gaussian-core rings (seed 1001); one step = 3 vorticity components"),
+            "precision": "plain FP64",
+            "blocks": wl.info["nblocks"],
+            "leaves": wl.info["nleaves"],
+            "leaf_cells": leaf_cells,
+            "points_per_s": leaf_cells * value,
+            "pairs_per_solve": counts["n_pair"],
+            "flops_per_solve": flops_per_solve,
+            "solve_tflops": flops_per_solve * value / 1e12,
+            "gpu_event_ms_per_step": gpu_ms / args.steps,
+        },
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": None,
+    }
+    if rank == 0 and not args.no_cpu_baseline:
+        # counts cover the FMM passes of one Laplace solve (each pass plan is built once)
+        result["cpu_baseline"] = cpu_baseline(wl, counts, args.cpu_seconds)
+    if rank == 0:
+        print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -334,6 +334,41 @@ int iblgf_gpu_stream(iblgf_gpu_ctx* ctx, void** stream)
     return IBLGF_OK;
 }
 
+int iblgf_gpu_timing_enable(iblgf_gpu_ctx* ctx, int enable)
+{
+    NEED_CTX("iblgf_gpu_timing_enable");
+    return guarded(ctx, [&] {
+        ctx->solver->sync();
+        ctx->solver->timer().enable(enable != 0);
+    });
+}
+
+int iblgf_gpu_timing_read(iblgf_gpu_ctx* ctx, double* ms5, long* launches5, long* work5)
+{
+    NEED_CTX("iblgf_gpu_timing_read");
+    return guarded(ctx, [&] {
+        ctx->solver->sync();
+        auto& t = ctx->solver->timer();
+        t.collect();
+        for (int k = 0; k < KC_COUNT; ++k)
+        {
+            if (ms5) ms5[k] = t.total_ms[k];
+            if (launches5) launches5[k] = t.launches[k];
+            if (work5) work5[k] = t.work[k];
+        }
+    });
+}
+
+int iblgf_gpu_timing_reset(iblgf_gpu_ctx* ctx)
+{
+    NEED_CTX("iblgf_gpu_timing_reset");
+    return guarded(ctx, [&] {
+        ctx->solver->sync();
+        ctx->solver->timer().collect();
+        ctx->solver->timer().reset();
+    });
+}
+
 int iblgf_gpu_counts(iblgf_gpu_ctx* ctx, long* c)
 {
     NEED_CTX("iblgf_gpu_counts");

@@ -47,6 +47,72 @@ template struct DevArray<double2>;
 template struct DevArray<MacGroup>;
 template struct DevArray<MacSrc>;
 
+// ---------------------------------------------------------------------------
+// Kernel timer
+// ---------------------------------------------------------------------------
+KernelTimer::~KernelTimer()
+{
+    for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+void KernelTimer::enable(bool on)
+{
+    on_ = on;
+    if (on && pool_.empty())
+        for (int i = 0; i < 2048; ++i) pool_.push_back(next_event());
+    used_ = 0;
+    recs_.clear();
+}
+
+hipEvent_t KernelTimer::next_event()
+{
+    hipEvent_t e;
+    IBLGF_HIP(hipEventCreate(&e));
+    return e;
+}
+
+void KernelTimer::begin(hipStream_t s)
+{
+    if (!on_) return;
+    if (used_ + 2 > pool_.size())
+        for (int i = 0; i < 1024; ++i) pool_.push_back(next_event());
+    IBLGF_HIP(hipEventRecord(pool_[used_], s));
+}
+
+void KernelTimer::end(hipStream_t s, int kind, long w)
+{
+    if (!on_) return;
+    IBLGF_HIP(hipEventRecord(pool_[used_ + 1], s));
+    recs_.push_back(Rec{kind, used_, w});
+    used_ += 2;
+}
+
+void KernelTimer::collect()
+{
+    for (const auto& r : recs_)
+    {
+        float ms = 0.f;
+        IBLGF_HIP(hipEventElapsedTime(&ms, pool_[r.ev], pool_[r.ev + 1]));
+        total_ms[r.kind] += ms;
+        launches[r.kind] += 1;
+        work[r.kind] += r.work;
+    }
+    recs_.clear();
+    used_ = 0;
+}
+
+void KernelTimer::reset()
+{
+    for (int k = 0; k < KC_COUNT; ++k)
+    {
+        total_ms[k] = 0.0;
+        launches[k] = 0;
+        work[k] = 0;
+    }
+    recs_.clear();
+    used_ = 0;
+}
+
 GpuSolver::GpuSolver(int device) : device_(device)
 {
     int count = 0;
@@ -326,6 +392,7 @@ void GpuSolver::build_pass(PassPlan& p, int L, int nl, bool nbonly)
             }
             p.antrp[l].upload(ent);
         }
+        p.ntransfer_antrp = p.ntransfer;
         for (int l = 1; l < L; ++l)
         {
             std::vector<int> ent;
@@ -549,24 +616,42 @@ void GpuSolver::run_pass(PassPlan& p, int src_field, int dst_field, double scale
 {
     double* fmm_s = F(f_fmm_s_);
     double* fmm_t = F(f_fmm_t_);
+    timer_.begin(stream_);
     launch::zero_blocks(stream_, n_, fmm_s, p.zero_s.p, p.zero_s.size());
     launch::zero_blocks(stream_, n_, fmm_t, p.zero_t.p, p.zero_t.size());
     launch::copy_blocks(stream_, n_, fmm_s, F(src_field), p.copy_s.p, p.copy_s.size(), true);
+    timer_.end(stream_, KC_OTHER, 0);
     if (!p.neighbor_only)
+    {
+        timer_.begin(stream_);
         for (int l = p.L - 1; l >= 0; --l)
             launch::transfer_gather(stream_, n_, fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9, m_nli_anterp_.p);
+        timer_.end(stream_, KC_TRANSFER, p.ntransfer_antrp);
+    }
+    timer_.begin(stream_);
     launch::forward_dft(stream_, n_, fmm_s, p.fwd.p, p.fwd.size(), sspec_.p);
+    timer_.end(stream_, KC_FORWARD, p.fwd.size());
     for (auto& m : p.mac)
     {
         double2* gh = laplace ? ghat_laplace(m.d) : ghat_if(alpha);
+        timer_.begin(stream_);
         launch::spectral_mac(stream_, n_, m.groups.p, m.groups.size(), m.srcs.p, gh, sspec_.p, tspec_.p);
+        timer_.end(stream_, KC_MAC, m.npairs);
     }
     const double P = 2.0 * n_ - 1.0;
+    timer_.begin(stream_);
     launch::inverse_dft(stream_, n_, tspec_.p, p.inv.p, p.inv.size(), fmm_t, scale / (P * P * P));
+    timer_.end(stream_, KC_INVERSE, p.inv.size());
     if (!p.neighbor_only)
+    {
+        timer_.begin(stream_);
         for (int l = 1; l < p.L; ++l)
             launch::transfer_scatter(stream_, n_, fmm_t, fmm_t, p.intrp[l].p, p.intrp[l].size() / 3, m_nli_interp_.p, true);
+        timer_.end(stream_, KC_TRANSFER, p.ntransfer - p.ntransfer_antrp);
+    }
+    timer_.begin(stream_);
     launch::axpy_blocks(stream_, n_, F(dst_field), fmm_t, p.add.p, p.add.size(), 1.0);
+    timer_.end(stream_, KC_OTHER, 0);
 }
 
 // PoissonSolver::apply_lgf (include/iblgf/solver/poisson/poisson.hpp:587-894)
@@ -577,25 +662,40 @@ void GpuSolver::apply_lgf(int src, int dst)
     double* st = F(f_src_tmp_);
     double* tt = F(f_tgt_tmp_);
     double* ct = F(f_corr_tmp_);
+    timer_.begin(stream_);
     fill_zero(st);
     fill_zero(tt);
     fill_zero(ct);
     launch::copy_blocks(stream_, n_, st, F(src), p_leaves_.p, p_leaves_.size(), false);
+    timer_.end(stream_, KC_OTHER, 0);
+    timer_.begin(stream_);
+    long ncoarse = 0;
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
+    {
         launch::transfer_gather(stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p);
+        ncoarse += p_kids_[ls].size();
+    }
+    timer_.end(stream_, KC_TRANSFER, ncoarse);
     for (int l = base_level_; l < depth_; ++l)
     {
         const double dx = dx_base_ / std::pow(2.0, l - base_level_);
         run_pass(pass(l, 1, false), f_src_tmp_, f_tgt_tmp_, dx * dx, true, 0.0);
+        timer_.begin(stream_);
         launch::transfer_scatter(stream_, n_, tt, tt, p_ccint_[l].p, p_ccint_[l].size() / 3, m_cc_interp_.p, false);
+        timer_.end(stream_, KC_TRANSFER, p_ccint_[l].size() / 3);
         run_pass(pass(l, 0, false), f_src_tmp_, f_tgt_tmp_, dx * dx, true, 0.0);
+        timer_.begin(stream_);
         launch::copy_blocks(stream_, n_, F(dst), tt, p_leaf_level_[l].p, p_leaf_level_[l].size(), false);
-        if (l == depth_ - 1) continue;
-        launch::buffer_exchange(stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p);
-        launch::transfer_scatter(stream_, n_, st, ct, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false);
-        const double dxc = dx / 2.0;
-        launch::correction_stencil(stream_, n_, tt, ct, p_kids_[l].p, p_kids_[l].size(), 1.0 / (dxc * dxc), 0.0);
-        launch::axpy_blocks(stream_, n_, st, ct, p_level_slots_[l + 1].p, p_level_slots_[l + 1].size(), 1.0);
+        if (l < depth_ - 1)
+        {
+            launch::buffer_exchange(stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p);
+            launch::transfer_scatter(
+                stream_, n_, st, ct, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false);
+            const double dxc = dx / 2.0;
+            launch::correction_stencil(stream_, n_, tt, ct, p_kids_[l].p, p_kids_[l].size(), 1.0 / (dxc * dxc), 0.0);
+            launch::axpy_blocks(stream_, n_, st, ct, p_level_slots_[l + 1].p, p_level_slots_[l + 1].size(), 1.0);
+        }
+        timer_.end(stream_, KC_OTHER, 0);
     }
     IBLGF_HIP(hipGetLastError());
 }

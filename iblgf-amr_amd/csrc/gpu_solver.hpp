@@ -67,11 +67,52 @@ struct PassPlan
     std::vector<MacLevelPlan>        mac;
     long                             npairs = 0;
     long                             ntransfer = 0;  // parent-child transfers (anterp + interp)
+    long                             ntransfer_antrp = 0;
 };
 
 struct SolveCounts
 {
     long passes = 0, n_src = 0, n_tgt = 0, n_pair = 0, n_transfer = 0, n_leaf = 0;
+};
+
+// Kernel classes timed with HIP events on the solver's stream.
+enum KernelClass
+{
+    KC_FORWARD = 0,   // forward block DFT
+    KC_MAC = 1,       // spectral multiply-accumulate
+    KC_INVERSE = 2,   // inverse block DFT
+    KC_TRANSFER = 3,  // anterpolation / interpolation / coarsify
+    KC_OTHER = 4,     // copies, halo, correction stencil
+    KC_COUNT = 5
+};
+
+// Event pairs around launches of each kernel class; read after a sync.
+class KernelTimer
+{
+  public:
+    ~KernelTimer();
+    void enable(bool on);
+    bool enabled() const { return on_; }
+    void begin(hipStream_t s);
+    void end(hipStream_t s, int kind, long work);
+    void collect();  // call after the stream is synchronised
+    void reset();
+    double total_ms[KC_COUNT] = {0, 0, 0, 0, 0};
+    long   launches[KC_COUNT] = {0, 0, 0, 0, 0};
+    long   work[KC_COUNT] = {0, 0, 0, 0, 0};
+
+  private:
+    hipEvent_t next_event();
+    bool                    on_ = false;
+    std::vector<hipEvent_t> pool_;
+    size_t                  used_ = 0;
+    struct Rec
+    {
+        int    kind;
+        size_t ev;
+        long   work;
+    };
+    std::vector<Rec> recs_;
 };
 
 class GpuSolver
@@ -104,6 +145,7 @@ class GpuSolver
     int         block_n() const { return n_; }
     SolveCounts lgf_counts() const;
     int         num_leaves() const { return n_leaf_; }
+    KernelTimer& timer() { return timer_; }
 
   private:
     void      build_pass(PassPlan& p, int L, int nl, bool neighbor_only);
@@ -117,6 +159,7 @@ class GpuSolver
     int         device_ = 0;
     hipStream_t stream_ = nullptr;
     bool        uploaded_ = false;
+    KernelTimer timer_;
 
     // tree mirror (host copies)
     Octree                  tree_;

@@ -72,7 +72,12 @@ SYMBOLS = {
     "iblgf_gpu_sync": (_I, [_P]),
     "iblgf_gpu_stream": (_I, [_P, ctypes.POINTER(_P)]),
     "iblgf_gpu_counts": (_I, [_P, ctypes.POINTER(ctypes.c_long)]),
+    "iblgf_gpu_timing_enable": (_I, [_P, _I]),
+    "iblgf_gpu_timing_read": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
+    "iblgf_gpu_timing_reset": (_I, [_P]),
 }
+
+KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
 
 
 class IblgfError(RuntimeError):
@@ -253,6 +258,20 @@ class PoissonSolver:
         c = (ctypes.c_long * 6)()
         _check(lib().iblgf_gpu_counts(self._h, c), self._h)
         return dict(passes=c[0], n_src=c[1], n_tgt=c[2], n_pair=c[3], n_transfer=c[4], n_leaf=c[5])
+
+    def timing(self, enable):
+        _check(lib().iblgf_gpu_timing_enable(self._h, 1 if enable else 0), self._h)
+
+    def timing_reset(self):
+        _check(lib().iblgf_gpu_timing_reset(self._h), self._h)
+
+    def timing_read(self):
+        """{class: (total_ms, launches, work)} from the HIP events recorded so far."""
+        ms = (ctypes.c_double * 5)()
+        ln = (ctypes.c_long * 5)()
+        wk = (ctypes.c_long * 5)()
+        _check(lib().iblgf_gpu_timing_read(self._h, ms, ln, wk), self._h)
+        return {k: (ms[i], ln[i], wk[i]) for i, k in enumerate(KERNEL_CLASSES)}
 
 
 def lgf_get(x, y, z):

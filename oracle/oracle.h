@@ -75,10 +75,13 @@ int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const doub
 int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double alpha_base,
     int faithful, int intrp_order);
 
-/* Restricted single-target evaluation used for the CPU baseline: one FMM pass
- * Bx for the given target slots only (counts pairs).  Returns pairs done. */
+/* Bounded sample used for the CPU baseline: Fmm::fmm_Bx of one pass
+ * (fmm.hpp:907-1027) for the given target slots only, with the kernel
+ * spectra built beforehand (warm).  Returns the pairs done; *seconds gets the
+ * wall time of the Bx part. */
 long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const double* fmm_s,
-    int base_level_fmm, int nl, int ntargets, const int* targets, int faithful, double* fmm_t);
+    int base_level_fmm, int nl, int ntargets, const int* targets, int faithful, double* fmm_t,
+    double* seconds);
 
 #ifdef __cplusplus
 }

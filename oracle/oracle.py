@@ -49,7 +49,7 @@ def lib():
         L.oracle_block_conv.argtypes = [_P, _I, _I, _P, _P, _I, _D, _P]
         L.oracle_apply_lgf.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _P, _I, _I]
         L.oracle_apply_if.argtypes = [ctypes.POINTER(OracleTree), _P, _P, _D, _I, _I]
-        L.oracle_bx_sample.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _I, _I, _I, _P, _I, _P]
+        L.oracle_bx_sample.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _I, _I, _I, _P, _I, _P, _P]
         L.oracle_bx_sample.restype = ctypes.c_long
         _lib = L
     return _lib
@@ -142,6 +142,7 @@ class Tree:
         fs = np.ascontiguousarray(fmm_s, np.float64)
         tg = np.ascontiguousarray(targets, np.int32)
         out = np.zeros((len(tg),) + fs.shape[1:])
+        sec = np.zeros(1)
         pairs = lib().oracle_bx_sample(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(fs), int(L), int(nl),
-                                       len(tg), _p(tg), 1 if faithful else 0, _p(out))
-        return pairs, out
+                                       len(tg), _p(tg), 1 if faithful else 0, _p(out), _p(sec))
+        return pairs, float(sec[0]), out
