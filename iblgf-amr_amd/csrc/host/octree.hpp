@@ -74,9 +74,14 @@ struct Morton
     }
     static void coord(uint64_t k, int c[3]) noexcept
     {
+        // A level-0 key has no coordinate bits; the shift below would reach
+        // 64 bits there (undefined in C++), so level 0 is (0,0,0) by definition.
         const int L = level(k);
         for (int d = 0; d < 3; ++d)
-            c[d] = compact(k >> (d + shift_to_coord + (max_level - L) * 3));
+        {
+            const int sh = d + shift_to_coord + (max_level - L) * 3;
+            c[d] = sh >= 64 ? 0 : compact(k >> sh);
+        }
     }
     // Top 3*l bits set (coordinate bits of levels 1..l).
     static uint64_t coord_mask(int l) noexcept

@@ -1,0 +1,93 @@
+"""The CPU oracle pinned by the reference's known-answer tests and by brute
+force: block convolution (tests/conv_test/convSingleBlockTest.cpp:60-106,
+convFFTTest.cpp:122-225, restated at 1e-12), the single-level solve against
+the direct LGF sum, and the faithful (per-pair transform) and cached modes."""
+import numpy as np
+import pytest
+
+from iblgf_amd import FLAG_LEAF, setups
+from oracle import oracle as O
+
+
+def test_delta_source_reproduces_lgf_values():
+    # convSingleBlockTest: a unit source at the block origin convolved with the
+    # LGF gives lgf.get(offset) on the target block.
+    n = 10
+    src = np.zeros((n, n, n))
+    src[0, 0, 0] = 1.0
+    for shift in [(0, 0, 0), (8, 0, 0), (-16, 8, 24)]:
+        out = O.block_conv(src, shift)
+        for (k, j, i) in [(0, 0, 0), (3, 5, 7), (9, 9, 9), (0, 9, 4)]:
+            ref = O.lgf_get(shift[0] + i, shift[1] + j, shift[2] + k)
+            assert abs(out[k, j, i] - ref) < 1e-12 * max(abs(ref), 1e-3)
+
+
+@pytest.mark.parametrize("step", [1, 2, 8])
+def test_fft_convolution_equals_direct_sum(step):
+    # convFFTTest: the zero-padded FFT convolution equals the direct sum.
+    n = 8
+    rng = np.random.default_rng(3 + step)
+    src = rng.standard_normal((n, n, n))
+    shift = (16, -8, 0)
+    out = O.block_conv(src, shift, step=step, scale=0.25)
+    kk, jj, ii = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    sx = np.stack([ii, jj, kk], -1).reshape(-1, 3)
+    for t in [(0, 0, 0), (7, 7, 7), (3, 1, 6)]:
+        tx = np.array([[shift[0] + t[0], shift[1] + t[1], shift[2] + t[2]]])
+        # G sampled at step * (x_t - x_s)
+        d = (tx - sx) * step
+        ref = 0.25 * sum(O.lgf_get(*row) * q for row, q in zip(d, src.reshape(-1)))
+        assert abs(out[t[2], t[1], t[0]] - ref) < 1e-12 * np.abs(out).max()
+
+
+def test_single_level_solve_equals_direct_sum():
+    g = setups.UniformGaussians(seed=1000)
+    src = g.source()
+    a, n = g.arrays, g.tree.n
+    ot = O.Tree(a, g.info, n)
+    phi = ot.apply_lgf(src)
+    leaves = np.nonzero(a["flags"] & FLAG_LEAF)[0]
+    sx, q = [], []
+    for s in leaves:
+        b = a["bbase"][s]
+        kk, jj, ii = np.meshgrid(np.arange(1, n - 1), np.arange(1, n - 1), np.arange(1, n - 1), indexing="ij")
+        sx.append(np.stack([b[0] + ii - 1, b[1] + jj - 1, b[2] + kk - 1], -1).reshape(-1, 3))
+        q.append(src[s, 1:-1, 1:-1, 1:-1].reshape(-1))
+    sx, q = np.concatenate(sx), np.concatenate(q)
+    for s in leaves[[0, 21, 63]]:
+        b = a["bbase"][s]
+        kk, jj, ii = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+        tx = np.stack([b[0] + ii - 1, b[1] + jj - 1, b[2] + kk - 1], -1).reshape(-1, 3)
+        direct = O.direct_sum(sx, q, tx, g.info["dx_base"] ** 2)
+        assert np.max(np.abs(phi[s].reshape(-1) - direct)) < 1e-12 * np.max(np.abs(direct))
+
+
+def test_faithful_and_cached_modes_agree():
+    v = setups.VortexRings(seed=7, block_extent=8, base_blocks=4, nref=2, nrings=3, centre_range=4.0,
+                           radius=(4.0, 8.0), core=(1.5, 2.5), refinement_factor=0.5)
+    src = v.source(1)
+    ot = O.Tree(v.arrays, v.info, v.tree.n)
+    a = ot.apply_lgf(src, faithful=False)
+    b = ot.apply_lgf(src, faithful=True)
+    assert np.array_equal(a, b)
+
+
+def test_multilevel_solve_reduces_to_leaf_laplacian():
+    """On the finest leaves, the discrete Laplacian of the solution returns the
+    source up to the interpolation error of the level coupling (the check of
+    PoissonSolver::apply_laplace, include/iblgf/solver/poisson/poisson.hpp:1290-1360)."""
+    v = setups.VortexRings(seed=7, block_extent=8, base_blocks=4, nref=2, nrings=3, centre_range=4.0,
+                           radius=(4.0, 8.0), core=(1.5, 2.5), refinement_factor=0.5)
+    src = v.source(2)
+    a, n, info = v.arrays, v.tree.n, v.info
+    phi = O.Tree(a, info, n).apply_lgf(src)
+    finest = info["depth"] - 1
+    dx = info["dx_base"] / 2 ** (finest - info["base_level"])
+    err, ref = 0.0, 0.0
+    for s in np.nonzero((a["flags"] & FLAG_LEAF) & (a["level"] == finest))[0]:
+        p = phi[s]
+        lap = (p[2:, 1:-1, 1:-1] + p[:-2, 1:-1, 1:-1] + p[1:-1, 2:, 1:-1] + p[1:-1, :-2, 1:-1]
+               + p[1:-1, 1:-1, 2:] + p[1:-1, 1:-1, :-2] - 6 * p[1:-1, 1:-1, 1:-1]) / dx ** 2
+        err = max(err, np.max(np.abs(lap - src[s, 1:-1, 1:-1, 1:-1])))
+        ref = max(ref, np.max(np.abs(src[s])))
+    assert err < 0.05 * ref
