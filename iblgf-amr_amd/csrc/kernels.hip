@@ -98,30 +98,74 @@ struct TransferLds
     double m[2][N * N];
 };
 
+// One leg of the separable transfer: every thread takes whole lines (N
+// points along the leg's axis) into registers and writes the N outputs of
+// out[o] = sum_a K[o][a] in[a]; K's row index is the same across the
+// workgroup at each step, so its LDS reads are broadcasts.
+template<int N>
+__device__ inline void leg_lines(const double* in, double* out, const double* K, int axis)
+{
+    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    {
+        int base, stride;
+        if (axis == 0)
+        {
+            base = l * N;
+            stride = 1;
+        }
+        else if (axis == 1)
+        {
+            base = (l / N) * N * N + (l % N);
+            stride = N;
+        }
+        else
+        {
+            base = l;
+            stride = N * N;
+        }
+        double v[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) v[a] = in[base + a * stride];
+#pragma unroll 2
+        for (int o = 0; o < N; ++o)
+        {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < N; ++a) s = fma(K[o * N + a], v[a], s);
+            out[base + o * stride] = s;
+        }
+    }
+}
+
 template<int N>
 __device__ inline void contract_xy(TransferLds<N>& L, int bx, int by)
 {
-    constexpr int N3 = Geo<N>::N3;
-    const double* Kx = L.m[bx];
-    const double* Ky = L.m[by];
-    for (int q = threadIdx.x; q < N3; q += kThreads)
-    {
-        const int i = q % N, cb = q / N;
-        double    s = 0.0;
-#pragma unroll
-        for (int a = 0; a < N; ++a) s += Kx[i * N + a] * L.in[cb * N + a];
-        L.t1[q] = s;
-    }
+    leg_lines<N>(L.in, L.t1, L.m[bx], 0);
     __syncthreads();
-    for (int q = threadIdx.x; q < N3; q += kThreads)
-    {
-        const int i = q % N, j = (q / N) % N, c = q / (N * N);
-        double    s = 0.0;
-#pragma unroll
-        for (int b = 0; b < N; ++b) s += Ky[j * N + b] * L.t1[(c * N + b) * N + i];
-        L.in[q] = s;  // t2 reuses the input buffer
-    }
+    leg_lines<N>(L.t1, L.in, L.m[by], 1);  // t2 reuses the input buffer
     __syncthreads();
+}
+
+// z leg straight to global memory: out[k][j][i] (+)= sum_c Kz[k][c] t2[c][j][i]
+template<int N>
+__device__ inline void leg_z_store(const double* t2, double* out, const double* Kz, bool accumulate)
+{
+    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    {
+        double v[N];
+#pragma unroll
+        for (int c = 0; c < N; ++c) v[c] = t2[c * N * N + l];
+#pragma unroll 2
+        for (int k = 0; k < N; ++k)
+        {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < N; ++c) s = fma(Kz[k * N + c], v[c], s);
+            if (accumulate) out[k * N * N + l] += s;
+            else
+                out[k * N * N + l] = s;
+        }
+    }
 }
 
 template<int N>
@@ -147,15 +191,7 @@ __global__ __launch_bounds__(kThreads) void k_transfer_gather(
         for (int q = threadIdx.x; q < N3; q += kThreads) L.in[q] = in[q];
         __syncthreads();
         contract_xy<N>(L, c & 1, (c >> 1) & 1);
-        const double* Kz = L.m[(c >> 2) & 1];
-        for (int q = threadIdx.x; q < N3; q += kThreads)
-        {
-            const int ji = q % (N * N), k = q / (N * N);
-            double    s = 0.0;
-#pragma unroll
-            for (int cc = 0; cc < N; ++cc) s += Kz[k * N + cc] * L.in[cc * N * N + ji];
-            out[q] += s;
-        }
+        leg_z_store<N>(L.in, out, L.m[(c >> 2) & 1], true);
     }
 }
 
@@ -174,18 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_transfer_scatter(
     for (int q = threadIdx.x; q < N3; q += kThreads) L.in[q] = in[q];
     __syncthreads();
     contract_xy<N>(L, c & 1, (c >> 1) & 1);
-    const double* Kz = L.m[(c >> 2) & 1];
-    double*       out = dst + (size_t)child * N3;
-    for (int q = threadIdx.x; q < N3; q += kThreads)
-    {
-        const int ji = q % (N * N), k = q / (N * N);
-        double    s = 0.0;
-#pragma unroll
-        for (int cc = 0; cc < N; ++cc) s += Kz[k * N + cc] * L.in[cc * N * N + ji];
-        if (accumulate) out[q] += s;
-        else
-            out[q] = s;
-    }
+    leg_z_store<N>(L.in, dst + (size_t)child * N3, L.m[(c >> 2) & 1], accumulate != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -407,6 +432,26 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
 // for its frequencies sit in LDS.  Each thread keeps the 8 target
 // accumulators of one group at one frequency.
 // ---------------------------------------------------------------------------
+template<int KF>
+__device__ inline void mac_entry(const double2* Gs, int f, int meta, double2 S, double2 (&acc)[8])
+{
+    const int r = meta & 255, tm = meta >> 8;
+    const int rx = r % 6, ry = (r / 6) % 6, rz = r / 36;
+    const int ob = (rx + 1) + 7 * (ry + 1) + 49 * (rz + 1);
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+    {
+        if (tm & (1 << t))
+        {
+            const int     oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
+            const double2 gv = Gs[oi * KF + f];
+            acc[t].x = fma(gv.x, S.x, fma(-gv.y, S.y, acc[t].x));
+            acc[t].y = fma(gv.x, S.y, fma(gv.y, S.x, acc[t].y));
+        }
+    }
+}
+
+// grid: x = frequency chunks of KF, y = chunks of kThreads/KF groups.
 template<int N, int KF>
 __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
     const MacSrc* __restrict__ srcs, const double2* __restrict__ ghat, const double2* __restrict__ sspec,
@@ -422,40 +467,41 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
         Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    const int  f = threadIdx.x % KF, gl = threadIdx.x / KF;
-    const int  fg = f0 + f;
-    const bool fok = fg < NF;
-    for (int g = gl; g < ngroups; g += GPW)
+    const int f = threadIdx.x % KF;
+    const int g = blockIdx.y * GPW + threadIdx.x / KF;
+    if (g >= ngroups) return;  // no barrier follows
+    const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
+    const MacGroup grp = groups[g];
+    double2        acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
+    const double2* sp = sspec + fg;
+    int            e = grp.begin;
+    const int      end = grp.begin + grp.count;
+    // four source loads in flight before their use
+    for (; e + 4 <= end; e += 4)
     {
-        const MacGroup grp = groups[g];
-        double2        acc[8];
+        const MacSrc  s0 = srcs[e], s1 = srcs[e + 1], s2 = srcs[e + 2], s3 = srcs[e + 3];
+        const double2 S0 = sp[(size_t)s0.slot * NF];
+        const double2 S1 = sp[(size_t)s1.slot * NF];
+        const double2 S2 = sp[(size_t)s2.slot * NF];
+        const double2 S3 = sp[(size_t)s3.slot * NF];
+        mac_entry<KF>(Gs, f, s0.meta, S0, acc);
+        mac_entry<KF>(Gs, f, s1.meta, S1, acc);
+        mac_entry<KF>(Gs, f, s2.meta, S2, acc);
+        mac_entry<KF>(Gs, f, s3.meta, S3, acc);
+    }
+    for (; e < end; ++e)
+    {
+        const MacSrc  s0 = srcs[e];
+        const double2 S0 = sp[(size_t)s0.slot * NF];
+        mac_entry<KF>(Gs, f, s0.meta, S0, acc);
+    }
+    if (f0 + f < NF)
+    {
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
-        for (int e = grp.begin; e < grp.begin + grp.count; ++e)
-        {
-            const MacSrc  sr = srcs[e];
-            const double2 S = fok ? sspec[(size_t)sr.slot * NF + fg] : make_double2(0.0, 0.0);
-            const int     r = sr.meta & 255, tm = sr.meta >> 8;
-            const int     rx = r % 6, ry = (r / 6) % 6, rz = r / 36;
-            const int     ob = (rx + 1) + 7 * (ry + 1) + 49 * (rz + 1);
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-            {
-                if (tm & (1 << t))
-                {
-                    const int     oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
-                    const double2 gv = Gs[oi * KF + f];
-                    acc[t].x += gv.x * S.x - gv.y * S.y;
-                    acc[t].y += gv.x * S.y + gv.y * S.x;
-                }
-            }
-        }
-        if (fok)
-        {
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
-        }
+        for (int t = 0; t < 8; ++t)
+            if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
     }
 }
 
@@ -680,9 +726,11 @@ template<int N> struct RunMac
     static void run(hipStream_t s, const MacGroup* g, int ng, const MacSrc* src, const double2* ghat,
         const double2* ss, double2* ts)
     {
-        constexpr int KF = 16;
+        constexpr int  KF = 8;
+        constexpr int  GPW = kThreads / KF;
         const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
-        hipLaunchKernelGGL((k_spectral_mac<N, KF>), dim3(nblk), dim3(kThreads), 0, s, g, ng, src, ghat, ss, ts);
+        const unsigned ngy = (ng + GPW - 1) / GPW;
+        hipLaunchKernelGGL((k_spectral_mac<N, KF>), dim3(nblk, ngy), dim3(kThreads), 0, s, g, ng, src, ghat, ss, ts);
     }
 };
 template<int N> struct RunLgf
