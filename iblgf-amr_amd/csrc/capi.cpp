@@ -108,6 +108,37 @@ int iblgf_tree_create(const int bd_base[3], const int bd_extent[3], int block_ex
     });
 }
 
+int iblgf_tree_import(const int base_shift[3], int block_extent, int base_level, double dx_base, int nkeys,
+    const uint64_t* keys, const uint32_t* flags, iblgf_tree** out)
+{
+    if (!base_shift || block_extent <= 0 || base_level < 0 || base_level > Morton::max_level || nkeys <= 0 ||
+        !keys || !flags || !out)
+        return fail(IBLGF_ERR_ARG, "iblgf_tree_import: invalid argument");
+    return guarded(nullptr, [&] {
+        auto   t = std::make_unique<iblgf_tree>();
+        Octree& tr = t->tree;
+        tr.block_extent = block_extent;
+        tr.dx_base = dx_base;
+        tr.base_level = base_level;
+        tr.depth = base_level + 1;
+        for (int d = 0; d < 3; ++d) tr.base_shift[d] = base_shift[d];
+        tr.insert(Morton::make(0, 0, 0, 0));
+        for (int i = 0; i < nkeys; ++i)
+        {
+            if ((keys[i] & 1ULL) == 0 || Morton::level(keys[i]) > Morton::max_level)
+                throw std::invalid_argument("iblgf_tree_import: malformed key");
+            const int o = tr.insert(keys[i]);
+            if (tr.oct[o].has_data) throw std::invalid_argument("iblgf_tree_import: duplicate key");
+            tr.oct[o].has_data = true;
+            tr.oct[o].leaf = (flags[i] & 1u) != 0;
+            tr.oct[o].correction = (flags[i] & 2u) != 0;
+        }
+        tr.construct_level_maps();
+        tr.construct_lists();
+        *out = t.release();
+    });
+}
+
 int iblgf_tree_init_refine(iblgf_tree* t, int nref, int level_up_max, iblgf_refine_fn fn, void* user)
 {
     if (!t || nref < 0 || level_up_max < 0 || (nref > 0 && !fn)) return fail(IBLGF_ERR_ARG, "iblgf_tree_init_refine: invalid argument");

@@ -265,17 +265,49 @@ __global__ __launch_bounds__(kThreads) void k_correction_stencil(
 }
 
 // ---------------------------------------------------------------------------
-// Forward r2c of an n^3 block zero-padded to P^3 (pruned legs), one block per
-// workgroup: x leg into LDS, then per kx plane the y and z legs.
+// Block DFT legs.  Every leg is a small dense matrix applied to lines of a
+// block: a thread holds one input line in registers and produces its outputs;
+// the twiddle index depends only on loop counters that are uniform across the
+// workgroup, so twiddle reads from LDS are broadcasts.  The kx axis is the
+// halved one; legs after the first run on GS kx-planes at a time so the
+// intermediate fits in LDS.
 // ---------------------------------------------------------------------------
+template<int N>
+struct DftPlan
+{
+    static constexpr int P = 2 * N - 1;
+    static constexpr int H = P / 2 + 1;
+    static constexpr int kLds = 163840 - 4096;
+    static constexpr int kFwdRaw = (kLds - N * N * H * 16 - P * 16) / (N * P * 16);
+    static constexpr int FWD_GS = kFwdRaw < H ? kFwdRaw : H;
+    static constexpr int kInvRaw = (kLds - H * N * N * 16 - P * 16) / (P * N * 16);
+    static constexpr int INV_GS = kInvRaw < H ? kInvRaw : H;
+};
+
+// a * (c - i s)
+__device__ inline void cmac_conj(double2 a, double2 w, double& sr, double& si)
+{
+    sr = fma(a.x, w.x, fma(a.y, w.y, sr));
+    si = fma(a.y, w.x, fma(-a.x, w.y, si));
+}
+// a * (c + i s)
+__device__ inline void cmac(double2 a, double2 w, double& sr, double& si)
+{
+    sr = fma(a.x, w.x, fma(-a.y, w.y, sr));
+    si = fma(a.x, w.y, fma(a.y, w.x, si));
+}
+
+// Forward r2c of an n^3 block zero-padded to P^3 (pruned legs): x leg into LDS,
+// then y and z legs per group of kx planes; output spec[kx][ky][kz].
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_forward_dft(
     const double* __restrict__ field, const int* __restrict__ slots, int count, double2* __restrict__ spec)
 {
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H;
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::FWD_GS;
+    static_assert(GS >= 1, "LDS too small for one plane");
     __shared__ double2 X1[N * N * H];  // [z][y][kx]
-    __shared__ double2 T[N * P];       // [z][ky]
+    __shared__ double2 T[GS * N * P];  // [g][z][ky]
     __shared__ double2 W[P];
     const int b = blockIdx.x;
     if (b >= count) return;
@@ -283,79 +315,92 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
     const double* in = field + (size_t)slot * G::N3;
     twiddles(W, P);
     __syncthreads();
-    for (int q = threadIdx.x; q < N * N * H; q += kThreads)
+    for (int l = threadIdx.x; l < N * N; l += kThreads)
     {
-        const int     zy = q / H, kx = q - zy * H;
-        const double* line = in + zy * N;
-        double        sr = 0.0, si = 0.0;
-        int           m = 0;
+        double v[N];
 #pragma unroll
-        for (int x = 0; x < N; ++x)
+        for (int x = 0; x < N; ++x) v[x] = in[l * N + x];
+        for (int kx = 0; kx < H; ++kx)
         {
-            const double  v = line[x];
-            const double2 w = W[m];
-            sr += v * w.x;
-            si -= v * w.y;
-            m += kx;
-            if (m >= P) m -= P;
+            double sr = 0.0, si = 0.0;
+            int    m = 0;
+#pragma unroll
+            for (int x = 0; x < N; ++x)
+            {
+                const double2 w = W[m];
+                sr = fma(v[x], w.x, sr);
+                si = fma(-v[x], w.y, si);
+                m += kx;
+                if (m >= P) m -= P;
+            }
+            X1[l * H + kx] = make_double2(sr, si);
         }
-        X1[q] = make_double2(sr, si);
     }
     __syncthreads();
     double2* out = spec + (size_t)slot * G::NF;
-    for (int kx = 0; kx < H; ++kx)
+    constexpr int K0 = (P + 1) / 2;  // y-leg outputs split over two threads per line
+    for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
-        for (int q = threadIdx.x; q < N * P; q += kThreads)
+        const int gs = min(GS, H - kx0);
+        for (int l = threadIdx.x; l < 2 * gs * N; l += kThreads)
         {
-            const int z = q / P, ky = q - z * P;
-            double    sr = 0.0, si = 0.0;
-            int       m = 0;
+            const int half = l / (gs * N), line = l - half * gs * N;
+            const int g = line / N, z = line - g * N, kx = kx0 + g;
+            double2   a[N];
 #pragma unroll
-            for (int y = 0; y < N; ++y)
+            for (int y = 0; y < N; ++y) a[y] = X1[(z * N + y) * H + kx];
+            const int k_lo = half ? K0 : 0, k_hi = half ? P : K0;
+            for (int ky = k_lo; ky < k_hi; ++ky)
             {
-                const double2 a = X1[(z * N + y) * H + kx];
-                const double2 w = W[m];
-                sr += a.x * w.x + a.y * w.y;
-                si += a.y * w.x - a.x * w.y;
-                m += ky;
-                if (m >= P) m -= P;
+                double sr = 0.0, si = 0.0;
+                int    m = 0;
+#pragma unroll
+                for (int y = 0; y < N; ++y)
+                {
+                    cmac_conj(a[y], W[m], sr, si);
+                    m += ky;
+                    if (m >= P) m -= P;
+                }
+                T[(g * N + z) * P + ky] = make_double2(sr, si);
             }
-            T[q] = make_double2(sr, si);
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < P * P; q += kThreads)
+        for (int l = threadIdx.x; l < gs * P; l += kThreads)
         {
-            const int ky = q / P, kz = q - ky * P;
-            double    sr = 0.0, si = 0.0;
-            int       m = 0;
+            const int g = l / P, ky = l - g * P, kx = kx0 + g;
+            double2   a[N];
 #pragma unroll
-            for (int z = 0; z < N; ++z)
+            for (int z = 0; z < N; ++z) a[z] = T[(g * N + z) * P + ky];
+            double2* o = out + (size_t)(kx * P + ky) * P;
+            for (int kz = 0; kz < P; ++kz)
             {
-                const double2 a = T[z * P + ky];
-                const double2 w = W[m];
-                sr += a.x * w.x + a.y * w.y;
-                si += a.y * w.x - a.x * w.y;
-                m += kz;
-                if (m >= P) m -= P;
+                double sr = 0.0, si = 0.0;
+                int    m = 0;
+#pragma unroll
+                for (int z = 0; z < N; ++z)
+                {
+                    cmac_conj(a[z], W[m], sr, si);
+                    m += kz;
+                    if (m >= P) m -= P;
+                }
+                o[kz] = make_double2(sr, si);
             }
-            out[(size_t)(kx * P + ky) * P + kz] = make_double2(sr, si);
         }
         __syncthreads();
     }
 }
 
-// ---------------------------------------------------------------------------
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
 // added into the target block.
-// ---------------------------------------------------------------------------
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restrict__ spec,
     const int* __restrict__ slots, int count, double* __restrict__ field, double scale)
 {
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H;
-    __shared__ double2 V[H * N * N];  // [kx][z][y]
-    __shared__ double2 U[P * N];      // [ky][z]
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::INV_GS;
+    static_assert(GS >= 1, "LDS too small for one plane");
+    __shared__ double2 V[H * N * N];   // [kx][z][y]
+    __shared__ double2 U[GS * P * N];  // [g][ky][z]
     __shared__ double2 W[P];
     const int b = blockIdx.x;
     if (b >= count) return;
@@ -363,66 +408,80 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
     const double2* in = spec + (size_t)slot * G::NF;
     twiddles(W, P);
     __syncthreads();
-    for (int kx = 0; kx < H; ++kx)
+    for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
-        for (int q = threadIdx.x; q < P * N; q += kThreads)
+        const int gs = min(GS, H - kx0);
+        // z leg: (g, ky) rows of P frequencies -> N window points
+        for (int l = threadIdx.x; l < gs * P; l += kThreads)
         {
-            const int      ky = q / N, z = q - ky * N;
-            const int      step = (N - 1 + z) % P;
+            const int      g = l / P, ky = l - g * P, kx = kx0 + g;
             const double2* row = in + (size_t)(kx * P + ky) * P;
-            double         sr = 0.0, si = 0.0;
-            int            m = 0;
-#pragma unroll 5
-            for (int kz = 0; kz < P; ++kz)
+            double2        a[P];
+#pragma unroll
+            for (int kz = 0; kz < P; ++kz) a[kz] = row[kz];
+            for (int z = 0; z < N; ++z)
             {
-                const double2 a = row[kz];
-                const double2 w = W[m];
-                sr += a.x * w.x - a.y * w.y;
-                si += a.x * w.y + a.y * w.x;
-                m += step;
-                if (m >= P) m -= P;
+                const int step = (N - 1 + z) % P;
+                double    sr = 0.0, si = 0.0;
+                int       m = 0;
+#pragma unroll
+                for (int kz = 0; kz < P; ++kz)
+                {
+                    cmac(a[kz], W[m], sr, si);
+                    m += step;
+                    if (m >= P) m -= P;
+                }
+                U[(g * P + ky) * N + z] = make_double2(sr, si);
             }
-            U[q] = make_double2(sr, si);
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < N * N; q += kThreads)
+        // y leg: (g, z) lines of P frequencies -> N window points
+        for (int l = threadIdx.x; l < gs * N; l += kThreads)
         {
-            const int z = q / N, y = q - z * N;
-            const int step = (N - 1 + y) % P;
-            double    sr = 0.0, si = 0.0;
-            int       m = 0;
-#pragma unroll 5
-            for (int ky = 0; ky < P; ++ky)
+            const int g = l / N, z = l - g * N, kx = kx0 + g;
+            double2   a[P];
+#pragma unroll
+            for (int ky = 0; ky < P; ++ky) a[ky] = U[(g * P + ky) * N + z];
+            for (int y = 0; y < N; ++y)
             {
-                const double2 a = U[ky * N + z];
-                const double2 w = W[m];
-                sr += a.x * w.x - a.y * w.y;
-                si += a.x * w.y + a.y * w.x;
-                m += step;
-                if (m >= P) m -= P;
+                const int step = (N - 1 + y) % P;
+                double    sr = 0.0, si = 0.0;
+                int       m = 0;
+#pragma unroll
+                for (int ky = 0; ky < P; ++ky)
+                {
+                    cmac(a[ky], W[m], sr, si);
+                    m += step;
+                    if (m >= P) m -= P;
+                }
+                V[(kx * N + z) * N + y] = make_double2(sr, si);
             }
-            V[(kx * N + z) * N + y] = make_double2(sr, si);
         }
         __syncthreads();
     }
+    // x leg, c2r: (z, y) lines of H half-spectrum values -> N window points
     double* out = field + (size_t)slot * G::N3;
-    for (int q = threadIdx.x; q < G::N3; q += kThreads)
+    for (int l = threadIdx.x; l < N * N; l += kThreads)
     {
-        const int x = q % N, y = (q / N) % N, z = q / (N * N);
-        const int step = (N - 1 + x) % P;
-        double    s = 0.0;
-        int       m = 0;
+        double2 a[H];
 #pragma unroll
-        for (int kx = 0; kx < H; ++kx)
+        for (int kx = 0; kx < H; ++kx) a[kx] = V[kx * N * N + l];
+        for (int x = 0; x < N; ++x)
         {
-            const double2 a = V[(kx * N + z) * N + y];
-            const double2 w = W[m];
-            const double  c = (kx == 0) ? 1.0 : 2.0;  // P odd: no Nyquist plane
-            s += c * (a.x * w.x - a.y * w.y);
-            m += step;
-            if (m >= P) m -= P;
+            const int step = (N - 1 + x) % P;
+            double    s = 0.0;
+            int       m = 0;
+#pragma unroll
+            for (int kx = 0; kx < H; ++kx)
+            {
+                const double2 w = W[m];
+                const double  c = (kx == 0) ? 1.0 : 2.0;  // P odd: no Nyquist plane
+                s = fma(c, fma(a[kx].x, w.x, -a[kx].y * w.y), s);
+                m += step;
+                if (m >= P) m -= P;
+            }
+            out[l * N + x] += scale * s;
         }
-        out[q] += scale * s;
     }
 }
 

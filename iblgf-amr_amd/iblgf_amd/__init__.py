@@ -44,6 +44,8 @@ _IP = ctypes.POINTER(ctypes.c_int)
 _DP = ctypes.POINTER(ctypes.c_double)
 SYMBOLS = {
     "iblgf_tree_create": (_I, [_IP, _IP, _I, _D, _I, _IP, ctypes.POINTER(_P)]),
+    "iblgf_tree_import": (_I, [_IP, _I, _I, _D, _I, ctypes.POINTER(_U64), ctypes.POINTER(ctypes.c_uint32),
+                               ctypes.POINTER(_P)]),
     "iblgf_tree_init_refine": (_I, [_P, _I, _I, REFINE_FN, _P]),
     "iblgf_tree_build_masks": (_I, [_P, _I]),
     "iblgf_tree_info": (_I, [_P, _IP, _IP, _IP, _IP, _DP, _IP]),
@@ -137,6 +139,22 @@ class Tree:
         self.block_extent = int(block_extent)
         self.n = self.block_extent + 2
         self.dx_base = float(dx_base)
+
+    @classmethod
+    def from_keys(cls, base_shift, block_extent, base_level, dx_base, keys, flags):
+        """A tree handed over by its keys and leaf/correction flags (the reference's own tree)."""
+        obj = cls.__new__(cls)
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        f = np.ascontiguousarray(flags, dtype=np.uint32)
+        h = ctypes.c_void_p()
+        _check(lib().iblgf_tree_import(_ip(base_shift), int(block_extent), int(base_level), float(dx_base), len(k),
+                                       k.ctypes.data_as(ctypes.POINTER(_U64)),
+                                       f.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(h)))
+        obj._h = h
+        obj.block_extent = int(block_extent)
+        obj.n = obj.block_extent + 2
+        obj.dx_base = float(dx_base)
+        return obj
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -150,3 +150,22 @@ def test_adaptive_vortex_ring_tree_size():
     v = setups.VortexRings()
     assert v.info["nblocks"] == 4290 and v.info["nleaves"] == 1933
     assert v.info["base_level"] == 5 and v.info["depth"] == 9
+
+
+@pytest.mark.parametrize("name", ["uniform", "rings", "rings_n8"])
+def test_tree_import_round_trip(trees, name):
+    """A tree handed over by keys and flags rebuilds the same lists and masks."""
+    t = trees[name]
+    a = t.arrays
+    # base shift = bounding-box base / block extent = root block base / e (root coord is 0)
+    assert int(a["level"][0]) == 0
+    shift = [int(a["bbase"][0][d] // t.block_extent) for d in range(3)]
+    perm = np.random.default_rng(5).permutation(len(a["key"]))
+    u = Tree.from_keys(shift, t.block_extent, t.info["base_level"], t.info["dx_base"], a["key"][perm],
+                       a["flags"][perm] & 3)
+    u.build_masks()
+    b = u.export()
+    for k in ("key", "level", "coord", "bbase", "parent", "children", "neighbor", "influence", "influence_num",
+              "mask_src", "mask_tgt"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["flags"] & 3, b["flags"] & 3)
