@@ -88,61 +88,52 @@ __global__ __launch_bounds__(kThreads) void k_axpy(
 
 // ---------------------------------------------------------------------------
 // Separable 3-leg transfer: out[k][j][i] = sum Kz[k][c] Ky[j][b] Kx[i][a] in[c][b][a]
-// Stages x, y in LDS; the z stage lands in per-thread registers.
+// Every leg is line-per-thread: a thread takes a whole line (N points along
+// the leg's axis) into registers and writes its N outputs; the matrix row
+// index is uniform across the workgroup, so matrix reads from LDS are
+// broadcasts.  The x leg reads the input block from global memory, the y leg
+// runs in place in the one LDS buffer, the z leg writes global memory.
 // ---------------------------------------------------------------------------
 template<int N>
 struct TransferLds
 {
-    double in[Geo<N>::N3];
-    double t1[Geo<N>::N3];
+    double buf[Geo<N>::N3];
     double m[2][N * N];
 };
 
-// One leg of the separable transfer: every thread takes whole lines (N
-// points along the leg's axis) into registers and writes the N outputs of
-// out[o] = sum_a K[o][a] in[a]; K's row index is the same across the
-// workgroup at each step, so its LDS reads are broadcasts.
 template<int N>
-__device__ inline void leg_lines(const double* in, double* out, const double* K, int axis)
+__device__ inline void line_apply(const double (&v)[N], const double* K, double* out, int base, int stride)
 {
-    for (int l = threadIdx.x; l < N * N; l += kThreads)
-    {
-        int base, stride;
-        if (axis == 0)
-        {
-            base = l * N;
-            stride = 1;
-        }
-        else if (axis == 1)
-        {
-            base = (l / N) * N * N + (l % N);
-            stride = N;
-        }
-        else
-        {
-            base = l;
-            stride = N * N;
-        }
-        double v[N];
-#pragma unroll
-        for (int a = 0; a < N; ++a) v[a] = in[base + a * stride];
 #pragma unroll 2
-        for (int o = 0; o < N; ++o)
-        {
-            double s = 0.0;
+    for (int o = 0; o < N; ++o)
+    {
+        double s = 0.0;
 #pragma unroll
-            for (int a = 0; a < N; ++a) s = fma(K[o * N + a], v[a], s);
-            out[base + o * stride] = s;
-        }
+        for (int a = 0; a < N; ++a) s = fma(K[o * N + a], v[a], s);
+        out[base + o * stride] = s;
     }
 }
 
+// x and y legs: global block -> LDS buffer (x), then y in place.
 template<int N>
-__device__ inline void contract_xy(TransferLds<N>& L, int bx, int by)
+__device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>& L, int bx, int by)
 {
-    leg_lines<N>(L.in, L.t1, L.m[bx], 0);
+    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    {
+        double v[N];
+#pragma unroll
+        for (int a = 0; a < N; ++a) v[a] = in[l * N + a];
+        line_apply<N>(v, L.m[bx], L.buf, l * N, 1);
+    }
     __syncthreads();
-    leg_lines<N>(L.t1, L.in, L.m[by], 1);  // t2 reuses the input buffer
+    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    {
+        const int base = (l / N) * N * N + (l % N);
+        double    v[N];
+#pragma unroll
+        for (int b = 0; b < N; ++b) v[b] = L.buf[base + b * N];
+        line_apply<N>(v, L.m[by], L.buf, base, N);  // the thread owns this line
+    }
     __syncthreads();
 }
 
@@ -168,6 +159,8 @@ __device__ inline void leg_z_store(const double* t2, double* out, const double* 
     }
 }
 
+// dst[parent] += sum over listed children of K(child); one workgroup per parent,
+// so the children are added in a fixed order (deterministic).
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_transfer_gather(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
@@ -179,19 +172,14 @@ __global__ __launch_bounds__(kThreads) void k_transfer_gather(
     if (e >= count) return;
     const int* ent = entries + 9 * e;
     for (int q = threadIdx.x; q < 2 * N * N; q += kThreads) (&L.m[0][0])[q] = mats[q];
-    // One workgroup owns the parent block, so the per-child contributions are
-    // added to it in place; each thread touches the same points every child.
     double* out = dst + (size_t)ent[0] * N3;
     for (int c = 0; c < 8; ++c)
     {
         const int ch = ent[1 + c];
         if (ch < 0) continue;  // uniform across the workgroup
-        __syncthreads();
-        const double* in = src + (size_t)ch * N3;
-        for (int q = threadIdx.x; q < N3; q += kThreads) L.in[q] = in[q];
-        __syncthreads();
-        contract_xy<N>(L, c & 1, (c >> 1) & 1);
-        leg_z_store<N>(L.in, out, L.m[(c >> 2) & 1], true);
+        __syncthreads();       // matrices loaded / previous z leg done with the buffer
+        transfer_xy<N>(src + (size_t)ch * N3, L, c & 1, (c >> 1) & 1);
+        leg_z_store<N>(L.buf, out, L.m[(c >> 2) & 1], true);
     }
 }
 
@@ -206,11 +194,9 @@ __global__ __launch_bounds__(kThreads) void k_transfer_scatter(
     if (e >= count) return;
     const int child = entries[3 * e], parent = entries[3 * e + 1], c = entries[3 * e + 2];
     for (int q = threadIdx.x; q < 2 * N * N; q += kThreads) (&L.m[0][0])[q] = mats[q];
-    const double* in = src + (size_t)parent * N3;
-    for (int q = threadIdx.x; q < N3; q += kThreads) L.in[q] = in[q];
     __syncthreads();
-    contract_xy<N>(L, c & 1, (c >> 1) & 1);
-    leg_z_store<N>(L.in, dst + (size_t)child * N3, L.m[(c >> 2) & 1], accumulate != 0);
+    transfer_xy<N>(src + (size_t)parent * N3, L, c & 1, (c >> 1) & 1);
+    leg_z_store<N>(L.buf, dst + (size_t)child * N3, L.m[(c >> 2) & 1], accumulate != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -526,41 +512,43 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
         Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    const int f = threadIdx.x % KF;
-    const int g = blockIdx.y * GPW + threadIdx.x / KF;
-    if (g >= ngroups) return;  // no barrier follows
-    const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
-    const MacGroup grp = groups[g];
-    double2        acc[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
+    const int      f = threadIdx.x % KF;
+    const int      fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
     const double2* sp = sspec + fg;
-    int            e = grp.begin;
-    const int      end = grp.begin + grp.count;
-    // four source loads in flight before their use
-    for (; e + 4 <= end; e += 4)
+    // The workgroup walks the (Morton-ordered) groups in tiles, so a source
+    // shared by neighbouring groups is re-read while its line is still in L2.
+    for (int g = blockIdx.y * GPW + threadIdx.x / KF; g < ngroups; g += GPW * gridDim.y)
     {
-        const MacSrc  s0 = srcs[e], s1 = srcs[e + 1], s2 = srcs[e + 2], s3 = srcs[e + 3];
-        const double2 S0 = sp[(size_t)s0.slot * NF];
-        const double2 S1 = sp[(size_t)s1.slot * NF];
-        const double2 S2 = sp[(size_t)s2.slot * NF];
-        const double2 S3 = sp[(size_t)s3.slot * NF];
-        mac_entry<KF>(Gs, f, s0.meta, S0, acc);
-        mac_entry<KF>(Gs, f, s1.meta, S1, acc);
-        mac_entry<KF>(Gs, f, s2.meta, S2, acc);
-        mac_entry<KF>(Gs, f, s3.meta, S3, acc);
-    }
-    for (; e < end; ++e)
-    {
-        const MacSrc  s0 = srcs[e];
-        const double2 S0 = sp[(size_t)s0.slot * NF];
-        mac_entry<KF>(Gs, f, s0.meta, S0, acc);
-    }
-    if (f0 + f < NF)
-    {
+        const MacGroup grp = groups[g];
+        double2        acc[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
-            if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
+        for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
+        int       e = grp.begin;
+        const int end = grp.begin + grp.count;
+        // eight source loads in flight before their use
+        for (; e + 8 <= end; e += 8)
+        {
+            MacSrc  s[8];
+            double2 S[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) s[i] = srcs[e + i];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) S[i] = sp[(size_t)s[i].slot * NF];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) mac_entry<KF>(Gs, f, s[i].meta, S[i], acc);
+        }
+        for (; e < end; ++e)
+        {
+            const MacSrc  s0 = srcs[e];
+            const double2 S0 = sp[(size_t)s0.slot * NF];
+            mac_entry<KF>(Gs, f, s0.meta, S0, acc);
+        }
+        if (f0 + f < NF)
+        {
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
+        }
     }
 }
 
@@ -786,10 +774,8 @@ template<int N> struct RunMac
         const double2* ss, double2* ts)
     {
         constexpr int  KF = 8;
-        constexpr int  GPW = kThreads / KF;
         const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
-        const unsigned ngy = (ng + GPW - 1) / GPW;
-        hipLaunchKernelGGL((k_spectral_mac<N, KF>), dim3(nblk, ngy), dim3(kThreads), 0, s, g, ng, src, ghat, ss, ts);
+        hipLaunchKernelGGL((k_spectral_mac<N, KF>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src, ghat, ss, ts);
     }
 };
 template<int N> struct RunLgf
