@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LGF solves/s + HBM/MFMA roofline %, batched-mixed vs plain FP64 at relL2≤1e-6"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64, vector and matrix
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32, vector and matrix
 PEAK_HBM_GBS = 8000.0
 
 
@@ -84,7 +85,7 @@ def main():
 
     import numpy as np
 
-    from iblgf_amd import PoissonSolver, hip_events, setups
+    from iblgf_amd import FLAG_LEAF, PREC_FP64, PREC_MIXED, PoissonSolver, hip_events, setups
 
     wl = setups.VortexRings(seed=1001)
     srcs = [wl.source(c) for c in range(3)]
@@ -93,68 +94,81 @@ def main():
     fout = [solver.field() for _ in range(3)]
     for f, s in zip(fin, srcs):
         solver.upload(f, s)
-    solver.prepare()
+    n = wl.tree.n
+    uf = unit_flops(n)
+    stream = solver.stream()
 
     def step():
         for c in range(3):
             solver.apply_lgf(fin[c], fout[c])
 
-    for _ in range(args.warmup):
-        step()
-    solver.sync()
+    def timed(precision):
+        solver.set_precision(precision)
+        solver.prepare()
+        for _ in range(args.warmup):
+            step()
+        solver.sync()
+        ev0, ev1 = hip_events.Event(), hip_events.Event()
+        solver.timing(True)
+        solver.sync()
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step()
+        ev1.record(stream)
+        solver.sync()
+        t1 = time.perf_counter()
+        kt = solver.timing_read()
+        solver.timing(False)
+        outs = [solver.download(f) for f in fout]
+        return dict(wall=t1 - t0, gpu_ms=ev0.elapsed_ms(ev1), kt=kt, outs=outs)
 
-    stream = solver.stream()
-    ev0, ev1 = hip_events.Event(), hip_events.Event()
-    solver.timing(True)
-    solver.sync()
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        step()
-    ev1.record(stream)
-    solver.sync()
-    t1 = time.perf_counter()
-    gpu_ms = ev0.elapsed_ms(ev1)
-    kt = solver.timing_read()
-    solver.timing(False)
+    r64 = timed(PREC_FP64)
+    rmx = timed(PREC_MIXED)
     counts = solver.counts()
-
-    wall = t1 - t0
+    leaves = (wl.arrays["flags"] & FLAG_LEAF) != 0
+    errors = []
+    for a, b in zip(rmx["outs"], r64["outs"]):
+        d = a[leaves] - b[leaves]
+        errors.append({"relL2": float(np.linalg.norm(d) / np.linalg.norm(b[leaves])),
+                       "relmax": float(np.max(np.abs(d)) / np.max(np.abs(b[leaves])))})
+    mixed_ok = max(e["relL2"] for e in errors) <= 1e-6
     solves = 3 * args.steps
-    value = solves / wall
-    n = wl.tree.n
-    uf = unit_flops(n)
-    kernels = {}
-    for k, (ms, launches, work) in kt.items():
-        if k == "other":
-            kernels[k] = {"ms_per_solve": ms / solves, "launches_per_solve": launches / solves}
-            continue
-        fl = work * uf[k]
-        kernels[k] = {
-            "ms_per_solve": ms / solves,
-            "launches_per_solve": launches / solves,
-            "work_per_solve": work / solves,
-            "achieved_tflops": (fl / (ms * 1e-3) / 1e12) if ms > 0 else 0.0,
+
+    def kernel_table(kt, peak):
+        out = {}
+        for k, (ms, launches, work) in kt.items():
+            if k == "other":
+                out[k] = {"ms_per_solve": ms / solves, "launches_per_solve": launches / solves}
+                continue
+            fl = work * uf[k]
+            out[k] = {"ms_per_solve": ms / solves, "launches_per_solve": launches / solves,
+                      "work_per_solve": work / solves,
+                      "achieved_tflops": (fl / (ms * 1e-3) / 1e12) if ms > 0 else 0.0}
+        return out
+
+    def roofline_of(kt, peak, peak_name):
+        table = kernel_table(kt, peak)
+        dom = max((k for k in table if k != "other"), key=lambda k: table[k]["ms_per_solve"])
+        ms, launches, work = kt[dom]
+        achieved = work * uf[dom] / (ms * 1e-3) / 1e12
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get(f"{peak_name}:{dom}")
+        return table, {
+            "bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": traffic, "avg_launch_ms": ms / max(launches, 1),
+            "flops_per_launch": work * uf[dom] / max(launches, 1), "arithmetic": peak_name,
         }
-    dom = max((k for k in kernels if k != "other"), key=lambda k: kernels[k]["ms_per_solve"])
-    ms, launches, work = kt[dom]
-    achieved = work * uf[dom] / (ms * 1e-3) / 1e12
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(dom)
-    roofline = {
-        "bound": "mfma",
-        "kernel": dom,
-        "achieved": achieved,
-        "peak": PEAK_FP64_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": achieved / PEAK_FP64_TFLOPS,
-        "traffic": traffic,
-        "avg_launch_ms": ms / max(launches, 1),
-        "flops_per_launch": work * uf[dom] / max(launches, 1),
-    }
+
+    k64, roof64 = roofline_of(r64["kt"], PEAK_FP64_TFLOPS, "fp64")
+    kmx, roofmx = roofline_of(rmx["kt"], PEAK_FP32_TFLOPS, "fp32")
+    head = rmx if mixed_ok else r64
+    value = solves / head["wall"]
+    v64 = solves / r64["wall"]
+    vmx = solves / rmx["wall"]
     flops_per_solve = sum(counts[c] * uf[k] for c, k in (("n_src", "forward_dft"), ("n_tgt", "inverse_dft"),
                                                              ("n_pair", "spectral_mac"), ("n_transfer", "transfer")))
     leaf_cells = counts["n_leaf"] * wl.block_extent ** 3
@@ -165,16 +179,17 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3,
+        "ms_per_step": head["wall"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed_ok else "f64",
         "data": "synthetic",
         "config": {
             "workload": ("adaptive vortex-ring octree Poisson: 8^3 base blocks of 16^3 cells + 3 refinement levels, "
                          "16 seeded Gaussian-core rings (seed 1001); one step = 3 vorticity components"),
-            "precision": "plain FP64",
+            "precision": ("mixed: FP32 spectra, DFT legs and spectral MAC; FP64 fields, kernel-spectrum "
+                          "generation and cross-level sums" if mixed_ok else "plain FP64"),
             "blocks": wl.info["nblocks"],
             "leaves": wl.info["nleaves"],
             "leaf_cells": leaf_cells,
@@ -182,10 +197,16 @@ def main():
             "pairs_per_solve": counts["n_pair"],
             "flops_per_solve": flops_per_solve,
             "solve_tflops": flops_per_solve * value / 1e12,
-            "gpu_event_ms_per_step": gpu_ms / args.steps,
+            "gpu_event_ms_per_step": head["gpu_ms"] / args.steps,
+            "fp64_solves_per_s": v64,
+            "mixed_solves_per_s": vmx,
+            "mixed_vs_fp64_speedup": vmx / v64,
+            "mixed_error_vs_fp64_per_component": errors,
+            "fp64_roofline": roof64,
+            "fp64_kernels": k64,
         },
-        "roofline": roofline,
-        "kernels": kernels,
+        "roofline": roofmx if mixed_ok else roof64,
+        "kernels": kmx if mixed_ok else k64,
         "cpu_baseline": None,
     }
     if rank == 0 and not args.no_cpu_baseline:

@@ -4,8 +4,11 @@
 #include "../../include/iblgf_gpu.h"
 
 #include "gpu_solver.hpp"
+#include "host/ib.hpp"
 #include "host/octree.hpp"
 #include "host/tables.hpp"
+#include "ib_solver.hpp"
+#include "ops.hpp"
 
 #include <memory>
 #include <mutex>
@@ -16,13 +19,17 @@ using namespace iblgf_amd;
 
 struct iblgf_tree
 {
-    Octree tree;
-    bool   masks_built = false;
+    Octree     tree;
+    bool       masks_built = false;
+    bool       has_ib = false;
+    IbGeometry ib;
 };
 
 struct iblgf_gpu_ctx
 {
     std::unique_ptr<GpuSolver> solver;
+    std::unique_ptr<IbSystem>  ib;
+    DevArray<double>           ib_in, ib_out;
     std::string                error;
 };
 
@@ -141,12 +148,15 @@ int iblgf_tree_import(const int base_shift[3], int block_extent, int base_level,
 
 int iblgf_tree_init_refine(iblgf_tree* t, int nref, int level_up_max, iblgf_refine_fn fn, void* user)
 {
-    if (!t || nref < 0 || level_up_max < 0 || (nref > 0 && !fn)) return fail(IBLGF_ERR_ARG, "iblgf_tree_init_refine: invalid argument");
+    if (!t || nref < 0 || level_up_max < 0 || (nref > 0 && !fn && !t->has_ib))
+        return fail(IBLGF_ERR_ARG, "iblgf_tree_init_refine: invalid argument");
     return guarded(nullptr, [&] {
         RefineCond cond = [&](const Octree& tr, int o, int diff) {
             int b[3];
             tr.block_base(o, b);
-            return fn(user, b, tr.block_extent, tr.refinement_level(o), diff) != 0;
+            if (t->has_ib && t->ib.block_overlap_any(b, tr.block_extent, tr.refinement_level(o), 1)) return 2;
+            const int r = fn ? fn(user, b, tr.block_extent, tr.refinement_level(o), diff) : 0;
+            return r == 2 ? 2 : (r != 0 ? 1 : 0);
         };
         t->tree.init_refine(nref, level_up_max, cond);
         t->masks_built = false;
@@ -157,9 +167,55 @@ int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf)
 {
     if (!t) return fail(IBLGF_ERR_ARG, "iblgf_tree_build_masks: null tree");
     return guarded(nullptr, [&] {
+        if (t->has_ib) update_ib_flags(t->tree, t->ib);
         t->tree.fmm_lgf_mask_build(subtract_non_leaf != 0);
         t->tree.fmm_ib_masks();
         t->masks_built = true;
+    });
+}
+
+int iblgf_tree_set_ib(iblgf_tree* t, const double* points, int npoints, int ib_level, int delta, double safety_dis)
+{
+    if (!t || !points || npoints <= 0 || ib_level < 0 || delta < IBLGF_DELTA_YANG3 || delta > IBLGF_DELTA_ROMA)
+        return fail(IBLGF_ERR_ARG, "iblgf_tree_set_ib: invalid argument");
+    return guarded(nullptr, [&] {
+        IbGeometry g;
+        g.x.assign(points, points + 3 * static_cast<size_t>(npoints));
+        g.ib_level = ib_level;
+        g.dx_base = t->tree.dx_base;
+        g.delta = delta;
+        g.safety_dis = safety_dis;
+        t->ib = std::move(g);
+        t->has_ib = true;
+        t->masks_built = false;
+    });
+}
+
+int iblgf_tree_ib_stencil(const iblgf_tree* t, long* sizes4, int* interp_ptr, int* interp_node, double* interp_w,
+    int* spread_ptr, int* spread_dst, int* spread_col, double* spread_w, int* owned)
+{
+    if (!t || !t->has_ib || !sizes4) return fail(IBLGF_ERR_ARG, "iblgf_tree_ib_stencil: invalid argument");
+    return guarded(nullptr, [&] {
+        std::vector<int> slot(t->tree.oct.size(), -1);
+        int              k = 0;
+        for (const auto& lm : t->tree.level_map)
+            for (int o : lm) slot[o] = k++;
+        const IbStencil st = build_ib_stencil(t->tree, slot, t->ib);
+        sizes4[0] = static_cast<long>(st.interp_w.size());
+        sizes4[1] = static_cast<long>(st.spread_ptr.size()) - 1;
+        sizes4[2] = static_cast<long>(st.spread_w.size());
+        sizes4[3] = st.npoints;
+        auto put = [](auto* dst, const auto& v) {
+            if (dst) std::copy(v.begin(), v.end(), dst);
+        };
+        put(interp_ptr, st.interp_ptr);
+        put(interp_node, st.interp_node);
+        put(interp_w, st.interp_w);
+        put(spread_ptr, st.spread_ptr);
+        put(spread_dst, st.spread_dst);
+        put(spread_col, st.spread_col);
+        put(spread_w, st.spread_w);
+        put(owned, st.owned);
     });
 }
 
@@ -212,7 +268,7 @@ int iblgf_tree_export(const iblgf_tree* t, uint64_t* key, int* level, int* coord
             }
             if (flags)
                 flags[s] = (o.leaf ? 1u : 0u) | (o.correction ? 2u : 0u) | (o.leaf_boundary ? 4u : 0u) |
-                           (o.physical ? 8u : 0u);
+                           (o.physical ? 8u : 0u) | (o.is_ib ? 16u : 0u) | (o.extended_ib ? 32u : 0u);
             if (parent) parent[s] = (o.parent >= 0 && tr.oct[o.parent].has_data) ? sl(o.parent) : -1;
             if (children)
                 for (int c = 0; c < 8; ++c)
@@ -275,7 +331,12 @@ int iblgf_gpu_create(int device, iblgf_gpu_ctx** out)
 int iblgf_gpu_destroy(iblgf_gpu_ctx* ctx)
 {
     if (!ctx) return fail(IBLGF_ERR_ARG, "iblgf_gpu_destroy: null context");
-    const int rc = guarded(ctx, [&] { ctx->solver.reset(); });
+    const int rc = guarded(ctx, [&] {
+        ctx->ib.reset();
+        ctx->ib_in.release();
+        ctx->ib_out.release();
+        ctx->solver.reset();
+    });
     delete ctx;
     return rc;
 }
@@ -290,7 +351,10 @@ int iblgf_gpu_tree_upload(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
     NEED_CTX("iblgf_gpu_tree_upload");
     if (!t) return fail(IBLGF_ERR_ARG, "iblgf_gpu_tree_upload: null tree", ctx);
     if (!t->masks_built) return fail(IBLGF_ERR_STATE, "iblgf_gpu_tree_upload: masks not built", ctx);
-    return guarded(ctx, [&] { ctx->solver->upload_tree(t->tree); });
+    return guarded(ctx, [&] {
+        ctx->ib.reset();  // its fields and weights belong to the previous tree
+        ctx->solver->upload_tree(t->tree);
+    });
 }
 
 int iblgf_gpu_field_create(iblgf_gpu_ctx* ctx, int* field_id)
@@ -337,6 +401,14 @@ int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int ke
         else
             ctx->solver->apply_if(src_field, dst_field, alpha_base);
     });
+}
+
+int iblgf_gpu_set_precision(iblgf_gpu_ctx* ctx, int precision)
+{
+    NEED_CTX("iblgf_gpu_set_precision");
+    if (precision != IBLGF_PREC_FP64 && precision != IBLGF_PREC_MIXED)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_set_precision: unknown precision", ctx);
+    return guarded(ctx, [&] { ctx->solver->set_precision(precision); });
 }
 
 int iblgf_gpu_prepare(iblgf_gpu_ctx* ctx, int kernel, double alpha_base)
@@ -397,6 +469,167 @@ int iblgf_gpu_timing_reset(iblgf_gpu_ctx* ctx)
         ctx->solver->sync();
         ctx->solver->timer().collect();
         ctx->solver->timer().reset();
+    });
+}
+
+int iblgf_gpu_block_op(iblgf_gpu_ctx* ctx, int op, const int* src_fields, const int* dst_fields, int level,
+    double scale)
+{
+    NEED_CTX("iblgf_gpu_block_op");
+    if (!src_fields || !dst_fields || op < IBLGF_OP_LAPLACE || op > IBLGF_OP_CURL_TRANSPOSE)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_block_op: invalid argument", ctx);
+    return guarded(ctx, [&] { ctx->solver->block_op(op, src_fields, dst_fields, level, scale); });
+}
+
+int iblgf_gpu_buffer_exchange(iblgf_gpu_ctx* ctx, int field, int level)
+{
+    NEED_CTX("iblgf_gpu_buffer_exchange");
+    return guarded(ctx, [&] {
+        if (level < 0 || level >= ctx->solver->tree().depth) throw std::invalid_argument("level out of range");
+        ctx->solver->field_ptr(field);
+        ctx->solver->buffer_exchange(field, level);
+    });
+}
+
+int iblgf_gpu_level_divergence(iblgf_gpu_ctx* ctx, const int src_fields[3], int dst_field, int level)
+{
+    NEED_CTX("iblgf_gpu_level_divergence");
+    if (!src_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_level_divergence: null fields", ctx);
+    return guarded(ctx, [&] { ctx->solver->level_divergence(src_fields, dst_field, level); });
+}
+
+int iblgf_gpu_level_gradient(iblgf_gpu_ctx* ctx, int src_field, const int dst_fields[3], int level)
+{
+    NEED_CTX("iblgf_gpu_level_gradient");
+    if (!dst_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_level_gradient: null fields", ctx);
+    return guarded(ctx, [&] { ctx->solver->level_gradient(src_field, dst_fields, level); });
+}
+
+int iblgf_gpu_clean_boundary(iblgf_gpu_ctx* ctx, int field, int level, int kind, int leaf_only, int width)
+{
+    NEED_CTX("iblgf_gpu_clean_boundary");
+    if (kind != 0 && kind != 1) return fail(IBLGF_ERR_ARG, "iblgf_gpu_clean_boundary: unknown kind", ctx);
+    return guarded(ctx, [&] {
+        if (level < 0 || level >= ctx->solver->tree().depth) throw std::invalid_argument("level out of range");
+        ctx->solver->field_ptr(field);
+        if (kind == 0) ctx->solver->clean_leaf_correction_boundary(field, level, leaf_only != 0, width);
+        else
+            ctx->solver->clean_ib_region_boundary(field, level, width);
+    });
+}
+
+int iblgf_gpu_apply_lgf_masked(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
+    int mask_type)
+{
+    NEED_CTX("iblgf_gpu_apply_lgf_masked");
+    if (kernel != IBLGF_KERNEL_LAPLACE && kernel != IBLGF_KERNEL_IF)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_masked: unknown kernel", ctx);
+    if (mask_type != IBLGF_MASK_IB2xIB && mask_type != IBLGF_MASK_xIB2IB)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_masked: unknown mask type", ctx);
+    return guarded(ctx, [&] {
+        ctx->solver->field_ptr(src_field);
+        ctx->solver->field_ptr(dst_field);
+        if (kernel == IBLGF_KERNEL_LAPLACE) ctx->solver->apply_lgf_masked(src_field, dst_field, mask_type);
+        else
+            ctx->solver->apply_if_masked(src_field, dst_field, alpha_base, mask_type);
+    });
+}
+
+int iblgf_gpu_ib_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
+{
+    NEED_CTX("iblgf_gpu_ib_create");
+    if (!t || !t->has_ib) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_create: tree without immersed-boundary points", ctx);
+    return guarded(ctx, [&] {
+        ctx->ib.reset();
+        ctx->ib = std::make_unique<IbSystem>(*ctx->solver, t->ib);
+        ctx->ib_in.alloc(3 * static_cast<size_t>(t->ib.size()));
+        ctx->ib_out.alloc(3 * static_cast<size_t>(t->ib.size()));
+    });
+}
+
+#define NEED_IB(name)                                                                              \
+    NEED_CTX(name);                                                                                \
+    if (!ctx->ib) return fail(IBLGF_ERR_STATE, name ": no immersed-boundary system", ctx)
+
+int iblgf_gpu_ib_info(iblgf_gpu_ctx* ctx, int* npoints, int* nowned, long* nnz)
+{
+    NEED_IB("iblgf_gpu_ib_info");
+    const auto& st = ctx->ib->stencil();
+    if (npoints) *npoints = st.npoints;
+    if (nowned)
+    {
+        int c = 0;
+        for (int o : st.owned) c += o;
+        *nowned = c;
+    }
+    if (nnz) *nnz = static_cast<long>(st.interp_w.size());
+    return IBLGF_OK;
+}
+
+namespace {
+void ib_to_device(iblgf_gpu_ctx* ctx, DevArray<double>& d, const double* h)
+{
+    const hipError_t e = hipMemcpyAsync(d.p, h, d.n * sizeof(double), hipMemcpyHostToDevice, ctx->solver->stream());
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e));
+}
+void ib_to_host(iblgf_gpu_ctx* ctx, double* h, const DevArray<double>& d)
+{
+    hipError_t e = hipMemcpyAsync(h, d.p, d.n * sizeof(double), hipMemcpyDeviceToHost, ctx->solver->stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->solver->stream());
+    if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e));
+}
+}  // namespace
+
+int iblgf_gpu_ib_smear(iblgf_gpu_ctx* ctx, const double* force, const int fields[3])
+{
+    NEED_IB("iblgf_gpu_ib_smear");
+    if (!force || !fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_smear: null argument", ctx);
+    return guarded(ctx, [&] {
+        for (int c = 0; c < 3; ++c) ctx->solver->field_ptr(fields[c]);
+        ib_to_device(ctx, ctx->ib_in, force);
+        ctx->ib->smear(ctx->ib_in.p, fields);
+        ctx->solver->sync();
+    });
+}
+
+int iblgf_gpu_ib_project(iblgf_gpu_ctx* ctx, const int fields[3], double* force)
+{
+    NEED_IB("iblgf_gpu_ib_project");
+    if (!force || !fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_project: null argument", ctx);
+    return guarded(ctx, [&] {
+        for (int c = 0; c < 3; ++c) ctx->solver->field_ptr(fields[c]);
+        ctx->ib->project(fields, ctx->ib_out.p);
+        ib_to_host(ctx, force, ctx->ib_out);
+    });
+}
+
+int iblgf_gpu_ib_apply(iblgf_gpu_ctx* ctx, const double* fin, double* fout, double alpha)
+{
+    NEED_IB("iblgf_gpu_ib_apply");
+    if (!fin || !fout) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_apply: null argument", ctx);
+    return guarded(ctx, [&] {
+        ib_to_device(ctx, ctx->ib_in, fin);
+        ctx->ib->apply(ctx->ib_in.p, ctx->ib_out.p, alpha);
+        ib_to_host(ctx, fout, ctx->ib_out);
+    });
+}
+
+int iblgf_gpu_ib_cg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
+    int* iterations, double* residual, double* history)
+{
+    NEED_IB("iblgf_gpu_ib_cg");
+    if (!uc || !f || maxit < 0 || !(tol > 0.0)) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_cg: invalid argument", ctx);
+    return guarded(ctx, [&] {
+        DevArray<double> d_uc;
+        d_uc.alloc(ctx->ib_in.n);
+        ib_to_device(ctx, d_uc, uc);
+        ib_to_device(ctx, ctx->ib_out, f);
+        const CgStats st = ctx->ib->cg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit);
+        ib_to_host(ctx, f, ctx->ib_out);
+        if (iterations) *iterations = st.iterations;
+        if (residual) *residual = st.residual;
+        if (history)
+            for (size_t k = 0; k < st.history.size(); ++k) history[k] = st.history[k];
     });
 }
 

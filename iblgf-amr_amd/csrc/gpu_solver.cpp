@@ -2,6 +2,8 @@
 // gpu_solver.hpp.
 #include "gpu_solver.hpp"
 
+#include "ops.hpp"
+
 #include "host/tables.hpp"
 
 #include <algorithm>
@@ -44,6 +46,7 @@ void DevArray<T>::upload(const std::vector<T>& v)
 template struct DevArray<int>;
 template struct DevArray<double>;
 template struct DevArray<double2>;
+template struct DevArray<float2>;
 template struct DevArray<MacGroup>;
 template struct DevArray<MacSrc>;
 
@@ -190,7 +193,7 @@ void GpuSolver::upload_tree(const Octree& t)
             coord_[3 * s + d] = o.coord[d];
             bbase_[3 * s + d] = b[d];
         }
-        flags_[s] = (o.leaf ? 1u : 0u) | (o.correction ? 2u : 0u);
+        flags_[s] = (o.leaf ? 1u : 0u) | (o.correction ? 2u : 0u) | (o.is_ib ? 4u : 0u) | (o.extended_ib ? 8u : 0u);
         if (o.leaf) ++n_leaf_;
         parent_[s] = (o.parent >= 0 && t.oct[o.parent].has_data) ? sl(o.parent) : -1;
         for (int c = 0; c < 8; ++c)
@@ -301,8 +304,12 @@ void GpuSolver::upload_tree(const Octree& t)
         p_kids_[l].upload(kids);
     }
     passes_.clear();
+    clean_lists_.clear();
+    corr_level_.clear();
     ghat_lap_.clear();
     ghat_if_.clear();
+    ghat_lap32_.clear();
+    ghat_if32_.clear();
     uploaded_ = true;
 }
 
@@ -338,13 +345,12 @@ void GpuSolver::fill_zero(double* f) { launch::zero_blocks(stream_, n_, f, p_all
 // ---------------------------------------------------------------------------
 // Pass plans: Fmm::apply for base level L and non_leaf_as_source nl
 // ---------------------------------------------------------------------------
-void GpuSolver::build_pass(PassPlan& p, int L, int nl, bool nbonly)
+void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
 {
     p.L = L;
-    p.nl = nl;
+    p.nl = (idx >= 15 && idx < 30) ? (idx - 15) % 2 : 0;
     p.neighbor_only = nbonly;
-    p.idx = fmm_mask_idx_gen(MASK_AMR2AMR, L - base_level_, nl);
-    const int idx = p.idx;
+    p.idx = idx;
     auto      S = [&](int s) { return (msrc_[s] >> idx) & 1ULL; };
     auto      T = [&](int s) { return (mtgt_[s] >> idx) & 1ULL; };
 
@@ -537,11 +543,16 @@ void GpuSolver::build_pass(PassPlan& p, int L, int nl, bool nbonly)
 
 PassPlan& GpuSolver::pass(int L, int nl, bool nbonly)
 {
-    const int key = (L * 2 + nl) * 2 + (nbonly ? 1 : 0);
+    return pass_idx(L, fmm_mask_idx_gen(MASK_AMR2AMR, L - base_level_, nl), nbonly);
+}
+
+PassPlan& GpuSolver::pass_idx(int L, int idx, bool nbonly)
+{
+    const int key = (L * 64 + idx) * 2 + (nbonly ? 1 : 0);
     auto      it = passes_.find(key);
     if (it != passes_.end()) return *it->second;
     auto p = std::make_unique<PassPlan>();
-    build_pass(*p, L, nl, nbonly);
+    build_pass(*p, L, idx, nbonly);
     auto& ref = *p;
     passes_[key] = std::move(p);
     return ref;
@@ -590,13 +601,43 @@ double2* GpuSolver::ghat_if(double alpha)
     return ptr;
 }
 
+const void* GpuSolver::ghat(bool laplace, int d, double alpha)
+{
+    double2* g64 = laplace ? ghat_laplace(d) : ghat_if(alpha);
+    if (precision_ == 0) return g64;
+    DevArray<float2>* slot = nullptr;
+    if (laplace)
+    {
+        auto it = ghat_lap32_.find(d);
+        if (it != ghat_lap32_.end()) return it->second.p;
+        slot = &ghat_lap32_[d];
+    }
+    else
+    {
+        auto it = ghat_if32_.find(alpha);
+        if (it != ghat_if32_.end()) return it->second.p;
+        slot = &ghat_if32_[alpha];
+    }
+    // generated in FP64, stored in FP32
+    slot->alloc(static_cast<size_t>(343) * nf_);
+    launch::to_float2(stream_, g64, slot->p, 343L * nf_);
+    IBLGF_HIP(hipGetLastError());
+    return slot->p;
+}
+
+void GpuSolver::set_precision(int p)
+{
+    if (p != 0 && p != 1) throw std::invalid_argument("precision must be 0 (FP64) or 1 (mixed FP32/FP64)");
+    precision_ = p;
+}
+
 void GpuSolver::prepare_laplace()
 {
     for (int l = base_level_; l < depth_; ++l)
         for (int nl = 0; nl < 2; ++nl)
         {
             PassPlan& p = pass(l, nl, false);
-            for (auto& m : p.mac) ghat_laplace(m.d);
+            for (auto& m : p.mac) ghat(true, m.d, 0.0);
         }
     sync();
 }
@@ -606,7 +647,7 @@ void GpuSolver::prepare_if(double alpha_base)
     for (int l = base_level_; l < depth_; ++l)
     {
         for (int nl = 0; nl < 2; ++nl) pass(l, nl, true);
-        ghat_if(alpha_base * std::pow(4.0, l - base_level_));
+        ghat(false, 0, alpha_base * std::pow(4.0, l - base_level_));
     }
     sync();
 }
@@ -629,18 +670,18 @@ void GpuSolver::run_pass(PassPlan& p, int src_field, int dst_field, double scale
         timer_.end(stream_, KC_TRANSFER, p.ntransfer_antrp);
     }
     timer_.begin(stream_);
-    launch::forward_dft(stream_, n_, fmm_s, p.fwd.p, p.fwd.size(), sspec_.p);
+    launch::forward_dft(stream_, n_, precision_, fmm_s, p.fwd.p, p.fwd.size(), sspec_.p);
     timer_.end(stream_, KC_FORWARD, p.fwd.size());
     for (auto& m : p.mac)
     {
-        double2* gh = laplace ? ghat_laplace(m.d) : ghat_if(alpha);
+        const void* gh = ghat(laplace, m.d, alpha);
         timer_.begin(stream_);
-        launch::spectral_mac(stream_, n_, m.groups.p, m.groups.size(), m.srcs.p, gh, sspec_.p, tspec_.p);
+        launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, gh, sspec_.p, tspec_.p);
         timer_.end(stream_, KC_MAC, m.npairs);
     }
     const double P = 2.0 * n_ - 1.0;
     timer_.begin(stream_);
-    launch::inverse_dft(stream_, n_, tspec_.p, p.inv.p, p.inv.size(), fmm_t, scale / (P * P * P));
+    launch::inverse_dft(stream_, n_, precision_, tspec_.p, p.inv.p, p.inv.size(), fmm_t, scale / (P * P * P));
     timer_.end(stream_, KC_INVERSE, p.inv.size());
     if (!p.neighbor_only)
     {
@@ -727,6 +768,151 @@ void GpuSolver::apply_if(int src, int dst, double alpha_base)
     IBLGF_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Immersed-boundary solves and level operators
+// ---------------------------------------------------------------------------
+void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
+{
+    if (!uploaded_) throw std::runtime_error("tree not uploaded");
+    if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB)
+        throw std::invalid_argument("mask type must be IB2xIB or xIB2IB");
+    IBLGF_HIP(hipSetDevice(device_));
+    const int L = depth_ - 1;
+    double*   st = F(f_src_tmp_);
+    double*   tt = F(f_tgt_tmp_);
+    fill_zero(st);
+    fill_zero(tt);
+    fill_zero(F(f_corr_tmp_));
+    launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[L].p, p_level_slots_[L].size(), true);
+    const double dx = dx_base_ / std::pow(2.0, L - base_level_);
+    run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), false), f_src_tmp_, f_tgt_tmp_, dx * dx, true, 0.0);
+    launch::copy_blocks(stream_, n_, F(dst), tt, p_leaf_level_[L].p, p_leaf_level_[L].size(), false);
+    IBLGF_HIP(hipGetLastError());
+}
+
+void GpuSolver::apply_if_masked(int src, int dst, double alpha_base, int mask_type)
+{
+    if (!uploaded_) throw std::runtime_error("tree not uploaded");
+    if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB)
+        throw std::invalid_argument("mask type must be IB2xIB or xIB2IB");
+    IBLGF_HIP(hipSetDevice(device_));
+    const int L = depth_ - 1;
+    double*   st = F(f_src_tmp_);
+    double*   tt = F(f_tgt_tmp_);
+    fill_zero(st);
+    fill_zero(tt);
+    launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[L].p, p_level_slots_[L].size(), true);
+    const double alpha = alpha_base * std::pow(4.0, L - base_level_);
+    run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), true), f_src_tmp_, f_tgt_tmp_, 1.0, false, alpha);
+    launch::copy_blocks(stream_, n_, F(dst), tt, p_level_slots_[L].p, p_level_slots_[L].size(), false);
+    IBLGF_HIP(hipGetLastError());
+}
+
+void GpuSolver::block_op(int op, const int* src, const int* dst, int level, double scale)
+{
+    if (!uploaded_) throw std::runtime_error("tree not uploaded");
+    if (level < 0 || level >= depth_) throw std::invalid_argument("level out of range");
+    const int ncs = (op == OP_LAPLACE || op == OP_GRADIENT) ? 1 : 3;
+    const int ncd = (op == OP_LAPLACE || op == OP_DIVERGENCE) ? 1 : 3;
+    const double* s[3] = {nullptr, nullptr, nullptr};
+    double*       d[3] = {nullptr, nullptr, nullptr};
+    for (int c = 0; c < ncs; ++c) s[c] = F(src[c]);
+    for (int c = 0; c < ncd; ++c) d[c] = F(dst[c]);
+    for (int c = 0; c < ncs; ++c)
+        for (int k = 0; k < ncd; ++k)
+            if (src[c] == dst[k]) throw std::invalid_argument("block operator output aliases its input");
+    const double dx = dx_base_ / std::pow(2.0, level - base_level_);
+    double       fac = 1.0 / dx;
+    if (op == OP_LAPLACE) fac = 1.0 / (dx * dx);
+    if (op == OP_CURL_TRANSPOSE) fac = 1.0 / dx * scale;
+    IBLGF_HIP(hipSetDevice(device_));
+    launch::block_op(stream_, n_, op, s, d, p_level_slots_[level].p, p_level_slots_[level].size(), fac);
+    IBLGF_HIP(hipGetLastError());
+}
+
+void GpuSolver::buffer_exchange(int f, int level)
+{
+    launch::buffer_exchange(stream_, n_, F(f), p_level_slots_[level].p, p_level_slots_[level].size(), d_neighbor_.p);
+}
+
+void GpuSolver::level_divergence(const int src[3], int dst, int level)
+{
+    for (int c = 0; c < 3; ++c) buffer_exchange(src[c], level);
+    block_op(OP_DIVERGENCE, src, &dst, level, 1.0);
+    clean_leaf_correction_boundary(dst, level, true, 2);
+}
+
+void GpuSolver::level_gradient(int src, const int dst[3], int level)
+{
+    block_op(OP_GRADIENT, &src, dst, level, 1.0);
+    for (int c = 0; c < 3; ++c) buffer_exchange(dst[c], level);
+}
+
+// kind 0: leaf/correction boundary (neighbour missing, or a correction block
+// when leaf_only); kind 1: IB region boundary (neighbour missing or not IB).
+const DevArray<int>& GpuSolver::face_clean_list(int kind, int level, bool leaf_only)
+{
+    const int key = (level * 2 + kind) * 2 + (leaf_only ? 1 : 0);
+    auto      it = clean_lists_.find(key);
+    if (it != clean_lists_.end()) return it->second;
+    static const int kFaceSlot[6] = {4, 10, 12, 14, 16, 22};
+    std::vector<int> ent;
+    for (int s = lvl_begin_[level]; s < lvl_begin_[level + 1]; ++s)
+    {
+        int mask = 0;
+        for (int b = 0; b < 6; ++b)
+        {
+            const int nb = neighbor_[27 * s + kFaceSlot[b]];
+            bool      clean = nb < 0;
+            if (!clean && kind == 0) clean = leaf_only && (flags_[nb] & 2u);
+            if (!clean && kind == 1) clean = !(flags_[nb] & 4u);
+            if (clean) mask |= 1 << b;
+        }
+        if (mask) ent.insert(ent.end(), {s, mask});
+    }
+    return clean_lists_[key] = [&] {
+        DevArray<int> a;
+        a.upload(ent);
+        return a;
+    }();
+}
+
+void GpuSolver::clean_leaf_correction_boundary(int f, int level, bool leaf_only, int width)
+{
+    if (leaf_only)
+    {
+        auto it = corr_level_.find(level);
+        if (it == corr_level_.end())
+        {
+            std::vector<int> cs;
+            for (int s = lvl_begin_[level]; s < lvl_begin_[level + 1]; ++s)
+                if (flags_[s] & 2u) cs.push_back(s);
+            DevArray<int> a;
+            a.upload(cs);
+            it = corr_level_.emplace(level, std::move(a)).first;
+        }
+        launch::zero_blocks(stream_, n_, F(f), it->second.p, it->second.size());
+    }
+    if (level == base_level_)
+    {
+        const auto& l = face_clean_list(0, level, leaf_only);
+        launch::clean_faces(stream_, n_, F(f), l.p, l.size() / 2, width);
+    }
+}
+
+void GpuSolver::clean_ib_region_boundary(int f, int level, int width)
+{
+    const auto& l = face_clean_list(1, level, false);
+    launch::clean_faces(stream_, n_, F(f), l.p, l.size() / 2, width);
+}
+
+void GpuSolver::domain_clean(int f) { fill_zero(F(f)); }
+
+void GpuSolver::add(int from, int to, double scale)
+{
+    launch::axpy_blocks(stream_, n_, F(to), F(from), p_all_.p, nb_, scale);
+}
+
 SolveCounts GpuSolver::lgf_counts() const
 {
     SolveCounts c;
@@ -734,7 +920,7 @@ SolveCounts GpuSolver::lgf_counts() const
     for (const auto& kv : passes_)
     {
         const PassPlan& p = *kv.second;
-        if (p.neighbor_only) continue;
+        if (p.neighbor_only || p.idx < 15 || p.idx >= 30) continue;  // Laplace AMR2AMR passes only
         c.passes += 1;
         c.n_src += p.fwd.size();
         c.n_tgt += p.inv.size();

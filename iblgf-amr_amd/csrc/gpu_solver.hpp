@@ -139,6 +139,36 @@ class GpuSolver
     void prepare_laplace();
     void prepare_if(double alpha_base);
 
+    // Finest-level solves with an immersed-boundary FMM mask (MASK_IB2xIB or
+    // MASK_xIB2IB): PoissonSolver::apply_lgf / apply_if with fmm_type set to
+    // it (poisson.hpp:587-894, 435-494).  The source is the interior of the
+    // finest-level blocks; there is no coarsening and one FMM pass.
+    // apply_lgf_masked writes the finest leaves, apply_if_masked every
+    // finest-level block (buffers included).
+    void apply_lgf_masked(int src, int dst, int mask_type);
+    void apply_if_masked(int src, int dst, double alpha_base, int mask_type);
+
+    // Block operators over the blocks with data at one level (BlockOp in
+    // ops.hpp; scale multiplies 1/dx for the curl transpose).
+    void block_op(int op, const int* src, const int* dst, int level, double scale);
+    void buffer_exchange(int f, int level);
+    // Operator::levelDivergence / levelGradient (operators.hpp:522-538, 588-604)
+    void level_divergence(const int src[3], int dst, int level);
+    void level_gradient(int src, const int dst[3], int level);
+    // Operator::clean_leaf_correction_boundary (operators.hpp:329-513) and
+    // clean_ib_region_boundary (73-187)
+    void clean_leaf_correction_boundary(int f, int level, bool leaf_only, int width);
+    void clean_ib_region_boundary(int f, int level, int width);
+    // Operator::domainClean (42-68) and Operator::add (3136-3153)
+    void domain_clean(int f);
+    void add(int from, int to, double scale);
+    const Octree& tree() const { return tree_; }
+    const std::vector<int>& slot_of() const { return slot_of_; }
+    // 0: plain FP64; 1: mixed (FP32 spectra, DFT legs and MAC; FP64 fields,
+    // kernel-spectrum generation and cross-level transfers).
+    void set_precision(int p);
+    int  precision() const { return precision_; }
+
     void        sync();
     hipStream_t stream() const { return stream_; }
     int         nblocks() const { return nb_; }
@@ -148,11 +178,14 @@ class GpuSolver
     KernelTimer& timer() { return timer_; }
 
   private:
-    void      build_pass(PassPlan& p, int L, int nl, bool neighbor_only);
+    void      build_pass(PassPlan& p, int L, int idx, bool neighbor_only);
     PassPlan& pass(int L, int nl, bool neighbor_only);
+    PassPlan& pass_idx(int L, int idx, bool neighbor_only);
+    const DevArray<int>& face_clean_list(int kind, int level, bool leaf_only);
     void      run_pass(PassPlan& p, int src_field, int dst_field, double scale, bool laplace, double alpha);
-    double2*  ghat_laplace(int d);
-    double2*  ghat_if(double alpha);
+    double2*    ghat_laplace(int d);
+    double2*    ghat_if(double alpha);
+    const void* ghat(bool laplace, int d, double alpha);  // in the current precision
     void      fill_zero(double* f);
     double*   F(int id) { return fields_.at(id).p; }
 
@@ -190,6 +223,9 @@ class GpuSolver
     int                                      lgf_nmax_ = 100;
     std::map<int, DevArray<double2>>         ghat_lap_;
     std::map<double, DevArray<double2>>      ghat_if_;
+    std::map<int, DevArray<float2>>          ghat_lap32_;
+    std::map<double, DevArray<float2>>       ghat_if32_;
+    int                                      precision_ = 0;
     DevArray<double>                         gen_sample_;
     DevArray<double2>                        gen_a_, gen_b_;
     DevArray<double>                         d_iv_;
@@ -204,6 +240,8 @@ class GpuSolver
     std::vector<DevArray<int>>               p_ccint_corr_;  // correction children only
     std::vector<DevArray<int>>               p_kids_;        // child slots of level-l octants
     DevArray<int>                            p_all_;
+    std::map<int, DevArray<int>>             clean_lists_;   // {slot, face mask} entries
+    std::map<int, DevArray<int>>             corr_level_;    // correction blocks per level
 };
 
 }  // namespace iblgf_amd

@@ -255,10 +255,10 @@ void Octree::init_refine(int nRef, int level_up_max, const RefineCond& cond)
         auto visit = [&](int cur) {
             if (refinement_level(cur) != l) return;
             if (!oct[cur].has_data) return;
-            if (cond(*this, cur, nRef - l))
-            {
-                if (try_2to1(oct[cur].key, checklist)) refine(cur, true);
-            }
+            const int r = cond(*this, cur, nRef - l);
+            if (r == 2) refine(cur, true);
+            else if (r == 1 && try_2to1(oct[cur].key, checklist))
+                refine(cur, true);
         };
         visit(0);  // the root is the first element, visited before it is popped
         std::vector<int> st{0};

@@ -237,8 +237,10 @@ struct DomainSpec
     std::vector<std::array<int, 6>> regions;  // base xyz, extent xyz
 };
 
-// Refinement predicate: (octant, diff_level) -> refine?
-using RefineCond = std::function<bool(const struct Octree&, int, int)>;
+// Refinement predicate: (octant, diff_level) -> 0 keep, 1 refine if the
+// 2:1 balance allows it (ref_cond_), 2 refine unconditionally (a block that
+// overlaps an immersed-boundary point, domain.hpp:887-891).
+using RefineCond = std::function<int(const struct Octree&, int, int)>;
 
 struct Octree
 {

@@ -1,0 +1,200 @@
+#include "ib_solver.hpp"
+
+#include "ops.hpp"
+
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <stdexcept>
+
+namespace iblgf_amd {
+
+#define IBLGF_HIP_IB(x)                                                                               \
+    do                                                                                                \
+    {                                                                                                 \
+        const hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+IbSystem::IbSystem(GpuSolver& solver, const IbGeometry& g) : s_(solver)
+{
+    if (solver.nblocks() == 0) throw std::runtime_error("tree not uploaded");
+    npoints_ = g.size();
+    if (npoints_ <= 0) throw std::invalid_argument("no immersed-boundary points");
+    st_ = build_ib_stencil(solver.tree(), solver.slot_of(), g);
+    d_sp_ptr_.upload(st_.spread_ptr);
+    d_sp_dst_.upload(st_.spread_dst);
+    d_sp_col_.upload(st_.spread_col);
+    d_sp_w_.upload(st_.spread_w);
+    d_in_ptr_.upload(st_.interp_ptr);
+    d_in_node_.upload(st_.interp_node);
+    d_in_w_.upload(st_.interp_w);
+    std::vector<double> own(3 * static_cast<size_t>(npoints_));
+    for (int i = 0; i < npoints_; ++i)
+        for (int c = 0; c < 3; ++c) own[3 * i + c] = st_.owned[i] ? 1.0 : 0.0;
+    d_owned_.upload(own);
+    for (int c = 0; c < 3; ++c)
+    {
+        field_[c] = s_.field_create();
+        aux_[c] = s_.field_create();
+    }
+    cell_ = s_.field_create();
+    for (auto& v : d_tmp_) v.alloc(3 * static_cast<size_t>(npoints_));
+    d_part_.alloc(launch::kDotBlocks);
+    d_out_.alloc(1);
+}
+
+void IbSystem::smear(const double* d_force, const int fields[3])
+{
+    for (int c = 0; c < 3; ++c) s_.domain_clean(fields[c]);
+    double* f[3] = {s_.field_ptr(fields[0]), s_.field_ptr(fields[1]), s_.field_ptr(fields[2])};
+    launch::ib_spread(s_.stream(), d_force, d_sp_ptr_.p, d_sp_dst_.p, d_sp_col_.p, d_sp_w_.p,
+        static_cast<int>(st_.spread_ptr.size()) - 1, f);
+    IBLGF_HIP_IB(hipGetLastError());
+}
+
+void IbSystem::project(const int fields[3], double* d_force)
+{
+    const double* f[3] = {s_.field_ptr(fields[0]), s_.field_ptr(fields[1]), s_.field_ptr(fields[2])};
+    launch::ib_interp(s_.stream(), f, d_in_ptr_.p, d_in_node_.p, d_in_w_.p, 3 * npoints_, d_force);
+    IBLGF_HIP_IB(hipGetLastError());
+}
+
+// LinSysSolver::ET_H_S_E with Field = face_aux2 and fmm types IB2xIB
+// (integrating factor) / xIB2IB (Schur complement), linsys.hpp:259-319.
+void IbSystem::apply(const double* d_fin, double* d_fout, double alpha)
+{
+    const int L = s_.tree().depth - 1;
+    for (int c = 0; c < 3; ++c)
+    {
+        s_.domain_clean(field_[c]);
+        s_.domain_clean(aux_[c]);
+    }
+    smear(d_fin, field_);
+    if (std::fabs(alpha) > 1e-14)
+        for (int c = 0; c < 3; ++c) s_.apply_if_masked(field_[c], field_[c], alpha, MASK_IB2xIB);
+    s_.level_divergence(field_, cell_, L);
+    s_.clean_ib_region_boundary(cell_, L, 2);
+    s_.apply_lgf_masked(cell_, cell_, MASK_xIB2IB);
+    s_.level_gradient(cell_, aux_, L);
+    for (int c = 0; c < 3; ++c) s_.clean_ib_region_boundary(aux_[c], L, 2);
+    for (int c = 0; c < 3; ++c) s_.add(aux_[c], field_[c], -1.0);
+    project(field_, d_fout);
+}
+
+double IbSystem::dot(const double* a, const double* b)
+{
+    launch::dot(s_.stream(), a, b, d_owned_.p, 3L * npoints_, d_part_.p, d_out_.p);
+    double v = 0.0;
+    IBLGF_HIP_IB(hipMemcpyAsync(&v, d_out_.p, sizeof(double), hipMemcpyDeviceToHost, s_.stream()));
+    IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
+    return v;
+}
+
+void IbSystem::add(double* a, const double* b, double s1, double s2)
+{
+    launch::axpby(s_.stream(), 3L * npoints_, s2, b, s1, a, d_owned_.p);
+}
+
+// CG_solve (linsys.hpp:170-230).
+CgStats IbSystem::cg(const double* d_uc, double* d_f, double alpha, double tol, int maxit)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    CgStats    st;
+    const long len = 3L * npoints_;
+    double *   r = d_tmp_[0].p, *p = d_tmp_[1].p, *Ap = d_tmp_[2].p, *Ax = d_tmp_[3].p;
+    apply(d_f, Ax, alpha);
+    ++st.applies;
+    IBLGF_HIP_IB(hipMemsetAsync(r, 0, len * sizeof(double), s_.stream()));
+    add(r, d_uc, 0.0, 1.0);
+    add(r, Ax, 1.0, -1.0);
+    IBLGF_HIP_IB(hipMemcpyAsync(p, r, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
+    double rsold = dot(r, r);
+    for (int k = 0; k < maxit; ++k)
+    {
+        apply(p, Ap, alpha);
+        ++st.applies;
+        const double pAp = dot(p, Ap);
+        if (pAp == 0.0) break;
+        const double a = rsold / pAp;
+        add(d_f, p, 1.0, a);
+        add(r, Ap, 1.0, -a);
+        const double rsnew = dot(r, r);
+        const double f2 = dot(d_f, d_f);
+        st.iterations = k + 1;
+        st.residual = std::sqrt(rsnew / f2);
+        st.history.push_back(st.residual);
+        if (on_iteration) on_iteration(k, r, p, d_f);
+        if (st.residual < tol) break;
+        add(p, r, rsnew / rsold, 1.0);
+        rsold = rsnew;
+    }
+    IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+// Flexible CG (Y. Notay, SIAM J. Sci. Comput. 22 (2000) 1444-1460) keeping
+// the last kKeep search directions: p_k = z_k - sum_i (z_k.q_i)/(p_i.q_i) p_i,
+// q = A p, alpha = (p_k.r_k)/(p_k.q_k).  A preconditioned direction that is
+// not finite or not a descent direction is replaced by r (a CG step).
+CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M)
+{
+    constexpr int kKeep = 4;
+    const auto    t0 = std::chrono::steady_clock::now();
+    CgStats       st;
+    const long    len = 3L * npoints_;
+    double *      r = d_tmp_[0].p, *z = d_tmp_[4].p, *Ax = d_tmp_[3].p;
+    std::vector<DevArray<double>> P(kKeep + 1), Q(kKeep + 1);
+    for (auto& v : P) v.alloc(len);
+    for (auto& v : Q) v.alloc(len);
+    std::deque<std::pair<int, double>> hist;  // (buffer index, p.q)
+    apply(d_f, Ax, alpha);
+    ++st.applies;
+    IBLGF_HIP_IB(hipMemsetAsync(r, 0, len * sizeof(double), s_.stream()));
+    add(r, d_uc, 0.0, 1.0);
+    add(r, Ax, 1.0, -1.0);
+    int next = 0;
+    for (int k = 0; k < maxit; ++k)
+    {
+        IBLGF_HIP_IB(hipMemsetAsync(z, 0, len * sizeof(double), s_.stream()));
+        M(s_.stream(), r, z);
+        // non-owned entries of z never reach p: p is built by masked updates
+        double zr = dot(z, r);
+        if (!std::isfinite(zr) || zr <= 0.0)
+        {
+            IBLGF_HIP_IB(hipMemcpyAsync(z, r, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
+            st.fell_back = true;
+        }
+        double* p = P[next].p;
+        double* q = Q[next].p;
+        IBLGF_HIP_IB(hipMemsetAsync(p, 0, len * sizeof(double), s_.stream()));
+        add(p, z, 0.0, 1.0);
+        for (const auto& h : hist)
+        {
+            const double beta = dot(z, Q[h.first].p) / h.second;
+            add(p, P[h.first].p, 1.0, -beta);
+        }
+        apply(p, q, alpha);
+        ++st.applies;
+        const double pq = dot(p, q);
+        if (pq == 0.0) break;
+        const double a = dot(p, r) / pq;
+        add(d_f, p, 1.0, a);
+        add(r, q, 1.0, -a);
+        const double rr = dot(r, r), f2 = dot(d_f, d_f);
+        st.iterations = k + 1;
+        st.residual = std::sqrt(rr / f2);
+        st.history.push_back(st.residual);
+        if (on_iteration) on_iteration(k, r, p, d_f);
+        if (st.residual < tol) break;
+        hist.emplace_back(next, pq);
+        if ((int)hist.size() > kKeep) hist.pop_front();
+        next = (next + 1) % (kKeep + 1);
+    }
+    IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return st;
+}
+
+}  // namespace iblgf_amd

@@ -1,0 +1,75 @@
+// Immersed-boundary force system on the device: the operator
+// A = E (I - G L^-1 D) H_alpha E^T of LinSysSolver::ET_H_S_E
+// (include/iblgf/solver/linsys/linsys.hpp:259-292, apply_Schur 294-319) and
+// its conjugate-gradient solve (CG_solve, 170-230), with FP64 dot products.
+//
+// Force vectors are N*3 FP64, point-major (the reference's force_type).
+// Points not strictly inside a finest leaf block (ib rank -1 in the
+// reference) are not owned: their residual is held at zero and they are
+// left out of every dot product, as LinSysSolver::dot / add skip them.
+#pragma once
+
+#include "gpu_solver.hpp"
+#include "host/ib.hpp"
+
+#include <functional>
+#include <vector>
+
+namespace iblgf_amd {
+
+struct CgStats
+{
+    int                 iterations = 0;
+    double              residual = 0.0;  // sqrt(r.r / f.f) of the recurrence
+    std::vector<double> history;         // residual after each iteration
+    double              seconds = 0.0;
+    int                 applies = 0;     // operator applications
+    bool                fell_back = false;
+};
+
+// Flexible preconditioner: z = M(r) on device vectors of length 3N.
+using Preconditioner = std::function<void(hipStream_t, const double* r, double* z)>;
+
+class IbSystem
+{
+  public:
+    IbSystem(GpuSolver& solver, const IbGeometry& g);
+
+    int npoints() const { return npoints_; }
+    const IbStencil& stencil() const { return st_; }
+    const double*    owned_mask() const { return d_owned_.p; }
+
+    // LinSysSolver::smearing / projection (linsys.hpp:350-376, 417-445)
+    void smear(const double* d_force, const int fields[3]);
+    void project(const int fields[3], double* d_force);
+    // ET_H_S_E: fout = A fin
+    void apply(const double* d_fin, double* d_fout, double alpha);
+
+    // CG_solve: f holds the initial guess and receives the solution.
+    CgStats cg(const double* d_uc, double* d_f, double alpha, double tol, int maxit);
+    // Flexible CG (Notay 2000, truncated to the last direction) with the
+    // preconditioner M; falls back to plain CG directions when M returns a
+    // non-finite or non-descent direction.
+    CgStats fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M);
+
+    // FP64 dot over owned entries (LinSysSolver::dot).
+    double dot(const double* a, const double* b);
+    // a = a*s1 + b*s2 over owned entries (LinSysSolver::add).
+    void add(double* a, const double* b, double s1, double s2);
+
+    // Called after each CG/FCG iteration with (k, r, z or p, f) device
+    // vectors; used to collect training pairs.
+    std::function<void(int, const double*, const double*, const double*)> on_iteration;
+
+  private:
+    GpuSolver&      s_;
+    int             npoints_ = 0;
+    IbStencil       st_;
+    DevArray<int>   d_sp_ptr_, d_sp_dst_, d_sp_col_, d_in_ptr_, d_in_node_;
+    DevArray<double> d_sp_w_, d_in_w_, d_owned_;
+    int             field_[3], aux_[3], cell_;
+    DevArray<double> d_tmp_[6];  // r, p, Ap, Ax, z, spare
+    DevArray<double> d_part_, d_out_;
+};
+
+}  // namespace iblgf_amd

@@ -256,28 +256,50 @@ __global__ __launch_bounds__(kThreads) void k_correction_stencil(
 // the twiddle index depends only on loop counters that are uniform across the
 // workgroup, so twiddle reads from LDS are broadcasts.  The kx axis is the
 // halved one; legs after the first run on GS kx-planes at a time so the
-// intermediate fits in LDS.
+// intermediate fits in LDS.  Arithmetic is FP64; T is the storage type of
+// the spectra and the LDS intermediates (double: plain FP64 path; float:
+// mixed path, whose fields stay FP64).
 // ---------------------------------------------------------------------------
-template<int N>
+template<class T>
+struct Cx;
+template<>
+struct Cx<double>
+{
+    using type = double2;
+    static __device__ inline double2 make(double a, double b) { return make_double2(a, b); }
+};
+template<>
+struct Cx<float>
+{
+    using type = float2;
+    static __device__ inline float2 make(float a, float b) { return make_float2(a, b); }
+};
+
+template<int N, class T>
 struct DftPlan
 {
     static constexpr int P = 2 * N - 1;
     static constexpr int H = P / 2 + 1;
-    static constexpr int kLds = 163840 - 4096;
-    static constexpr int kFwdRaw = (kLds - N * N * H * 16 - P * 16) / (N * P * 16);
+    static constexpr int CS = 2 * (int)sizeof(T);
+    // FP64 needs most of the 160 KB LDS for one block; FP32 is held to half
+    // of it so two workgroups share a CU.
+    static constexpr int kLds = sizeof(T) == 8 ? 163840 - 4096 : 81920 - 2048;
+    static constexpr int kFwdRaw = (kLds - N * N * H * CS - P * 16) / (N * P * CS);
     static constexpr int FWD_GS = kFwdRaw < H ? kFwdRaw : H;
-    static constexpr int kInvRaw = (kLds - H * N * N * 16 - P * 16) / (P * N * 16);
+    static constexpr int kInvRaw = (kLds - H * N * N * CS - P * 16) / (P * N * CS);
     static constexpr int INV_GS = kInvRaw < H ? kInvRaw : H;
 };
 
 // a * (c - i s)
-__device__ inline void cmac_conj(double2 a, double2 w, double& sr, double& si)
+template<class C, class T>
+__device__ inline void cmac_conj(C a, C w, T& sr, T& si)
 {
     sr = fma(a.x, w.x, fma(a.y, w.y, sr));
     si = fma(a.y, w.x, fma(-a.x, w.y, si));
 }
 // a * (c + i s)
-__device__ inline void cmac(double2 a, double2 w, double& sr, double& si)
+template<class C, class T>
+__device__ inline void cmac(C a, C w, T& sr, T& si)
 {
     sr = fma(a.x, w.x, fma(-a.y, w.y, sr));
     si = fma(a.x, w.y, fma(a.y, w.x, si));
@@ -285,15 +307,19 @@ __device__ inline void cmac(double2 a, double2 w, double& sr, double& si)
 
 // Forward r2c of an n^3 block zero-padded to P^3 (pruned legs): x leg into LDS,
 // then y and z legs per group of kx planes; output spec[kx][ky][kz].
-template<int N>
-__global__ __launch_bounds__(kThreads) void k_forward_dft(
-    const double* __restrict__ field, const int* __restrict__ slots, int count, double2* __restrict__ spec)
+// Arithmetic is FP64 on both paths; T is only the storage type of the
+// intermediates in LDS and of the spectrum (FP32 storage rounds each leg's
+// output once, the legs' sums keep FP64 accuracy).
+template<int N, class T>
+__global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restrict__ field,
+    const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec)
 {
+    using C = typename Cx<T>::type;
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::FWD_GS;
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N, T>::FWD_GS;
     static_assert(GS >= 1, "LDS too small for one plane");
-    __shared__ double2 X1[N * N * H];  // [z][y][kx]
-    __shared__ double2 T[GS * N * P];  // [g][z][ky]
+    __shared__ C       X1[N * N * H];  // [z][y][kx]
+    __shared__ C       Tp[GS * N * P];  // [g][z][ky]
     __shared__ double2 W[P];
     const int b = blockIdx.x;
     if (b >= count) return;
@@ -308,7 +334,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
         for (int x = 0; x < N; ++x) v[x] = in[l * N + x];
         for (int kx = 0; kx < H; ++kx)
         {
-            double sr = 0.0, si = 0.0;
+            double sr = 0, si = 0;
             int    m = 0;
 #pragma unroll
             for (int x = 0; x < N; ++x)
@@ -319,11 +345,11 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
                 m += kx;
                 if (m >= P) m -= P;
             }
-            X1[l * H + kx] = make_double2(sr, si);
+            X1[l * H + kx] = Cx<T>::make((T)sr, (T)si);
         }
     }
     __syncthreads();
-    double2* out = spec + (size_t)slot * G::NF;
+    C*            out = spec + (size_t)slot * G::NF;
     constexpr int K0 = (P + 1) / 2;  // y-leg outputs split over two threads per line
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
@@ -334,11 +360,15 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
             const int g = line / N, z = line - g * N, kx = kx0 + g;
             double2   a[N];
 #pragma unroll
-            for (int y = 0; y < N; ++y) a[y] = X1[(z * N + y) * H + kx];
+            for (int y = 0; y < N; ++y)
+            {
+                const C c = X1[(z * N + y) * H + kx];
+                a[y] = make_double2(c.x, c.y);
+            }
             const int k_lo = half ? K0 : 0, k_hi = half ? P : K0;
             for (int ky = k_lo; ky < k_hi; ++ky)
             {
-                double sr = 0.0, si = 0.0;
+                double sr = 0, si = 0;
                 int    m = 0;
 #pragma unroll
                 for (int y = 0; y < N; ++y)
@@ -347,7 +377,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
                     m += ky;
                     if (m >= P) m -= P;
                 }
-                T[(g * N + z) * P + ky] = make_double2(sr, si);
+                Tp[(g * N + z) * P + ky] = Cx<T>::make((T)sr, (T)si);
             }
         }
         __syncthreads();
@@ -356,11 +386,15 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
             const int g = l / P, ky = l - g * P, kx = kx0 + g;
             double2   a[N];
 #pragma unroll
-            for (int z = 0; z < N; ++z) a[z] = T[(g * N + z) * P + ky];
-            double2* o = out + (size_t)(kx * P + ky) * P;
+            for (int z = 0; z < N; ++z)
+            {
+                const C c = Tp[(g * N + z) * P + ky];
+                a[z] = make_double2(c.x, c.y);
+            }
+            C* o = out + (size_t)(kx * P + ky) * P;
             for (int kz = 0; kz < P; ++kz)
             {
-                double sr = 0.0, si = 0.0;
+                double sr = 0, si = 0;
                 int    m = 0;
 #pragma unroll
                 for (int z = 0; z < N; ++z)
@@ -369,7 +403,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
                     m += kz;
                     if (m >= P) m -= P;
                 }
-                o[kz] = make_double2(sr, si);
+                o[kz] = Cx<T>::make((T)sr, (T)si);
             }
         }
         __syncthreads();
@@ -377,21 +411,22 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(
 }
 
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
-// added into the target block.
-template<int N>
-__global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restrict__ spec,
+// added into the (FP64) target block.  FP64 arithmetic, T-typed storage.
+template<int N, class T>
+__global__ __launch_bounds__(kThreads) void k_inverse_dft(const typename Cx<T>::type* __restrict__ spec,
     const int* __restrict__ slots, int count, double* __restrict__ field, double scale)
 {
+    using C = typename Cx<T>::type;
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::INV_GS;
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N, T>::INV_GS;
     static_assert(GS >= 1, "LDS too small for one plane");
-    __shared__ double2 V[H * N * N];   // [kx][z][y]
-    __shared__ double2 U[GS * P * N];  // [g][ky][z]
+    __shared__ C       V[H * N * N];   // [kx][z][y]
+    __shared__ C       U[GS * P * N];  // [g][ky][z]
     __shared__ double2 W[P];
     const int b = blockIdx.x;
     if (b >= count) return;
-    const int      slot = slots[b];
-    const double2* in = spec + (size_t)slot * G::NF;
+    const int slot = slots[b];
+    const C*  in = spec + (size_t)slot * G::NF;
     twiddles(W, P);
     __syncthreads();
     for (int kx0 = 0; kx0 < H; kx0 += GS)
@@ -400,15 +435,19 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
         // z leg: (g, ky) rows of P frequencies -> N window points
         for (int l = threadIdx.x; l < gs * P; l += kThreads)
         {
-            const int      g = l / P, ky = l - g * P, kx = kx0 + g;
-            const double2* row = in + (size_t)(kx * P + ky) * P;
-            double2        a[P];
+            const int g = l / P, ky = l - g * P, kx = kx0 + g;
+            const C*  row = in + (size_t)(kx * P + ky) * P;
+            double2   a[P];
 #pragma unroll
-            for (int kz = 0; kz < P; ++kz) a[kz] = row[kz];
+            for (int kz = 0; kz < P; ++kz)
+            {
+                const C c = row[kz];
+                a[kz] = make_double2(c.x, c.y);
+            }
             for (int z = 0; z < N; ++z)
             {
                 const int step = (N - 1 + z) % P;
-                double    sr = 0.0, si = 0.0;
+                double    sr = 0, si = 0;
                 int       m = 0;
 #pragma unroll
                 for (int kz = 0; kz < P; ++kz)
@@ -417,7 +456,7 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
                     m += step;
                     if (m >= P) m -= P;
                 }
-                U[(g * P + ky) * N + z] = make_double2(sr, si);
+                U[(g * P + ky) * N + z] = Cx<T>::make((T)sr, (T)si);
             }
         }
         __syncthreads();
@@ -427,11 +466,15 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
             const int g = l / N, z = l - g * N, kx = kx0 + g;
             double2   a[P];
 #pragma unroll
-            for (int ky = 0; ky < P; ++ky) a[ky] = U[(g * P + ky) * N + z];
+            for (int ky = 0; ky < P; ++ky)
+            {
+                const C c = U[(g * P + ky) * N + z];
+                a[ky] = make_double2(c.x, c.y);
+            }
             for (int y = 0; y < N; ++y)
             {
                 const int step = (N - 1 + y) % P;
-                double    sr = 0.0, si = 0.0;
+                double    sr = 0, si = 0;
                 int       m = 0;
 #pragma unroll
                 for (int ky = 0; ky < P; ++ky)
@@ -440,7 +483,7 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
                     m += step;
                     if (m >= P) m -= P;
                 }
-                V[(kx * N + z) * N + y] = make_double2(sr, si);
+                V[(kx * N + z) * N + y] = Cx<T>::make((T)sr, (T)si);
             }
         }
         __syncthreads();
@@ -451,11 +494,15 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
     {
         double2 a[H];
 #pragma unroll
-        for (int kx = 0; kx < H; ++kx) a[kx] = V[kx * N * N + l];
+        for (int kx = 0; kx < H; ++kx)
+        {
+            const C c = V[kx * N * N + l];
+            a[kx] = make_double2(c.x, c.y);
+        }
         for (int x = 0; x < N; ++x)
         {
             const int step = (N - 1 + x) % P;
-            double    s = 0.0;
+            double    s = 0;
             int       m = 0;
 #pragma unroll
             for (int kx = 0; kx < H; ++kx)
@@ -477,8 +524,22 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restr
 // for its frequencies sit in LDS.  Each thread keeps the 8 target
 // accumulators of one group at one frequency.
 // ---------------------------------------------------------------------------
-template<int KF>
-__device__ inline void mac_entry(const double2* Gs, int f, int meta, double2 S, double2 (&acc)[8])
+// Accumulators are FP64 on both paths.  On the mixed path the products are
+// FP32 and each is widened before the add: over the ~200 sources of a
+// target an FP32 running sum loses ~15x more than the spectra's own rounding.
+__device__ inline void cmul_acc(double2 g, double2 S, double2& acc)
+{
+    acc.x = fma(g.x, S.x, fma(-g.y, S.y, acc.x));
+    acc.y = fma(g.x, S.y, fma(g.y, S.x, acc.y));
+}
+__device__ inline void cmul_acc(float2 g, float2 S, double2& acc)
+{
+    acc.x += (double)fmaf(g.x, S.x, -g.y * S.y);
+    acc.y += (double)fmaf(g.x, S.y, g.y * S.x);
+}
+
+template<int KF, class C>
+__device__ inline void mac_entry(const C* Gs, int f, int meta, C S, double2 (&acc)[8])
 {
     const int r = meta & 255, tm = meta >> 8;
     const int rx = r % 6, ry = (r / 6) % 6, rz = r / 36;
@@ -488,33 +549,31 @@ __device__ inline void mac_entry(const double2* Gs, int f, int meta, double2 S, 
     {
         if (tm & (1 << t))
         {
-            const int     oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
-            const double2 gv = Gs[oi * KF + f];
-            acc[t].x = fma(gv.x, S.x, fma(-gv.y, S.y, acc[t].x));
-            acc[t].y = fma(gv.x, S.y, fma(gv.y, S.x, acc[t].y));
+            const int oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
+            cmul_acc(Gs[oi * KF + f], S, acc[t]);
         }
     }
 }
 
-// grid: x = frequency chunks of KF, y = chunks of kThreads/KF groups.
-template<int N, int KF>
+template<int N, int KF, class T>
 __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
-    const MacSrc* __restrict__ srcs, const double2* __restrict__ ghat, const double2* __restrict__ sspec,
-    double2* __restrict__ tspec)
+    const MacSrc* __restrict__ srcs, const typename Cx<T>::type* __restrict__ ghat,
+    const typename Cx<T>::type* __restrict__ sspec, typename Cx<T>::type* __restrict__ tspec)
 {
+    using C = typename Cx<T>::type;
     constexpr int NF = Geo<N>::NF;
     constexpr int GPW = kThreads / KF;
-    __shared__ double2 Gs[343 * KF];
+    __shared__ C Gs[343 * KF];
     const int f0 = blockIdx.x * KF;
     for (int q = threadIdx.x; q < 343 * KF; q += kThreads)
     {
         const int o = q / KF, f = q - o * KF;
-        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : make_double2(0.0, 0.0);
+        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
     }
     __syncthreads();
-    const int      f = threadIdx.x % KF;
-    const int      fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
-    const double2* sp = sspec + fg;
+    const int f = threadIdx.x % KF;
+    const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
+    const C*  sp = sspec + fg;
     // The workgroup walks the (Morton-ordered) groups in tiles, so a source
     // shared by neighbouring groups is re-read while its line is still in L2.
     for (int g = blockIdx.y * GPW + threadIdx.x / KF; g < ngroups; g += GPW * gridDim.y)
@@ -528,28 +587,34 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
         // eight source loads in flight before their use
         for (; e + 8 <= end; e += 8)
         {
-            MacSrc  s[8];
-            double2 S[8];
+            MacSrc s[8];
+            C      S[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) s[i] = srcs[e + i];
 #pragma unroll
             for (int i = 0; i < 8; ++i) S[i] = sp[(size_t)s[i].slot * NF];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) mac_entry<KF>(Gs, f, s[i].meta, S[i], acc);
+            for (int i = 0; i < 8; ++i) mac_entry<KF, C>(Gs, f, s[i].meta, S[i], acc);
         }
         for (; e < end; ++e)
         {
-            const MacSrc  s0 = srcs[e];
-            const double2 S0 = sp[(size_t)s0.slot * NF];
-            mac_entry<KF>(Gs, f, s0.meta, S0, acc);
+            const MacSrc s0 = srcs[e];
+            const C      S0 = sp[(size_t)s0.slot * NF];
+            mac_entry<KF, C>(Gs, f, s0.meta, S0, acc);
         }
         if (f0 + f < NF)
         {
 #pragma unroll
             for (int t = 0; t < 8; ++t)
-                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
+                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = Cx<T>::make((T)acc[t].x, (T)acc[t].y);
         }
     }
+}
+
+__global__ __launch_bounds__(kThreads) void k_to_float2(const double2* __restrict__ in, float2* __restrict__ out, long n)
+{
+    const long q = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (q < n) out[q] = make_float2((float)in[q].x, (float)in[q].y);
 }
 
 // ---------------------------------------------------------------------------
@@ -756,26 +821,47 @@ template<int N> struct RunStencil
 };
 template<int N> struct RunFwd
 {
-    static void run(hipStream_t s, const double* f, const int* slots, int count, double2* spec)
+    static void run(hipStream_t s, int prec, const double* f, const int* slots, int count, void* spec)
     {
-        hipLaunchKernelGGL(k_forward_dft<N>, dim3(count), dim3(kThreads), 0, s, f, slots, count, spec);
+        if (prec == 0)
+            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count), dim3(kThreads), 0, s, f, slots, count,
+                (double2*)spec);
+        else
+            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count), dim3(kThreads), 0, s, f, slots, count,
+                (float2*)spec);
     }
 };
 template<int N> struct RunInv
 {
-    static void run(hipStream_t s, const double2* spec, const int* slots, int count, double* f, double scale)
+    static void run(hipStream_t s, int prec, const void* spec, const int* slots, int count, double* f, double scale)
     {
-        hipLaunchKernelGGL(k_inverse_dft<N>, dim3(count), dim3(kThreads), 0, s, spec, slots, count, f, scale);
+        if (prec == 0)
+            hipLaunchKernelGGL((k_inverse_dft<N, double>), dim3(count), dim3(kThreads), 0, s, (const double2*)spec,
+                slots, count, f, scale);
+        else
+            hipLaunchKernelGGL((k_inverse_dft<N, float>), dim3(count), dim3(kThreads), 0, s, (const float2*)spec,
+                slots, count, f, scale);
     }
 };
 template<int N> struct RunMac
 {
-    static void run(hipStream_t s, const MacGroup* g, int ng, const MacSrc* src, const double2* ghat,
-        const double2* ss, double2* ts)
+    static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const void* ghat,
+        const void* ss, void* ts)
     {
-        constexpr int  KF = 8;
-        const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
-        hipLaunchKernelGGL((k_spectral_mac<N, KF>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src, ghat, ss, ts);
+        if (prec == 0)
+        {
+            constexpr int  KF = 8;
+            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
+            hipLaunchKernelGGL((k_spectral_mac<N, KF, double>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src,
+                (const double2*)ghat, (const double2*)ss, (double2*)ts);
+        }
+        else
+        {
+            constexpr int  KF = 16;
+            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
+            hipLaunchKernelGGL((k_spectral_mac<N, KF, float>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src,
+                (const float2*)ghat, (const float2*)ss, (float2*)ts);
+        }
     }
 };
 template<int N> struct RunLgf
@@ -830,19 +916,23 @@ void correction_stencil(hipStream_t s, int n, const double* t, double* c, const 
 {
     if (count > 0) dispatch_n<RunStencil>(n, s, t, c, slots, count, inv_dx2, omega2);
 }
-void forward_dft(hipStream_t s, int n, const double* field, const int* slots, int count, double2* spec)
+void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec)
 {
-    if (count > 0) dispatch_n<RunFwd>(n, s, field, slots, count, spec);
+    if (count > 0) dispatch_n<RunFwd>(n, s, prec, field, slots, count, spec);
 }
-void inverse_dft(hipStream_t s, int n, const double2* spec, const int* slots, int count, double* field,
+void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale)
 {
-    if (count > 0) dispatch_n<RunInv>(n, s, spec, slots, count, field, scale);
+    if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale);
 }
-void spectral_mac(hipStream_t s, int n, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const double2* ghat, const double2* sspec, double2* tspec)
+void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
+    const void* ghat, const void* sspec, void* tspec)
 {
-    if (ngroups > 0) dispatch_n<RunMac>(n, s, groups, ngroups, srcs, ghat, sspec, tspec);
+    if (ngroups > 0) dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, ghat, sspec, tspec);
+}
+void to_float2(hipStream_t s, const double2* in, float2* out, long n)
+{
+    if (n > 0) hipLaunchKernelGGL(k_to_float2, dim3(grid1(n)), dim3(kThreads), 0, s, in, out, n);
 }
 void lgf_spectra(hipStream_t s, int n, int kind, int d, const double* g, int nmax, const double* iv, int kmax,
     double* sample, double2* a, double2* b, double2* ghat)

@@ -52,11 +52,14 @@ void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int coun
 void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* corr, const int* slots,
     int count, double inv_dx2, double omega2);
 
-void forward_dft(hipStream_t s, int n, const double* field, const int* slots, int count, double2* spec);
-void inverse_dft(hipStream_t s, int n, const double2* spec, const int* slots, int count, double* field,
+// prec 0: spectra and DFT/MAC arithmetic in FP64 (double2);
+// prec 1: spectra and DFT/MAC arithmetic in FP32 (float2).  Fields are FP64.
+void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec);
+void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale);
-void spectral_mac(hipStream_t s, int n, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const double2* ghat, const double2* sspec, double2* tspec);
+void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
+    const void* ghat, const void* sspec, void* tspec);
+void to_float2(hipStream_t s, const double2* in, float2* out, long n);
 
 // Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:
 // block[a] = K(step*(-o*e - (n-1) + a)), a in [0,P)^3, step = 2^d, then r2c.

@@ -1,0 +1,282 @@
+// gfx950 kernels of the block operators and the immersed-boundary transfer.
+//
+// Reference operations (IBLGF-AMR):
+//   block_op      Operator::laplace / gradient / divergence / curl / curl_transpose,
+//                 include/iblgf/operators/operators.hpp:1387-1511, 2420-2453
+//   clean_faces   Operator::clean_leaf_correction_boundary (329-513) and
+//                 clean_ib_region_boundary (73-187): zero two planes per side
+//   ib_spread     Operator::ib_smearing (face data, 996-1015) over the points'
+//                 influence nodes, LinSysSolver::smearing (linsys.hpp:350-376)
+//   ib_interp     Operator::ib_projection (844-861), LinSysSolver::projection (417-445)
+//   dot / axpby   LinSysSolver::dot / add (linsys.hpp:500-531)
+//
+// All stencils are memory-bound (a handful of flops per 8-byte load): one
+// workgroup per block streams its interior with unit-stride x, neighbours
+// come from the same lines through L1/L2.  Contraction into FMA is off so
+// each result is the reference's sequence of roundings.
+#include "ops.hpp"
+
+#include <stdexcept>
+
+namespace iblgf_amd {
+namespace {
+
+constexpr int kThreads = 256;
+
+template<int N, int OP>
+__global__ __launch_bounds__(kThreads) void k_block_op(const double* __restrict__ s0, const double* __restrict__ s1,
+    const double* __restrict__ s2, double* __restrict__ d0, double* __restrict__ d1, double* __restrict__ d2,
+    const int* __restrict__ slots, int count, double fac)
+{
+#pragma clang fp contract(off)
+    constexpr int E = N - 2, NN = N * N, N3 = N * N * N, E3 = E * E * E;
+    const int     b = blockIdx.x;
+    if (b >= count) return;
+    const size_t off = (size_t)slots[b] * N3;
+    for (int t = threadIdx.x; t < E3; t += kThreads)
+    {
+        const int i = t % E + 1, j = (t / E) % E + 1, k = t / (E * E) + 1;
+        const size_t q = off + (size_t)(k * NN + j * N + i);
+        if constexpr (OP == OP_LAPLACE)
+        {
+            double v = -6.0 * s0[q];
+            v = v + s0[q - NN];
+            v = v + s0[q + NN];
+            v = v + s0[q - N];
+            v = v + s0[q + N];
+            v = v + s0[q - 1];
+            v = v + s0[q + 1];
+            d0[q] = v * fac;
+        }
+        else if constexpr (OP == OP_GRADIENT)
+        {
+            const double c = s0[q];
+            d0[q] = fac * (c - s0[q - 1]);
+            d1[q] = fac * (c - s0[q - N]);
+            d2[q] = fac * (c - s0[q - NN]);
+        }
+        else if constexpr (OP == OP_DIVERGENCE)
+        {
+            double v = -s0[q];
+            v = v - s1[q];
+            v = v - s2[q];
+            v = v + s0[q + 1];
+            v = v + s1[q + N];
+            v = v + s2[q + NN];
+            d0[q] = v * fac;
+        }
+        else if constexpr (OP == OP_CURL)
+        {
+            double a = s2[q] - s2[q - N];
+            a = a - s1[q];
+            a = a + s1[q - NN];
+            double c = s0[q] - s0[q - NN];
+            c = c - s2[q];
+            c = c + s2[q - 1];
+            double e = s1[q] - s1[q - 1];
+            e = e - s0[q];
+            e = e + s0[q - N];
+            d0[q] = a * fac;
+            d1[q] = c * fac;
+            d2[q] = e * fac;
+        }
+        else  // OP_CURL_TRANSPOSE
+        {
+            double a = s1[q] - s1[q + NN];
+            a = a + s2[q + N];
+            a = a - s2[q];
+            double c = s2[q] - s2[q + 1];
+            c = c + s0[q + NN];
+            c = c - s0[q];
+            double e = s0[q] - s0[q + N];
+            e = e + s1[q + 1];
+            e = e - s1[q];
+            d0[q] = a * fac;
+            d1[q] = c * fac;
+            d2[q] = e * fac;
+        }
+    }
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_clean_faces(double* f, const int* __restrict__ entries, int count,
+    int width)
+{
+    constexpr int N3 = N * N * N;
+    const int     b = blockIdx.x;
+    if (b >= count) return;
+    const int    mask = entries[2 * b + 1];
+    double*      blk = f + (size_t)entries[2 * b] * N3;
+    for (int q = threadIdx.x; q < N3; q += kThreads)
+    {
+        const int i = q % N, j = (q / N) % N, k = q / (N * N);
+        const bool z = ((mask & 1) && k < width) || ((mask & 32) && k >= N - width);
+        const bool y = ((mask & 2) && j < width) || ((mask & 16) && j >= N - width);
+        const bool x = ((mask & 4) && i < width) || ((mask & 8) && i >= N - width);
+        if (x || y || z) blk[q] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_ib_spread(const double* __restrict__ force,
+    const int* __restrict__ row_ptr, const int* __restrict__ row_dst, const int* __restrict__ col,
+    const double* __restrict__ w, int nrows, double* f0, double* f1, double* f2)
+{
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * kThreads + threadIdx.x;
+    if (r >= nrows) return;
+    double s = 0.0;
+    for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) s = s + force[col[k]] * w[k];
+    const int comp = row_dst[2 * r], q = row_dst[2 * r + 1];
+    double*   f = comp == 0 ? f0 : (comp == 1 ? f1 : f2);
+    f[q] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_ib_interp(const double* __restrict__ f0,
+    const double* __restrict__ f1, const double* __restrict__ f2, const int* __restrict__ row_ptr,
+    const int* __restrict__ node, const double* __restrict__ w, int nrows, double* __restrict__ force)
+{
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * kThreads + threadIdx.x;
+    if (r >= nrows) return;
+    const int     comp = r % 3;
+    const double* f = comp == 0 ? f0 : (comp == 1 ? f1 : f2);
+    double        s = 0.0;
+    for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) s = s + f[node[k]] * w[k];
+    force[r] = s;
+}
+
+__device__ inline double block_sum(double v, double* red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1)
+    {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_dot_partial(const double* __restrict__ a,
+    const double* __restrict__ b, const double* __restrict__ mask, long len, double* __restrict__ partials)
+{
+    __shared__ double red[kThreads];
+    double            v = 0.0;
+    for (long i = (long)blockIdx.x * kThreads + threadIdx.x; i < len; i += (long)gridDim.x * kThreads)
+        if (!mask || mask[i] != 0.0) v = fma(a[i], b[i], v);
+    const double s = block_sum(v, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_dot_final(const double* __restrict__ partials, int np,
+    double* __restrict__ out)
+{
+    __shared__ double red[kThreads];
+    double            v = 0.0;
+    for (int i = threadIdx.x; i < np; i += kThreads) v += partials[i];
+    const double s = block_sum(v, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ __launch_bounds__(kThreads) void k_axpby(long len, double a, const double* __restrict__ x, double b,
+    double* __restrict__ y, const double* __restrict__ mask)
+{
+#pragma clang fp contract(off)
+    const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (i < len && (!mask || mask[i] != 0.0)) y[i] = y[i] * b + x[i] * a;
+}
+
+template<template<int> class F, typename... Args>
+void dispatch_n(int n, Args... args)
+{
+    switch (n)
+    {
+        case 8: F<8>::run(args...); break;
+        case 10: F<10>::run(args...); break;
+        case 12: F<12>::run(args...); break;
+        case 16: F<16>::run(args...); break;
+        case 18: F<18>::run(args...); break;
+        default: throw std::runtime_error("unsupported block size n = " + std::to_string(n));
+    }
+}
+
+template<int N>
+struct RunOp
+{
+    static void run(hipStream_t s, int op, const double* const* src, double* const* dst, const int* slots, int count,
+        double fac)
+    {
+        const dim3 g(count), t(kThreads);
+#define IBLGF_OP(K)                                                                                           \
+    hipLaunchKernelGGL((k_block_op<N, K>), g, t, 0, s, src[0], src[1], src[2], dst[0], dst[1], dst[2], slots, \
+        count, fac)
+        switch (op)
+        {
+            case OP_LAPLACE: IBLGF_OP(OP_LAPLACE); break;
+            case OP_GRADIENT: IBLGF_OP(OP_GRADIENT); break;
+            case OP_DIVERGENCE: IBLGF_OP(OP_DIVERGENCE); break;
+            case OP_CURL: IBLGF_OP(OP_CURL); break;
+            case OP_CURL_TRANSPOSE: IBLGF_OP(OP_CURL_TRANSPOSE); break;
+            default: throw std::runtime_error("unknown block operator");
+        }
+#undef IBLGF_OP
+    }
+};
+
+template<int N>
+struct RunClean
+{
+    static void run(hipStream_t s, double* f, const int* entries, int count, int width)
+    {
+        hipLaunchKernelGGL((k_clean_faces<N>), dim3(count), dim3(kThreads), 0, s, f, entries, count, width);
+    }
+};
+
+}  // namespace
+
+namespace launch {
+
+void block_op(hipStream_t s, int n, int op, const double* const src[3], double* const dst[3], const int* slots,
+    int count, double fac)
+{
+    if (count > 0) dispatch_n<RunOp>(n, s, op, src, dst, slots, count, fac);
+}
+
+void clean_faces(hipStream_t s, int n, double* f, const int* entries, int count, int width)
+{
+    if (width < 0 || width > n) throw std::runtime_error("clean width out of range");
+    if (count > 0) dispatch_n<RunClean>(n, s, f, entries, count, width);
+}
+
+void ib_spread(hipStream_t s, const double* force, const int* row_ptr, const int* row_dst, const int* col,
+    const double* w, int nrows, double* const fields[3])
+{
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_ib_spread, dim3((nrows + kThreads - 1) / kThreads), dim3(kThreads), 0, s, force, row_ptr,
+        row_dst, col, w, nrows, fields[0], fields[1], fields[2]);
+}
+
+void ib_interp(hipStream_t s, const double* const fields[3], const int* row_ptr, const int* node, const double* w,
+    int nrows, double* force)
+{
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_ib_interp, dim3((nrows + kThreads - 1) / kThreads), dim3(kThreads), 0, s, fields[0],
+        fields[1], fields[2], row_ptr, node, w, nrows, force);
+}
+
+void dot(hipStream_t s, const double* a, const double* b, const double* mask, long len, double* partials,
+    double* out)
+{
+    hipLaunchKernelGGL(k_dot_partial, dim3(kDotBlocks), dim3(kThreads), 0, s, a, b, mask, len, partials);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kThreads), 0, s, partials, kDotBlocks, out);
+}
+
+void axpby(hipStream_t s, long len, double a, const double* x, double b, double* y, const double* mask)
+{
+    if (len <= 0) return;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((len + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, len, a, x,
+        b, y, mask);
+}
+
+}  // namespace launch
+}  // namespace iblgf_amd

@@ -7,7 +7,10 @@ The classes mirror the reference's interfaces for this path:
   include/iblgf/fmm/fmm.hpp:392-680);
 * :class:`PoissonSolver` -- ``PoissonSolver::apply_lgf`` / ``apply_lgf_IF``
   (include/iblgf/solver/poisson/poisson.hpp:130-135, 427-494, 587-894) on one
-  GPU with the tree resident.
+  GPU with the tree resident, and the block operators of ``domain::Operator``
+  (include/iblgf/operators/operators.hpp);
+* :class:`IbSystem` -- ``LinSysSolver`` smearing / projection, ``ET_H_S_E``
+  and ``CG_solve`` (include/iblgf/solver/linsys/linsys.hpp:170-531).
 
 Everything calls the C ABI declared in ``include/iblgf_gpu.h``.  There is no
 CPU fallback: constructing a :class:`PoissonSolver` without the built library
@@ -24,10 +27,22 @@ LIB_PATH = os.path.join(_HERE, "libiblgf_gpu.so")
 KERNEL_LAPLACE = 0
 KERNEL_IF = 1
 
+PREC_FP64 = 0
+PREC_MIXED = 1
+
 FLAG_LEAF = 1
 FLAG_CORRECTION = 2
 FLAG_LEAF_BOUNDARY = 4
 FLAG_PHYSICAL = 8
+FLAG_IB = 16
+FLAG_EXTENDED_IB = 32
+
+OP_LAPLACE, OP_GRADIENT, OP_DIVERGENCE, OP_CURL, OP_CURL_TRANSPOSE = range(5)
+CLEAN_LEAF_CORRECTION = 0
+CLEAN_IB_REGION = 1
+MASK_IB2xIB = 2
+MASK_xIB2IB = 3
+DELTA_YANG3, DELTA_YANG4, DELTA_ROMA = range(3)
 
 _lib = None
 
@@ -71,12 +86,27 @@ SYMBOLS = {
     "iblgf_gpu_field_download": (_I, [_P, _I, _DP]),
     "iblgf_gpu_apply_lgf": (_I, [_P, _I, _I, _I, _D]),
     "iblgf_gpu_prepare": (_I, [_P, _I, _D]),
+    "iblgf_gpu_set_precision": (_I, [_P, _I]),
     "iblgf_gpu_sync": (_I, [_P]),
     "iblgf_gpu_stream": (_I, [_P, ctypes.POINTER(_P)]),
     "iblgf_gpu_counts": (_I, [_P, ctypes.POINTER(ctypes.c_long)]),
     "iblgf_gpu_timing_enable": (_I, [_P, _I]),
     "iblgf_gpu_timing_read": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
     "iblgf_gpu_timing_reset": (_I, [_P]),
+    "iblgf_tree_set_ib": (_I, [_P, _DP, _I, _I, _I, _D]),
+    "iblgf_tree_ib_stencil": (_I, [_P, ctypes.POINTER(ctypes.c_long)] + [_P] * 8),
+    "iblgf_gpu_block_op": (_I, [_P, _I, _IP, _IP, _I, _D]),
+    "iblgf_gpu_buffer_exchange": (_I, [_P, _I, _I]),
+    "iblgf_gpu_level_divergence": (_I, [_P, _IP, _I, _I]),
+    "iblgf_gpu_level_gradient": (_I, [_P, _I, _IP, _I]),
+    "iblgf_gpu_clean_boundary": (_I, [_P, _I, _I, _I, _I, _I]),
+    "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
+    "iblgf_gpu_ib_create": (_I, [_P, _P]),
+    "iblgf_gpu_ib_info": (_I, [_P, _IP, _IP, ctypes.POINTER(ctypes.c_long)]),
+    "iblgf_gpu_ib_smear": (_I, [_P, _DP, _IP]),
+    "iblgf_gpu_ib_project": (_I, [_P, _IP, _DP]),
+    "iblgf_gpu_ib_apply": (_I, [_P, _DP, _DP, _D]),
+    "iblgf_gpu_ib_cg": (_I, [_P, _DP, _DP, _D, _D, _I, _IP, _DP, _DP]),
 }
 
 KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
@@ -162,13 +192,38 @@ class Tree:
             _lib.iblgf_tree_destroy(h)
             self._h = None
 
+    def set_ib(self, points, ib_level, delta=DELTA_YANG3, safety_dis=4.0):
+        """Immersed-boundary points (N, 3) at refinement level ib_level (IB::init)."""
+        x = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        _check(lib().iblgf_tree_set_ib(self._h, x.ctypes.data_as(_DP), x.shape[0], int(ib_level), int(delta),
+                                       float(safety_dis)))
+        self.ib_points = x.copy()
+        return self
+
+    def ib_stencil(self):
+        """Host-built delta weights (the device's): dict of numpy arrays."""
+        sz = (ctypes.c_long * 4)()
+        _check(lib().iblgf_tree_ib_stencil(self._h, sz, *([None] * 8)))
+        ni, nr, ns, npnt = sz[0], sz[1], sz[2], sz[3]
+        d = dict(interp_ptr=np.zeros(3 * npnt + 1, np.int32), interp_node=np.zeros(ni, np.int32),
+                 interp_w=np.zeros(ni), spread_ptr=np.zeros(nr + 1, np.int32), spread_dst=np.zeros((nr, 2), np.int32),
+                 spread_col=np.zeros(ns, np.int32), spread_w=np.zeros(ns), owned=np.zeros(npnt, np.int32))
+        _check(lib().iblgf_tree_ib_stencil(self._h, sz, *[_ptr(d[k]) for k in (
+            "interp_ptr", "interp_node", "interp_w", "spread_ptr", "spread_dst", "spread_col", "spread_w", "owned")]))
+        return d
+
     def init_refine(self, nref, cond=None, level_up_max=0):
-        """Domain::init_refine.  cond(block_base(3,), extent, refinement_level, diff_level) -> bool."""
+        """Domain::init_refine.  cond(block_base(3,), extent, refinement_level, diff_level) -> bool
+        (or 2 to refine without the 2:1 check).  Blocks near immersed-boundary points set with
+        set_ib() are refined first, as the reference does."""
         err = []
 
         def _cb(user, base, ext, rl, diff):
             try:
-                return 1 if cond((base[0], base[1], base[2]), ext, rl, diff) else 0
+                if cond is None:
+                    return 0
+                r = cond((base[0], base[1], base[2]), ext, rl, diff)
+                return 2 if (r is not True and r == 2) else (1 if r else 0)
             except Exception as e:  # noqa: BLE001 - surfaced after the call
                 err.append(e)
                 return 0
@@ -253,6 +308,10 @@ class PoissonSolver:
         _check(lib().iblgf_gpu_field_download(self._h, int(fid), a.ctypes.data_as(_DP)), self._h)
         return a
 
+    def set_precision(self, precision):
+        """PREC_FP64 (plain) or PREC_MIXED (FP32 spectra/DFT/MAC, FP64 fields and level sums)."""
+        _check(lib().iblgf_gpu_set_precision(self._h, int(precision)), self._h)
+
     def prepare(self, kernel=KERNEL_LAPLACE, alpha_base=0.0):
         _check(lib().iblgf_gpu_prepare(self._h, int(kernel), float(alpha_base)), self._h)
 
@@ -290,6 +349,79 @@ class PoissonSolver:
         wk = (ctypes.c_long * 5)()
         _check(lib().iblgf_gpu_timing_read(self._h, ms, ln, wk), self._h)
         return {k: (ms[i], ln[i], wk[i]) for i, k in enumerate(KERNEL_CLASSES)}
+
+    # -- block operators (domain::Operator) ---------------------------------
+    def block_op(self, op, src, dst, level, scale=1.0):
+        """dst = op(src) on the interior of the level's blocks (asynchronous); src/dst are field ids
+        (an int for cell data, a 3-sequence for face/edge data)."""
+        s = [src] if np.isscalar(src) else list(src)
+        d = [dst] if np.isscalar(dst) else list(dst)
+        _check(lib().iblgf_gpu_block_op(self._h, int(op), _ip(s), _ip(d), int(level), float(scale)), self._h)
+
+    def buffer_exchange(self, field, level):
+        _check(lib().iblgf_gpu_buffer_exchange(self._h, int(field), int(level)), self._h)
+
+    def level_divergence(self, src3, dst, level):
+        _check(lib().iblgf_gpu_level_divergence(self._h, _ip(src3), int(dst), int(level)), self._h)
+
+    def level_gradient(self, src, dst3, level):
+        _check(lib().iblgf_gpu_level_gradient(self._h, int(src), _ip(dst3), int(level)), self._h)
+
+    def clean_boundary(self, field, level, kind, leaf_only=True, width=2):
+        _check(lib().iblgf_gpu_clean_boundary(self._h, int(field), int(level), int(kind), 1 if leaf_only else 0,
+                                              int(width)), self._h)
+
+    def apply_lgf_masked(self, src, dst, mask_type, kernel=KERNEL_LAPLACE, alpha_base=0.0):
+        """apply_lgf / apply_lgf_IF with fmm_type IB2xIB or xIB2IB (finest level, one pass)."""
+        _check(lib().iblgf_gpu_apply_lgf_masked(self._h, int(src), int(dst), int(kernel), float(alpha_base),
+                                                int(mask_type)), self._h)
+
+
+class IbSystem:
+    """Immersed-boundary force system on the device (LinSysSolver on one GPU).
+
+    Forces are (N, 3) FP64 host arrays; the points and delta kernel come from
+    the tree (Tree.set_ib), which must be the one the solver uploaded."""
+
+    def __init__(self, solver):
+        self.solver = solver
+        _check(lib().iblgf_gpu_ib_create(solver._h, solver.tree._h), solver._h)
+        npts, nown, nnz = ctypes.c_int(), ctypes.c_int(), ctypes.c_long()
+        _check(lib().iblgf_gpu_ib_info(solver._h, ctypes.byref(npts), ctypes.byref(nown), ctypes.byref(nnz)),
+               solver._h)
+        self.npoints, self.nowned, self.nnz = npts.value, nown.value, nnz.value
+
+    def _vec(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+        if a.size != 3 * self.npoints:
+            raise ValueError(f"force has {a.size} values, expected {3 * self.npoints}")
+        return a
+
+    def smear(self, force, fields3):
+        _check(lib().iblgf_gpu_ib_smear(self.solver._h, self._vec(force).ctypes.data_as(_DP), _ip(fields3)),
+               self.solver._h)
+
+    def project(self, fields3):
+        out = np.zeros((self.npoints, 3), np.float64)
+        _check(lib().iblgf_gpu_ib_project(self.solver._h, _ip(fields3), out.ctypes.data_as(_DP)), self.solver._h)
+        return out
+
+    def apply(self, fin, alpha):
+        """A fin with A = E (I - G L^-1 D) H_alpha E^T (ET_H_S_E)."""
+        out = np.zeros((self.npoints, 3), np.float64)
+        _check(lib().iblgf_gpu_ib_apply(self.solver._h, self._vec(fin).ctypes.data_as(_DP),
+                                        out.ctypes.data_as(_DP), float(alpha)), self.solver._h)
+        return out
+
+    def cg(self, uc, alpha, tol=1e-8, maxit=200, f0=None):
+        """CG_solve; returns (f, iterations, residual, history)."""
+        f = np.zeros(3 * self.npoints) if f0 is None else self._vec(f0).copy()
+        it, res = ctypes.c_int(), ctypes.c_double()
+        hist = np.zeros(max(int(maxit), 1))
+        _check(lib().iblgf_gpu_ib_cg(self.solver._h, self._vec(uc).ctypes.data_as(_DP), f.ctypes.data_as(_DP),
+                                     float(alpha), float(tol), int(maxit), ctypes.byref(it), ctypes.byref(res),
+                                     hist.ctypes.data_as(_DP)), self.solver._h)
+        return f.reshape(-1, 3), it.value, res.value, hist[: it.value].copy()
 
 
 def lgf_get(x, y, z):

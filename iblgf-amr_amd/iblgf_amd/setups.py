@@ -218,3 +218,63 @@ class VortexRings:
             w = self.vorticity(P)[..., component]
             f[s][inner] = w[inner]
         return f
+
+
+# ---------------------------------------------------------------------------
+# Immersed-boundary sphere
+# ---------------------------------------------------------------------------
+def fibonacci_sphere(npoints, R=0.5):
+    """IB::read_points, geometry "sphere" (include/iblgf/domain/ib.hpp:179-200)."""
+    lam = (1.0 + math.sqrt(5)) / 2.0
+    x = np.arange(npoints) + 0.5
+    phi = np.arccos(1.0 - 2 * x / npoints)
+    theta = 2 * math.pi * x / lam
+    return np.stack([R * np.cos(theta) * np.sin(phi), R * np.sin(theta) * np.sin(phi), R * np.cos(phi)], axis=1)
+
+
+class IbSphere:
+    """Sphere of radius R with N seeded quasi-uniform surface points in a
+    (nref+1)-level adaptive mesh refined around it.
+
+    N ~ U{5000..20000} (first draw of SplitMix64(seed)) unless given; the
+    point spacing is sqrt(pi / (0.866 N)) as the reference's Fibonacci sphere
+    (ib.hpp:185), the finest cell is spacing / ibph, and every point gets a
+    jitter U(-jitter, jitter) * spacing per coordinate (draws in point order,
+    x, y, z).  The base mesh is a cube of 2*half_width, rounded up to an even
+    number (at least 4) of base blocks a side.  Blocks whose grown region holds a point are
+    refined (Domain::init_refine, domain.hpp:887-891); the safety distance of
+    the extended IB region is 5 / (Re dx_ib) + 1 (ib.hpp:82).  The right-hand
+    side is the free stream u = (1, 0, 0) at every point (LinSysSolver::ib_solve
+    with a uniform boundary velocity, linsys.hpp:112-129).
+    """
+
+    def __init__(self, seed=1004, npoints=None, block_extent=16, nref=2, ibph=1.5, half_width=1.0, R=0.5,
+                 jitter=0.1, delta=2, Re=1000.0):
+        from . import DELTA_ROMA  # noqa: F401  (delta codes: 0 yang3, 1 yang4, 2 roma)
+
+        rng = SplitMix64(seed)
+        if npoints is None:
+            npoints = 5000 + min(int(rng.uniform() * 15001), 15000)
+        self.npoints = int(npoints)
+        self.spacing = math.sqrt(math.pi / (0.86602540378 * self.npoints))
+        self.dx_ib = self.spacing / ibph
+        self.dx_base = self.dx_ib * 2 ** nref
+        e = int(block_extent)
+        nb = int(math.ceil(2 * half_width / self.dx_base / e))
+        nb = max(nb + nb % 2, 4)  # at least 4 base blocks a side: the influence lists need no wrap-around
+        ext = nb * e
+        base = -ext // 2
+        pts = fibonacci_sphere(self.npoints, R)
+        jit = np.array([[rng.uniform(-jitter, jitter) for _ in range(3)] for _ in range(self.npoints)])
+        self.points = pts + jit * self.spacing
+        self.block_extent = e
+        self.nref = int(nref)
+        self.delta = int(delta)
+        self.safety_dis = 5.0 / (Re * self.dx_ib) + 1.0
+        self.tree = Tree([base] * 3, [ext] * 3, e, self.dx_base, [[base, base, base, ext, ext, ext]])
+        self.tree.set_ib(self.points, self.nref, self.delta, self.safety_dis)
+        self.tree.init_refine(self.nref, None)
+        self.tree.build_masks()
+        self.info = self.tree.info()
+        self.arrays = self.tree.export()
+        self.uc = np.tile(np.array([1.0, 0.0, 0.0]), (self.npoints, 1))

@@ -38,10 +38,13 @@ enum iblgf_status
 typedef struct iblgf_tree    iblgf_tree;
 typedef struct iblgf_gpu_ctx iblgf_gpu_ctx;
 
-/* Refinement condition: return nonzero to refine the block.  block_base is
- * the block's lower corner in cells of its own refinement level.
- * Stands in for Domain::register_refinement_condition
- * (include/iblgf/domain/domain.hpp:890-900, tests/poisson/vortexrings.hpp:471-520). */
+/* Refinement condition: return 1 to refine the block where the 2:1 balance
+ * allows it (Tree::try_2to1), 2 to refine it unconditionally (what
+ * Domain::init_refine does for blocks that overlap an immersed-boundary
+ * point, domain.hpp:887-891), 0 to keep it.  block_base is the block's lower
+ * corner in cells of its own refinement level.  Stands in for
+ * Domain::register_refinement_condition (include/iblgf/domain/domain.hpp:890-900,
+ * tests/poisson/vortexrings.hpp:471-520). */
 typedef int (*iblgf_refine_fn)(void* user, const int block_base[3], int block_extent, int refinement_level,
     int diff_level);
 
@@ -66,15 +69,44 @@ int iblgf_tree_import(const int base_shift[3], int block_extent, int base_level,
 int iblgf_tree_init_refine(iblgf_tree* t, int nref, int level_up_max, iblgf_refine_fn fn, void* user);
 
 /* FmmMaskBuilder::fmm_lgf_mask_build (include/iblgf/fmm/fmm.hpp:392-400),
- * and the IB2xIB / xIB2IB masks (fmm.hpp:70-144). */
+ * and the IB2xIB / xIB2IB masks (fmm.hpp:70-144).  With immersed-boundary
+ * points set, the is_ib / extended_ib flags are computed first
+ * (Server::update_ib_flag, include/iblgf/domain/decomposition/server.hpp:544-575). */
 int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf);
 
 int iblgf_tree_info(const iblgf_tree* t, int* nblocks, int* base_level, int* depth, int* block_extent,
     double* dx_base, int* nleaves);
 
+/* Immersed-boundary points (physical coordinates, npoints x 3) of the tree,
+ * at refinement level ib_level (the finest level the tree will have), with
+ * the regularised delta kernel delta (IBLGF_DELTA_*) and the safety distance
+ * of the extended IB region: IB::init / read_points
+ * (include/iblgf/domain/ib.hpp:60-121, 179-200).  Set before init_refine:
+ * refinement then refines every block whose grown region holds a point. */
+enum iblgf_delta
+{
+    IBLGF_DELTA_YANG3 = 0, /* reference default */
+    IBLGF_DELTA_YANG4 = 1,
+    IBLGF_DELTA_ROMA = 2   /* 3-point delta, support 1.5 */
+};
+int iblgf_tree_set_ib(iblgf_tree* t, const double* points, int npoints, int ib_level, int delta, double safety_dis);
+
+/* The sparse delta weights the device uses (host-side, for checking): the
+ * influence nodes of Client::update_ib_rank_and_infl
+ * (include/iblgf/domain/decomposition/client.hpp:1264-1320) with nonzero
+ * weight delta(dist + face offset) (operators.hpp:844-861, 996-1015).
+ * sizes4 receives {interp nnz, spread rows, spread nnz, npoints}; every
+ * other pointer may be NULL (call once with NULLs for the sizes).
+ * interp: rows 3*i+c, row_ptr[3N+1], node index slot*n^3 + q, weight.
+ * spread: row_ptr[rows+1], dst {comp, index} per row, col 3*i+c, weight.
+ * owned[N]: 1 if a finest leaf strictly contains the point. */
+int iblgf_tree_ib_stencil(const iblgf_tree* t, long* sizes4, int* interp_ptr, int* interp_node, double* interp_w,
+    int* spread_ptr, int* spread_dst, int* spread_col, double* spread_w, int* owned);
+
 /* Flat block table in slot order; any pointer may be NULL.  Array sizes:
  * key[nb], level[nb], coord[3nb], bbase[3nb], flags[nb] (bit0 leaf, bit1
- * correction, bit2 leaf_boundary, bit3 physical), parent[nb], children[8nb],
+ * correction, bit2 leaf_boundary, bit3 physical, bit4 is_ib, bit5
+ * extended_ib), parent[nb], children[8nb],
  * neighbor[27nb], influence[189nb], influence_num[nb], mask_src[nb],
  * mask_tgt[nb].  Octant::neighbor/influence (include/iblgf/domain/octree/octant.hpp:461-467),
  * Tree::construct_lists (include/iblgf/domain/octree/tree.hpp:559-603). */
@@ -139,6 +171,17 @@ enum iblgf_kernel
  * leaf blocks only; IF writes every block with data.  Asynchronous. */
 int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base);
 
+enum iblgf_precision
+{
+    IBLGF_PREC_FP64 = 0,  /* plain path: spectra, DFT legs and MAC in FP64 */
+    IBLGF_PREC_MIXED = 1  /* spectra, DFT legs and MAC in FP32; kernel spectra generated in
+                             FP64 and stored FP32; fields and cross-level transfers FP64 */
+};
+
+/* Arithmetic of the subsequent solves (no reference counterpart: the
+ * reference computes in FP64 throughout, include/iblgf/types.hpp). */
+int iblgf_gpu_set_precision(iblgf_gpu_ctx* ctx, int precision);
+
 /* Builds the kernel spectra a solve will use (first-use work of
  * LGF_Base::dft, include/iblgf/lgf/lgf.hpp:77-98), so timed solves are warm. */
 int iblgf_gpu_prepare(iblgf_gpu_ctx* ctx, int kernel, double alpha_base);
@@ -162,6 +205,77 @@ int iblgf_gpu_timing_reset(iblgf_gpu_ctx* ctx);
  * transforms, target transforms, (source, target) pairs, parent-child
  * transfers, leaves. */
 int iblgf_gpu_counts(iblgf_gpu_ctx* ctx, long* counts6);
+
+/* ---- block operators (device) ------------------------------------------- */
+
+enum iblgf_block_op
+{
+    IBLGF_OP_LAPLACE = 0,        /* Operator::laplace, include/iblgf/operators/operators.hpp:1387-1417 */
+    IBLGF_OP_GRADIENT = 1,       /* Operator::gradient, cell -> face, operators.hpp:1425-1444 */
+    IBLGF_OP_DIVERGENCE = 2,     /* Operator::divergence, face -> cell, operators.hpp:1453-1474 */
+    IBLGF_OP_CURL = 3,           /* Operator::curl, face -> edge, operators.hpp:1483-1511 */
+    IBLGF_OP_CURL_TRANSPOSE = 4  /* Operator::curl_transpose, edge -> face, operators.hpp:2420-2453 */
+};
+
+/* dst = op(src) on the interior of every block with data at one tree level,
+ * dx = dx_base / 2^(level - base_level); 1 field for cell data, 3 (x, y, z)
+ * for face and edge data; scale is curl_transpose's factor.  dst must not
+ * alias src.  Asynchronous. */
+int iblgf_gpu_block_op(iblgf_gpu_ctx* ctx, int op, const int* src_fields, const int* dst_fields, int level,
+    double scale);
+
+/* Halo fill of one level: buffer cells from the same-level neighbours'
+ * interiors (Client::buffer_exchange, include/iblgf/domain/decomposition/client.hpp). */
+int iblgf_gpu_buffer_exchange(iblgf_gpu_ctx* ctx, int field, int level);
+
+/* Operator::levelDivergence (face -> cell, with halo fill and leaf/correction
+ * boundary clean) and Operator::levelGradient (cell -> face, then halo fill)
+ * (operators.hpp:522-538, 588-604). */
+int iblgf_gpu_level_divergence(iblgf_gpu_ctx* ctx, const int src_fields[3], int dst_field, int level);
+int iblgf_gpu_level_gradient(iblgf_gpu_ctx* ctx, int src_field, const int dst_fields[3], int level);
+
+/* Zero width planes on block sides: kind 0 Operator::clean_leaf_correction_boundary
+ * (operators.hpp:329-513; with leaf_only, correction blocks are zeroed whole and
+ * count as missing neighbours; sides are cleaned at the base level only),
+ * kind 1 Operator::clean_ib_region_boundary (operators.hpp:73-187; sides whose
+ * neighbour is missing or not IB). */
+int iblgf_gpu_clean_boundary(iblgf_gpu_ctx* ctx, int field, int level, int kind, int leaf_only, int width);
+
+/* PoissonSolver::apply_lgf / apply_lgf_IF with an immersed-boundary FMM type
+ * (poisson.hpp:587-894, 435-494): mask_type IBLGF_MASK_IB2xIB or
+ * IBLGF_MASK_xIB2IB; one FMM pass at the finest level from the interior of
+ * the finest blocks.  Laplace writes the finest leaves, IF every finest
+ * block.  src may equal dst. */
+enum iblgf_mask_type
+{
+    IBLGF_MASK_IB2xIB = 2,
+    IBLGF_MASK_xIB2IB = 3
+};
+int iblgf_gpu_apply_lgf_masked(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
+    int mask_type);
+
+/* ---- immersed-boundary force system (device) ---------------------------- */
+
+/* Sparse delta weights of the tree's immersed-boundary points on the finest
+ * leaf blocks of the uploaded tree (Client::update_ib_rank_and_infl,
+ * include/iblgf/domain/decomposition/client.hpp:1264-1320).  The tree must be
+ * the one uploaded, with points set. */
+int iblgf_gpu_ib_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t);
+/* npoints, points owned (strictly inside a finest leaf), nonzero weights. */
+int iblgf_gpu_ib_info(iblgf_gpu_ctx* ctx, int* npoints, int* nowned, long* nnz);
+/* LinSysSolver::smearing (face data; fields cleaned first) and projection
+ * (include/iblgf/solver/linsys/linsys.hpp:350-376, 417-445); forces are host
+ * arrays npoints x 3. */
+int iblgf_gpu_ib_smear(iblgf_gpu_ctx* ctx, const double* force, const int fields[3]);
+int iblgf_gpu_ib_project(iblgf_gpu_ctx* ctx, const int fields[3], double* force);
+/* fout = E (I - G L^-1 D) H_alpha E^T fin: LinSysSolver::ET_H_S_E (linsys.hpp:259-319). */
+int iblgf_gpu_ib_apply(iblgf_gpu_ctx* ctx, const double* fin, double* fout, double alpha);
+/* LinSysSolver::CG_solve (linsys.hpp:170-230): f holds the initial guess and
+ * receives the solution; stops when sqrt(r.r / f.f) < tol or after maxit
+ * iterations.  history (maxit entries, may be NULL) receives the residual
+ * per iteration. */
+int iblgf_gpu_ib_cg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
+    int* iterations, double* residual, double* history);
 
 #ifdef __cplusplus
 }

@@ -766,12 +766,11 @@ static long influence_field(ostate* o, int tg, int L, int idx, int neighbor_only
     return count;
 }
 
-/* Fmm::apply (fmm.hpp:729-870) */
-static void fmm_apply(ostate* o, const double* Source, double* Target, int L, int nl, double add_scale)
+/* Fmm::apply (fmm.hpp:729-870) with fmm mask index idx */
+static void fmm_apply_idx(ostate* o, const double* Source, double* Target, int L, int idx, double add_scale)
 {
     const oracle_tree* t = o->t;
     const int          rl = L - t->base_level;
-    const int          idx = 15 + 2 * rl + nl;
     const int          neighbor_only = o->ker.is_if;
     const double       dx_level = t->dx_base / pow(2.0, rl);
     int                l, s, c, i, j, k;
@@ -847,6 +846,12 @@ static void fmm_apply(ostate* o, const double* Source, double* Target, int L, in
             size_t q;
             for (q = 0; q < o->n3; ++q) BLK(Target, s)[q] += BLK(o->fmm_t, s)[q] * add_scale;
         }
+}
+
+/* AMR2AMR mask index 15 + 2 rl + nl (octant.hpp:94-131) */
+static void fmm_apply(ostate* o, const double* Source, double* Target, int L, int nl, double add_scale)
+{
+    fmm_apply_idx(o, Source, Target, L, 15 + 2 * (L - o->t->base_level) + nl, add_scale);
 }
 
 /* halo exchange at one level: buffer cells <- neighbour interiors */
@@ -1048,4 +1053,419 @@ long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const dou
     if (seconds) *seconds = now_seconds() - t0;
     ostate_free(o);
     return pairs;
+}
+
+/* ======================================================================== */
+/* Immersed-boundary solves, block operators, IB transfer, force CG          */
+/* ======================================================================== */
+
+#define IS_IB(s) (o->t->flags[s] & 16u)
+
+/* apply_lgf with fmm_type IB2xIB / xIB2IB (poisson.hpp:587-894): copy_level of
+ * the finest level's interior, no coarsening, one pass at the finest level,
+ * copy of the leaves' target_tmp. */
+int oracle_apply_lgf_masked(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
+    int mask_type, int faithful, int intrp_order)
+{
+    ostate o_;
+    ostate* o = &o_;
+    const int L = t->depth - 1;
+    int       s, i, j, k, n = t->n;
+    ostate_init(o, t, faithful, intrp_order);
+    o->ker.is_if = 0;
+    o->ker.g = g;
+    o->ker.nmax = nmax;
+    for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
+        for (k = 1; k < n - 1; ++k)
+            for (j = 1; j < n - 1; ++j)
+                for (i = 1; i < n - 1; ++i)
+                {
+                    size_t q = ((size_t)k * n + j) * n + i;
+                    BLK(o->source_tmp, s)[q] = BLK(src, s)[q];
+                }
+    fmm_apply_idx(o, o->source_tmp, o->target_tmp, L, mask_type == 2 ? 1 : 2, 1.0);
+    for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
+        if (IS_LEAF(s)) memcpy(BLK(dst, s), BLK(o->target_tmp, s), sizeof(double) * o->n3);
+    ostate_free(o);
+    return 0;
+}
+
+/* apply_if with fmm_type IB2xIB / xIB2IB (poisson.hpp:435-494): one
+ * neighbour-only pass at the finest level, copy_level with buffers. */
+int oracle_apply_if_masked(const oracle_tree* t, const double* src, double* dst, double alpha_base, int mask_type,
+    int faithful, int intrp_order)
+{
+    ostate o_;
+    ostate* o = &o_;
+    const int L = t->depth - 1;
+    int       s, i, j, k, n = t->n;
+    ostate_init(o, t, faithful, intrp_order);
+    o->ker.is_if = 1;
+    o->ker.alpha = alpha_base * pow(4.0, L - t->base_level);
+    for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
+        for (k = 1; k < n - 1; ++k)
+            for (j = 1; j < n - 1; ++j)
+                for (i = 1; i < n - 1; ++i)
+                {
+                    size_t q = ((size_t)k * n + j) * n + i;
+                    BLK(o->source_tmp, s)[q] = BLK(src, s)[q];
+                }
+    fmm_apply_idx(o, o->source_tmp, o->target_tmp, L, mask_type == 2 ? 1 : 2, 1.0);
+    for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
+        memcpy(BLK(dst, s), BLK(o->target_tmp, s), sizeof(double) * o->n3);
+    ostate_free(o);
+    return 0;
+}
+
+static int level_first(const oracle_tree* t, int l)
+{
+    int s = 0;
+    while (s < t->nb && t->level[s] < l) ++s;
+    return s;
+}
+
+/* Operator::laplace / gradient / divergence / curl / curl_transpose
+ * (operators.hpp:1387-1511, 2420-2453) on the interior nodes of every block
+ * at level l.  Fields are [nb][n^3], x fastest; vector fields 3 arrays. */
+void oracle_block_op(const oracle_tree* t, int op, const double* s0, const double* s1, const double* s2, double* d0,
+    double* d1, double* d2, int l, double scale)
+{
+    const int    n = t->n;
+    const size_t n3 = (size_t)n * n * n, X = 1, Y = n, Z = (size_t)n * n;
+    const double dx = t->dx_base / pow(2.0, l - t->base_level);
+    int          b, i, j, k;
+    for (b = level_first(t, l); b < t->nb && t->level[b] == l; ++b)
+        for (k = 1; k < n - 1; ++k)
+            for (j = 1; j < n - 1; ++j)
+                for (i = 1; i < n - 1; ++i)
+                {
+                    const size_t q = (size_t)b * n3 + (k * Z + j * Y + i * X);
+                    if (op == 0)
+                    {
+                        const double fac = 1.0 / (dx * dx);
+                        d0[q] = -6.0 * s0[q] + s0[q - Z] + s0[q + Z] + s0[q - Y] + s0[q + Y] + s0[q - X] + s0[q + X];
+                        d0[q] *= fac;
+                    }
+                    else if (op == 1)
+                    {
+                        const double fac = 1.0 / dx;
+                        d0[q] = fac * (s0[q] - s0[q - X]);
+                        d1[q] = fac * (s0[q] - s0[q - Y]);
+                        d2[q] = fac * (s0[q] - s0[q - Z]);
+                    }
+                    else if (op == 2)
+                    {
+                        const double fac = 1.0 / dx;
+                        d0[q] = -s0[q] - s1[q] - s2[q] + s0[q + X] + s1[q + Y] + s2[q + Z];
+                        d0[q] *= fac;
+                    }
+                    else if (op == 3)
+                    {
+                        const double fac = 1.0 / dx;
+                        d0[q] = s2[q] - s2[q - Y] - s1[q] + s1[q - Z];
+                        d0[q] *= fac;
+                        d1[q] = s0[q] - s0[q - Z] - s2[q] + s2[q - X];
+                        d1[q] *= fac;
+                        d2[q] = s1[q] - s1[q - X] - s0[q] + s0[q - Y];
+                        d2[q] *= fac;
+                    }
+                    else
+                    {
+                        const double fac = 1.0 / dx * scale;
+                        d0[q] = +s1[q] - s1[q + Z] + s2[q + Y] - s2[q];
+                        d0[q] *= fac;
+                        d1[q] = +s2[q] - s2[q + X] + s0[q + Z] - s0[q];
+                        d1[q] *= fac;
+                        d2[q] = +s0[q] - s0[q + Y] + s1[q + X] - s1[q];
+                        d2[q] *= fac;
+                    }
+                }
+}
+
+void oracle_buffer_exchange(const oracle_tree* t, double* f, int l)
+{
+    const int    n = t->n, e = n - 2;
+    const size_t n3 = (size_t)n * n * n;
+    int          s, i, j, k;
+    for (s = level_first(t, l); s < t->nb && t->level[s] == l; ++s)
+        for (k = 0; k < n; ++k)
+            for (j = 0; j < n; ++j)
+                for (i = 0; i < n; ++i)
+                {
+                    int dx = (i == 0) ? -1 : (i == n - 1 ? 1 : 0);
+                    int dy = (j == 0) ? -1 : (j == n - 1 ? 1 : 0);
+                    int dz = (k == 0) ? -1 : (k == n - 1 ? 1 : 0);
+                    int nb;
+                    if (!dx && !dy && !dz) continue;
+                    nb = t->neighbor[27 * s + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+                    if (nb < 0) continue;
+                    f[s * n3 + ((size_t)k * n + j) * n + i] =
+                        f[nb * n3 + ((size_t)(k - dz * e) * n + (j - dy * e)) * n + (i - dx * e)];
+                }
+}
+
+/* zero `width` planes on the side of face-neighbour slot i (4,10,12,14,16,22) */
+static void zero_side(double* blk, int n, int slot, int width)
+{
+    int i, j, k;
+    for (k = 0; k < n; ++k)
+        for (j = 0; j < n; ++j)
+            for (i = 0; i < n; ++i)
+            {
+                int hit = (slot == 4 && k < width) || (slot == 22 && k >= n - width) ||
+                          (slot == 10 && j < width) || (slot == 16 && j >= n - width) ||
+                          (slot == 12 && i < width) || (slot == 14 && i >= n - width);
+                if (hit) blk[((size_t)k * n + j) * n + i] = 0.0;
+            }
+}
+
+/* kind 0: Operator::clean_leaf_correction_boundary (operators.hpp:329-513);
+ * kind 1: Operator::clean_ib_region_boundary (73-187). */
+void oracle_clean_boundary(const oracle_tree* t, double* f, int l, int kind, int leaf_only, int width)
+{
+    static const int faces[6] = {4, 10, 12, 14, 16, 22};
+    const int        n = t->n;
+    const size_t     n3 = (size_t)n * n * n;
+    int              s, b;
+    if (kind == 0)
+    {
+        if (leaf_only)
+            for (s = level_first(t, l); s < t->nb && t->level[s] == l; ++s)
+                if (t->flags[s] & 2u) memset(f + s * n3, 0, n3 * sizeof(double));
+        if (l != t->base_level) return;
+    }
+    for (s = level_first(t, l); s < t->nb && t->level[s] == l; ++s)
+        for (b = 0; b < 6; ++b)
+        {
+            const int nb = t->neighbor[27 * s + faces[b]];
+            int       clean;
+            if (kind == 0) clean = nb < 0 || (leaf_only && (t->flags[nb] & 2u));
+            else
+                clean = nb < 0 || !(t->flags[nb] & 16u);
+            if (clean) zero_side(f + s * n3, n, faces[b], width);
+        }
+}
+
+/* ---- immersed boundary ------------------------------------------------- */
+
+static double ddf1(int kind, double x)
+{
+    const double r = fabs(x), r2 = r * r, s3 = sqrt(3);
+    if (kind == 2)
+    {
+        if (r > 1.5) return 0;
+        if (r <= 0.5) return (1.0 + sqrt(-3. * r2 + 1.0)) / 3.0;
+        return (5.0 - 3.0 * r - sqrt(1.0 - 3.0 * (1 - r) * (1 - r))) / 6.0;
+    }
+    if (kind == 0)
+    {
+        if (r > 2) return 0;
+        if (r <= 1.0)
+            return 17.0 / 48.0 + s3 * O_PI / 108.0 + r / 4.0 - r2 / 4.0 +
+                   (1.0 - 2.0 * r) / 16 * sqrt(-12.0 * r2 + 12.0 * r + 1.0) - s3 / 12.0 * asin(s3 / 2.0 * (2.0 * r - 1.0));
+        return 55.0 / 48.0 - s3 * O_PI / 108.0 - 13.0 * r / 12.0 + r2 / 4.0 +
+               (2.0 * r - 3.0) / 48.0 * sqrt(-12.0 * r2 + 36.0 * r - 23.0) + s3 / 36.0 * asin(s3 / 2.0 * (2 * r - 3.0));
+    }
+    if (r > 2.5) return 0;
+    if (r <= 0.5) return 3.0 / 8 + O_PI / 32.0 - r2 / 4;
+    if (r <= 1.5) return 1.0 / 4 + (1.0 - r) / 8.0 * sqrt(-2.0 + 8 * r - 4 * r2) - 1.0 / 8 * asin(sqrt(2) * (r - 1));
+    /* the reference's 1/16 factor of the arcsine is an integer quotient (0) */
+    return 17.0 / 16 - O_PI / 64.0 - 3.0 / 4 * r + r2 / 8 + (r - 2.0) / 16.0 * sqrt(-14.0 + 16 * r - 4 * r2);
+}
+
+double oracle_delta_1d(int kind, double x) { return ddf1(kind, x); }
+
+/* IB::ib_block_overlap (ib.hpp:648-678) */
+static int ib_overlap(const oracle_tree* t, const oracle_ib* ib, int i, int s, int radius_level)
+{
+    const int    rl = t->level[s] - t->base_level;
+    const int    f = 1 << (ib->ib_level - rl);
+    const int    e = t->n - 2;
+    double       added = 0;
+    int          d;
+    const double factor = pow(2, ib->ib_level) / t->dx_base;
+    if (radius_level == 2) added += ib->ddf_radius + ib->safety_dis + 10.0;
+    else if (radius_level == 1)
+        added += ib->ddf_radius + 5.0;
+    for (d = 0; d < 3; ++d)
+    {
+        int base = t->bbase[3 * s + d] * f, ext = e * f;
+        ext += (int)(2 * added);
+        base -= (int)added;
+        if ((base + ext - 1 + 1) < ib->x[3 * i + d] * factor || base >= ib->x[3 * i + d] * factor) return 0;
+    }
+    return 1;
+}
+
+/* The reference's loop over points, their influence blocks (finest leaves
+ * with data that overlap with radius 1, client.hpp:1264-1320) and the block
+ * nodes within ddf_radius+1; mode 0 spreads (operators.hpp:996-1015), mode 1
+ * interpolates (operators.hpp:844-861). */
+static void ib_transfer(const oracle_tree* t, const oracle_ib* ib, int mode, const double* fin, double** fields,
+    double* fout)
+{
+    const int    L = t->depth - 1, n = t->n, e = n - 2;
+    const size_t n3 = (size_t)n * n * n;
+    int          i, s, a, b, c, comp, d;
+    if (mode == 1) memset(fout, 0, 3 * (size_t)ib->npoints * sizeof(double));
+    for (i = 0; i < ib->npoints; ++i)
+        for (s = level_first(t, L); s < t->nb && t->level[s] == L; ++s)
+        {
+            double p, X[3];
+            if (!(t->flags[s] & 1u) || !ib_overlap(t, ib, i, s, 1)) continue;
+            p = pow(2, t->level[s] - t->base_level);
+            for (d = 0; d < 3; ++d) X[d] = ib->x[3 * i + d] * p / t->dx_base;
+            for (c = 0; c < e; ++c)
+                for (b = 0; b < e; ++b)
+                    for (a = 0; a < e; ++a)
+                    {
+                        const int nc[3] = {t->bbase[3 * s] + a, t->bbase[3 * s + 1] + b, t->bbase[3 * s + 2] + c};
+                        double    dist[3];
+                        int       inside = 1;
+                        size_t    q = s * n3 + (((size_t)c + 1) * n + (b + 1)) * n + (a + 1);
+                        for (d = 0; d < 3; ++d)
+                        {
+                            dist[d] = nc[d] - X[d];
+                            if (fabs(dist[d]) >= ib->ddf_radius + 1.0) inside = 0;
+                        }
+                        if (!inside) continue;
+                        for (comp = 0; comp < 3; ++comp)
+                        {
+                            double w, v[3];
+                            for (d = 0; d < 3; ++d) v[d] = dist[d] + (d == comp ? 0.0 : 0.5);
+                            w = ddf1(ib->delta, v[0]) * ddf1(ib->delta, v[1]) * ddf1(ib->delta, v[2]);
+                            if (mode == 0) fields[comp][q] += fin[3 * i + comp] * w * 1.0;
+                            else
+                                fout[3 * i + comp] += fields[comp][q] * w;
+                        }
+                    }
+        }
+}
+
+void oracle_ib_smear(const oracle_tree* t, const oracle_ib* ib, const double* force, double* f0, double* f1, double* f2)
+{
+    double*      fl[3] = {f0, f1, f2};
+    const size_t sz = (size_t)t->nb * t->n * t->n * t->n;
+    int          c;
+    for (c = 0; c < 3; ++c) memset(fl[c], 0, sz * sizeof(double));
+    ib_transfer(t, ib, 0, force, fl, NULL);
+}
+
+void oracle_ib_project(const oracle_tree* t, const oracle_ib* ib, const double* f0, const double* f1,
+    const double* f2, double* force)
+{
+    double* fl[3] = {(double*)f0, (double*)f1, (double*)f2};
+    ib_transfer(t, ib, 1, NULL, fl, force);
+}
+
+/* points strictly inside a finest leaf block (ib rank != -1) */
+void oracle_ib_owned(const oracle_tree* t, const oracle_ib* ib, int* owned)
+{
+    const int L = t->depth - 1;
+    int       i, s;
+    for (i = 0; i < ib->npoints; ++i)
+    {
+        owned[i] = 0;
+        for (s = level_first(t, L); s < t->nb && t->level[s] == L; ++s)
+            if ((t->flags[s] & 1u) && ib_overlap(t, ib, i, s, 1) && ib_overlap(t, ib, i, s, 0))
+            {
+                owned[i] = 1;
+                break;
+            }
+    }
+}
+
+/* LinSysSolver::ET_H_S_E (linsys.hpp:259-319) */
+int oracle_ib_apply(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, const double* fin,
+    double* fout, double alpha)
+{
+    const size_t sz = (size_t)t->nb * t->n * t->n * t->n;
+    const int    L = t->depth - 1;
+    double *     F[3], *A[3], *cell = (double*)calloc(sz, sizeof(double));
+    size_t       q;
+    int          c;
+    for (c = 0; c < 3; ++c)
+    {
+        F[c] = (double*)calloc(sz, sizeof(double));
+        A[c] = (double*)calloc(sz, sizeof(double));
+    }
+    oracle_ib_smear(t, ib, fin, F[0], F[1], F[2]);
+    if (fabs(alpha) > 1e-14)
+        for (c = 0; c < 3; ++c) oracle_apply_if_masked(t, F[c], F[c], alpha, 2, 0, 5);
+    for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, F[c], L);
+    oracle_block_op(t, 2, F[0], F[1], F[2], cell, NULL, NULL, L, 1.0);
+    oracle_clean_boundary(t, cell, L, 0, 1, 2);
+    oracle_clean_boundary(t, cell, L, 1, 0, 2);
+    oracle_apply_lgf_masked(t, g, nmax, cell, cell, 3, 0, 5);
+    oracle_block_op(t, 1, cell, NULL, NULL, A[0], A[1], A[2], L, 1.0);
+    for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, A[c], L);
+    for (c = 0; c < 3; ++c) oracle_clean_boundary(t, A[c], L, 1, 0, 2);
+    for (c = 0; c < 3; ++c)
+        for (q = 0; q < sz; ++q) F[c][q] += A[c][q] * -1.0;
+    oracle_ib_project(t, ib, F[0], F[1], F[2], fout);
+    for (c = 0; c < 3; ++c)
+    {
+        free(F[c]);
+        free(A[c]);
+    }
+    free(cell);
+    return 0;
+}
+
+/* LinSysSolver::CG_solve / dot / add (linsys.hpp:170-230, 500-531) */
+int oracle_ib_cg(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, const double* uc, double* f,
+    double alpha, double tol, int maxit, double* residual)
+{
+    const int N = ib->npoints;
+    double *  Ax = (double*)calloc(3 * N, sizeof(double)), *r = (double*)calloc(3 * N, sizeof(double));
+    double *  p = (double*)calloc(3 * N, sizeof(double)), *Ap = (double*)calloc(3 * N, sizeof(double));
+    int*      own = (int*)calloc(N, sizeof(int));
+    int       i, d, k, iters = 0;
+    double    rsold = 0, res = 0;
+    oracle_ib_owned(t, ib, own);
+    oracle_ib_apply(t, ib, g, nmax, f, Ax, alpha);
+    for (i = 0; i < N; ++i)
+        for (d = 0; d < 3; ++d) r[3 * i + d] = own[i] ? uc[3 * i + d] - Ax[3 * i + d] : 0.0;
+    memcpy(p, r, 3 * N * sizeof(double));
+    for (i = 0; i < N; ++i)
+        if (own[i])
+            for (d = 0; d < 3; ++d) rsold += r[3 * i + d] * r[3 * i + d];
+    for (k = 0; k < maxit; ++k)
+    {
+        double pAp = 0, a, rsnew = 0, f2 = 0;
+        oracle_ib_apply(t, ib, g, nmax, p, Ap, alpha);
+        for (i = 0; i < N; ++i)
+            if (own[i])
+                for (d = 0; d < 3; ++d) pAp += p[3 * i + d] * Ap[3 * i + d];
+        if (pAp == 0.0) break;
+        a = rsold / pAp;
+        for (i = 0; i < N; ++i)
+            if (own[i])
+                for (d = 0; d < 3; ++d)
+                {
+                    f[3 * i + d] = f[3 * i + d] * 1.0 + p[3 * i + d] * a;
+                    r[3 * i + d] = r[3 * i + d] * 1.0 + Ap[3 * i + d] * -a;
+                }
+        for (i = 0; i < N; ++i)
+            if (own[i])
+                for (d = 0; d < 3; ++d)
+                {
+                    rsnew += r[3 * i + d] * r[3 * i + d];
+                    f2 += f[3 * i + d] * f[3 * i + d];
+                }
+        iters = k + 1;
+        res = sqrt(rsnew / f2);
+        if (res < tol) break;
+        for (i = 0; i < N; ++i)
+            if (own[i])
+                for (d = 0; d < 3; ++d) p[3 * i + d] = p[3 * i + d] * (rsnew / rsold) + r[3 * i + d] * 1.0;
+        rsold = rsnew;
+    }
+    if (residual) *residual = res;
+    free(Ax);
+    free(r);
+    free(p);
+    free(Ap);
+    free(own);
+    return iters;
 }

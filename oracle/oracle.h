@@ -28,7 +28,7 @@ typedef struct oracle_tree
     const int*      level;       /* [nb] */
     const int*      coord;       /* [3nb] key coordinates */
     const int*      bbase;       /* [3nb] block base, cells of its level */
-    const uint32_t* flags;       /* [nb] bit0 leaf, bit1 correction */
+    const uint32_t* flags;       /* [nb] bit0 leaf, bit1 correction, bit4 is_ib, bit5 extended_ib */
     const int*      parent;      /* [nb] slot or -1 */
     const int*      children;    /* [8nb] slot or -1 */
     const int*      neighbor;    /* [27nb] slot or -1 */
@@ -82,6 +82,45 @@ int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double
 long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const double* fmm_s,
     int base_level_fmm, int nl, int ntargets, const int* targets, int faithful, double* fmm_t,
     double* seconds);
+
+/* ---- immersed boundary, block operators -------------------------------- */
+
+typedef struct oracle_ib
+{
+    int           npoints;
+    const double* x;          /* [3 npoints] physical coordinates */
+    int           ib_level;   /* refinement level of the points */
+    int           delta;      /* 0 yang3, 1 yang4, 2 roma (ib.hpp:686-738) */
+    double        ddf_radius; /* 4.5 (ib.hpp:80) */
+    double        safety_dis;
+} oracle_ib;
+
+/* apply_lgf / apply_if with fmm_type IB2xIB (2) or xIB2IB (3), poisson.hpp:587-894, 435-494. */
+int oracle_apply_lgf_masked(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
+    int mask_type, int faithful, int intrp_order);
+int oracle_apply_if_masked(const oracle_tree* t, const double* src, double* dst, double alpha_base, int mask_type,
+    int faithful, int intrp_order);
+/* op 0 laplace, 1 gradient, 2 divergence, 3 curl, 4 curl_transpose (operators.hpp:1387-1511, 2420-2453);
+ * unused component pointers may be NULL. */
+void oracle_block_op(const oracle_tree* t, int op, const double* s0, const double* s1, const double* s2, double* d0,
+    double* d1, double* d2, int level, double scale);
+void oracle_buffer_exchange(const oracle_tree* t, double* f, int level);
+/* kind 0 clean_leaf_correction_boundary, 1 clean_ib_region_boundary (operators.hpp:329-513, 73-187) */
+void oracle_clean_boundary(const oracle_tree* t, double* f, int level, int kind, int leaf_only, int width);
+double oracle_delta_1d(int kind, double x);
+/* LinSysSolver::smearing / projection (linsys.hpp:350-445) with Operator::ib_smearing /
+ * ib_projection (operators.hpp:996-1015, 844-861) over the influence lists of
+ * Client::update_ib_rank_and_infl (client.hpp:1264-1320). */
+void oracle_ib_smear(const oracle_tree* t, const oracle_ib* ib, const double* force, double* f0, double* f1,
+    double* f2);
+void oracle_ib_project(const oracle_tree* t, const oracle_ib* ib, const double* f0, const double* f1,
+    const double* f2, double* force);
+void oracle_ib_owned(const oracle_tree* t, const oracle_ib* ib, int* owned);
+/* LinSysSolver::ET_H_S_E and CG_solve (linsys.hpp:170-319); returns the iterations. */
+int oracle_ib_apply(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, const double* fin,
+    double* fout, double alpha);
+int oracle_ib_cg(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, const double* uc, double* f,
+    double alpha, double tol, int maxit, double* residual);
 
 #ifdef __cplusplus
 }

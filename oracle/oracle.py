@@ -19,6 +19,11 @@ _I = ctypes.c_int
 _P = ctypes.c_void_p
 
 
+class OracleIb(ctypes.Structure):
+    _fields_ = [("npoints", _I), ("x", _P), ("ib_level", _I), ("delta", _I), ("ddf_radius", _D),
+                ("safety_dis", _D)]
+
+
 class OracleTree(ctypes.Structure):
     _fields_ = [
         ("nb", _I), ("n", _I), ("base_level", _I), ("depth", _I), ("dx_base", _D),
@@ -51,6 +56,19 @@ def lib():
         L.oracle_apply_if.argtypes = [ctypes.POINTER(OracleTree), _P, _P, _D, _I, _I]
         L.oracle_bx_sample.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _I, _I, _I, _P, _I, _P, _P]
         L.oracle_bx_sample.restype = ctypes.c_long
+        T, B = ctypes.POINTER(OracleTree), ctypes.POINTER(OracleIb)
+        L.oracle_apply_lgf_masked.argtypes = [T, _P, _I, _P, _P, _I, _I, _I]
+        L.oracle_apply_if_masked.argtypes = [T, _P, _P, _D, _I, _I, _I]
+        L.oracle_block_op.argtypes = [T, _I, _P, _P, _P, _P, _P, _P, _I, _D]
+        L.oracle_buffer_exchange.argtypes = [T, _P, _I]
+        L.oracle_clean_boundary.argtypes = [T, _P, _I, _I, _I, _I]
+        L.oracle_delta_1d.argtypes = [_I, _D]
+        L.oracle_delta_1d.restype = _D
+        L.oracle_ib_smear.argtypes = [T, B, _P, _P, _P, _P]
+        L.oracle_ib_project.argtypes = [T, B, _P, _P, _P, _P]
+        L.oracle_ib_owned.argtypes = [T, B, _P]
+        L.oracle_ib_apply.argtypes = [T, B, _P, _I, _P, _P, _D]
+        L.oracle_ib_cg.argtypes = [T, B, _P, _I, _P, _P, _D, _D, _I, _P]
         _lib = L
     return _lib
 
@@ -146,3 +164,83 @@ class Tree:
         pairs = lib().oracle_bx_sample(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(fs), int(L), int(nl),
                                        len(tg), _p(tg), 1 if faithful else 0, _p(out), _p(sec))
         return pairs, float(sec[0]), out
+
+    # -- block operators and immersed boundary --------------------------------
+    def apply_lgf_masked(self, src, mask_type, nmax=100):
+        s = np.ascontiguousarray(src, np.float64)
+        dst = np.zeros_like(s)
+        lib().oracle_apply_lgf_masked(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(s), _p(dst),
+                                      int(mask_type), 0, 5)
+        return dst
+
+    def apply_if_masked(self, src, alpha_base, mask_type):
+        s = np.ascontiguousarray(src, np.float64)
+        dst = np.zeros_like(s)
+        lib().oracle_apply_if_masked(ctypes.byref(self.s), _p(s), _p(dst), float(alpha_base), int(mask_type), 0, 5)
+        return dst
+
+    def block_op(self, op, src, level, scale=1.0, dst=None):
+        """src: one field or a list of 3; returns the output field(s) (zeros outside the level's interiors
+        unless dst is given)."""
+        srcs = [np.ascontiguousarray(x, np.float64) for x in (src if isinstance(src, (list, tuple)) else [src])]
+        nd = 1 if op in (0, 2) else 3
+        outs = [np.zeros_like(srcs[0]) for _ in range(nd)] if dst is None else dst
+        sp = [_p(x) for x in srcs] + [None] * (3 - len(srcs))
+        dp = [_p(x) for x in outs] + [None] * (3 - len(outs))
+        lib().oracle_block_op(ctypes.byref(self.s), int(op), *sp, *dp, int(level), float(scale))
+        return outs[0] if nd == 1 else outs
+
+    def buffer_exchange(self, f, level):
+        lib().oracle_buffer_exchange(ctypes.byref(self.s), _p(f), int(level))
+        return f
+
+    def clean_boundary(self, f, level, kind, leaf_only=True, width=2):
+        lib().oracle_clean_boundary(ctypes.byref(self.s), _p(f), int(level), int(kind), 1 if leaf_only else 0,
+                                    int(width))
+        return f
+
+
+class Ib:
+    """Immersed-boundary points for the oracle (IB::init semantics)."""
+
+    def __init__(self, points, ib_level, delta=0, ddf_radius=4.5, safety_dis=4.0):
+        self.x = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+        self.s = OracleIb(self.x.shape[0], _p(self.x), int(ib_level), int(delta), float(ddf_radius),
+                          float(safety_dis))
+
+    def smear(self, tree, force):
+        f = np.ascontiguousarray(force, np.float64).reshape(-1)
+        shape = (tree.info["nblocks"], tree.n, tree.n, tree.n)
+        out = [np.zeros(shape) for _ in range(3)]
+        lib().oracle_ib_smear(ctypes.byref(tree.s), ctypes.byref(self.s), _p(f), *[_p(o) for o in out])
+        return out
+
+    def project(self, tree, fields):
+        fl = [np.ascontiguousarray(x, np.float64) for x in fields]
+        out = np.zeros((self.x.shape[0], 3))
+        lib().oracle_ib_project(ctypes.byref(tree.s), ctypes.byref(self.s), *[_p(x) for x in fl], _p(out))
+        return out
+
+    def owned(self, tree):
+        o = np.zeros(self.x.shape[0], np.int32)
+        lib().oracle_ib_owned(ctypes.byref(tree.s), ctypes.byref(self.s), _p(o))
+        return o
+
+    def apply(self, tree, fin, alpha, nmax=100):
+        f = np.ascontiguousarray(fin, np.float64).reshape(-1)
+        out = np.zeros((self.x.shape[0], 3))
+        lib().oracle_ib_apply(ctypes.byref(tree.s), ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(f), _p(out),
+                              float(alpha))
+        return out
+
+    def cg(self, tree, uc, alpha, tol, maxit, f0=None, nmax=100):
+        u = np.ascontiguousarray(uc, np.float64).reshape(-1)
+        f = np.zeros_like(u) if f0 is None else np.ascontiguousarray(f0, np.float64).reshape(-1).copy()
+        res = np.zeros(1)
+        it = lib().oracle_ib_cg(ctypes.byref(tree.s), ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(u), _p(f),
+                                float(alpha), float(tol), int(maxit), _p(res))
+        return f.reshape(-1, 3), it, float(res[0])
+
+
+def delta_1d(kind, x):
+    return lib().oracle_delta_1d(int(kind), float(x))

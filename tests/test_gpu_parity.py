@@ -113,6 +113,34 @@ def test_integrating_factor_matches_oracle(oracle_mod, alpha):
     assert l2 < REL_L2 and mx < REL_MAX
 
 
+@pytest.mark.parametrize("case", ["uniform", "rings"])
+def test_mixed_precision_error(oracle_mod, case):
+    """Mixed path (FP32 spectra, DFT legs and MAC; FP64 fields and level sums)
+    against the FP64 oracle: relative L2 <= 1e-6 per field component."""
+    from iblgf_amd import PREC_MIXED
+
+    print(f"case: mixed precision, {case}", flush=True)
+    if case == "uniform":
+        g = setups.UniformGaussians(seed=1000)
+        srcs, tree, arrays, info = [g.source()], g.tree, g.arrays, g.info
+    else:
+        v = _small_rings()
+        srcs, tree, arrays, info = [v.source(c) for c in range(3)], v.tree, v.arrays, v.info
+    solver = PoissonSolver(tree)
+    solver.set_precision(PREC_MIXED)
+    fs, fd = solver.field(), solver.field()
+    ot = oracle_mod.Tree(arrays, info, tree.n)
+    leaves = _leaf_mask(arrays)
+    for c, src in enumerate(srcs):
+        solver.upload(fs, src)
+        solver.apply_lgf(fs, fd)
+        got = solver.download(fd)
+        ref = ot.apply_lgf(src)
+        l2, mx = _rel(got[leaves], ref[leaves])
+        print(f"component {c}: relL2={l2:.3e} relmax={mx:.3e}")
+        assert l2 < 1e-6
+
+
 def test_full_size_vortex_rings_properties():
     """Size-independent properties at the full adaptive vortex-ring size
     (4290 blocks): linearity, repeatability and agreement with the oracle's
