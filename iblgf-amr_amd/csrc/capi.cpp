@@ -8,6 +8,7 @@
 #include "host/octree.hpp"
 #include "host/tables.hpp"
 #include "ib_solver.hpp"
+#include "learned_precond.hpp"
 #include "ops.hpp"
 
 #include <memory>
@@ -29,6 +30,9 @@ struct iblgf_gpu_ctx
 {
     std::unique_ptr<GpuSolver> solver;
     std::unique_ptr<IbSystem>  ib;
+    std::unique_ptr<LearnedPrecond> pc;
+    int                        collect_stride = 0;
+    CgStats                    last;
     DevArray<double>           ib_in, ib_out;
     std::string                error;
 };
@@ -332,6 +336,7 @@ int iblgf_gpu_destroy(iblgf_gpu_ctx* ctx)
 {
     if (!ctx) return fail(IBLGF_ERR_ARG, "iblgf_gpu_destroy: null context");
     const int rc = guarded(ctx, [&] {
+        ctx->pc.reset();
         ctx->ib.reset();
         ctx->ib_in.release();
         ctx->ib_out.release();
@@ -352,7 +357,8 @@ int iblgf_gpu_tree_upload(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
     if (!t) return fail(IBLGF_ERR_ARG, "iblgf_gpu_tree_upload: null tree", ctx);
     if (!t->masks_built) return fail(IBLGF_ERR_STATE, "iblgf_gpu_tree_upload: masks not built", ctx);
     return guarded(ctx, [&] {
-        ctx->ib.reset();  // its fields and weights belong to the previous tree
+        ctx->pc.reset();  // its fields and weights belong to the previous tree
+        ctx->ib.reset();
         ctx->solver->upload_tree(t->tree);
     });
 }
@@ -540,6 +546,7 @@ int iblgf_gpu_ib_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
     NEED_CTX("iblgf_gpu_ib_create");
     if (!t || !t->has_ib) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_create: tree without immersed-boundary points", ctx);
     return guarded(ctx, [&] {
+        ctx->pc.reset();
         ctx->ib.reset();
         ctx->ib = std::make_unique<IbSystem>(*ctx->solver, t->ib);
         ctx->ib_in.alloc(3 * static_cast<size_t>(t->ib.size()));
@@ -614,22 +621,162 @@ int iblgf_gpu_ib_apply(iblgf_gpu_ctx* ctx, const double* fin, double* fout, doub
     });
 }
 
+namespace {
+int ib_solve(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit, int use_mlp,
+    bool flexible, int* iterations, double* residual, double* history)
+{
+    return guarded(ctx, [&] {
+        if (use_mlp && !ctx->pc) throw std::runtime_error("no learned preconditioner");
+        DevArray<double> d_uc;
+        d_uc.alloc(ctx->ib_in.n);
+        ib_to_device(ctx, d_uc, uc);
+        ib_to_device(ctx, ctx->ib_out, f);
+        const bool collect = ctx->pc && ctx->collect_stride > 0;
+        if (collect) ctx->pc->begin_collect(ctx->collect_stride);
+        try
+        {
+            if (!flexible) ctx->last = ctx->ib->cg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit);
+            else if (use_mlp)
+            {
+                LearnedPrecond* pc = ctx->pc.get();
+                ctx->last = ctx->ib->fcg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit,
+                    [pc](hipStream_t s, const double* r, double* z) { pc->apply(s, r, z); });
+            }
+            else
+            {
+                const size_t n = ctx->ib_in.n;
+                ctx->last = ctx->ib->fcg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit,
+                    [n](hipStream_t s, const double* r, double* z) {
+                        const hipError_t e = hipMemcpyAsync(z, r, n * sizeof(double), hipMemcpyDeviceToDevice, s);
+                        if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e));
+                    });
+            }
+        }
+        catch (...)
+        {
+            if (collect) ctx->ib->on_iteration = nullptr;
+            throw;
+        }
+        if (collect) ctx->pc->end_collect(ctx->ib_out.p);
+        ib_to_host(ctx, f, ctx->ib_out);
+        if (iterations) *iterations = ctx->last.iterations;
+        if (residual) *residual = ctx->last.residual;
+        if (history)
+            for (size_t k = 0; k < ctx->last.history.size(); ++k) history[k] = ctx->last.history[k];
+    });
+}
+}  // namespace
+
 int iblgf_gpu_ib_cg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
     int* iterations, double* residual, double* history)
 {
     NEED_IB("iblgf_gpu_ib_cg");
     if (!uc || !f || maxit < 0 || !(tol > 0.0)) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_cg: invalid argument", ctx);
+    return ib_solve(ctx, uc, f, alpha, tol, maxit, 0, false, iterations, residual, history);
+}
+
+int iblgf_gpu_ib_fcg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
+    int use_mlp, int* iterations, double* residual, double* history)
+{
+    NEED_IB("iblgf_gpu_ib_fcg");
+    if (!uc || !f || maxit < 0 || !(tol > 0.0)) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_fcg: invalid argument", ctx);
+    return ib_solve(ctx, uc, f, alpha, tol, maxit, use_mlp, true, iterations, residual, history);
+}
+
+int iblgf_gpu_ib_stats(iblgf_gpu_ctx* ctx, double* seconds, int* applies, int* fell_back, int* stop)
+{
+    NEED_IB("iblgf_gpu_ib_stats");
+    if (seconds) *seconds = ctx->last.seconds;
+    if (applies) *applies = ctx->last.applies;
+    if (fell_back) *fell_back = ctx->last.fell_back ? 1 : 0;
+    if (stop) *stop = ctx->last.stop == "converged" ? 0 : (ctx->last.stop == "iteration limit" ? 1 : 2);
+    return IBLGF_OK;
+}
+
+int iblgf_gpu_ib_precond_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t, int K, int hidden, uint64_t seed)
+{
+    NEED_IB("iblgf_gpu_ib_precond_create");
+    if (!t || !t->has_ib) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_precond_create: tree without points", ctx);
     return guarded(ctx, [&] {
-        DevArray<double> d_uc;
-        d_uc.alloc(ctx->ib_in.n);
-        ib_to_device(ctx, d_uc, uc);
-        ib_to_device(ctx, ctx->ib_out, f);
-        const CgStats st = ctx->ib->cg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit);
-        ib_to_host(ctx, f, ctx->ib_out);
-        if (iterations) *iterations = st.iterations;
-        if (residual) *residual = st.residual;
-        if (history)
-            for (size_t k = 0; k < st.history.size(); ++k) history[k] = st.history[k];
+        ctx->pc.reset();
+        ctx->collect_stride = 0;
+        ctx->pc = std::make_unique<LearnedPrecond>(*ctx->ib, t->ib, K, hidden, seed);
+    });
+}
+
+#define NEED_PC(name)                                                                              \
+    NEED_IB(name);                                                                                 \
+    if (!ctx->pc) return fail(IBLGF_ERR_STATE, name ": no learned preconditioner", ctx)
+
+int iblgf_gpu_ib_precond_collect(iblgf_gpu_ctx* ctx, int stride)
+{
+    NEED_PC("iblgf_gpu_ib_precond_collect");
+    if (stride < 0) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_precond_collect: negative stride", ctx);
+    ctx->collect_stride = stride;
+    return IBLGF_OK;
+}
+
+int iblgf_gpu_ib_precond_train(iblgf_gpu_ctx* ctx, int steps, int batch_rows, double lr, double* loss, int* npairs)
+{
+    NEED_PC("iblgf_gpu_ib_precond_train");
+    if (steps < 0 || batch_rows < 1 || !(lr > 0)) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_precond_train: invalid argument", ctx);
+    return guarded(ctx, [&] {
+        const double l = steps > 0 ? ctx->pc->train(steps, batch_rows, (float)lr) : 0.0;
+        if (loss) *loss = l;
+        if (npairs) *npairs = ctx->pc->npairs();
+    });
+}
+
+int iblgf_gpu_ib_precond_info(iblgf_gpu_ctx* ctx, int* K, int* hidden, int* dim, long* nparam, double* out_scale)
+{
+    NEED_PC("iblgf_gpu_ib_precond_info");
+    if (K) *K = ctx->pc->feature_dim() / 6;
+    if (hidden) *hidden = ctx->pc->hidden();
+    if (dim) *dim = ctx->pc->feature_dim();
+    if (nparam) *nparam = static_cast<long>(ctx->pc->params().size());
+    if (out_scale) *out_scale = ctx->pc->output_scale();
+    return IBLGF_OK;
+}
+
+int iblgf_gpu_ib_precond_params(iblgf_gpu_ctx* ctx, float* params, long nparam)
+{
+    NEED_PC("iblgf_gpu_ib_precond_params");
+    if (!params) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_precond_params: null output", ctx);
+    return guarded(ctx, [&] {
+        const auto p = ctx->pc->params();
+        if ((long)p.size() != nparam) throw std::invalid_argument("parameter count mismatch");
+        std::copy(p.begin(), p.end(), params);
+    });
+}
+
+int iblgf_gpu_ib_precond_set_params(iblgf_gpu_ctx* ctx, const float* params, long nparam)
+{
+    NEED_PC("iblgf_gpu_ib_precond_set_params");
+    if (!params || nparam <= 0) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_precond_set_params: invalid argument", ctx);
+    return guarded(ctx, [&] { ctx->pc->set_params(std::vector<float>(params, params + nparam)); });
+}
+
+int iblgf_gpu_ib_precond_features(iblgf_gpu_ctx* ctx, const double* r, float* X)
+{
+    NEED_PC("iblgf_gpu_ib_precond_features");
+    if (!r || !X) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_precond_features: null argument", ctx);
+    return guarded(ctx, [&] {
+        ib_to_device(ctx, ctx->ib_in, r);
+        const auto v = ctx->pc->features(ctx->ib_in.p);
+        std::copy(v.begin(), v.end(), X);
+    });
+}
+
+int iblgf_gpu_ib_precond_grad(iblgf_gpu_ctx* ctx, const float* X, const float* T, int rows, float* grads, double* loss)
+{
+    NEED_PC("iblgf_gpu_ib_precond_grad");
+    if (!X || !T || !grads || rows < 1) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_precond_grad: invalid argument", ctx);
+    return guarded(ctx, [&] {
+        const int          D = ctx->pc->feature_dim();
+        std::vector<float> x(X, X + (size_t)rows * D), t(T, T + 3 * (size_t)rows), g;
+        const double       l = ctx->pc->loss_and_grad(x, t, rows, g);
+        std::copy(g.begin(), g.end(), grads);
+        if (loss) *loss = l;
     });
 }
 

@@ -48,13 +48,23 @@ struct DevArray
     int  size() const { return static_cast<int>(n); }
 };
 
+// One launch of the staged-source MAC: the groups, their entries (whose
+// .slot indexes usrc) and the unique source slots it stages in LDS.
+struct MacChunk
+{
+    DevArray<MacGroup> groups;
+    DevArray<MacSrc>   ents;
+    DevArray<int>      usrc;
+};
+
 struct MacLevelPlan
 {
-    int                  lambda = 0;  // level of the targets
-    int                  d = 0;       // level difference of the kernel
-    DevArray<MacGroup>   groups;
-    DevArray<MacSrc>     srcs;
-    long                 npairs = 0;
+    int                   lambda = 0;  // level of the targets
+    int                   d = 0;       // level difference of the kernel
+    std::vector<MacChunk> chunks;      // first stores, the rest accumulate
+    long                  npairs = 0;
+    long                  nentries = 0;
+    long                  nunique = 0;
 };
 
 struct PassPlan

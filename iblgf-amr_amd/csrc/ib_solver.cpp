@@ -115,7 +115,11 @@ CgStats IbSystem::cg(const double* d_uc, double* d_f, double alpha, double tol, 
         apply(p, Ap, alpha);
         ++st.applies;
         const double pAp = dot(p, Ap);
-        if (pAp == 0.0) break;
+        if (pAp == 0.0)
+        {
+            st.stop = "breakdown (p.Ap = 0)";
+            break;
+        }
         const double a = rsold / pAp;
         add(d_f, p, 1.0, a);
         add(r, Ap, 1.0, -a);
@@ -125,73 +129,96 @@ CgStats IbSystem::cg(const double* d_uc, double* d_f, double alpha, double tol, 
         st.residual = std::sqrt(rsnew / f2);
         st.history.push_back(st.residual);
         if (on_iteration) on_iteration(k, r, p, d_f);
-        if (st.residual < tol) break;
+        if (st.residual < tol)
+        {
+            st.stop = "converged";
+            break;
+        }
         add(p, r, rsnew / rsold, 1.0);
         rsold = rsnew;
     }
+    if (st.stop.empty()) st.stop = "iteration limit";
     IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return st;
 }
 
-// Flexible CG (Y. Notay, SIAM J. Sci. Comput. 22 (2000) 1444-1460) keeping
-// the last kKeep search directions: p_k = z_k - sum_i (z_k.q_i)/(p_i.q_i) p_i,
-// q = A p, alpha = (p_k.r_k)/(p_k.q_k).  A preconditioned direction that is
-// not finite or not a descent direction is replaced by r (a CG step).
+// Flexible CG in its conjugate-residual form (flexible GCR; Eisenstat, Elman &
+// Schultz 1983, with Notay's flexible preconditioning, SIAM J. Sci. Comput.
+// 22 (2000) 1444-1460): z = M(r), q = A z, both orthogonalised against the
+// kept directions in the q-inner product, alpha = (r.q)/(q.q).  Each step
+// minimises |r| over the kept space, so the method converges for an operator
+// whose symmetric part is positive definite even when it is only nearly
+// symmetric, as the FMM-approximated force operator is.  kKeep directions are
+// kept; when the store is full it restarts from the current residual.  A
+// preconditioned direction that is not finite or not a descent direction
+// (z.r <= 0) is replaced by r.
 CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M)
 {
-    constexpr int kKeep = 4;
+    constexpr int kKeep = 30;
     const auto    t0 = std::chrono::steady_clock::now();
     CgStats       st;
     const long    len = 3L * npoints_;
-    double *      r = d_tmp_[0].p, *z = d_tmp_[4].p, *Ax = d_tmp_[3].p;
-    std::vector<DevArray<double>> P(kKeep + 1), Q(kKeep + 1);
-    for (auto& v : P) v.alloc(len);
-    for (auto& v : Q) v.alloc(len);
-    std::deque<std::pair<int, double>> hist;  // (buffer index, p.q)
+    double *      r = d_tmp_[0].p, *Ax = d_tmp_[3].p;
+    if ((int)krylov_p_.size() < kKeep)
+    {
+        krylov_p_.resize(kKeep);
+        krylov_q_.resize(kKeep);
+        for (auto& v : krylov_p_) v.alloc(len);
+        for (auto& v : krylov_q_) v.alloc(len);
+    }
+    std::vector<double> qq;  // (q_i.q_i) of the kept directions
     apply(d_f, Ax, alpha);
     ++st.applies;
     IBLGF_HIP_IB(hipMemsetAsync(r, 0, len * sizeof(double), s_.stream()));
     add(r, d_uc, 0.0, 1.0);
     add(r, Ax, 1.0, -1.0);
-    int next = 0;
     for (int k = 0; k < maxit; ++k)
     {
-        IBLGF_HIP_IB(hipMemsetAsync(z, 0, len * sizeof(double), s_.stream()));
-        M(s_.stream(), r, z);
-        // non-owned entries of z never reach p: p is built by masked updates
-        double zr = dot(z, r);
+        if ((int)qq.size() == kKeep) qq.clear();  // restart: the store is full
+        const int j = static_cast<int>(qq.size());
+        double*   p = krylov_p_[j].p;
+        double*   q = krylov_q_[j].p;
+        IBLGF_HIP_IB(hipMemsetAsync(p, 0, len * sizeof(double), s_.stream()));
+        M(s_.stream(), r, p);
+        const double zr = dot(p, r);
         if (!std::isfinite(zr) || zr <= 0.0)
         {
-            IBLGF_HIP_IB(hipMemcpyAsync(z, r, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
+            IBLGF_HIP_IB(hipMemcpyAsync(p, r, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
             st.fell_back = true;
         }
-        double* p = P[next].p;
-        double* q = Q[next].p;
-        IBLGF_HIP_IB(hipMemsetAsync(p, 0, len * sizeof(double), s_.stream()));
-        add(p, z, 0.0, 1.0);
-        for (const auto& h : hist)
-        {
-            const double beta = dot(z, Q[h.first].p) / h.second;
-            add(p, P[h.first].p, 1.0, -beta);
-        }
+        // non-owned entries of p are zeroed: they never enter the update
+        launch::mask_zero(s_.stream(), len, d_owned_.p, p);
         apply(p, q, alpha);
         ++st.applies;
-        const double pq = dot(p, q);
-        if (pq == 0.0) break;
-        const double a = dot(p, r) / pq;
+        for (int i = 0; i < j; ++i)
+        {
+            const double beta = dot(q, krylov_q_[i].p) / qq[i];
+            add(p, krylov_p_[i].p, 1.0, -beta);
+            add(q, krylov_q_[i].p, 1.0, -beta);
+        }
+        const double qn = dot(q, q);
+        if (!(qn > 0.0))
+        {
+            st.stop = "breakdown (q.q = 0)";
+            break;
+        }
+        const double a = dot(r, q) / qn;
         add(d_f, p, 1.0, a);
         add(r, q, 1.0, -a);
+        qq.push_back(qn);
         const double rr = dot(r, r), f2 = dot(d_f, d_f);
         st.iterations = k + 1;
         st.residual = std::sqrt(rr / f2);
         st.history.push_back(st.residual);
         if (on_iteration) on_iteration(k, r, p, d_f);
-        if (st.residual < tol) break;
-        hist.emplace_back(next, pq);
-        if ((int)hist.size() > kKeep) hist.pop_front();
-        next = (next + 1) % (kKeep + 1);
+        if (st.residual < tol)
+        {
+            st.stop = "converged";
+            break;
+        }
     }
+    if (st.stop.empty()) st.stop = "iteration limit";
     IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return st;

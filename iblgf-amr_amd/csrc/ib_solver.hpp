@@ -13,6 +13,7 @@
 #include "host/ib.hpp"
 
 #include <functional>
+#include <string>
 #include <vector>
 
 namespace iblgf_amd {
@@ -25,6 +26,7 @@ struct CgStats
     double              seconds = 0.0;
     int                 applies = 0;     // operator applications
     bool                fell_back = false;
+    std::string         stop;            // converged / iteration limit / breakdown
 };
 
 // Flexible preconditioner: z = M(r) on device vectors of length 3N.
@@ -36,6 +38,7 @@ class IbSystem
     IbSystem(GpuSolver& solver, const IbGeometry& g);
 
     int npoints() const { return npoints_; }
+    hipStream_t stream() const { return s_.stream(); }
     const IbStencil& stencil() const { return st_; }
     const double*    owned_mask() const { return d_owned_.p; }
 
@@ -47,9 +50,9 @@ class IbSystem
 
     // CG_solve: f holds the initial guess and receives the solution.
     CgStats cg(const double* d_uc, double* d_f, double alpha, double tol, int maxit);
-    // Flexible CG (Notay 2000, truncated to the last direction) with the
-    // preconditioner M; falls back to plain CG directions when M returns a
-    // non-finite or non-descent direction.
+    // Flexible CG in conjugate-residual form (flexible GCR, 30 kept
+    // directions, restarted) with the preconditioner M; a non-finite or
+    // non-descent preconditioned direction is replaced by r.
     CgStats fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M);
 
     // FP64 dot over owned entries (LinSysSolver::dot).
@@ -70,6 +73,7 @@ class IbSystem
     int             field_[3], aux_[3], cell_;
     DevArray<double> d_tmp_[6];  // r, p, Ap, Ax, z, spare
     DevArray<double> d_part_, d_out_;
+    std::vector<DevArray<double>> krylov_p_, krylov_q_;
 };
 
 }  // namespace iblgf_amd

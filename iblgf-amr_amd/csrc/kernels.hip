@@ -611,6 +611,91 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
     }
 }
 
+// Staged-source variant: a workgroup owns KF frequencies of one launch and
+// first copies the spectra of the launch's unique sources at those
+// frequencies into LDS (each source spectrum is read from HBM once per
+// launch instead of once per sibling group it appears in), together with the
+// kernel spectra of the 343 offsets.  Work items are (group, frequency);
+// `parts` adjacent lanes share an item, split its source entries and reduce
+// with cross-lane shuffles.  Entry .slot is the index into usrc.  The first
+// source chunk of a launch stores the target spectra, later chunks add.
+constexpr int kMacThreads = 512;
+constexpr int kMacSrcCap = 2048;
+
+template<int N, int KF, class T>
+__global__ __launch_bounds__(kMacThreads) void k_spectral_mac_staged(const MacGroup* __restrict__ groups,
+    int ngroups, const MacSrc* __restrict__ ent, const int* __restrict__ usrc, int nu, int parts,
+    const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
+    typename Cx<T>::type* __restrict__ tspec, int accumulate)
+{
+    using C = typename Cx<T>::type;
+    constexpr int NF = Geo<N>::NF;
+    __shared__ C Gs[343 * KF];
+    __shared__ C Ss[kMacSrcCap * KF];
+    const int f0 = blockIdx.x * KF;
+    for (int q = threadIdx.x; q < 343 * KF; q += kMacThreads)
+    {
+        const int o = q / KF, f = q - o * KF;
+        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
+    }
+    for (int q = threadIdx.x; q < nu * KF; q += kMacThreads)
+    {
+        const int u = q / KF, f = q - u * KF;
+        Ss[q] = (f0 + f < NF) ? sspec[(size_t)usrc[u] * NF + f0 + f] : Cx<T>::make(0, 0);
+    }
+    __syncthreads();
+    const int items = ngroups * KF;
+    for (int q = threadIdx.x; q < items * parts; q += kMacThreads)
+    {
+        const int      item = q / parts, p = q - item * parts;
+        const int      g = item / KF, f = item - g * KF;
+        const MacGroup grp = groups[g];
+        double2        acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
+        const int end = grp.begin + grp.count;
+        int       e = grp.begin + p;
+        for (; e + 3 * parts < end; e += 4 * parts)
+        {
+            MacSrc m[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) m[i] = ent[e + i * parts];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) mac_entry<KF, C>(Gs, f, m[i].meta, Ss[m[i].slot * KF + f], acc);
+        }
+        for (; e < end; e += parts)
+        {
+            const MacSrc m = ent[e];
+            mac_entry<KF, C>(Gs, f, m.meta, Ss[m.slot * KF + f], acc);
+        }
+        for (int off = 1; off < parts; off <<= 1)
+        {
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+            {
+                acc[t].x += __shfl_xor(acc[t].x, off);
+                acc[t].y += __shfl_xor(acc[t].y, off);
+            }
+        }
+        if (p == 0 && f0 + f < NF)
+        {
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+            {
+                if (grp.tslot[t] < 0) continue;
+                C* d = tspec + (size_t)grp.tslot[t] * NF + f0 + f;
+                if (accumulate)
+                {
+                    const C o = *d;
+                    *d = Cx<T>::make((T)(o.x + acc[t].x), (T)(o.y + acc[t].y));
+                }
+                else
+                    *d = Cx<T>::make((T)acc[t].x, (T)acc[t].y);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_to_float2(const double2* __restrict__ in, float2* __restrict__ out, long n)
 {
     const long q = (long)blockIdx.x * kThreads + threadIdx.x;
@@ -864,6 +949,27 @@ template<int N> struct RunMac
         }
     }
 };
+template<int N> struct RunMacStaged
+{
+    static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* ent, const int* usrc, int nu,
+        int parts, const void* ghat, const void* ss, void* ts, int accumulate)
+    {
+        if (prec == 0)
+        {
+            constexpr int  KF = 2;
+            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
+            hipLaunchKernelGGL((k_spectral_mac_staged<N, KF, double>), dim3(nblk), dim3(kMacThreads), 0, s, g, ng,
+                ent, usrc, nu, parts, (const double2*)ghat, (const double2*)ss, (double2*)ts, accumulate);
+        }
+        else
+        {
+            constexpr int  KF = 4;
+            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
+            hipLaunchKernelGGL((k_spectral_mac_staged<N, KF, float>), dim3(nblk), dim3(kMacThreads), 0, s, g, ng,
+                ent, usrc, nu, parts, (const float2*)ghat, (const float2*)ss, (float2*)ts, accumulate);
+        }
+    }
+};
 template<int N> struct RunLgf
 {
     static void run(hipStream_t s, int kind, int d, const double* g, int nmax, const double* iv, int kmax,
@@ -930,6 +1036,20 @@ void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ng
 {
     if (ngroups > 0) dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, ghat, sspec, tspec);
 }
+int mac_source_cap() { return kMacSrcCap; }
+int mac_frequencies(int prec) { return prec == 0 ? 2 : 4; }
+int mac_threads() { return kMacThreads; }
+
+void spectral_mac_staged(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* entries,
+    const int* usrc, int nu, int parts, const void* ghat, const void* sspec, void* tspec, bool accumulate)
+{
+    if (nu > kMacSrcCap) throw std::runtime_error("too many sources in one MAC chunk");
+    if (parts < 1 || parts > 64 || (parts & (parts - 1))) throw std::runtime_error("MAC parts must be a power of two <= 64");
+    if (ngroups > 0)
+        dispatch_n<RunMacStaged>(n, s, prec, groups, ngroups, entries, usrc, nu, parts, ghat, sspec, tspec,
+            accumulate ? 1 : 0);
+}
+
 void to_float2(hipStream_t s, const double2* in, float2* out, long n)
 {
     if (n > 0) hipLaunchKernelGGL(k_to_float2, dim3(grid1(n)), dim3(kThreads), 0, s, in, out, n);

@@ -59,6 +59,14 @@ void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* sl
     double scale);
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
     const void* ghat, const void* sspec, void* tspec);
+// Staged-source MAC: entries' .slot indexes usrc (the chunk's unique source
+// slots, at most mac_source_cap()); `parts` lanes (a power of two) share a
+// (group, frequency) item.  accumulate = false stores, true adds.
+int  mac_source_cap();
+int  mac_frequencies(int prec);  // frequencies per workgroup
+int  mac_threads();
+void spectral_mac_staged(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* entries,
+    const int* usrc, int nu, int parts, const void* ghat, const void* sspec, void* tspec, bool accumulate);
 void to_float2(hipStream_t s, const double2* in, float2* out, long n);
 
 // Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:

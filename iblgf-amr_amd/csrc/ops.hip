@@ -186,6 +186,13 @@ __global__ __launch_bounds__(kThreads) void k_axpby(long len, double a, const do
     if (i < len && (!mask || mask[i] != 0.0)) y[i] = y[i] * b + x[i] * a;
 }
 
+__global__ __launch_bounds__(kThreads) void k_mask_zero(long len, const double* __restrict__ mask,
+    double* __restrict__ y)
+{
+    const long i = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (i < len && mask[i] == 0.0) y[i] = 0.0;
+}
+
 template<template<int> class F, typename... Args>
 void dispatch_n(int n, Args... args)
 {
@@ -276,6 +283,13 @@ void axpby(hipStream_t s, long len, double a, const double* x, double b, double*
     if (len <= 0) return;
     hipLaunchKernelGGL(k_axpby, dim3((unsigned)((len + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, len, a, x,
         b, y, mask);
+}
+
+void mask_zero(hipStream_t s, long len, const double* mask, double* y)
+{
+    if (len <= 0) return;
+    hipLaunchKernelGGL(k_mask_zero, dim3((unsigned)((len + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, len, mask,
+        y);
 }
 
 }  // namespace launch

@@ -51,6 +51,8 @@ void dot(hipStream_t s, const double* a, const double* b, const double* mask, lo
 // y = y*b + x*a elementwise where mask is nonzero (mask may be null); the
 // product and sum are rounded separately, as the reference's add().
 void axpby(hipStream_t s, long len, double a, const double* x, double b, double* y, const double* mask);
+// y = 0 where mask is zero.
+void mask_zero(hipStream_t s, long len, const double* mask, double* y);
 
 }  // namespace launch
 }  // namespace iblgf_amd

@@ -107,6 +107,16 @@ SYMBOLS = {
     "iblgf_gpu_ib_project": (_I, [_P, _IP, _DP]),
     "iblgf_gpu_ib_apply": (_I, [_P, _DP, _DP, _D]),
     "iblgf_gpu_ib_cg": (_I, [_P, _DP, _DP, _D, _D, _I, _IP, _DP, _DP]),
+    "iblgf_gpu_ib_stats": (_I, [_P, _DP, _IP, _IP, _IP]),
+    "iblgf_gpu_ib_precond_create": (_I, [_P, _P, _I, _I, _U64]),
+    "iblgf_gpu_ib_precond_collect": (_I, [_P, _I]),
+    "iblgf_gpu_ib_precond_train": (_I, [_P, _I, _I, _D, _DP, _IP]),
+    "iblgf_gpu_ib_precond_info": (_I, [_P, _IP, _IP, _IP, ctypes.POINTER(ctypes.c_long), _DP]),
+    "iblgf_gpu_ib_precond_params": (_I, [_P, _P, ctypes.c_long]),
+    "iblgf_gpu_ib_precond_set_params": (_I, [_P, _P, ctypes.c_long]),
+    "iblgf_gpu_ib_precond_features": (_I, [_P, _DP, _P]),
+    "iblgf_gpu_ib_precond_grad": (_I, [_P, _P, _P, _I, _P, _DP]),
+    "iblgf_gpu_ib_fcg": (_I, [_P, _DP, _DP, _D, _D, _I, _I, _IP, _DP, _DP]),
 }
 
 KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
@@ -422,6 +432,79 @@ class IbSystem:
                                      float(alpha), float(tol), int(maxit), ctypes.byref(it), ctypes.byref(res),
                                      hist.ctypes.data_as(_DP)), self.solver._h)
         return f.reshape(-1, 3), it.value, res.value, hist[: it.value].copy()
+
+    def fcg(self, uc, alpha, tol=1e-8, maxit=200, f0=None, use_mlp=True):
+        """Flexible CG in conjugate-residual form with the learned preconditioner (identity when
+        use_mlp is False); returns (f, it, res, history)."""
+        f = np.zeros(3 * self.npoints) if f0 is None else self._vec(f0).copy()
+        it, res = ctypes.c_int(), ctypes.c_double()
+        hist = np.zeros(max(int(maxit), 1))
+        _check(lib().iblgf_gpu_ib_fcg(self.solver._h, self._vec(uc).ctypes.data_as(_DP), f.ctypes.data_as(_DP),
+                                      float(alpha), float(tol), int(maxit), 1 if use_mlp else 0, ctypes.byref(it),
+                                      ctypes.byref(res), hist.ctypes.data_as(_DP)), self.solver._h)
+        return f.reshape(-1, 3), it.value, res.value, hist[: it.value].copy()
+
+    STOP = ("converged", "iteration limit", "breakdown")
+
+    def stats(self):
+        """(seconds, operator applications, preconditioner fell back, stop reason) of the last solve."""
+        sec, ap, fb, st = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().iblgf_gpu_ib_stats(self.solver._h, ctypes.byref(sec), ctypes.byref(ap), ctypes.byref(fb),
+                                        ctypes.byref(st)), self.solver._h)
+        return sec.value, ap.value, bool(fb.value), self.STOP[st.value]
+
+
+class LearnedPreconditioner:
+    """The MLP preconditioner of an IbSystem's flexible CG (trained on the device)."""
+
+    def __init__(self, system, K=16, hidden=128, seed=7):
+        self.system = system
+        h = system.solver._h
+        self._h = h
+        _check(lib().iblgf_gpu_ib_precond_create(h, system.solver.tree._h, int(K), int(hidden), int(seed)), h)
+        k, hid, dim, npar, cs = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_long(), ctypes.c_double()
+        _check(lib().iblgf_gpu_ib_precond_info(h, ctypes.byref(k), ctypes.byref(hid), ctypes.byref(dim),
+                                               ctypes.byref(npar), ctypes.byref(cs)), h)
+        self.K, self.hidden, self.dim, self.nparam = k.value, hid.value, dim.value, npar.value
+
+    def collect(self, stride=1):
+        """Record every stride-th iterate of the following solves (0 stops)."""
+        _check(lib().iblgf_gpu_ib_precond_collect(self._h, int(stride)), self._h)
+
+    def train(self, steps=2000, batch_rows=4096, lr=1e-3):
+        loss, npairs = ctypes.c_double(), ctypes.c_int()
+        _check(lib().iblgf_gpu_ib_precond_train(self._h, int(steps), int(batch_rows), float(lr), ctypes.byref(loss),
+                                                ctypes.byref(npairs)), self._h)
+        return loss.value, npairs.value
+
+    def output_scale(self):
+        cs = ctypes.c_double()
+        _check(lib().iblgf_gpu_ib_precond_info(self._h, None, None, None, None, ctypes.byref(cs)), self._h)
+        return cs.value
+
+    def params(self):
+        p = np.zeros(self.nparam, np.float32)
+        _check(lib().iblgf_gpu_ib_precond_params(self._h, _ptr(p), self.nparam), self._h)
+        return p
+
+    def set_params(self, p):
+        q = np.ascontiguousarray(p, np.float32)
+        _check(lib().iblgf_gpu_ib_precond_set_params(self._h, _ptr(q), q.size), self._h)
+
+    def features(self, r):
+        X = np.zeros((self.system.npoints, self.dim), np.float32)
+        _check(lib().iblgf_gpu_ib_precond_features(self._h, self.system._vec(r).ctypes.data_as(_DP), _ptr(X)),
+               self._h)
+        return X
+
+    def loss_and_grad(self, X, T):
+        x = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.nparam, np.float32)
+        loss = ctypes.c_double()
+        _check(lib().iblgf_gpu_ib_precond_grad(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), ctypes.byref(loss)),
+               self._h)
+        return loss.value, g
 
 
 def lgf_get(x, y, z):

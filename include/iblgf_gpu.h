@@ -277,6 +277,43 @@ int iblgf_gpu_ib_apply(iblgf_gpu_ctx* ctx, const double* fin, double* fout, doub
 int iblgf_gpu_ib_cg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
     int* iterations, double* residual, double* history);
 
+/* Wall time (s), operator applications, preconditioner fallback flag and
+ * stop reason (0 converged, 1 iteration limit, 2 breakdown) of the last CG /
+ * flexible-CG solve. */
+int iblgf_gpu_ib_stats(iblgf_gpu_ctx* ctx, double* seconds, int* applies, int* fell_back, int* stop);
+
+/* ---- learned preconditioner (no reference counterpart: the reference
+ * solves with plain CG, linsys.hpp:170-230; this is its replacement for
+ * callers that repeat force solves) ------------------------------------- */
+
+/* A per-point MLP (3 layers, hidden width `hidden`, tanh, BF16 weights, FP32
+ * accumulation) over the residuals of each point's K nearest surface
+ * neighbours in the point's local frame.  Weights are seeded from `seed`. */
+int iblgf_gpu_ib_precond_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t, int K, int hidden, uint64_t seed);
+/* Record every stride-th iterate of the following iblgf_gpu_ib_cg /
+ * iblgf_gpu_ib_fcg solves as training pairs (r_k, f* - f_k); stride 0 stops. */
+int iblgf_gpu_ib_precond_collect(iblgf_gpu_ctx* ctx, int stride);
+/* Adam steps on random point batches of the recorded pairs, on the device;
+ * loss = mean squared error of the last steps, npairs = pairs held. */
+int iblgf_gpu_ib_precond_train(iblgf_gpu_ctx* ctx, int steps, int batch_rows, double lr, double* loss, int* npairs);
+/* Sizes: K, hidden, feature dimension (6K), parameter count, output scale. */
+int iblgf_gpu_ib_precond_info(iblgf_gpu_ctx* ctx, int* K, int* hidden, int* dim, long* nparam, double* out_scale);
+/* Checking hooks: FP32 master parameters (get/set), the features of a host
+ * residual (npoints x 6K), and loss + parameter gradients of a batch
+ * X [rows][6K], T [rows][3] through the device kernels. */
+int iblgf_gpu_ib_precond_params(iblgf_gpu_ctx* ctx, float* params, long nparam);
+int iblgf_gpu_ib_precond_set_params(iblgf_gpu_ctx* ctx, const float* params, long nparam);
+int iblgf_gpu_ib_precond_features(iblgf_gpu_ctx* ctx, const double* r, float* X);
+int iblgf_gpu_ib_precond_grad(iblgf_gpu_ctx* ctx, const float* X, const float* T, int rows, float* grads, double* loss);
+/* Flexible CG in conjugate-residual form (flexible GCR with 30 kept
+ * directions, restarted; minimises |r| each step, so it converges although
+ * the FMM-approximated operator is only symmetric to ~1e-4, where plain CG
+ * stalls), preconditioned by the learned network when use_mlp is nonzero
+ * (the identity otherwise, and wherever the network's direction is not a
+ * finite descent direction); same stopping rule and outputs as iblgf_gpu_ib_cg. */
+int iblgf_gpu_ib_fcg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
+    int use_mlp, int* iterations, double* residual, double* history);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,7 @@ def test_level_divergence_gradient_and_cleaning_bitwise(sphere):
     solver.upload(fc, np.zeros(shape))
     for level in (s.info["base_level"], L):
         print(f"level_divergence / gradient level {level}", flush=True)
+        solver.upload(fc, np.zeros(shape))
         solver.level_divergence(fs, fc, level)
         got = solver.download(fc)
         ref_src = [a.copy() for a in srcs]
@@ -161,10 +162,13 @@ def test_ib_operator_and_cg_match_oracle(sphere, alpha):
     assert it == ito == 4 and l2 < 1e-8
 
 
-def test_full_sphere_cg_converges():
-    """The C4-shape sphere (N seeded in 5000..20000, 3-level 16^3-block mesh):
-    A is symmetric on owned points, CG reaches 1e-8 and the true residual
-    agrees with the recurrence."""
+def test_full_sphere_operator_and_solvers():
+    """The C4-shape sphere (N seeded in 5000..20000, 3-level 16^3-block mesh).
+    The FMM's anterpolation/interpolation and the IB-region cleaning make A
+    symmetric only to ~1e-4 (a property of the reference's operator), so plain
+    CG is checked at the reference's own threshold (cg_threshold = 1e-3,
+    linsys.hpp:63) and the flexible conjugate-residual solver at 1e-8; the
+    true residual uc - A f agrees with each recurrence."""
     s = setups.IbSphere()
     print(f"case: full IB sphere N={s.npoints} blocks={s.info['nblocks']}", flush=True)
     solver = PoissonSolver(s.tree)
@@ -174,10 +178,14 @@ def test_full_sphere_cg_converges():
     alpha = 0.5
     ax, ay = sysm.apply(x, alpha), sysm.apply(y, alpha)
     sym = abs(np.sum(x * ay) - np.sum(y * ax)) / abs(np.sum(x * ay))
-    print(f"symmetry {sym:.3e}")
-    assert sym < 1e-10
-    f, it, res, hist = sysm.cg(s.uc, alpha, tol=1e-8, maxit=600)
-    r = s.uc - sysm.apply(f, alpha)
-    true_res = np.linalg.norm(r) / np.linalg.norm(f)
-    print(f"CG iterations {it} recurrence {res:.3e} true {true_res:.3e}")
+    pos = np.sum(x * ax) / np.sum(x * x)
+    print(f"asymmetry {sym:.3e}, x.Ax/x.x {pos:.3e}", flush=True)
+    assert sym < 1e-3 and pos > 0
+    f, it, res, _ = sysm.cg(s.uc, alpha, tol=1e-3, maxit=400)
+    true_res = np.linalg.norm(s.uc - sysm.apply(f, alpha)) / np.linalg.norm(f)
+    print(f"CG to 1e-3: iterations {it} recurrence {res:.3e} true {true_res:.3e}", flush=True)
+    assert res < 1e-3 and true_res < 2e-3
+    f, it, res, _ = sysm.fcg(s.uc, alpha, tol=1e-8, maxit=3000, use_mlp=False)
+    true_res = np.linalg.norm(s.uc - sysm.apply(f, alpha)) / np.linalg.norm(f)
+    print(f"flexible GCR to 1e-8: iterations {it} recurrence {res:.3e} true {true_res:.3e} stop {sysm.stats()[3]}")
     assert res < 1e-8 and true_res < 1e-6
