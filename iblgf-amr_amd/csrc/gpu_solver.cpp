@@ -530,40 +530,10 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
         MacLevelPlan m;
         m.lambda = lam;
         m.d = L - lam;
+        m.groups.upload(groups);
+        m.srcs.upload(srcs);
         m.npairs = pairs_here;
         m.nentries = static_cast<long>(srcs.size());
-        // unique sources in first-appearance order, split into LDS-sized chunks
-        std::unordered_map<int, int> uid;
-        std::vector<int>             uniq;
-        for (const auto& e : srcs)
-            if (uid.emplace(e.slot, (int)uniq.size()).second) uniq.push_back(e.slot);
-        m.nunique = static_cast<long>(uniq.size());
-        const int cap = launch::mac_source_cap();
-        const int nchunk = std::max(1, (int)((uniq.size() + cap - 1) / cap));
-        for (int c = 0; c < nchunk; ++c)
-        {
-            std::vector<MacGroup> cg;
-            std::vector<MacSrc>   ce;
-            for (const auto& g : groups)
-            {
-                MacGroup h = g;
-                h.begin = static_cast<int>(ce.size());
-                for (int e = g.begin; e < g.begin + g.count; ++e)
-                {
-                    const int u = uid[srcs[e].slot];
-                    if (u / cap != c) continue;
-                    ce.push_back(MacSrc{u - c * cap, srcs[e].meta});
-                }
-                h.count = static_cast<int>(ce.size()) - h.begin;
-                if (c == 0 || h.count > 0) cg.push_back(h);  // the storing chunk covers every group
-            }
-            MacChunk ch;
-            ch.groups.upload(cg);
-            ch.ents.upload(ce);
-            ch.usrc.upload(std::vector<int>(uniq.begin() + c * cap,
-                uniq.begin() + std::min<size_t>(uniq.size(), (size_t)(c + 1) * cap)));
-            m.chunks.push_back(std::move(ch));
-        }
         p.npairs += pairs_here;
         p.mac.push_back(std::move(m));
     }
@@ -709,16 +679,7 @@ void GpuSolver::run_pass(PassPlan& p, int src_field, int dst_field, double scale
     {
         const void* gh = ghat(laplace, m.d, alpha);
         timer_.begin(stream_);
-        const int kf = launch::mac_frequencies(precision_);
-        for (size_t c = 0; c < m.chunks.size(); ++c)
-        {
-            const MacChunk& ch = m.chunks[c];
-            const int       items = ch.groups.size() * kf;
-            int             parts = 1;  // lanes per (group, frequency), up to one round of the workgroup
-            while (parts < 8 && items * parts * 2 <= 2 * launch::mac_threads()) parts *= 2;
-            launch::spectral_mac_staged(stream_, n_, precision_, ch.groups.p, ch.groups.size(), ch.ents.p, ch.usrc.p,
-                ch.usrc.size(), parts, gh, sspec_.p, tspec_.p, c > 0);
-        }
+        launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, gh, sspec_.p, tspec_.p);
         timer_.end(stream_, KC_MAC, m.npairs);
     }
     const double P = 2.0 * n_ - 1.0;

@@ -48,23 +48,14 @@ struct DevArray
     int  size() const { return static_cast<int>(n); }
 };
 
-// One launch of the staged-source MAC: the groups, their entries (whose
-// .slot indexes usrc) and the unique source slots it stages in LDS.
-struct MacChunk
-{
-    DevArray<MacGroup> groups;
-    DevArray<MacSrc>   ents;
-    DevArray<int>      usrc;
-};
-
 struct MacLevelPlan
 {
-    int                   lambda = 0;  // level of the targets
-    int                   d = 0;       // level difference of the kernel
-    std::vector<MacChunk> chunks;      // first stores, the rest accumulate
-    long                  npairs = 0;
-    long                  nentries = 0;
-    long                  nunique = 0;
+    int                lambda = 0;  // level of the targets
+    int                d = 0;       // level difference of the kernel
+    DevArray<MacGroup> groups;
+    DevArray<MacSrc>   srcs;
+    long               npairs = 0;
+    long               nentries = 0;
 };
 
 struct PassPlan
@@ -174,8 +165,8 @@ class GpuSolver
     void add(int from, int to, double scale);
     const Octree& tree() const { return tree_; }
     const std::vector<int>& slot_of() const { return slot_of_; }
-    // 0: plain FP64; 1: mixed (FP32 spectra, DFT legs and MAC; FP64 fields,
-    // kernel-spectrum generation and cross-level transfers).
+    // 0: plain FP64; 1: mixed (source and kernel spectra stored FP32 and
+    // multiplied in FP32 with FP64 accumulation; everything else FP64).
     void set_precision(int p);
     int  precision() const { return precision_; }
 

@@ -256,9 +256,10 @@ __global__ __launch_bounds__(kThreads) void k_correction_stencil(
 // the twiddle index depends only on loop counters that are uniform across the
 // workgroup, so twiddle reads from LDS are broadcasts.  The kx axis is the
 // halved one; legs after the first run on GS kx-planes at a time so the
-// intermediate fits in LDS.  Arithmetic is FP64; T is the storage type of
-// the spectra and the LDS intermediates (double: plain FP64 path; float:
-// mixed path, whose fields stay FP64).
+// intermediate fits in LDS.  Arithmetic and LDS intermediates are FP64; the
+// forward transform's output spectrum is stored as T (double: plain FP64
+// path; float: mixed path, where the source spectra that the MAC streams are
+// FP32).
 // ---------------------------------------------------------------------------
 template<class T>
 struct Cx;
@@ -275,15 +276,13 @@ struct Cx<float>
     static __device__ inline float2 make(float a, float b) { return make_float2(a, b); }
 };
 
-template<int N, class T>
+template<int N>
 struct DftPlan
 {
     static constexpr int P = 2 * N - 1;
     static constexpr int H = P / 2 + 1;
-    static constexpr int CS = 2 * (int)sizeof(T);
-    // FP64 needs most of the 160 KB LDS for one block; FP32 is held to half
-    // of it so two workgroups share a CU.
-    static constexpr int kLds = sizeof(T) == 8 ? 163840 - 4096 : 81920 - 2048;
+    static constexpr int CS = 16;  // double2 intermediates
+    static constexpr int kLds = 163840 - 4096;
     static constexpr int kFwdRaw = (kLds - N * N * H * CS - P * 16) / (N * P * CS);
     static constexpr int FWD_GS = kFwdRaw < H ? kFwdRaw : H;
     static constexpr int kInvRaw = (kLds - H * N * N * CS - P * 16) / (P * N * CS);
@@ -314,9 +313,9 @@ template<int N, class T>
 __global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restrict__ field,
     const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec)
 {
-    using C = typename Cx<T>::type;
+    using C = double2;  // LDS intermediates
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H, GS = DftPlan<N, T>::FWD_GS;
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::FWD_GS;
     static_assert(GS >= 1, "LDS too small for one plane");
     __shared__ C       X1[N * N * H];  // [z][y][kx]
     __shared__ C       Tp[GS * N * P];  // [g][z][ky]
@@ -345,12 +344,12 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restri
                 m += kx;
                 if (m >= P) m -= P;
             }
-            X1[l * H + kx] = Cx<T>::make((T)sr, (T)si);
+            X1[l * H + kx] = make_double2(sr, si);
         }
     }
     __syncthreads();
-    C*            out = spec + (size_t)slot * G::NF;
-    constexpr int K0 = (P + 1) / 2;  // y-leg outputs split over two threads per line
+    typename Cx<T>::type* out = spec + (size_t)slot * G::NF;
+    constexpr int         K0 = (P + 1) / 2;  // y-leg outputs split over two threads per line
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
         const int gs = min(GS, H - kx0);
@@ -377,7 +376,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restri
                     m += ky;
                     if (m >= P) m -= P;
                 }
-                Tp[(g * N + z) * P + ky] = Cx<T>::make((T)sr, (T)si);
+                Tp[(g * N + z) * P + ky] = make_double2(sr, si);
             }
         }
         __syncthreads();
@@ -391,7 +390,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restri
                 const C c = Tp[(g * N + z) * P + ky];
                 a[z] = make_double2(c.x, c.y);
             }
-            C* o = out + (size_t)(kx * P + ky) * P;
+            typename Cx<T>::type* o = out + (size_t)(kx * P + ky) * P;
             for (int kz = 0; kz < P; ++kz)
             {
                 double sr = 0, si = 0;
@@ -411,14 +410,14 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restri
 }
 
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
-// added into the (FP64) target block.  FP64 arithmetic, T-typed storage.
-template<int N, class T>
-__global__ __launch_bounds__(kThreads) void k_inverse_dft(const typename Cx<T>::type* __restrict__ spec,
+// added into the (FP64) target block.  FP64 throughout.
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restrict__ spec,
     const int* __restrict__ slots, int count, double* __restrict__ field, double scale)
 {
-    using C = typename Cx<T>::type;
+    using C = double2;
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H, GS = DftPlan<N, T>::INV_GS;
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::INV_GS;
     static_assert(GS >= 1, "LDS too small for one plane");
     __shared__ C       V[H * N * N];   // [kx][z][y]
     __shared__ C       U[GS * P * N];  // [g][ky][z]
@@ -456,7 +455,7 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const typename Cx<T>::
                     m += step;
                     if (m >= P) m -= P;
                 }
-                U[(g * P + ky) * N + z] = Cx<T>::make((T)sr, (T)si);
+                U[(g * P + ky) * N + z] = make_double2(sr, si);
             }
         }
         __syncthreads();
@@ -483,7 +482,7 @@ __global__ __launch_bounds__(kThreads) void k_inverse_dft(const typename Cx<T>::
                     m += step;
                     if (m >= P) m -= P;
                 }
-                V[(kx * N + z) * N + y] = Cx<T>::make((T)sr, (T)si);
+                V[(kx * N + z) * N + y] = make_double2(sr, si);
             }
         }
         __syncthreads();
@@ -558,7 +557,7 @@ __device__ inline void mac_entry(const C* Gs, int f, int meta, C S, double2 (&ac
 template<int N, int KF, class T>
 __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
     const MacSrc* __restrict__ srcs, const typename Cx<T>::type* __restrict__ ghat,
-    const typename Cx<T>::type* __restrict__ sspec, typename Cx<T>::type* __restrict__ tspec)
+    const typename Cx<T>::type* __restrict__ sspec, double2* __restrict__ tspec)
 {
     using C = typename Cx<T>::type;
     constexpr int NF = Geo<N>::NF;
@@ -606,92 +605,7 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
         {
 #pragma unroll
             for (int t = 0; t < 8; ++t)
-                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = Cx<T>::make((T)acc[t].x, (T)acc[t].y);
-        }
-    }
-}
-
-// Staged-source variant: a workgroup owns KF frequencies of one launch and
-// first copies the spectra of the launch's unique sources at those
-// frequencies into LDS (each source spectrum is read from HBM once per
-// launch instead of once per sibling group it appears in), together with the
-// kernel spectra of the 343 offsets.  Work items are (group, frequency);
-// `parts` adjacent lanes share an item, split its source entries and reduce
-// with cross-lane shuffles.  Entry .slot is the index into usrc.  The first
-// source chunk of a launch stores the target spectra, later chunks add.
-constexpr int kMacThreads = 512;
-constexpr int kMacSrcCap = 2048;
-
-template<int N, int KF, class T>
-__global__ __launch_bounds__(kMacThreads) void k_spectral_mac_staged(const MacGroup* __restrict__ groups,
-    int ngroups, const MacSrc* __restrict__ ent, const int* __restrict__ usrc, int nu, int parts,
-    const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
-    typename Cx<T>::type* __restrict__ tspec, int accumulate)
-{
-    using C = typename Cx<T>::type;
-    constexpr int NF = Geo<N>::NF;
-    __shared__ C Gs[343 * KF];
-    __shared__ C Ss[kMacSrcCap * KF];
-    const int f0 = blockIdx.x * KF;
-    for (int q = threadIdx.x; q < 343 * KF; q += kMacThreads)
-    {
-        const int o = q / KF, f = q - o * KF;
-        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
-    }
-    for (int q = threadIdx.x; q < nu * KF; q += kMacThreads)
-    {
-        const int u = q / KF, f = q - u * KF;
-        Ss[q] = (f0 + f < NF) ? sspec[(size_t)usrc[u] * NF + f0 + f] : Cx<T>::make(0, 0);
-    }
-    __syncthreads();
-    const int items = ngroups * KF;
-    for (int q = threadIdx.x; q < items * parts; q += kMacThreads)
-    {
-        const int      item = q / parts, p = q - item * parts;
-        const int      g = item / KF, f = item - g * KF;
-        const MacGroup grp = groups[g];
-        double2        acc[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
-        const int end = grp.begin + grp.count;
-        int       e = grp.begin + p;
-        for (; e + 3 * parts < end; e += 4 * parts)
-        {
-            MacSrc m[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) m[i] = ent[e + i * parts];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) mac_entry<KF, C>(Gs, f, m[i].meta, Ss[m[i].slot * KF + f], acc);
-        }
-        for (; e < end; e += parts)
-        {
-            const MacSrc m = ent[e];
-            mac_entry<KF, C>(Gs, f, m.meta, Ss[m.slot * KF + f], acc);
-        }
-        for (int off = 1; off < parts; off <<= 1)
-        {
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-            {
-                acc[t].x += __shfl_xor(acc[t].x, off);
-                acc[t].y += __shfl_xor(acc[t].y, off);
-            }
-        }
-        if (p == 0 && f0 + f < NF)
-        {
-#pragma unroll
-            for (int t = 0; t < 8; ++t)
-            {
-                if (grp.tslot[t] < 0) continue;
-                C* d = tspec + (size_t)grp.tslot[t] * NF + f0 + f;
-                if (accumulate)
-                {
-                    const C o = *d;
-                    *d = Cx<T>::make((T)(o.x + acc[t].x), (T)(o.y + acc[t].y));
-                }
-                else
-                    *d = Cx<T>::make((T)acc[t].x, (T)acc[t].y);
-            }
+                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
         }
     }
 }
@@ -920,12 +834,9 @@ template<int N> struct RunInv
 {
     static void run(hipStream_t s, int prec, const void* spec, const int* slots, int count, double* f, double scale)
     {
-        if (prec == 0)
-            hipLaunchKernelGGL((k_inverse_dft<N, double>), dim3(count), dim3(kThreads), 0, s, (const double2*)spec,
-                slots, count, f, scale);
-        else
-            hipLaunchKernelGGL((k_inverse_dft<N, float>), dim3(count), dim3(kThreads), 0, s, (const float2*)spec,
-                slots, count, f, scale);
+        (void)prec;  // target spectra are FP64 on both paths
+        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count), dim3(kThreads), 0, s, (const double2*)spec, slots, count,
+            f, scale);
     }
 };
 template<int N> struct RunMac
@@ -945,28 +856,7 @@ template<int N> struct RunMac
             constexpr int  KF = 16;
             const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
             hipLaunchKernelGGL((k_spectral_mac<N, KF, float>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src,
-                (const float2*)ghat, (const float2*)ss, (float2*)ts);
-        }
-    }
-};
-template<int N> struct RunMacStaged
-{
-    static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* ent, const int* usrc, int nu,
-        int parts, const void* ghat, const void* ss, void* ts, int accumulate)
-    {
-        if (prec == 0)
-        {
-            constexpr int  KF = 2;
-            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
-            hipLaunchKernelGGL((k_spectral_mac_staged<N, KF, double>), dim3(nblk), dim3(kMacThreads), 0, s, g, ng,
-                ent, usrc, nu, parts, (const double2*)ghat, (const double2*)ss, (double2*)ts, accumulate);
-        }
-        else
-        {
-            constexpr int  KF = 4;
-            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
-            hipLaunchKernelGGL((k_spectral_mac_staged<N, KF, float>), dim3(nblk), dim3(kMacThreads), 0, s, g, ng,
-                ent, usrc, nu, parts, (const float2*)ghat, (const float2*)ss, (float2*)ts, accumulate);
+                (const float2*)ghat, (const float2*)ss, (double2*)ts);
         }
     }
 };
@@ -1036,20 +926,6 @@ void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ng
 {
     if (ngroups > 0) dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, ghat, sspec, tspec);
 }
-int mac_source_cap() { return kMacSrcCap; }
-int mac_frequencies(int prec) { return prec == 0 ? 2 : 4; }
-int mac_threads() { return kMacThreads; }
-
-void spectral_mac_staged(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* entries,
-    const int* usrc, int nu, int parts, const void* ghat, const void* sspec, void* tspec, bool accumulate)
-{
-    if (nu > kMacSrcCap) throw std::runtime_error("too many sources in one MAC chunk");
-    if (parts < 1 || parts > 64 || (parts & (parts - 1))) throw std::runtime_error("MAC parts must be a power of two <= 64");
-    if (ngroups > 0)
-        dispatch_n<RunMacStaged>(n, s, prec, groups, ngroups, entries, usrc, nu, parts, ghat, sspec, tspec,
-            accumulate ? 1 : 0);
-}
-
 void to_float2(hipStream_t s, const double2* in, float2* out, long n)
 {
     if (n > 0) hipLaunchKernelGGL(k_to_float2, dim3(grid1(n)), dim3(kThreads), 0, s, in, out, n);

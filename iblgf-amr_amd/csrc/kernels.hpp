@@ -52,21 +52,15 @@ void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int coun
 void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* corr, const int* slots,
     int count, double inv_dx2, double omega2);
 
-// prec 0: spectra and DFT/MAC arithmetic in FP64 (double2);
-// prec 1: spectra and DFT/MAC arithmetic in FP32 (float2).  Fields are FP64.
+// prec 0: every spectrum FP64 (double2);
+// prec 1: source spectra and kernel spectra stored FP32 (float2), read by the
+// MAC as FP32 products accumulated in FP64; target spectra, DFT arithmetic,
+// LDS intermediates and fields FP64.
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec);
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale);
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
     const void* ghat, const void* sspec, void* tspec);
-// Staged-source MAC: entries' .slot indexes usrc (the chunk's unique source
-// slots, at most mac_source_cap()); `parts` lanes (a power of two) share a
-// (group, frequency) item.  accumulate = false stores, true adds.
-int  mac_source_cap();
-int  mac_frequencies(int prec);  // frequencies per workgroup
-int  mac_threads();
-void spectral_mac_staged(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* entries,
-    const int* usrc, int nu, int parts, const void* ghat, const void* sspec, void* tspec, bool accumulate);
 void to_float2(hipStream_t s, const double2* in, float2* out, long n);
 
 // Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:
