@@ -182,6 +182,7 @@ def main():
         solver.sync()
         ev0, ev1 = hip_events.Event(), hip_events.Event()
         solver.timing(True)
+        solver.timing_reset()
         solver.sync()
         t0 = time.perf_counter()
         ev0.record(stream)

@@ -49,6 +49,7 @@ template struct DevArray<double2>;
 template struct DevArray<float2>;
 template struct DevArray<float>;
 template struct DevArray<uint16_t>;
+template struct DevArray<short>;
 template struct DevArray<MacGroup>;
 template struct DevArray<MacSrc>;
 
@@ -534,6 +535,28 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
         m.srcs.upload(srcs);
         m.npairs = pairs_here;
         m.nentries = static_cast<long>(srcs.size());
+        {
+            std::vector<char> used(343, 0);
+            for (const auto& e : srcs)
+            {
+                const int r = e.meta & 255, tm = e.meta >> 8;
+                const int ob = (r % 6 + 1) + 7 * ((r / 6) % 6 + 1) + 49 * (r / 36 + 1);
+                for (int t = 0; t < 8; ++t)
+                    if (tm & (1 << t)) used[ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1))] = 1;
+            }
+            std::vector<int>   ol;
+            std::vector<short> om(343, 0);
+            for (int o = 0; o < 343; ++o)
+                if (used[o])
+                {
+                    om[o] = static_cast<short>(ol.size());
+                    ol.push_back(o);
+                }
+            if (ol.empty()) ol.push_back(0);
+            m.nused = static_cast<int>(ol.size());
+            m.olist.upload(ol);
+            m.omap.upload(om);
+        }
         p.npairs += pairs_here;
         p.mac.push_back(std::move(m));
     }
@@ -679,7 +702,8 @@ void GpuSolver::run_pass(PassPlan& p, int src_field, int dst_field, double scale
     {
         const void* gh = ghat(laplace, m.d, alpha);
         timer_.begin(stream_);
-        launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, gh, sspec_.p, tspec_.p);
+        launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.olist.p, m.nused,
+            m.omap.p, gh, sspec_.p, tspec_.p);
         timer_.end(stream_, KC_MAC, m.npairs);
     }
     const double P = 2.0 * n_ - 1.0;

@@ -54,6 +54,9 @@ struct MacLevelPlan
     int                d = 0;       // level difference of the kernel
     DevArray<MacGroup> groups;
     DevArray<MacSrc>   srcs;
+    DevArray<int>      olist;  // kernel-spectrum rows (offsets) the entries use
+    DevArray<short>    omap;   // offset -> position in olist
+    int                nused = 0;
     long               npairs = 0;
     long               nentries = 0;
 };

@@ -94,6 +94,10 @@ __global__ __launch_bounds__(kThreads) void k_axpy(
 // broadcasts.  The x leg reads the input block from global memory, the y leg
 // runs in place in the one LDS buffer, the z leg writes global memory.
 // ---------------------------------------------------------------------------
+// One thread per line of a leg: the n^2 = 324 lines of an 18^3 block fit one
+// round of 384 threads.
+constexpr int kTransferThreads = 384;
+
 template<int N>
 struct TransferLds
 {
@@ -118,7 +122,7 @@ __device__ inline void line_apply(const double (&v)[N], const double* K, double*
 template<int N>
 __device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>& L, int bx, int by)
 {
-    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    for (int l = threadIdx.x; l < N * N; l += kTransferThreads)
     {
         double v[N];
 #pragma unroll
@@ -126,7 +130,7 @@ __device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>
         line_apply<N>(v, L.m[bx], L.buf, l * N, 1);
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    for (int l = threadIdx.x; l < N * N; l += kTransferThreads)
     {
         const int base = (l / N) * N * N + (l % N);
         double    v[N];
@@ -141,7 +145,7 @@ __device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>
 template<int N>
 __device__ inline void leg_z_store(const double* t2, double* out, const double* Kz, bool accumulate)
 {
-    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    for (int l = threadIdx.x; l < N * N; l += kTransferThreads)
     {
         double v[N];
 #pragma unroll
@@ -162,7 +166,7 @@ __device__ inline void leg_z_store(const double* t2, double* out, const double* 
 // dst[parent] += sum over listed children of K(child); one workgroup per parent,
 // so the children are added in a fixed order (deterministic).
 template<int N>
-__global__ __launch_bounds__(kThreads) void k_transfer_gather(
+__global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
     const double* __restrict__ mats)
 {
@@ -171,7 +175,7 @@ __global__ __launch_bounds__(kThreads) void k_transfer_gather(
     const int e = blockIdx.x;
     if (e >= count) return;
     const int* ent = entries + 9 * e;
-    for (int q = threadIdx.x; q < 2 * N * N; q += kThreads) (&L.m[0][0])[q] = mats[q];
+    for (int q = threadIdx.x; q < 2 * N * N; q += kTransferThreads) (&L.m[0][0])[q] = mats[q];
     double* out = dst + (size_t)ent[0] * N3;
     for (int c = 0; c < 8; ++c)
     {
@@ -184,7 +188,7 @@ __global__ __launch_bounds__(kThreads) void k_transfer_gather(
 }
 
 template<int N>
-__global__ __launch_bounds__(kThreads) void k_transfer_scatter(
+__global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
     const double* __restrict__ mats, int accumulate)
 {
@@ -193,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void k_transfer_scatter(
     const int e = blockIdx.x;
     if (e >= count) return;
     const int child = entries[3 * e], parent = entries[3 * e + 1], c = entries[3 * e + 2];
-    for (int q = threadIdx.x; q < 2 * N * N; q += kThreads) (&L.m[0][0])[q] = mats[q];
+    for (int q = threadIdx.x; q < 2 * N * N; q += kTransferThreads) (&L.m[0][0])[q] = mats[q];
     __syncthreads();
     transfer_xy<N>(src + (size_t)parent * N3, L, c & 1, (c >> 1) & 1);
     leg_z_store<N>(L.buf, dst + (size_t)child * N3, L.m[(c >> 2) & 1], accumulate != 0);
@@ -281,13 +285,13 @@ struct DftPlan
 {
     static constexpr int P = 2 * N - 1;
     static constexpr int H = P / 2 + 1;
-    static constexpr int CS = 16;  // double2 intermediates
     static constexpr int kLds = 163840 - 4096;
-    static constexpr int kFwdRaw = (kLds - N * N * H * CS - P * 16) / (N * P * CS);
-    static constexpr int FWD_GS = kFwdRaw < H ? kFwdRaw : H;
-    static constexpr int kInvRaw = (kLds - H * N * N * CS - P * 16) / (P * N * CS);
-    static constexpr int INV_GS = kInvRaw < H ? kInvRaw : H;
+    // kx-planes per group: the twiddle table (P*N complex) plus, per plane,
+    // one n x n and one n x P complex slab of intermediates.
+    static constexpr int kRaw = (kLds - P * N * 16) / (N * N * 16 + N * P * 16);
+    static constexpr int GS = kRaw < H ? kRaw : H;
 };
+constexpr int kDftThreads = 384;
 
 // a * (c - i s)
 template<class C, class T>
@@ -304,104 +308,93 @@ __device__ inline void cmac(C a, C w, T& sr, T& si)
     si = fma(a.x, w.y, fma(a.y, w.x, si));
 }
 
-// Forward r2c of an n^3 block zero-padded to P^3 (pruned legs): x leg into LDS,
-// then y and z legs per group of kx planes; output spec[kx][ky][kz].
-// Arithmetic is FP64 on both paths; T is only the storage type of the
-// intermediates in LDS and of the spectrum (FP32 storage rounds each leg's
-// output once, the legs' sums keep FP64 accuracy).
+// Forward r2c of an n^3 block zero-padded to P^3 (pruned legs), in groups of
+// GS kx-planes: the x leg for the group's planes (the block's n^2 x-lines are
+// re-read from L2 per group), then the y leg (outputs split over two threads
+// per line) and the z leg to global memory; output spec[kx][ky][kz].  The
+// twiddles come from a table T[k][j] = w^(j k) so the inner loops carry no
+// modular index arithmetic.  Arithmetic and LDS intermediates are FP64; the
+// spectrum is stored as T (double: plain FP64 path; float: mixed path).
 template<int N, class T>
-__global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restrict__ field,
+__global__ __launch_bounds__(kDftThreads) void k_forward_dft(const double* __restrict__ field,
     const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec)
 {
-    using C = double2;  // LDS intermediates
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::FWD_GS;
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
     static_assert(GS >= 1, "LDS too small for one plane");
-    __shared__ C       X1[N * N * H];  // [z][y][kx]
-    __shared__ C       Tp[GS * N * P];  // [g][z][ky]
-    __shared__ double2 W[P];
+    __shared__ double2 Tw[P * N];        // [k][j]
+    __shared__ double2 X1[GS * N * N];   // [g][z][y]
+    __shared__ double2 Tp[GS * N * P];   // [g][z][ky]
     const int b = blockIdx.x;
     if (b >= count) return;
     const int     slot = slots[b];
     const double* in = field + (size_t)slot * G::N3;
-    twiddles(W, P);
-    __syncthreads();
-    for (int l = threadIdx.x; l < N * N; l += kThreads)
+    for (int q = threadIdx.x; q < P * N; q += kDftThreads)
     {
-        double v[N];
-#pragma unroll
-        for (int x = 0; x < N; ++x) v[x] = in[l * N + x];
-        for (int kx = 0; kx < H; ++kx)
-        {
-            double sr = 0, si = 0;
-            int    m = 0;
-#pragma unroll
-            for (int x = 0; x < N; ++x)
-            {
-                const double2 w = W[m];
-                sr = fma(v[x], w.x, sr);
-                si = fma(-v[x], w.y, si);
-                m += kx;
-                if (m >= P) m -= P;
-            }
-            X1[l * H + kx] = make_double2(sr, si);
-        }
+        const int k = q / N, j = q - k * N;
+        double    sn, cs;
+        sincospi(2.0 * ((j * k) % P) / P, &sn, &cs);
+        Tw[q] = make_double2(cs, sn);
     }
     __syncthreads();
     typename Cx<T>::type* out = spec + (size_t)slot * G::NF;
-    constexpr int         K0 = (P + 1) / 2;  // y-leg outputs split over two threads per line
+    constexpr int         K0 = (P + 1) / 2;
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
         const int gs = min(GS, H - kx0);
-        for (int l = threadIdx.x; l < 2 * gs * N; l += kThreads)
+        // x leg: line l = (z, y), real input -> gs complex outputs
+        for (int l = threadIdx.x; l < N * N; l += kDftThreads)
+        {
+            double v[N];
+#pragma unroll
+            for (int x = 0; x < N; ++x) v[x] = in[l * N + x];
+            for (int g = 0; g < gs; ++g)
+            {
+                const double2* w = Tw + (kx0 + g) * N;
+                double         sr = 0, si = 0;
+#pragma unroll
+                for (int x = 0; x < N; ++x)
+                {
+                    sr = fma(v[x], w[x].x, sr);
+                    si = fma(-v[x], w[x].y, si);
+                }
+                X1[g * N * N + l] = make_double2(sr, si);
+            }
+        }
+        __syncthreads();
+        // y leg: line (g, z), outputs ky split over two threads
+        for (int l = threadIdx.x; l < 2 * gs * N; l += kDftThreads)
         {
             const int half = l / (gs * N), line = l - half * gs * N;
-            const int g = line / N, z = line - g * N, kx = kx0 + g;
+            const int g = line / N, z = line - g * N;
             double2   a[N];
 #pragma unroll
-            for (int y = 0; y < N; ++y)
-            {
-                const C c = X1[(z * N + y) * H + kx];
-                a[y] = make_double2(c.x, c.y);
-            }
+            for (int y = 0; y < N; ++y) a[y] = X1[g * N * N + z * N + y];
             const int k_lo = half ? K0 : 0, k_hi = half ? P : K0;
             for (int ky = k_lo; ky < k_hi; ++ky)
             {
-                double sr = 0, si = 0;
-                int    m = 0;
+                const double2* w = Tw + ky * N;
+                double         sr = 0, si = 0;
 #pragma unroll
-                for (int y = 0; y < N; ++y)
-                {
-                    cmac_conj(a[y], W[m], sr, si);
-                    m += ky;
-                    if (m >= P) m -= P;
-                }
+                for (int y = 0; y < N; ++y) cmac_conj(a[y], w[y], sr, si);
                 Tp[(g * N + z) * P + ky] = make_double2(sr, si);
             }
         }
         __syncthreads();
-        for (int l = threadIdx.x; l < gs * P; l += kThreads)
+        // z leg: line (g, ky) -> P outputs kz, to global memory
+        for (int l = threadIdx.x; l < gs * P; l += kDftThreads)
         {
             const int g = l / P, ky = l - g * P, kx = kx0 + g;
             double2   a[N];
 #pragma unroll
-            for (int z = 0; z < N; ++z)
-            {
-                const C c = Tp[(g * N + z) * P + ky];
-                a[z] = make_double2(c.x, c.y);
-            }
+            for (int z = 0; z < N; ++z) a[z] = Tp[(g * N + z) * P + ky];
             typename Cx<T>::type* o = out + (size_t)(kx * P + ky) * P;
             for (int kz = 0; kz < P; ++kz)
             {
-                double sr = 0, si = 0;
-                int    m = 0;
+                const double2* w = Tw + kz * N;
+                double         sr = 0, si = 0;
 #pragma unroll
-                for (int z = 0; z < N; ++z)
-                {
-                    cmac_conj(a[z], W[m], sr, si);
-                    m += kz;
-                    if (m >= P) m -= P;
-                }
+                for (int z = 0; z < N; ++z) cmac_conj(a[z], w[z], sr, si);
                 o[kz] = Cx<T>::make((T)sr, (T)si);
             }
         }
@@ -410,110 +403,99 @@ __global__ __launch_bounds__(kThreads) void k_forward_dft(const double* __restri
 }
 
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
-// added into the (FP64) target block.  FP64 throughout.
+// added into the target block.  Per group of GS kx-planes: z leg from global
+// memory, y leg in LDS, and the group's share of the c2r x leg accumulated in
+// registers (a thread owns one output x-line).  Twiddles come from a table
+// I[o][k] = w^((n-1+o) k).  FP64 throughout.
 template<int N>
-__global__ __launch_bounds__(kThreads) void k_inverse_dft(const double2* __restrict__ spec,
+__global__ __launch_bounds__(kDftThreads) void k_inverse_dft(const double2* __restrict__ spec,
     const int* __restrict__ slots, int count, double* __restrict__ field, double scale)
 {
-    using C = double2;
     using G = Geo<N>;
-    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::INV_GS;
+    constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
     static_assert(GS >= 1, "LDS too small for one plane");
-    __shared__ C       V[H * N * N];   // [kx][z][y]
-    __shared__ C       U[GS * P * N];  // [g][ky][z]
-    __shared__ double2 W[P];
+    static_assert(N * N <= kDftThreads, "one output line per thread");
+    __shared__ double2 Iw[N * P];       // [o][k]
+    __shared__ double2 U[GS * P * N];   // [g][ky][z]
+    __shared__ double2 V[GS * N * N];   // [g][z][y]
     const int b = blockIdx.x;
     if (b >= count) return;
-    const int slot = slots[b];
-    const C*  in = spec + (size_t)slot * G::NF;
-    twiddles(W, P);
+    const int      slot = slots[b];
+    const double2* in = spec + (size_t)slot * G::NF;
+    for (int q = threadIdx.x; q < N * P; q += kDftThreads)
+    {
+        const int o = q / P, k = q - o * P;
+        double    sn, cs;
+        sincospi(2.0 * (((N - 1 + o) * k) % P) / P, &sn, &cs);
+        Iw[q] = make_double2(cs, sn);
+    }
+    double acc[N];
+#pragma unroll
+    for (int x = 0; x < N; ++x) acc[x] = 0.0;
     __syncthreads();
+    constexpr int Y0 = (N + 1) / 2;
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
         const int gs = min(GS, H - kx0);
         // z leg: (g, ky) rows of P frequencies -> N window points
-        for (int l = threadIdx.x; l < gs * P; l += kThreads)
+        for (int l = threadIdx.x; l < gs * P; l += kDftThreads)
         {
-            const int g = l / P, ky = l - g * P, kx = kx0 + g;
-            const C*  row = in + (size_t)(kx * P + ky) * P;
-            double2   a[P];
+            const int      g = l / P, ky = l - g * P, kx = kx0 + g;
+            const double2* row = in + (size_t)(kx * P + ky) * P;
+            double2        a[P];
 #pragma unroll
-            for (int kz = 0; kz < P; ++kz)
-            {
-                const C c = row[kz];
-                a[kz] = make_double2(c.x, c.y);
-            }
+            for (int kz = 0; kz < P; ++kz) a[kz] = row[kz];
             for (int z = 0; z < N; ++z)
             {
-                const int step = (N - 1 + z) % P;
-                double    sr = 0, si = 0;
-                int       m = 0;
+                const double2* w = Iw + z * P;
+                double         sr = 0, si = 0;
 #pragma unroll
-                for (int kz = 0; kz < P; ++kz)
-                {
-                    cmac(a[kz], W[m], sr, si);
-                    m += step;
-                    if (m >= P) m -= P;
-                }
+                for (int kz = 0; kz < P; ++kz) cmac(a[kz], w[kz], sr, si);
                 U[(g * P + ky) * N + z] = make_double2(sr, si);
             }
         }
         __syncthreads();
-        // y leg: (g, z) lines of P frequencies -> N window points
-        for (int l = threadIdx.x; l < gs * N; l += kThreads)
+        // y leg: (g, z) lines of P frequencies -> N window points, outputs split over two threads
+        for (int l = threadIdx.x; l < 2 * gs * N; l += kDftThreads)
         {
-            const int g = l / N, z = l - g * N, kx = kx0 + g;
+            const int half = l / (gs * N), line = l - half * gs * N;
+            const int g = line / N, z = line - g * N;
             double2   a[P];
 #pragma unroll
-            for (int ky = 0; ky < P; ++ky)
+            for (int ky = 0; ky < P; ++ky) a[ky] = U[(g * P + ky) * N + z];
+            const int y_lo = half ? Y0 : 0, y_hi = half ? N : Y0;
+            for (int y = y_lo; y < y_hi; ++y)
             {
-                const C c = U[(g * P + ky) * N + z];
-                a[ky] = make_double2(c.x, c.y);
-            }
-            for (int y = 0; y < N; ++y)
-            {
-                const int step = (N - 1 + y) % P;
-                double    sr = 0, si = 0;
-                int       m = 0;
+                const double2* w = Iw + y * P;
+                double         sr = 0, si = 0;
 #pragma unroll
-                for (int ky = 0; ky < P; ++ky)
-                {
-                    cmac(a[ky], W[m], sr, si);
-                    m += step;
-                    if (m >= P) m -= P;
-                }
-                V[(kx * N + z) * N + y] = make_double2(sr, si);
+                for (int ky = 0; ky < P; ++ky) cmac(a[ky], w[ky], sr, si);
+                V[(g * N + z) * N + y] = make_double2(sr, si);
             }
         }
         __syncthreads();
-    }
-    // x leg, c2r: (z, y) lines of H half-spectrum values -> N window points
-    double* out = field + (size_t)slot * G::N3;
-    for (int l = threadIdx.x; l < N * N; l += kThreads)
-    {
-        double2 a[H];
-#pragma unroll
-        for (int kx = 0; kx < H; ++kx)
-        {
-            const C c = V[kx * N * N + l];
-            a[kx] = make_double2(c.x, c.y);
-        }
-        for (int x = 0; x < N; ++x)
-        {
-            const int step = (N - 1 + x) % P;
-            double    s = 0;
-            int       m = 0;
-#pragma unroll
-            for (int kx = 0; kx < H; ++kx)
+        // the group's part of the c2r x leg: line l = (z, y) -> N window points
+        const int l = threadIdx.x;
+        if (l < N * N)
+            for (int g = 0; g < gs; ++g)
             {
-                const double2 w = W[m];
+                const int     kx = kx0 + g;
+                const double2 a = V[g * N * N + l];
                 const double  c = (kx == 0) ? 1.0 : 2.0;  // P odd: no Nyquist plane
-                s = fma(c, fma(a[kx].x, w.x, -a[kx].y * w.y), s);
-                m += step;
-                if (m >= P) m -= P;
+#pragma unroll
+                for (int x = 0; x < N; ++x)
+                {
+                    const double2 w = Iw[x * P + kx];
+                    acc[x] = fma(c, fma(a.x, w.x, -a.y * w.y), acc[x]);
+                }
             }
-            out[l * N + x] += scale * s;
-        }
+        __syncthreads();
+    }
+    if (threadIdx.x < N * N)
+    {
+        double* out = field + (size_t)slot * G::N3 + threadIdx.x * N;
+#pragma unroll
+        for (int x = 0; x < N; ++x) out[x] += scale * acc[x];
     }
 }
 
@@ -538,7 +520,7 @@ __device__ inline void cmul_acc(float2 g, float2 S, double2& acc)
 }
 
 template<int KF, class C>
-__device__ inline void mac_entry(const C* Gs, int f, int meta, C S, double2 (&acc)[8])
+__device__ inline void mac_entry(const C* Gs, const short* omap, int f, int meta, C S, double2 (&acc)[8])
 {
     const int r = meta & 255, tm = meta >> 8;
     const int rx = r % 6, ry = (r / 6) % 6, rz = r / 36;
@@ -549,26 +531,36 @@ __device__ inline void mac_entry(const C* Gs, int f, int meta, C S, double2 (&ac
         if (tm & (1 << t))
         {
             const int oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
-            cmul_acc(Gs[oi * KF + f], S, acc[t]);
+            cmul_acc(Gs[omap[oi] * KF + f], S, acc[t]);
         }
     }
 }
 
+// One launch covers the sibling groups of one target level.  A workgroup owns
+// KF frequencies: it loads the kernel spectra of the offsets this launch uses
+// (olist, nused; omap maps an offset 0..342 to its row) into LDS and walks
+// every group; a thread keeps the 8 target accumulators of one group at one
+// frequency.  Entry loads are software-pipelined: the next eight entries and
+// their source values are in flight while the current eight are multiplied.
 template<int N, int KF, class T>
 __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
-    const MacSrc* __restrict__ srcs, const typename Cx<T>::type* __restrict__ ghat,
-    const typename Cx<T>::type* __restrict__ sspec, double2* __restrict__ tspec)
+    const MacSrc* __restrict__ srcs, const int* __restrict__ olist, int nused, const short* __restrict__ omap,
+    const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
+    double2* __restrict__ tspec)
 {
     using C = typename Cx<T>::type;
     constexpr int NF = Geo<N>::NF;
     constexpr int GPW = kThreads / KF;
-    __shared__ C Gs[343 * KF];
+    constexpr int U = 8;
+    __shared__ C     Gs[343 * KF];
+    __shared__ short Om[343];
     const int f0 = blockIdx.x * KF;
-    for (int q = threadIdx.x; q < 343 * KF; q += kThreads)
+    for (int q = threadIdx.x; q < nused * KF; q += kThreads)
     {
-        const int o = q / KF, f = q - o * KF;
-        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
+        const int c = q / KF, f = q - c * KF;
+        Gs[q] = (f0 + f < NF) ? ghat[(size_t)olist[c] * NF + f0 + f] : Cx<T>::make(0, 0);
     }
+    for (int q = threadIdx.x; q < 343; q += kThreads) Om[q] = omap[q];
     __syncthreads();
     const int f = threadIdx.x % KF;
     const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
@@ -581,25 +573,42 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
         double2        acc[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
-        int       e = grp.begin;
-        const int end = grp.begin + grp.count;
-        // eight source loads in flight before their use
-        for (; e + 8 <= end; e += 8)
+        const MacSrc* ep = srcs + grp.begin;
+        const int     cnt = grp.count;
+        int           e = 0;
+        if (cnt >= U)
         {
-            MacSrc s[8];
-            C      S[8];
+            MacSrc m[U];
+            C      S[U];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) s[i] = srcs[e + i];
+            for (int i = 0; i < U; ++i) m[i] = ep[i];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) S[i] = sp[(size_t)s[i].slot * NF];
+            for (int i = 0; i < U; ++i) S[i] = sp[(size_t)m[i].slot * NF];
+            for (; e + 2 * U <= cnt; e += U)
+            {
+                MacSrc mn[U];
+                C      Sn[U];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) mac_entry<KF, C>(Gs, f, s[i].meta, S[i], acc);
+                for (int i = 0; i < U; ++i) mn[i] = ep[e + U + i];
+#pragma unroll
+                for (int i = 0; i < U; ++i) Sn[i] = sp[(size_t)mn[i].slot * NF];
+#pragma unroll
+                for (int i = 0; i < U; ++i) mac_entry<KF, C>(Gs, Om, f, m[i].meta, S[i], acc);
+#pragma unroll
+                for (int i = 0; i < U; ++i)
+                {
+                    m[i] = mn[i];
+                    S[i] = Sn[i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < U; ++i) mac_entry<KF, C>(Gs, Om, f, m[i].meta, S[i], acc);
+            e += U;
         }
-        for (; e < end; ++e)
+        for (; e < cnt; ++e)
         {
-            const MacSrc s0 = srcs[e];
-            const C      S0 = sp[(size_t)s0.slot * NF];
-            mac_entry<KF, C>(Gs, f, s0.meta, S0, acc);
+            const MacSrc s0 = ep[e];
+            mac_entry<KF, C>(Gs, Om, f, s0.meta, sp[(size_t)s0.slot * NF], acc);
         }
         if (f0 + f < NF)
         {
@@ -794,14 +803,14 @@ template<int N> struct RunGather
 {
     static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m)
     {
-        hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count), dim3(kThreads), 0, s, src, dst, e, count, m);
+        hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count), dim3(kTransferThreads), 0, s, src, dst, e, count, m);
     }
 };
 template<int N> struct RunScatter
 {
     static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m, int acc)
     {
-        hipLaunchKernelGGL(k_transfer_scatter<N>, dim3(count), dim3(kThreads), 0, s, src, dst, e, count, m, acc);
+        hipLaunchKernelGGL(k_transfer_scatter<N>, dim3(count), dim3(kTransferThreads), 0, s, src, dst, e, count, m, acc);
     }
 };
 template<int N> struct RunHalo
@@ -823,10 +832,10 @@ template<int N> struct RunFwd
     static void run(hipStream_t s, int prec, const double* f, const int* slots, int count, void* spec)
     {
         if (prec == 0)
-            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count), dim3(kThreads), 0, s, f, slots, count,
+            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count), dim3(kDftThreads), 0, s, f, slots, count,
                 (double2*)spec);
         else
-            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count), dim3(kThreads), 0, s, f, slots, count,
+            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count), dim3(kDftThreads), 0, s, f, slots, count,
                 (float2*)spec);
     }
 };
@@ -835,28 +844,28 @@ template<int N> struct RunInv
     static void run(hipStream_t s, int prec, const void* spec, const int* slots, int count, double* f, double scale)
     {
         (void)prec;  // target spectra are FP64 on both paths
-        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count), dim3(kThreads), 0, s, (const double2*)spec, slots, count,
+        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count), dim3(kDftThreads), 0, s, (const double2*)spec, slots, count,
             f, scale);
     }
 };
 template<int N> struct RunMac
 {
-    static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const void* ghat,
-        const void* ss, void* ts)
+    static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const int* olist,
+        int nused, const short* omap, const void* ghat, const void* ss, void* ts)
     {
         if (prec == 0)
         {
             constexpr int  KF = 8;
             const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
             hipLaunchKernelGGL((k_spectral_mac<N, KF, double>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src,
-                (const double2*)ghat, (const double2*)ss, (double2*)ts);
+                olist, nused, omap, (const double2*)ghat, (const double2*)ss, (double2*)ts);
         }
         else
         {
             constexpr int  KF = 16;
             const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
             hipLaunchKernelGGL((k_spectral_mac<N, KF, float>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src,
-                (const float2*)ghat, (const float2*)ss, (double2*)ts);
+                olist, nused, omap, (const float2*)ghat, (const float2*)ss, (double2*)ts);
         }
     }
 };
@@ -922,9 +931,10 @@ void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* sl
     if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale);
 }
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const void* ghat, const void* sspec, void* tspec)
+    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec)
 {
-    if (ngroups > 0) dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, ghat, sspec, tspec);
+    if (nused < 1 || nused > 343) throw std::runtime_error("offset count out of range");
+    if (ngroups > 0) dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, olist, nused, omap, ghat, sspec, tspec);
 }
 void to_float2(hipStream_t s, const double2* in, float2* out, long n)
 {

@@ -59,8 +59,11 @@ void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* 
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec);
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale);
+// olist[nused]: the offsets (0..342, o = (ox+3) + 7 (oy+3) + 49 (oz+3)) the
+// launch's entries use; omap[343]: offset -> row in olist (rows of unused
+// offsets are never read).
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const void* ghat, const void* sspec, void* tspec);
+    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec);
 void to_float2(hipStream_t s, const double2* in, float2* out, long n);
 
 // Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:
