@@ -68,13 +68,12 @@ def cpu_baseline(wl, counts, budget_s):
     }
 
 
-def learned_fcg(seed=1004, alpha=0.5, tol=1e-8):
+def learned_fcg(seed=1004, alpha=0.5, tol=1e-6):
     """IB force solve on the sphere workload (seeded N in 5000..20000, 3-level
     16^3-block mesh, roma delta).  Plain CG (the reference's algorithm) runs to
-    the reference's threshold 1e-3 and, capped, towards 1e-8, where it stalls
-    because the FMM-approximated operator is symmetric only to ~1e-4.  The
-    flexible conjugate-residual solver runs to 1e-8 with the identity and
-    with the MLP trained on the device from the iterates of earlier solves."""
+    the reference's threshold 1e-3 and, capped, towards tol.  The flexible
+    conjugate-residual solver runs to tol with the identity and with the MLP
+    trained on the device from the iterates of earlier solves."""
     import numpy as np
 
     from iblgf_amd import IbSystem, LearnedPreconditioner, PoissonSolver, setups
@@ -87,54 +86,60 @@ def learned_fcg(seed=1004, alpha=0.5, tol=1e-8):
     solver = PoissonSolver(s.tree)
     sysm = IbSystem(solver)
     sysm.apply(s.uc, alpha)  # kernel spectra (cold-path work)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        sysm.apply(s.uc, alpha)
+    t_apply = (time.perf_counter() - t0) / 5
 
     def true_res(f):
         return float(np.linalg.norm(s.uc - sysm.apply(f, alpha)) / np.linalg.norm(f))
 
-    out = {}
-    f, it, res, _ = sysm.cg(s.uc, alpha, tol=1e-3, maxit=2000)
+    out = {"apply_ms": t_apply * 1e3}
+    f, it, res, _ = sysm.cg(s.uc, alpha, tol=1e-3, maxit=200)
     out["cg_to_1e-3"] = {"iterations": it, "seconds": sysm.stats()[0], "residual": res, "true_residual": true_res(f)}
     log(f"CG to 1e-3: {it} iterations")
-    f, it, res, _ = sysm.cg(s.uc, alpha, tol=tol, maxit=600)
-    out["cg_to_1e-8"] = {"iterations": it, "seconds": sysm.stats()[0], "residual": res, "stop": sysm.stats()[3]}
-    log(f"CG towards 1e-8: {it} iterations, residual {res:.2e}")
-    f_id, it_id, res_id, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=5000, use_mlp=False)
+    f_cg, it, res, _ = sysm.cg(s.uc, alpha, tol=tol, maxit=1500)
+    out["cg_to_tol"] = {"iterations": it, "seconds": sysm.stats()[0], "residual": res, "stop": sysm.stats()[3]}
+    log(f"CG towards {tol}: {it} iterations, residual {res:.2e}")
+    f_id, it_id, res_id, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500, use_mlp=False)
     t_id = sysm.stats()[0]
-    out["flexible_identity"] = {"iterations": it_id, "seconds": t_id, "residual": res_id, "true_residual": true_res(f_id)}
-    log(f"flexible (identity) to 1e-8: {it_id} iterations")
+    out["flexible_identity"] = {"iterations": it_id, "seconds": t_id, "residual": res_id,
+                                "true_residual": true_res(f_id), "stop": sysm.stats()[3]}
+    log(f"flexible (identity) to {tol}: {it_id} iterations")
     pc = LearnedPreconditioner(sysm, K=16, hidden=128, seed=seed)
     pc.collect(stride=3)
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
-    nsolves = 4
-    for k in range(nsolves):  # earlier solves: streams in random directions and rough forcings
+    nsolves = 2
+    for k in range(nsolves):  # earlier solves: a stream in a random direction and a rough forcing
         if k % 2 == 0:
             d = rng.standard_normal(3)
             b = np.tile(d / np.linalg.norm(d), (s.npoints, 1))
         else:
             b = rng.standard_normal((s.npoints, 3))
-        sysm.fcg(b, alpha, tol=1e-9, maxit=5000, use_mlp=False)
+        sysm.fcg(b, alpha, tol=tol / 10, maxit=1000, use_mlp=False)
         log(f"earlier solve {k + 1}/{nsolves} done")
     t_collect = time.perf_counter() - t0
     pc.collect(0)
     t0 = time.perf_counter()
-    loss, npairs = pc.train(steps=3000, batch_rows=4096, lr=1e-3)
+    loss0, npairs = pc.train(steps=10, batch_rows=4096, lr=1e-3)
+    loss, npairs = pc.train(steps=2000, batch_rows=4096, lr=1e-3)
     t_train = time.perf_counter() - t0
-    log(f"trained on {npairs} pairs in {t_train:.1f} s, loss {loss:.3e}")
-    f_l, it_l, res_l, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=5000)
+    log(f"trained on {npairs} pairs in {t_train:.1f} s, loss {loss0:.3e} -> {loss:.3e}")
+    f_l, it_l, res_l, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500)
     t_l, _, fell, stop = sysm.stats()
     out["flexible_learned"] = {"iterations": it_l, "seconds": t_l, "residual": res_l, "true_residual": true_res(f_l),
                                "fell_back": fell, "stop": stop}
-    log(f"flexible (learned) to 1e-8: {it_l} iterations")
+    log(f"flexible (learned) to {tol}: {it_l} iterations")
     out.update({
         "workload": (f"IB sphere R=0.5, N={s.npoints} seeded points (seed {seed}), roma delta, "
                      f"{s.info['nblocks']} blocks of 16^3 over 3 levels; A = E (I - G L^-1 D) H_alpha E^T, "
-                     f"alpha={alpha}; rhs u=(1,0,0); stop at sqrt(r.r/f.f) < tol"),
+                     f"alpha={alpha}; rhs u=(1,0,0); stop at sqrt(r.r/f.f) < {tol}"),
+        "flexible_vs_cg_iterations": out["cg_to_tol"]["iterations"] / max(it_id, 1),
         "learned_iteration_gain": it_id / max(it_l, 1),
         "learned_wall_gain": t_id / max(t_l, 1e-12),
-        "solution_rel_diff": float(np.linalg.norm(f_l - f_id) / np.linalg.norm(f_id)),
         "training": {"earlier_solves": nsolves, "collect_seconds": t_collect, "pairs": npairs,
-                     "train_steps": 3000, "train_seconds": t_train, "final_loss": loss,
+                     "train_steps": 2010, "train_seconds": t_train, "loss_start": loss0, "loss_end": loss,
                      "network": "3-layer MLP, hidden 128, tanh, BF16 weights, FP32 accumulation, K=16 neighbours"},
     })
     return out

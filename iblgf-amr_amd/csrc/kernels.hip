@@ -519,7 +519,7 @@ __device__ inline void cmul_acc(float2 g, float2 S, double2& acc)
     acc.y += (double)fmaf(g.x, S.y, g.y * S.x);
 }
 
-template<int KF, class C>
+template<int KF, bool COMPACT, class C>
 __device__ inline void mac_entry(const C* Gs, const short* omap, int f, int meta, C S, double2 (&acc)[8])
 {
     const int r = meta & 255, tm = meta >> 8;
@@ -531,18 +531,18 @@ __device__ inline void mac_entry(const C* Gs, const short* omap, int f, int meta
         if (tm & (1 << t))
         {
             const int oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
-            cmul_acc(Gs[omap[oi] * KF + f], S, acc[t]);
+            cmul_acc(Gs[(COMPACT ? omap[oi] : oi) * KF + f], S, acc[t]);
         }
     }
 }
 
 // One launch covers the sibling groups of one target level.  A workgroup owns
-// KF frequencies: it loads the kernel spectra of the offsets this launch uses
-// (olist, nused; omap maps an offset 0..342 to its row) into LDS and walks
-// every group; a thread keeps the 8 target accumulators of one group at one
-// frequency.  Entry loads are software-pipelined: the next eight entries and
-// their source values are in flight while the current eight are multiplied.
-template<int N, int KF, class T>
+// KF frequencies: it loads kernel spectra into LDS and walks every group; a
+// thread keeps the 8 target accumulators of one group at one frequency.
+// COMPACT launches (few offsets used, the small coarse-level launches) load
+// only the rows in olist and map offsets through omap; the others load all
+// 343 rows and index them directly.
+template<int N, int KF, bool COMPACT, class T>
 __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
     const MacSrc* __restrict__ srcs, const int* __restrict__ olist, int nused, const short* __restrict__ omap,
     const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
@@ -551,16 +551,18 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
     using C = typename Cx<T>::type;
     constexpr int NF = Geo<N>::NF;
     constexpr int GPW = kThreads / KF;
-    constexpr int U = 8;
     __shared__ C     Gs[343 * KF];
-    __shared__ short Om[343];
+    __shared__ short Om[COMPACT ? 343 : 1];
     const int f0 = blockIdx.x * KF;
-    for (int q = threadIdx.x; q < nused * KF; q += kThreads)
+    const int rows = COMPACT ? nused : 343;
+    for (int q = threadIdx.x; q < rows * KF; q += kThreads)
     {
         const int c = q / KF, f = q - c * KF;
-        Gs[q] = (f0 + f < NF) ? ghat[(size_t)olist[c] * NF + f0 + f] : Cx<T>::make(0, 0);
+        const int o = COMPACT ? olist[c] : c;
+        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
     }
-    for (int q = threadIdx.x; q < 343; q += kThreads) Om[q] = omap[q];
+    if (COMPACT)
+        for (int q = threadIdx.x; q < 343; q += kThreads) Om[q] = omap[q];
     __syncthreads();
     const int f = threadIdx.x % KF;
     const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
@@ -573,42 +575,24 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
         double2        acc[8];
 #pragma unroll
         for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
-        const MacSrc* ep = srcs + grp.begin;
-        const int     cnt = grp.count;
-        int           e = 0;
-        if (cnt >= U)
+        int       e = grp.begin;
+        const int end = grp.begin + grp.count;
+        // eight source loads in flight before their use
+        for (; e + 8 <= end; e += 8)
         {
-            MacSrc m[U];
-            C      S[U];
+            MacSrc s[8];
+            C      S[8];
 #pragma unroll
-            for (int i = 0; i < U; ++i) m[i] = ep[i];
+            for (int i = 0; i < 8; ++i) s[i] = srcs[e + i];
 #pragma unroll
-            for (int i = 0; i < U; ++i) S[i] = sp[(size_t)m[i].slot * NF];
-            for (; e + 2 * U <= cnt; e += U)
-            {
-                MacSrc mn[U];
-                C      Sn[U];
+            for (int i = 0; i < 8; ++i) S[i] = sp[(size_t)s[i].slot * NF];
 #pragma unroll
-                for (int i = 0; i < U; ++i) mn[i] = ep[e + U + i];
-#pragma unroll
-                for (int i = 0; i < U; ++i) Sn[i] = sp[(size_t)mn[i].slot * NF];
-#pragma unroll
-                for (int i = 0; i < U; ++i) mac_entry<KF, C>(Gs, Om, f, m[i].meta, S[i], acc);
-#pragma unroll
-                for (int i = 0; i < U; ++i)
-                {
-                    m[i] = mn[i];
-                    S[i] = Sn[i];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < U; ++i) mac_entry<KF, C>(Gs, Om, f, m[i].meta, S[i], acc);
-            e += U;
+            for (int i = 0; i < 8; ++i) mac_entry<KF, COMPACT, C>(Gs, Om, f, s[i].meta, S[i], acc);
         }
-        for (; e < cnt; ++e)
+        for (; e < end; ++e)
         {
-            const MacSrc s0 = ep[e];
-            mac_entry<KF, C>(Gs, Om, f, s0.meta, sp[(size_t)s0.slot * NF], acc);
+            const MacSrc s0 = srcs[e];
+            mac_entry<KF, COMPACT, C>(Gs, Om, f, s0.meta, sp[(size_t)s0.slot * NF], acc);
         }
         if (f0 + f < NF)
         {
@@ -853,20 +837,23 @@ template<int N> struct RunMac
     static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const int* olist,
         int nused, const short* omap, const void* ghat, const void* ss, void* ts)
     {
+        const bool compact = nused <= 200;
+#define IBLGF_MAC(KF, CP, T, C)                                                                                  \
+    hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3((Geo<N>::NF + KF - 1) / KF, 1), dim3(kThreads), 0, s, \
+        g, ng, src, olist, nused, omap, (const C*)ghat, (const C*)ss, (double2*)ts)
         if (prec == 0)
         {
-            constexpr int  KF = 8;
-            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
-            hipLaunchKernelGGL((k_spectral_mac<N, KF, double>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src,
-                olist, nused, omap, (const double2*)ghat, (const double2*)ss, (double2*)ts);
+            if (compact) IBLGF_MAC(8, true, double, double2);
+            else
+                IBLGF_MAC(8, false, double, double2);
         }
         else
         {
-            constexpr int  KF = 16;
-            const unsigned nblk = (Geo<N>::NF + KF - 1) / KF;
-            hipLaunchKernelGGL((k_spectral_mac<N, KF, float>), dim3(nblk, 1), dim3(kThreads), 0, s, g, ng, src,
-                olist, nused, omap, (const float2*)ghat, (const float2*)ss, (double2*)ts);
+            if (compact) IBLGF_MAC(16, true, float, float2);
+            else
+                IBLGF_MAC(16, false, float, float2);
         }
+#undef IBLGF_MAC
     }
 };
 template<int N> struct RunLgf
