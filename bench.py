@@ -5,8 +5,12 @@ Workload: the adaptive vortex-ring octree (8^3 base blocks of 16^3 cells plus
 3 refinement levels, 16 seeded Gaussian-core rings, seed 1001).  One step is
 one vector Poisson solve: PoissonSolver::apply_lgf over the three vorticity
 components, i.e. three scalar LGF solves over the whole tree, all levels.
-Inputs are resident in HBM before the timed region; kernel spectra are built
-during warm-up (the reference times its second, warm solve).
+Four ways are timed: the plain FP64 path one component at a time, the FP64
+and mixed paths with the three components batched through every launch, and
+the mixed path one component at a time.  The headline is the batched mixed
+path when all its components are within relative L2 1e-6 of the plain FP64
+result.  Inputs are resident in HBM before the timed region; kernel spectra
+are built during warm-up (the reference times its second, warm solve).
 
 Prints one JSON line with the throughput (solves/s), the roofline of the
 dominant kernel class (timed live with HIP events on the solver's stream
@@ -153,6 +157,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="skip the learned-FCG immersed-boundary comparison")
+    ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched"],
+                    help="time one way only and print its rate (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -175,11 +181,15 @@ def main():
     uf = unit_flops(n)
     stream = solver.stream()
 
-    def step():
+    def step_single():
         for c in range(3):
             solver.apply_lgf(fin[c], fout[c])
 
-    def timed(precision):
+    def step_batched():
+        solver.apply_lgf_batch(fin, fout)
+
+    def timed(precision, batched):
+        step = step_batched if batched else step_single
         solver.set_precision(precision)
         solver.prepare()
         for _ in range(args.warmup):
@@ -201,15 +211,38 @@ def main():
         outs = [solver.download(f) for f in fout]
         return dict(wall=t1 - t0, gpu_ms=ev0.elapsed_ms(ev1), kt=kt, outs=outs)
 
-    r64 = timed(PREC_FP64)
-    rmx = timed(PREC_MIXED)
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if args.only:
+        prec = PREC_MIXED if args.only.startswith("mixed") else PREC_FP64
+        r = timed(prec, args.only.endswith("batched"))
+        print(json.dumps({"only": args.only, "solves_per_s": 3 * args.steps / r["wall"],
+                          "ms_per_step": r["wall"] / args.steps * 1e3, "steps": args.steps}))
+        return 0
+
+    r64 = timed(PREC_FP64, False)
+    log(f"plain FP64, one component at a time: {3 * args.steps / r64['wall']:.2f} solves/s")
+    r64b = timed(PREC_FP64, True)
+    log(f"FP64, 3 components batched: {3 * args.steps / r64b['wall']:.2f} solves/s")
+    rmx = timed(PREC_MIXED, False)
+    log(f"mixed, one component at a time: {3 * args.steps / rmx['wall']:.2f} solves/s")
+    rmxb = timed(PREC_MIXED, True)
+    log(f"mixed, 3 components batched: {3 * args.steps / rmxb['wall']:.2f} solves/s")
     counts = solver.counts()
     leaves = (wl.arrays["flags"] & FLAG_LEAF) != 0
-    errors = []
-    for a, b in zip(rmx["outs"], r64["outs"]):
-        d = a[leaves] - b[leaves]
-        errors.append({"relL2": float(np.linalg.norm(d) / np.linalg.norm(b[leaves])),
-                       "relmax": float(np.max(np.abs(d)) / np.max(np.abs(b[leaves])))})
+
+    def errors_vs_fp64(r):
+        out = []
+        for a, b in zip(r["outs"], r64["outs"]):
+            d = a[leaves] - b[leaves]
+            out.append({"relL2": float(np.linalg.norm(d) / np.linalg.norm(b[leaves])),
+                        "relmax": float(np.max(np.abs(d)) / np.max(np.abs(b[leaves])))})
+        return out
+
+    errors = errors_vs_fp64(rmxb)
+    errors_single = errors_vs_fp64(rmx)
+    batched_fp64_identical = all(np.array_equal(a, b) for a, b in zip(r64b["outs"], r64["outs"]))
     mixed_ok = max(e["relL2"] for e in errors) <= 1e-6
     solves = 3 * args.steps
 
@@ -242,12 +275,15 @@ def main():
         }
 
     k64, roof64 = roofline_of(r64["kt"], PEAK_FP64_TFLOPS, "fp64")
+    k64b, roof64b = roofline_of(r64b["kt"], PEAK_FP64_TFLOPS, "fp64_batched")
     # the mixed MAC accumulates in FP64 and MI355X runs unpacked FP32 and FP64 FMA at one rate
-    kmx, roofmx = roofline_of(rmx["kt"], PEAK_FP64_TFLOPS, "mixed")
-    head = rmx if mixed_ok else r64
+    kmxb, roofmxb = roofline_of(rmxb["kt"], PEAK_FP64_TFLOPS, "mixed_batched")
+    head = rmxb if mixed_ok else r64b
     value = solves / head["wall"]
     v64 = solves / r64["wall"]
+    v64b = solves / r64b["wall"]
     vmx = solves / rmx["wall"]
+    vmxb = solves / rmxb["wall"]
     flops_per_solve = sum(counts[c] * uf[k] for c, k in (("n_src", "forward_dft"), ("n_tgt", "inverse_dft"),
                                                              ("n_pair", "spectral_mac"), ("n_transfer", "transfer")))
     leaf_cells = counts["n_leaf"] * wl.block_extent ** 3
@@ -266,7 +302,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": ("adaptive vortex-ring octree Poisson: 8^3 base blocks of 16^3 cells + 3 refinement levels, "
-                         "16 seeded Gaussian-core rings (seed 1001); one step = 3 vorticity components"),
+                         "16 seeded Gaussian-core rings (seed 1001); one step = 3 vorticity components, "
+                         "batched through every launch"),
             "precision": ("mixed: source and kernel spectra stored FP32, multiplied in FP32 and accumulated in "
                           "FP64 by the MAC; DFT arithmetic, target spectra, fields and level sums FP64"
                           if mixed_ok else "plain FP64"),
@@ -279,14 +316,20 @@ def main():
             "solve_tflops": flops_per_solve * value / 1e12,
             "gpu_event_ms_per_step": head["gpu_ms"] / args.steps,
             "fp64_solves_per_s": v64,
+            "fp64_batched_solves_per_s": v64b,
             "mixed_solves_per_s": vmx,
-            "mixed_vs_fp64_speedup": vmx / v64,
+            "mixed_batched_solves_per_s": vmxb,
+            "batched_mixed_vs_plain_fp64_speedup": vmxb / v64,
+            "batched_fp64_vs_plain_fp64_speedup": v64b / v64,
+            "batched_fp64_bitwise_equal_to_single": batched_fp64_identical,
             "mixed_error_vs_fp64_per_component": errors,
+            "mixed_single_error_vs_fp64_per_component": errors_single,
             "fp64_roofline": roof64,
             "fp64_kernels": k64,
+            "fp64_batched_kernels": k64b,
         },
-        "roofline": roofmx if mixed_ok else roof64,
-        "kernels": kmx if mixed_ok else k64,
+        "roofline": roofmxb if mixed_ok else roof64b,
+        "kernels": kmxb if mixed_ok else k64b,
         "cpu_baseline": None,
     }
     if not args.no_ib:

@@ -409,6 +409,26 @@ int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int ke
     });
 }
 
+int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
+    double alpha_base)
+{
+    NEED_CTX("iblgf_gpu_apply_lgf_batch");
+    if (kernel != IBLGF_KERNEL_LAPLACE && kernel != IBLGF_KERNEL_IF)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_batch: unknown kernel", ctx);
+    if (!src_fields || !dst_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_batch: null field list", ctx);
+    if (k < 1 || k > iblgf_amd::GpuSolver::kMaxComponents)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_batch: component count out of range", ctx);
+    for (int c = 0; c < k; ++c)
+        for (int e = 0; e < k; ++e)
+            if (src_fields[c] == dst_fields[e] || (c != e && dst_fields[c] == dst_fields[e]))
+                return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_batch: fields overlap", ctx);
+    return guarded(ctx, [&] {
+        if (kernel == IBLGF_KERNEL_LAPLACE) ctx->solver->apply_lgf_batch(src_fields, dst_fields, k);
+        else
+            ctx->solver->apply_if_batch(src_fields, dst_fields, k, alpha_base);
+    });
+}
+
 int iblgf_gpu_set_precision(iblgf_gpu_ctx* ctx, int precision)
 {
     NEED_CTX("iblgf_gpu_set_precision");

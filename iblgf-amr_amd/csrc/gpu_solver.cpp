@@ -246,15 +246,8 @@ void GpuSolver::upload_tree(const Octree& t)
 
     // fields and spectra
     fields_.clear();
-    f_src_tmp_ = field_create();
-    f_tgt_tmp_ = field_create();
-    f_corr_tmp_ = field_create();
-    f_fmm_s_ = field_create();
-    f_fmm_t_ = field_create();
-    sspec_.alloc(static_cast<size_t>(nb_) * nf_);
-    tspec_.alloc(static_cast<size_t>(nb_) * nf_);
-    IBLGF_HIP(hipMemset(sspec_.p, 0, sspec_.n * sizeof(double2)));
-    IBLGF_HIP(hipMemset(tspec_.p, 0, tspec_.n * sizeof(double2)));
+    work_ = Work{};
+    reserve_components(1);
 
     // solver-level block lists
     std::vector<int> leaves, all(nb_);
@@ -343,7 +336,36 @@ void GpuSolver::field_download(int id, double* host)
     sync();
 }
 
-void GpuSolver::fill_zero(double* f) { launch::zero_blocks(stream_, n_, f, p_all_.p, nb_); }
+void GpuSolver::fill_zero(double* f, const Batch& b) { launch::zero_blocks(stream_, n_, f, p_all_.p, nb_, b); }
+
+void GpuSolver::reserve_components(int k)
+{
+    if (k < 1 || k > kMaxComponents) throw std::invalid_argument("component count out of range");
+    if (k <= work_.k) return;
+    if (stream_) sync();  // solves in flight still use the old arrays
+    const size_t fs = field_size() * k, ss = static_cast<size_t>(nb_) * nf_ * k;
+    Work w;
+    for (DevArray<double>* a : {&w.src, &w.tgt, &w.corr, &w.fmm_s, &w.fmm_t})
+    {
+        a->alloc(fs);
+        IBLGF_HIP(hipMemset(a->p, 0, fs * sizeof(double)));
+    }
+    w.sspec.alloc(ss);
+    w.tspec.alloc(ss);
+    IBLGF_HIP(hipMemset(w.sspec.p, 0, ss * sizeof(double2)));
+    IBLGF_HIP(hipMemset(w.tspec.p, 0, ss * sizeof(double2)));
+    w.k = k;
+    work_ = std::move(w);
+}
+
+Batch GpuSolver::batch(int k) const
+{
+    Batch b;
+    b.k = k;
+    b.fstride = static_cast<long>(field_size());
+    b.sstride = static_cast<long>(nb_) * nf_;
+    return b;
+}
 
 // ---------------------------------------------------------------------------
 // Pass plans: Fmm::apply for base level L and non_leaf_as_source nl
@@ -678,90 +700,116 @@ void GpuSolver::prepare_if(double alpha_base)
     sync();
 }
 
-// Fmm::apply (include/iblgf/fmm/fmm.hpp:729-870)
-void GpuSolver::run_pass(PassPlan& p, int src_field, int dst_field, double scale, bool laplace, double alpha)
+// Fmm::apply (include/iblgf/fmm/fmm.hpp:729-870), for the b.k components held
+// in the work arrays.
+void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, const Batch& b)
 {
-    double* fmm_s = F(f_fmm_s_);
-    double* fmm_t = F(f_fmm_t_);
+    double* fmm_s = work_.fmm_s.p;
+    double* fmm_t = work_.fmm_t.p;
     timer_.begin(stream_);
-    launch::zero_blocks(stream_, n_, fmm_s, p.zero_s.p, p.zero_s.size());
-    launch::zero_blocks(stream_, n_, fmm_t, p.zero_t.p, p.zero_t.size());
-    launch::copy_blocks(stream_, n_, fmm_s, F(src_field), p.copy_s.p, p.copy_s.size(), true);
+    launch::zero_blocks(stream_, n_, fmm_s, p.zero_s.p, p.zero_s.size(), b);
+    launch::zero_blocks(stream_, n_, fmm_t, p.zero_t.p, p.zero_t.size(), b);
+    launch::copy_blocks(stream_, n_, fmm_s, work_.src.p, p.copy_s.p, p.copy_s.size(), true, b);
     timer_.end(stream_, KC_OTHER, 0);
     if (!p.neighbor_only)
     {
         timer_.begin(stream_);
         for (int l = p.L - 1; l >= 0; --l)
-            launch::transfer_gather(stream_, n_, fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9, m_nli_anterp_.p);
-        timer_.end(stream_, KC_TRANSFER, p.ntransfer_antrp);
+            launch::transfer_gather(
+                stream_, n_, fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9, m_nli_anterp_.p, b);
+        timer_.end(stream_, KC_TRANSFER, p.ntransfer_antrp * b.k);
     }
     timer_.begin(stream_);
-    launch::forward_dft(stream_, n_, precision_, fmm_s, p.fwd.p, p.fwd.size(), sspec_.p);
-    timer_.end(stream_, KC_FORWARD, p.fwd.size());
+    launch::forward_dft(stream_, n_, precision_, fmm_s, p.fwd.p, p.fwd.size(), work_.sspec.p, b);
+    timer_.end(stream_, KC_FORWARD, static_cast<long>(p.fwd.size()) * b.k);
     for (auto& m : p.mac)
     {
         const void* gh = ghat(laplace, m.d, alpha);
         timer_.begin(stream_);
         launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.olist.p, m.nused,
-            m.omap.p, gh, sspec_.p, tspec_.p);
-        timer_.end(stream_, KC_MAC, m.npairs);
+            m.omap.p, gh, work_.sspec.p, work_.tspec.p, b);
+        timer_.end(stream_, KC_MAC, m.npairs * b.k);
     }
     const double P = 2.0 * n_ - 1.0;
     timer_.begin(stream_);
-    launch::inverse_dft(stream_, n_, precision_, tspec_.p, p.inv.p, p.inv.size(), fmm_t, scale / (P * P * P));
-    timer_.end(stream_, KC_INVERSE, p.inv.size());
+    launch::inverse_dft(
+        stream_, n_, precision_, work_.tspec.p, p.inv.p, p.inv.size(), fmm_t, scale / (P * P * P), b);
+    timer_.end(stream_, KC_INVERSE, static_cast<long>(p.inv.size()) * b.k);
     if (!p.neighbor_only)
     {
         timer_.begin(stream_);
         for (int l = 1; l < p.L; ++l)
-            launch::transfer_scatter(stream_, n_, fmm_t, fmm_t, p.intrp[l].p, p.intrp[l].size() / 3, m_nli_interp_.p, true);
-        timer_.end(stream_, KC_TRANSFER, p.ntransfer - p.ntransfer_antrp);
+            launch::transfer_scatter(
+                stream_, n_, fmm_t, fmm_t, p.intrp[l].p, p.intrp[l].size() / 3, m_nli_interp_.p, true, b);
+        timer_.end(stream_, KC_TRANSFER, (p.ntransfer - p.ntransfer_antrp) * b.k);
     }
     timer_.begin(stream_);
-    launch::axpy_blocks(stream_, n_, F(dst_field), fmm_t, p.add.p, p.add.size(), 1.0);
+    launch::axpy_blocks(stream_, n_, work_.tgt.p, fmm_t, p.add.p, p.add.size(), 1.0, b);
     timer_.end(stream_, KC_OTHER, 0);
 }
 
-// PoissonSolver::apply_lgf (include/iblgf/solver/poisson/poisson.hpp:587-894)
-void GpuSolver::apply_lgf(int src, int dst)
+void GpuSolver::check_batch(const int* src, const int* dst, int k)
 {
     if (!uploaded_) throw std::runtime_error("tree not uploaded");
+    if (k < 1 || k > kMaxComponents) throw std::invalid_argument("component count out of range");
+    for (int c = 0; c < k; ++c)
+    {
+        field_ptr(src[c]);
+        field_ptr(dst[c]);
+    }
     IBLGF_HIP(hipSetDevice(device_));
-    double* st = F(f_src_tmp_);
-    double* tt = F(f_tgt_tmp_);
-    double* ct = F(f_corr_tmp_);
+    reserve_components(k);
+}
+
+// PoissonSolver::apply_lgf (include/iblgf/solver/poisson/poisson.hpp:587-894)
+void GpuSolver::apply_lgf(int src, int dst) { apply_lgf_batch(&src, &dst, 1); }
+
+void GpuSolver::apply_lgf_batch(const int* src, const int* dst, int k)
+{
+    check_batch(src, dst, k);
+    const Batch b = batch(k);
+    const size_t fs = field_size();
+    double* st = work_.src.p;
+    double* tt = work_.tgt.p;
+    double* ct = work_.corr.p;
     timer_.begin(stream_);
-    fill_zero(st);
-    fill_zero(tt);
-    fill_zero(ct);
-    launch::copy_blocks(stream_, n_, st, F(src), p_leaves_.p, p_leaves_.size(), false);
+    fill_zero(st, b);
+    fill_zero(tt, b);
+    fill_zero(ct, b);
+    for (int c = 0; c < k; ++c)
+        launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_leaves_.p, p_leaves_.size(), false);
     timer_.end(stream_, KC_OTHER, 0);
     timer_.begin(stream_);
     long ncoarse = 0;
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
     {
-        launch::transfer_gather(stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p);
+        launch::transfer_gather(
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, b);
         ncoarse += p_kids_[ls].size();
     }
-    timer_.end(stream_, KC_TRANSFER, ncoarse);
+    timer_.end(stream_, KC_TRANSFER, ncoarse * k);
     for (int l = base_level_; l < depth_; ++l)
     {
         const double dx = dx_base_ / std::pow(2.0, l - base_level_);
-        run_pass(pass(l, 1, false), f_src_tmp_, f_tgt_tmp_, dx * dx, true, 0.0);
+        run_pass(pass(l, 1, false), dx * dx, true, 0.0, b);
         timer_.begin(stream_);
-        launch::transfer_scatter(stream_, n_, tt, tt, p_ccint_[l].p, p_ccint_[l].size() / 3, m_cc_interp_.p, false);
-        timer_.end(stream_, KC_TRANSFER, p_ccint_[l].size() / 3);
-        run_pass(pass(l, 0, false), f_src_tmp_, f_tgt_tmp_, dx * dx, true, 0.0);
+        launch::transfer_scatter(stream_, n_, tt, tt, p_ccint_[l].p, p_ccint_[l].size() / 3, m_cc_interp_.p, false, b);
+        timer_.end(stream_, KC_TRANSFER, static_cast<long>(p_ccint_[l].size() / 3) * k);
+        run_pass(pass(l, 0, false), dx * dx, true, 0.0, b);
         timer_.begin(stream_);
-        launch::copy_blocks(stream_, n_, F(dst), tt, p_leaf_level_[l].p, p_leaf_level_[l].size(), false);
+        for (int c = 0; c < k; ++c)
+            launch::copy_blocks(
+                stream_, n_, F(dst[c]), tt + c * fs, p_leaf_level_[l].p, p_leaf_level_[l].size(), false);
         if (l < depth_ - 1)
         {
-            launch::buffer_exchange(stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p);
+            launch::buffer_exchange(
+                stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p, b);
             launch::transfer_scatter(
-                stream_, n_, st, ct, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false);
+                stream_, n_, st, ct, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false, b);
             const double dxc = dx / 2.0;
-            launch::correction_stencil(stream_, n_, tt, ct, p_kids_[l].p, p_kids_[l].size(), 1.0 / (dxc * dxc), 0.0);
-            launch::axpy_blocks(stream_, n_, st, ct, p_level_slots_[l + 1].p, p_level_slots_[l + 1].size(), 1.0);
+            launch::correction_stencil(
+                stream_, n_, tt, ct, p_kids_[l].p, p_kids_[l].size(), 1.0 / (dxc * dxc), 0.0, b);
+            launch::axpy_blocks(stream_, n_, st, ct, p_level_slots_[l + 1].p, p_level_slots_[l + 1].size(), 1.0, b);
         }
         timer_.end(stream_, KC_OTHER, 0);
     }
@@ -769,28 +817,36 @@ void GpuSolver::apply_lgf(int src, int dst)
 }
 
 // PoissonSolver::apply_if (include/iblgf/solver/poisson/poisson.hpp:435-494)
-void GpuSolver::apply_if(int src, int dst, double alpha_base)
+void GpuSolver::apply_if(int src, int dst, double alpha_base) { apply_if_batch(&src, &dst, 1, alpha_base); }
+
+void GpuSolver::apply_if_batch(const int* src, const int* dst, int k, double alpha_base)
 {
-    if (!uploaded_) throw std::runtime_error("tree not uploaded");
-    IBLGF_HIP(hipSetDevice(device_));
-    double* st = F(f_src_tmp_);
-    double* tt = F(f_tgt_tmp_);
-    fill_zero(st);
-    fill_zero(tt);
-    launch::copy_blocks(stream_, n_, st, F(src), p_leaves_.p, p_leaves_.size(), false);
+    check_batch(src, dst, k);
+    const Batch b = batch(k);
+    const size_t fs = field_size();
+    double* st = work_.src.p;
+    double* tt = work_.tgt.p;
+    fill_zero(st, b);
+    fill_zero(tt, b);
+    for (int c = 0; c < k; ++c)
+        launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_leaves_.p, p_leaves_.size(), false);
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
-        launch::transfer_gather(stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p);
+        launch::transfer_gather(
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, b);
     for (int l = depth_ - 2; l >= base_level_; --l)
     {
-        launch::buffer_exchange(stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p);
-        launch::transfer_scatter(stream_, n_, st, st, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false);
+        launch::buffer_exchange(stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p, b);
+        launch::transfer_scatter(
+            stream_, n_, st, st, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false, b);
     }
     for (int l = base_level_; l < depth_; ++l)
     {
         const double alpha = alpha_base * std::pow(4.0, l - base_level_);
-        run_pass(pass(l, 0, true), f_src_tmp_, f_tgt_tmp_, 1.0, false, alpha);
-        run_pass(pass(l, 1, true), f_src_tmp_, f_tgt_tmp_, 1.0, false, alpha);
-        launch::copy_blocks(stream_, n_, F(dst), tt, p_level_slots_[l].p, p_level_slots_[l].size(), false);
+        run_pass(pass(l, 0, true), 1.0, false, alpha, b);
+        run_pass(pass(l, 1, true), 1.0, false, alpha, b);
+        for (int c = 0; c < k; ++c)
+            launch::copy_blocks(
+                stream_, n_, F(dst[c]), tt + c * fs, p_level_slots_[l].p, p_level_slots_[l].size(), false);
     }
     IBLGF_HIP(hipGetLastError());
 }
@@ -800,37 +856,37 @@ void GpuSolver::apply_if(int src, int dst, double alpha_base)
 // ---------------------------------------------------------------------------
 void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
 {
-    if (!uploaded_) throw std::runtime_error("tree not uploaded");
     if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB)
         throw std::invalid_argument("mask type must be IB2xIB or xIB2IB");
-    IBLGF_HIP(hipSetDevice(device_));
+    check_batch(&src, &dst, 1);
+    const Batch b = batch(1);
     const int L = depth_ - 1;
-    double*   st = F(f_src_tmp_);
-    double*   tt = F(f_tgt_tmp_);
-    fill_zero(st);
-    fill_zero(tt);
-    fill_zero(F(f_corr_tmp_));
+    double*   st = work_.src.p;
+    double*   tt = work_.tgt.p;
+    fill_zero(st, b);
+    fill_zero(tt, b);
+    fill_zero(work_.corr.p, b);
     launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[L].p, p_level_slots_[L].size(), true);
     const double dx = dx_base_ / std::pow(2.0, L - base_level_);
-    run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), false), f_src_tmp_, f_tgt_tmp_, dx * dx, true, 0.0);
+    run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), false), dx * dx, true, 0.0, b);
     launch::copy_blocks(stream_, n_, F(dst), tt, p_leaf_level_[L].p, p_leaf_level_[L].size(), false);
     IBLGF_HIP(hipGetLastError());
 }
 
 void GpuSolver::apply_if_masked(int src, int dst, double alpha_base, int mask_type)
 {
-    if (!uploaded_) throw std::runtime_error("tree not uploaded");
     if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB)
         throw std::invalid_argument("mask type must be IB2xIB or xIB2IB");
-    IBLGF_HIP(hipSetDevice(device_));
+    check_batch(&src, &dst, 1);
+    const Batch b = batch(1);
     const int L = depth_ - 1;
-    double*   st = F(f_src_tmp_);
-    double*   tt = F(f_tgt_tmp_);
-    fill_zero(st);
-    fill_zero(tt);
+    double*   st = work_.src.p;
+    double*   tt = work_.tgt.p;
+    fill_zero(st, b);
+    fill_zero(tt, b);
     launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[L].p, p_level_slots_[L].size(), true);
     const double alpha = alpha_base * std::pow(4.0, L - base_level_);
-    run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), true), f_src_tmp_, f_tgt_tmp_, 1.0, false, alpha);
+    run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), true), 1.0, false, alpha, b);
     launch::copy_blocks(stream_, n_, F(dst), tt, p_level_slots_[L].p, p_level_slots_[L].size(), false);
     IBLGF_HIP(hipGetLastError());
 }

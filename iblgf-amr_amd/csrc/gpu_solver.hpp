@@ -139,6 +139,14 @@ class GpuSolver
     // Integrating factor e^{alpha Lap} of one scalar component; dst is
     // written on every block with data.
     void apply_if(int src, int dst, double alpha_base);
+    // The same solves of k components (src[c] -> dst[c], e.g. the three
+    // vorticity or velocity components) with every launch covering all k;
+    // each component's result equals its single-component solve.
+    void apply_lgf_batch(const int* src, const int* dst, int k);
+    void apply_if_batch(const int* src, const int* dst, int k, double alpha_base);
+    // Size the work arrays for k components ahead of a batched solve.
+    void reserve_components(int k);
+    static constexpr int kMaxComponents = 64;
     // Build the kernel spectra that a solve will use (cold-path work).
     void prepare_laplace();
     void prepare_if(double alpha_base);
@@ -186,11 +194,13 @@ class GpuSolver
     PassPlan& pass(int L, int nl, bool neighbor_only);
     PassPlan& pass_idx(int L, int idx, bool neighbor_only);
     const DevArray<int>& face_clean_list(int kind, int level, bool leaf_only);
-    void      run_pass(PassPlan& p, int src_field, int dst_field, double scale, bool laplace, double alpha);
+    void      run_pass(PassPlan& p, double scale, bool laplace, double alpha, const Batch& b);
+    void      check_batch(const int* src, const int* dst, int k);
+    Batch     batch(int k) const;
     double2*    ghat_laplace(int d);
     double2*    ghat_if(double alpha);
     const void* ghat(bool laplace, int d, double alpha);  // in the current precision
-    void      fill_zero(double* f);
+    void      fill_zero(double* f, const Batch& b = {});
     double*   F(int id) { return fields_.at(id).p; }
 
     int         device_ = 0;
@@ -219,8 +229,17 @@ class GpuSolver
 
     // fields
     std::vector<DevArray<double>> fields_;
-    int f_src_tmp_ = -1, f_tgt_tmp_ = -1, f_corr_tmp_ = -1, f_fmm_s_ = -1, f_fmm_t_ = -1;
-    DevArray<double2> sspec_, tspec_;
+    // Work arrays of the solves, k components each, component-major
+    // (component c of a field array at c * field_size(), of a spectrum array
+    // at c * nb * NF): source and target temporaries, correction, the FMM
+    // pass's source/target and the source/target spectra.
+    struct Work
+    {
+        DevArray<double>  src, tgt, corr, fmm_s, fmm_t;
+        DevArray<double2> sspec, tspec;
+        int               k = 0;
+    };
+    Work work_;
 
     // kernel spectra
     DevArray<double>                         d_lgf_table_;

@@ -48,21 +48,21 @@ __device__ inline void twiddles(double2* W, int P)
 // Elementwise block kernels
 // ---------------------------------------------------------------------------
 template<int N>
-__global__ __launch_bounds__(kThreads) void k_zero(double* f, const int* slots, int count)
+__global__ __launch_bounds__(kThreads) void k_zero(double* f, const int* slots, int count, long cs)
 {
     const int b = blockIdx.x;
     if (b >= count) return;
-    double* p = f + (size_t)slots[b] * Geo<N>::N3;
+    double* p = f + blockIdx.y * cs + (size_t)slots[b] * Geo<N>::N3;
     for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads) p[q] = 0.0;
 }
 
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_copy(
-    double* dst, const double* src, const int* slots, int count, int interior)
+    double* dst, const double* src, const int* slots, int count, int interior, long cs)
 {
     const int b = blockIdx.x;
     if (b >= count) return;
-    const size_t  off = (size_t)slots[b] * Geo<N>::N3;
+    const size_t  off = blockIdx.y * cs + (size_t)slots[b] * Geo<N>::N3;
     double*       d = dst + off;
     const double* s = src + off;
     for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads)
@@ -78,11 +78,11 @@ __global__ __launch_bounds__(kThreads) void k_copy(
 
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_axpy(
-    double* dst, const double* src, const int* slots, int count, double a)
+    double* dst, const double* src, const int* slots, int count, double a, long cs)
 {
     const int b = blockIdx.x;
     if (b >= count) return;
-    const size_t off = (size_t)slots[b] * Geo<N>::N3;
+    const size_t off = blockIdx.y * cs + (size_t)slots[b] * Geo<N>::N3;
     for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads) dst[off + q] += src[off + q] * a;
 }
 
@@ -168,12 +168,14 @@ __device__ inline void leg_z_store(const double* t2, double* out, const double* 
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
-    const double* __restrict__ mats)
+    const double* __restrict__ mats, long cs)
 {
     constexpr int N3 = Geo<N>::N3;
     __shared__ TransferLds<N> L;
     const int e = blockIdx.x;
     if (e >= count) return;
+    src += blockIdx.y * cs;
+    dst += blockIdx.y * cs;
     const int* ent = entries + 9 * e;
     for (int q = threadIdx.x; q < 2 * N * N; q += kTransferThreads) (&L.m[0][0])[q] = mats[q];
     double* out = dst + (size_t)ent[0] * N3;
@@ -190,12 +192,14 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
-    const double* __restrict__ mats, int accumulate)
+    const double* __restrict__ mats, int accumulate, long cs)
 {
     constexpr int N3 = Geo<N>::N3;
     __shared__ TransferLds<N> L;
     const int e = blockIdx.x;
     if (e >= count) return;
+    src += blockIdx.y * cs;
+    dst += blockIdx.y * cs;
     const int child = entries[3 * e], parent = entries[3 * e + 1], c = entries[3 * e + 2];
     for (int q = threadIdx.x; q < 2 * N * N; q += kTransferThreads) (&L.m[0][0])[q] = mats[q];
     __syncthreads();
@@ -208,11 +212,12 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
 // ---------------------------------------------------------------------------
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_buffer_exchange(
-    double* f, const int* slots, int count, const int* __restrict__ neighbor)
+    double* f, const int* slots, int count, const int* __restrict__ neighbor, long cs)
 {
     constexpr int N3 = Geo<N>::N3, E = Geo<N>::E;
     const int b = blockIdx.x;
     if (b >= count) return;
+    f += blockIdx.y * cs;
     const int s = slots[b];
     for (int q = threadIdx.x; q < N3; q += kThreads)
     {
@@ -229,12 +234,13 @@ __global__ __launch_bounds__(kThreads) void k_buffer_exchange(
 
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_correction_stencil(
-    const double* __restrict__ tt, double* __restrict__ cc, const int* slots, int count, double inv, double om2)
+    const double* __restrict__ tt, double* __restrict__ cc, const int* slots, int count, double inv, double om2,
+    long cs)
 {
     constexpr int N3 = Geo<N>::N3;
     const int b = blockIdx.x;
     if (b >= count) return;
-    const size_t  off = (size_t)slots[b] * N3;
+    const size_t  off = blockIdx.y * cs + (size_t)slots[b] * N3;
     const double* t = tt + off;
     double*       c = cc + off;
     for (int q = threadIdx.x; q < N3; q += kThreads)
@@ -317,7 +323,7 @@ __device__ inline void cmac(C a, C w, T& sr, T& si)
 // spectrum is stored as T (double: plain FP64 path; float: mixed path).
 template<int N, class T>
 __global__ __launch_bounds__(kDftThreads) void k_forward_dft(const double* __restrict__ field,
-    const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec)
+    const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec, long fcs, long scs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
@@ -328,7 +334,7 @@ __global__ __launch_bounds__(kDftThreads) void k_forward_dft(const double* __res
     const int b = blockIdx.x;
     if (b >= count) return;
     const int     slot = slots[b];
-    const double* in = field + (size_t)slot * G::N3;
+    const double* in = field + blockIdx.y * fcs + (size_t)slot * G::N3;
     for (int q = threadIdx.x; q < P * N; q += kDftThreads)
     {
         const int k = q / N, j = q - k * N;
@@ -337,7 +343,7 @@ __global__ __launch_bounds__(kDftThreads) void k_forward_dft(const double* __res
         Tw[q] = make_double2(cs, sn);
     }
     __syncthreads();
-    typename Cx<T>::type* out = spec + (size_t)slot * G::NF;
+    typename Cx<T>::type* out = spec + blockIdx.y * scs + (size_t)slot * G::NF;
     constexpr int         K0 = (P + 1) / 2;
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
@@ -409,7 +415,7 @@ __global__ __launch_bounds__(kDftThreads) void k_forward_dft(const double* __res
 // I[o][k] = w^((n-1+o) k).  FP64 throughout.
 template<int N>
 __global__ __launch_bounds__(kDftThreads) void k_inverse_dft(const double2* __restrict__ spec,
-    const int* __restrict__ slots, int count, double* __restrict__ field, double scale)
+    const int* __restrict__ slots, int count, double* __restrict__ field, double scale, long scs, long fcs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
@@ -421,7 +427,7 @@ __global__ __launch_bounds__(kDftThreads) void k_inverse_dft(const double2* __re
     const int b = blockIdx.x;
     if (b >= count) return;
     const int      slot = slots[b];
-    const double2* in = spec + (size_t)slot * G::NF;
+    const double2* in = spec + blockIdx.y * scs + (size_t)slot * G::NF;
     for (int q = threadIdx.x; q < N * P; q += kDftThreads)
     {
         const int o = q / P, k = q - o * P;
@@ -493,7 +499,7 @@ __global__ __launch_bounds__(kDftThreads) void k_inverse_dft(const double2* __re
     }
     if (threadIdx.x < N * N)
     {
-        double* out = field + (size_t)slot * G::N3 + threadIdx.x * N;
+        double* out = field + blockIdx.y * fcs + (size_t)slot * G::N3 + threadIdx.x * N;
 #pragma unroll
         for (int x = 0; x < N; ++x) out[x] += scale * acc[x];
     }
@@ -536,41 +542,47 @@ __device__ inline void mac_entry(const C* Gs, const short* omap, int f, int meta
     }
 }
 
-// One launch covers the sibling groups of one target level.  A workgroup owns
-// KF frequencies: it loads kernel spectra into LDS and walks every group; a
-// thread keeps the 8 target accumulators of one group at one frequency.
+// One launch covers the sibling groups of one target level, for k components
+// of the same tree.  A workgroup owns KF frequencies: it loads kernel spectra
+// into LDS and walks every (group, component) pair, the components of a group
+// on neighbouring threads (same entries, same kernel rows); a thread keeps the
+// 8 target accumulators of one group and component at one frequency.
 // COMPACT launches (few offsets used, the small coarse-level launches) load
 // only the rows in olist and map offsets through omap; the others load all
 // 343 rows and index them directly.
 template<int N, int KF, bool COMPACT, class T>
-__global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
+__global__ __launch_bounds__(1024) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
     const MacSrc* __restrict__ srcs, const int* __restrict__ olist, int nused, const short* __restrict__ omap,
     const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
-    double2* __restrict__ tspec)
+    double2* __restrict__ tspec, int k, long scs, long tcs)
 {
     using C = typename Cx<T>::type;
     constexpr int NF = Geo<N>::NF;
-    constexpr int GPW = kThreads / KF;
+    const int     nt = blockDim.x;  // a multiple of KF, sized to the launch's group count
+    const int     GPW = nt / KF;
     __shared__ C     Gs[343 * KF];
     __shared__ short Om[COMPACT ? 343 : 1];
     const int f0 = blockIdx.x * KF;
     const int rows = COMPACT ? nused : 343;
-    for (int q = threadIdx.x; q < rows * KF; q += kThreads)
+    for (int q = threadIdx.x; q < rows * KF; q += nt)
     {
         const int c = q / KF, f = q - c * KF;
         const int o = COMPACT ? olist[c] : c;
         Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
     }
     if (COMPACT)
-        for (int q = threadIdx.x; q < 343; q += kThreads) Om[q] = omap[q];
+        for (int q = threadIdx.x; q < 343; q += nt) Om[q] = omap[q];
     __syncthreads();
     const int f = threadIdx.x % KF;
     const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
-    const C*  sp = sspec + fg;
     // The workgroup walks the (Morton-ordered) groups in tiles, so a source
     // shared by neighbouring groups is re-read while its line is still in L2.
-    for (int g = blockIdx.y * GPW + threadIdx.x / KF; g < ngroups; g += GPW * gridDim.y)
+    const int nvg = ngroups * k;
+    for (int vg = threadIdx.x / KF; vg < nvg; vg += GPW)
     {
+        const int      g = vg / k, c = vg - g * k;
+        const C*       sp = sspec + c * scs + fg;
+        double2*       tp = tspec + c * tcs + fg;
         const MacGroup grp = groups[g];
         double2        acc[8];
 #pragma unroll
@@ -598,7 +610,7 @@ __global__ __launch_bounds__(kThreads) void k_spectral_mac(const MacGroup* __res
         {
 #pragma unroll
             for (int t = 0; t < 8; ++t)
-                if (grp.tslot[t] >= 0) tspec[(size_t)grp.tslot[t] * NF + fg] = acc[t];
+                if (grp.tslot[t] >= 0) tp[(size_t)grp.tslot[t] * NF] = acc[t];
         }
     }
 }
@@ -764,94 +776,115 @@ inline unsigned grid1(long total) { return (unsigned)((total + kThreads - 1) / k
 
 template<int N> struct RunZero
 {
-    static void run(hipStream_t s, double* f, const int* slots, int count)
+    static void run(hipStream_t s, double* f, const int* slots, int count, const Batch& b)
     {
-        hipLaunchKernelGGL(k_zero<N>, dim3(count), dim3(kThreads), 0, s, f, slots, count);
+        hipLaunchKernelGGL(k_zero<N>, dim3(count, b.k), dim3(kThreads), 0, s, f, slots, count, b.fstride);
     }
 };
 template<int N> struct RunCopy
 {
-    static void run(hipStream_t s, double* d, const double* src, const int* slots, int count, int interior)
+    static void run(hipStream_t s, double* d, const double* src, const int* slots, int count, int interior,
+        const Batch& b)
     {
-        hipLaunchKernelGGL(k_copy<N>, dim3(count), dim3(kThreads), 0, s, d, src, slots, count, interior);
+        hipLaunchKernelGGL(k_copy<N>, dim3(count, b.k), dim3(kThreads), 0, s, d, src, slots, count, interior,
+            b.fstride);
     }
 };
 template<int N> struct RunAxpy
 {
-    static void run(hipStream_t s, double* d, const double* src, const int* slots, int count, double a)
+    static void run(hipStream_t s, double* d, const double* src, const int* slots, int count, double a,
+        const Batch& b)
     {
-        hipLaunchKernelGGL(k_axpy<N>, dim3(count), dim3(kThreads), 0, s, d, src, slots, count, a);
+        hipLaunchKernelGGL(k_axpy<N>, dim3(count, b.k), dim3(kThreads), 0, s, d, src, slots, count, a, b.fstride);
     }
 };
 template<int N> struct RunGather
 {
-    static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m)
+    static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m,
+        const Batch& b)
     {
-        hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count), dim3(kTransferThreads), 0, s, src, dst, e, count, m);
+        hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count, b.k), dim3(kTransferThreads), 0, s, src, dst, e, count,
+            m, b.fstride);
     }
 };
 template<int N> struct RunScatter
 {
-    static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m, int acc)
+    static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m, int acc,
+        const Batch& b)
     {
-        hipLaunchKernelGGL(k_transfer_scatter<N>, dim3(count), dim3(kTransferThreads), 0, s, src, dst, e, count, m, acc);
+        hipLaunchKernelGGL(k_transfer_scatter<N>, dim3(count, b.k), dim3(kTransferThreads), 0, s, src, dst, e, count,
+            m, acc, b.fstride);
     }
 };
 template<int N> struct RunHalo
 {
-    static void run(hipStream_t s, double* f, const int* slots, int count, const int* nb)
+    static void run(hipStream_t s, double* f, const int* slots, int count, const int* nb, const Batch& b)
     {
-        hipLaunchKernelGGL(k_buffer_exchange<N>, dim3(count), dim3(kThreads), 0, s, f, slots, count, nb);
+        hipLaunchKernelGGL(k_buffer_exchange<N>, dim3(count, b.k), dim3(kThreads), 0, s, f, slots, count, nb,
+            b.fstride);
     }
 };
 template<int N> struct RunStencil
 {
-    static void run(hipStream_t s, const double* t, double* c, const int* slots, int count, double inv, double om2)
+    static void run(hipStream_t s, const double* t, double* c, const int* slots, int count, double inv, double om2,
+        const Batch& b)
     {
-        hipLaunchKernelGGL(k_correction_stencil<N>, dim3(count), dim3(kThreads), 0, s, t, c, slots, count, inv, om2);
+        hipLaunchKernelGGL(k_correction_stencil<N>, dim3(count, b.k), dim3(kThreads), 0, s, t, c, slots, count, inv,
+            om2, b.fstride);
     }
 };
 template<int N> struct RunFwd
 {
-    static void run(hipStream_t s, int prec, const double* f, const int* slots, int count, void* spec)
+    static void run(hipStream_t s, int prec, const double* f, const int* slots, int count, void* spec,
+        const Batch& b)
     {
         if (prec == 0)
-            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count), dim3(kDftThreads), 0, s, f, slots, count,
-                (double2*)spec);
+            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count, b.k), dim3(kDftThreads), 0, s, f, slots, count,
+                (double2*)spec, b.fstride, b.sstride);
         else
-            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count), dim3(kDftThreads), 0, s, f, slots, count,
-                (float2*)spec);
+            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count, b.k), dim3(kDftThreads), 0, s, f, slots, count,
+                (float2*)spec, b.fstride, b.sstride);
     }
 };
 template<int N> struct RunInv
 {
-    static void run(hipStream_t s, int prec, const void* spec, const int* slots, int count, double* f, double scale)
+    static void run(hipStream_t s, int prec, const void* spec, const int* slots, int count, double* f, double scale,
+        const Batch& b)
     {
         (void)prec;  // target spectra are FP64 on both paths
-        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count), dim3(kDftThreads), 0, s, (const double2*)spec, slots, count,
-            f, scale);
+        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count, b.k), dim3(kDftThreads), 0, s, (const double2*)spec, slots,
+            count, f, scale, b.sstride, b.fstride);
     }
 };
+// Workgroup size of a MAC launch: every (group, component) pair of the launch
+// in one pass when it fits (at most 1024 threads), in whole waves.
+inline int mac_threads(long nvg, int kf)
+{
+    const long want = ((nvg * kf + 63) / 64) * 64;
+    return want < 128 ? 128 : (want > 1024 ? 1024 : (int)want);
+}
+
 template<int N> struct RunMac
 {
     static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const int* olist,
-        int nused, const short* omap, const void* ghat, const void* ss, void* ts)
+        int nused, const short* omap, const void* ghat, const void* ss, void* ts, const Batch& b)
     {
         const bool compact = nused <= 200;
 #define IBLGF_MAC(KF, CP, T, C)                                                                                  \
-    hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3((Geo<N>::NF + KF - 1) / KF, 1), dim3(kThreads), 0, s, \
-        g, ng, src, olist, nused, omap, (const C*)ghat, (const C*)ss, (double2*)ts)
+    hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3((Geo<N>::NF + KF - 1) / KF),                          \
+        dim3(mac_threads((long)ng * b.k, KF)), 0, s, g, ng, src, olist, nused, omap, (const C*)ghat, (const C*)ss,  \
+        (double2*)ts, b.k, b.sstride, b.sstride)
         if (prec == 0)
         {
-            if (compact) IBLGF_MAC(8, true, double, double2);
+            if (compact) IBLGF_MAC(4, true, double, double2);
             else
-                IBLGF_MAC(8, false, double, double2);
+                IBLGF_MAC(4, false, double, double2);
         }
         else
         {
-            if (compact) IBLGF_MAC(16, true, float, float2);
+            if (compact) IBLGF_MAC(8, true, float, float2);
             else
-                IBLGF_MAC(16, false, float, float2);
+                IBLGF_MAC(8, false, float, float2);
         }
 #undef IBLGF_MAC
     }
@@ -876,52 +909,58 @@ namespace launch {
 
 bool supported_n(int n) { return n == 8 || n == 10 || n == 12 || n == 16 || n == 18; }
 
-void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count)
+void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunZero>(n, s, f, slots, count);
+    if (count > 0) dispatch_n<RunZero>(n, s, f, slots, count, b);
 }
 void copy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
-    bool interior_only)
+    bool interior_only, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunCopy>(n, s, dst, src, slots, count, interior_only ? 1 : 0);
+    if (count > 0) dispatch_n<RunCopy>(n, s, dst, src, slots, count, interior_only ? 1 : 0, b);
 }
-void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a)
+void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a,
+    const Batch& b)
 {
-    if (count > 0) dispatch_n<RunAxpy>(n, s, dst, src, slots, count, a);
+    if (count > 0) dispatch_n<RunAxpy>(n, s, dst, src, slots, count, a, b);
 }
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats)
+    const double* mats, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunGather>(n, s, src, dst, entries, count, mats);
+    if (count > 0) dispatch_n<RunGather>(n, s, src, dst, entries, count, mats, b);
 }
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate)
+    const double* mats, bool accumulate, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunScatter>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0);
+    if (count > 0) dispatch_n<RunScatter>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, b);
 }
-void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor)
+void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor,
+    const Batch& b)
 {
-    if (count > 0) dispatch_n<RunHalo>(n, s, f, slots, count, neighbor);
+    if (count > 0) dispatch_n<RunHalo>(n, s, f, slots, count, neighbor, b);
 }
 void correction_stencil(hipStream_t s, int n, const double* t, double* c, const int* slots, int count,
-    double inv_dx2, double omega2)
+    double inv_dx2, double omega2, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunStencil>(n, s, t, c, slots, count, inv_dx2, omega2);
+    if (count > 0) dispatch_n<RunStencil>(n, s, t, c, slots, count, inv_dx2, omega2, b);
 }
-void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec)
+void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
+    const Batch& b)
 {
-    if (count > 0) dispatch_n<RunFwd>(n, s, prec, field, slots, count, spec);
+    if (count > 0) dispatch_n<RunFwd>(n, s, prec, field, slots, count, spec, b);
 }
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
-    double scale)
+    double scale, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale);
+    if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale, b);
 }
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec)
+    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec,
+    const Batch& b)
 {
     if (nused < 1 || nused > 343) throw std::runtime_error("offset count out of range");
-    if (ngroups > 0) dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, olist, nused, omap, ghat, sspec, tspec);
+    if (b.k < 1 || b.k > 65535) throw std::runtime_error("component count out of range");
+    if (ngroups > 0)
+        dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, olist, nused, omap, ghat, sspec, tspec, b);
 }
 void to_float2(hipStream_t s, const double2* in, float2* out, long n)
 {

@@ -30,40 +30,54 @@ struct MacSrc
     int meta;  // r | (tmask << 8)
 };
 
+// k components of one tree in one launch (grid dimension y for the block
+// kernels, a component loop for the multiply-accumulate): component c of a
+// field array starts c*fstride doubles in, of a spectrum array c*sstride
+// complex values in.  The default is one component.
+struct Batch
+{
+    int  k = 1;
+    long fstride = 0;
+    long sstride = 0;
+};
+
 namespace launch {
 
 bool supported_n(int n);
 
-void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count);
+void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count, const Batch& b = {});
 void copy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
-    bool interior_only);
+    bool interior_only, const Batch& b = {});
 void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
-    double a);
+    double a, const Batch& b = {});
 
 // dst[entry.parent] += sum_c K(child) for entries {parent, child[8]}; K built
 // from the side matrices mats[2][n*n] (row = output point, col = input point).
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats);
+    const double* mats, const Batch& b = {});
 // dst[entry.child] (=|+=) K(src[entry.parent]) for entries {child, parent, cidx}.
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate);
+    const double* mats, bool accumulate, const Batch& b = {});
 
-void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor);
+void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor,
+    const Batch& b = {});
 void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* corr, const int* slots,
-    int count, double inv_dx2, double omega2);
+    int count, double inv_dx2, double omega2, const Batch& b = {});
 
 // prec 0: every spectrum FP64 (double2);
 // prec 1: source spectra and kernel spectra stored FP32 (float2), read by the
 // MAC as FP32 products accumulated in FP64; target spectra, DFT arithmetic,
 // LDS intermediates and fields FP64.
-void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec);
+void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
+    const Batch& b = {});
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
-    double scale);
+    double scale, const Batch& b = {});
 // olist[nused]: the offsets (0..342, o = (ox+3) + 7 (oy+3) + 49 (oz+3)) the
 // launch's entries use; omap[343]: offset -> row in olist (rows of unused
 // offsets are never read).
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec);
+    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec,
+    const Batch& b = {});
 void to_float2(hipStream_t s, const double2* in, float2* out, long n);
 
 // Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:

@@ -85,6 +85,7 @@ SYMBOLS = {
     "iblgf_gpu_field_upload": (_I, [_P, _I, _DP]),
     "iblgf_gpu_field_download": (_I, [_P, _I, _DP]),
     "iblgf_gpu_apply_lgf": (_I, [_P, _I, _I, _I, _D]),
+    "iblgf_gpu_apply_lgf_batch": (_I, [_P, _IP, _IP, _I, _I, _D]),
     "iblgf_gpu_prepare": (_I, [_P, _I, _D]),
     "iblgf_gpu_set_precision": (_I, [_P, _I]),
     "iblgf_gpu_sync": (_I, [_P]),
@@ -332,6 +333,16 @@ class PoissonSolver:
     def apply_lgf_IF(self, src, dst, alpha_base):
         """PoissonSolver::apply_lgf_IF for one component (asynchronous)."""
         _check(lib().iblgf_gpu_apply_lgf(self._h, int(src), int(dst), KERNEL_IF, float(alpha_base)), self._h)
+
+    def apply_lgf_batch(self, srcs, dsts, kernel=KERNEL_LAPLACE, alpha_base=0.0):
+        """The component loop of apply_lgf / apply_lgf_IF (poisson.hpp:130-145,
+        419-434) with all components in every launch (asynchronous)."""
+        k = len(srcs)
+        if len(dsts) != k:
+            raise ValueError("srcs and dsts differ in length")
+        a = (ctypes.c_int * k)(*[int(x) for x in srcs])
+        b = (ctypes.c_int * k)(*[int(x) for x in dsts])
+        _check(lib().iblgf_gpu_apply_lgf_batch(self._h, a, b, k, int(kernel), float(alpha_base)), self._h)
 
     def sync(self):
         _check(lib().iblgf_gpu_sync(self._h), self._h)

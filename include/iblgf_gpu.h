@@ -171,11 +171,21 @@ enum iblgf_kernel
  * leaf blocks only; IF writes every block with data.  Asynchronous. */
 int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base);
 
+/* The component loop of apply_lgf / apply_lgf_IF over a vector field
+ * (poisson.hpp:130-145, 419-434: one scalar solve per entry of
+ * Source::nFields()), with the k components run through every launch
+ * together: src_fields[c] -> dst_fields[c] for c < k (1 <= k <= 64), each
+ * result equal to its single-component solve.  The work arrays grow to k
+ * components on first use.  Asynchronous. */
+int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
+    double alpha_base);
+
 enum iblgf_precision
 {
     IBLGF_PREC_FP64 = 0,  /* plain path: spectra, DFT legs and MAC in FP64 */
-    IBLGF_PREC_MIXED = 1  /* spectra, DFT legs and MAC in FP32; kernel spectra generated in
-                             FP64 and stored FP32; fields and cross-level transfers FP64 */
+    IBLGF_PREC_MIXED = 1  /* source and kernel spectra stored FP32 (kernel spectra generated in
+                             FP64), MAC products in FP32 accumulated in FP64; DFT arithmetic,
+                             target spectra, fields and cross-level transfers FP64 */
 };
 
 /* Arithmetic of the subsequent solves (no reference counterpart: the

@@ -115,8 +115,9 @@ def test_integrating_factor_matches_oracle(oracle_mod, alpha):
 
 @pytest.mark.parametrize("case", ["uniform", "rings"])
 def test_mixed_precision_error(oracle_mod, case):
-    """Mixed path (FP32 spectra, DFT legs and MAC; FP64 fields and level sums)
-    against the FP64 oracle: relative L2 <= 1e-6 per field component."""
+    """Mixed path (source and kernel spectra stored FP32, MAC products in FP32
+    accumulated in FP64; DFT arithmetic, fields and level sums FP64) against
+    the FP64 oracle: relative L2 <= 1e-6 per field component."""
     from iblgf_amd import PREC_MIXED
 
     print(f"case: mixed precision, {case}", flush=True)
@@ -139,6 +140,46 @@ def test_mixed_precision_error(oracle_mod, case):
         l2, mx = _rel(got[leaves], ref[leaves])
         print(f"component {c}: relL2={l2:.3e} relmax={mx:.3e}")
         assert l2 < 1e-6
+
+
+@pytest.mark.parametrize("kernel,prec", [(0, 0), (1, 0), (0, 1)])
+def test_component_batch_equals_single_solves(oracle_mod, kernel, prec):
+    """Three components in one batched solve: every launch covers all three,
+    and each component's arithmetic is the single-component solve's in the
+    same order, so the results are bitwise equal; component 1 is also checked
+    against the oracle."""
+    from iblgf_amd import KERNEL_IF
+
+    print(f"case: component batch kernel={kernel} precision={prec}", flush=True)
+    v = _small_rings()
+    srcs = [v.source(c) for c in range(3)]
+    solver = PoissonSolver(v.tree)
+    solver.set_precision(prec)
+    fs = [solver.field() for _ in range(3)]
+    fb = [solver.field() for _ in range(3)]
+    f1 = solver.field()
+    for f, a in zip(fs, srcs):
+        solver.upload(f, a)
+    alpha = 0.6
+    solver.apply_lgf_batch(fs, fb, kernel, alpha)
+    batched = [solver.download(f) for f in fb]
+    for c in range(3):
+        solver.upload(f1, np.zeros(solver.shape))
+        if kernel == KERNEL_IF:
+            solver.apply_lgf_IF(fs[c], f1, alpha)
+        else:
+            solver.apply_lgf(fs[c], f1)
+        assert np.array_equal(solver.download(f1), batched[c]), f"component {c} differs from its single solve"
+    ot = oracle_mod.Tree(v.arrays, v.info, v.tree.n)
+    if kernel == KERNEL_IF:
+        ref, sel = ot.apply_if(srcs[1], alpha), slice(None)
+    else:
+        ref, sel = ot.apply_lgf(srcs[1]), _leaf_mask(v.arrays)
+    l2, mx = _rel(batched[1][sel], ref[sel])
+    print(f"component 1 vs oracle relL2={l2:.3e} relmax={mx:.3e}")
+    assert l2 < (REL_L2 if prec == 0 else 1e-6)
+    with pytest.raises(RuntimeError):
+        solver.apply_lgf_batch([fs[0], fs[1]], [fb[0], fs[1]])
 
 
 def test_full_size_vortex_rings_properties():
