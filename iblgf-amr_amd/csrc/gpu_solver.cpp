@@ -486,7 +486,7 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
                 gidx[pk] = gi;
                 GroupBuild b;
                 std::fill(b.g.tslot, b.g.tslot + 8, -1);
-                b.g.begin = b.g.count = 0;
+                b.g.begin = b.g.count = b.g.nfull = 0;
                 gb.push_back(std::move(b));
             }
             else
@@ -542,10 +542,25 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
         {
             b.g.begin = static_cast<int>(srcs.size());
             std::vector<MacSrc> loc;
-            for (auto& en : b.ent) loc.push_back(MacSrc{en[0], en[1] | (en[2] << 8)});
-            std::stable_sort(loc.begin(), loc.end(), [](const MacSrc& a, const MacSrc& c) {
-                return (a.meta & 255) < (c.meta & 255);
+            for (auto& en : b.ent)
+            {
+                const int r = en[1];
+                const int ob = (r % 6 + 1) + 7 * ((r / 6) % 6 + 1) + 49 * (r / 36 + 1);
+                loc.push_back(MacSrc{en[0], ob | (en[2] << 9)});
+            }
+            // Entries acting on every target of the group first (the kernel
+            // then computes all 8 children for them; absent children are
+            // never stored), each part in source-position order.
+            int gmask = 0;
+            for (int c = 0; c < 8; ++c)
+                if (b.g.tslot[c] >= 0) gmask |= 1 << c;
+            std::stable_sort(loc.begin(), loc.end(), [gmask](const MacSrc& a, const MacSrc& c) {
+                const bool fa = (a.meta >> 9) == gmask, fc = (c.meta >> 9) == gmask;
+                if (fa != fc) return fa;
+                return (a.meta & 511) < (c.meta & 511);
             });
+            b.g.nfull = 0;
+            for (const auto& en : loc) b.g.nfull += (en.meta >> 9) == gmask ? 1 : 0;
             srcs.insert(srcs.end(), loc.begin(), loc.end());
             b.g.count = static_cast<int>(loc.size());
             groups.push_back(b.g);
@@ -561,8 +576,7 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
             std::vector<char> used(343, 0);
             for (const auto& e : srcs)
             {
-                const int r = e.meta & 255, tm = e.meta >> 8;
-                const int ob = (r % 6 + 1) + 7 * ((r / 6) % 6 + 1) + 49 * (r / 36 + 1);
+                const int ob = e.meta & 511, tm = e.meta >> 9;
                 for (int t = 0; t < 8; ++t)
                     if (tm & (1 << t)) used[ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1))] = 1;
             }

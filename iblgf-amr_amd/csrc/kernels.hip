@@ -514,44 +514,142 @@ __global__ __launch_bounds__(kDftThreads) void k_inverse_dft(const double2* __re
 // Accumulators are FP64 on both paths.  On the mixed path the products are
 // FP32 and each is widened before the add: over the ~200 sources of a
 // target an FP32 running sum loses ~15x more than the spectra's own rounding.
-__device__ inline void cmul_acc(double2 g, double2 S, double2& acc)
+// Target accumulators of one (group, component) at one frequency.  FP64
+// spectra accumulate in FP64 directly.  FP32 spectra form every product and
+// the partial sums over at most 8 source entries in FP32; each partial sum is
+// then added into the FP64 accumulator.
+template<class C> struct MacAcc;
+template<> struct MacAcc<double2>
 {
-    acc.x = fma(g.x, S.x, fma(-g.y, S.y, acc.x));
-    acc.y = fma(g.x, S.y, fma(g.y, S.x, acc.y));
-}
-__device__ inline void cmul_acc(float2 g, float2 S, double2& acc)
+    double2 a[8];
+    __device__ void zero()
+    {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) a[t] = make_double2(0.0, 0.0);
+    }
+    __device__ void add(int t, double2 g, double2 S)
+    {
+        a[t].x = fma(g.x, S.x, fma(-g.y, S.y, a[t].x));
+        a[t].y = fma(g.x, S.y, fma(g.y, S.x, a[t].y));
+    }
+    __device__ void flush() {}
+};
+template<> struct MacAcc<float2>
 {
-    acc.x += (double)fmaf(g.x, S.x, -g.y * S.y);
-    acc.y += (double)fmaf(g.x, S.y, g.y * S.x);
+    double2 a[8];
+    float2  p[8];
+    __device__ void zero()
+    {
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+        {
+            a[t] = make_double2(0.0, 0.0);
+            p[t] = make_float2(0.f, 0.f);
+        }
+    }
+    __device__ void add(int t, float2 g, float2 S)
+    {
+        p[t].x = fmaf(g.x, S.x, fmaf(-g.y, S.y, p[t].x));
+        p[t].y = fmaf(g.x, S.y, fmaf(g.y, S.x, p[t].y));
+    }
+    __device__ void flush()
+    {
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+        {
+            a[t].x += (double)p[t].x;
+            a[t].y += (double)p[t].y;
+            p[t] = make_float2(0.f, 0.f);
+        }
+    }
+};
+
+// Row offset of target child t = (cx, cy, cz) relative to the entry's base row.
+__device__ constexpr int mac_delta(int t) { return (t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1); }
+
+// An entry acting on every target of its group: all 8 children are computed
+// (children that are not targets are never stored), every kernel row is a
+// constant offset from the entry's base row, so the 8 LDS reads carry
+// immediate offsets.  The base ob is at least 57 = mac_delta(7) and at most
+// 342, so every row read lies in the table.
+template<int KF, class C>
+__device__ inline void mac_full(const C* Gs, int f, int meta, C S, MacAcc<C>& acc)
+{
+    const C* Gb = Gs + ((meta & 511) - 57) * KF + f;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc.add(t, Gb[(57 - mac_delta(t)) * KF], S);
 }
 
+// An entry with a partial target mask, branch-free: targets outside the mask
+// read the all-zero row zrow, so the 8 LDS reads still issue together.  Every
+// row index is in range: for r in [0,6)^3 and any child the offset lies in
+// [-3,3]^3.
 template<int KF, bool COMPACT, class C>
-__device__ inline void mac_entry(const C* Gs, const short* omap, int f, int meta, C S, double2 (&acc)[8])
+__device__ inline void mac_masked(const C* Gs, const short* omap, int zrow, int f, int meta, C S, MacAcc<C>& acc)
 {
-    const int r = meta & 255, tm = meta >> 8;
-    const int rx = r % 6, ry = (r / 6) % 6, rz = r / 36;
-    const int ob = (rx + 1) + 7 * (ry + 1) + 49 * (rz + 1);
+    const int ob = meta & 511, tm = meta >> 9;
+    C         g[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t)
     {
-        if (tm & (1 << t))
-        {
-            const int oi = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
-            cmul_acc(Gs[(COMPACT ? omap[oi] : oi) * KF + f], S, acc[t]);
-        }
+        const int oi = ob - mac_delta(t);
+        const int row = ((tm >> t) & 1) ? (COMPACT ? (int)omap[oi] : oi) : zrow;
+        g[t] = Gs[row * KF + f];
     }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc.add(t, g[t], S);
+}
+
+// Entries [e, end) of one (group, component) in batches of B, the batch's
+// source loads issued before their use (B = 4 where the offset map's LDS
+// reads would otherwise spill registers).
+template<bool FULL, int NF, int KF, bool COMPACT, class C>
+__device__ inline void mac_range(const MacSrc* __restrict__ srcs, const C* __restrict__ sp, const C* Gs,
+    const short* Om, int zrow, int f, int e, int end, MacAcc<C>& acc)
+{
+    constexpr int B = COMPACT ? 4 : 8;
+    for (; e + B <= end; e += B)
+    {
+        MacSrc s[B];
+        C      S[B];
+#pragma unroll
+        for (int i = 0; i < B; ++i) s[i] = srcs[e + i];
+#pragma unroll
+        for (int i = 0; i < B; ++i) S[i] = sp[(size_t)s[i].slot * NF];
+#pragma unroll
+        for (int i = 0; i < B; ++i)
+        {
+            if constexpr (FULL) mac_full<KF, C>(Gs, f, s[i].meta, S[i], acc);
+            else
+                mac_masked<KF, COMPACT, C>(Gs, Om, zrow, f, s[i].meta, S[i], acc);
+        }
+        acc.flush();
+    }
+    for (; e < end; ++e)
+    {
+        const MacSrc s0 = srcs[e];
+        const C      S0 = sp[(size_t)s0.slot * NF];
+        if constexpr (FULL) mac_full<KF, C>(Gs, f, s0.meta, S0, acc);
+        else
+            mac_masked<KF, COMPACT, C>(Gs, Om, zrow, f, s0.meta, S0, acc);
+    }
+    acc.flush();
 }
 
 // One launch covers the sibling groups of one target level, for k components
 // of the same tree.  A workgroup owns KF frequencies: it loads kernel spectra
 // into LDS and walks every (group, component) pair, the components of a group
 // on neighbouring threads (same entries, same kernel rows); a thread keeps the
-// 8 target accumulators of one group and component at one frequency.
-// COMPACT launches (few offsets used, the small coarse-level launches) load
-// only the rows in olist and map offsets through omap; the others load all
-// 343 rows and index them directly.
+// 8 target accumulators of one group and component at one frequency.  A
+// group's entries that act on all 8 targets come first (nfull of them); the
+// rest carry partial masks.  COMPACT launches (few offsets used, the small
+// coarse-level launches) load only the rows in olist, map offsets through
+// omap and take every entry through the masked form; the others load all 343
+// rows and index them directly.
+constexpr int mac_max_threads(bool compact) { return compact ? 512 : 1024; }
+
 template<int N, int KF, bool COMPACT, class T>
-__global__ __launch_bounds__(1024) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
+__global__ __launch_bounds__(mac_max_threads(COMPACT)) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
     const MacSrc* __restrict__ srcs, const int* __restrict__ olist, int nused, const short* __restrict__ omap,
     const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
     double2* __restrict__ tspec, int k, long scs, long tcs)
@@ -560,15 +658,15 @@ __global__ __launch_bounds__(1024) void k_spectral_mac(const MacGroup* __restric
     constexpr int NF = Geo<N>::NF;
     const int     nt = blockDim.x;  // a multiple of KF, sized to the launch's group count
     const int     GPW = nt / KF;
-    __shared__ C     Gs[343 * KF];
+    __shared__ C     Gs[344 * KF];  // the launch's rows, then one all-zero row
     __shared__ short Om[COMPACT ? 343 : 1];
     const int f0 = blockIdx.x * KF;
     const int rows = COMPACT ? nused : 343;
-    for (int q = threadIdx.x; q < rows * KF; q += nt)
+    for (int q = threadIdx.x; q < (rows + 1) * KF; q += nt)
     {
         const int c = q / KF, f = q - c * KF;
-        const int o = COMPACT ? olist[c] : c;
-        Gs[q] = (f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
+        const int o = COMPACT ? olist[min(c, nused - 1)] : min(c, 342);
+        Gs[q] = (c < rows && f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
     }
     if (COMPACT)
         for (int q = threadIdx.x; q < 343; q += nt) Om[q] = omap[q];
@@ -584,33 +682,21 @@ __global__ __launch_bounds__(1024) void k_spectral_mac(const MacGroup* __restric
         const C*       sp = sspec + c * scs + fg;
         double2*       tp = tspec + c * tcs + fg;
         const MacGroup grp = groups[g];
-        double2        acc[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = make_double2(0.0, 0.0);
-        int       e = grp.begin;
-        const int end = grp.begin + grp.count;
-        // eight source loads in flight before their use
-        for (; e + 8 <= end; e += 8)
+        MacAcc<C>      acc;
+        acc.zero();
+        const int e0 = grp.begin, e2 = grp.begin + grp.count;
+        int       e1 = e0;
+        if constexpr (!COMPACT)
         {
-            MacSrc s[8];
-            C      S[8];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) s[i] = srcs[e + i];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) S[i] = sp[(size_t)s[i].slot * NF];
-#pragma unroll
-            for (int i = 0; i < 8; ++i) mac_entry<KF, COMPACT, C>(Gs, Om, f, s[i].meta, S[i], acc);
+            e1 = e0 + grp.nfull;
+            mac_range<true, NF, KF, COMPACT, C>(srcs, sp, Gs, Om, rows, f, e0, e1, acc);
         }
-        for (; e < end; ++e)
-        {
-            const MacSrc s0 = srcs[e];
-            mac_entry<KF, COMPACT, C>(Gs, Om, f, s0.meta, sp[(size_t)s0.slot * NF], acc);
-        }
+        mac_range<false, NF, KF, COMPACT, C>(srcs, sp, Gs, Om, rows, f, e1, e2, acc);
         if (f0 + f < NF)
         {
 #pragma unroll
             for (int t = 0; t < 8; ++t)
-                if (grp.tslot[t] >= 0) tp[(size_t)grp.tslot[t] * NF] = acc[t];
+                if (grp.tslot[t] >= 0) tp[(size_t)grp.tslot[t] * NF] = acc.a[t];
         }
     }
 }
@@ -857,11 +943,11 @@ template<int N> struct RunInv
     }
 };
 // Workgroup size of a MAC launch: every (group, component) pair of the launch
-// in one pass when it fits (at most 1024 threads), in whole waves.
-inline int mac_threads(long nvg, int kf)
+// in one pass when it fits (at most the variant's maximum), in whole waves.
+inline int mac_threads(long nvg, int kf, bool compact)
 {
-    const long want = ((nvg * kf + 63) / 64) * 64;
-    return want < 128 ? 128 : (want > 1024 ? 1024 : (int)want);
+    const long want = ((nvg * kf + 63) / 64) * 64, cap = mac_max_threads(compact);
+    return want < 128 ? 128 : (want > cap ? (int)cap : (int)want);
 }
 
 template<int N> struct RunMac
@@ -872,7 +958,7 @@ template<int N> struct RunMac
         const bool compact = nused <= 200;
 #define IBLGF_MAC(KF, CP, T, C)                                                                                  \
     hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3((Geo<N>::NF + KF - 1) / KF),                          \
-        dim3(mac_threads((long)ng * b.k, KF)), 0, s, g, ng, src, olist, nused, omap, (const C*)ghat, (const C*)ss,  \
+        dim3(mac_threads((long)ng * b.k, KF, CP)), 0, s, g, ng, src, olist, nused, omap, (const C*)ghat, (const C*)ss,  \
         (double2*)ts, b.k, b.sstride, b.sstride)
         if (prec == 0)
         {

@@ -19,15 +19,18 @@ struct MacGroup
     int tslot[8];  // target slot per child id, -1 if not a target
     int begin;     // first source entry
     int count;     // number of source entries
+    int nfull;     // the first nfull entries act on every target of the group
 };
 
-// A source entry: block slot, position r = rx + 6 ry + 36 rz of the source in
-// the 6^3 block region spanned by the children of the parent's 27 neighbours,
-// and the 8-bit mask of the group's targets that it acts on.
+// A source entry: block slot, position r = (rx, ry, rz) of the source in the
+// 6^3 block region spanned by the children of the parent's 27 neighbours,
+// stored as the kernel-row base ob = (rx+1) + 7 (ry+1) + 49 (rz+1) (target
+// child c = (cx, cy, cz) uses offset row ob - (cx + 7 cy + 49 cz)), and the
+// 8-bit mask of the group's targets that it acts on.
 struct MacSrc
 {
     int slot;
-    int meta;  // r | (tmask << 8)
+    int meta;  // ob | (tmask << 9)
 };
 
 // k components of one tree in one launch (grid dimension y for the block
