@@ -52,6 +52,7 @@ template struct DevArray<uint16_t>;
 template struct DevArray<short>;
 template struct DevArray<MacGroup>;
 template struct DevArray<MacSrc>;
+template struct DevArray<uint4>;
 
 // ---------------------------------------------------------------------------
 // Kernel timer
@@ -580,18 +581,34 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
                 for (int t = 0; t < 8; ++t)
                     if (tm & (1 << t)) used[ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1))] = 1;
             }
-            std::vector<int>   ol;
-            std::vector<short> om(343, 0);
+            std::vector<int> ol, row(343, 0);
             for (int o = 0; o < 343; ++o)
                 if (used[o])
                 {
-                    om[o] = static_cast<short>(ol.size());
+                    row[o] = static_cast<int>(ol.size());
                     ol.push_back(o);
                 }
             if (ol.empty()) ol.push_back(0);
             m.nused = static_cast<int>(ol.size());
+            m.compact = m.nused <= kMacCompactMax;
+            // LDS byte offsets of every entry's 8 kernel rows; children outside
+            // the entry's mask read the zero row after the staged rows.
+            const int        zrow = m.compact ? m.nused : 343;
+            std::vector<uint4> ro(srcs.size());
+            for (size_t i = 0; i < srcs.size(); ++i)
+            {
+                const int ob = srcs[i].meta & 511, tm = srcs[i].meta >> 9;
+                unsigned  w[4] = {0, 0, 0, 0};
+                for (int t = 0; t < 8; ++t)
+                {
+                    const int o = ob - ((t & 1) + 7 * ((t >> 1) & 1) + 49 * ((t >> 2) & 1));
+                    const int r = (tm & (1 << t)) ? (m.compact ? row[o] : o) : zrow;
+                    w[t >> 1] |= static_cast<unsigned>(r * kMacRowBytes) << (16 * (t & 1));
+                }
+                ro[i] = make_uint4(w[0], w[1], w[2], w[3]);
+            }
             m.olist.upload(ol);
-            m.omap.upload(om);
+            m.rowoff.upload(ro);
         }
         p.npairs += pairs_here;
         p.mac.push_back(std::move(m));
@@ -740,8 +757,8 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
     {
         const void* gh = ghat(laplace, m.d, alpha);
         timer_.begin(stream_);
-        launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.olist.p, m.nused,
-            m.omap.p, gh, work_.sspec.p, work_.tspec.p, b);
+        launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.rowoff.p, m.olist.p,
+            m.nused, m.compact, gh, work_.sspec.p, work_.tspec.p, b);
         timer_.end(stream_, KC_MAC, m.npairs * b.k);
     }
     const double P = 2.0 * n_ - 1.0;

@@ -54,9 +54,10 @@ struct MacLevelPlan
     int                d = 0;       // level difference of the kernel
     DevArray<MacGroup> groups;
     DevArray<MacSrc>   srcs;
-    DevArray<int>      olist;  // kernel-spectrum rows (offsets) the entries use
-    DevArray<short>    omap;   // offset -> position in olist
+    DevArray<int>      olist;   // kernel-spectrum rows (offsets) the entries use
+    DevArray<uint4>    rowoff;  // per entry: LDS byte offsets of its 8 rows
     int                nused = 0;
+    bool               compact = false;  // stage only the olist rows
     long               npairs = 0;
     long               nentries = 0;
 };

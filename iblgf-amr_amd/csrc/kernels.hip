@@ -580,48 +580,47 @@ __device__ inline void mac_full(const C* Gs, int f, int meta, C S, MacAcc<C>& ac
     for (int t = 0; t < 8; ++t) acc.add(t, Gb[(57 - mac_delta(t)) * KF], S);
 }
 
-// An entry with a partial target mask, branch-free: targets outside the mask
-// read the all-zero row zrow, so the 8 LDS reads still issue together.  Every
-// row index is in range: for r in [0,6)^3 and any child the offset lies in
-// [-3,3]^3.
-template<int KF, bool COMPACT, class C>
-__device__ inline void mac_masked(const C* Gs, const short* omap, int zrow, int f, int meta, C S, MacAcc<C>& acc)
+// An entry with a partial target mask, branch-free: ro holds the entry's 8
+// LDS byte offsets (row * 64; targets outside the mask point at the all-zero
+// row), precomputed by the plan, so each target costs one add before its LDS
+// read and the 8 reads issue together.
+template<int KF, class C>
+__device__ inline void mac_masked(const C* Gs, int fbyte, uint4 ro, C S, MacAcc<C>& acc)
 {
-    const int ob = meta & 511, tm = meta >> 9;
-    C         g[8];
+    const char* base = reinterpret_cast<const char*>(Gs) + fbyte;
+    const unsigned w[4] = {ro.x, ro.y, ro.z, ro.w};
+    C              g[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t)
     {
-        const int oi = ob - mac_delta(t);
-        const int row = ((tm >> t) & 1) ? (COMPACT ? (int)omap[oi] : oi) : zrow;
-        g[t] = Gs[row * KF + f];
+        const unsigned off = (t & 1) ? (w[t >> 1] >> 16) : (w[t >> 1] & 0xffffu);
+        g[t] = *reinterpret_cast<const C*>(base + off);
     }
 #pragma unroll
     for (int t = 0; t < 8; ++t) acc.add(t, g[t], S);
 }
 
-// Entries [e, end) of one (group, component) in batches of B, the batch's
-// source loads issued before their use (B = 4 where the offset map's LDS
-// reads would otherwise spill registers).
-template<bool FULL, int NF, int KF, bool COMPACT, class C>
-__device__ inline void mac_range(const MacSrc* __restrict__ srcs, const C* __restrict__ sp, const C* Gs,
-    const short* Om, int zrow, int f, int e, int end, MacAcc<C>& acc)
+// Entries [e, end) of one (group, component) in batches of 8, the batch's
+// source loads issued before their use.
+template<bool FULL, int NF, int KF, class C>
+__device__ inline void mac_range(const MacSrc* __restrict__ srcs, const uint4* __restrict__ rowoff,
+    const C* __restrict__ sp, const C* Gs, int f, int e, int end, MacAcc<C>& acc)
 {
-    constexpr int B = COMPACT ? 4 : 8;
-    for (; e + B <= end; e += B)
+    const int fbyte = f * (int)sizeof(C);
+    for (; e + 8 <= end; e += 8)
     {
-        MacSrc s[B];
-        C      S[B];
+        MacSrc s[8];
+        C      S[8];
 #pragma unroll
-        for (int i = 0; i < B; ++i) s[i] = srcs[e + i];
+        for (int i = 0; i < 8; ++i) s[i] = srcs[e + i];
 #pragma unroll
-        for (int i = 0; i < B; ++i) S[i] = sp[(size_t)s[i].slot * NF];
+        for (int i = 0; i < 8; ++i) S[i] = sp[(size_t)s[i].slot * NF];
 #pragma unroll
-        for (int i = 0; i < B; ++i)
+        for (int i = 0; i < 8; ++i)
         {
             if constexpr (FULL) mac_full<KF, C>(Gs, f, s[i].meta, S[i], acc);
             else
-                mac_masked<KF, COMPACT, C>(Gs, Om, zrow, f, s[i].meta, S[i], acc);
+                mac_masked<KF, C>(Gs, fbyte, rowoff[e + i], S[i], acc);
         }
         acc.flush();
     }
@@ -631,45 +630,49 @@ __device__ inline void mac_range(const MacSrc* __restrict__ srcs, const C* __res
         const C      S0 = sp[(size_t)s0.slot * NF];
         if constexpr (FULL) mac_full<KF, C>(Gs, f, s0.meta, S0, acc);
         else
-            mac_masked<KF, COMPACT, C>(Gs, Om, zrow, f, s0.meta, S0, acc);
+            mac_masked<KF, C>(Gs, fbyte, rowoff[e], S0, acc);
     }
     acc.flush();
 }
 
 // One launch covers the sibling groups of one target level, for k components
-// of the same tree.  A workgroup owns KF frequencies: it loads kernel spectra
-// into LDS and walks every (group, component) pair, the components of a group
-// on neighbouring threads (same entries, same kernel rows); a thread keeps the
-// 8 target accumulators of one group and component at one frequency.  A
-// group's entries that act on all 8 targets come first (nfull of them); the
-// rest carry partial masks.  COMPACT launches (few offsets used, the small
-// coarse-level launches) load only the rows in olist, map offsets through
-// omap and take every entry through the masked form; the others load all 343
-// rows and index them directly.
-constexpr int mac_max_threads(bool compact) { return compact ? 512 : 1024; }
+// of the same tree.  A workgroup owns KF frequencies (KF * sizeof(C) = 64 B
+// per kernel row): it loads kernel spectra into LDS and walks every (group,
+// component) pair, the components of a group on neighbouring threads (same
+// entries, same kernel rows); a thread keeps the 8 target accumulators of one
+// group and component at one frequency.  A group's entries that act on every
+// target come first (nfull of them) and index the table directly; the rest
+// carry partial masks and read their rows through rowoff.  COMPACT launches
+// (few offsets used, the small coarse-level launches) load only the rows in
+// olist and take every entry through rowoff; the others load all 343 rows.
+constexpr int kMacMaxThreads = 1024;
 
 template<int N, int KF, bool COMPACT, class T>
-__global__ __launch_bounds__(mac_max_threads(COMPACT)) void k_spectral_mac(const MacGroup* __restrict__ groups, int ngroups,
-    const MacSrc* __restrict__ srcs, const int* __restrict__ olist, int nused, const short* __restrict__ omap,
-    const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
+__global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup* __restrict__ groups,
+    int ngroups, const MacSrc* __restrict__ srcs, const uint4* __restrict__ rowoff, const int* __restrict__ olist,
+    int nused, const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
     double2* __restrict__ tspec, int k, long scs, long tcs)
 {
     using C = typename Cx<T>::type;
+    static_assert(KF * sizeof(C) == kMacRowBytes, "row offsets assume 64-byte kernel rows");
     constexpr int NF = Geo<N>::NF;
     const int     nt = blockDim.x;  // a multiple of KF, sized to the launch's group count
     const int     GPW = nt / KF;
-    __shared__ C     Gs[344 * KF];  // the launch's rows, then one all-zero row
-    __shared__ short Om[COMPACT ? 343 : 1];
-    const int f0 = blockIdx.x * KF;
+    __shared__ C  Gs[344 * KF];  // the launch's rows, then one all-zero row
+    // Workgroups are dealt to the 8 XCDs round-robin (workgroup i on XCD
+    // i mod 8).  A source spectrum's 128-byte line holds 2 (FP32) or 4 (FP64)
+    // frequency chunks, so neighbouring chunks are given to workgroups of the
+    // same XCD, which then share each line in that XCD's L2: XCD x owns a
+    // contiguous range of chunks (a bijection of the grid onto the chunks).
+    const int nwg = gridDim.x, xcd = blockIdx.x & 7, q = nwg >> 3, rem = nwg & 7;
+    const int f0 = (xcd * q + min(xcd, rem) + (int)(blockIdx.x >> 3)) * KF;
     const int rows = COMPACT ? nused : 343;
-    for (int q = threadIdx.x; q < (rows + 1) * KF; q += nt)
+    for (int i = threadIdx.x; i < (rows + 1) * KF; i += nt)
     {
-        const int c = q / KF, f = q - c * KF;
+        const int c = i / KF, f = i - c * KF;
         const int o = COMPACT ? olist[min(c, nused - 1)] : min(c, 342);
-        Gs[q] = (c < rows && f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
+        Gs[i] = (c < rows && f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
     }
-    if (COMPACT)
-        for (int q = threadIdx.x; q < 343; q += nt) Om[q] = omap[q];
     __syncthreads();
     const int f = threadIdx.x % KF;
     const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
@@ -689,9 +692,9 @@ __global__ __launch_bounds__(mac_max_threads(COMPACT)) void k_spectral_mac(const
         if constexpr (!COMPACT)
         {
             e1 = e0 + grp.nfull;
-            mac_range<true, NF, KF, COMPACT, C>(srcs, sp, Gs, Om, rows, f, e0, e1, acc);
+            mac_range<true, NF, KF, C>(srcs, rowoff, sp, Gs, f, e0, e1, acc);
         }
-        mac_range<false, NF, KF, COMPACT, C>(srcs, sp, Gs, Om, rows, f, e1, e2, acc);
+        mac_range<false, NF, KF, C>(srcs, rowoff, sp, Gs, f, e1, e2, acc);
         if (f0 + f < NF)
         {
 #pragma unroll
@@ -943,23 +946,22 @@ template<int N> struct RunInv
     }
 };
 // Workgroup size of a MAC launch: every (group, component) pair of the launch
-// in one pass when it fits (at most the variant's maximum), in whole waves.
-inline int mac_threads(long nvg, int kf, bool compact)
+// in one pass when it fits (at most kMacMaxThreads), in whole waves.
+inline int mac_threads(long nvg, int kf)
 {
-    const long want = ((nvg * kf + 63) / 64) * 64, cap = mac_max_threads(compact);
-    return want < 128 ? 128 : (want > cap ? (int)cap : (int)want);
+    const long want = ((nvg * kf + 63) / 64) * 64;
+    return want < 128 ? 128 : (want > kMacMaxThreads ? kMacMaxThreads : (int)want);
 }
 
 template<int N> struct RunMac
 {
-    static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const int* olist,
-        int nused, const short* omap, const void* ghat, const void* ss, void* ts, const Batch& b)
+    static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const uint4* rowoff,
+        const int* olist, int nused, bool compact, const void* ghat, const void* ss, void* ts, const Batch& b)
     {
-        const bool compact = nused <= 200;
 #define IBLGF_MAC(KF, CP, T, C)                                                                                  \
     hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3((Geo<N>::NF + KF - 1) / KF),                          \
-        dim3(mac_threads((long)ng * b.k, KF, CP)), 0, s, g, ng, src, olist, nused, omap, (const C*)ghat, (const C*)ss,  \
-        (double2*)ts, b.k, b.sstride, b.sstride)
+        dim3(mac_threads((long)ng * b.k, KF)), 0, s, g, ng, src, rowoff, olist, nused, (const C*)ghat,          \
+        (const C*)ss, (double2*)ts, b.k, b.sstride, b.sstride)
         if (prec == 0)
         {
             if (compact) IBLGF_MAC(4, true, double, double2);
@@ -1040,13 +1042,14 @@ void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* sl
     if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale, b);
 }
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec,
-    const Batch& b)
+    const uint4* rowoff, const int* olist, int nused, bool compact, const void* ghat, const void* sspec,
+    void* tspec, const Batch& b)
 {
     if (nused < 1 || nused > 343) throw std::runtime_error("offset count out of range");
+    if (compact && nused > kMacCompactMax) throw std::runtime_error("compact launch with too many offsets");
     if (b.k < 1 || b.k > 65535) throw std::runtime_error("component count out of range");
     if (ngroups > 0)
-        dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, olist, nused, omap, ghat, sspec, tspec, b);
+        dispatch_n<RunMac>(n, s, prec, groups, ngroups, srcs, rowoff, olist, nused, compact, ghat, sspec, tspec, b);
 }
 void to_float2(hipStream_t s, const double2* in, float2* out, long n)
 {

@@ -44,6 +44,12 @@ struct Batch
     long sstride = 0;
 };
 
+// Bytes per staged kernel row of the multiply-accumulate (KF frequencies of
+// one offset: 4 FP64 or 8 FP32 complex values), and the largest offset count
+// of a compact launch.
+constexpr int kMacRowBytes = 64;
+constexpr int kMacCompactMax = 200;
+
 namespace launch {
 
 bool supported_n(int n);
@@ -76,11 +82,14 @@ void forward_dft(hipStream_t s, int n, int prec, const double* field, const int*
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale, const Batch& b = {});
 // olist[nused]: the offsets (0..342, o = (ox+3) + 7 (oy+3) + 49 (oz+3)) the
-// launch's entries use; omap[343]: offset -> row in olist (rows of unused
-// offsets are never read).
+// launch's entries use.  A compact launch (nused <= kMacCompactMax) stages
+// only those rows, row i = olist[i]; otherwise row o = offset o.  Row nrows
+// (nused or 343) is all zeros.  rowoff[e] holds entry e's 8 LDS byte offsets
+// row * kMacRowBytes (target child t in 16-bit field t; children outside the
+// entry's mask point at the zero row).
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
-    const int* olist, int nused, const short* omap, const void* ghat, const void* sspec, void* tspec,
-    const Batch& b = {});
+    const uint4* rowoff, const int* olist, int nused, bool compact, const void* ghat, const void* sspec,
+    void* tspec, const Batch& b = {});
 void to_float2(hipStream_t s, const double2* in, float2* out, long n);
 
 // Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:
