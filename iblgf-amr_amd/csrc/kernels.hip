@@ -286,18 +286,33 @@ struct Cx<float>
     static __device__ inline float2 make(float a, float b) { return make_float2(a, b); }
 };
 
+constexpr int kDftThreads = 384;
+// Workgroups of the DFT kernels resident per CU (LDS split between them), and
+// the waves per SIMD that the register budget must allow for it.  One
+// workgroup per CU with GS = 9 planes per group measured faster at n = 18 than
+// two with GS = 3 (the legs of a small group leave most threads idle).
+constexpr int kDftGroupsPerCu = 1;
+constexpr int kDftWavesPerEu = (kDftGroupsPerCu * kDftThreads / 64 + 3) / 4;
+
 template<int N>
 struct DftPlan
 {
     static constexpr int P = 2 * N - 1;
     static constexpr int H = P / 2 + 1;
-    static constexpr int kLds = 163840 - 4096;
+    // kDftGroupsPerCu workgroups share the 160 KB LDS (less a margin).
+    static constexpr int kLds = 163840 / kDftGroupsPerCu - 2048;
     // kx-planes per group: the twiddle table (P*N complex) plus, per plane,
-    // one n x n and one n x P complex slab of intermediates.
+    // one n x n and one n x P complex slab of intermediates; the largest
+    // divisor of H that fits, so every group is full.
     static constexpr int kRaw = (kLds - P * N * 16) / (N * N * 16 + N * P * 16);
-    static constexpr int GS = kRaw < H ? kRaw : H;
+    static constexpr int largest_divisor(int h, int cap)
+    {
+        int d = cap < h ? cap : h;
+        while (d > 1 && h % d) --d;
+        return d;
+    }
+    static constexpr int GS = largest_divisor(H, kRaw);
 };
-constexpr int kDftThreads = 384;
 
 // a * (c - i s)
 template<class C, class T>
@@ -314,6 +329,21 @@ __device__ inline void cmac(C a, C w, T& sr, T& si)
     si = fma(a.x, w.y, fma(a.y, w.x, si));
 }
 
+// Work split of one DFT leg over the workgroup: `lines` independent lines,
+// each producing `outs` outputs; when there are fewer lines than threads the
+// outputs of a line are divided into `parts` contiguous ranges of at most
+// `per` outputs, item i = part * lines + line (threads of one wave share the
+// part, so their twiddle reads are broadcasts).
+template<int LINES, int OUTS>
+struct LegSplit
+{
+    static constexpr int lines = LINES;
+    static constexpr int raw = kDftThreads / LINES;
+    static constexpr int parts = raw < 1 ? 1 : (raw > OUTS ? OUTS : raw);
+    static constexpr int per = (OUTS + parts - 1) / parts;
+    static constexpr int items = LINES * parts;
+};
+
 // Forward r2c of an n^3 block zero-padded to P^3 (pruned legs), in groups of
 // GS kx-planes: the x leg for the group's planes (the block's n^2 x-lines are
 // re-read from L2 per group), then the y leg (outputs split over two threads
@@ -322,7 +352,7 @@ __device__ inline void cmac(C a, C w, T& sr, T& si)
 // modular index arithmetic.  Arithmetic and LDS intermediates are FP64; the
 // spectrum is stored as T (double: plain FP64 path; float: mixed path).
 template<int N, class T>
-__global__ __launch_bounds__(kDftThreads) void k_forward_dft(const double* __restrict__ field,
+__global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDftWavesPerEu))) void k_forward_dft(const double* __restrict__ field,
     const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec, long fcs, long scs)
 {
     using G = Geo<N>;
@@ -410,17 +440,21 @@ __global__ __launch_bounds__(kDftThreads) void k_forward_dft(const double* __res
 
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
 // added into the target block.  Per group of GS kx-planes: z leg from global
-// memory, y leg in LDS, and the group's share of the c2r x leg accumulated in
-// registers (a thread owns one output x-line).  Twiddles come from a table
-// I[o][k] = w^((n-1+o) k).  FP64 throughout.
+// memory, y leg in LDS (both with outputs in registers and streamed inputs,
+// split over the threads as LegSplit), and the group's share of the c2r x leg
+// accumulated in registers (a thread owns one output x-line).  Twiddles come
+// from a table I[o][k] = w^((n-1+o) k).  FP64 throughout.
 template<int N>
-__global__ __launch_bounds__(kDftThreads) void k_inverse_dft(const double2* __restrict__ spec,
-    const int* __restrict__ slots, int count, double* __restrict__ field, double scale, long scs, long fcs)
+__global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDftWavesPerEu))) void k_inverse_dft(
+    const double2* __restrict__ spec, const int* __restrict__ slots, int count, double* __restrict__ field,
+    double scale, long scs, long fcs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
-    static_assert(GS >= 1, "LDS too small for one plane");
+    static_assert(GS >= 1 && H % GS == 0, "groups of whole kx-planes");
     static_assert(N * N <= kDftThreads, "one output line per thread");
+    using SZ = LegSplit<GS * P, N>;  // z leg: lines (g, ky), outputs z
+    using SY = LegSplit<GS * N, N>;  // y leg: lines (g, z), outputs y
     __shared__ double2 Iw[N * P];       // [o][k]
     __shared__ double2 U[GS * P * N];   // [g][ky][z]
     __shared__ double2 V[GS * N * N];   // [g][z][y]
@@ -439,51 +473,57 @@ __global__ __launch_bounds__(kDftThreads) void k_inverse_dft(const double2* __re
 #pragma unroll
     for (int x = 0; x < N; ++x) acc[x] = 0.0;
     __syncthreads();
-    constexpr int Y0 = (N + 1) / 2;
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
-        const int gs = min(GS, H - kx0);
-        // z leg: (g, ky) rows of P frequencies -> N window points
-        for (int l = threadIdx.x; l < gs * P; l += kDftThreads)
+        // z leg: row (g, ky) of P frequencies -> window points z in [z0, z0 + per)
+        for (int it = threadIdx.x; it < SZ::items; it += kDftThreads)
         {
-            const int      g = l / P, ky = l - g * P, kx = kx0 + g;
+            const int      part = it / SZ::lines, line = it - part * SZ::lines;
+            const int      g = line / P, ky = line - g * P, kx = kx0 + g;
+            const int      z0 = part * SZ::per;
             const double2* row = in + (size_t)(kx * P + ky) * P;
-            double2        a[P];
+            double         sr[SZ::per], si[SZ::per];
 #pragma unroll
-            for (int kz = 0; kz < P; ++kz) a[kz] = row[kz];
-            for (int z = 0; z < N; ++z)
+            for (int o = 0; o < SZ::per; ++o) sr[o] = si[o] = 0.0;
+#pragma unroll 7
+            for (int kz = 0; kz < P; ++kz)
             {
-                const double2* w = Iw + z * P;
-                double         sr = 0, si = 0;
+                const double2 a = row[kz];
 #pragma unroll
-                for (int kz = 0; kz < P; ++kz) cmac(a[kz], w[kz], sr, si);
-                U[(g * P + ky) * N + z] = make_double2(sr, si);
+                for (int o = 0; o < SZ::per; ++o)
+                    if (z0 + o < N) cmac(a, Iw[(z0 + o) * P + kz], sr[o], si[o]);
             }
+#pragma unroll
+            for (int o = 0; o < SZ::per; ++o)
+                if (z0 + o < N) U[(g * P + ky) * N + z0 + o] = make_double2(sr[o], si[o]);
         }
         __syncthreads();
-        // y leg: (g, z) lines of P frequencies -> N window points, outputs split over two threads
-        for (int l = threadIdx.x; l < 2 * gs * N; l += kDftThreads)
+        // y leg: line (g, z) of P frequencies -> window points y in [y0, y0 + per)
+        for (int it = threadIdx.x; it < SY::items; it += kDftThreads)
         {
-            const int half = l / (gs * N), line = l - half * gs * N;
+            const int part = it / SY::lines, line = it - part * SY::lines;
             const int g = line / N, z = line - g * N;
-            double2   a[P];
+            const int y0 = part * SY::per;
+            double    sr[SY::per], si[SY::per];
 #pragma unroll
-            for (int ky = 0; ky < P; ++ky) a[ky] = U[(g * P + ky) * N + z];
-            const int y_lo = half ? Y0 : 0, y_hi = half ? N : Y0;
-            for (int y = y_lo; y < y_hi; ++y)
+            for (int o = 0; o < SY::per; ++o) sr[o] = si[o] = 0.0;
+#pragma unroll 7
+            for (int ky = 0; ky < P; ++ky)
             {
-                const double2* w = Iw + y * P;
-                double         sr = 0, si = 0;
+                const double2 a = U[(g * P + ky) * N + z];
 #pragma unroll
-                for (int ky = 0; ky < P; ++ky) cmac(a[ky], w[ky], sr, si);
-                V[(g * N + z) * N + y] = make_double2(sr, si);
+                for (int o = 0; o < SY::per; ++o)
+                    if (y0 + o < N) cmac(a, Iw[(y0 + o) * P + ky], sr[o], si[o]);
             }
+#pragma unroll
+            for (int o = 0; o < SY::per; ++o)
+                if (y0 + o < N) V[(g * N + z) * N + y0 + o] = make_double2(sr[o], si[o]);
         }
         __syncthreads();
         // the group's part of the c2r x leg: line l = (z, y) -> N window points
         const int l = threadIdx.x;
         if (l < N * N)
-            for (int g = 0; g < gs; ++g)
+            for (int g = 0; g < GS; ++g)
             {
                 const int     kx = kx0 + g;
                 const double2 a = V[g * N * N + l];
