@@ -102,11 +102,14 @@ template<int N>
 struct TransferLds
 {
     double buf[Geo<N>::N3];
-    double m[2][N * N];
 };
 
+// out[base + o*stride] = sum_a K[o][a] v[a].  K is read from global memory at
+// indices uniform across the workgroup, so the reads are scalar loads and
+// the FMAs take the matrix from scalar registers.
 template<int N>
-__device__ inline void line_apply(const double (&v)[N], const double* K, double* out, int base, int stride)
+__device__ inline void line_apply(const double (&v)[N], const double* __restrict__ K, double* out, int base,
+    int stride)
 {
 #pragma unroll 2
     for (int o = 0; o < N; ++o)
@@ -120,51 +123,51 @@ __device__ inline void line_apply(const double (&v)[N], const double* K, double*
 
 // x and y legs: global block -> LDS buffer (x), then y in place.
 template<int N>
-__device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>& L, int bx, int by)
+__device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>& L, const double* __restrict__ Kx,
+    const double* __restrict__ Ky)
 {
-    for (int l = threadIdx.x; l < N * N; l += kTransferThreads)
+    static_assert(N * N <= kTransferThreads, "one line per thread");
+    const int l = threadIdx.x;
+    if (l < N * N)
     {
         double v[N];
 #pragma unroll
         for (int a = 0; a < N; ++a) v[a] = in[l * N + a];
-        line_apply<N>(v, L.m[bx], L.buf, l * N, 1);
+        line_apply<N>(v, Kx, L.buf, l * N, 1);
     }
     __syncthreads();
-    for (int l = threadIdx.x; l < N * N; l += kTransferThreads)
+    if (l < N * N)
     {
         const int base = (l / N) * N * N + (l % N);
         double    v[N];
 #pragma unroll
         for (int b = 0; b < N; ++b) v[b] = L.buf[base + b * N];
-        line_apply<N>(v, L.m[by], L.buf, base, N);  // the thread owns this line
+        line_apply<N>(v, Ky, L.buf, base, N);  // the thread owns this line
     }
     __syncthreads();
 }
 
-// z leg straight to global memory: out[k][j][i] (+)= sum_c Kz[k][c] t2[c][j][i]
+// z leg of line l = (j, i): acc[k] (+)= sum_c Kz[k][c] t2[c][j][i]
 template<int N>
-__device__ inline void leg_z_store(const double* t2, double* out, const double* Kz, bool accumulate)
+__device__ inline void leg_z_acc(const double* t2, const double* __restrict__ Kz, double (&acc)[N])
 {
-    for (int l = threadIdx.x; l < N * N; l += kTransferThreads)
+    const int l = threadIdx.x;
+    double    v[N];
+#pragma unroll
+    for (int c = 0; c < N; ++c) v[c] = t2[c * N * N + l];
+#pragma unroll
+    for (int k = 0; k < N; ++k)  // fully unrolled: acc stays in registers
     {
-        double v[N];
+        double s = 0.0;
 #pragma unroll
-        for (int c = 0; c < N; ++c) v[c] = t2[c * N * N + l];
-#pragma unroll 2
-        for (int k = 0; k < N; ++k)
-        {
-            double s = 0.0;
-#pragma unroll
-            for (int c = 0; c < N; ++c) s = fma(Kz[k * N + c], v[c], s);
-            if (accumulate) out[k * N * N + l] += s;
-            else
-                out[k * N * N + l] = s;
-        }
+        for (int c = 0; c < N; ++c) s = fma(Kz[k * N + c], v[c], s);
+        acc[k] += s;
     }
 }
 
-// dst[parent] += sum over listed children of K(child); one workgroup per parent,
-// so the children are added in a fixed order (deterministic).
+// dst[parent] += sum over listed children of K(child); one workgroup per
+// parent, the children's contributions summed in registers in child order and
+// added to the parent once (deterministic).
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
@@ -177,18 +180,28 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     src += blockIdx.y * cs;
     dst += blockIdx.y * cs;
     const int* ent = entries + 9 * e;
-    for (int q = threadIdx.x; q < 2 * N * N; q += kTransferThreads) (&L.m[0][0])[q] = mats[q];
-    double* out = dst + (size_t)ent[0] * N3;
+    double     acc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) acc[k] = 0.0;
+    bool first = true;
     for (int c = 0; c < 8; ++c)
     {
         const int ch = ent[1 + c];
         if (ch < 0) continue;  // uniform across the workgroup
-        __syncthreads();       // matrices loaded / previous z leg done with the buffer
-        transfer_xy<N>(src + (size_t)ch * N3, L, c & 1, (c >> 1) & 1);
-        leg_z_store<N>(L.buf, out, L.m[(c >> 2) & 1], true);
+        if (!first) __syncthreads();  // previous z leg done with the buffer
+        first = false;
+        transfer_xy<N>(src + (size_t)ch * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N);
+        if (threadIdx.x < N * N) leg_z_acc<N>(L.buf, mats + ((c >> 2) & 1) * N * N, acc);
+    }
+    if (threadIdx.x < N * N)
+    {
+        double* out = dst + (size_t)ent[0] * N3 + threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < N; ++k) out[k * N * N] += acc[k];
     }
 }
 
+// dst[entry.child] (=|+=) K(src[entry.parent])
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
@@ -201,10 +214,22 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
     src += blockIdx.y * cs;
     dst += blockIdx.y * cs;
     const int child = entries[3 * e], parent = entries[3 * e + 1], c = entries[3 * e + 2];
-    for (int q = threadIdx.x; q < 2 * N * N; q += kTransferThreads) (&L.m[0][0])[q] = mats[q];
-    __syncthreads();
-    transfer_xy<N>(src + (size_t)parent * N3, L, c & 1, (c >> 1) & 1);
-    leg_z_store<N>(L.buf, dst + (size_t)child * N3, L.m[(c >> 2) & 1], accumulate != 0);
+    transfer_xy<N>(src + (size_t)parent * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N);
+    if (threadIdx.x < N * N)
+    {
+        double acc[N];
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc[k] = 0.0;
+        leg_z_acc<N>(L.buf, mats + ((c >> 2) & 1) * N * N, acc);
+        double* out = dst + (size_t)child * N3 + threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+        {
+            if (accumulate) out[k * N * N] += acc[k];
+            else
+                out[k * N * N] = acc[k];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
