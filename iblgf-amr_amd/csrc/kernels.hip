@@ -359,11 +359,11 @@ __device__ inline void cmac(C a, C w, T& sr, T& si)
 // outputs of a line are divided into `parts` contiguous ranges of at most
 // `per` outputs, item i = part * lines + line (threads of one wave share the
 // part, so their twiddle reads are broadcasts).
-template<int LINES, int OUTS>
+template<int LINES, int OUTS, int THREADS = kDftThreads>
 struct LegSplit
 {
     static constexpr int lines = LINES;
-    static constexpr int raw = kDftThreads / LINES;
+    static constexpr int raw = THREADS / LINES;
     static constexpr int parts = raw < 1 ? 1 : (raw > OUTS ? OUTS : raw);
     static constexpr int per = (OUTS + parts - 1) / parts;
     static constexpr int items = LINES * parts;
@@ -465,21 +465,24 @@ __global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDf
 
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
 // added into the target block.  Per group of GS kx-planes: z leg from global
-// memory, y leg in LDS (both with outputs in registers and streamed inputs,
-// split over the threads as LegSplit), and the group's share of the c2r x leg
-// accumulated in registers (a thread owns one output x-line).  Twiddles come
-// from a table I[o][k] = w^((n-1+o) k).  FP64 throughout.
+// memory, y leg in LDS, and the group's share of the c2r x leg accumulated in
+// registers; every leg keeps its outputs in registers, streams its inputs and
+// is split over the workgroup's kInvThreads threads as LegSplit (twice the
+// forward kernel's threads: the same LDS footprint then holds 12 waves per
+// CU).  Twiddles come from a table I[o][k] = w^((n-1+o) k).  FP64 throughout.
+constexpr int kInvThreads = 768;
+
 template<int N>
-__global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDftWavesPerEu))) void k_inverse_dft(
-    const double2* __restrict__ spec, const int* __restrict__ slots, int count, double* __restrict__ field,
-    double scale, long scs, long fcs)
+__global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __restrict__ spec,
+    const int* __restrict__ slots, int count, double* __restrict__ field, double scale, long scs, long fcs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
     static_assert(GS >= 1 && H % GS == 0, "groups of whole kx-planes");
-    static_assert(N * N <= kDftThreads, "one output line per thread");
-    using SZ = LegSplit<GS * P, N>;  // z leg: lines (g, ky), outputs z
-    using SY = LegSplit<GS * N, N>;  // y leg: lines (g, z), outputs y
+    using SZ = LegSplit<GS * P, N, kInvThreads>;  // z leg: lines (g, ky), outputs z
+    using SY = LegSplit<GS * N, N, kInvThreads>;  // y leg: lines (g, z), outputs y
+    using SX = LegSplit<N * N, N, kInvThreads>;   // x leg: lines (z, y), outputs x
+    static_assert(SX::items <= kInvThreads, "one x-leg item per thread");
     __shared__ double2 Iw[N * P];       // [o][k]
     __shared__ double2 U[GS * P * N];   // [g][ky][z]
     __shared__ double2 V[GS * N * N];   // [g][z][y]
@@ -487,21 +490,24 @@ __global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDf
     if (b >= count) return;
     const int      slot = slots[b];
     const double2* in = spec + blockIdx.y * scs + (size_t)slot * G::NF;
-    for (int q = threadIdx.x; q < N * P; q += kDftThreads)
+    for (int q = threadIdx.x; q < N * P; q += kInvThreads)
     {
         const int o = q / P, k = q - o * P;
         double    sn, cs;
         sincospi(2.0 * (((N - 1 + o) * k) % P) / P, &sn, &cs);
         Iw[q] = make_double2(cs, sn);
     }
-    double acc[N];
+    // x-leg item of this thread: line (z, y), outputs x in [x0, x0 + per)
+    const int xpart = threadIdx.x / SX::lines, xline = threadIdx.x - xpart * SX::lines;
+    const int x0 = xpart * SX::per;
+    double    acc[SX::per];
 #pragma unroll
-    for (int x = 0; x < N; ++x) acc[x] = 0.0;
+    for (int x = 0; x < SX::per; ++x) acc[x] = 0.0;
     __syncthreads();
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
         // z leg: row (g, ky) of P frequencies -> window points z in [z0, z0 + per)
-        for (int it = threadIdx.x; it < SZ::items; it += kDftThreads)
+        for (int it = threadIdx.x; it < SZ::items; it += kInvThreads)
         {
             const int      part = it / SZ::lines, line = it - part * SZ::lines;
             const int      g = line / P, ky = line - g * P, kx = kx0 + g;
@@ -524,7 +530,7 @@ __global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDf
         }
         __syncthreads();
         // y leg: line (g, z) of P frequencies -> window points y in [y0, y0 + per)
-        for (int it = threadIdx.x; it < SY::items; it += kDftThreads)
+        for (int it = threadIdx.x; it < SY::items; it += kInvThreads)
         {
             const int part = it / SY::lines, line = it - part * SY::lines;
             const int g = line / N, z = line - g * N;
@@ -545,28 +551,29 @@ __global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDf
                 if (y0 + o < N) V[(g * N + z) * N + y0 + o] = make_double2(sr[o], si[o]);
         }
         __syncthreads();
-        // the group's part of the c2r x leg: line l = (z, y) -> N window points
-        const int l = threadIdx.x;
-        if (l < N * N)
+        // the group's part of the c2r x leg: line (z, y) -> window points x in [x0, x0 + per)
+        if (threadIdx.x < SX::items)
             for (int g = 0; g < GS; ++g)
             {
                 const int     kx = kx0 + g;
-                const double2 a = V[g * N * N + l];
+                const double2 a = V[g * N * N + xline];
                 const double  c = (kx == 0) ? 1.0 : 2.0;  // P odd: no Nyquist plane
 #pragma unroll
-                for (int x = 0; x < N; ++x)
-                {
-                    const double2 w = Iw[x * P + kx];
-                    acc[x] = fma(c, fma(a.x, w.x, -a.y * w.y), acc[x]);
-                }
+                for (int x = 0; x < SX::per; ++x)
+                    if (x0 + x < N)
+                    {
+                        const double2 w = Iw[(x0 + x) * P + kx];
+                        acc[x] = fma(c, fma(a.x, w.x, -a.y * w.y), acc[x]);
+                    }
             }
         __syncthreads();
     }
-    if (threadIdx.x < N * N)
+    if (threadIdx.x < SX::items)
     {
-        double* out = field + blockIdx.y * fcs + (size_t)slot * G::N3 + threadIdx.x * N;
+        double* out = field + blockIdx.y * fcs + (size_t)slot * G::N3 + xline * N + x0;
 #pragma unroll
-        for (int x = 0; x < N; ++x) out[x] += scale * acc[x];
+        for (int x = 0; x < SX::per; ++x)
+            if (x0 + x < N) out[x] += scale * acc[x];
     }
 }
 
@@ -1006,7 +1013,7 @@ template<int N> struct RunInv
         const Batch& b)
     {
         (void)prec;  // target spectra are FP64 on both paths
-        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count, b.k), dim3(kDftThreads), 0, s, (const double2*)spec, slots,
+        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count, b.k), dim3(kInvThreads), 0, s, (const double2*)spec, slots,
             count, f, scale, b.sstride, b.fstride);
     }
 };
