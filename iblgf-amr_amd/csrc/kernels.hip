@@ -311,21 +311,15 @@ struct Cx<float>
     static __device__ inline float2 make(float a, float b) { return make_float2(a, b); }
 };
 
-constexpr int kDftThreads = 384;
-// Workgroups of the DFT kernels resident per CU (LDS split between them), and
-// the waves per SIMD that the register budget must allow for it.  One
-// workgroup per CU with GS = 9 planes per group measured faster at n = 18 than
-// two with GS = 3 (the legs of a small group leave most threads idle).
-constexpr int kDftGroupsPerCu = 1;
-constexpr int kDftWavesPerEu = (kDftGroupsPerCu * kDftThreads / 64 + 3) / 4;
-
+// One DFT workgroup per CU takes nearly all of the 160 KB LDS: at n = 18 a
+// group of GS = 9 kx-planes with 768 threads (12 waves) measured faster than
+// two workgroups per CU with GS = 3, whose legs leave most threads idle.
 template<int N>
 struct DftPlan
 {
     static constexpr int P = 2 * N - 1;
     static constexpr int H = P / 2 + 1;
-    // kDftGroupsPerCu workgroups share the 160 KB LDS (less a margin).
-    static constexpr int kLds = 163840 / kDftGroupsPerCu - 2048;
+    static constexpr int kLds = 163840 - 2048;
     // kx-planes per group: the twiddle table (P*N complex) plus, per plane,
     // one n x n and one n x P complex slab of intermediates; the largest
     // divisor of H that fits, so every group is full.
@@ -359,7 +353,7 @@ __device__ inline void cmac(C a, C w, T& sr, T& si)
 // outputs of a line are divided into `parts` contiguous ranges of at most
 // `per` outputs, item i = part * lines + line (threads of one wave share the
 // part, so their twiddle reads are broadcasts).
-template<int LINES, int OUTS, int THREADS = kDftThreads>
+template<int LINES, int OUTS, int THREADS>
 struct LegSplit
 {
     static constexpr int lines = LINES;
@@ -371,18 +365,25 @@ struct LegSplit
 
 // Forward r2c of an n^3 block zero-padded to P^3 (pruned legs), in groups of
 // GS kx-planes: the x leg for the group's planes (the block's n^2 x-lines are
-// re-read from L2 per group), then the y leg (outputs split over two threads
-// per line) and the z leg to global memory; output spec[kx][ky][kz].  The
-// twiddles come from a table T[k][j] = w^(j k) so the inner loops carry no
-// modular index arithmetic.  Arithmetic and LDS intermediates are FP64; the
-// spectrum is stored as T (double: plain FP64 path; float: mixed path).
+// re-read from L2 per group), then the y leg and the z leg to global memory;
+// output spec[kx][ky][kz].  Every leg keeps its outputs in registers, streams
+// its inputs and is split over the workgroup's kFwdThreads threads as
+// LegSplit.  Twiddles come from a table T[k][j] = w^(j k), so the inner loops
+// carry no modular index arithmetic.  Arithmetic and LDS intermediates are
+// FP64; the spectrum is stored as T (double: plain FP64 path; float: mixed
+// path).
+constexpr int kFwdThreads = 768;
+
 template<int N, class T>
-__global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDftWavesPerEu))) void k_forward_dft(const double* __restrict__ field,
+__global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __restrict__ field,
     const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec, long fcs, long scs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
-    static_assert(GS >= 1, "LDS too small for one plane");
+    static_assert(GS >= 1 && H % GS == 0, "groups of whole kx-planes");
+    using SX = LegSplit<N * N, GS, kFwdThreads>;   // x leg: lines (z, y), outputs kx in the group
+    using SY = LegSplit<GS * N, P, kFwdThreads>;   // y leg: lines (g, z), outputs ky
+    using SZ = LegSplit<GS * P, P, kFwdThreads>;   // z leg: lines (g, ky), outputs kz
     __shared__ double2 Tw[P * N];        // [k][j]
     __shared__ double2 X1[GS * N * N];   // [g][z][y]
     __shared__ double2 Tp[GS * N * P];   // [g][z][ky]
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDf
     if (b >= count) return;
     const int     slot = slots[b];
     const double* in = field + blockIdx.y * fcs + (size_t)slot * G::N3;
-    for (int q = threadIdx.x; q < P * N; q += kDftThreads)
+    for (int q = threadIdx.x; q < P * N; q += kFwdThreads)
     {
         const int k = q / N, j = q - k * N;
         double    sn, cs;
@@ -399,65 +400,79 @@ __global__ __launch_bounds__(kDftThreads) __attribute__((amdgpu_waves_per_eu(kDf
     }
     __syncthreads();
     typename Cx<T>::type* out = spec + blockIdx.y * scs + (size_t)slot * G::NF;
-    constexpr int         K0 = (P + 1) / 2;
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
-        const int gs = min(GS, H - kx0);
-        // x leg: line l = (z, y), real input -> gs complex outputs
-        for (int l = threadIdx.x; l < N * N; l += kDftThreads)
+        // x leg: real line (z, y) of N points -> planes g in [g0, g0 + per)
+        for (int it = threadIdx.x; it < SX::items; it += kFwdThreads)
         {
-            double v[N];
+            const int     part = it / SX::lines, line = it - part * SX::lines;
+            const int     g0 = part * SX::per;
+            const double* src = in + line * N;
+            double        sr[SX::per], si[SX::per];
 #pragma unroll
-            for (int x = 0; x < N; ++x) v[x] = in[l * N + x];
-            for (int g = 0; g < gs; ++g)
+            for (int o = 0; o < SX::per; ++o) sr[o] = si[o] = 0.0;
+#pragma unroll 2
+            for (int x = 0; x < N; ++x)
             {
-                const double2* w = Tw + (kx0 + g) * N;
-                double         sr = 0, si = 0;
+                const double v = src[x];
 #pragma unroll
-                for (int x = 0; x < N; ++x)
-                {
-                    sr = fma(v[x], w[x].x, sr);
-                    si = fma(-v[x], w[x].y, si);
-                }
-                X1[g * N * N + l] = make_double2(sr, si);
+                for (int o = 0; o < SX::per; ++o)
+                    if (g0 + o < GS)
+                    {
+                        const double2 w = Tw[(kx0 + g0 + o) * N + x];
+                        sr[o] = fma(v, w.x, sr[o]);
+                        si[o] = fma(-v, w.y, si[o]);
+                    }
             }
+#pragma unroll
+            for (int o = 0; o < SX::per; ++o)
+                if (g0 + o < GS) X1[(g0 + o) * N * N + line] = make_double2(sr[o], si[o]);
         }
         __syncthreads();
-        // y leg: line (g, z), outputs ky split over two threads
-        for (int l = threadIdx.x; l < 2 * gs * N; l += kDftThreads)
+        // y leg: line (g, z), N inputs y -> outputs ky in [k0, k0 + per)
+        for (int it = threadIdx.x; it < SY::items; it += kFwdThreads)
         {
-            const int half = l / (gs * N), line = l - half * gs * N;
+            const int part = it / SY::lines, line = it - part * SY::lines;
             const int g = line / N, z = line - g * N;
-            double2   a[N];
+            const int k0 = part * SY::per;
+            double    sr[SY::per], si[SY::per];
 #pragma unroll
-            for (int y = 0; y < N; ++y) a[y] = X1[g * N * N + z * N + y];
-            const int k_lo = half ? K0 : 0, k_hi = half ? P : K0;
-            for (int ky = k_lo; ky < k_hi; ++ky)
+            for (int o = 0; o < SY::per; ++o) sr[o] = si[o] = 0.0;
+            const double2* src = X1 + g * N * N + z * N;
+#pragma unroll 1
+            for (int y = 0; y < N; ++y)
             {
-                const double2* w = Tw + ky * N;
-                double         sr = 0, si = 0;
+                const double2 a = src[y];
 #pragma unroll
-                for (int y = 0; y < N; ++y) cmac_conj(a[y], w[y], sr, si);
-                Tp[(g * N + z) * P + ky] = make_double2(sr, si);
+                for (int o = 0; o < SY::per; ++o)
+                    if (k0 + o < P) cmac_conj(a, Tw[(k0 + o) * N + y], sr[o], si[o]);
             }
+#pragma unroll
+            for (int o = 0; o < SY::per; ++o)
+                if (k0 + o < P) Tp[(g * N + z) * P + k0 + o] = make_double2(sr[o], si[o]);
         }
         __syncthreads();
-        // z leg: line (g, ky) -> P outputs kz, to global memory
-        for (int l = threadIdx.x; l < gs * P; l += kDftThreads)
+        // z leg: line (g, ky), N inputs z -> outputs kz in [k0, k0 + per), to global memory
+        for (int it = threadIdx.x; it < SZ::items; it += kFwdThreads)
         {
-            const int g = l / P, ky = l - g * P, kx = kx0 + g;
-            double2   a[N];
+            const int part = it / SZ::lines, line = it - part * SZ::lines;
+            const int g = line / P, ky = line - g * P, kx = kx0 + g;
+            const int k0 = part * SZ::per;
+            double    sr[SZ::per], si[SZ::per];
 #pragma unroll
-            for (int z = 0; z < N; ++z) a[z] = Tp[(g * N + z) * P + ky];
-            typename Cx<T>::type* o = out + (size_t)(kx * P + ky) * P;
-            for (int kz = 0; kz < P; ++kz)
+            for (int o = 0; o < SZ::per; ++o) sr[o] = si[o] = 0.0;
+#pragma unroll 1
+            for (int z = 0; z < N; ++z)
             {
-                const double2* w = Tw + kz * N;
-                double         sr = 0, si = 0;
+                const double2 a = Tp[(g * N + z) * P + ky];
 #pragma unroll
-                for (int z = 0; z < N; ++z) cmac_conj(a[z], w[z], sr, si);
-                o[kz] = Cx<T>::make((T)sr, (T)si);
+                for (int o = 0; o < SZ::per; ++o)
+                    if (k0 + o < P) cmac_conj(a, Tw[(k0 + o) * N + z], sr[o], si[o]);
             }
+            typename Cx<T>::type* o = out + (size_t)(kx * P + ky) * P + k0;
+#pragma unroll
+            for (int q = 0; q < SZ::per; ++q)
+                if (k0 + q < P) o[q] = Cx<T>::make((T)sr[q], (T)si[q]);
         }
         __syncthreads();
     }
@@ -1000,10 +1015,10 @@ template<int N> struct RunFwd
         const Batch& b)
     {
         if (prec == 0)
-            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count, b.k), dim3(kDftThreads), 0, s, f, slots, count,
+            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count, b.k), dim3(kFwdThreads), 0, s, f, slots, count,
                 (double2*)spec, b.fstride, b.sstride);
         else
-            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count, b.k), dim3(kDftThreads), 0, s, f, slots, count,
+            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count, b.k), dim3(kFwdThreads), 0, s, f, slots, count,
                 (float2*)spec, b.fstride, b.sstride);
     }
 };
