@@ -77,7 +77,11 @@ def learned_fcg(seed=1004, alpha=0.5, tol=1e-6):
     16^3-block mesh, roma delta).  Plain CG (the reference's algorithm) runs to
     the reference's threshold 1e-3 and, capped, towards tol.  The flexible
     conjugate-residual solver runs to tol with the identity and with the MLP
-    trained on the device from the iterates of earlier solves."""
+    trained on the device from the iterates of earlier solves; the learned
+    solve is monitored and falls back to the identity when the network
+    converges slower than the last identity solve.  The tolerance is 1e-6:
+    towards 1e-8 both plain CG (2000 iterations) and the flexible solver
+    (3000) stall near 2e-7 on this operator (half its spectrum below 1e-8)."""
     import numpy as np
 
     from iblgf_amd import IbSystem, LearnedPreconditioner, PoissonSolver, setups
@@ -102,10 +106,10 @@ def learned_fcg(seed=1004, alpha=0.5, tol=1e-6):
     f, it, res, _ = sysm.cg(s.uc, alpha, tol=1e-3, maxit=200)
     out["cg_to_1e-3"] = {"iterations": it, "seconds": sysm.stats()[0], "residual": res, "true_residual": true_res(f)}
     log(f"CG to 1e-3: {it} iterations")
-    f_cg, it, res, _ = sysm.cg(s.uc, alpha, tol=tol, maxit=1500)
+    f_cg, it, res, _ = sysm.cg(s.uc, alpha, tol=tol, maxit=2000)
     out["cg_to_tol"] = {"iterations": it, "seconds": sysm.stats()[0], "residual": res, "stop": sysm.stats()[3]}
     log(f"CG towards {tol}: {it} iterations, residual {res:.2e}")
-    f_id, it_id, res_id, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500, use_mlp=False)
+    f_id, it_id, res_id, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=3000, use_mlp=False)
     t_id = sysm.stats()[0]
     out["flexible_identity"] = {"iterations": it_id, "seconds": t_id, "residual": res_id,
                                 "true_residual": true_res(f_id), "stop": sysm.stats()[3]}
@@ -121,7 +125,7 @@ def learned_fcg(seed=1004, alpha=0.5, tol=1e-6):
             b = np.tile(d / np.linalg.norm(d), (s.npoints, 1))
         else:
             b = rng.standard_normal((s.npoints, 3))
-        sysm.fcg(b, alpha, tol=tol / 10, maxit=1000, use_mlp=False)
+        sysm.fcg(b, alpha, tol=max(tol, 1e-7), maxit=1000, use_mlp=False)
         log(f"earlier solve {k + 1}/{nsolves} done")
     t_collect = time.perf_counter() - t0
     pc.collect(0)
@@ -130,16 +134,18 @@ def learned_fcg(seed=1004, alpha=0.5, tol=1e-6):
     loss, npairs = pc.train(steps=2000, batch_rows=4096, lr=1e-3)
     t_train = time.perf_counter() - t0
     log(f"trained on {npairs} pairs in {t_train:.1f} s, loss {loss0:.3e} -> {loss:.3e}")
-    f_l, it_l, res_l, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500)
+    f_l, it_l, res_l, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=3000)
     t_l, _, fell, stop = sysm.stats()
     out["flexible_learned"] = {"iterations": it_l, "seconds": t_l, "residual": res_l, "true_residual": true_res(f_l),
-                               "fell_back": fell, "stop": stop}
+                               "fell_back": fell, "identity_from_iteration": sysm.switched_at(), "stop": stop}
     log(f"flexible (learned) to {tol}: {it_l} iterations")
     out.update({
         "workload": (f"IB sphere R=0.5, N={s.npoints} seeded points (seed {seed}), roma delta, "
                      f"{s.info['nblocks']} blocks of 16^3 over 3 levels; A = E (I - G L^-1 D) H_alpha E^T, "
                      f"alpha={alpha}; rhs u=(1,0,0); stop at sqrt(r.r/f.f) < {tol}"),
         "flexible_vs_cg_iterations": out["cg_to_tol"]["iterations"] / max(it_id, 1),
+        "learned_vs_cg_iterations": out["cg_to_tol"]["iterations"] / max(it_l, 1),
+        "learned_vs_cg_wall": out["cg_to_tol"]["seconds"] / max(t_l, 1e-12),
         "learned_iteration_gain": it_id / max(it_l, 1),
         "learned_wall_gain": t_id / max(t_l, 1e-12),
         "training": {"earlier_solves": nsolves, "collect_seconds": t_collect, "pairs": npairs,

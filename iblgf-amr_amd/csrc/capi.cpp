@@ -659,18 +659,13 @@ int ib_solve(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, doub
             else if (use_mlp)
             {
                 LearnedPrecond* pc = ctx->pc.get();
+                FcgMonitor      mon;
+                mon.enabled = true;
                 ctx->last = ctx->ib->fcg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit,
-                    [pc](hipStream_t s, const double* r, double* z) { pc->apply(s, r, z); });
+                    [pc](hipStream_t s, const double* r, double* z) { pc->apply(s, r, z); }, mon);
             }
             else
-            {
-                const size_t n = ctx->ib_in.n;
-                ctx->last = ctx->ib->fcg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit,
-                    [n](hipStream_t s, const double* r, double* z) {
-                        const hipError_t e = hipMemcpyAsync(z, r, n * sizeof(double), hipMemcpyDeviceToDevice, s);
-                        if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e));
-                    });
-            }
+                ctx->last = ctx->ib->fcg(d_uc.p, ctx->ib_out.p, alpha, tol, maxit, Preconditioner());
         }
         catch (...)
         {
@@ -710,6 +705,14 @@ int iblgf_gpu_ib_stats(iblgf_gpu_ctx* ctx, double* seconds, int* applies, int* f
     if (applies) *applies = ctx->last.applies;
     if (fell_back) *fell_back = ctx->last.fell_back ? 1 : 0;
     if (stop) *stop = ctx->last.stop == "converged" ? 0 : (ctx->last.stop == "iteration limit" ? 1 : 2);
+    return IBLGF_OK;
+}
+
+int iblgf_gpu_ib_switch_iteration(iblgf_gpu_ctx* ctx, int* iteration)
+{
+    NEED_IB("iblgf_gpu_ib_switch_iteration");
+    if (!iteration) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_switch_iteration: null output", ctx);
+    *iteration = ctx->last.switched_at;
     return IBLGF_OK;
 }
 

@@ -153,7 +153,8 @@ CgStats IbSystem::cg(const double* d_uc, double* d_f, double alpha, double tol, 
 // kept; when the store is full it restarts from the current residual.  A
 // preconditioned direction that is not finite or not a descent direction
 // (z.r <= 0) is replaced by r.
-CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M)
+CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M,
+    const FcgMonitor& mon)
 {
     constexpr int kKeep = 30;
     const auto    t0 = std::chrono::steady_clock::now();
@@ -173,20 +174,27 @@ CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol,
     IBLGF_HIP_IB(hipMemsetAsync(r, 0, len * sizeof(double), s_.stream()));
     add(r, d_uc, 0.0, 1.0);
     add(r, Ax, 1.0, -1.0);
+    const bool monitored = mon.enabled && M && identity_rate_ > 0.0 && mon.window > 0;
+    bool       use_m = static_cast<bool>(M);
     for (int k = 0; k < maxit; ++k)
     {
         if ((int)qq.size() == kKeep) qq.clear();  // restart: the store is full
         const int j = static_cast<int>(qq.size());
         double*   p = krylov_p_[j].p;
         double*   q = krylov_q_[j].p;
-        IBLGF_HIP_IB(hipMemsetAsync(p, 0, len * sizeof(double), s_.stream()));
-        M(s_.stream(), r, p);
-        const double zr = dot(p, r);
-        if (!std::isfinite(zr) || zr <= 0.0)
+        bool      identity = !use_m;
+        if (use_m)
         {
-            IBLGF_HIP_IB(hipMemcpyAsync(p, r, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
-            st.fell_back = true;
+            IBLGF_HIP_IB(hipMemsetAsync(p, 0, len * sizeof(double), s_.stream()));
+            M(s_.stream(), r, p);
+            const double zr = dot(p, r);
+            if (!std::isfinite(zr) || zr <= 0.0)
+            {
+                identity = true;
+                st.fell_back = true;
+            }
         }
+        if (identity) IBLGF_HIP_IB(hipMemcpyAsync(p, r, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
         // non-owned entries of p are zeroed: they never enter the update
         launch::mask_zero(s_.stream(), len, d_owned_.p, p);
         apply(p, q, alpha);
@@ -217,8 +225,26 @@ CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol,
             st.stop = "converged";
             break;
         }
+        const int it = k + 1;
+        if (monitored && use_m && it % mon.window == 0 && it >= mon.window)
+        {
+            const double h0 = st.history[it - mon.window], h1 = st.history[it - 1];
+            const double rate = (std::log10(h0) - std::log10(h1)) / mon.window;
+            if (!(rate >= mon.slack * identity_rate_))
+            {
+                use_m = false;
+                st.fell_back = true;
+                st.switched_at = it;
+            }
+        }
     }
     if (st.stop.empty()) st.stop = "iteration limit";
+    if (!M && st.history.size() >= 2)
+    {
+        const double r0 = st.history.front(), r1 = st.history.back();
+        if (r0 > 0.0 && r1 > 0.0)
+            identity_rate_ = (std::log10(r0) - std::log10(r1)) / static_cast<double>(st.history.size() - 1);
+    }
     IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
     st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return st;

@@ -26,7 +26,20 @@ struct CgStats
     double              seconds = 0.0;
     int                 applies = 0;     // operator applications
     bool                fell_back = false;
+    int                 switched_at = -1;  // iteration after which the identity replaced M
     std::string         stop;            // converged / iteration limit / breakdown
+};
+
+// Monitoring of a learned preconditioner inside the flexible solver: every
+// `window` iterations the mean log10 residual reduction per iteration over the
+// window is compared with `slack` times the rate of the last identity-
+// preconditioned solve; below it, the identity replaces M for the rest of the
+// solve (the reference's plain method is the fallback).
+struct FcgMonitor
+{
+    bool   enabled = false;
+    int    window = 10;
+    double slack = 0.8;
 };
 
 // Flexible preconditioner: z = M(r) on device vectors of length 3N.
@@ -51,9 +64,15 @@ class IbSystem
     // CG_solve: f holds the initial guess and receives the solution.
     CgStats cg(const double* d_uc, double* d_f, double alpha, double tol, int maxit);
     // Flexible CG in conjugate-residual form (flexible GCR, 30 kept
-    // directions, restarted) with the preconditioner M; a non-finite or
-    // non-descent preconditioned direction is replaced by r.
-    CgStats fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M);
+    // directions, restarted) with the preconditioner M (null: the identity);
+    // a non-finite or non-descent preconditioned direction is replaced by r.
+    // An identity solve records its convergence rate; a monitored solve
+    // falls back to the identity when M converges slower than that rate.
+    CgStats fcg(const double* d_uc, double* d_f, double alpha, double tol, int maxit, const Preconditioner& M,
+        const FcgMonitor& mon = {});
+    // Mean log10 residual reduction per iteration of the last identity solve
+    // (0 before any).
+    double identity_rate() const { return identity_rate_; }
 
     // FP64 dot over owned entries (LinSysSolver::dot).
     double dot(const double* a, const double* b);
@@ -74,6 +93,7 @@ class IbSystem
     DevArray<double> d_tmp_[6];  // r, p, Ap, Ax, z, spare
     DevArray<double> d_part_, d_out_;
     std::vector<DevArray<double>> krylov_p_, krylov_q_;
+    double                        identity_rate_ = 0.0;
 };
 
 }  // namespace iblgf_amd

@@ -118,6 +118,7 @@ SYMBOLS = {
     "iblgf_gpu_ib_precond_features": (_I, [_P, _DP, _P]),
     "iblgf_gpu_ib_precond_grad": (_I, [_P, _P, _P, _I, _P, _DP]),
     "iblgf_gpu_ib_fcg": (_I, [_P, _DP, _DP, _D, _D, _I, _I, _IP, _DP, _DP]),
+    "iblgf_gpu_ib_switch_iteration": (_I, [_P, _IP]),
 }
 
 KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
@@ -463,6 +464,12 @@ class IbSystem:
         _check(lib().iblgf_gpu_ib_stats(self.solver._h, ctypes.byref(sec), ctypes.byref(ap), ctypes.byref(fb),
                                         ctypes.byref(st)), self.solver._h)
         return sec.value, ap.value, bool(fb.value), self.STOP[st.value]
+
+    def switched_at(self):
+        """Iteration after which the last learned solve continued with the identity (-1: never)."""
+        it = ctypes.c_int()
+        _check(lib().iblgf_gpu_ib_switch_iteration(self.solver._h, ctypes.byref(it)), self.solver._h)
+        return it.value
 
 
 class LearnedPreconditioner:

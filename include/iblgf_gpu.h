@@ -291,6 +291,11 @@ int iblgf_gpu_ib_cg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alph
  * stop reason (0 converged, 1 iteration limit, 2 breakdown) of the last CG /
  * flexible-CG solve. */
 int iblgf_gpu_ib_stats(iblgf_gpu_ctx* ctx, double* seconds, int* applies, int* fell_back, int* stop);
+/* Iteration after which the last learned flexible solve continued with the
+ * identity because the network converged slower than the last identity
+ * solve (-1: it did not switch).  No counterpart in the reference, whose only
+ * method is plain CG (linsys.hpp:170-230). */
+int iblgf_gpu_ib_switch_iteration(iblgf_gpu_ctx* ctx, int* iteration);
 
 /* ---- learned preconditioner (no reference counterpart: the reference
  * solves with plain CG, linsys.hpp:170-230; this is its replacement for
@@ -320,7 +325,11 @@ int iblgf_gpu_ib_precond_grad(iblgf_gpu_ctx* ctx, const float* X, const float* T
  * the FMM-approximated operator is only symmetric to ~1e-4, where plain CG
  * stalls), preconditioned by the learned network when use_mlp is nonzero
  * (the identity otherwise, and wherever the network's direction is not a
- * finite descent direction); same stopping rule and outputs as iblgf_gpu_ib_cg. */
+ * finite descent direction).  A learned solve is monitored: every 10
+ * iterations its mean residual reduction is compared with 0.8 times that of
+ * the last identity solve, and below it the identity takes over for the rest
+ * of the solve (iblgf_gpu_ib_switch_iteration).  Same stopping rule and
+ * outputs as iblgf_gpu_ib_cg. */
 int iblgf_gpu_ib_fcg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
     int use_mlp, int* iterations, double* residual, double* history);
 

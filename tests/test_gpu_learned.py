@@ -58,29 +58,34 @@ def test_feature_invariants(small):
 def test_learned_preconditioner_trains_and_solves(small):
     """Training on the device lowers the loss on pairs recorded from solves,
     and the flexible conjugate-residual solver with the network returns a
-    finite force whose true residual agrees with its recurrence.  The gain
-    over the same solver with the identity is measured and printed; on this
-    operator (half its spectrum below 1e-8, rough surface modes) the local
-    network does not reduce the iteration count, which the design notes
-    record."""
+    finite force whose true residual agrees with its recurrence.  The learned
+    solve is monitored against the last identity solve's convergence rate and
+    hands over to the identity when the network is slower, so it converges
+    within a bounded margin of the identity's iteration count.  The gain over
+    the identity is measured and printed; on this operator (half its spectrum
+    below 1e-8, rough surface modes) the local network does not reduce the
+    iteration count, which the design notes record."""
     s, solver, sysm = small
-    alpha, tol = 0.5, 1e-6
-    f_id, it_id, res_id, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500, use_mlp=False)
-    print(f"identity iterations {it_id} residual {res_id:.3e} stop {sysm.stats()[3]}", flush=True)
+    alpha, tol = 0.5, 1e-5
     pc = LearnedPreconditioner(sysm, K=16, hidden=128, seed=5)
     pc.collect(stride=2)
     rng = np.random.default_rng(23)
     for t in range(4):
         b = rng.standard_normal((s.npoints, 3)) if t % 2 else np.tile(rng.standard_normal(3), (s.npoints, 1))
-        sysm.fcg(b, alpha, tol=1e-7, maxit=800, use_mlp=False)
+        sysm.fcg(b, alpha, tol=1e-6, maxit=800, use_mlp=False)
     pc.collect(0)
     loss0, npairs = pc.train(steps=10, batch_rows=600, lr=1e-3)
     loss1, _ = pc.train(steps=2000, batch_rows=600, lr=1e-3)
     print(f"pairs {npairs} loss {loss0:.4e} -> {loss1:.4e}", flush=True)
     assert npairs > 0 and loss1 < loss0
+    f_id, it_id, res_id, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500, use_mlp=False)
+    print(f"identity iterations {it_id} residual {res_id:.3e} stop {sysm.stats()[3]}", flush=True)
+    assert sysm.stats()[3] == "converged"
     f_l, it_l, res_l, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500)
     assert np.all(np.isfinite(f_l))
     true_res = np.linalg.norm(s.uc - sysm.apply(f_l, alpha)) / np.linalg.norm(f_l)
     print(f"learned iterations {it_l} residual {res_l:.3e} true {true_res:.3e}; identity {it_id}; "
-          f"gain {it_id / max(it_l, 1):.2f}; stop {sysm.stats()[3]} fell_back {sysm.stats()[2]}")
+          f"gain {it_id / max(it_l, 1):.2f}; stop {sysm.stats()[3]} fell_back {sysm.stats()[2]} "
+          f"identity from iteration {sysm.switched_at()}")
     assert abs(true_res - res_l) <= 1e-3 * max(true_res, 1e-12) + 1e-12
+    assert sysm.stats()[3] == "converged" and it_l <= 1.5 * it_id + 20
