@@ -620,6 +620,7 @@ template<> struct MacAcc<double2>
         a[t].y = fma(g.x, S.y, fma(g.y, S.x, a[t].y));
     }
     __device__ void flush() {}
+    __device__ double2 get(int t) const { return a[t]; }
 };
 template<> struct MacAcc<float2>
 {
@@ -649,6 +650,7 @@ template<> struct MacAcc<float2>
             p[t] = make_float2(0.f, 0.f);
         }
     }
+    __device__ double2 get(int t) const { return a[t]; }
 };
 
 // Row offset of target child t = (cx, cy, cz) relative to the entry's base row.
@@ -786,7 +788,7 @@ __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup*
         {
 #pragma unroll
             for (int t = 0; t < 8; ++t)
-                if (grp.tslot[t] >= 0) tp[(size_t)grp.tslot[t] * NF] = acc.a[t];
+                if (grp.tslot[t] >= 0) tp[(size_t)grp.tslot[t] * NF] = acc.get(t);
         }
     }
 }
