@@ -163,7 +163,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="skip the learned-FCG immersed-boundary comparison")
-    ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched"],
+    ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph"],
                     help="time one way only and print its rate (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -194,15 +194,16 @@ def main():
     def step_batched():
         solver.apply_lgf_batch(fin, fout)
 
-    def timed(precision, batched):
+    def timed(precision, batched, graphs=False):
         step = step_batched if batched else step_single
         solver.set_precision(precision)
+        solver.set_graphs(graphs)
         solver.prepare()
         for _ in range(args.warmup):
             step()
         solver.sync()
         ev0, ev1 = hip_events.Event(), hip_events.Event()
-        solver.timing(True)
+        solver.timing(not graphs)  # per-class events need eager launches
         solver.timing_reset()
         solver.sync()
         t0 = time.perf_counter()
@@ -212,8 +213,9 @@ def main():
         ev1.record(stream)
         solver.sync()
         t1 = time.perf_counter()
-        kt = solver.timing_read()
+        kt = None if graphs else solver.timing_read()
         solver.timing(False)
+        solver.set_graphs(False)
         outs = [solver.download(f) for f in fout]
         return dict(wall=t1 - t0, gpu_ms=ev0.elapsed_ms(ev1), kt=kt, outs=outs)
 
@@ -222,7 +224,7 @@ def main():
 
     if args.only:
         prec = PREC_MIXED if args.only.startswith("mixed") else PREC_FP64
-        r = timed(prec, args.only.endswith("batched"))
+        r = timed(prec, "batched" in args.only, args.only.endswith("graph"))
         print(json.dumps({"only": args.only, "solves_per_s": 3 * args.steps / r["wall"],
                           "ms_per_step": r["wall"] / args.steps * 1e3, "steps": args.steps}))
         return 0
@@ -235,6 +237,8 @@ def main():
     log(f"mixed, one component at a time: {3 * args.steps / rmx['wall']:.2f} solves/s")
     rmxb = timed(PREC_MIXED, True)
     log(f"mixed, 3 components batched: {3 * args.steps / rmxb['wall']:.2f} solves/s")
+    rmxg = timed(PREC_MIXED, True, graphs=True)
+    log(f"mixed, 3 components batched, one HIP graph per step: {3 * args.steps / rmxg['wall']:.2f} solves/s")
     counts = solver.counts()
     leaves = (wl.arrays["flags"] & FLAG_LEAF) != 0
 
@@ -325,6 +329,8 @@ def main():
             "fp64_batched_solves_per_s": v64b,
             "mixed_solves_per_s": vmx,
             "mixed_batched_solves_per_s": vmxb,
+            "mixed_batched_graph_solves_per_s": solves / rmxg["wall"],
+            "graph_replay_bitwise_equal_to_eager": all(np.array_equal(a, b) for a, b in zip(rmxg["outs"], rmxb["outs"])),
             "batched_mixed_vs_plain_fp64_speedup": vmxb / v64,
             "batched_fp64_vs_plain_fp64_speedup": v64b / v64,
             "batched_fp64_bitwise_equal_to_single": batched_fp64_identical,

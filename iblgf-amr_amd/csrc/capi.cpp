@@ -429,6 +429,12 @@ int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const i
     });
 }
 
+int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on)
+{
+    NEED_CTX("iblgf_gpu_set_graphs");
+    return guarded(ctx, [&] { ctx->solver->set_graphs(on != 0); });
+}
+
 int iblgf_gpu_set_precision(iblgf_gpu_ctx* ctx, int precision)
 {
     NEED_CTX("iblgf_gpu_set_precision");

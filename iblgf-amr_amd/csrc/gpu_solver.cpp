@@ -133,6 +133,7 @@ GpuSolver::~GpuSolver()
 {
     (void)hipSetDevice(device_);
     (void)hipStreamSynchronize(stream_);
+    drop_graphs();
     passes_.clear();
     fields_.clear();
     (void)hipStreamDestroy(stream_);
@@ -247,6 +248,7 @@ void GpuSolver::upload_tree(const Octree& t)
 
     // fields and spectra
     fields_.clear();
+    drop_graphs();
     work_ = Work{};
     reserve_components(1);
 
@@ -344,6 +346,7 @@ void GpuSolver::reserve_components(int k)
     if (k < 1 || k > kMaxComponents) throw std::invalid_argument("component count out of range");
     if (k <= work_.k) return;
     if (stream_) sync();  // solves in flight still use the old arrays
+    drop_graphs();        // captured graphs point at the old arrays
     const size_t fs = field_size() * k, ss = static_cast<size_t>(nb_) * nf_ * k;
     Work w;
     for (DevArray<double>* a : {&w.src, &w.tgt, &w.corr, &w.fmm_s, &w.fmm_t})
@@ -795,9 +798,82 @@ void GpuSolver::check_batch(const int* src, const int* dst, int k)
 // PoissonSolver::apply_lgf (include/iblgf/solver/poisson/poisson.hpp:587-894)
 void GpuSolver::apply_lgf(int src, int dst) { apply_lgf_batch(&src, &dst, 1); }
 
+// Replays the batch from a captured graph when graphs are on (and kernel
+// timing off): the first call with a key prepares the plans, kernel spectra
+// and work arrays (host work and allocations stay outside the capture), then
+// records the launches of `body` on the solver stream into a graph.
+void GpuSolver::run_captured(const std::vector<int64_t>& key, const std::function<void()>& prepare,
+    const std::function<void()>& body)
+{
+    if (!graphs_ || timer_.enabled())
+    {
+        body();
+        return;
+    }
+    auto it = graph_cache_.find(key);
+    if (it == graph_cache_.end())
+    {
+        prepare();
+        sync();
+        hipGraph_t g = nullptr;
+        IBLGF_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try
+        {
+            body();
+        }
+        catch (...)
+        {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(stream_, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
+        }
+        IBLGF_HIP(hipStreamEndCapture(stream_, &g));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        IBLGF_HIP(e);
+        it = graph_cache_.emplace(key, exec).first;
+    }
+    IBLGF_HIP(hipGraphLaunch(it->second, stream_));
+}
+
+void GpuSolver::drop_graphs()
+{
+    for (auto& kv : graph_cache_) (void)hipGraphExecDestroy(kv.second);
+    graph_cache_.clear();
+}
+
+void GpuSolver::set_graphs(bool on)
+{
+    if (!on) drop_graphs();
+    graphs_ = on;
+}
+
+std::vector<int64_t> GpuSolver::batch_key(int kind, const int* src, const int* dst, int k, double alpha) const
+{
+    int64_t a = 0;
+    static_assert(sizeof(a) == sizeof(alpha), "64-bit alpha");
+    std::memcpy(&a, &alpha, sizeof(a));
+    std::vector<int64_t> key = {kind, k, precision_, a};
+    for (int c = 0; c < k; ++c)
+    {
+        key.push_back(src[c]);
+        key.push_back(dst[c]);
+    }
+    return key;
+}
+
 void GpuSolver::apply_lgf_batch(const int* src, const int* dst, int k)
 {
     check_batch(src, dst, k);
+    run_captured(
+        batch_key(0, src, dst, k, 0.0), [this] { prepare_laplace(); },
+        [this, src, dst, k] { lgf_batch_body(src, dst, k); });
+}
+
+void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
+{
     const Batch b = batch(k);
     const size_t fs = field_size();
     double* st = work_.src.p;
@@ -853,6 +929,13 @@ void GpuSolver::apply_if(int src, int dst, double alpha_base) { apply_if_batch(&
 void GpuSolver::apply_if_batch(const int* src, const int* dst, int k, double alpha_base)
 {
     check_batch(src, dst, k);
+    run_captured(
+        batch_key(1, src, dst, k, alpha_base), [this, alpha_base] { prepare_if(alpha_base); },
+        [this, src, dst, k, alpha_base] { if_batch_body(src, dst, k, alpha_base); });
+}
+
+void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alpha_base)
+{
     const Batch b = batch(k);
     const size_t fs = field_size();
     double* st = work_.src.p;

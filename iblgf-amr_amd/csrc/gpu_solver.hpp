@@ -13,6 +13,8 @@
 #include "host/octree.hpp"
 #include "kernels.hpp"
 
+#include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -148,6 +150,14 @@ class GpuSolver
     // Size the work arrays for k components ahead of a batched solve.
     void reserve_components(int k);
     static constexpr int kMaxComponents = 64;
+    // Batched solves replayed from captured HIP graphs: the first call with a
+    // given (kernel, fields, k, precision, alpha) prepares its plans and
+    // kernel spectra, then records its launches into a graph; later calls
+    // launch the graph (one launch per solve).  The cache is dropped when the
+    // work arrays grow, the tree is re-uploaded or graphs are switched off.
+    // Calls run eagerly while kernel timing is on.
+    void set_graphs(bool on);
+    bool graphs() const { return graphs_; }
     // Build the kernel spectra that a solve will use (cold-path work).
     void prepare_laplace();
     void prepare_if(double alpha_base);
@@ -196,6 +206,12 @@ class GpuSolver
     PassPlan& pass_idx(int L, int idx, bool neighbor_only);
     const DevArray<int>& face_clean_list(int kind, int level, bool leaf_only);
     void      run_pass(PassPlan& p, double scale, bool laplace, double alpha, const Batch& b);
+    void      lgf_batch_body(const int* src, const int* dst, int k);
+    void      if_batch_body(const int* src, const int* dst, int k, double alpha_base);
+    void      run_captured(const std::vector<int64_t>& key, const std::function<void()>& prepare,
+             const std::function<void()>& body);
+    std::vector<int64_t> batch_key(int kind, const int* src, const int* dst, int k, double alpha) const;
+    void      drop_graphs();
     void      check_batch(const int* src, const int* dst, int k);
     Batch     batch(int k) const;
     double2*    ghat_laplace(int d);
@@ -241,6 +257,8 @@ class GpuSolver
         int               k = 0;
     };
     Work work_;
+    bool                                          graphs_ = false;
+    std::map<std::vector<int64_t>, hipGraphExec_t> graph_cache_;
 
     // kernel spectra
     DevArray<double>                         d_lgf_table_;

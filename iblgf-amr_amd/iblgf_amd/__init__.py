@@ -86,6 +86,7 @@ SYMBOLS = {
     "iblgf_gpu_field_download": (_I, [_P, _I, _DP]),
     "iblgf_gpu_apply_lgf": (_I, [_P, _I, _I, _I, _D]),
     "iblgf_gpu_apply_lgf_batch": (_I, [_P, _IP, _IP, _I, _I, _D]),
+    "iblgf_gpu_set_graphs": (_I, [_P, _I]),
     "iblgf_gpu_prepare": (_I, [_P, _I, _D]),
     "iblgf_gpu_set_precision": (_I, [_P, _I]),
     "iblgf_gpu_sync": (_I, [_P]),
@@ -344,6 +345,10 @@ class PoissonSolver:
         a = (ctypes.c_int * k)(*[int(x) for x in srcs])
         b = (ctypes.c_int * k)(*[int(x) for x in dsts])
         _check(lib().iblgf_gpu_apply_lgf_batch(self._h, a, b, k, int(kernel), float(alpha_base)), self._h)
+
+    def set_graphs(self, on):
+        """Replay repeated solves from captured HIP graphs (iblgf_gpu_set_graphs)."""
+        _check(lib().iblgf_gpu_set_graphs(self._h, 1 if on else 0), self._h)
 
     def sync(self):
         _check(lib().iblgf_gpu_sync(self._h), self._h)

@@ -180,6 +180,17 @@ int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int ke
 int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
     double alpha_base);
 
+/* Replay batched solves (iblgf_gpu_apply_lgf, iblgf_gpu_apply_lgf_batch) from
+ * captured HIP graphs: the first call with a given kernel, field set, k,
+ * precision and alpha prepares its plans and kernel spectra and records its
+ * launches into a graph; later identical calls launch that graph (one launch
+ * per solve).  Field contents may change between calls; field ids may not be
+ * destroyed.  The cache is dropped when the work arrays grow or on re-upload.
+ * Calls run eagerly while kernel timing is on.  No counterpart in the
+ * reference (it has no device path); it serves time-step loops that repeat the
+ * same solves.  Off by default. */
+int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on);
+
 enum iblgf_precision
 {
     IBLGF_PREC_FP64 = 0,  /* plain path: spectra, DFT legs and MAC in FP64 */
