@@ -722,6 +722,13 @@ int iblgf_gpu_ib_switch_iteration(iblgf_gpu_ctx* ctx, int* iteration)
     return IBLGF_OK;
 }
 
+int iblgf_gpu_ib_recycle(iblgf_gpu_ctx* ctx, int capacity)
+{
+    NEED_IB("iblgf_gpu_ib_recycle");
+    if (capacity < 0 || capacity > 4096) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_recycle: capacity out of range", ctx);
+    return guarded(ctx, [&] { ctx->ib->set_recycling(capacity); });
+}
+
 int iblgf_gpu_ib_precond_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t, int K, int hidden, uint64_t seed)
 {
     NEED_IB("iblgf_gpu_ib_precond_create");

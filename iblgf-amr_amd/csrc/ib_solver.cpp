@@ -2,6 +2,7 @@
 
 #include "ops.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -96,6 +97,48 @@ void IbSystem::add(double* a, const double* b, double s1, double s2)
     launch::axpby(s_.stream(), 3L * npoints_, s2, b, s1, a, d_owned_.p);
 }
 
+void IbSystem::set_recycling(int capacity)
+{
+    if (capacity < 0) throw std::invalid_argument("negative recycling capacity");
+    rec_n_ = rec_next_ = 0;
+    rec_cap_ = capacity;
+    rec_dirty_ = false;
+    if (capacity == 0)
+    {
+        rec_P_ = DevArray<double>();
+        rec_Q_ = DevArray<double>();
+        rec_qq_ = DevArray<double>();
+        rec_coef_ = DevArray<double>();
+        rec_qq_host_.clear();
+        return;
+    }
+    const size_t len = 3 * static_cast<size_t>(npoints_);
+    rec_P_.alloc(len * capacity);
+    rec_Q_.alloc(len * capacity);
+    rec_qq_.alloc(capacity);
+    rec_coef_.alloc(capacity);
+    rec_qq_host_.assign(capacity, 1.0);
+}
+
+void IbSystem::recycle_push(const double* p, const double* q, double qq)
+{
+    const size_t len = 3 * static_cast<size_t>(npoints_);
+    const int    slot = rec_next_;
+    IBLGF_HIP_IB(hipMemcpyAsync(rec_P_.p + slot * len, p, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
+    IBLGF_HIP_IB(hipMemcpyAsync(rec_Q_.p + slot * len, q, len * sizeof(double), hipMemcpyDeviceToDevice, s_.stream()));
+    rec_qq_host_[slot] = qq;
+    rec_dirty_ = true;
+    rec_next_ = (rec_next_ + 1) % rec_cap_;
+    rec_n_ = std::min(rec_n_ + 1, rec_cap_);
+}
+
+void IbSystem::recycle_sync_qq()
+{
+    if (!rec_dirty_) return;
+    IBLGF_HIP_IB(hipMemcpy(rec_qq_.p, rec_qq_host_.data(), rec_cap_ * sizeof(double), hipMemcpyHostToDevice));
+    rec_dirty_ = false;
+}
+
 // CG_solve (linsys.hpp:170-230).
 CgStats IbSystem::cg(const double* d_uc, double* d_f, double alpha, double tol, int maxit)
 {
@@ -169,16 +212,36 @@ CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol,
         for (auto& v : krylov_q_) v.alloc(len);
     }
     std::vector<double> qq;  // (q_i.q_i) of the kept directions
+    const bool          recycle = rec_cap_ > 0;
+    if (recycle && rec_n_ > 0 && rec_alpha_ != alpha) rec_n_ = rec_next_ = 0;  // another operator
+    rec_alpha_ = alpha;
     apply(d_f, Ax, alpha);
     ++st.applies;
     IBLGF_HIP_IB(hipMemsetAsync(r, 0, len * sizeof(double), s_.stream()));
     add(r, d_uc, 0.0, 1.0);
     add(r, Ax, 1.0, -1.0);
+    if (recycle && rec_n_ > 0)
+    {
+        // f += sum b_i p_i, r -= sum b_i q_i with b_i = (r.q_i) / (q_i.q_i)
+        recycle_sync_qq();
+        launch::multi_dot(s_.stream(), rec_Q_.p, rec_n_, len, r, d_owned_.p, rec_coef_.p);
+        launch::multi_update(s_.stream(), d_f, 1.0, rec_P_.p, r, -1.0, rec_Q_.p, rec_coef_.p, rec_qq_.p, rec_n_, len,
+            d_owned_.p);
+    }
+    // moves the current cycle's directions into the recycled store
+    auto keep_cycle = [&]() {
+        if (!recycle) return;
+        for (size_t i = 0; i < qq.size(); ++i) recycle_push(krylov_p_[i].p, krylov_q_[i].p, qq[i]);
+    };
     const bool monitored = mon.enabled && M && identity_rate_ > 0.0 && mon.window > 0;
     bool       use_m = static_cast<bool>(M);
     for (int k = 0; k < maxit; ++k)
     {
-        if ((int)qq.size() == kKeep) qq.clear();  // restart: the store is full
+        if ((int)qq.size() == kKeep)  // restart: the cycle's store is full
+        {
+            keep_cycle();
+            qq.clear();
+        }
         const int j = static_cast<int>(qq.size());
         double*   p = krylov_p_[j].p;
         double*   q = krylov_q_[j].p;
@@ -199,6 +262,14 @@ CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol,
         launch::mask_zero(s_.stream(), len, d_owned_.p, p);
         apply(p, q, alpha);
         ++st.applies;
+        if (recycle && rec_n_ > 0)
+        {
+            // orthogonalise against the recycled pairs: p -= sum b_i p_i, q -= sum b_i q_i
+            recycle_sync_qq();
+            launch::multi_dot(s_.stream(), rec_Q_.p, rec_n_, len, q, d_owned_.p, rec_coef_.p);
+            launch::multi_update(s_.stream(), p, -1.0, rec_P_.p, q, -1.0, rec_Q_.p, rec_coef_.p, rec_qq_.p, rec_n_,
+                len, d_owned_.p);
+        }
         for (int i = 0; i < j; ++i)
         {
             const double beta = dot(q, krylov_q_[i].p) / qq[i];
@@ -239,6 +310,7 @@ CgStats IbSystem::fcg(const double* d_uc, double* d_f, double alpha, double tol,
         }
     }
     if (st.stop.empty()) st.stop = "iteration limit";
+    keep_cycle();
     if (!M && st.history.size() >= 2)
     {
         const double r0 = st.history.front(), r1 = st.history.back();

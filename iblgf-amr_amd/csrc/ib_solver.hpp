@@ -74,6 +74,16 @@ class IbSystem
     // (0 before any).
     double identity_rate() const { return identity_rate_; }
 
+    // Krylov recycling in fcg: the (p, A p) pairs of finished restart cycles
+    // (mutually orthogonal in the owned inner product) are kept, up to
+    // `capacity` pairs (oldest replaced first), while alpha stays the same.
+    // Each solve first projects its residual onto the kept span, and every
+    // new direction is orthogonalised against it, so later solves with the
+    // same operator skip the slow modes earlier solves resolved.  0 turns it
+    // off and drops the kept pairs.
+    void set_recycling(int capacity);
+    int  recycled() const { return rec_n_; }
+
     // FP64 dot over owned entries (LinSysSolver::dot).
     double dot(const double* a, const double* b);
     // a = a*s1 + b*s2 over owned entries (LinSysSolver::add).
@@ -94,6 +104,14 @@ class IbSystem
     DevArray<double> d_part_, d_out_;
     std::vector<DevArray<double>> krylov_p_, krylov_q_;
     double                        identity_rate_ = 0.0;
+
+    void recycle_push(const double* p, const double* q, double qq);
+    void recycle_sync_qq();
+    DevArray<double>    rec_P_, rec_Q_, rec_qq_, rec_coef_;
+    std::vector<double> rec_qq_host_;
+    bool                rec_dirty_ = false;
+    int                 rec_cap_ = 0, rec_n_ = 0, rec_next_ = 0;
+    double              rec_alpha_ = 0.0;
 };
 
 }  // namespace iblgf_amd

@@ -186,6 +186,41 @@ __global__ __launch_bounds__(kThreads) void k_axpby(long len, double a, const do
     if (i < len && (!mask || mask[i] != 0.0)) y[i] = y[i] * b + x[i] * a;
 }
 
+// out[i] = sum_j mask_j V[i][j] w[j] for the nv rows of V (row stride len):
+// one workgroup per row, the same fixed reduction order on every call.
+__global__ __launch_bounds__(kThreads) void k_multi_dot(const double* __restrict__ V, int nv, long len,
+    const double* __restrict__ w, const double* __restrict__ mask, double* __restrict__ out)
+{
+    __shared__ double red[kThreads];
+    const int         i = blockIdx.x;
+    if (i >= nv) return;
+    const double* v = V + (size_t)i * len;
+    double        acc = 0.0;
+    for (long j = threadIdx.x; j < len; j += kThreads)
+        if (!mask || mask[j] != 0.0) acc = fma(v[j], w[j], acc);
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) out[i] = s;
+}
+
+// x += sx * sum_i b_i X[i], y += sy * sum_i b_i Y[i] with b_i = c[i] / d[i],
+// on the entries where mask is nonzero, the rows summed in order.
+__global__ __launch_bounds__(kThreads) void k_multi_update(double* __restrict__ x, double sx,
+    const double* __restrict__ X, double* __restrict__ y, double sy, const double* __restrict__ Y,
+    const double* __restrict__ c, const double* __restrict__ d, int nv, long len, const double* __restrict__ mask)
+{
+    const long j = (long)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= len || (mask && mask[j] == 0.0)) return;
+    double ax = 0.0, ay = 0.0;
+    for (int i = 0; i < nv; ++i)
+    {
+        const double b = c[i] / d[i];
+        ax = fma(b, X[(size_t)i * len + j], ax);
+        ay = fma(b, Y[(size_t)i * len + j], ay);
+    }
+    x[j] = fma(sx, ax, x[j]);
+    y[j] = fma(sy, ay, y[j]);
+}
+
 __global__ __launch_bounds__(kThreads) void k_mask_zero(long len, const double* __restrict__ mask,
     double* __restrict__ y)
 {
@@ -283,6 +318,19 @@ void axpby(hipStream_t s, long len, double a, const double* x, double b, double*
     if (len <= 0) return;
     hipLaunchKernelGGL(k_axpby, dim3((unsigned)((len + kThreads - 1) / kThreads)), dim3(kThreads), 0, s, len, a, x,
         b, y, mask);
+}
+
+void multi_dot(hipStream_t s, const double* V, int nv, long len, const double* w, const double* mask, double* out)
+{
+    if (nv > 0) hipLaunchKernelGGL(k_multi_dot, dim3(nv), dim3(kThreads), 0, s, V, nv, len, w, mask, out);
+}
+
+void multi_update(hipStream_t s, double* x, double sx, const double* X, double* y, double sy, const double* Y,
+    const double* c, const double* d, int nv, long len, const double* mask)
+{
+    if (nv > 0 && len > 0)
+        hipLaunchKernelGGL(k_multi_update, dim3((unsigned)((len + kThreads - 1) / kThreads)), dim3(kThreads), 0, s,
+            x, sx, X, y, sy, Y, c, d, nv, len, mask);
 }
 
 void mask_zero(hipStream_t s, long len, const double* mask, double* y)

@@ -53,6 +53,12 @@ void dot(hipStream_t s, const double* a, const double* b, const double* mask, lo
 void axpby(hipStream_t s, long len, double a, const double* x, double b, double* y, const double* mask);
 // y = 0 where mask is zero.
 void mask_zero(hipStream_t s, long len, const double* mask, double* y);
+// out[i] = sum_j mask_j V[i][j] w[j] for nv rows of length len (deterministic).
+void multi_dot(hipStream_t s, const double* V, int nv, long len, const double* w, const double* mask, double* out);
+// x += sx * sum_i (c_i / d_i) X[i] and y += sy * sum_i (c_i / d_i) Y[i] where mask != 0;
+// c and d are device arrays of nv values.
+void multi_update(hipStream_t s, double* x, double sx, const double* X, double* y, double sy, const double* Y,
+    const double* c, const double* d, int nv, long len, const double* mask);
 
 }  // namespace launch
 }  // namespace iblgf_amd

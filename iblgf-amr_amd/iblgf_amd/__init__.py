@@ -120,6 +120,7 @@ SYMBOLS = {
     "iblgf_gpu_ib_precond_grad": (_I, [_P, _P, _P, _I, _P, _DP]),
     "iblgf_gpu_ib_fcg": (_I, [_P, _DP, _DP, _D, _D, _I, _I, _IP, _DP, _DP]),
     "iblgf_gpu_ib_switch_iteration": (_I, [_P, _IP]),
+    "iblgf_gpu_ib_recycle": (_I, [_P, _I]),
 }
 
 KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
@@ -469,6 +470,10 @@ class IbSystem:
         _check(lib().iblgf_gpu_ib_stats(self.solver._h, ctypes.byref(sec), ctypes.byref(ap), ctypes.byref(fb),
                                         ctypes.byref(st)), self.solver._h)
         return sec.value, ap.value, bool(fb.value), self.STOP[st.value]
+
+    def recycle(self, capacity):
+        """Keep up to `capacity` (p, A p) pairs across flexible solves with the same alpha (0: off)."""
+        _check(lib().iblgf_gpu_ib_recycle(self.solver._h, int(capacity)), self.solver._h)
 
     def switched_at(self):
         """Iteration after which the last learned solve continued with the identity (-1: never)."""

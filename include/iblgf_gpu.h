@@ -307,6 +307,15 @@ int iblgf_gpu_ib_stats(iblgf_gpu_ctx* ctx, double* seconds, int* applies, int* f
  * solve (-1: it did not switch).  No counterpart in the reference, whose only
  * method is plain CG (linsys.hpp:170-230). */
 int iblgf_gpu_ib_switch_iteration(iblgf_gpu_ctx* ctx, int* iteration);
+/* Krylov recycling for iblgf_gpu_ib_fcg: keep up to `capacity` (p, A p) pairs
+ * of finished restart cycles (mutually orthogonal, oldest replaced first)
+ * while alpha stays the same; every flexible solve projects its initial
+ * residual onto them and orthogonalises its new directions against them.
+ * 0 turns it off and drops the pairs.  No counterpart in the reference (plain
+ * CG from the previous force, linsys.hpp:170-230); it serves callers that
+ * repeat force solves with one operator (every IF-HERK stage of a time step
+ * with fixed geometry). */
+int iblgf_gpu_ib_recycle(iblgf_gpu_ctx* ctx, int capacity);
 
 /* ---- learned preconditioner (no reference counterpart: the reference
  * solves with plain CG, linsys.hpp:170-230; this is its replacement for
