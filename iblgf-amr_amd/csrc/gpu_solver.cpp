@@ -989,19 +989,28 @@ void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
 
 void GpuSolver::apply_if_masked(int src, int dst, double alpha_base, int mask_type)
 {
+    apply_if_masked_batch(&src, &dst, 1, alpha_base, mask_type);
+}
+
+void GpuSolver::apply_if_masked_batch(const int* src, const int* dst, int k, double alpha_base, int mask_type)
+{
     if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB)
         throw std::invalid_argument("mask type must be IB2xIB or xIB2IB");
-    check_batch(&src, &dst, 1);
-    const Batch b = batch(1);
-    const int L = depth_ - 1;
-    double*   st = work_.src.p;
-    double*   tt = work_.tgt.p;
+    check_batch(src, dst, k);
+    const Batch  b = batch(k);
+    const size_t fs = field_size();
+    const int    L = depth_ - 1;
+    double*      st = work_.src.p;
+    double*      tt = work_.tgt.p;
     fill_zero(st, b);
     fill_zero(tt, b);
-    launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[L].p, p_level_slots_[L].size(), true);
+    for (int c = 0; c < k; ++c)
+        launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_level_slots_[L].p, p_level_slots_[L].size(), true);
     const double alpha = alpha_base * std::pow(4.0, L - base_level_);
     run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), true), 1.0, false, alpha, b);
-    launch::copy_blocks(stream_, n_, F(dst), tt, p_level_slots_[L].p, p_level_slots_[L].size(), false);
+    for (int c = 0; c < k; ++c)
+        launch::copy_blocks(
+            stream_, n_, F(dst[c]), tt + c * fs, p_level_slots_[L].p, p_level_slots_[L].size(), false);
     IBLGF_HIP(hipGetLastError());
 }
 

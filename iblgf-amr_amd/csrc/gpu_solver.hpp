@@ -170,6 +170,9 @@ class GpuSolver
     // finest-level block (buffers included).
     void apply_lgf_masked(int src, int dst, int mask_type);
     void apply_if_masked(int src, int dst, double alpha_base, int mask_type);
+    // The same for k components through every launch (src[c] -> dst[c]; a
+    // component may be solved in place, src[c] == dst[c]).
+    void apply_if_masked_batch(const int* src, const int* dst, int k, double alpha_base, int mask_type);
 
     // Block operators over the blocks with data at one level (BlockOp in
     // ops.hpp; scale multiplies 1/dx for the curl transpose).

@@ -72,8 +72,7 @@ void IbSystem::apply(const double* d_fin, double* d_fout, double alpha)
         s_.domain_clean(aux_[c]);
     }
     smear(d_fin, field_);
-    if (std::fabs(alpha) > 1e-14)
-        for (int c = 0; c < 3; ++c) s_.apply_if_masked(field_[c], field_[c], alpha, MASK_IB2xIB);
+    if (std::fabs(alpha) > 1e-14) s_.apply_if_masked_batch(field_, field_, 3, alpha, MASK_IB2xIB);
     s_.level_divergence(field_, cell_, L);
     s_.clean_ib_region_boundary(cell_, L, 2);
     s_.apply_lgf_masked(cell_, cell_, MASK_xIB2IB);
