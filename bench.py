@@ -5,11 +5,12 @@ Workload: the adaptive vortex-ring octree (8^3 base blocks of 16^3 cells plus
 3 refinement levels, 16 seeded Gaussian-core rings, seed 1001).  One step is
 one vector Poisson solve: PoissonSolver::apply_lgf over the three vorticity
 components, i.e. three scalar LGF solves over the whole tree, all levels.
-Four ways are timed: the plain FP64 path one component at a time, the FP64
-and mixed paths with the three components batched through every launch, and
-the mixed path one component at a time.  The headline is the batched mixed
-path when all its components are within relative L2 1e-6 of the plain FP64
-result.  Inputs are resident in HBM before the timed region; kernel spectra
+Five ways are timed: the plain FP64 path one component at a time, the FP64
+and mixed paths with the three components batched through every launch, the
+mixed path one component at a time, and the batched mixed path replayed from
+one captured HIP graph per step.  The headline is the eager batched mixed
+path (its HIP events give the per-kernel times) when all its components are
+within relative L2 1e-6 of the plain FP64 result.  Inputs are resident in HBM before the timed region; kernel spectra
 are built during warm-up (the reference times its second, warm solve).
 
 Prints one JSON line with the throughput (solves/s), the roofline of the
