@@ -312,8 +312,9 @@ struct Cx<float>
 };
 
 // One DFT workgroup per CU takes nearly all of the 160 KB LDS: at n = 18 a
-// group of GS = 9 kx-planes with 768 threads (12 waves) measured faster than
-// two workgroups per CU with GS = 3, whose legs leave most threads idle.
+// group of GS = 9 kx-planes with 768-1024 threads (12-16 waves) measured
+// faster than two workgroups per CU with GS = 3, whose legs leave most
+// threads idle.
 template<int N>
 struct DftPlan
 {
