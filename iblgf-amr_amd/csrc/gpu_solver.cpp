@@ -49,7 +49,6 @@ template struct DevArray<double2>;
 template struct DevArray<float2>;
 template struct DevArray<float>;
 template struct DevArray<uint16_t>;
-template struct DevArray<short>;
 template struct DevArray<MacGroup>;
 template struct DevArray<MacSrc>;
 template struct DevArray<uint4>;

@@ -600,13 +600,11 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
 // for its frequencies sit in LDS.  Each thread keeps the 8 target
 // accumulators of one group at one frequency.
 // ---------------------------------------------------------------------------
-// Accumulators are FP64 on both paths.  On the mixed path the products are
-// FP32 and each is widened before the add: over the ~200 sources of a
-// target an FP32 running sum loses ~15x more than the spectra's own rounding.
-// Target accumulators of one (group, component) at one frequency.  FP64
-// spectra accumulate in FP64 directly.  FP32 spectra form every product and
-// the partial sums over at most 8 source entries in FP32; each partial sum is
-// then added into the FP64 accumulator.
+// Target accumulators of one (group, component) at one frequency, FP64 on
+// both paths.  FP64 spectra accumulate in FP64 directly.  FP32 spectra form
+// every product and the partial sums over at most 8 source entries in FP32;
+// each partial sum is then added into the FP64 accumulator (an FP32 running
+// sum over all of a target's sources measured twice the error).
 template<class C> struct MacAcc;
 template<> struct MacAcc<double2>
 {
