@@ -267,6 +267,11 @@ def main():
     rmxg = timed(PREC_MIXED, True, graphs=True)
     log(f"mixed, 3 components batched, one HIP graph per step: {3 * args.steps / rmxg['wall']:.2f} solves/s")
     counts = solver.counts()
+    mac_plan = solver.mac_counts()
+    # Every entry issues 8 target products (children outside its mask read a
+    # zero kernel row); the pairs are the useful ones.
+    mac_plan["issued_products"] = 8 * mac_plan["entries"]
+    mac_plan["useful_fraction"] = mac_plan["pairs"] / max(1, mac_plan["issued_products"])
     leaves = (wl.arrays["flags"] & FLAG_LEAF) != 0
 
     def errors_vs_fp64(r):
@@ -349,6 +354,7 @@ def main():
             "leaf_cells": leaf_cells,
             "points_per_s": leaf_cells * value,
             "pairs_per_solve": counts["n_pair"],
+            "mac_plan_per_solve": mac_plan,
             "flops_per_solve": flops_per_solve,
             "solve_tflops": flops_per_solve * value / 1e12,
             "gpu_event_ms_per_step": head["gpu_ms"] / args.steps,

@@ -830,4 +830,17 @@ int iblgf_gpu_counts(iblgf_gpu_ctx* ctx, long* c)
     return IBLGF_OK;
 }
 
+int iblgf_gpu_mac_counts(iblgf_gpu_ctx* ctx, long* c)
+{
+    NEED_CTX("iblgf_gpu_mac_counts");
+    if (!c) return fail(IBLGF_ERR_ARG, "iblgf_gpu_mac_counts: null output", ctx);
+    const SolveCounts s = ctx->solver->lgf_counts();
+    c[0] = s.n_entry;
+    c[1] = s.n_entry_full;
+    c[2] = s.n_pair_full;
+    c[3] = s.n_pair;
+    for (int i = 0; i < 8; ++i) c[4 + i] = s.masked_pop[i];
+    return IBLGF_OK;
+}
+
 }  // extern "C"

@@ -541,6 +541,7 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
         if (gb.empty()) continue;
         std::vector<MacGroup> groups;
         std::vector<MacSrc>   srcs;
+        long                  m_nfull = 0, m_pairs_full = 0, m_pop[8] = {};
         for (auto& b : gb)
         {
             b.g.begin = static_cast<int>(srcs.size());
@@ -564,6 +565,17 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
             });
             b.g.nfull = 0;
             for (const auto& en : loc) b.g.nfull += (en.meta >> 9) == gmask ? 1 : 0;
+            for (const auto& en : loc)
+            {
+                const int pc = __builtin_popcount(static_cast<unsigned>(en.meta >> 9));
+                if ((en.meta >> 9) == gmask)
+                {
+                    ++m_nfull;
+                    m_pairs_full += pc;
+                }
+                else
+                    ++m_pop[pc - 1];
+            }
             srcs.insert(srcs.end(), loc.begin(), loc.end());
             b.g.count = static_cast<int>(loc.size());
             groups.push_back(b.g);
@@ -575,6 +587,9 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
         m.srcs.upload(srcs);
         m.npairs = pairs_here;
         m.nentries = static_cast<long>(srcs.size());
+        m.nfull = m_nfull;
+        m.pairs_full = m_pairs_full;
+        std::copy(m_pop, m_pop + 8, m.masked_pop);
         {
             std::vector<char> used(343, 0);
             for (const auto& e : srcs)
@@ -1131,6 +1146,13 @@ SolveCounts GpuSolver::lgf_counts() const
         c.n_tgt += p.inv.size();
         c.n_pair += p.npairs;
         c.n_transfer += p.ntransfer;
+        for (const auto& m : p.mac)
+        {
+            c.n_entry += m.nentries;
+            c.n_entry_full += m.nfull;
+            c.n_pair_full += m.pairs_full;
+            for (int i = 0; i < 8; ++i) c.masked_pop[i] += m.masked_pop[i];
+        }
     }
     return c;
 }

@@ -62,6 +62,9 @@ struct MacLevelPlan
     bool               compact = false;  // stage only the olist rows
     long               npairs = 0;
     long               nentries = 0;
+    long               nfull = 0;          // entries acting on every target of their group
+    long               pairs_full = 0;     // pairs carried by those entries
+    long               masked_pop[8] = {};  // other entries by target count 1..8
 };
 
 struct PassPlan
@@ -80,6 +83,7 @@ struct PassPlan
 struct SolveCounts
 {
     long passes = 0, n_src = 0, n_tgt = 0, n_pair = 0, n_transfer = 0, n_leaf = 0;
+    long n_entry = 0, n_entry_full = 0, n_pair_full = 0, masked_pop[8] = {};
 };
 
 // Kernel classes timed with HIP events on the solver's stream.

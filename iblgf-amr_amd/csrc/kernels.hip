@@ -670,7 +670,7 @@ __device__ inline void mac_full(const C* Gs, int f, int meta, C S, MacAcc<C>& ac
 }
 
 // An entry with a partial target mask, branch-free: ro holds the entry's 8
-// LDS byte offsets (row * 64; targets outside the mask point at the all-zero
+// LDS byte offsets (row * 128; targets outside the mask point at the all-zero
 // row), precomputed by the plan, so each target costs one add before its LDS
 // read and the 8 reads issue together.
 template<int KF, class C>
@@ -725,15 +725,22 @@ __device__ inline void mac_range(const MacSrc* __restrict__ srcs, const uint4* _
 }
 
 // One launch covers the sibling groups of one target level, for k components
-// of the same tree.  A workgroup owns KF frequencies (KF * sizeof(C) = 64 B
-// per kernel row): it loads kernel spectra into LDS and walks every (group,
-// component) pair, the components of a group on neighbouring threads (same
-// entries, same kernel rows); a thread keeps the 8 target accumulators of one
-// group and component at one frequency.  A group's entries that act on every
-// target come first (nfull of them) and index the table directly; the rest
-// carry partial masks and read their rows through rowoff.  COMPACT launches
-// (few offsets used, the small coarse-level launches) load only the rows in
-// olist and take every entry through rowoff; the others load all 343 rows.
+// of the same tree.  The work is a sequence of units (frequency chunk,
+// group and component), chunk-major; a chunk is KF frequencies (KF *
+// sizeof(C) = 128 B per kernel row, one cache line of a source spectrum per
+// group and entry).  With W workgroups, workgroup w takes the whole chunks
+// r W + w and an equal share of the units of the chunks left over: for every
+// chunk it touches it stages that chunk's kernel spectra in LDS and walks its
+// (group, component) pairs, the components of a group on neighbouring threads
+// (same entries, same kernel rows); a thread keeps the 8 target accumulators
+// of one group and component at one frequency.  With W = chunk count a
+// workgroup owns one chunk; a large launch uses W = the workgroups resident
+// on the device at once, so no partial last round of workgroups is left.  A
+// group's entries that act on every target come first
+// (nfull of them) and index the table directly; the rest carry partial masks
+// and read their rows through rowoff.  COMPACT launches (few offsets used,
+// the small coarse-level launches) load only the rows in olist and take
+// every entry through rowoff; the others load all 343 rows.
 constexpr int kMacMaxThreads = 1024;
 
 template<int N, int KF, bool COMPACT, class T>
@@ -743,52 +750,76 @@ __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup*
     double2* __restrict__ tspec, int k, long scs, long tcs)
 {
     using C = typename Cx<T>::type;
-    static_assert(KF * sizeof(C) == kMacRowBytes, "row offsets assume 64-byte kernel rows");
+    static_assert(KF * sizeof(C) == kMacRowBytes, "row offsets assume 128-byte kernel rows");
     constexpr int NF = Geo<N>::NF;
+    constexpr int NCHUNK = (NF + KF - 1) / KF;
     const int     nt = blockDim.x;  // a multiple of KF, sized to the launch's group count
     const int     GPW = nt / KF;
-    __shared__ C  Gs[344 * KF];  // the launch's rows, then one all-zero row
+    __shared__ C  Gs[344 * KF];  // the chunk's rows, then one all-zero row
     // Workgroups are dealt to the 8 XCDs round-robin (workgroup i on XCD
-    // i mod 8).  A source spectrum's 128-byte line holds 2 (FP32) or 4 (FP64)
-    // frequency chunks, so neighbouring chunks are given to workgroups of the
-    // same XCD, which then share each line in that XCD's L2: XCD x owns a
-    // contiguous range of chunks (a bijection of the grid onto the chunks).
+    // i mod 8); XCD x takes a contiguous range of the logical workgroup
+    // indices w (a bijection of the grid onto 0..W-1), so the units of one
+    // XCD are contiguous and neighbouring groups' shared source lines meet in
+    // its L2.
     const int nwg = gridDim.x, xcd = blockIdx.x & 7, q = nwg >> 3, rem = nwg & 7;
-    const int f0 = (xcd * q + min(xcd, rem) + (int)(blockIdx.x >> 3)) * KF;
-    const int rows = COMPACT ? nused : 343;
-    for (int i = threadIdx.x; i < (rows + 1) * KF; i += nt)
-    {
-        const int c = i / KF, f = i - c * KF;
-        const int o = COMPACT ? olist[min(c, nused - 1)] : min(c, 342);
-        Gs[i] = (c < rows && f0 + f < NF) ? ghat[(size_t)o * NF + f0 + f] : Cx<T>::make(0, 0);
-    }
-    __syncthreads();
-    const int f = threadIdx.x % KF;
-    const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
-    // The workgroup walks the (Morton-ordered) groups in tiles, so a source
-    // shared by neighbouring groups is re-read while its line is still in L2.
+    const int w = xcd * q + min(xcd, rem) + (int)(blockIdx.x >> 3);
     const int nvg = ngroups * k;
-    for (int vg = threadIdx.x / KF; vg < nvg; vg += GPW)
+    // Whole chunks first, chunk r W + w in round r (the workgroups of a round
+    // walk the same groups together); the T = NCHUNK mod W chunks left over
+    // are split as units, workgroup w taking units [Ut w / W, Ut (w+1) / W)
+    // of their Ut = T nvg.
+    const int  R = NCHUNK / nwg, T0 = R * nwg;
+    const long Ut = (long)nvg * (NCHUNK - T0);
+    const long ua = Ut * w / nwg, ub = Ut * (w + 1) / nwg;
+    const int  ca = T0 + (int)(ua / nvg), cb = ub > ua ? T0 + (int)((ub - 1) / nvg) : ca - 1;
+    const int  va = (int)(ua - (long)(ca - T0) * nvg), vb = (int)(ub - (long)(cb - T0) * nvg);
+    const int  nseg = R + (cb - ca + 1);
+    const int  rows = COMPACT ? nused : 343;
+    const int  f = threadIdx.x % KF;
+    for (int sg = 0; sg < nseg; ++sg)
     {
-        const int      g = vg / k, c = vg - g * k;
-        const C*       sp = sspec + c * scs + fg;
-        double2*       tp = tspec + c * tcs + fg;
-        const MacGroup grp = groups[g];
-        MacAcc<C>      acc;
-        acc.zero();
-        const int e0 = grp.begin, e2 = grp.begin + grp.count;
-        int       e1 = e0;
-        if constexpr (!COMPACT)
+        const bool tail = sg >= R;
+        const int  chunk = tail ? ca + (sg - R) : sg * nwg + w;
+        const int  vlo = tail && chunk == ca ? va : 0, vhi = tail && chunk == cb ? vb : nvg;
+        const int  f0 = chunk * KF;
+        __syncthreads();  // the previous chunk's rows are no longer read
+        for (int i = threadIdx.x; i < (rows + 1) * KF; i += nt)
         {
-            e1 = e0 + grp.nfull;
-            mac_range<true, NF, KF, C>(srcs, rowoff, sp, Gs, f, e0, e1, acc);
+            const int c = i / KF, fi = i - c * KF;
+            const int o = COMPACT ? olist[min(c, nused - 1)] : min(c, 342);
+            Gs[i] = (c < rows && f0 + fi < NF) ? ghat[(size_t)o * NF + f0 + fi] : Cx<T>::make(0, 0);
         }
-        mac_range<false, NF, KF, C>(srcs, rowoff, sp, Gs, f, e1, e2, acc);
-        if (f0 + f < NF)
+        __syncthreads();
+        const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
+        // The workgroup walks the (Morton-ordered) groups in tiles, so a
+        // source shared by neighbouring groups is re-read while its line is
+        // still in L2.
+        for (int vg = vlo + threadIdx.x / KF; vg < vhi; vg += GPW)
         {
+            const int      g = vg / k, c = vg - g * k;
+            const C*       sp = sspec + c * scs + fg;
+            double2*       tp = tspec + c * tcs + fg;
+            const MacGroup* gp = groups + g;
+            MacAcc<C>       acc;
+            acc.zero();
+            const int e0 = gp->begin, e2 = e0 + gp->count;
+            int       e1 = e0;
+            if constexpr (!COMPACT)
+            {
+                e1 = e0 + gp->nfull;
+                mac_range<true, NF, KF, C>(srcs, rowoff, sp, Gs, f, e0, e1, acc);
+            }
+            mac_range<false, NF, KF, C>(srcs, rowoff, sp, Gs, f, e1, e2, acc);
+            if (f0 + f < NF)
+            {
+                // target slots read only now: not held in registers over the entries
+                int ts[8];
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
-                if (grp.tslot[t] >= 0) tp[(size_t)grp.tslot[t] * NF] = acc.get(t);
+                for (int t = 0; t < 8; ++t) ts[t] = gp->tslot[t];
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    if (ts[t] >= 0) tp[(size_t)ts[t] * NF] = acc.get(t);
+            }
         }
     }
 }
@@ -1042,26 +1073,47 @@ inline int mac_threads(long nvg, int kf)
     return want < 128 ? 128 : (want > kMacMaxThreads ? kMacMaxThreads : (int)want);
 }
 
+// Grid of a MAC launch.  A launch of full-size workgroups whose chunk count
+// exceeds the workgroups the device holds at once is spread over exactly that
+// many (each takes whole chunks in rounds and an equal share of the chunks
+// left over); every other launch gives each workgroup one chunk.
+template<class F>
+inline int mac_grid(F kernel, int nt, int nchunk)
+{
+    if (nt < kMacMaxThreads) return nchunk;
+    int dev = 0, ncu = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, nt, 0) != hipSuccess || ncu <= 0 || occ <= 0)
+        return nchunk;
+    const int slots = ncu * occ;
+    return nchunk > slots ? slots : nchunk;
+}
+
 template<int N> struct RunMac
 {
     static void run(hipStream_t s, int prec, const MacGroup* g, int ng, const MacSrc* src, const uint4* rowoff,
         const int* olist, int nused, bool compact, const void* ghat, const void* ss, void* ts, const Batch& b)
     {
 #define IBLGF_MAC(KF, CP, T, C)                                                                                  \
-    hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3((Geo<N>::NF + KF - 1) / KF),                          \
-        dim3(mac_threads((long)ng * b.k, KF)), 0, s, g, ng, src, rowoff, olist, nused, (const C*)ghat,          \
-        (const C*)ss, (double2*)ts, b.k, b.sstride, b.sstride)
+    do                                                                                                           \
+    {                                                                                                            \
+        const int nt = mac_threads((long)ng * b.k, KF);                                                          \
+        const int nwg = mac_grid(k_spectral_mac<N, KF, CP, T>, nt, (Geo<N>::NF + KF - 1) / KF);                 \
+        hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3(nwg), dim3(nt), 0, s, g, ng, src, rowoff, olist,  \
+            nused, (const C*)ghat, (const C*)ss, (double2*)ts, b.k, b.sstride, b.sstride);                      \
+    } while (0)
         if (prec == 0)
         {
-            if (compact) IBLGF_MAC(4, true, double, double2);
+            if (compact) IBLGF_MAC(8, true, double, double2);
             else
-                IBLGF_MAC(4, false, double, double2);
+                IBLGF_MAC(8, false, double, double2);
         }
         else
         {
-            if (compact) IBLGF_MAC(8, true, float, float2);
+            if (compact) IBLGF_MAC(16, true, float, float2);
             else
-                IBLGF_MAC(8, false, float, float2);
+                IBLGF_MAC(16, false, float, float2);
         }
 #undef IBLGF_MAC
     }

@@ -45,9 +45,9 @@ struct Batch
 };
 
 // Bytes per staged kernel row of the multiply-accumulate (KF frequencies of
-// one offset: 4 FP64 or 8 FP32 complex values), and the largest offset count
+// one offset: 8 FP64 or 16 FP32 complex values), and the largest offset count
 // of a compact launch.
-constexpr int kMacRowBytes = 64;
+constexpr int kMacRowBytes = 128;
 constexpr int kMacCompactMax = 200;
 
 namespace launch {

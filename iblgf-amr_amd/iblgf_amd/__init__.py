@@ -92,6 +92,7 @@ SYMBOLS = {
     "iblgf_gpu_sync": (_I, [_P]),
     "iblgf_gpu_stream": (_I, [_P, ctypes.POINTER(_P)]),
     "iblgf_gpu_counts": (_I, [_P, ctypes.POINTER(ctypes.c_long)]),
+    "iblgf_gpu_mac_counts": (_I, [_P, ctypes.POINTER(ctypes.c_long)]),
     "iblgf_gpu_timing_enable": (_I, [_P, _I]),
     "iblgf_gpu_timing_read": (_I, [_P, _DP, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
     "iblgf_gpu_timing_reset": (_I, [_P]),
@@ -363,6 +364,14 @@ class PoissonSolver:
         c = (ctypes.c_long * 6)()
         _check(lib().iblgf_gpu_counts(self._h, c), self._h)
         return dict(passes=c[0], n_src=c[1], n_tgt=c[2], n_pair=c[3], n_transfer=c[4], n_leaf=c[5])
+
+    def mac_counts(self):
+        """Multiply-accumulate plan statistics of the Laplace passes built so
+        far: entries, full-mask entries, pairs in those, all pairs, and the
+        partial-mask entries by target count (1..8)."""
+        c = (ctypes.c_long * 12)()
+        _check(lib().iblgf_gpu_mac_counts(self._h, c), self._h)
+        return dict(entries=c[0], full_entries=c[1], full_pairs=c[2], pairs=c[3], masked_by_targets=list(c[4:12]))
 
     def timing(self, enable):
         _check(lib().iblgf_gpu_timing_enable(self._h, 1 if enable else 0), self._h)

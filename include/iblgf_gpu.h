@@ -227,6 +227,13 @@ int iblgf_gpu_timing_reset(iblgf_gpu_ctx* ctx);
  * transfers, leaves. */
 int iblgf_gpu_counts(iblgf_gpu_ctx* ctx, long* counts6);
 
+/* Multiply-accumulate plan statistics of the same passes (no counterpart in
+ * the reference; for roofline accounting of issued against useful products):
+ * entries (one source spectrum acting on the targets of one sibling group),
+ * entries acting on every target of their group, pairs carried by those,
+ * all pairs, then the other entries by target count 1..8. */
+int iblgf_gpu_mac_counts(iblgf_gpu_ctx* ctx, long* counts12);
+
 /* ---- block operators (device) ------------------------------------------- */
 
 enum iblgf_block_op
