@@ -252,14 +252,17 @@ void GpuSolver::upload_tree(const Octree& t)
     reserve_components(1);
 
     // solver-level block lists
-    std::vector<int> leaves, all(nb_);
+    std::vector<int> leaves, all(nb_), interior;
     for (int s = 0; s < nb_; ++s)
     {
         all[s] = s;
         if (flags_[s] & 1u) leaves.push_back(s);
     }
+    for (int s = lvl_begin_[base_level_]; s < nb_; ++s)
+        if (!(flags_[s] & 1u)) interior.push_back(s);
     p_leaves_.upload(leaves);
     p_all_.upload(all);
+    p_nonleaf_.upload(interior);
     p_coarsify_.clear();
     p_level_slots_.clear();
     p_leaf_level_.clear();
@@ -348,7 +351,7 @@ void GpuSolver::reserve_components(int k)
     drop_graphs();        // captured graphs point at the old arrays
     const size_t fs = field_size() * k, ss = static_cast<size_t>(nb_) * nf_ * k;
     Work w;
-    for (DevArray<double>* a : {&w.src, &w.tgt, &w.corr, &w.fmm_s, &w.fmm_t})
+    for (DevArray<double>* a : {&w.src, &w.tgt, &w.fmm_s, &w.fmm_t})
     {
         a->alloc(fs);
         IBLGF_HIP(hipMemset(a->p, 0, fs * sizeof(double)));
@@ -382,20 +385,18 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
     auto      S = [&](int s) { return (msrc_[s] >> idx) & 1ULL; };
     auto      T = [&](int s) { return (mtgt_[s] >> idx) & 1ULL; };
 
-    std::vector<int> zs, zt, cs, add;
-    for (int l = L; l >= 0; --l)
+    // Level-L sources are copied in (interior, zero ring) and coarser ones
+    // accumulate anterpolated values, so only the coarser ones are cleared.
+    std::vector<int> zs, cs, add;
+    for (int l = L - 1; l >= 0; --l)
         for (int s = lvl_begin_[l]; s < lvl_begin_[l + 1]; ++s)
-        {
             if (S(s)) zs.push_back(s);
-            if (T(s)) zt.push_back(s);
-        }
     for (int s = lvl_begin_[L]; s < lvl_begin_[L + 1]; ++s)
     {
         if (S(s)) cs.push_back(s);
         if (T(s)) add.push_back(s);
     }
     p.zero_s.upload(zs);
-    p.zero_t.upload(zt);
     p.copy_s.upload(cs);
     p.add.upload(add);
 
@@ -635,6 +636,16 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
         if (is_src[s]) fwd.push_back(s);
     p.fwd.upload(fwd);
     p.inv.upload(inv);
+    // The inverse transform writes its targets whole; the targets it does not
+    // write (no sources) are cleared, since interpolation and the final add
+    // read every target.
+    std::vector<char> written(nb_, 0);
+    for (int t : inv) written[t] = 1;
+    std::vector<int> zt;
+    for (int l = L; l >= 0; --l)
+        for (int s = lvl_begin_[l]; s < lvl_begin_[l + 1]; ++s)
+            if (T(s) && !written[s]) zt.push_back(s);
+    p.zero_t.upload(zt);
 }
 
 PassPlan& GpuSolver::pass(int L, int nl, bool nbonly)
@@ -892,11 +903,13 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
     const size_t fs = field_size();
     double* st = work_.src.p;
     double* tt = work_.tgt.p;
-    double* ct = work_.corr.p;
     timer_.begin(stream_);
-    fill_zero(st, b);
-    fill_zero(tt, b);
-    fill_zero(ct, b);
+    // Source: leaves copied whole, the blocks above them cleared for the
+    // coarsening to accumulate into.  Target: the base level is cleared for
+    // the passes to add into; every finer block is first written whole by the
+    // interpolation from its parent.
+    launch::zero_blocks(stream_, n_, st, p_nonleaf_.p, p_nonleaf_.size(), b);
+    launch::zero_blocks(stream_, n_, tt, p_level_slots_[base_level_].p, p_level_slots_[base_level_].size(), b);
     for (int c = 0; c < k; ++c)
         launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_leaves_.p, p_leaves_.size(), false);
     timer_.end(stream_, KC_OTHER, 0);
@@ -923,14 +936,16 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
                 stream_, n_, F(dst[c]), tt + c * fs, p_leaf_level_[l].p, p_leaf_level_[l].size(), false);
         if (l < depth_ - 1)
         {
+            // Source correction of the children, added into the source in
+            // place: the interpolated parent source (correction children),
+            // then the stencil of the interpolated target (interior points).
             launch::buffer_exchange(
                 stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p, b);
             launch::transfer_scatter(
-                stream_, n_, st, ct, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false, b);
+                stream_, n_, st, st, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, true, b);
             const double dxc = dx / 2.0;
             launch::correction_stencil(
-                stream_, n_, tt, ct, p_kids_[l].p, p_kids_[l].size(), 1.0 / (dxc * dxc), 0.0, b);
-            launch::axpy_blocks(stream_, n_, st, ct, p_level_slots_[l + 1].p, p_level_slots_[l + 1].size(), 1.0, b);
+                stream_, n_, tt, st, p_kids_[l].p, p_kids_[l].size(), 1.0 / (dxc * dxc), 0.0, b);
         }
         timer_.end(stream_, KC_OTHER, 0);
     }
@@ -954,7 +969,7 @@ void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alph
     const size_t fs = field_size();
     double* st = work_.src.p;
     double* tt = work_.tgt.p;
-    fill_zero(st, b);
+    launch::zero_blocks(stream_, n_, st, p_nonleaf_.p, p_nonleaf_.size(), b);
     fill_zero(tt, b);
     for (int c = 0; c < k; ++c)
         launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_leaves_.p, p_leaves_.size(), false);
@@ -991,9 +1006,9 @@ void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
     const int L = depth_ - 1;
     double*   st = work_.src.p;
     double*   tt = work_.tgt.p;
-    fill_zero(st, b);
-    fill_zero(tt, b);
-    fill_zero(work_.corr.p, b);
+    // The pass reads the source's level-L interiors only and adds into the
+    // level-L targets only.
+    launch::zero_blocks(stream_, n_, tt, p_level_slots_[L].p, p_level_slots_[L].size(), b);
     launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[L].p, p_level_slots_[L].size(), true);
     const double dx = dx_base_ / std::pow(2.0, L - base_level_);
     run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), false), dx * dx, true, 0.0, b);
@@ -1016,8 +1031,9 @@ void GpuSolver::apply_if_masked_batch(const int* src, const int* dst, int k, dou
     const int    L = depth_ - 1;
     double*      st = work_.src.p;
     double*      tt = work_.tgt.p;
-    fill_zero(st, b);
-    fill_zero(tt, b);
+    // The pass reads the source's level-L interiors only and adds into the
+    // level-L targets only.
+    launch::zero_blocks(stream_, n_, tt, p_level_slots_[L].p, p_level_slots_[L].size(), b);
     for (int c = 0; c < k; ++c)
         launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_level_slots_[L].p, p_level_slots_[L].size(), true);
     const double alpha = alpha_base * std::pow(4.0, L - base_level_);

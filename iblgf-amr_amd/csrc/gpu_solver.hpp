@@ -259,7 +259,7 @@ class GpuSolver
     // pass's source/target and the source/target spectra.
     struct Work
     {
-        DevArray<double>  src, tgt, corr, fmm_s, fmm_t;
+        DevArray<double>  src, tgt, fmm_s, fmm_t;
         DevArray<double2> sspec, tspec;
         int               k = 0;
     };
@@ -282,6 +282,7 @@ class GpuSolver
     // plans
     std::map<int, std::unique_ptr<PassPlan>> passes_;
     DevArray<int>                            p_leaves_;
+    DevArray<int>                            p_nonleaf_;  // non-leaf blocks at levels >= base
     std::vector<DevArray<int>>               p_coarsify_;    // by level: parent + 8 children
     std::vector<DevArray<int>>               p_level_slots_; // by level
     std::vector<DevArray<int>>               p_leaf_level_;  // leaves by level

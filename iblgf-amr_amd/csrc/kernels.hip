@@ -67,12 +67,13 @@ __global__ __launch_bounds__(kThreads) void k_copy(
     const double* s = src + off;
     for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads)
     {
+        bool ring = false;
         if (interior)
         {
             const int i = q % N, j = (q / N) % N, k = q / (N * N);
-            if (i == 0 || j == 0 || k == 0 || i == N - 1 || j == N - 1 || k == N - 1) continue;
+            ring = i == 0 || j == 0 || k == 0 || i == N - 1 || j == N - 1 || k == N - 1;
         }
-        d[q] = s[q];
+        d[q] = ring ? 0.0 : s[q];
     }
 }
 
@@ -378,7 +379,8 @@ constexpr int kFwdThreads = 1024;
 
 template<int N, class T>
 __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __restrict__ field,
-    const int* __restrict__ slots, int count, typename Cx<T>::type* __restrict__ spec, long fcs, long scs)
+    const int* __restrict__ slots, int count, int ncomp, typename Cx<T>::type* __restrict__ spec, long fcs,
+    long scs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
@@ -389,10 +391,6 @@ __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __res
     __shared__ double2 Tw[P * N];        // [k][j]
     __shared__ double2 X1[GS * N * N];   // [g][z][y]
     __shared__ double2 Tp[GS * N * P];   // [g][z][ky]
-    const int b = blockIdx.x;
-    if (b >= count) return;
-    const int     slot = slots[b];
-    const double* in = field + blockIdx.y * fcs + (size_t)slot * G::N3;
     for (int q = threadIdx.x; q < P * N; q += kFwdThreads)
     {
         const int k = q / N, j = q - k * N;
@@ -401,87 +399,94 @@ __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __res
         Tw[q] = make_double2(cs, sn);
     }
     __syncthreads();
-    typename Cx<T>::type* out = spec + blockIdx.y * scs + (size_t)slot * G::NF;
-    for (int kx0 = 0; kx0 < H; kx0 += GS)
+    // (block, component) items strided over the grid: every kx group ends
+    // with a barrier, so the LDS intermediates are free for the next item.
+    for (int item = blockIdx.x; item < count * ncomp; item += gridDim.x)
     {
-        // x leg: real line (z, y) of N points -> planes g in [g0, g0 + per)
-        for (int it = threadIdx.x; it < SX::items; it += kFwdThreads)
+        const int     c = item / count, slot = slots[item - c * count];
+        const double* in = field + c * fcs + (size_t)slot * G::N3;
+        typename Cx<T>::type* out = spec + c * scs + (size_t)slot * G::NF;
+        for (int kx0 = 0; kx0 < H; kx0 += GS)
         {
-            const int     part = it / SX::lines, line = it - part * SX::lines;
-            const int     g0 = part * SX::per;
-            const double* src = in + line * N;
-            double        sr[SX::per], si[SX::per];
-#pragma unroll
-            for (int o = 0; o < SX::per; ++o) sr[o] = si[o] = 0.0;
-#pragma unroll 2
-            for (int x = 0; x < N; ++x)
+            // x leg: real line (z, y) of N points -> planes g in [g0, g0 + per)
+            for (int it = threadIdx.x; it < SX::items; it += kFwdThreads)
             {
-                const double v = src[x];
+                const int     part = it / SX::lines, line = it - part * SX::lines;
+                const int     g0 = part * SX::per;
+                const double* src = in + line * N;
+                double        sr[SX::per], si[SX::per];
+#pragma unroll
+                for (int o = 0; o < SX::per; ++o) sr[o] = si[o] = 0.0;
+#pragma unroll 2
+                for (int x = 0; x < N; ++x)
+                {
+                    const double v = src[x];
+#pragma unroll
+                    for (int o = 0; o < SX::per; ++o)
+                        if (g0 + o < GS)
+                        {
+                            const double2 w = Tw[(kx0 + g0 + o) * N + x];
+                            sr[o] = fma(v, w.x, sr[o]);
+                            si[o] = fma(-v, w.y, si[o]);
+                        }
+                }
 #pragma unroll
                 for (int o = 0; o < SX::per; ++o)
-                    if (g0 + o < GS)
-                    {
-                        const double2 w = Tw[(kx0 + g0 + o) * N + x];
-                        sr[o] = fma(v, w.x, sr[o]);
-                        si[o] = fma(-v, w.y, si[o]);
-                    }
+                    if (g0 + o < GS) X1[(g0 + o) * N * N + line] = make_double2(sr[o], si[o]);
             }
-#pragma unroll
-            for (int o = 0; o < SX::per; ++o)
-                if (g0 + o < GS) X1[(g0 + o) * N * N + line] = make_double2(sr[o], si[o]);
-        }
-        __syncthreads();
-        // y leg: line (g, z), N inputs y -> outputs ky in [k0, k0 + per)
-        for (int it = threadIdx.x; it < SY::items; it += kFwdThreads)
-        {
-            const int part = it / SY::lines, line = it - part * SY::lines;
-            const int g = line / N, z = line - g * N;
-            const int k0 = part * SY::per;
-            double    sr[SY::per], si[SY::per];
-#pragma unroll
-            for (int o = 0; o < SY::per; ++o) sr[o] = si[o] = 0.0;
-            const double2* src = X1 + g * N * N + z * N;
-#pragma unroll 1
-            for (int y = 0; y < N; ++y)
+            __syncthreads();
+            // y leg: line (g, z), N inputs y -> outputs ky in [k0, k0 + per)
+            for (int it = threadIdx.x; it < SY::items; it += kFwdThreads)
             {
-                const double2 a = src[y];
+                const int part = it / SY::lines, line = it - part * SY::lines;
+                const int g = line / N, z = line - g * N;
+                const int k0 = part * SY::per;
+                double    sr[SY::per], si[SY::per];
+#pragma unroll
+                for (int o = 0; o < SY::per; ++o) sr[o] = si[o] = 0.0;
+                const double2* src = X1 + g * N * N + z * N;
+#pragma unroll 1
+                for (int y = 0; y < N; ++y)
+                {
+                    const double2 a = src[y];
+#pragma unroll
+                    for (int o = 0; o < SY::per; ++o)
+                        if (k0 + o < P) cmac_conj(a, Tw[(k0 + o) * N + y], sr[o], si[o]);
+                }
 #pragma unroll
                 for (int o = 0; o < SY::per; ++o)
-                    if (k0 + o < P) cmac_conj(a, Tw[(k0 + o) * N + y], sr[o], si[o]);
+                    if (k0 + o < P) Tp[(g * N + z) * P + k0 + o] = make_double2(sr[o], si[o]);
             }
-#pragma unroll
-            for (int o = 0; o < SY::per; ++o)
-                if (k0 + o < P) Tp[(g * N + z) * P + k0 + o] = make_double2(sr[o], si[o]);
-        }
-        __syncthreads();
-        // z leg: line (g, ky), N inputs z -> outputs kz in [k0, k0 + per), to global memory
-        for (int it = threadIdx.x; it < SZ::items; it += kFwdThreads)
-        {
-            const int part = it / SZ::lines, line = it - part * SZ::lines;
-            const int g = line / P, ky = line - g * P, kx = kx0 + g;
-            const int k0 = part * SZ::per;
-            double    sr[SZ::per], si[SZ::per];
-#pragma unroll
-            for (int o = 0; o < SZ::per; ++o) sr[o] = si[o] = 0.0;
-#pragma unroll 1
-            for (int z = 0; z < N; ++z)
+            __syncthreads();
+            // z leg: line (g, ky), N inputs z -> outputs kz in [k0, k0 + per), to global memory
+            for (int it = threadIdx.x; it < SZ::items; it += kFwdThreads)
             {
-                const double2 a = Tp[(g * N + z) * P + ky];
+                const int part = it / SZ::lines, line = it - part * SZ::lines;
+                const int g = line / P, ky = line - g * P, kx = kx0 + g;
+                const int k0 = part * SZ::per;
+                double    sr[SZ::per], si[SZ::per];
 #pragma unroll
-                for (int o = 0; o < SZ::per; ++o)
-                    if (k0 + o < P) cmac_conj(a, Tw[(k0 + o) * N + z], sr[o], si[o]);
+                for (int o = 0; o < SZ::per; ++o) sr[o] = si[o] = 0.0;
+#pragma unroll 1
+                for (int z = 0; z < N; ++z)
+                {
+                    const double2 a = Tp[(g * N + z) * P + ky];
+#pragma unroll
+                    for (int o = 0; o < SZ::per; ++o)
+                        if (k0 + o < P) cmac_conj(a, Tw[(k0 + o) * N + z], sr[o], si[o]);
+                }
+                typename Cx<T>::type* o = out + (size_t)(kx * P + ky) * P + k0;
+#pragma unroll
+                for (int q = 0; q < SZ::per; ++q)
+                    if (k0 + q < P) o[q] = Cx<T>::make((T)sr[q], (T)si[q]);
             }
-            typename Cx<T>::type* o = out + (size_t)(kx * P + ky) * P + k0;
-#pragma unroll
-            for (int q = 0; q < SZ::per; ++q)
-                if (k0 + q < P) o[q] = Cx<T>::make((T)sr[q], (T)si[q]);
+            __syncthreads();
         }
-        __syncthreads();
     }
 }
 
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
-// added into the target block.  Per group of GS kx-planes: z leg from global
+// written to the target block (every point of it).  Per group of GS kx-planes: z leg from global
 // memory, y leg in LDS, and the group's share of the c2r x leg accumulated in
 // registers; every leg keeps its outputs in registers, streams its inputs and
 // is split over the workgroup's kInvThreads threads as LegSplit (twice the
@@ -590,7 +595,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
         double* out = field + blockIdx.y * fcs + (size_t)slot * G::N3 + xline * N + x0;
 #pragma unroll
         for (int x = 0; x < SX::per; ++x)
-            if (x0 + x < N) out[x] += scale * acc[x];
+            if (x0 + x < N) out[x] = scale * acc[x];
     }
 }
 
@@ -1042,17 +1047,35 @@ template<int N> struct RunStencil
             om2, b.fstride);
     }
 };
+// Workgroups of a launch of `items` independent work items (a loop over
+// items strided by the grid inside the kernel): as many as the device holds
+// at once, at most one per item.
+template<class F>
+inline int resident_grid(F kernel, int threads, long items)
+{
+    int dev = 0, ncu = 0, occ = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess || ncu <= 0 || occ <= 0)
+        return static_cast<int>(items);
+    const long slots = static_cast<long>(ncu) * occ;
+    return static_cast<int>(items < slots ? items : slots);
+}
+
 template<int N> struct RunFwd
 {
     static void run(hipStream_t s, int prec, const double* f, const int* slots, int count, void* spec,
         const Batch& b)
     {
+        const long items = static_cast<long>(count) * b.k;
         if (prec == 0)
-            hipLaunchKernelGGL((k_forward_dft<N, double>), dim3(count, b.k), dim3(kFwdThreads), 0, s, f, slots, count,
-                (double2*)spec, b.fstride, b.sstride);
+            hipLaunchKernelGGL((k_forward_dft<N, double>),
+                dim3(resident_grid(k_forward_dft<N, double>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, slots,
+                count, b.k, (double2*)spec, b.fstride, b.sstride);
         else
-            hipLaunchKernelGGL((k_forward_dft<N, float>), dim3(count, b.k), dim3(kFwdThreads), 0, s, f, slots, count,
-                (float2*)spec, b.fstride, b.sstride);
+            hipLaunchKernelGGL((k_forward_dft<N, float>),
+                dim3(resident_grid(k_forward_dft<N, float>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, slots,
+                count, b.k, (float2*)spec, b.fstride, b.sstride);
     }
 };
 template<int N> struct RunInv
@@ -1061,6 +1084,8 @@ template<int N> struct RunInv
         const Batch& b)
     {
         (void)prec;  // target spectra are FP64 on both paths
+        // One workgroup per (block, component): a workgroup looping over
+        // blocks, as the forward transform does, measured slower here.
         hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count, b.k), dim3(kInvThreads), 0, s, (const double2*)spec, slots,
             count, f, scale, b.sstride, b.fstride);
     }
@@ -1080,14 +1105,7 @@ inline int mac_threads(long nvg, int kf)
 template<class F>
 inline int mac_grid(F kernel, int nt, int nchunk)
 {
-    if (nt < kMacMaxThreads) return nchunk;
-    int dev = 0, ncu = 0, occ = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, nt, 0) != hipSuccess || ncu <= 0 || occ <= 0)
-        return nchunk;
-    const int slots = ncu * occ;
-    return nchunk > slots ? slots : nchunk;
+    return nt < kMacMaxThreads ? nchunk : resident_grid(kernel, nt, nchunk);
 }
 
 template<int N> struct RunMac

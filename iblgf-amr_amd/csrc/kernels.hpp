@@ -55,6 +55,8 @@ namespace launch {
 bool supported_n(int n);
 
 void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count, const Batch& b = {});
+// Whole blocks; interior_only copies the interior points and writes zeros on
+// the one-point boundary ring of dst.
 void copy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
     bool interior_only, const Batch& b = {});
 void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
@@ -79,6 +81,7 @@ void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* 
 // LDS intermediates and fields FP64.
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
     const Batch& b = {});
+// Writes (does not accumulate) every point of each listed field block.
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale, const Batch& b = {});
 // olist[nused]: the offsets (0..342, o = (ox+3) + 7 (oy+3) + 49 (oz+3)) the

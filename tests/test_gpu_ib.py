@@ -125,6 +125,33 @@ def test_ib_masked_solves(sphere):
     assert l2 < 1e-10
 
 
+def test_masked_solves_independent_of_earlier_solves(sphere):
+    """A masked solve after masked and full solves of noise gives bitwise the
+    result of a fresh solver (the solver clears only what it accumulates into)."""
+    s, _, _, _ = sphere
+    rng = np.random.default_rng(17)
+    fresh = PoissonSolver(s.tree)
+    used = PoissonSolver(s.tree)
+    src = rng.standard_normal(fresh.shape)
+    noise = 1e3 * rng.standard_normal(fresh.shape)
+
+    def masked(solver, x, mask, kernel, alpha):
+        fs, fd = solver.field(), solver.field()
+        solver.upload(fs, x)
+        solver.apply_lgf_masked(fs, fd, mask, kernel, alpha)
+        return solver.download(fd)
+
+    for mask, kernel, alpha in [(MASK_xIB2IB, 0, 0.0), (MASK_IB2xIB, KERNEL_IF, 0.7)]:
+        print(f"case: masked solve after noise, mask {mask}, kernel {kernel}", flush=True)
+        want = masked(fresh, src, mask, kernel, alpha)
+        fs, fd = used.field(), used.field()
+        used.upload(fs, noise)
+        used.apply_lgf(fs, fd)
+        masked(used, noise, MASK_xIB2IB, 0, 0.0)
+        masked(used, noise, MASK_IB2xIB, KERNEL_IF, 0.7)
+        assert np.array_equal(masked(used, src, mask, kernel, alpha), want)
+
+
 def test_ib_transfer_bitwise(sphere):
     s, solver, ot, ib = sphere
     sysm = IbSystem(solver)
