@@ -343,6 +343,21 @@ void GpuSolver::field_download(int id, double* host)
 
 void GpuSolver::fill_zero(double* f, const Batch& b) { launch::zero_blocks(stream_, n_, f, p_all_.p, nb_, b); }
 
+FieldTable GpuSolver::field_table(const int* ids, int k)
+{
+    static_assert(kMaxComponents <= kMaxFieldTable, "field tables hold every component");
+    FieldTable t{};
+    for (int c = 0; c < k; ++c) t.p[c] = F(ids[c]);
+    return t;
+}
+
+FieldTable GpuSolver::work_table(double* base, int k) const
+{
+    FieldTable t{};
+    for (int c = 0; c < k; ++c) t.p[c] = base + c * field_size();
+    return t;
+}
+
 void GpuSolver::reserve_components(int k)
 {
     if (k < 1 || k > kMaxComponents) throw std::invalid_argument("component count out of range");
@@ -387,18 +402,14 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
 
     // Level-L sources are copied in (interior, zero ring) and coarser ones
     // accumulate anterpolated values, so only the coarser ones are cleared.
-    std::vector<int> zs, cs, add;
+    std::vector<int> zs, cs;
     for (int l = L - 1; l >= 0; --l)
         for (int s = lvl_begin_[l]; s < lvl_begin_[l + 1]; ++s)
             if (S(s)) zs.push_back(s);
     for (int s = lvl_begin_[L]; s < lvl_begin_[L + 1]; ++s)
-    {
         if (S(s)) cs.push_back(s);
-        if (T(s)) add.push_back(s);
-    }
     p.zero_s.upload(zs);
     p.copy_s.upload(cs);
-    p.add.upload(add);
 
     p.antrp.clear();
     p.antrp.resize(L + 1);
@@ -636,13 +647,20 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
         if (is_src[s]) fwd.push_back(s);
     p.fwd.upload(fwd);
     p.inv.upload(inv);
-    // The inverse transform writes its targets whole; the targets it does not
-    // write (no sources) are cleared, since interpolation and the final add
-    // read every target.
+    p.ninv_top = 0;
+    while (p.ninv_top < static_cast<int>(inv.size()) && inv[p.ninv_top] >= lvl_begin_[L] &&
+           inv[p.ninv_top] < lvl_begin_[L + 1])
+        ++p.ninv_top;
+    for (size_t i = p.ninv_top; i < inv.size(); ++i)
+        if (inv[i] >= lvl_begin_[L]) throw std::logic_error("pass plan: level-L targets must lead the inverse list");
+    // The level-L targets are added into the solve's target array directly;
+    // the coarser targets are written whole by the inverse transform, and the
+    // ones it does not write (no sources) are cleared for the interpolation
+    // to read and add into.
     std::vector<char> written(nb_, 0);
     for (int t : inv) written[t] = 1;
     std::vector<int> zt;
-    for (int l = L; l >= 0; --l)
+    for (int l = L - 1; l >= 0; --l)
         for (int s = lvl_begin_[l]; s < lvl_begin_[l + 1]; ++s)
             if (T(s) && !written[s]) zt.push_back(s);
     p.zero_t.upload(zt);
@@ -789,22 +807,25 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
             m.nused, m.compact, gh, work_.sspec.p, work_.tspec.p, b);
         timer_.end(stream_, KC_MAC, m.npairs * b.k);
     }
+    // Level-L targets (first in inv) add into the target array, coarser ones
+    // are written to fmm_t and interpolated down, the last interpolation
+    // adding into the level-L targets.
     const double P = 2.0 * n_ - 1.0;
+    const int    ntop = p.ninv_top, nrest = static_cast<int>(p.inv.size()) - ntop;
     timer_.begin(stream_);
+    launch::inverse_dft(stream_, n_, precision_, work_.tspec.p, p.inv.p, ntop, work_.tgt.p, scale / (P * P * P),
+        true, b);
     launch::inverse_dft(
-        stream_, n_, precision_, work_.tspec.p, p.inv.p, p.inv.size(), fmm_t, scale / (P * P * P), b);
+        stream_, n_, precision_, work_.tspec.p, p.inv.p + ntop, nrest, fmm_t, scale / (P * P * P), false, b);
     timer_.end(stream_, KC_INVERSE, static_cast<long>(p.inv.size()) * b.k);
     if (!p.neighbor_only)
     {
         timer_.begin(stream_);
         for (int l = 1; l < p.L; ++l)
-            launch::transfer_scatter(
-                stream_, n_, fmm_t, fmm_t, p.intrp[l].p, p.intrp[l].size() / 3, m_nli_interp_.p, true, b);
+            launch::transfer_scatter(stream_, n_, fmm_t, l + 1 == p.L ? work_.tgt.p : fmm_t, p.intrp[l].p,
+                p.intrp[l].size() / 3, m_nli_interp_.p, true, b);
         timer_.end(stream_, KC_TRANSFER, (p.ntransfer - p.ntransfer_antrp) * b.k);
     }
-    timer_.begin(stream_);
-    launch::axpy_blocks(stream_, n_, work_.tgt.p, fmm_t, p.add.p, p.add.size(), 1.0, b);
-    timer_.end(stream_, KC_OTHER, 0);
 }
 
 void GpuSolver::check_batch(const int* src, const int* dst, int k)
@@ -900,7 +921,6 @@ void GpuSolver::apply_lgf_batch(const int* src, const int* dst, int k)
 void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
 {
     const Batch b = batch(k);
-    const size_t fs = field_size();
     double* st = work_.src.p;
     double* tt = work_.tgt.p;
     timer_.begin(stream_);
@@ -910,8 +930,8 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
     // interpolation from its parent.
     launch::zero_blocks(stream_, n_, st, p_nonleaf_.p, p_nonleaf_.size(), b);
     launch::zero_blocks(stream_, n_, tt, p_level_slots_[base_level_].p, p_level_slots_[base_level_].size(), b);
-    for (int c = 0; c < k; ++c)
-        launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_leaves_.p, p_leaves_.size(), false);
+    launch::copy_blocks_table(
+        stream_, n_, work_table(st, k), field_table(src, k), p_leaves_.p, p_leaves_.size(), false, k);
     timer_.end(stream_, KC_OTHER, 0);
     timer_.begin(stream_);
     long ncoarse = 0;
@@ -931,9 +951,8 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
         timer_.end(stream_, KC_TRANSFER, static_cast<long>(p_ccint_[l].size() / 3) * k);
         run_pass(pass(l, 0, false), dx * dx, true, 0.0, b);
         timer_.begin(stream_);
-        for (int c = 0; c < k; ++c)
-            launch::copy_blocks(
-                stream_, n_, F(dst[c]), tt + c * fs, p_leaf_level_[l].p, p_leaf_level_[l].size(), false);
+        launch::copy_blocks_table(stream_, n_, field_table(dst, k), work_table(tt, k), p_leaf_level_[l].p,
+            p_leaf_level_[l].size(), false, k);
         if (l < depth_ - 1)
         {
             // Source correction of the children, added into the source in
@@ -966,13 +985,12 @@ void GpuSolver::apply_if_batch(const int* src, const int* dst, int k, double alp
 void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alpha_base)
 {
     const Batch b = batch(k);
-    const size_t fs = field_size();
     double* st = work_.src.p;
     double* tt = work_.tgt.p;
     launch::zero_blocks(stream_, n_, st, p_nonleaf_.p, p_nonleaf_.size(), b);
     fill_zero(tt, b);
-    for (int c = 0; c < k; ++c)
-        launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_leaves_.p, p_leaves_.size(), false);
+    launch::copy_blocks_table(
+        stream_, n_, work_table(st, k), field_table(src, k), p_leaves_.p, p_leaves_.size(), false, k);
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
         launch::transfer_gather(
             stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, b);
@@ -987,9 +1005,8 @@ void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alph
         const double alpha = alpha_base * std::pow(4.0, l - base_level_);
         run_pass(pass(l, 0, true), 1.0, false, alpha, b);
         run_pass(pass(l, 1, true), 1.0, false, alpha, b);
-        for (int c = 0; c < k; ++c)
-            launch::copy_blocks(
-                stream_, n_, F(dst[c]), tt + c * fs, p_level_slots_[l].p, p_level_slots_[l].size(), false);
+        launch::copy_blocks_table(stream_, n_, field_table(dst, k), work_table(tt, k), p_level_slots_[l].p,
+            p_level_slots_[l].size(), false, k);
     }
     IBLGF_HIP(hipGetLastError());
 }
@@ -1027,20 +1044,18 @@ void GpuSolver::apply_if_masked_batch(const int* src, const int* dst, int k, dou
         throw std::invalid_argument("mask type must be IB2xIB or xIB2IB");
     check_batch(src, dst, k);
     const Batch  b = batch(k);
-    const size_t fs = field_size();
     const int    L = depth_ - 1;
     double*      st = work_.src.p;
     double*      tt = work_.tgt.p;
     // The pass reads the source's level-L interiors only and adds into the
     // level-L targets only.
     launch::zero_blocks(stream_, n_, tt, p_level_slots_[L].p, p_level_slots_[L].size(), b);
-    for (int c = 0; c < k; ++c)
-        launch::copy_blocks(stream_, n_, st + c * fs, F(src[c]), p_level_slots_[L].p, p_level_slots_[L].size(), true);
+    launch::copy_blocks_table(stream_, n_, work_table(st, k), field_table(src, k), p_level_slots_[L].p,
+        p_level_slots_[L].size(), true, k);
     const double alpha = alpha_base * std::pow(4.0, L - base_level_);
     run_pass(pass_idx(L, fmm_mask_idx_gen(mask_type), true), 1.0, false, alpha, b);
-    for (int c = 0; c < k; ++c)
-        launch::copy_blocks(
-            stream_, n_, F(dst[c]), tt + c * fs, p_level_slots_[L].p, p_level_slots_[L].size(), false);
+    launch::copy_blocks_table(stream_, n_, field_table(dst, k), work_table(tt, k), p_level_slots_[L].p,
+        p_level_slots_[L].size(), false, k);
     IBLGF_HIP(hipGetLastError());
 }
 

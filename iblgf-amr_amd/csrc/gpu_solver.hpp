@@ -71,7 +71,8 @@ struct PassPlan
 {
     int                              L = 0, nl = 0, idx = 0;
     bool                             neighbor_only = false;
-    DevArray<int>                    zero_s, zero_t, copy_s, fwd, inv, add;
+    DevArray<int>                    zero_s, zero_t, copy_s, fwd, inv;
+    int                              ninv_top = 0;  // inv starts with the level-L targets
     std::vector<DevArray<int>>       antrp;  // by level (parents)
     std::vector<DevArray<int>>       intrp;  // by level (parents at level)
     std::vector<MacLevelPlan>        mac;
@@ -224,8 +225,10 @@ class GpuSolver
     double2*    ghat_laplace(int d);
     double2*    ghat_if(double alpha);
     const void* ghat(bool laplace, int d, double alpha);  // in the current precision
-    void      fill_zero(double* f, const Batch& b = {});
-    double*   F(int id) { return fields_.at(id).p; }
+    void       fill_zero(double* f, const Batch& b = {});
+    double*    F(int id) { return fields_.at(id).p; }
+    FieldTable field_table(const int* ids, int k);           // the fields ids[0..k)
+    FieldTable work_table(double* base, int k) const;        // components of a work array
 
     int         device_ = 0;
     hipStream_t stream_ = nullptr;

@@ -57,14 +57,8 @@ __global__ __launch_bounds__(kThreads) void k_zero(double* f, const int* slots, 
 }
 
 template<int N>
-__global__ __launch_bounds__(kThreads) void k_copy(
-    double* dst, const double* src, const int* slots, int count, int interior, long cs)
+__device__ inline void copy_block(double* d, const double* s, int interior)
 {
-    const int b = blockIdx.x;
-    if (b >= count) return;
-    const size_t  off = blockIdx.y * cs + (size_t)slots[b] * Geo<N>::N3;
-    double*       d = dst + off;
-    const double* s = src + off;
     for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads)
     {
         bool ring = false;
@@ -75,6 +69,26 @@ __global__ __launch_bounds__(kThreads) void k_copy(
         }
         d[q] = ring ? 0.0 : s[q];
     }
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_copy(
+    double* dst, const double* src, const int* slots, int count, int interior, long cs)
+{
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const size_t off = blockIdx.y * cs + (size_t)slots[b] * Geo<N>::N3;
+    copy_block<N>(dst + off, src + off, interior);
+}
+
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_copy_table(
+    FieldTable dst, FieldTable src, const int* slots, int count, int interior)
+{
+    const int b = blockIdx.x;
+    if (b >= count) return;
+    const size_t off = (size_t)slots[b] * Geo<N>::N3;
+    copy_block<N>(dst.p[blockIdx.y] + off, src.p[blockIdx.y] + off, interior);
 }
 
 template<int N>
@@ -236,53 +250,84 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
 // ---------------------------------------------------------------------------
 // Halo fill and correction stencil
 // ---------------------------------------------------------------------------
+// Boundary-ring point r (0 <= r < N^3 - (N-2)^3) of a block: the z = 0 and
+// z = N-1 planes, then per interior z layer the square's perimeter.
+template<int N>
+__device__ inline int ring_point(int r)
+{
+    constexpr int NN = N * N, PER = 4 * (N - 1);
+    if (r < 2 * NN) return r < NN ? r : (N - 1) * NN + (r - NN);
+    r -= 2 * NN;
+    const int k = 1 + r / PER, p = r - (k - 1) * PER;
+    int       i, j;
+    if (p < N) i = p, j = 0;
+    else if (p < 2 * N) i = p - N, j = N - 1;
+    else if (p < 3 * N - 2) i = 0, j = 1 + (p - 2 * N);
+    else
+        i = N - 1, j = 1 + (p - 3 * N + 2);
+    return k * NN + j * N + i;
+}
+
+// Ring points take the facing interior points of the face, edge or corner
+// neighbour (the reads are interior points only, so blocks run concurrently).
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_buffer_exchange(
     double* f, const int* slots, int count, const int* __restrict__ neighbor, long cs)
 {
-    constexpr int N3 = Geo<N>::N3, E = Geo<N>::E;
+    constexpr int N3 = Geo<N>::N3, E = Geo<N>::E, RING = N3 - E * E * E;
     const int b = blockIdx.x;
     if (b >= count) return;
     f += blockIdx.y * cs;
     const int s = slots[b];
-    for (int q = threadIdx.x; q < N3; q += kThreads)
+    for (int r = threadIdx.x; r < RING; r += kThreads)
     {
+        const int q = ring_point<N>(r);
         const int i = q % N, j = (q / N) % N, k = q / (N * N);
         const int dx = i == 0 ? -1 : (i == N - 1 ? 1 : 0);
         const int dy = j == 0 ? -1 : (j == N - 1 ? 1 : 0);
         const int dz = k == 0 ? -1 : (k == N - 1 ? 1 : 0);
-        if (!dx && !dy && !dz) continue;
         const int nb = neighbor[27 * s + (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
         if (nb < 0) continue;
         f[(size_t)s * N3 + q] = f[(size_t)nb * N3 + q - (dz * E * N * N + dy * E * N + dx * E)];
     }
 }
 
+// c += (6 t - sum of the 6 neighbours) inv + om2 t on interior points; one
+// thread per interior (i, j) column walks z with its z neighbours in
+// registers.
 template<int N>
 __global__ __launch_bounds__(kThreads) void k_correction_stencil(
     const double* __restrict__ tt, double* __restrict__ cc, const int* slots, int count, double inv, double om2,
     long cs)
 {
-    constexpr int N3 = Geo<N>::N3;
+    constexpr int N3 = Geo<N>::N3, E = Geo<N>::E, NN = N * N;
     const int b = blockIdx.x;
     if (b >= count) return;
     const size_t  off = blockIdx.y * cs + (size_t)slots[b] * N3;
     const double* t = tt + off;
     double*       c = cc + off;
-    for (int q = threadIdx.x; q < N3; q += kThreads)
+    for (int col = threadIdx.x; col < E * E; col += kThreads)
     {
-        const int i = q % N, j = (q / N) % N, k = q / (N * N);
-        if (i == 0 || j == 0 || k == 0 || i == N - 1 || j == N - 1 || k == N - 1) continue;
-        double v = c[q];
-        v += 6.0 * t[q] * inv;
-        v -= t[q - N * N] * inv;
-        v -= t[q + N * N] * inv;
-        v -= t[q - N] * inv;
-        v -= t[q + N] * inv;
-        v -= t[q + 1] * inv;
-        v -= t[q - 1] * inv;
-        v += om2 * t[q];
-        c[q] = v;
+        const int q0 = (1 + col / E) * N + 1 + col % E;  // (i, j) at z = 0
+        double    below = t[q0], mid = t[q0 + NN];
+#pragma unroll 4
+        for (int k = 1; k <= E; ++k)
+        {
+            const int    q = q0 + k * NN;
+            const double above = t[q + NN];
+            double       v = c[q];
+            v += 6.0 * mid * inv;
+            v -= below * inv;
+            v -= above * inv;
+            v -= t[q - N] * inv;
+            v -= t[q + N] * inv;
+            v -= t[q + 1] * inv;
+            v -= t[q - 1] * inv;
+            v += om2 * mid;
+            c[q] = v;
+            below = mid;
+            mid = above;
+        }
     }
 }
 
@@ -486,7 +531,7 @@ __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __res
 }
 
 // Inverse c2r restricted to the output window [n-1, 2n-2]^3, scaled and
-// written to the target block (every point of it).  Per group of GS kx-planes: z leg from global
+// written to (or added into) the target block, every point of it.  Per group of GS kx-planes: z leg from global
 // memory, y leg in LDS, and the group's share of the c2r x leg accumulated in
 // registers; every leg keeps its outputs in registers, streams its inputs and
 // is split over the workgroup's kInvThreads threads as LegSplit (twice the
@@ -496,7 +541,8 @@ constexpr int kInvThreads = 768;
 
 template<int N>
 __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __restrict__ spec,
-    const int* __restrict__ slots, int count, double* __restrict__ field, double scale, long scs, long fcs)
+    const int* __restrict__ slots, int count, double* __restrict__ field, double scale, int accumulate, long scs,
+    long fcs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
@@ -526,6 +572,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
 #pragma unroll
     for (int x = 0; x < SX::per; ++x) acc[x] = 0.0;
     __syncthreads();
+#pragma unroll
     for (int kx0 = 0; kx0 < H; kx0 += GS)
     {
         // z leg: row (g, ky) of P frequencies -> window points z in [z0, z0 + per)
@@ -573,6 +620,17 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
                 if (y0 + o < N) V[(g * N + z) * N + y0 + o] = make_double2(sr[o], si[o]);
         }
         __syncthreads();
+        // The last group also writes the block; a block added into has its
+        // old values loaded before that group's x leg, so the loads overlap
+        // it (and live only through it).
+        const bool last = kx0 + GS >= H;
+        double*    out = field + blockIdx.y * fcs + (size_t)slot * G::N3 + xline * N + x0;
+        double     prev[SX::per];
+        if (last && accumulate && threadIdx.x < SX::items)
+        {
+#pragma unroll
+            for (int x = 0; x < SX::per; ++x) prev[x] = x0 + x < N ? out[x] : 0.0;
+        }
         // the group's part of the c2r x leg: line (z, y) -> window points x in [x0, x0 + per)
         if (threadIdx.x < SX::items)
             for (int g = 0; g < GS; ++g)
@@ -588,14 +646,13 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
                         acc[x] = fma(c, fma(a.x, w.x, -a.y * w.y), acc[x]);
                     }
             }
-        __syncthreads();
-    }
-    if (threadIdx.x < SX::items)
-    {
-        double* out = field + blockIdx.y * fcs + (size_t)slot * G::N3 + xline * N + x0;
+        if (last && threadIdx.x < SX::items)
+        {
 #pragma unroll
-        for (int x = 0; x < SX::per; ++x)
-            if (x0 + x < N) out[x] = scale * acc[x];
+            for (int x = 0; x < SX::per; ++x)
+                if (x0 + x < N) out[x] = accumulate ? prev[x] + scale * acc[x] : scale * acc[x];
+        }
+        __syncthreads();
     }
 }
 
@@ -1004,6 +1061,14 @@ template<int N> struct RunCopy
             b.fstride);
     }
 };
+template<int N> struct RunCopyTable
+{
+    static void run(hipStream_t s, const FieldTable& d, const FieldTable& src, const int* slots, int count,
+        int interior, int k)
+    {
+        hipLaunchKernelGGL(k_copy_table<N>, dim3(count, k), dim3(kThreads), 0, s, d, src, slots, count, interior);
+    }
+};
 template<int N> struct RunAxpy
 {
     static void run(hipStream_t s, double* d, const double* src, const int* slots, int count, double a,
@@ -1081,13 +1146,13 @@ template<int N> struct RunFwd
 template<int N> struct RunInv
 {
     static void run(hipStream_t s, int prec, const void* spec, const int* slots, int count, double* f, double scale,
-        const Batch& b)
+        int accumulate, const Batch& b)
     {
         (void)prec;  // target spectra are FP64 on both paths
         // One workgroup per (block, component): a workgroup looping over
         // blocks, as the forward transform does, measured slower here.
         hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count, b.k), dim3(kInvThreads), 0, s, (const double2*)spec, slots,
-            count, f, scale, b.sstride, b.fstride);
+            count, f, scale, accumulate, b.sstride, b.fstride);
     }
 };
 // Workgroup size of a MAC launch: every (group, component) pair of the launch
@@ -1165,6 +1230,12 @@ void copy_blocks(hipStream_t s, int n, double* dst, const double* src, const int
 {
     if (count > 0) dispatch_n<RunCopy>(n, s, dst, src, slots, count, interior_only ? 1 : 0, b);
 }
+void copy_blocks_table(hipStream_t s, int n, const FieldTable& dst, const FieldTable& src, const int* slots,
+    int count, bool interior_only, int k)
+{
+    if (k < 1 || k > kMaxFieldTable) throw std::invalid_argument("copy_blocks_table: component count out of range");
+    if (count > 0) dispatch_n<RunCopyTable>(n, s, dst, src, slots, count, interior_only ? 1 : 0, k);
+}
 void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a,
     const Batch& b)
 {
@@ -1196,9 +1267,9 @@ void forward_dft(hipStream_t s, int n, int prec, const double* field, const int*
     if (count > 0) dispatch_n<RunFwd>(n, s, prec, field, slots, count, spec, b);
 }
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
-    double scale, const Batch& b)
+    double scale, bool accumulate, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale, b);
+    if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale, accumulate ? 1 : 0, b);
 }
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
     const uint4* rowoff, const int* olist, int nused, bool compact, const void* ghat, const void* sspec,

@@ -44,6 +44,13 @@ struct Batch
     long sstride = 0;
 };
 
+// Per-component field base pointers, passed by value (component c in p[c]).
+constexpr int kMaxFieldTable = 64;
+struct FieldTable
+{
+    double* p[kMaxFieldTable];
+};
+
 // Bytes per staged kernel row of the multiply-accumulate (KF frequencies of
 // one offset: 8 FP64 or 16 FP32 complex values), and the largest offset count
 // of a compact launch.
@@ -59,6 +66,9 @@ void zero_blocks(hipStream_t s, int n, double* f, const int* slots, int count, c
 // the one-point boundary ring of dst.
 void copy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
     bool interior_only, const Batch& b = {});
+// The same for k components whose fields sit at unrelated addresses, in one launch.
+void copy_blocks_table(hipStream_t s, int n, const FieldTable& dst, const FieldTable& src, const int* slots,
+    int count, bool interior_only, int k);
 void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
     double a, const Batch& b = {});
 
@@ -81,9 +91,9 @@ void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* 
 // LDS intermediates and fields FP64.
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
     const Batch& b = {});
-// Writes (does not accumulate) every point of each listed field block.
+// Writes (accumulate: adds into) every point of each listed field block.
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
-    double scale, const Batch& b = {});
+    double scale, bool accumulate, const Batch& b = {});
 // olist[nused]: the offsets (0..342, o = (ox+3) + 7 (oy+3) + 49 (oz+3)) the
 // launch's entries use.  A compact launch (nused <= kMacCompactMax) stages
 // only those rows, row i = olist[i]; otherwise row o = offset o.  Row nrows
