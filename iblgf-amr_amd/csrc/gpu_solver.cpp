@@ -400,15 +400,9 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
     auto      S = [&](int s) { return (msrc_[s] >> idx) & 1ULL; };
     auto      T = [&](int s) { return (mtgt_[s] >> idx) & 1ULL; };
 
-    // Level-L sources are copied in (interior, zero ring) and coarser ones
-    // accumulate anterpolated values, so only the coarser ones are cleared.
-    std::vector<int> zs, cs;
-    for (int l = L - 1; l >= 0; --l)
-        for (int s = lvl_begin_[l]; s < lvl_begin_[l + 1]; ++s)
-            if (S(s)) zs.push_back(s);
+    std::vector<int> cs;
     for (int s = lvl_begin_[L]; s < lvl_begin_[L + 1]; ++s)
         if (S(s)) cs.push_back(s);
-    p.zero_s.upload(zs);
     p.copy_s.upload(cs);
 
     p.antrp.clear();
@@ -439,6 +433,22 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
             p.antrp[l].upload(ent);
         }
         p.ntransfer_antrp = p.ntransfer;
+        // Level-L sources are copied in (interior, zero ring) and the coarser
+        // parents are written whole by the anterpolation; the coarser sources
+        // it does not write (no source children) are cleared.
+        std::vector<char> written(nb_, 0);
+        for (int s = 0; s < lvl_begin_[L]; ++s)
+            if (S(s))
+                for (int c = 0; c < 8; ++c)
+                {
+                    const int ch = children_[8 * s + c];
+                    if (ch >= 0 && S(ch)) written[s] = 1;
+                }
+        std::vector<int> zs;
+        for (int l = L - 1; l >= 0; --l)
+            for (int s = lvl_begin_[l]; s < lvl_begin_[l + 1]; ++s)
+                if (S(s) && !written[s]) zs.push_back(s);
+        p.zero_s.upload(zs);
         for (int l = 1; l < L; ++l)
         {
             std::vector<int> ent;
@@ -793,7 +803,7 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
         timer_.begin(stream_);
         for (int l = p.L - 1; l >= 0; --l)
             launch::transfer_gather(
-                stream_, n_, fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9, m_nli_anterp_.p, b);
+                stream_, n_, fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9, m_nli_anterp_.p, false, b);
         timer_.end(stream_, KC_TRANSFER, p.ntransfer_antrp * b.k);
     }
     timer_.begin(stream_);
@@ -924,10 +934,12 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
     double* st = work_.src.p;
     double* tt = work_.tgt.p;
     timer_.begin(stream_);
-    // Source: leaves copied whole, the blocks above them cleared for the
-    // coarsening to accumulate into.  Target: the base level is cleared for
-    // the passes to add into; every finer block is first written whole by the
-    // interpolation from its parent.
+    // Source: leaves copied whole, the other blocks at levels >= base cleared;
+    // the coarsening then adds each parent's restricted children into it (a
+    // leaf can have correction children, so parents are added into, not
+    // overwritten).  Target: the base level is cleared for the passes to add
+    // into; every finer block is first written whole by the interpolation from
+    // its parent.
     launch::zero_blocks(stream_, n_, st, p_nonleaf_.p, p_nonleaf_.size(), b);
     launch::zero_blocks(stream_, n_, tt, p_level_slots_[base_level_].p, p_level_slots_[base_level_].size(), b);
     launch::copy_blocks_table(
@@ -938,7 +950,7 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
     {
         launch::transfer_gather(
-            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, b);
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b);
         ncoarse += p_kids_[ls].size();
     }
     timer_.end(stream_, KC_TRANSFER, ncoarse * k);
@@ -993,7 +1005,7 @@ void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alph
         stream_, n_, work_table(st, k), field_table(src, k), p_leaves_.p, p_leaves_.size(), false, k);
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
         launch::transfer_gather(
-            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, b);
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b);
     for (int l = depth_ - 2; l >= base_level_; --l)
     {
         launch::buffer_exchange(stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p, b);

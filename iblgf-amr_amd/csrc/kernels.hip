@@ -180,13 +180,13 @@ __device__ inline void leg_z_acc(const double* t2, const double* __restrict__ Kz
     }
 }
 
-// dst[parent] += sum over listed children of K(child); one workgroup per
+// dst[parent] (=|+=) sum over listed children of K(child); one workgroup per
 // parent, the children's contributions summed in registers in child order and
-// added to the parent once (deterministic).
+// written to (added into) the parent once, every point of it (deterministic).
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
-    const double* __restrict__ mats, long cs)
+    const double* __restrict__ mats, int accumulate, long cs)
 {
     constexpr int N3 = Geo<N>::N3;
     __shared__ TransferLds<N> L;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     {
         double* out = dst + (size_t)ent[0] * N3 + threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < N; ++k) out[k * N * N] += acc[k];
+        for (int k = 0; k < N; ++k) out[k * N * N] = accumulate ? out[k * N * N] + acc[k] : acc[k];
     }
 }
 
@@ -1080,10 +1080,10 @@ template<int N> struct RunAxpy
 template<int N> struct RunGather
 {
     static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m,
-        const Batch& b)
+        int accumulate, const Batch& b)
     {
         hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count, b.k), dim3(kTransferThreads), 0, s, src, dst, e, count,
-            m, b.fstride);
+            m, accumulate, b.fstride);
     }
 };
 template<int N> struct RunScatter
@@ -1242,9 +1242,9 @@ void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int
     if (count > 0) dispatch_n<RunAxpy>(n, s, dst, src, slots, count, a, b);
 }
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, const Batch& b)
+    const double* mats, bool accumulate, const Batch& b)
 {
-    if (count > 0) dispatch_n<RunGather>(n, s, src, dst, entries, count, mats, b);
+    if (count > 0) dispatch_n<RunGather>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, b);
 }
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
     const double* mats, bool accumulate, const Batch& b)

@@ -72,10 +72,11 @@ void copy_blocks_table(hipStream_t s, int n, const FieldTable& dst, const FieldT
 void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count,
     double a, const Batch& b = {});
 
-// dst[entry.parent] += sum_c K(child) for entries {parent, child[8]}; K built
-// from the side matrices mats[2][n*n] (row = output point, col = input point).
+// dst[entry.parent] (=|+=) sum_c K(child) for entries {parent, child[8]}; K
+// built from the side matrices mats[2][n*n] (row = output point, col = input
+// point).
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, const Batch& b = {});
+    const double* mats, bool accumulate, const Batch& b = {});
 // dst[entry.child] (=|+=) K(src[entry.parent]) for entries {child, parent, cidx}.
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
     const double* mats, bool accumulate, const Batch& b = {});
