@@ -400,10 +400,6 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
     auto      S = [&](int s) { return (msrc_[s] >> idx) & 1ULL; };
     auto      T = [&](int s) { return (mtgt_[s] >> idx) & 1ULL; };
 
-    std::vector<int> cs;
-    for (int s = lvl_begin_[L]; s < lvl_begin_[L + 1]; ++s)
-        if (S(s)) cs.push_back(s);
-    p.copy_s.upload(cs);
 
     p.antrp.clear();
     p.antrp.resize(L + 1);
@@ -433,9 +429,9 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
             p.antrp[l].upload(ent);
         }
         p.ntransfer_antrp = p.ntransfer;
-        // Level-L sources are copied in (interior, zero ring) and the coarser
-        // parents are written whole by the anterpolation; the coarser sources
-        // it does not write (no source children) are cleared.
+        // Level-L sources are read in place (interior, ring as zero) and the
+        // coarser parents are written whole by the anterpolation; the coarser
+        // sources it does not write (no source children) are cleared.
         std::vector<char> written(nb_, 0);
         for (int s = 0; s < lvl_begin_[L]; ++s)
             if (S(s))
@@ -656,6 +652,10 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
     for (int s = 0; s < nb_; ++s)
         if (is_src[s]) fwd.push_back(s);
     p.fwd.upload(fwd);
+    p.nfwd_coarse = 0;
+    while (p.nfwd_coarse < static_cast<int>(fwd.size()) && fwd[p.nfwd_coarse] < lvl_begin_[L]) ++p.nfwd_coarse;
+    for (size_t i = p.nfwd_coarse; i < fwd.size(); ++i)
+        if (fwd[i] >= lvl_begin_[L + 1]) throw std::logic_error("pass plan: sources finer than the pass level");
     p.inv.upload(inv);
     p.ninv_top = 0;
     while (p.ninv_top < static_cast<int>(inv.size()) && inv[p.ninv_top] >= lvl_begin_[L] &&
@@ -796,18 +796,24 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
     timer_.begin(stream_);
     launch::zero_blocks(stream_, n_, fmm_s, p.zero_s.p, p.zero_s.size(), b);
     launch::zero_blocks(stream_, n_, fmm_t, p.zero_t.p, p.zero_t.size(), b);
-    launch::copy_blocks(stream_, n_, fmm_s, work_.src.p, p.copy_s.p, p.copy_s.size(), true, b);
     timer_.end(stream_, KC_OTHER, 0);
+    // The level-L sources are read from the solve's source array in place,
+    // their boundary rings as zero (Fmm::apply copies interiors into zeroed
+    // blocks); the coarser ones are anterpolated into fmm_s.
     if (!p.neighbor_only)
     {
         timer_.begin(stream_);
         for (int l = p.L - 1; l >= 0; --l)
-            launch::transfer_gather(
-                stream_, n_, fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9, m_nli_anterp_.p, false, b);
+        {
+            const bool top = l == p.L - 1;
+            launch::transfer_gather(stream_, n_, top ? work_.src.p : fmm_s, fmm_s, p.antrp[l].p,
+                p.antrp[l].size() / 9, m_nli_anterp_.p, false, b, top);
+        }
         timer_.end(stream_, KC_TRANSFER, p.ntransfer_antrp * b.k);
     }
     timer_.begin(stream_);
-    launch::forward_dft(stream_, n_, precision_, fmm_s, p.fwd.p, p.fwd.size(), work_.sspec.p, b);
+    launch::forward_dft(stream_, n_, precision_, fmm_s, p.fwd.p, p.fwd.size(), work_.sspec.p, b, work_.src.p,
+        p.nfwd_coarse);
     timer_.end(stream_, KC_FORWARD, static_cast<long>(p.fwd.size()) * b.k);
     for (auto& m : p.mac)
     {

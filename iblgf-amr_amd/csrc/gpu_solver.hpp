@@ -71,8 +71,9 @@ struct PassPlan
 {
     int                              L = 0, nl = 0, idx = 0;
     bool                             neighbor_only = false;
-    DevArray<int>                    zero_s, zero_t, copy_s, fwd, inv;
-    int                              ninv_top = 0;  // inv starts with the level-L targets
+    DevArray<int>                    zero_s, zero_t, fwd, inv;
+    int                              ninv_top = 0;     // inv starts with the level-L targets
+    int                              nfwd_coarse = 0;  // fwd ends with the level-L sources
     std::vector<DevArray<int>>       antrp;  // by level (parents)
     std::vector<DevArray<int>>       intrp;  // by level (parents at level)
     std::vector<MacLevelPlan>        mac;

@@ -139,15 +139,23 @@ __device__ inline void line_apply(const double (&v)[N], const double* __restrict
 // x and y legs: global block -> LDS buffer (x), then y in place.
 template<int N>
 __device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>& L, const double* __restrict__ Kx,
-    const double* __restrict__ Ky)
+    const double* __restrict__ Ky, bool ring0 = false)
 {
     static_assert(N * N <= kTransferThreads, "one line per thread");
     const int l = threadIdx.x;
     if (l < N * N)
     {
-        double v[N];
+        // ring0: the input block's boundary ring is read as zero
+        const int  lz = l / N, ly = l - lz * N;
+        const bool zl = ring0 && (lz == 0 || ly == 0 || lz == N - 1 || ly == N - 1);
+        double     v[N];
 #pragma unroll
-        for (int a = 0; a < N; ++a) v[a] = in[l * N + a];
+        for (int a = 0; a < N; ++a) v[a] = in[l * N + a];  // loaded unconditionally, then masked (no branches)
+        if (ring0)
+        {
+#pragma unroll
+            for (int a = 0; a < N; ++a) v[a] = (zl || a == 0 || a == N - 1) ? 0.0 : v[a];
+        }
         line_apply<N>(v, Kx, L.buf, l * N, 1);
     }
     __syncthreads();
@@ -186,7 +194,7 @@ __device__ inline void leg_z_acc(const double* t2, const double* __restrict__ Kz
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
-    const double* __restrict__ mats, int accumulate, long cs)
+    const double* __restrict__ mats, int accumulate, int ring0, long cs)
 {
     constexpr int N3 = Geo<N>::N3;
     __shared__ TransferLds<N> L;
@@ -205,7 +213,7 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
         if (ch < 0) continue;  // uniform across the workgroup
         if (!first) __syncthreads();  // previous z leg done with the buffer
         first = false;
-        transfer_xy<N>(src + (size_t)ch * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N);
+        transfer_xy<N>(src + (size_t)ch * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N, ring0);
         if (threadIdx.x < N * N) leg_z_acc<N>(L.buf, mats + ((c >> 2) & 1) * N * N, acc);
     }
     if (threadIdx.x < N * N)
@@ -424,8 +432,8 @@ constexpr int kFwdThreads = 1024;
 
 template<int N, class T>
 __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __restrict__ field,
-    const int* __restrict__ slots, int count, int ncomp, typename Cx<T>::type* __restrict__ spec, long fcs,
-    long scs)
+    const double* __restrict__ field_in, int split, const int* __restrict__ slots, int count, int ncomp,
+    typename Cx<T>::type* __restrict__ spec, long fcs, long scs)
 {
     using G = Geo<N>;
     constexpr int P = G::P, H = G::H, GS = DftPlan<N>::GS;
@@ -448,8 +456,9 @@ __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __res
     // with a barrier, so the LDS intermediates are free for the next item.
     for (int item = blockIdx.x; item < count * ncomp; item += gridDim.x)
     {
-        const int     c = item / count, slot = slots[item - c * count];
-        const double* in = field + c * fcs + (size_t)slot * G::N3;
+        const int     c = item / count, b = item - c * count, slot = slots[b];
+        const bool    ring0 = b >= split;  // read field_in, its boundary ring taken as zero
+        const double* in = (ring0 ? field_in : field) + c * fcs + (size_t)slot * G::N3;
         typename Cx<T>::type* out = spec + c * scs + (size_t)slot * G::NF;
         for (int kx0 = 0; kx0 < H; kx0 += GS)
         {
@@ -459,11 +468,14 @@ __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __res
                 const int     part = it / SX::lines, line = it - part * SX::lines;
                 const int     g0 = part * SX::per;
                 const double* src = in + line * N;
+                const int     lz = line / N, ly = line - lz * N;
+                const bool    skip = ring0 && (lz == 0 || ly == 0 || lz == N - 1 || ly == N - 1);
+                const int     xa = ring0 ? 1 : 0, xb = skip ? xa : (ring0 ? N - 1 : N);
                 double        sr[SX::per], si[SX::per];
 #pragma unroll
                 for (int o = 0; o < SX::per; ++o) sr[o] = si[o] = 0.0;
 #pragma unroll 2
-                for (int x = 0; x < N; ++x)
+                for (int x = xa; x < xb; ++x)
                 {
                     const double v = src[x];
 #pragma unroll
@@ -1080,10 +1092,10 @@ template<int N> struct RunAxpy
 template<int N> struct RunGather
 {
     static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m,
-        int accumulate, const Batch& b)
+        int accumulate, int ring0, const Batch& b)
     {
         hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count, b.k), dim3(kTransferThreads), 0, s, src, dst, e, count,
-            m, accumulate, b.fstride);
+            m, accumulate, ring0, b.fstride);
     }
 };
 template<int N> struct RunScatter
@@ -1129,18 +1141,18 @@ inline int resident_grid(F kernel, int threads, long items)
 
 template<int N> struct RunFwd
 {
-    static void run(hipStream_t s, int prec, const double* f, const int* slots, int count, void* spec,
-        const Batch& b)
+    static void run(hipStream_t s, int prec, const double* f, const double* fin, int split, const int* slots,
+        int count, void* spec, const Batch& b)
     {
         const long items = static_cast<long>(count) * b.k;
         if (prec == 0)
             hipLaunchKernelGGL((k_forward_dft<N, double>),
-                dim3(resident_grid(k_forward_dft<N, double>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, slots,
-                count, b.k, (double2*)spec, b.fstride, b.sstride);
+                dim3(resident_grid(k_forward_dft<N, double>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, fin,
+                split, slots, count, b.k, (double2*)spec, b.fstride, b.sstride);
         else
             hipLaunchKernelGGL((k_forward_dft<N, float>),
-                dim3(resident_grid(k_forward_dft<N, float>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, slots,
-                count, b.k, (float2*)spec, b.fstride, b.sstride);
+                dim3(resident_grid(k_forward_dft<N, float>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, fin,
+                split, slots, count, b.k, (float2*)spec, b.fstride, b.sstride);
     }
 };
 template<int N> struct RunInv
@@ -1242,9 +1254,10 @@ void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int
     if (count > 0) dispatch_n<RunAxpy>(n, s, dst, src, slots, count, a, b);
 }
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate, const Batch& b)
+    const double* mats, bool accumulate, const Batch& b, bool children_interior)
 {
-    if (count > 0) dispatch_n<RunGather>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, b);
+    if (count > 0)
+        dispatch_n<RunGather>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, children_interior ? 1 : 0, b);
 }
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
     const double* mats, bool accumulate, const Batch& b)
@@ -1262,9 +1275,12 @@ void correction_stencil(hipStream_t s, int n, const double* t, double* c, const 
     if (count > 0) dispatch_n<RunStencil>(n, s, t, c, slots, count, inv_dx2, omega2, b);
 }
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
-    const Batch& b)
+    const Batch& b, const double* field_in, int split)
 {
-    if (count > 0) dispatch_n<RunFwd>(n, s, prec, field, slots, count, spec, b);
+    if (split < 0) split = count;
+    if (split > count) throw std::invalid_argument("forward_dft: split outside the block list");
+    if (split < count && !field_in) throw std::invalid_argument("forward_dft: no field for the interior-read blocks");
+    if (count > 0) dispatch_n<RunFwd>(n, s, prec, field, field_in, split, slots, count, spec, b);
 }
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale, bool accumulate, const Batch& b)

@@ -75,8 +75,9 @@ void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int
 // dst[entry.parent] (=|+=) sum_c K(child) for entries {parent, child[8]}; K
 // built from the side matrices mats[2][n*n] (row = output point, col = input
 // point).
+// children_interior: the children's boundary rings are read as zero.
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate, const Batch& b = {});
+    const double* mats, bool accumulate, const Batch& b = {}, bool children_interior = false);
 // dst[entry.child] (=|+=) K(src[entry.parent]) for entries {child, parent, cidx}.
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
     const double* mats, bool accumulate, const Batch& b = {});
@@ -90,8 +91,10 @@ void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* 
 // prec 1: source spectra and kernel spectra stored FP32 (float2), read by the
 // MAC as FP32 products accumulated in FP64; target spectra, DFT arithmetic,
 // LDS intermediates and fields FP64.
+// Blocks slots[split..count) are read from field_in with their boundary ring
+// taken as zero, the others from field.
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
-    const Batch& b = {});
+    const Batch& b = {}, const double* field_in = nullptr, int split = -1);
 // Writes (accumulate: adds into) every point of each listed field block.
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale, bool accumulate, const Batch& b = {});
