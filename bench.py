@@ -8,14 +8,18 @@ components, i.e. three scalar LGF solves over the whole tree, all levels.
 Five ways are timed: the plain FP64 path one component at a time, the FP64
 and mixed paths with the three components batched through every launch, the
 mixed path one component at a time, and the batched mixed path replayed from
-one captured HIP graph per step.  The headline is the eager batched mixed
-path (its HIP events give the per-kernel times) when all its components are
-within relative L2 1e-6 of the plain FP64 result.  Inputs are resident in HBM before the timed region; kernel spectra
-are built during warm-up (the reference times its second, warm solve).
+one captured HIP graph per step.  These throughput runs use the solver's
+default of two streams per FMM pass (the coarse chain beside the finest
+level's work).  The headline is the eager batched mixed path when all its
+components are within relative L2 1e-6 of the plain FP64 result.  The
+per-kernel times and the roofline come from one-stream eager runs of the same
+steps, timed with HIP events on the solver stream inside their timed region
+(on two streams an event interval would include waiting for CUs).  Inputs are
+resident in HBM before the timed region; kernel spectra are built during
+warm-up (the reference times its second, warm solve).
 
 Prints one JSON line with the throughput (solves/s), the roofline of the
-dominant kernel class (timed live with HIP events on the solver's stream
-inside the timed region) and the CPU oracle timed on a bounded sample.
+dominant kernel class and the CPU oracle timed on a bounded sample.
 """
 import argparse
 import json
@@ -192,6 +196,8 @@ def main():
     ap.add_argument("--no-ib", action="store_true", help="skip the learned-FCG immersed-boundary comparison")
     ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph"],
                     help="time one way only and print its rate (for profiler runs)")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="with --only: run each FMM pass on one stream (the configuration the roofline is timed in)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -221,16 +227,22 @@ def main():
     def step_batched():
         solver.apply_lgf_batch(fin, fout)
 
-    def timed(precision, batched, graphs=False):
+    def timed(precision, batched, graphs=False, concurrent=True, timing=False):
+        """One timed configuration.  Throughput runs use the solver's default
+        two streams per FMM pass with kernel timing off; per-kernel HIP events
+        bracket kernel time only on one stream (a launch on the second stream
+        can wait for CUs held by the first), so the per-class tables and the
+        roofline come from one-stream runs (timing=True)."""
         step = step_batched if batched else step_single
         solver.set_precision(precision)
         solver.set_graphs(graphs)
+        solver.set_concurrency(concurrent)
         solver.prepare()
         for _ in range(args.warmup):
             step()
         solver.sync()
         ev0, ev1 = hip_events.Event(), hip_events.Event()
-        solver.timing(not graphs)  # per-class events need eager launches
+        solver.timing(timing and not graphs)  # per-class events need eager launches
         solver.timing_reset()
         solver.sync()
         t0 = time.perf_counter()
@@ -240,9 +252,10 @@ def main():
         ev1.record(stream)
         solver.sync()
         t1 = time.perf_counter()
-        kt = None if graphs else solver.timing_read()
+        kt = solver.timing_read() if timing and not graphs else None
         solver.timing(False)
         solver.set_graphs(False)
+        solver.set_concurrency(True)
         outs = [solver.download(f) for f in fout]
         return dict(wall=t1 - t0, gpu_ms=ev0.elapsed_ms(ev1), kt=kt, outs=outs)
 
@@ -251,19 +264,24 @@ def main():
 
     if args.only:
         prec = PREC_MIXED if args.only.startswith("mixed") else PREC_FP64
-        r = timed(prec, "batched" in args.only, args.only.endswith("graph"))
-        print(json.dumps({"only": args.only, "solves_per_s": 3 * args.steps / r["wall"],
+        r = timed(prec, "batched" in args.only, args.only.endswith("graph"), concurrent=not args.one_stream)
+        print(json.dumps({"only": args.only, "one_stream": args.one_stream, "solves_per_s": 3 * args.steps / r["wall"],
                           "ms_per_step": r["wall"] / args.steps * 1e3, "steps": args.steps}))
         return 0
 
     r64 = timed(PREC_FP64, False)
     log(f"plain FP64, one component at a time: {3 * args.steps / r64['wall']:.2f} solves/s")
+    r64t = timed(PREC_FP64, False, concurrent=False, timing=True)
+    log(f"plain FP64, one stream, kernels timed: {3 * args.steps / r64t['wall']:.2f} solves/s")
     r64b = timed(PREC_FP64, True)
     log(f"FP64, 3 components batched: {3 * args.steps / r64b['wall']:.2f} solves/s")
+    r64bt = timed(PREC_FP64, True, concurrent=False, timing=True)
     rmx = timed(PREC_MIXED, False)
     log(f"mixed, one component at a time: {3 * args.steps / rmx['wall']:.2f} solves/s")
     rmxb = timed(PREC_MIXED, True)
     log(f"mixed, 3 components batched: {3 * args.steps / rmxb['wall']:.2f} solves/s")
+    rmxbt = timed(PREC_MIXED, True, concurrent=False, timing=True)
+    log(f"mixed, 3 components batched, one stream, kernels timed: {3 * args.steps / rmxbt['wall']:.2f} solves/s")
     rmxg = timed(PREC_MIXED, True, graphs=True)
     log(f"mixed, 3 components batched, one HIP graph per step: {3 * args.steps / rmxg['wall']:.2f} solves/s")
     counts = solver.counts()
@@ -316,10 +334,12 @@ def main():
             "flops_per_launch": work * uf[dom] / max(launches, 1), "arithmetic": peak_name,
         }
 
-    k64, roof64 = roofline_of(r64["kt"], PEAK_FP64_TFLOPS, "fp64")
-    k64b, roof64b = roofline_of(r64b["kt"], PEAK_FP64_TFLOPS, "fp64_batched")
+    k64, roof64 = roofline_of(r64t["kt"], PEAK_FP64_TFLOPS, "fp64")
+    k64b, roof64b = roofline_of(r64bt["kt"], PEAK_FP64_TFLOPS, "fp64_batched")
     # the mixed MAC accumulates in FP64 and MI355X runs unpacked FP32 and FP64 FMA at one rate
-    kmxb, roofmxb = roofline_of(rmxb["kt"], PEAK_FP64_TFLOPS, "mixed_batched")
+    kmxb, roofmxb = roofline_of(rmxbt["kt"], PEAK_FP64_TFLOPS, "mixed_batched")
+    for roof in (roof64, roof64b, roofmxb):
+        roof["timed_in"] = "one-stream eager run of the same step (HIP events on the solver stream)"
     head = rmxb if mixed_ok else r64b
     value = solves / head["wall"]
     v64 = solves / r64["wall"]
@@ -363,7 +383,9 @@ def main():
             "mixed_solves_per_s": vmx,
             "mixed_batched_solves_per_s": vmxb,
             "mixed_batched_graph_solves_per_s": solves / rmxg["wall"],
+            "mixed_batched_one_stream_solves_per_s": solves / rmxbt["wall"],
             "graph_replay_bitwise_equal_to_eager": all(np.array_equal(a, b) for a, b in zip(rmxg["outs"], rmxb["outs"])),
+            "two_streams_bitwise_equal_to_one": all(np.array_equal(a, b) for a, b in zip(rmxbt["outs"], rmxb["outs"])),
             "batched_mixed_vs_plain_fp64_speedup": vmxb / v64,
             "batched_fp64_vs_plain_fp64_speedup": v64b / v64,
             "batched_fp64_bitwise_equal_to_single": batched_fp64_identical,
@@ -374,6 +396,7 @@ def main():
             "fp64_batched_kernels": k64b,
         },
         "roofline": roofmxb if mixed_ok else roof64b,
+        # per-class times of the one-stream timed run (the throughput run overlaps classes)
         "kernels": kmxb if mixed_ok else k64b,
         "cpu_baseline": None,
     }

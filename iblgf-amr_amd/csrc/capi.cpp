@@ -435,6 +435,12 @@ int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on)
     return guarded(ctx, [&] { ctx->solver->set_graphs(on != 0); });
 }
 
+int iblgf_gpu_set_concurrency(iblgf_gpu_ctx* ctx, int on)
+{
+    NEED_CTX("iblgf_gpu_set_concurrency");
+    return guarded(ctx, [&] { ctx->solver->set_concurrency(on != 0); });
+}
+
 int iblgf_gpu_set_precision(iblgf_gpu_ctx* ctx, int precision)
 {
     NEED_CTX("iblgf_gpu_set_precision");

@@ -126,6 +126,9 @@ GpuSolver::GpuSolver(int device) : device_(device)
     if (device < 0 || device >= count) throw std::runtime_error("invalid device index");
     IBLGF_HIP(hipSetDevice(device));
     IBLGF_HIP(hipStreamCreate(&stream_));
+    IBLGF_HIP(hipStreamCreate(&side_));
+    IBLGF_HIP(hipEventCreateWithFlags(&ev_fork_, hipEventDisableTiming));
+    IBLGF_HIP(hipEventCreateWithFlags(&ev_join_, hipEventDisableTiming));
 }
 
 GpuSolver::~GpuSolver()
@@ -135,6 +138,9 @@ GpuSolver::~GpuSolver()
     drop_graphs();
     passes_.clear();
     fields_.clear();
+    (void)hipEventDestroy(ev_fork_);
+    (void)hipEventDestroy(ev_join_);
+    (void)hipStreamDestroy(side_);
     (void)hipStreamDestroy(stream_);
 }
 
@@ -791,57 +797,101 @@ void GpuSolver::prepare_if(double alpha_base)
 // in the work arrays.
 void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, const Batch& b)
 {
-    double* fmm_s = work_.fmm_s.p;
-    double* fmm_t = work_.fmm_t.p;
-    timer_.begin(stream_);
-    launch::zero_blocks(stream_, n_, fmm_s, p.zero_s.p, p.zero_s.size(), b);
-    launch::zero_blocks(stream_, n_, fmm_t, p.zero_t.p, p.zero_t.size(), b);
-    timer_.end(stream_, KC_OTHER, 0);
-    // The level-L sources are read from the solve's source array in place,
-    // their boundary rings as zero (Fmm::apply copies interiors into zeroed
-    // blocks); the coarser ones are anterpolated into fmm_s.
-    if (!p.neighbor_only)
+    double*      fmm_s = work_.fmm_s.p;
+    double*      fmm_t = work_.fmm_t.p;
+    const int    L = p.L;
+    const double P = 2.0 * n_ - 1.0, iscale = scale / (P * P * P);
+    // Kernel spectra first: building one launches work on the solver stream.
+    std::vector<const void*> gh(p.mac.size());
+    for (size_t i = 0; i < p.mac.size(); ++i) gh[i] = ghat(laplace, p.mac[i].d, alpha);
+    // Two chains over disjoint blocks.  Fine (solver stream): the level-L
+    // sources' forward transform, read in place from the solve's source array
+    // with their rings as zero (Fmm::apply copies interiors into zeroed
+    // blocks), the level-L MAC and the inverse added into the level-L targets.
+    // Coarse (side stream, when on): clears, anterpolation into fmm_s, the
+    // coarser transforms and MACs, the inverse into fmm_t and the
+    // interpolation down to level L-1.  They join before the last
+    // interpolation, which adds into the level-L targets.  Issued in an order
+    // that is also valid on one stream.
+    const bool  coarse = !p.neighbor_only && L > 0;
+    const bool  two = coarse && concurrent_;
+    hipStream_t A = stream_, B = two ? side_ : stream_;
+    if (two)
     {
-        timer_.begin(stream_);
-        for (int l = p.L - 1; l >= 0; --l)
-        {
-            const bool top = l == p.L - 1;
-            launch::transfer_gather(stream_, n_, top ? work_.src.p : fmm_s, fmm_s, p.antrp[l].p,
-                p.antrp[l].size() / 9, m_nli_anterp_.p, false, b, top);
-        }
-        timer_.end(stream_, KC_TRANSFER, p.ntransfer_antrp * b.k);
+        IBLGF_HIP(hipEventRecord(ev_fork_, A));
+        IBLGF_HIP(hipStreamWaitEvent(B, ev_fork_, 0));
     }
-    timer_.begin(stream_);
-    launch::forward_dft(stream_, n_, precision_, fmm_s, p.fwd.p, p.fwd.size(), work_.sspec.p, b, work_.src.p,
-        p.nfwd_coarse);
-    timer_.end(stream_, KC_FORWARD, static_cast<long>(p.fwd.size()) * b.k);
-    for (auto& m : p.mac)
+    timer_.begin(B);
+    launch::zero_blocks(B, n_, fmm_s, p.zero_s.p, p.zero_s.size(), b);
+    launch::zero_blocks(B, n_, fmm_t, p.zero_t.p, p.zero_t.size(), b);
+    timer_.end(B, KC_OTHER, 0);
+    if (coarse)
     {
-        const void* gh = ghat(laplace, m.d, alpha);
-        timer_.begin(stream_);
-        launch::spectral_mac(stream_, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.rowoff.p, m.olist.p,
-            m.nused, m.compact, gh, work_.sspec.p, work_.tspec.p, b);
-        timer_.end(stream_, KC_MAC, m.npairs * b.k);
+        timer_.begin(B);
+        for (int l = L - 1; l >= 0; --l)
+        {
+            const bool top = l == L - 1;
+            launch::transfer_gather(B, n_, top ? work_.src.p : fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9,
+                m_nli_anterp_.p, false, b, top);
+        }
+        timer_.end(B, KC_TRANSFER, p.ntransfer_antrp * b.k);
+    }
+    const int nfine = static_cast<int>(p.fwd.size()) - p.nfwd_coarse;
+    timer_.begin(A);
+    launch::forward_dft(A, n_, precision_, fmm_s, p.fwd.p + p.nfwd_coarse, nfine, work_.sspec.p, b, work_.src.p, 0);
+    timer_.end(A, KC_FORWARD, static_cast<long>(nfine) * b.k);
+    timer_.begin(B);
+    launch::forward_dft(B, n_, precision_, fmm_s, p.fwd.p, p.nfwd_coarse, work_.sspec.p, b);
+    timer_.end(B, KC_FORWARD, static_cast<long>(p.nfwd_coarse) * b.k);
+    for (size_t i = 0; i < p.mac.size(); ++i)
+    {
+        const MacLevelPlan& m = p.mac[i];
+        hipStream_t         s = m.lambda == L ? A : B;
+        timer_.begin(s);
+        launch::spectral_mac(s, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.rowoff.p, m.olist.p,
+            m.nused, m.compact, gh[i], work_.sspec.p, work_.tspec.p, b);
+        timer_.end(s, KC_MAC, m.npairs * b.k);
     }
     // Level-L targets (first in inv) add into the target array, coarser ones
-    // are written to fmm_t and interpolated down, the last interpolation
-    // adding into the level-L targets.
-    const double P = 2.0 * n_ - 1.0;
-    const int    ntop = p.ninv_top, nrest = static_cast<int>(p.inv.size()) - ntop;
-    timer_.begin(stream_);
-    launch::inverse_dft(stream_, n_, precision_, work_.tspec.p, p.inv.p, ntop, work_.tgt.p, scale / (P * P * P),
-        true, b);
-    launch::inverse_dft(
-        stream_, n_, precision_, work_.tspec.p, p.inv.p + ntop, nrest, fmm_t, scale / (P * P * P), false, b);
-    timer_.end(stream_, KC_INVERSE, static_cast<long>(p.inv.size()) * b.k);
-    if (!p.neighbor_only)
+    // are written to fmm_t and interpolated down.
+    const int ntop = p.ninv_top, nrest = static_cast<int>(p.inv.size()) - ntop;
+    timer_.begin(A);
+    launch::inverse_dft(A, n_, precision_, work_.tspec.p, p.inv.p, ntop, work_.tgt.p, iscale, true, b);
+    timer_.end(A, KC_INVERSE, static_cast<long>(ntop) * b.k);
+    timer_.begin(B);
+    launch::inverse_dft(B, n_, precision_, work_.tspec.p, p.inv.p + ntop, nrest, fmm_t, iscale, false, b);
+    timer_.end(B, KC_INVERSE, static_cast<long>(nrest) * b.k);
+    if (coarse)
     {
-        timer_.begin(stream_);
-        for (int l = 1; l < p.L; ++l)
-            launch::transfer_scatter(stream_, n_, fmm_t, l + 1 == p.L ? work_.tgt.p : fmm_t, p.intrp[l].p,
-                p.intrp[l].size() / 3, m_nli_interp_.p, true, b);
-        timer_.end(stream_, KC_TRANSFER, (p.ntransfer - p.ntransfer_antrp) * b.k);
+        long w = 0;
+        timer_.begin(B);
+        for (int l = 1; l + 1 < L; ++l)
+        {
+            launch::transfer_scatter(B, n_, fmm_t, fmm_t, p.intrp[l].p, p.intrp[l].size() / 3, m_nli_interp_.p, true, b);
+            w += static_cast<long>(p.intrp[l].size() / 3);
+        }
+        timer_.end(B, KC_TRANSFER, w * b.k);
+        if (two)
+        {
+            IBLGF_HIP(hipEventRecord(ev_join_, B));
+            IBLGF_HIP(hipStreamWaitEvent(A, ev_join_, 0));
+        }
+        if (L >= 2)
+        {
+            timer_.begin(A);
+            launch::transfer_scatter(A, n_, fmm_t, work_.tgt.p, p.intrp[L - 1].p, p.intrp[L - 1].size() / 3,
+                m_nli_interp_.p, true, b);
+            timer_.end(A, KC_TRANSFER, static_cast<long>(p.intrp[L - 1].size() / 3) * b.k);
+        }
     }
+}
+
+void GpuSolver::set_concurrency(bool on)
+{
+    if (on == concurrent_) return;
+    sync();
+    drop_graphs();  // captured graphs hold the old launch order
+    concurrent_ = on;
 }
 
 void GpuSolver::check_batch(const int* src, const int* dst, int k)

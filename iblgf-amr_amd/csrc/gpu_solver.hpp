@@ -163,6 +163,9 @@ class GpuSolver
     // work arrays grow, the tree is re-uploaded or graphs are switched off.
     // Calls run eagerly while kernel timing is on.
     void set_graphs(bool on);
+    // Run each FMM pass's coarse chain on a second stream (default on); the
+    // results are bit for bit those of one stream.
+    void set_concurrency(bool on);
     bool graphs() const { return graphs_; }
     // Build the kernel spectra that a solve will use (cold-path work).
     void prepare_laplace();
@@ -233,6 +236,9 @@ class GpuSolver
 
     int         device_ = 0;
     hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;  // the coarse chain of an FMM pass
+    hipEvent_t  ev_fork_ = nullptr, ev_join_ = nullptr;
+    bool        concurrent_ = true;
     bool        uploaded_ = false;
     KernelTimer timer_;
 

@@ -87,6 +87,7 @@ SYMBOLS = {
     "iblgf_gpu_apply_lgf": (_I, [_P, _I, _I, _I, _D]),
     "iblgf_gpu_apply_lgf_batch": (_I, [_P, _IP, _IP, _I, _I, _D]),
     "iblgf_gpu_set_graphs": (_I, [_P, _I]),
+    "iblgf_gpu_set_concurrency": (_I, [_P, _I]),
     "iblgf_gpu_prepare": (_I, [_P, _I, _D]),
     "iblgf_gpu_set_precision": (_I, [_P, _I]),
     "iblgf_gpu_sync": (_I, [_P]),
@@ -351,6 +352,10 @@ class PoissonSolver:
     def set_graphs(self, on):
         """Replay repeated solves from captured HIP graphs (iblgf_gpu_set_graphs)."""
         _check(lib().iblgf_gpu_set_graphs(self._h, 1 if on else 0), self._h)
+
+    def set_concurrency(self, on):
+        """Run each FMM pass's coarse chain on a second stream (iblgf_gpu_set_concurrency)."""
+        _check(lib().iblgf_gpu_set_concurrency(self._h, 1 if on else 0), self._h)
 
     def sync(self):
         _check(lib().iblgf_gpu_sync(self._h), self._h)

@@ -191,6 +191,13 @@ int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const i
  * same solves.  Off by default. */
 int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on);
 
+/* on = 1: each FMM pass runs its coarse chain (anterpolation, coarser
+ * transforms and multiply-accumulates, interpolation above the finest level)
+ * on a second stream beside the finest level's work, joining before the last
+ * interpolation; on = 0: one stream.  Results are bit for bit the same.  No
+ * counterpart in the reference (it has no device path).  On by default. */
+int iblgf_gpu_set_concurrency(iblgf_gpu_ctx* ctx, int on);
+
 enum iblgf_precision
 {
     IBLGF_PREC_FP64 = 0,  /* plain path: spectra, DFT legs and MAC in FP64 */

@@ -216,6 +216,35 @@ def test_solve_independent_of_earlier_solves(kind):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("kind,prec", [("laplace_batch", 0), ("laplace_batch", 1), ("if", 0)])
+def test_two_streams_match_one_stream(kind, prec):
+    """Each FMM pass's coarse chain on a second stream gives bit for bit the
+    result of one stream (the chains touch disjoint blocks)."""
+    print(f"case: concurrency on/off, {kind}, precision {prec}", flush=True)
+    v = _small_rings()
+    srcs = [v.source(c) for c in range(3)]
+    solver = PoissonSolver(v.tree)
+    solver.set_precision(prec)
+    fin = [solver.field() for _ in srcs]
+    fout = [solver.field() for _ in srcs]
+    for f, x in zip(fin, srcs):
+        solver.upload(f, x)
+    out = {}
+    for on in (False, True, False, True):
+        solver.set_concurrency(on)
+        if kind == "if":
+            for a, b in zip(fin, fout):
+                solver.apply_lgf_IF(a, b, 0.9)
+        else:
+            solver.apply_lgf_batch(fin, fout)
+        got = [solver.download(f) for f in fout]
+        if on in out:
+            assert all(np.array_equal(a, b) for a, b in zip(got, out[on]))
+        out[on] = got
+    for a, b in zip(out[True], out[False]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_graph_replay_matches_eager(prec):
     """Batched solves replayed from a captured HIP graph equal the eager
