@@ -770,14 +770,29 @@ __device__ inline void mac_range(const MacSrc* __restrict__ srcs, const uint4* _
     const C* __restrict__ sp, const C* Gs, int f, int e, int end, MacAcc<C>& acc)
 {
     const int fbyte = f * (int)sizeof(C);
+    // FP32 spectra: the next batch's entries are loaded while this batch
+    // multiplies, so a batch waits for one global load (its sources) instead
+    // of two.  (The FP64 variant has no registers to spare for it.)
+    constexpr bool kAhead = sizeof(C) == 8;
+    MacSrc         nxt[8];
+    if (kAhead && e + 8 <= end)
+    {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) nxt[i] = srcs[e + i];
+    }
     for (; e + 8 <= end; e += 8)
     {
         MacSrc s[8];
         C      S[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) s[i] = srcs[e + i];
+        for (int i = 0; i < 8; ++i) s[i] = kAhead ? nxt[i] : srcs[e + i];
 #pragma unroll
         for (int i = 0; i < 8; ++i) S[i] = sp[(size_t)s[i].slot * NF];
+        if (kAhead && e + 16 <= end)
+        {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) nxt[i] = srcs[e + 8 + i];
+        }
 #pragma unroll
         for (int i = 0; i < 8; ++i)
         {
