@@ -16,7 +16,10 @@
 //                               include/iblgf/lgf/lgf.hpp:77-98, 221-262
 #include "kernels.hpp"
 
+#include <map>
+#include <mutex>
 #include <stdexcept>
+#include <tuple>
 
 namespace iblgf_amd {
 namespace {
@@ -1141,16 +1144,33 @@ template<int N> struct RunStencil
 };
 // Workgroups of a launch of `items` independent work items (a loop over
 // items strided by the grid inside the kernel): as many as the device holds
-// at once, at most one per item.
+// at once, at most one per item.  The device's count per (device, kernel,
+// workgroup size) is queried once and kept.
 template<class F>
 inline int resident_grid(F kernel, int threads, long items)
 {
-    int dev = 0, ncu = 0, occ = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess || ncu <= 0 || occ <= 0)
-        return static_cast<int>(items);
-    const long slots = static_cast<long>(ncu) * occ;
+    static std::mutex                                            lock;
+    static std::map<std::tuple<int, const void*, int>, long> slots_of;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return static_cast<int>(items);
+    const auto key = std::make_tuple(dev, reinterpret_cast<const void*>(kernel), threads);
+    long       slots = 0;
+    {
+        std::lock_guard<std::mutex> g(lock);
+        const auto                  it = slots_of.find(key);
+        if (it != slots_of.end()) slots = it->second;
+    }
+    if (slots == 0)
+    {
+        int ncu = 0, occ = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, threads, 0) != hipSuccess || ncu <= 0 ||
+            occ <= 0)
+            return static_cast<int>(items);
+        slots = static_cast<long>(ncu) * occ;
+        std::lock_guard<std::mutex> g(lock);
+        slots_of[key] = slots;
+    }
     return static_cast<int>(items < slots ? items : slots);
 }
 
