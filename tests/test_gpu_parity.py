@@ -216,6 +216,26 @@ def test_solve_independent_of_earlier_solves(kind):
         assert np.array_equal(a, b)
 
 
+def test_mac_plan_statistics_are_consistent():
+    """iblgf_gpu_mac_counts partitions the multiply-accumulate entries: full
+    entries plus partial ones by target count add up to all entries, their
+    pairs add up to the solve's pair count, and no entry carries more than 8."""
+    v = _small_rings()
+    solver = PoissonSolver(v.tree)
+    fs, fd = solver.field(), solver.field()
+    solver.upload(fs, v.source(0))
+    solver.apply_lgf(fs, fd)
+    solver.sync()
+    m, c = solver.mac_counts(), solver.counts()
+    print(f"mac plan: {m}", flush=True)
+    masked = m["masked_by_targets"]
+    assert m["pairs"] == c["n_pair"] > 0
+    assert m["full_entries"] + sum(masked) == m["entries"]
+    assert m["full_pairs"] + sum((t + 1) * n for t, n in enumerate(masked)) == m["pairs"]
+    assert m["full_entries"] <= m["full_pairs"] <= 8 * m["full_entries"]
+    assert m["entries"] <= m["pairs"] <= 8 * m["entries"]
+
+
 @pytest.mark.parametrize("kind,prec", [("laplace_batch", 0), ("laplace_batch", 1), ("if", 0)])
 def test_two_streams_match_one_stream(kind, prec):
     """Each FMM pass's coarse chain on a second stream gives bit for bit the
