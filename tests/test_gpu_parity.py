@@ -97,6 +97,32 @@ def test_adaptive_n8_matches_oracle(oracle_mod):
     assert l2 < REL_L2 and mx < REL_MAX
 
 
+@pytest.mark.parametrize("extent", [10, 14])
+def test_adaptive_block_sizes_match_oracle(oracle_mod, extent):
+    """The other supported block sizes (n = 12, 16): FP64 against the oracle,
+    and the batched mixed path within relative L2 1e-6 of it."""
+    print(f"case: adaptive rings, block extent {extent} (n={extent + 2})", flush=True)
+    v = _small_rings(block_extent=extent)
+    srcs = [v.source(c) for c in range(3)]
+    solver = PoissonSolver(v.tree)
+    fin = [solver.field() for _ in srcs]
+    fout = [solver.field() for _ in srcs]
+    for f, x in zip(fin, srcs):
+        solver.upload(f, x)
+    ot = oracle_mod.Tree(v.arrays, v.info, v.tree.n)
+    ref = ot.apply_lgf(srcs[1])
+    leaves = _leaf_mask(v.arrays)
+    solver.apply_lgf(fin[1], fout[1])
+    l2, mx = _rel(solver.download(fout[1])[leaves], ref[leaves])
+    print(f"fp64 relL2={l2:.3e} relmax={mx:.3e}", flush=True)
+    assert l2 < REL_L2 and mx < REL_MAX
+    solver.set_precision(1)
+    solver.apply_lgf_batch(fin, fout)
+    l2m, _ = _rel(solver.download(fout[1])[leaves], ref[leaves])
+    print(f"mixed batched relL2={l2m:.3e}", flush=True)
+    assert l2m < 1e-6
+
+
 @pytest.mark.parametrize("alpha", [0.1, 1.7])
 def test_integrating_factor_matches_oracle(oracle_mod, alpha):
     print(f"case: IF alpha={alpha}", flush=True)
