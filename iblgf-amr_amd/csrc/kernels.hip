@@ -367,6 +367,8 @@ struct Cx<float>
     using type = float2;
     static __device__ inline float2 make(float a, float b) { return make_float2(a, b); }
 };
+__device__ inline double2 widen(double2 v) { return v; }
+__device__ inline double2 widen(float2 v) { return make_double2(v.x, v.y); }
 
 // One DFT workgroup per CU takes nearly all of the 160 KB LDS: at n = 18 a
 // group of GS = 9 kx-planes with 768-1024 threads (12-16 waves) measured
@@ -551,11 +553,13 @@ __global__ __launch_bounds__(kFwdThreads) void k_forward_dft(const double* __res
 // registers; every leg keeps its outputs in registers, streams its inputs and
 // is split over the workgroup's kInvThreads threads as LegSplit (twice the
 // forward kernel's threads: the same LDS footprint then holds 12 waves per
-// CU).  Twiddles come from a table I[o][k] = w^((n-1+o) k).  FP64 throughout.
+// CU).  Twiddles come from a table I[o][k] = w^((n-1+o) k).  FP64 arithmetic;
+// the spectrum is read as T (double: plain FP64 path; float: mixed path) and
+// widened on load.
 constexpr int kInvThreads = 768;
 
-template<int N>
-__global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __restrict__ spec,
+template<int N, class T>
+__global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const typename Cx<T>::type* __restrict__ spec,
     const int* __restrict__ slots, int count, double* __restrict__ field, double scale, int accumulate, long scs,
     long fcs)
 {
@@ -572,7 +576,7 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
     const int b = blockIdx.x;
     if (b >= count) return;
     const int      slot = slots[b];
-    const double2* in = spec + blockIdx.y * scs + (size_t)slot * G::NF;
+    const typename Cx<T>::type* in = spec + blockIdx.y * scs + (size_t)slot * G::NF;
     for (int q = threadIdx.x; q < N * P; q += kInvThreads)
     {
         const int o = q / P, k = q - o * P;
@@ -596,14 +600,14 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
             const int      part = it / SZ::lines, line = it - part * SZ::lines;
             const int      g = line / P, ky = line - g * P, kx = kx0 + g;
             const int      z0 = part * SZ::per;
-            const double2* row = in + (size_t)(kx * P + ky) * P;
+            const typename Cx<T>::type* row = in + (size_t)(kx * P + ky) * P;
             double         sr[SZ::per], si[SZ::per];
 #pragma unroll
             for (int o = 0; o < SZ::per; ++o) sr[o] = si[o] = 0.0;
 #pragma unroll 7
             for (int kz = 0; kz < P; ++kz)
             {
-                const double2 a = row[kz];
+                const double2 a = widen(row[kz]);
 #pragma unroll
                 for (int o = 0; o < SZ::per; ++o)
                     if (z0 + o < N) cmac(a, Iw[(z0 + o) * P + kz], sr[o], si[o]);
@@ -681,7 +685,8 @@ __global__ __launch_bounds__(kInvThreads) void k_inverse_dft(const double2* __re
 // both paths.  FP64 spectra accumulate in FP64 directly.  FP32 spectra form
 // every product and the partial sums over at most 8 source entries in FP32;
 // each partial sum is then added into the FP64 accumulator (an FP32 running
-// sum over all of a target's sources measured twice the error).
+// sum over all of a target's sources measured twice the error).  The mixed
+// path stores the finished target spectrum rounded to FP32.
 template<class C> struct MacAcc;
 template<> struct MacAcc<double2>
 {
@@ -839,7 +844,7 @@ template<int N, int KF, bool COMPACT, class T>
 __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup* __restrict__ groups,
     int ngroups, const MacSrc* __restrict__ srcs, const uint4* __restrict__ rowoff, const int* __restrict__ olist,
     int nused, const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
-    double2* __restrict__ tspec, int k, long scs, long tcs)
+    typename Cx<T>::type* __restrict__ tspec, int k, long scs, long tcs)
 {
     using C = typename Cx<T>::type;
     static_assert(KF * sizeof(C) == kMacRowBytes, "row offsets assume 128-byte kernel rows");
@@ -890,7 +895,7 @@ __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup*
         {
             const int      g = vg / k, c = vg - g * k;
             const C*       sp = sspec + c * scs + fg;
-            double2*       tp = tspec + c * tcs + fg;
+            C*             tp = tspec + c * tcs + fg;
             const MacGroup* gp = groups + g;
             MacAcc<C>       acc;
             acc.zero();
@@ -910,7 +915,11 @@ __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup*
                 for (int t = 0; t < 8; ++t) ts[t] = gp->tslot[t];
 #pragma unroll
                 for (int t = 0; t < 8; ++t)
-                    if (ts[t] >= 0) tp[(size_t)ts[t] * NF] = acc.get(t);
+                    if (ts[t] >= 0)
+                    {
+                        const double2 v = acc.get(t);
+                        tp[(size_t)ts[t] * NF] = Cx<T>::make((T)v.x, (T)v.y);
+                    }
             }
         }
     }
@@ -1195,11 +1204,14 @@ template<int N> struct RunInv
     static void run(hipStream_t s, int prec, const void* spec, const int* slots, int count, double* f, double scale,
         int accumulate, const Batch& b)
     {
-        (void)prec;  // target spectra are FP64 on both paths
         // One workgroup per (block, component): a workgroup looping over
         // blocks, as the forward transform does, measured slower here.
-        hipLaunchKernelGGL((k_inverse_dft<N>), dim3(count, b.k), dim3(kInvThreads), 0, s, (const double2*)spec, slots,
-            count, f, scale, accumulate, b.sstride, b.fstride);
+        if (prec == 0)
+            hipLaunchKernelGGL((k_inverse_dft<N, double>), dim3(count, b.k), dim3(kInvThreads), 0, s,
+                (const double2*)spec, slots, count, f, scale, accumulate, b.sstride, b.fstride);
+        else
+            hipLaunchKernelGGL((k_inverse_dft<N, float>), dim3(count, b.k), dim3(kInvThreads), 0, s,
+                (const float2*)spec, slots, count, f, scale, accumulate, b.sstride, b.fstride);
     }
 };
 // Workgroup size of a MAC launch: every (group, component) pair of the launch
@@ -1231,7 +1243,7 @@ template<int N> struct RunMac
         const int nt = mac_threads((long)ng * b.k, KF);                                                          \
         const int nwg = mac_grid(k_spectral_mac<N, KF, CP, T>, nt, (Geo<N>::NF + KF - 1) / KF);                 \
         hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3(nwg), dim3(nt), 0, s, g, ng, src, rowoff, olist,  \
-            nused, (const C*)ghat, (const C*)ss, (double2*)ts, b.k, b.sstride, b.sstride);                      \
+            nused, (const C*)ghat, (const C*)ss, (C*)ts, b.k, b.sstride, b.sstride);                      \
     } while (0)
         if (prec == 0)
         {

@@ -88,9 +88,9 @@ void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* 
     int count, double inv_dx2, double omega2, const Batch& b = {});
 
 // prec 0: every spectrum FP64 (double2);
-// prec 1: source spectra and kernel spectra stored FP32 (float2), read by the
-// MAC as FP32 products accumulated in FP64; target spectra, DFT arithmetic,
-// LDS intermediates and fields FP64.
+// prec 1: source, kernel and target spectra stored FP32 (float2); the MAC
+// forms FP32 products, accumulates them in FP64 and rounds each target
+// spectrum once on store; DFT arithmetic, LDS intermediates and fields FP64.
 // Blocks slots[split..count) are read from field_in with their boundary ring
 // taken as zero, the others from field.
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
