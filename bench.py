@@ -367,7 +367,7 @@ def main():
                          "16 seeded Gaussian-core rings (seed 1001); one step = 3 vorticity components, "
                          "batched through every launch"),
             "precision": ("mixed: source and kernel spectra stored FP32, multiplied in FP32 and accumulated in "
-                          "FP64 by the MAC; DFT arithmetic, target spectra, fields and level sums FP64"
+                          "FP64 by the MAC, target spectra stored FP32; DFT arithmetic, fields and level sums FP64"
                           if mixed_ok else "plain FP64"),
             "blocks": wl.info["nblocks"],
             "leaves": wl.info["nleaves"],

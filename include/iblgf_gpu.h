@@ -201,9 +201,10 @@ int iblgf_gpu_set_concurrency(iblgf_gpu_ctx* ctx, int on);
 enum iblgf_precision
 {
     IBLGF_PREC_FP64 = 0,  /* plain path: spectra, DFT legs and MAC in FP64 */
-    IBLGF_PREC_MIXED = 1  /* source and kernel spectra stored FP32 (kernel spectra generated in
-                             FP64), MAC products in FP32 accumulated in FP64; DFT arithmetic,
-                             target spectra, fields and cross-level transfers FP64 */
+    IBLGF_PREC_MIXED = 1  /* source, kernel and target spectra stored FP32 (kernel spectra
+                             generated in FP64; target spectra accumulated in FP64 and rounded
+                             once on store), MAC products in FP32; DFT arithmetic, fields and
+                             cross-level transfers FP64 */
 };
 
 /* Arithmetic of the subsequent solves (no reference counterpart: the
