@@ -139,10 +139,16 @@ __device__ inline void line_apply(const double (&v)[N], const double* __restrict
     }
 }
 
-// x and y legs: global block -> LDS buffer (x), then y in place.
+// x and y legs: global block -> LDS buffer (x), then y in place.  `next`
+// (optional): a block the workgroup reads next; once this block's loads are
+// issued and consumed, each thread touches one 128-byte line of it, so its
+// lines are on their way while the y and z legs run on LDS.  The touched value
+// is held in `pend` and folded into `touch` only after the following block's
+// loads are issued, so no wait is placed on it before then.
 template<int N>
 __device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>& L, const double* __restrict__ Kx,
-    const double* __restrict__ Ky, bool ring0 = false)
+    const double* __restrict__ Ky, bool ring0 = false, const double* __restrict__ next = nullptr,
+    double* pend = nullptr, double* touch = nullptr)
 {
     static_assert(N * N <= kTransferThreads, "one line per thread");
     const int l = threadIdx.x;
@@ -154,6 +160,7 @@ __device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>
         double     v[N];
 #pragma unroll
         for (int a = 0; a < N; ++a) v[a] = in[l * N + a];  // loaded unconditionally, then masked (no branches)
+        if (touch) *touch += *pend;
         if (ring0)
         {
 #pragma unroll
@@ -161,6 +168,8 @@ __device__ inline void transfer_xy(const double* __restrict__ in, TransferLds<N>
         }
         line_apply<N>(v, Kx, L.buf, l * N, 1);
     }
+    constexpr int kLines = (Geo<N>::N3 * 8 + 127) / 128;
+    if (next && l < kLines) *pend = next[l * 16];
     __syncthreads();
     if (l < N * N)
     {
@@ -209,16 +218,21 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     double     acc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) acc[k] = 0.0;
-    bool first = true;
+    bool   first = true;
+    double touch = 0.0, pend = 0.0;
     for (int c = 0; c < 8; ++c)
     {
         const int ch = ent[1 + c];
         if (ch < 0) continue;  // uniform across the workgroup
         if (!first) __syncthreads();  // previous z leg done with the buffer
         first = false;
-        transfer_xy<N>(src + (size_t)ch * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N, ring0);
+        int nx = -1;  // the next listed child, touched ahead of its x leg
+        for (int d = c + 1; d < 8 && nx < 0; ++d) nx = ent[1 + d];
+        transfer_xy<N>(src + (size_t)ch * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N, ring0,
+            nx >= 0 ? src + (size_t)nx * N3 : nullptr, &pend, &touch);
         if (threadIdx.x < N * N) leg_z_acc<N>(L.buf, mats + ((c >> 2) & 1) * N * N, acc);
     }
+    asm volatile("" ::"v"(touch + pend));  // the touch loads are kept, their value unused
     if (threadIdx.x < N * N)
     {
         double* out = dst + (size_t)ent[0] * N3 + threadIdx.x;
