@@ -8,7 +8,7 @@ single-level case is additionally checked against the brute-force direct sum.
 import numpy as np
 import pytest
 
-from iblgf_amd import FLAG_LEAF, PoissonSolver, setups
+from iblgf_amd import FLAG_LEAF, KERNEL_IF, PoissonSolver, setups
 
 pytestmark = pytest.mark.gpu
 
@@ -291,14 +291,15 @@ def test_two_streams_match_one_stream(kind, prec):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
-def test_graph_replay_matches_eager(prec):
+@pytest.mark.parametrize("prec,kernel", [(0, "laplace"), (1, "laplace"), (0, "if")])
+def test_graph_replay_matches_eager(prec, kernel):
     """Batched solves replayed from a captured HIP graph equal the eager
     solves bit for bit: one capture, then three replays of the same graph,
     with eager solves of other fields, synchronisations and new source values
     between them, each compared with an eager solve of the same inputs (kernel
     timing on makes a call run eagerly without dropping the graph)."""
-    print(f"case: graph replay, 3 components, precision {prec}", flush=True)
+    print(f"case: graph replay, 3 components, precision {prec}, kernel {kernel}", flush=True)
+    kw = dict(kernel=KERNEL_IF, alpha_base=0.8) if kernel == "if" else {}
     v = _small_rings()
     rng = np.random.default_rng(31)
     solver = PoissonSolver(v.tree)
@@ -314,10 +315,10 @@ def test_graph_replay_matches_eager(prec):
         for f, a in zip(fs, srcs):
             solver.upload(f, a)
         solver.timing(True)                          # eager reference of the same inputs
-        solver.apply_lgf_batch(fs, fe)
+        solver.apply_lgf_batch(fs, fe, **kw)
         solver.timing(False)
         eager = [solver.download(f) for f in fe]
-        solver.apply_lgf_batch(fs, fg)               # rep 0 captures, later reps replay
+        solver.apply_lgf_batch(fs, fg, **kw)         # rep 0 captures, later reps replay
         solver.sync()
         solver.timing(True)                          # eager work between replays
         solver.upload(scratch_in, srcs[0])
