@@ -22,6 +22,7 @@ struct iblgf_tree
 {
     Octree     tree;
     bool       masks_built = false;
+    bool       subtract_non_leaf = false;  // the masks were built for it
     bool       has_ib = false;
     IbGeometry ib;
 };
@@ -175,6 +176,7 @@ int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf)
         t->tree.fmm_lgf_mask_build(subtract_non_leaf != 0);
         t->tree.fmm_ib_masks();
         t->masks_built = true;
+        t->subtract_non_leaf = subtract_non_leaf != 0;
     });
 }
 
@@ -353,6 +355,12 @@ const char* iblgf_gpu_last_error(const iblgf_gpu_ctx* ctx) { return ctx ? ctx->e
 
 int iblgf_gpu_tree_upload(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
 {
+    // Checked before the context: the refusal does not depend on the device.
+    if (t && t->masks_built && t->subtract_non_leaf)
+        return fail(IBLGF_ERR_UNSUPPORTED,
+            "iblgf_gpu_tree_upload: masks built with subtract_non_leaf = 1; the device solves implement the "
+            "reference default subtract_non_leaf = false (poisson.hpp:488-490, 670-716)",
+            ctx);
     NEED_CTX("iblgf_gpu_tree_upload");
     if (!t) return fail(IBLGF_ERR_ARG, "iblgf_gpu_tree_upload: null tree", ctx);
     if (!t->masks_built) return fail(IBLGF_ERR_STATE, "iblgf_gpu_tree_upload: masks not built", ctx);
@@ -571,6 +579,33 @@ int iblgf_gpu_apply_lgf_masked(iblgf_gpu_ctx* ctx, int src_field, int dst_field,
         else
             ctx->solver->apply_if_masked(src_field, dst_field, alpha_base, mask_type);
     });
+}
+
+int iblgf_gpu_apply_lgf_typed(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
+    int fmm_type)
+{
+    // The type is checked first, so an unbuilt type is refused the same way
+    // with or without a device.
+    switch (fmm_type)
+    {
+        case IBLGF_FMM_AMR2AMR:
+        case IBLGF_FMM_IB2xIB:
+        case IBLGF_FMM_xIB2IB:
+            break;
+        case IBLGF_FMM_STREAM:
+        case IBLGF_FMM_IB2AMR:
+        case IBLGF_FMM_LAPLACIAN_BC:
+        case IBLGF_FMM_REF_FOURIER_STREAM:
+        case IBLGF_FMM_REF_FOURIER_LGF:
+            return fail(IBLGF_ERR_UNSUPPORTED,
+                "iblgf_gpu_apply_lgf_typed: fmm_type " + std::to_string(fmm_type) +
+                    " has no device path (built: AMR2AMR, IB2xIB, xIB2IB)",
+                ctx);
+        default:
+            return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_typed: unknown fmm_type", ctx);
+    }
+    if (fmm_type == IBLGF_FMM_AMR2AMR) return iblgf_gpu_apply_lgf(ctx, src_field, dst_field, kernel, alpha_base);
+    return iblgf_gpu_apply_lgf_masked(ctx, src_field, dst_field, kernel, alpha_base, fmm_type);
 }
 
 int iblgf_gpu_ib_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t)

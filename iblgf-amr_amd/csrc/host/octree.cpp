@@ -505,85 +505,52 @@ void Octree::fmm_dry(int base, bool nl, bool subtract_non_leaf)
     fmm_clean_no_inf_masks(base, idx);
 }
 
-void Octree::fmm_dry_init_base_level_masks(
-    int base, bool nl, int idx, bool subtract_non_leaf, int load_factor, bool neighbor_only)
+// Source (which = 0) and target (which = 1) masks of one FMM pass at its own
+// level, for non_leaf_as_source = nl (rules of fmm.hpp:592-680).  A leaf or
+// correction block is "leaf-like"; the subtract_non_leaf variants are built
+// for export, the device solves refuse them.
+void Octree::fmm_dry_init_base_level_masks(int base, bool nl, int idx, bool subtract_non_leaf)
 {
     if (base >= (int)level_map.size()) return;
     for (int o : level_map[base])
     {
-        oct[o].set_mask(idx, 0, false);
-        oct[o].set_mask(idx, 1, false);
-    }
-    for (int o : level_map[base])
-    {
         Octant& it = oct[o];
-        if (!it.has_data) continue;
-        if (nl)
+        bool    src = false, tgt = false;
+        if (it.has_data)
         {
-            bool correction_parent = false;
-            for (int i = 0; i < 8; ++i)
+            const bool leaf_like = it.leaf || it.correction;
+            if (nl && subtract_non_leaf)
             {
-                const int c = it.children[i];
-                if (c >= 0 && oct[c].has_data && oct[c].correction) correction_parent = true;
+                bool correction_child = false;
+                for (int i = 0; i < 8; ++i)
+                {
+                    const int c = it.children[i];
+                    if (c >= 0 && oct[c].has_data && oct[c].correction) correction_child = true;
+                }
+                src = !leaf_like;
+                tgt = leaf_like ? correction_child : true;
             }
-            if (subtract_non_leaf)
+            else if (nl)
             {
-                if (it.leaf || it.correction)
-                {
-                    it.set_mask(idx, 0, false);
-                    it.set_mask(idx, 1, correction_parent);
-                }
-                else
-                {
-                    it.set_mask(idx, 0, true);
-                    it.set_mask(idx, 1, true);
-                    if (!neighbor_only) it.load += it.influence_num * load_factor;
-                    it.load += 27 * load_factor;
-                }
+                src = leaf_like;
+                tgt = leaf_like ? it.leaf : true;
+            }
+            else if (subtract_non_leaf)
+            {
+                src = tgt = true;
             }
             else
             {
-                if (it.leaf || it.correction)
-                {
-                    it.set_mask(idx, 0, true);
-                    it.set_mask(idx, 1, it.leaf);
-                }
-                else
-                {
-                    it.set_mask(idx, 0, false);
-                    it.set_mask(idx, 1, true);
-                    if (!neighbor_only) it.load += it.influence_num * load_factor;
-                    it.load += 27 * load_factor;
-                }
+                src = !leaf_like;
+                tgt = it.leaf;
             }
         }
-        else
-        {
-            if (subtract_non_leaf)
-            {
-                it.set_mask(idx, 0, true);
-                it.set_mask(idx, 1, true);
-            }
-            else
-            {
-                if (it.leaf || it.correction)
-                {
-                    it.set_mask(idx, 0, false);
-                    it.set_mask(idx, 1, it.leaf);
-                }
-                else
-                {
-                    it.set_mask(idx, 0, true);
-                    it.set_mask(idx, 1, false);
-                }
-            }
-            if (!neighbor_only) it.load += it.influence_num * load_factor;
-            it.load += 27 * load_factor;
-        }
+        it.set_mask(idx, 0, src);
+        it.set_mask(idx, 1, tgt);
     }
 }
 
-void Octree::fmm_upward_pass_masks(int base, int idx, double base_factor)
+void Octree::fmm_upward_pass_masks(int base, int idx)
 {
     for (int level = base - 1; level >= 0; --level)
     {
@@ -613,12 +580,6 @@ void Octree::fmm_upward_pass_masks(int base, int idx, double base_factor)
                     break;
                 }
             }
-            if (oct[o].mask(idx, 1))
-                for (int i = 0; i < oct[o].influence_num; ++i)
-                {
-                    const int inf = oct[o].influence[i];
-                    if (inf >= 0 && oct[inf].mask(idx, 0)) oct[o].load += 1.0 * base_factor;
-                }
         }
     }
 }

@@ -211,7 +211,6 @@ struct Octant
     int      influence_num = 0;
     uint64_t mask_src = 0;  // bit i: source mask of fmm mask index i
     uint64_t mask_tgt = 0;  // bit i: target mask of fmm mask index i
-    double   load = 0.0;
 
     bool mask(int idx, int which) const noexcept
     {
@@ -287,9 +286,8 @@ struct Octree
     // -- masks -------------------------------------------------------------
     void fmm_lgf_mask_build(bool subtract_non_leaf);
     void fmm_dry(int base, bool non_leaf_as_source, bool subtract_non_leaf);
-    void fmm_dry_init_base_level_masks(int base, bool nl, int idx, bool subtract_non_leaf,
-        int load_factor = 1, bool neighbor_only = false);
-    void fmm_upward_pass_masks(int base, int idx, double base_factor = 1.0);
+    void fmm_dry_init_base_level_masks(int base, bool nl, int idx, bool subtract_non_leaf);
+    void fmm_upward_pass_masks(int base, int idx);
     void fmm_clean_no_inf_masks(int base, int idx);
     void fmm_ib_masks();
 

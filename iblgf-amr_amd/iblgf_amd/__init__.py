@@ -42,6 +42,9 @@ CLEAN_LEAF_CORRECTION = 0
 CLEAN_IB_REGION = 1
 MASK_IB2xIB = 2
 MASK_xIB2IB = 3
+# the reference's MASK_TYPE (include/iblgf/domain/octree/octant.hpp:75-85)
+(FMM_STREAM, FMM_AMR2AMR, FMM_IB2xIB, FMM_xIB2IB, FMM_IB2AMR, FMM_LAPLACIAN_BC, FMM_REF_FOURIER_STREAM,
+ FMM_REF_FOURIER_LGF) = range(8)
 DELTA_YANG3, DELTA_YANG4, DELTA_ROMA = range(3)
 
 _lib = None
@@ -105,6 +108,7 @@ SYMBOLS = {
     "iblgf_gpu_level_gradient": (_I, [_P, _I, _IP, _I]),
     "iblgf_gpu_clean_boundary": (_I, [_P, _I, _I, _I, _I, _I]),
     "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
+    "iblgf_gpu_apply_lgf_typed": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_ib_create": (_I, [_P, _P]),
     "iblgf_gpu_ib_info": (_I, [_P, _IP, _IP, ctypes.POINTER(ctypes.c_long)]),
     "iblgf_gpu_ib_smear": (_I, [_P, _DP, _IP]),
@@ -412,6 +416,13 @@ class PoissonSolver:
     def clean_boundary(self, field, level, kind, leaf_only=True, width=2):
         _check(lib().iblgf_gpu_clean_boundary(self._h, int(field), int(level), int(kind), 1 if leaf_only else 0,
                                               int(width)), self._h)
+
+    def apply_lgf_typed(self, src, dst, fmm_type=FMM_AMR2AMR, kernel=KERNEL_LAPLACE, alpha_base=0.0):
+        """apply_lgf / apply_lgf_IF with the reference's fmm_type
+        (poisson.hpp:130-135, 427-433); types without a device path raise
+        IblgfError with code IBLGF_ERR_UNSUPPORTED (4)."""
+        _check(lib().iblgf_gpu_apply_lgf_typed(self._h, int(src), int(dst), int(kernel), float(alpha_base),
+                                               int(fmm_type)), self._h)
 
     def apply_lgf_masked(self, src, dst, mask_type, kernel=KERNEL_LAPLACE, alpha_base=0.0):
         """apply_lgf / apply_lgf_IF with fmm_type IB2xIB or xIB2IB (finest level, one pass)."""

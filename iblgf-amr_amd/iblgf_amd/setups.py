@@ -16,7 +16,7 @@ import math
 
 import numpy as np
 
-from . import FLAG_LEAF, Tree
+from . import FLAG_CORRECTION, FLAG_LEAF, Tree
 
 MASK64 = (1 << 64) - 1
 
@@ -157,20 +157,25 @@ class VortexRings:
         self.arrays = self.tree.export()
         self.info = self.tree.info()
 
+    @staticmethod
+    def _ring(ring, P):
+        c, R, a, axis, gam = ring
+        d = P - c
+        zax = d @ axis
+        perp = d - zax[..., None] * axis
+        rp = np.sqrt(np.sum(perp * perp, axis=-1))
+        rho2 = (rp - R) ** 2 + zax ** 2
+        amp = gam / (math.pi * a * a) * np.exp(-rho2 / (a * a))
+        et = np.cross(np.broadcast_to(axis, perp.shape), perp)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            et = np.where(rp[..., None] > 0, et / rp[..., None], 0.0)
+        return amp[..., None] * et
+
     def vorticity(self, P):
         """omega at points P[...,3] (physical units), returns [...,3]."""
         out = np.zeros(P.shape)
-        for c, R, a, axis, gam in self.rings:
-            d = P - c
-            zax = d @ axis
-            perp = d - zax[..., None] * axis
-            rp = np.sqrt(np.sum(perp * perp, axis=-1))
-            rho2 = (rp - R) ** 2 + zax ** 2
-            amp = gam / (math.pi * a * a) * np.exp(-rho2 / (a * a))
-            et = np.cross(np.broadcast_to(axis, perp.shape), perp)
-            with np.errstate(invalid="ignore", divide="ignore"):
-                et = np.where(rp[..., None] > 0, et / rp[..., None], 0.0)
-            out += amp[..., None] * et
+        for ring in self.rings:
+            out += self._ring(ring, P)
         return out
 
     def _refine(self, base, ext, rl, diff):
@@ -205,19 +210,157 @@ class VortexRings:
         w = self.vorticity(P)
         return bool(np.max(np.sqrt(np.sum(w * w, axis=-1))) > thr)
 
-    def source(self, component):
-        """Component `component` of the vorticity on the interior cells of every leaf."""
+    def sources(self, components=(0, 1, 2)):
+        """The listed vorticity components on the interior cells of every leaf,
+        [len(components), nblocks, n, n, n]; the vorticity is evaluated once
+        per leaf, over chunks of leaves at a time."""
         a = self.arrays
         n = self.tree.n
         nb = self.info["nblocks"]
-        f = np.zeros((nb, n, n, n))
-        inner = _interior_mask(n)
+        comps = list(components)
+        f = np.zeros((len(comps), nb, n, n, n))
         base_level = self.info["base_level"]
-        for s in np.nonzero(a["flags"] & FLAG_LEAF)[0]:
-            P = cell_centres(a, s, n, self.dx_base, base_level)
-            w = self.vorticity(P)[..., component]
-            f[s][inner] = w[inner]
+        leaves = np.nonzero(a["flags"] & FLAG_LEAF)[0]
+        idx = np.arange(1, n - 1) - 1
+        for c0 in range(0, len(leaves), 128):
+            chunk = leaves[c0:c0 + 128]
+            rl = np.maximum(a["level"][chunk].astype(np.int64) - base_level, 0)
+            dx = (self.dx_base / 2.0 ** rl)[:, None]
+            b = a["bbase"][chunk].astype(np.float64)
+            gx = (b[:, 0:1] + idx + 0.5) * dx
+            gy = (b[:, 1:2] + idx + 0.5) * dx
+            gz = (b[:, 2:3] + idx + 0.5) * dx
+            m = n - 2
+            P = np.empty((len(chunk), m, m, m, 3))
+            P[..., 0] = gx[:, None, None, :]
+            P[..., 1] = gy[:, None, :, None]
+            P[..., 2] = gz[:, :, None, None]
+            w = self.vorticity(P)
+            for i, c in enumerate(comps):
+                f[i][chunk, 1:-1, 1:-1, 1:-1] = w[..., c]
         return f
+
+    def source(self, component):
+        """Component `component` of the vorticity on the interior cells of every leaf."""
+        return self.sources((component,))[0]
+
+
+# ---------------------------------------------------------------------------
+# Manufactured vortex ring with a closed-form solution
+# ---------------------------------------------------------------------------
+class AnalyticVortexRing:
+    """The reference's Poisson acceptance test (tests/poisson/vortexrings.hpp,
+    configs tests/poisson/configs/config_1_lv_{0,1}).
+
+    psi = c1 exp(-c2 / (1 - t^2)) inside the torus t < 1, where
+    t = sqrt((r - R)^2 + z^2) / R and r = sqrt(x^2 + y^2) (vortexrings.hpp:132-142);
+    the source is its Laplacian, so the unbounded solve of Lap phi = source
+    should return psi to the scheme's order.  The Laplacian is written here
+    from the chain rule in cylindrical coordinates with s = t^2 and
+    f(s) = c1 exp(-c2 / (1 - s)):
+
+        Lap psi = f''(s) 4 s / R^2 + f'(s) (4 / R^2 + 2 (r - R) / (r R^2)),
+        f' = -c2 q^2 f,  f'' = (c2^2 q^4 - 2 c2 q^3) f,  q = 1 / (1 - s).
+
+    Domain of the configs: Lx = 2.5 over the bounding box (-96)^3 + 192^3
+    base cells, so dx_base = 2.5 / 192; 6^3-cell blocks over the central 96^3
+    cells; ring c1 = 1e6, c2 = 15, R = 0.125 at the box centre (the origin).
+    Cell centres sit at (i + 0.5) dx_level (vortexrings.hpp:372-412).  nlevels
+    refinement levels with the reference's rule: refine where |source| over
+    the block grown by two cells exceeds source_max * 0.125^diff_level, with
+    source_max = |source| on the ring's core circle (vortexrings.hpp:165-178,
+    471-520).  ``scale`` multiplies every block count a side (and divides
+    dx_base) for refinement studies at a fixed physical problem.
+    """
+
+    C1, C2, R = 1.0e6, 15.0, 0.125
+
+    def __init__(self, nlevels=0, scale=1, block_extent=6, half_box=96, region=96, Lx=2.5, refinement_factor=0.125):
+        self.block_extent = int(block_extent)
+        hb, reg = int(half_box * scale), int(region * scale)
+        self.dx_base = Lx / (2 * hb)
+        self.refinement_factor = refinement_factor
+        self.source_max = abs(float(self.laplacian(np.array([self.R]), np.array([0.0]), np.array([0.0]))[0]))
+        self.tree = Tree((-hb,) * 3, (2 * hb,) * 3, self.block_extent, self.dx_base,
+                         [(-reg // 2,) * 3 + (reg,) * 3])
+        self.tree.init_refine(int(nlevels), self._refine)
+        self.tree.build_masks()
+        self.arrays = self.tree.export()
+        self.info = self.tree.info()
+
+    @classmethod
+    def psi(cls, x, y, z):
+        r = np.sqrt(x * x + y * y)
+        s = ((r - cls.R) ** 2 + z * z) / (cls.R * cls.R)
+        out = np.zeros(np.shape(s))
+        m = s < 1.0
+        out[m] = cls.C1 * np.exp(-cls.C2 / (1.0 - s[m]))
+        return out
+
+    @classmethod
+    def laplacian(cls, x, y, z):
+        R, c2 = cls.R, cls.C2
+        r = np.sqrt(x * x + y * y)
+        s = ((r - R) ** 2 + z * z) / (R * R)
+        out = np.zeros(np.shape(s))
+        m = (s < 1.0) & (r > 0.0)
+        sm, rm = s[m], r[m]
+        q = 1.0 / (1.0 - sm)
+        f = cls.C1 * np.exp(-c2 * q)
+        f1 = -c2 * q * q * f
+        f2 = (c2 * c2 * q ** 4 - 2.0 * c2 * q ** 3) * f
+        out[m] = f2 * 4.0 * sm / (R * R) + f1 * (4.0 / (R * R) + 2.0 * (rm - R) / (rm * R * R))
+        return out
+
+    def _refine(self, base, ext, rl, diff):
+        dx = self.dx_base / (2 ** rl)
+        i = np.arange(-2, ext + 2)
+        gx = (base[0] + i + 0.5) * dx
+        gy = (base[1] + i + 0.5) * dx
+        gz = (base[2] + i + 0.5) * dx
+        reach = 2.0 * self.R
+        if gx[-1] < -reach or gx[0] > reach or gy[-1] < -reach or gy[0] > reach or gz[-1] < -self.R or gz[0] > self.R:
+            return False  # the source vanishes outside the torus |z| < R, r < 2R
+        Z, Y, X = np.meshgrid(gz, gy, gx, indexing="ij")
+        thr = self.source_max * self.refinement_factor ** diff
+        return bool(np.max(np.abs(self.laplacian(X, Y, Z))) > thr)
+
+    def _points(self, s):
+        return cell_centres(self.arrays, s, self.tree.n, self.dx_base, self.info["base_level"])
+
+    def source(self):
+        """The source on the interior cells of every leaf (vortexrings.hpp:384-410)."""
+        n = self.tree.n
+        f = np.zeros((self.info["nblocks"], n, n, n))
+        inner = _interior_mask(n)
+        for s in np.nonzero(self.arrays["flags"] & FLAG_LEAF)[0]:
+            P = self._points(s)
+            f[s][inner] = self.laplacian(P[..., 0], P[..., 1], P[..., 2])[inner]
+        return f
+
+    def exact(self):
+        """psi at every point of every block."""
+        n = self.tree.n
+        f = np.zeros((self.info["nblocks"], n, n, n))
+        for s in range(self.info["nblocks"]):
+            P = self._points(s)
+            f[s] = self.psi(P[..., 0], P[..., 1], P[..., 2])
+        return f
+
+    def finest_blocks(self):
+        """Slots of the non-correction leaves at the finest level: where the
+        reference takes its L-infinity error (compute_errors skips correction
+        blocks, setups/setup_base.hpp:207-224; finest_level_Linf_error_phi_,
+        vortexrings.hpp:245-268)."""
+        a = self.arrays
+        L = self.info["depth"] - 1
+        return np.nonzero((a["level"] == L) & ((a["flags"] & FLAG_LEAF) != 0) & ((a["flags"] & FLAG_CORRECTION) == 0))[0]
+
+    def linf_error(self, phi):
+        """max |phi - psi| over the interiors of finest_blocks()."""
+        sel = self.finest_blocks()
+        ex = np.stack([self.psi(*np.moveaxis(self._points(s), -1, 0)) for s in sel])
+        return float(np.max(np.abs(phi[sel][:, 1:-1, 1:-1, 1:-1] - ex[:, 1:-1, 1:-1, 1:-1])))
 
 
 # ---------------------------------------------------------------------------

@@ -71,7 +71,10 @@ int iblgf_tree_init_refine(iblgf_tree* t, int nref, int level_up_max, iblgf_refi
 /* FmmMaskBuilder::fmm_lgf_mask_build (include/iblgf/fmm/fmm.hpp:392-400),
  * and the IB2xIB / xIB2IB masks (fmm.hpp:70-144).  With immersed-boundary
  * points set, the is_ib / extended_ib flags are computed first
- * (Server::update_ib_flag, include/iblgf/domain/decomposition/server.hpp:544-575). */
+ * (Server::update_ib_flag, include/iblgf/domain/decomposition/server.hpp:544-575).
+ * Masks built with subtract_non_leaf = 1 can be exported, but
+ * iblgf_gpu_tree_upload refuses such a tree with IBLGF_ERR_UNSUPPORTED: the
+ * device solves implement the reference default subtract_non_leaf = false. */
 int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf);
 
 int iblgf_tree_info(const iblgf_tree* t, int* nblocks, int* base_level, int* depth, int* block_extent,
@@ -289,6 +292,30 @@ enum iblgf_mask_type
 };
 int iblgf_gpu_apply_lgf_masked(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
     int mask_type);
+
+/* The fmm_type argument of PoissonSolver::apply_lgf<Src,Tgt>(int fmm_type) and
+ * apply_lgf_IF<Src,Tgt>(double alpha, int fmm_type)
+ * (include/iblgf/solver/poisson/poisson.hpp:130-135, 427-433, 587-653): the
+ * reference's MASK_TYPE values (include/iblgf/domain/octree/octant.hpp:75-85). */
+enum iblgf_fmm_type
+{
+    IBLGF_FMM_STREAM = 0,
+    IBLGF_FMM_AMR2AMR = 1,
+    IBLGF_FMM_IB2xIB = 2,
+    IBLGF_FMM_xIB2IB = 3,
+    IBLGF_FMM_IB2AMR = 4,
+    IBLGF_FMM_LAPLACIAN_BC = 5,
+    IBLGF_FMM_REF_FOURIER_STREAM = 6,
+    IBLGF_FMM_REF_FOURIER_LGF = 7
+};
+/* apply_lgf / apply_lgf_IF with the reference's fmm_type: AMR2AMR runs
+ * iblgf_gpu_apply_lgf, IB2xIB and xIB2IB run iblgf_gpu_apply_lgf_masked;
+ * STREAM, IB2AMR, Laplacian_BC and the Fourier-mode types return
+ * IBLGF_ERR_UNSUPPORTED (checked before anything else, so a caller can route
+ * them to its own path), any other value IBLGF_ERR_ARG.  Call sites:
+ * ifherk.hpp:1471, 1555; linsys.hpp:142, 277, 332. */
+int iblgf_gpu_apply_lgf_typed(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
+    int fmm_type);
 
 /* ---- immersed-boundary force system (device) ---------------------------- */
 

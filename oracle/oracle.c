@@ -511,39 +511,62 @@ static void cache_clear(spec_cache* c)
     c->n = 0;
 }
 
-static const spec_entry* lgf_spectrum(spec_cache* c, const dft_t* d, const kernel_t* k, int ld,
-    const int base[3])
+static spec_entry* cache_find(spec_cache* c, int ld, const int base[3])
 {
     int i;
     for (i = 0; i < c->n; ++i)
         if (c->e[i].d == ld && c->e[i].bx == base[0] && c->e[i].by == base[1] && c->e[i].bz == base[2])
             return &c->e[i];
+    return NULL;
+}
+
+/* a new entry with no spectrum yet */
+static spec_entry* cache_add(spec_cache* c, int ld, const int base[3])
+{
+    spec_entry* s;
     if (c->n == c->cap)
     {
         c->cap = c->cap ? 2 * c->cap : 64;
         c->e = (spec_entry*)realloc(c->e, sizeof(spec_entry) * c->cap);
     }
-    {
-        spec_entry* s = &c->e[c->n++];
-        const int   P = d->P, step = 1 << ld;
-        size_t      P3 = (size_t)P * P * P;
-        double*     blk = (double*)malloc(sizeof(double) * P3);
-        int         x, y, z;
-        s->d = ld;
-        s->bx = base[0];
-        s->by = base[1];
-        s->bz = base[2];
-        s->re = (double*)malloc(sizeof(double) * spec_size(d));
-        s->im = (double*)malloc(sizeof(double) * spec_size(d));
-        for (z = 0; z < P; ++z)
-            for (y = 0; y < P; ++y)
-                for (x = 0; x < P; ++x)
-                    blk[((size_t)z * P + y) * P + x] = kernel_value(
-                        k, (base[0] + x) * step, (base[1] + y) * step, (base[2] + z) * step, NULL, 0);
-        dft_forward(d, blk, P, s->re, s->im);
-        free(blk);
-        return s;
-    }
+    s = &c->e[c->n++];
+    s->d = ld;
+    s->bx = base[0];
+    s->by = base[1];
+    s->bz = base[2];
+    s->re = s->im = NULL;
+    return s;
+}
+
+/* LGF_Base::dft (lgf.hpp:77-98, 221-262): the kernel sampled at step 2^d on
+ * the (2n-1)^3 LGF block, then r2c */
+static void spec_fill(spec_entry* s, const dft_t* d, const kernel_t* k)
+{
+    const int P = d->P, step = 1 << s->d;
+    size_t    P3 = (size_t)P * P * P;
+    double*   blk = (double*)malloc(sizeof(double) * P3);
+    int       x, y, z;
+    double*   re = (double*)malloc(sizeof(double) * spec_size(d));
+    double*   im = (double*)malloc(sizeof(double) * spec_size(d));
+    for (z = 0; z < P; ++z)
+        for (y = 0; y < P; ++y)
+            for (x = 0; x < P; ++x)
+                blk[((size_t)z * P + y) * P + x] = kernel_value(
+                    k, (s->bx + x) * step, (s->by + y) * step, (s->bz + z) * step, NULL, 0);
+    dft_forward(d, blk, P, re, im);
+    free(blk);
+    s->re = re;
+    s->im = im;
+}
+
+static const spec_entry* lgf_spectrum(spec_cache* c, const dft_t* d, const kernel_t* k, int ld,
+    const int base[3])
+{
+    spec_entry* s = cache_find(c, ld, base);
+    if (s && s->re) return s;
+    if (!s) s = cache_add(c, ld, base);
+    spec_fill(s, d, k);
+    return s;
 }
 
 void oracle_block_conv(const double* g, int nmax, int n, const double* src, const int shift[3],
@@ -766,6 +789,50 @@ static long influence_field(ostate* o, int tg, int L, int idx, int neighbor_only
     return count;
 }
 
+/* Every kernel spectrum (and, when cached, every source spectrum) that the
+ * targets tg[0..nt) will read, made ahead of the concurrent Bx loop. */
+static void prepare_spectra(ostate* o, const int* tg, int nt, int L, int idx, int neighbor_only)
+{
+    const oracle_tree* t = o->t;
+    int                i, j, first = o->cache.n, nnew, src;
+    char*              need = (char*)calloc(t->nb, 1);
+    int*               list = (int*)malloc(sizeof(int) * (size_t)(t->nb + 1));
+    int                nl = 0;
+    for (i = 0; i < nt; ++i)
+    {
+        const int s = tg[i], lvl = t->level[s];
+        for (j = 0; j < 27 + t->influence_num[s]; ++j)
+        {
+            int base[3], d;
+            if (j < 27) src = lvl == L ? t->neighbor[27 * s + j] : -1;
+            else src = neighbor_only ? -1 : t->influence[189 * s + j - 27];
+            if (src < 0 || !MS(src, idx)) continue;
+            for (d = 0; d < 3; ++d) base[d] = (t->bbase[3 * s + d] - t->bbase[3 * src + d]) - (o->n - 1);
+            if (!cache_find(&o->cache, j < 27 ? 0 : L - lvl, base)) cache_add(&o->cache, j < 27 ? 0 : L - lvl, base);
+            if (!o->faithful && !o->src_re[src] && !need[src])
+            {
+                need[src] = 1;
+                list[nl++] = src;
+            }
+        }
+    }
+    nnew = o->cache.n - first;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (i = 0; i < nnew; ++i) spec_fill(&o->cache.e[first + i], &o->dft, &o->ker);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (i = 0; i < nl; ++i)
+    {
+        const size_t S = spec_size(&o->dft);
+        double*      re = (double*)malloc(sizeof(double) * S);
+        double*      im = (double*)malloc(sizeof(double) * S);
+        dft_forward(&o->dft, BLK(o->fmm_s, list[i]), o->n, re, im);
+        o->src_re[list[i]] = re;
+        o->src_im[list[i]] = im;
+    }
+    free(need);
+    free(list);
+}
+
 /* Fmm::apply (fmm.hpp:729-870) with fmm mask index idx */
 static void fmm_apply_idx(ostate* o, const double* Source, double* Target, int L, int idx, double add_scale)
 {
@@ -815,13 +882,22 @@ static void fmm_apply_idx(ostate* o, const double* Source, double* Target, int L
                 }
             }
     }
-    /* Bx over targets at levels L..0 (L only for neighbour-only kernels) */
-    for (l = L; l >= (neighbor_only ? L : 0); --l)
-        for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
-        {
-            if (!MT(s, idx)) continue;
-            influence_field(o, s, L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
-        }
+    /* Bx over targets at levels L..0 (L only for neighbour-only kernels).  The
+     * kernel and source spectra the targets read are made first; then targets
+     * run concurrently, each writing only its own fmm_t block and summing its
+     * sources in the serial order, so results do not depend on the threads. */
+    {
+        int* tg = (int*)malloc(sizeof(int) * (size_t)(t->nb + 1));
+        int  nt = 0, it;
+        for (l = L; l >= (neighbor_only ? L : 0); --l)
+            for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+                if (MT(s, idx)) tg[nt++] = s;
+        prepare_spectra(o, tg, nt, L, idx, neighbor_only);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (it = 0; it < nt; ++it)
+            influence_field(o, tg[it], L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
+        free(tg);
+    }
     if (!neighbor_only)
     {
         /* interpolation (fmm.hpp:1108-1126) */

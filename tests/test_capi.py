@@ -75,3 +75,38 @@ def test_tree_export_shapes():
     nb = t.info()["nblocks"]
     assert a["neighbor"].shape == (nb, 27) and a["influence"].shape == (nb, 189)
     assert np.all(a["key"] & np.uint64(1) == 1)
+
+
+def build_c_caller(tmp_path):
+    """tests/capi_check.c compiled as C99 against include/iblgf_gpu.h and
+    linked to the built library."""
+    import subprocess
+
+    exe = str(tmp_path / "capi_check")
+    libdir = os.path.dirname(P.LIB_PATH)
+    cmd = ["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+           os.path.join(ROOT, "tests", "capi_check.c"), "-o", exe, "-L", libdir, "-liblgf_gpu",
+           f"-Wl,-rpath,{libdir}"]
+    subprocess.run(cmd, check=True)
+    return exe
+
+
+def test_c_caller_status_codes(tmp_path):
+    """A C caller sees ERR_UNSUPPORTED for the fmm types without a device
+    path (STREAM, IB2AMR, Laplacian_BC, Fourier modes) and for a tree whose
+    masks were built with subtract_non_leaf = 1, ERR_ARG otherwise."""
+    import subprocess
+
+    r = subprocess.run([build_c_caller(tmp_path), "cpu"], capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout
+
+
+def test_subtract_non_leaf_refused_from_python():
+    t = P.Tree((-16, -16, -16), (32, 32, 32), 8, 1 / 32, [(-16, -16, -16, 32, 32, 32)]).init_refine(0)
+    t.build_masks(subtract_non_leaf=True)
+    assert t.export()["mask_src"].any()  # the masks themselves are built and exported
+    L = P.lib()
+    assert L.iblgf_gpu_tree_upload(None, t._h) == 4  # IBLGF_ERR_UNSUPPORTED
+    assert b"subtract_non_leaf" in L.iblgf_last_error()

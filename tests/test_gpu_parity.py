@@ -97,10 +97,11 @@ def test_adaptive_n8_matches_oracle(oracle_mod):
     assert l2 < REL_L2 and mx < REL_MAX
 
 
-@pytest.mark.parametrize("extent", [10, 14])
+@pytest.mark.parametrize("extent", [10, 14, 16])
 def test_adaptive_block_sizes_match_oracle(oracle_mod, extent):
-    """The other supported block sizes (n = 12, 16): FP64 against the oracle,
-    and the batched mixed path within relative L2 1e-6 of it."""
+    """The other supported block sizes (n = 12, 16, 18; 18 is the adaptive
+    vortex-ring workload's): FP64 against the oracle, and the batched mixed
+    path within relative L2 1e-6 of it on every component."""
     print(f"case: adaptive rings, block extent {extent} (n={extent + 2})", flush=True)
     v = _small_rings(block_extent=extent)
     srcs = [v.source(c) for c in range(3)]
@@ -118,15 +119,17 @@ def test_adaptive_block_sizes_match_oracle(oracle_mod, extent):
     assert l2 < REL_L2 and mx < REL_MAX
     solver.set_precision(1)
     solver.apply_lgf_batch(fin, fout)
-    l2m, _ = _rel(solver.download(fout[1])[leaves], ref[leaves])
-    print(f"mixed batched relL2={l2m:.3e}", flush=True)
-    assert l2m < 1e-6
+    for c in range(3):
+        refc = ref if c == 1 else ot.apply_lgf(srcs[c])
+        l2m, mxm = _rel(solver.download(fout[c])[leaves], refc[leaves])
+        print(f"mixed batched component {c}: relL2={l2m:.3e} relmax={mxm:.3e}", flush=True)
+        assert l2m < 1e-6
 
 
-@pytest.mark.parametrize("alpha", [0.1, 1.7])
-def test_integrating_factor_matches_oracle(oracle_mod, alpha):
-    print(f"case: IF alpha={alpha}", flush=True)
-    v = _small_rings()
+@pytest.mark.parametrize("alpha,extent", [(0.1, 8), (1.7, 8), (0.1, 16), (1.7, 16)])
+def test_integrating_factor_matches_oracle(oracle_mod, alpha, extent):
+    print(f"case: IF alpha={alpha}, block extent {extent}", flush=True)
+    v = _small_rings(block_extent=extent)
     src = v.source(0)
     solver = PoissonSolver(v.tree)
     fs, fd = solver.field(), solver.field()
