@@ -383,6 +383,7 @@ void GpuSolver::reserve_components(int k)
     IBLGF_HIP(hipMemset(w.tspec.p, 0, ss * sizeof(double2)));
     w.k = k;
     work_ = std::move(w);
+    wp_ = WorkPtrs{work_.src.p, work_.tgt.p, work_.fmm_s.p, work_.fmm_t.p, work_.sspec.p, work_.tspec.p};
 }
 
 Batch GpuSolver::batch(int k) const
@@ -768,7 +769,8 @@ const void* GpuSolver::ghat(bool laplace, int d, double alpha)
 
 void GpuSolver::set_precision(int p)
 {
-    if (p != 0 && p != 1) throw std::invalid_argument("precision must be 0 (FP64) or 1 (mixed FP32/FP64)");
+    if (p < kPrecFp64 || p > kPrecI8x4)
+        throw std::invalid_argument("precision must be 0 (FP64), 1 (mixed FP32/FP64) or 2-5 (matrix-core DFT legs)");
     precision_ = p;
 }
 
@@ -797,8 +799,8 @@ void GpuSolver::prepare_if(double alpha_base)
 // in the work arrays.
 void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, const Batch& b)
 {
-    double*      fmm_s = work_.fmm_s.p;
-    double*      fmm_t = work_.fmm_t.p;
+    double*      fmm_s = wp_.fmm_s;
+    double*      fmm_t = wp_.fmm_t;
     const int    L = p.L;
     const double P = 2.0 * n_ - 1.0, iscale = scale / (P * P * P);
     // Kernel spectra first: building one launches work on the solver stream.
@@ -831,17 +833,17 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
         for (int l = L - 1; l >= 0; --l)
         {
             const bool top = l == L - 1;
-            launch::transfer_gather(B, n_, top ? work_.src.p : fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9,
+            launch::transfer_gather(B, n_, top ? wp_.src : fmm_s, fmm_s, p.antrp[l].p, p.antrp[l].size() / 9,
                 m_nli_anterp_.p, false, b, top);
         }
         timer_.end(B, KC_TRANSFER, p.ntransfer_antrp * b.k);
     }
     const int nfine = static_cast<int>(p.fwd.size()) - p.nfwd_coarse;
     timer_.begin(A);
-    launch::forward_dft(A, n_, precision_, fmm_s, p.fwd.p + p.nfwd_coarse, nfine, work_.sspec.p, b, work_.src.p, 0);
+    launch::forward_dft(A, n_, precision_, fmm_s, p.fwd.p + p.nfwd_coarse, nfine, wp_.sspec, b, wp_.src, 0);
     timer_.end(A, KC_FORWARD, static_cast<long>(nfine) * b.k);
     timer_.begin(B);
-    launch::forward_dft(B, n_, precision_, fmm_s, p.fwd.p, p.nfwd_coarse, work_.sspec.p, b);
+    launch::forward_dft(B, n_, precision_, fmm_s, p.fwd.p, p.nfwd_coarse, wp_.sspec, b);
     timer_.end(B, KC_FORWARD, static_cast<long>(p.nfwd_coarse) * b.k);
     for (size_t i = 0; i < p.mac.size(); ++i)
     {
@@ -849,17 +851,17 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
         hipStream_t         s = m.lambda == L ? A : B;
         timer_.begin(s);
         launch::spectral_mac(s, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.rowoff.p, m.olist.p,
-            m.nused, m.compact, gh[i], work_.sspec.p, work_.tspec.p, b);
+            m.nused, m.compact, gh[i], wp_.sspec, wp_.tspec, b);
         timer_.end(s, KC_MAC, m.npairs * b.k);
     }
     // Level-L targets (first in inv) add into the target array, coarser ones
     // are written to fmm_t and interpolated down.
     const int ntop = p.ninv_top, nrest = static_cast<int>(p.inv.size()) - ntop;
     timer_.begin(A);
-    launch::inverse_dft(A, n_, precision_, work_.tspec.p, p.inv.p, ntop, work_.tgt.p, iscale, true, b);
+    launch::inverse_dft(A, n_, precision_, wp_.tspec, p.inv.p, ntop, wp_.tgt, iscale, true, b);
     timer_.end(A, KC_INVERSE, static_cast<long>(ntop) * b.k);
     timer_.begin(B);
-    launch::inverse_dft(B, n_, precision_, work_.tspec.p, p.inv.p + ntop, nrest, fmm_t, iscale, false, b);
+    launch::inverse_dft(B, n_, precision_, wp_.tspec, p.inv.p + ntop, nrest, fmm_t, iscale, false, b);
     timer_.end(B, KC_INVERSE, static_cast<long>(nrest) * b.k);
     if (coarse)
     {
@@ -879,7 +881,7 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
         if (L >= 2)
         {
             timer_.begin(A);
-            launch::transfer_scatter(A, n_, fmm_t, work_.tgt.p, p.intrp[L - 1].p, p.intrp[L - 1].size() / 3,
+            launch::transfer_scatter(A, n_, fmm_t, wp_.tgt, p.intrp[L - 1].p, p.intrp[L - 1].size() / 3,
                 m_nli_interp_.p, true, b);
             timer_.end(A, KC_TRANSFER, static_cast<long>(p.intrp[L - 1].size() / 3) * b.k);
         }
@@ -987,8 +989,8 @@ void GpuSolver::apply_lgf_batch(const int* src, const int* dst, int k)
 void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
 {
     const Batch b = batch(k);
-    double* st = work_.src.p;
-    double* tt = work_.tgt.p;
+    double* st = wp_.src;
+    double* tt = wp_.tgt;
     timer_.begin(stream_);
     // Source: leaves copied whole, the other blocks at levels >= base cleared;
     // the coarsening then adds each parent's restricted children into it (a
@@ -1053,8 +1055,8 @@ void GpuSolver::apply_if_batch(const int* src, const int* dst, int k, double alp
 void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alpha_base)
 {
     const Batch b = batch(k);
-    double* st = work_.src.p;
-    double* tt = work_.tgt.p;
+    double* st = wp_.src;
+    double* tt = wp_.tgt;
     launch::zero_blocks(stream_, n_, st, p_nonleaf_.p, p_nonleaf_.size(), b);
     fill_zero(tt, b);
     launch::copy_blocks_table(
@@ -1089,8 +1091,8 @@ void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
     check_batch(&src, &dst, 1);
     const Batch b = batch(1);
     const int L = depth_ - 1;
-    double*   st = work_.src.p;
-    double*   tt = work_.tgt.p;
+    double*   st = wp_.src;
+    double*   tt = wp_.tgt;
     // The pass reads the source's level-L interiors only and adds into the
     // level-L targets only.
     launch::zero_blocks(stream_, n_, tt, p_level_slots_[L].p, p_level_slots_[L].size(), b);
@@ -1113,8 +1115,8 @@ void GpuSolver::apply_if_masked_batch(const int* src, const int* dst, int k, dou
     check_batch(src, dst, k);
     const Batch  b = batch(k);
     const int    L = depth_ - 1;
-    double*      st = work_.src.p;
-    double*      tt = work_.tgt.p;
+    double*      st = wp_.src;
+    double*      tt = wp_.tgt;
     // The pass reads the source's level-L interiors only and adds into the
     // level-L targets only.
     launch::zero_blocks(stream_, n_, tt, p_level_slots_[L].p, p_level_slots_[L].size(), b);

@@ -274,6 +274,19 @@ class GpuSolver
         int               k = 0;
     };
     Work work_;
+    // Where the solves find their work arrays: work_'s arrays, or a caller's
+    // scratch region for the duration of an allocation-free solve.  The
+    // spectra hold double2 (precision 0) or float2 values.
+    struct WorkPtrs
+    {
+        double* src = nullptr;
+        double* tgt = nullptr;
+        double* fmm_s = nullptr;
+        double* fmm_t = nullptr;
+        void*   sspec = nullptr;
+        void*   tspec = nullptr;
+    };
+    WorkPtrs wp_;
     bool                                          graphs_ = false;
     std::map<std::vector<int64_t>, hipGraphExec_t> graph_cache_;
 

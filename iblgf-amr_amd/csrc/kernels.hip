@@ -1341,11 +1341,21 @@ void forward_dft(hipStream_t s, int n, int prec, const double* field, const int*
     if (split < 0) split = count;
     if (split > count) throw std::invalid_argument("forward_dft: split outside the block list");
     if (split < count && !field_in) throw std::invalid_argument("forward_dft: no field for the interior-read blocks");
+    if (prec_mfma(prec))
+    {
+        forward_dft_mfma(s, n, prec, field, slots, count, static_cast<float2*>(spec), b, field_in, split);
+        return;
+    }
     if (count > 0) dispatch_n<RunFwd>(n, s, prec, field, field_in, split, slots, count, spec, b);
 }
 void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* slots, int count, double* field,
     double scale, bool accumulate, const Batch& b)
 {
+    if (prec_mfma(prec))
+    {
+        inverse_dft_mfma(s, n, prec, static_cast<const float2*>(spec), slots, count, field, scale, accumulate, b);
+        return;
+    }
     if (count > 0) dispatch_n<RunInv>(n, s, prec, spec, slots, count, field, scale, accumulate ? 1 : 0, b);
 }
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,

@@ -57,6 +57,14 @@ struct FieldTable
 constexpr int kMacRowBytes = 128;
 constexpr int kMacCompactMax = 200;
 
+// Arithmetic of the block transforms (the solver's precision codes): 0 FP64
+// (VALU), 1 FP64 legs with FP32 spectra (VALU), and the matrix-core legs with
+// FP32 spectra: BF16 three-term split, FP16 three-term split with power-of-two
+// line scaling, FP16 one term with line scaling, four signed 7-bit integer
+// digits with exact INT32 accumulation.
+constexpr int kPrecFp64 = 0, kPrecMixed = 1, kPrecBf16x3 = 2, kPrecF16x3 = 3, kPrecF16 = 4, kPrecI8x4 = 5;
+constexpr bool prec_mfma(int p) { return p >= kPrecBf16x3 && p <= kPrecI8x4; }
+
 namespace launch {
 
 bool supported_n(int n);
@@ -90,7 +98,9 @@ void correction_stencil(hipStream_t s, int n, const double* target_tmp, double* 
 // prec 0: every spectrum FP64 (double2);
 // prec 1: source, kernel and target spectra stored FP32 (float2); the MAC
 // forms FP32 products, accumulates them in FP64 and rounds each target
-// spectrum once on store; DFT arithmetic, LDS intermediates and fields FP64.
+// spectrum once on store; DFT arithmetic, LDS intermediates and fields FP64;
+// prec 2-5 (prec_mfma): spectra and MAC as prec 1, the DFT legs on the
+// matrix cores (forward_dft_mfma / inverse_dft_mfma).
 // Blocks slots[split..count) are read from field_in with their boundary ring
 // taken as zero, the others from field.
 void forward_dft(hipStream_t s, int n, int prec, const double* field, const int* slots, int count, void* spec,
@@ -108,6 +118,14 @@ void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ng
     const uint4* rowoff, const int* olist, int nused, bool compact, const void* ghat, const void* sspec,
     void* tspec, const Batch& b = {});
 void to_float2(hipStream_t s, const double2* in, float2* out, long n);
+
+// The forward and inverse transforms with every 1-D leg on the matrix cores
+// (FP32 spectra, FP32 intermediates); prec is one of the split precisions
+// below, the arguments are those of forward_dft / inverse_dft.
+void forward_dft_mfma(hipStream_t s, int n, int prec, const double* field, const int* slots, int count,
+    float2* spec, const Batch& b = {}, const double* field_in = nullptr, int split = -1);
+void inverse_dft_mfma(hipStream_t s, int n, int prec, const float2* spec, const int* slots, int count,
+    double* field, double scale, bool accumulate, const Batch& b = {});
 
 // Kernel spectra for the 343 offsets o in [-3,3]^3 at level difference d:
 // block[a] = K(step*(-o*e - (n-1) + a)), a in [0,P)^3, step = 2^d, then r2c.

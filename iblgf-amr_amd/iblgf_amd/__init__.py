@@ -24,11 +24,18 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libiblgf_gpu.so")
 
+# status codes of the C ABI (enum iblgf_status)
+IBLGF_OK, IBLGF_ERR_ARG, IBLGF_ERR_STATE, IBLGF_ERR_DEVICE, IBLGF_ERR_UNSUPPORTED, IBLGF_ERR_INTERNAL = range(6)
+
 KERNEL_LAPLACE = 0
 KERNEL_IF = 1
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_BF16X3 = 2
+PREC_FP16X3 = 3
+PREC_FP16 = 4
+PREC_INT8X4 = 5
 
 FLAG_LEAF = 1
 FLAG_CORRECTION = 2
@@ -329,7 +336,9 @@ class PoissonSolver:
         return a
 
     def set_precision(self, precision):
-        """PREC_FP64 (plain) or PREC_MIXED (FP32 spectra/DFT/MAC, FP64 fields and level sums)."""
+        """PREC_FP64 (plain); PREC_MIXED (FP32 spectra and MAC products, FP64 DFT legs,
+        fields and level sums); PREC_BF16X3, PREC_FP16X3, PREC_FP16, PREC_INT8X4 (as
+        PREC_MIXED with the DFT legs on the matrix cores in that arithmetic)."""
         _check(lib().iblgf_gpu_set_precision(self._h, int(precision)), self._h)
 
     def prepare(self, kernel=KERNEL_LAPLACE, alpha_base=0.0):

@@ -204,10 +204,18 @@ int iblgf_gpu_set_concurrency(iblgf_gpu_ctx* ctx, int on);
 enum iblgf_precision
 {
     IBLGF_PREC_FP64 = 0,  /* plain path: spectra, DFT legs and MAC in FP64 */
-    IBLGF_PREC_MIXED = 1  /* source, kernel and target spectra stored FP32 (kernel spectra
-                             generated in FP64; target spectra accumulated in FP64 and rounded
-                             once on store), MAC products in FP32; DFT arithmetic, fields and
-                             cross-level transfers FP64 */
+    IBLGF_PREC_MIXED = 1,  /* source, kernel and target spectra stored FP32 (kernel spectra
+                              generated in FP64; target spectra accumulated in FP64 and rounded
+                              once on store), MAC products in FP32; DFT arithmetic, fields and
+                              cross-level transfers FP64 */
+    /* Spectra, MAC, fields and transfers as IBLGF_PREC_MIXED, every 1-D DFT
+     * leg a GEMM on the matrix cores with FP32 intermediates: */
+    IBLGF_PREC_BF16X3 = 2,  /* BF16 hi/lo split, three products, FP32 accumulation */
+    IBLGF_PREC_FP16X3 = 3,  /* FP16 hi/lo split of power-of-two scaled lines, three products,
+                               FP32 accumulation */
+    IBLGF_PREC_FP16 = 4,    /* FP16 of power-of-two scaled lines, one product */
+    IBLGF_PREC_INT8X4 = 5   /* four signed 7-bit digits per value, digit products summed
+                               exactly in INT32, combined in FP64 */
 };
 
 /* Arithmetic of the subsequent solves (no reference counterpart: the
