@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "LGF solves/s + HBM/MFMA roofline %, batched-mixed vs plain FP64 at relL2≤1e-6"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64, vector and matrix
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (published; reached only by packed FP32 FMAs)
 PEAK_HBM_GBS = 8000.0
 
 
@@ -44,36 +45,78 @@ def unit_flops(n):
     return dict(forward_dft=f_fwd, inverse_dft=f_fwd, spectral_mac=8 * H * P ** 2, transfer=6 * n ** 4)
 
 
-def cpu_baseline(wl, counts, budget_s):
-    """FP64 oracle in the reference's faithful mode (per-pair source transform)
-    on a bounded sample of the finest-level FMM pass, scaled by pair count."""
+def host_cpu():
+    """Host CPU description: the cores this process may run on and lscpu's
+    socket/core/model lines."""
+    import subprocess
+
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(wl, src, counts, budget_s, nsolves=3):
+    """The FP64 oracle's whole warm solve on the host cores.
+
+    W workers (the process's CPU share: OMP_NUM_THREADS, else its affinity
+    set) each take a contiguous, load-balanced range of every FMM pass's
+    targets (the reference's Server::split, server.hpp:127-170); one cold solve
+    builds the kernel spectra, then the median of `nsolves` warm solves is
+    reported (the reference times its warm solve, vortexrings.hpp:292-338).
+    The oracle transforms each source once per pass; the reference
+    re-transforms it for every pair (convolution.hpp:259-260), which a second
+    figure prices on one thread from a bounded sample of the finest pass."""
     import numpy as np
 
     from oracle import oracle as O
 
     a, info, n = wl.arrays, wl.info, wl.tree.n
     ot = O.Tree(a, info, n)
+    host = host_cpu()
+    W = int(os.environ.get("OMP_NUM_THREADS") or 0) or host["affinity"]
+    W = max(1, min(W, host["affinity"]))
+    t0 = time.perf_counter()
+    sec, _ = ot.lgf_baseline(src, ranks=W, nsolves=nsolves)
+    total = time.perf_counter() - t0
+    t_warm = float(np.median(sec))
+
+    # per-pair transform (the reference's algorithm) on one thread: Bx of a
+    # bounded set of finest-level targets, scaled by pair count
     L = info["depth"] - 1
     nl = 1
     idx = 15 + 2 * (L - info["base_level"]) + nl
-    lvl = a["level"]
     tmask = (a["mask_tgt"] >> np.uint64(idx)) & np.uint64(1)
-    targets = [int(s) for s in np.nonzero((lvl == L) & (tmask == 1))[0]]
-    rng = np.random.default_rng(0)
-    fmm_s = rng.standard_normal((info["nblocks"], n, n, n))
-    pairs1, sec1, _ = ot.bx_sample(fmm_s, L, nl, targets[:2], faithful=True)
-    per_target = sec1 / 2.0
-    k = max(2, min(len(targets), int(budget_s / max(per_target, 1e-6))))
-    pairs, sec, _ = ot.bx_sample(fmm_s, L, nl, targets[:k], faithful=True)
-    t_solve = sec / pairs * counts["n_pair"]
+    targets = [int(s) for s in np.nonzero((a["level"] == L) & (tmask == 1))[0]]
+    fmm_s = np.random.default_rng(0).standard_normal((info["nblocks"], n, n, n))
+    _, sec1, _ = ot.bx_sample(fmm_s, L, nl, targets[:2], faithful=True)
+    k = max(2, min(len(targets), int(budget_s / max(sec1 / 2.0, 1e-6))))
+    pairs, secf, _ = ot.bx_sample(fmm_s, L, nl, targets[:k], faithful=True)
+    t_faithful = secf / pairs * counts["n_pair"]
     return {
-        "value": 1.0 / t_solve,
+        "value": 1.0 / t_warm,
         "unit": "solves/s",
-        "cores": 1,
+        "cores": W,
         "kind": "port",
-        "sample": (f"FP64 oracle, faithful mode (source re-transformed per pair as convolution.hpp:259-260), "
-                   f"Bx of the finest-level pass over {k} targets = {pairs} pairs in {sec:.1f} s (warm spectra), "
-                   f"scaled by pair count to one full solve ({counts['n_pair']} pairs); 1 thread"),
+        "sample": (f"FP64 oracle (C restatement of apply_lgf, direct DFT legs at P = 2n-1, no FFTW), whole warm "
+                   f"scalar solve of the full workload, {W} workers with contiguous load-balanced target ranges "
+                   f"(Server::split); median of {nsolves} warm solves after one cold solve, {total:.1f} s in all; "
+                   f"each source transformed once per pass"),
+        "warm_solve_seconds": [float(x) for x in sec],
+        "host": host,
+        "per_pair_transform_1_thread": {
+            "value": 1.0 / t_faithful,
+            "unit": "solves/s",
+            "sample": (f"reference's per-pair source transform (convolution.hpp:259-260), 1 thread: Bx of {k} "
+                       f"finest-level targets = {pairs} pairs in {secf:.1f} s, scaled by pair count to one solve "
+                       f"({counts['n_pair']} pairs); an extrapolation, not a measured solve"),
+        },
     }
 
 
@@ -191,10 +234,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0,
+                    help="budget of the one-thread per-pair-transform sample of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="skip the learned-FCG immersed-boundary comparison")
-    ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph"],
+    ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph",
+                                       "int8x4_batched", "fp16x3_batched", "bf16x3_batched", "fp16_batched"],
                     help="time one way only and print its rate (for profiler runs)")
     ap.add_argument("--one-stream", action="store_true",
                     help="with --only: run each FMM pass on one stream (the configuration the roofline is timed in)")
@@ -207,7 +252,10 @@ def main():
 
     import numpy as np
 
-    from iblgf_amd import FLAG_LEAF, PREC_FP64, PREC_MIXED, PoissonSolver, hip_events, setups
+    from iblgf_amd import (FLAG_LEAF, PREC_BF16X3, PREC_FP16, PREC_FP16X3, PREC_FP64, PREC_INT8X4, PREC_MIXED,
+                           PoissonSolver, hip_events, setups)
+
+    split_precs = {"int8x4": PREC_INT8X4, "fp16x3": PREC_FP16X3, "bf16x3": PREC_BF16X3, "fp16": PREC_FP16}
 
     wl = setups.VortexRings(seed=1001)
     srcs = [wl.source(c) for c in range(3)]
@@ -263,7 +311,7 @@ def main():
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     if args.only:
-        prec = PREC_MIXED if args.only.startswith("mixed") else PREC_FP64
+        prec = PREC_MIXED if args.only.startswith("mixed") else split_precs.get(args.only.split("_")[0], PREC_FP64)
         r = timed(prec, "batched" in args.only, args.only.endswith("graph"), concurrent=not args.one_stream)
         print(json.dumps({"only": args.only, "one_stream": args.one_stream, "solves_per_s": 3 * args.steps / r["wall"],
                           "ms_per_step": r["wall"] / args.steps * 1e3, "steps": args.steps}))
@@ -284,6 +332,12 @@ def main():
     log(f"mixed, 3 components batched, one stream, kernels timed: {3 * args.steps / rmxbt['wall']:.2f} solves/s")
     rmxg = timed(PREC_MIXED, True, graphs=True)
     log(f"mixed, 3 components batched, one HIP graph per step: {3 * args.steps / rmxg['wall']:.2f} solves/s")
+    rsp, rspt = {}, {}
+    for name, prec in split_precs.items():
+        rsp[name] = timed(prec, True)
+        log(f"{name} DFT legs on the matrix cores, 3 components batched: {3 * args.steps / rsp[name]['wall']:.2f} "
+            f"solves/s")
+        rspt[name] = timed(prec, True, concurrent=False, timing=True)
     counts = solver.counts()
     mac_plan = solver.mac_counts()
     # Every entry issues 8 target products (children outside its mask read a
@@ -303,8 +357,14 @@ def main():
     errors = errors_vs_fp64(rmxb)
     errors_single = errors_vs_fp64(rmx)
     batched_fp64_identical = all(np.array_equal(a, b) for a, b in zip(r64b["outs"], r64["outs"]))
-    mixed_ok = max(e["relL2"] for e in errors) <= 1e-6
     solves = 3 * args.steps
+    # batched candidates for the headline: the fastest within relative L2 1e-6
+    # of plain FP64 on every component
+    cands = {"mixed": (rmxb, rmxbt, errors)}
+    for name in split_precs:
+        cands[name] = (rsp[name], rspt[name], errors_vs_fp64(rsp[name]))
+    ok = {k: v for k, v in cands.items() if max(e["relL2"] for e in v[2]) <= 1e-6}
+    head_name = max(ok, key=lambda k: solves / ok[k][0]["wall"]) if ok else None
 
     def kernel_table(kt, peak):
         out = {}
@@ -318,7 +378,7 @@ def main():
                       "achieved_tflops": (fl / (ms * 1e-3) / 1e12) if ms > 0 else 0.0}
         return out
 
-    def roofline_of(kt, peak, peak_name):
+    def roofline_of(kt, peak, peak_name, bound="valu"):
         table = kernel_table(kt, peak)
         dom = max((k for k in table if k != "other"), key=lambda k: table[k]["ms_per_solve"])
         ms, launches, work = kt[dom]
@@ -329,18 +389,40 @@ def main():
             with open(tpath) as f:
                 traffic = json.load(f).get(f"{peak_name}:{dom}")
         return table, {
-            "bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": traffic, "avg_launch_ms": ms / max(launches, 1),
             "flops_per_launch": work * uf[dom] / max(launches, 1), "arithmetic": peak_name,
         }
 
     k64, roof64 = roofline_of(r64t["kt"], PEAK_FP64_TFLOPS, "fp64")
     k64b, roof64b = roofline_of(r64bt["kt"], PEAK_FP64_TFLOPS, "fp64_batched")
-    # the mixed MAC accumulates in FP64 and MI355X runs unpacked FP32 and FP64 FMA at one rate
-    kmxb, roofmxb = roofline_of(rmxbt["kt"], PEAK_FP64_TFLOPS, "mixed_batched")
-    for roof in (roof64, roof64b, roofmxb):
+    # The MAC runs on the vector ALUs (no matrix-core instruction): the FP64 path
+    # against the FP64 vector peak; the mixed path (FP32 products and partial
+    # sums, FP64 flushes) against the published FP32 vector peak, with its
+    # fraction of the FP64 rate (which unpacked FP32 FMAs issue at) beside it.
+    kmxb, roofmxb = roofline_of(rmxbt["kt"], PEAK_FP32_TFLOPS, "mixed_batched")
+    roofmxb["frac_of_fp64_vector_rate"] = roofmxb["achieved"] / PEAK_FP64_TFLOPS
+    split_out = {}
+    for name in split_precs:
+        kt_, roof_ = roofline_of(rspt[name]["kt"], PEAK_FP32_TFLOPS, name + "_batched")
+        roof_["frac_of_fp64_vector_rate"] = roof_["achieved"] / PEAK_FP64_TFLOPS
+        split_out[name] = {
+            "solves_per_s": solves / rsp[name]["wall"],
+            "vs_plain_fp64_speedup": (solves / rsp[name]["wall"]) / (solves / r64["wall"]),
+            "one_stream_solves_per_s": solves / rspt[name]["wall"],
+            "error_vs_fp64_per_component": cands[name][2],
+            "kernels": kt_,
+            "dominant_roofline": roof_,
+        }
+    if head_name is None:
+        head, head_t, head_kt, roof_head = r64b, r64bt, k64b, roof64b
+    elif head_name == "mixed":
+        head, head_t, head_kt, roof_head = rmxb, rmxbt, kmxb, roofmxb
+    else:
+        head, head_t = rsp[head_name], rspt[head_name]
+        head_kt, roof_head = split_out[head_name]["kernels"], split_out[head_name]["dominant_roofline"]
+    for roof in (roof64, roof64b, roofmxb, *(v["dominant_roofline"] for v in split_out.values())):
         roof["timed_in"] = "one-stream eager run of the same step (HIP events on the solver stream)"
-    head = rmxb if mixed_ok else r64b
     value = solves / head["wall"]
     v64 = solves / r64["wall"]
     v64b = solves / r64b["wall"]
@@ -360,15 +442,25 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f32+f64" if mixed_ok else "f64",
+        "dtype": {None: "f64", "mixed": "f32+f64", "int8x4": "i8+f32+f64", "fp16x3": "f16+f32+f64",
+                  "bf16x3": "bf16+f32+f64", "fp16": "f16+f32+f64"}[head_name],
         "data": "synthetic",
         "config": {
             "workload": ("adaptive vortex-ring octree Poisson: 8^3 base blocks of 16^3 cells + 3 refinement levels, "
                          "16 seeded Gaussian-core rings (seed 1001); one step = 3 vorticity components, "
                          "batched through every launch"),
-            "precision": ("mixed: source and kernel spectra stored FP32, multiplied in FP32 and accumulated in "
-                          "FP64 by the MAC, target spectra stored FP32; DFT arithmetic, fields and level sums FP64"
-                          if mixed_ok else "plain FP64"),
+            "headline_path": head_name or "fp64_batched",
+            "precision": {
+                None: "plain FP64 (no reduced-precision path met relative L2 1e-6)",
+                "mixed": ("mixed: source and kernel spectra stored FP32, multiplied in FP32 and accumulated in "
+                          "FP64 by the MAC, target spectra stored FP32; DFT arithmetic, fields and level sums FP64"),
+                "int8x4": ("DFT legs as INT8 digit-split GEMMs on the matrix cores (4 signed 7-bit digits, exact "
+                           "INT32 sums), leg outputs FP32; spectra and MAC as the mixed path; fields and level sums "
+                           "FP64"),
+                "fp16x3": "DFT legs as FP16 three-term GEMMs of power-of-two scaled lines on the matrix cores",
+                "bf16x3": "DFT legs as BF16 three-term GEMMs on the matrix cores",
+                "fp16": "DFT legs as FP16 one-term GEMMs of power-of-two scaled lines on the matrix cores",
+            }[head_name],
             "blocks": wl.info["nblocks"],
             "leaves": wl.info["nleaves"],
             "leaf_cells": leaf_cells,
@@ -391,20 +483,22 @@ def main():
             "batched_fp64_bitwise_equal_to_single": batched_fp64_identical,
             "mixed_error_vs_fp64_per_component": errors,
             "mixed_single_error_vs_fp64_per_component": errors_single,
+            "split_precision_batched": split_out,
+            "headline_vs_plain_fp64_speedup": value / v64,
             "fp64_roofline": roof64,
             "fp64_kernels": k64,
             "fp64_batched_kernels": k64b,
         },
-        "roofline": roofmxb if mixed_ok else roof64b,
+        "roofline": roof_head,
         # per-class times of the one-stream timed run (the throughput run overlaps classes)
-        "kernels": kmxb if mixed_ok else k64b,
+        "kernels": head_kt,
         "cpu_baseline": None,
     }
     if not args.no_ib:
         result["config"]["learned_fcg"] = learned_fcg()
     if rank == 0 and not args.no_cpu_baseline:
         # counts cover the FMM passes of one Laplace solve (each pass plan is built once)
-        result["cpu_baseline"] = cpu_baseline(wl, counts, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(wl, srcs[0], counts, args.cpu_seconds)
     if rank == 0:
         print(json.dumps(result))
     return 0

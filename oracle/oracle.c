@@ -20,6 +20,7 @@
 #include "oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <time.h>
 #include <stdlib.h>
 #include <string.h>
@@ -617,6 +618,7 @@ typedef struct
     kernel_t           ker;
     int                faithful;
     int                intrp_order;
+    int                ranks; /* > 0: Bx and source transforms on `ranks` workers, contiguous load split */
     /* transfer matrices: nli (anterp/interp) and cell-centre (restrict/interp) */
     double *nli_sub[2], *nli_subT[2], *cc_restrict[2], *cc_interpT[2];
     /* working fields [nb][n^3] */
@@ -789,6 +791,63 @@ static long influence_field(ostate* o, int tg, int L, int idx, int neighbor_only
     return count;
 }
 
+/* Server::split (include/iblgf/domain/decomposition/server.hpp:127-170): items
+ * in their tree order go to W workers as contiguous ranges, worker w taking
+ * items while the running load is at most (w + 1)/W of the total (and at least
+ * one item); a zero load counts as 1.  begin[w]..begin[w+1] is worker w's range. */
+static void split_ranges(const long* load, int n, int W, int* begin)
+{
+    double total = 0.0, current = 0.0;
+    int    i, w, it = 0;
+    for (i = 0; i < n; ++i) total += load[i] > 0 ? (double)load[i] : 1.0;
+    for (w = 0; w <= W; ++w) begin[w] = n;
+    for (w = 0; w < W && it < n; ++w)
+    {
+        double target = (double)(w + 1) / W * total;
+        int    count = 0;
+        if (target > total) target = total;
+        begin[w] = it;
+        while (it < n && (current <= target || count == 0))
+        {
+            current += load[it] > 0 ? (double)load[it] : 1.0;
+            ++it;
+            ++count;
+        }
+    }
+    for (; w <= W; ++w) begin[w] = n;
+}
+
+/* pairs a target takes in one pass: its load for the split */
+static long target_load(const ostate* o, int tg, int L, int idx, int neighbor_only)
+{
+    const oracle_tree* t = o->t;
+    long               c = 0;
+    int                i;
+    if (t->level[tg] == L)
+        for (i = 0; i < 27; ++i)
+        {
+            const int s = t->neighbor[27 * tg + i];
+            if (s >= 0 && MS(s, idx)) ++c;
+        }
+    if (!neighbor_only)
+        for (i = 0; i < t->influence_num[tg]; ++i)
+        {
+            const int s = t->influence[189 * tg + i];
+            if (s >= 0 && MS(s, idx)) ++c;
+        }
+    return c;
+}
+
+static void source_spectrum(ostate* o, int s)
+{
+    const size_t S = spec_size(&o->dft);
+    double*      re = (double*)malloc(sizeof(double) * S);
+    double*      im = (double*)malloc(sizeof(double) * S);
+    dft_forward(&o->dft, BLK(o->fmm_s, s), o->n, re, im);
+    o->src_re[s] = re;
+    o->src_im[s] = im;
+}
+
 /* Every kernel spectrum (and, when cached, every source spectrum) that the
  * targets tg[0..nt) will read, made ahead of the concurrent Bx loop. */
 static void prepare_spectra(ostate* o, const int* tg, int nt, int L, int idx, int neighbor_only)
@@ -819,15 +878,26 @@ static void prepare_spectra(ostate* o, const int* tg, int nt, int L, int idx, in
     nnew = o->cache.n - first;
 #pragma omp parallel for schedule(dynamic, 1)
     for (i = 0; i < nnew; ++i) spec_fill(&o->cache.e[first + i], &o->dft, &o->ker);
-#pragma omp parallel for schedule(dynamic, 1)
-    for (i = 0; i < nl; ++i)
+    if (o->ranks > 0)
     {
-        const size_t S = spec_size(&o->dft);
-        double*      re = (double*)malloc(sizeof(double) * S);
-        double*      im = (double*)malloc(sizeof(double) * S);
-        dft_forward(&o->dft, BLK(o->fmm_s, list[i]), o->n, re, im);
-        o->src_re[list[i]] = re;
-        o->src_im[list[i]] = im;
+        /* the sources in contiguous equal shares, one per worker */
+        int*  b = (int*)malloc(sizeof(int) * (size_t)(o->ranks + 1));
+        long* ld = (long*)malloc(sizeof(long) * (size_t)(nl + 1));
+        for (i = 0; i < nl; ++i) ld[i] = 1;
+        split_ranges(ld, nl, o->ranks, b);
+#pragma omp parallel num_threads(o->ranks)
+        {
+            const int w = omp_get_thread_num();
+            int       k;
+            for (k = b[w]; k < b[w + 1]; ++k) source_spectrum(o, list[k]);
+        }
+        free(b);
+        free(ld);
+    }
+    else
+    {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (i = 0; i < nl; ++i) source_spectrum(o, list[i]);
     }
     free(need);
     free(list);
@@ -893,9 +963,29 @@ static void fmm_apply_idx(ostate* o, const double* Source, double* Target, int L
             for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
                 if (MT(s, idx)) tg[nt++] = s;
         prepare_spectra(o, tg, nt, L, idx, neighbor_only);
+        if (o->ranks > 0)
+        {
+            /* the targets, in tree order, as contiguous ranges of equal pair load */
+            int*  b = (int*)malloc(sizeof(int) * (size_t)(o->ranks + 1));
+            long* ld = (long*)malloc(sizeof(long) * (size_t)(nt + 1));
+            for (it = 0; it < nt; ++it) ld[it] = target_load(o, tg[it], L, idx, neighbor_only);
+            split_ranges(ld, nt, o->ranks, b);
+#pragma omp parallel num_threads(o->ranks)
+            {
+                const int w = omp_get_thread_num();
+                int       k;
+                for (k = b[w]; k < b[w + 1]; ++k)
+                    influence_field(o, tg[k], L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
+            }
+            free(b);
+            free(ld);
+        }
+        else
+        {
 #pragma omp parallel for schedule(dynamic, 1)
-        for (it = 0; it < nt; ++it)
-            influence_field(o, tg[it], L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
+            for (it = 0; it < nt; ++it)
+                influence_field(o, tg[it], L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
+        }
         free(tg);
     }
     if (!neighbor_only)
@@ -1017,17 +1107,12 @@ static void add_source_correction(ostate* o, int s, double dx, double omega)
     }
 }
 
-int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
-    int faithful, int intrp_order)
+/* apply_lgf's level loop (poisson.hpp:587-894) on an initialised state */
+static void lgf_solve(ostate* o, const double* src, double* dst)
 {
-    ostate o_;
-    ostate* o = &o_;
-    int     l, s;
-    size_t  q;
-    ostate_init(o, t, faithful, intrp_order);
-    o->ker.is_if = 0;
-    o->ker.g = g;
-    o->ker.nmax = nmax;
+    const oracle_tree* t = o->t;
+    int                l, s;
+    size_t             q;
     /* copy_leaf with buffer */
     for (s = 0; s < t->nb; ++s)
         if (IS_LEAF(s)) memcpy(BLK(o->source_tmp, s), BLK(src, s), sizeof(double) * o->n3);
@@ -1048,6 +1133,47 @@ int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const doub
         for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s) add_source_correction(o, s, dx / 2.0, 0.0);
         for (s = o->lvl_begin[l + 1]; s < o->lvl_begin[l + 2]; ++s)
             for (q = 0; q < o->n3; ++q) BLK(o->source_tmp, s)[q] += BLK(o->correction_tmp, s)[q];
+    }
+}
+
+int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
+    int faithful, int intrp_order)
+{
+    ostate  o_;
+    ostate* o = &o_;
+    ostate_init(o, t, faithful, intrp_order);
+    o->ker.is_if = 0;
+    o->ker.g = g;
+    o->ker.nmax = nmax;
+    lgf_solve(o, src, dst);
+    ostate_free(o);
+    return 0;
+}
+
+int oracle_lgf_baseline(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
+    int faithful, int ranks, int nsolves, double* seconds)
+{
+    ostate  o_;
+    ostate* o = &o_;
+    int     i;
+    ostate_init(o, t, faithful, 5);
+    o->ker.is_if = 0;
+    o->ker.g = g;
+    o->ker.nmax = nmax;
+    o->ranks = ranks;
+    for (i = 0; i <= nsolves; ++i)
+    {
+        const size_t all = o->n3 * (size_t)t->nb * sizeof(double);
+        const double t0 = omp_get_wtime();
+        /* every solve starts from cleared work fields, as clean_field does
+         * (poisson.hpp:597-600); the kernel spectra stay cached */
+        memset(o->source_tmp, 0, all);
+        memset(o->target_tmp, 0, all);
+        memset(o->correction_tmp, 0, all);
+        memset(o->fmm_s, 0, all);
+        memset(o->fmm_t, 0, all);
+        lgf_solve(o, src, dst);
+        if (i > 0) seconds[i - 1] = omp_get_wtime() - t0;
     }
     ostate_free(o);
     return 0;

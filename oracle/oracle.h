@@ -70,6 +70,17 @@ void oracle_block_conv(const double* g, int nmax, int n, const double* src, cons
 int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const double* src,
     double* dst, int faithful, int intrp_order);
 
+/* CPU baseline: oracle_apply_lgf (intrp_order 5) run 1 + nsolves times on one
+ * state whose kernel-spectrum cache persists, so solves after the first are
+ * warm (the reference times its warm second solve,
+ * tests/poisson/vortexrings.hpp:292-338).  ranks > 0: each FMM pass's Bx
+ * targets go to `ranks` workers as contiguous ranges of equal pair load
+ * (Server::split, include/iblgf/domain/decomposition/server.hpp:127-170) and
+ * its source transforms in equal contiguous shares; ranks = 0: dynamic
+ * scheduling.  seconds[i] = wall time of warm solve i; dst = the last result. */
+int oracle_lgf_baseline(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
+    int faithful, int ranks, int nsolves, double* seconds);
+
 /* PoissonSolver::apply_lgf_IF (poisson.hpp:427-494), integrating-factor kernel
  * E_a with a_l = a_base * 4^(l-base). */
 int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double alpha_base,

@@ -54,6 +54,7 @@ def lib():
         L.oracle_block_conv.argtypes = [_P, _I, _I, _P, _P, _I, _D, _P]
         L.oracle_apply_lgf.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _P, _I, _I]
         L.oracle_apply_if.argtypes = [ctypes.POINTER(OracleTree), _P, _P, _D, _I, _I]
+        L.oracle_lgf_baseline.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _P, _I, _I, _I, _P]
         L.oracle_bx_sample.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _I, _I, _I, _P, _I, _P, _P]
         L.oracle_bx_sample.restype = ctypes.c_long
         T, B = ctypes.POINTER(OracleTree), ctypes.POINTER(OracleIb)
@@ -147,6 +148,18 @@ class Tree:
                                     1 if faithful else 0, int(intrp_order))
         assert rc == 0
         return dst
+
+    def lgf_baseline(self, src, ranks, nsolves, faithful=False, nmax=100):
+        """1 + nsolves apply_lgf solves with the kernel spectra kept; returns
+        (seconds of each warm solve, last result).  ranks > 0 splits each pass's
+        targets over that many workers by contiguous load (server.hpp:127-170)."""
+        s = np.ascontiguousarray(src, np.float64)
+        dst = np.zeros_like(s)
+        sec = np.zeros(max(int(nsolves), 1))
+        rc = lib().oracle_lgf_baseline(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(s), _p(dst),
+                                       1 if faithful else 0, int(ranks), int(nsolves), _p(sec))
+        assert rc == 0
+        return sec[:int(nsolves)], dst
 
     def apply_if(self, src, alpha_base, faithful=False, intrp_order=5):
         s = np.ascontiguousarray(src, np.float64)
