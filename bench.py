@@ -239,7 +239,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="skip the learned-FCG immersed-boundary comparison")
     ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph",
-                                       "int8x4_batched", "fp16x3_batched", "bf16x3_batched", "fp16_batched"],
+                                       "fp16x3forward_batched", "int8x4_batched", "fp16x3_batched", "bf16x3_batched",
+                                       "fp16_batched"],
                     help="time one way only and print its rate (for profiler runs)")
     ap.add_argument("--one-stream", action="store_true",
                     help="with --only: run each FMM pass on one stream (the configuration the roofline is timed in)")
@@ -252,10 +253,11 @@ def main():
 
     import numpy as np
 
-    from iblgf_amd import (FLAG_LEAF, PREC_BF16X3, PREC_FP16, PREC_FP16X3, PREC_FP64, PREC_INT8X4, PREC_MIXED,
-                           PoissonSolver, hip_events, setups)
+    from iblgf_amd import (FLAG_LEAF, PREC_BF16X3, PREC_FP16, PREC_FP16X3, PREC_FP16X3_FORWARD, PREC_FP64,
+                           PREC_INT8X4, PREC_MIXED, PoissonSolver, hip_events, setups)
 
-    split_precs = {"int8x4": PREC_INT8X4, "fp16x3": PREC_FP16X3, "bf16x3": PREC_BF16X3, "fp16": PREC_FP16}
+    split_precs = {"fp16x3forward": PREC_FP16X3_FORWARD, "int8x4": PREC_INT8X4, "fp16x3": PREC_FP16X3,
+                   "bf16x3": PREC_BF16X3, "fp16": PREC_FP16}
 
     wl = setups.VortexRings(seed=1001)
     srcs = [wl.source(c) for c in range(3)]
@@ -442,7 +444,8 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": {None: "f64", "mixed": "f32+f64", "int8x4": "i8+f32+f64", "fp16x3": "f16+f32+f64",
+        "dtype": {None: "f64", "mixed": "f32+f64", "fp16x3forward": "f16+f32+f64", "int8x4": "i8+f32+f64",
+                  "fp16x3": "f16+f32+f64",
                   "bf16x3": "bf16+f32+f64", "fp16": "f16+f32+f64"}[head_name],
         "data": "synthetic",
         "config": {
@@ -454,6 +457,10 @@ def main():
                 None: "plain FP64 (no reduced-precision path met relative L2 1e-6)",
                 "mixed": ("mixed: source and kernel spectra stored FP32, multiplied in FP32 and accumulated in "
                           "FP64 by the MAC, target spectra stored FP32; DFT arithmetic, fields and level sums FP64"),
+                "fp16x3forward": ("forward DFT legs as FP16 three-term GEMMs of power-of-two scaled lines on the "
+                                  "matrix cores (FP32 accumulation, FP32 intermediates), inverse DFT legs FP64 on the "
+                                  "vector ALUs; source, kernel and target spectra stored FP32, MAC products FP32 with "
+                                  "FP64 accumulation; fields and level sums FP64"),
                 "int8x4": ("DFT legs as INT8 digit-split GEMMs on the matrix cores (4 signed 7-bit digits, exact "
                            "INT32 sums), leg outputs FP32; spectra and MAC as the mixed path; fields and level sums "
                            "FP64"),

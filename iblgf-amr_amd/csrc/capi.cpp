@@ -168,7 +168,7 @@ int iblgf_gpu_set_concurrency(iblgf_gpu_ctx* ctx, int on)
 int iblgf_gpu_set_precision(iblgf_gpu_ctx* ctx, int precision)
 {
     NEED_CTX("iblgf_gpu_set_precision");
-    if (precision < IBLGF_PREC_FP64 || precision > IBLGF_PREC_INT8X4)
+    if (precision < IBLGF_PREC_FP64 || precision > IBLGF_PREC_FP16X3_FORWARD)
         return fail(IBLGF_ERR_ARG, "iblgf_gpu_set_precision: unknown precision", ctx);
     return guarded(ctx, [&] { ctx->solver->set_precision(precision); });
 }

@@ -769,8 +769,9 @@ const void* GpuSolver::ghat(bool laplace, int d, double alpha)
 
 void GpuSolver::set_precision(int p)
 {
-    if (p < kPrecFp64 || p > kPrecI8x4)
-        throw std::invalid_argument("precision must be 0 (FP64), 1 (mixed FP32/FP64) or 2-5 (matrix-core DFT legs)");
+    if (p < kPrecFp64 || p > kPrecF16x3Forward)
+        throw std::invalid_argument(
+            "precision must be 0 (FP64), 1 (mixed FP32/FP64), 2-5 (matrix-core DFT legs) or 6 (matrix-core forward legs)");
     precision_ = p;
 }
 
@@ -797,6 +798,9 @@ void GpuSolver::prepare_if(double alpha_base)
 
 // Fmm::apply (include/iblgf/fmm/fmm.hpp:729-870), for the b.k components held
 // in the work arrays.
+int GpuSolver::forward_legs() const { return precision_ == kPrecF16x3Forward ? kPrecF16x3 : precision_; }
+int GpuSolver::inverse_legs() const { return precision_ == kPrecF16x3Forward ? kPrecMixed : precision_; }
+
 void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, const Batch& b)
 {
     double*      fmm_s = wp_.fmm_s;
@@ -840,10 +844,10 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
     }
     const int nfine = static_cast<int>(p.fwd.size()) - p.nfwd_coarse;
     timer_.begin(A);
-    launch::forward_dft(A, n_, precision_, fmm_s, p.fwd.p + p.nfwd_coarse, nfine, wp_.sspec, b, wp_.src, 0);
+    launch::forward_dft(A, n_, forward_legs(), fmm_s, p.fwd.p + p.nfwd_coarse, nfine, wp_.sspec, b, wp_.src, 0);
     timer_.end(A, KC_FORWARD, static_cast<long>(nfine) * b.k);
     timer_.begin(B);
-    launch::forward_dft(B, n_, precision_, fmm_s, p.fwd.p, p.nfwd_coarse, wp_.sspec, b);
+    launch::forward_dft(B, n_, forward_legs(), fmm_s, p.fwd.p, p.nfwd_coarse, wp_.sspec, b);
     timer_.end(B, KC_FORWARD, static_cast<long>(p.nfwd_coarse) * b.k);
     for (size_t i = 0; i < p.mac.size(); ++i)
     {
@@ -858,10 +862,10 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
     // are written to fmm_t and interpolated down.
     const int ntop = p.ninv_top, nrest = static_cast<int>(p.inv.size()) - ntop;
     timer_.begin(A);
-    launch::inverse_dft(A, n_, precision_, wp_.tspec, p.inv.p, ntop, wp_.tgt, iscale, true, b);
+    launch::inverse_dft(A, n_, inverse_legs(), wp_.tspec, p.inv.p, ntop, wp_.tgt, iscale, true, b);
     timer_.end(A, KC_INVERSE, static_cast<long>(ntop) * b.k);
     timer_.begin(B);
-    launch::inverse_dft(B, n_, precision_, wp_.tspec, p.inv.p + ntop, nrest, fmm_t, iscale, false, b);
+    launch::inverse_dft(B, n_, inverse_legs(), wp_.tspec, p.inv.p + ntop, nrest, fmm_t, iscale, false, b);
     timer_.end(B, KC_INVERSE, static_cast<long>(nrest) * b.k);
     if (coarse)
     {

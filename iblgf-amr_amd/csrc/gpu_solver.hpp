@@ -230,6 +230,8 @@ class GpuSolver
     double2*    ghat_if(double alpha);
     const void* ghat(bool laplace, int d, double alpha);  // in the current precision
     void       fill_zero(double* f, const Batch& b = {});
+    int        forward_legs() const;  // arithmetic of the forward / inverse transforms' legs
+    int        inverse_legs() const;
     double*    F(int id) { return fields_.at(id).p; }
     FieldTable field_table(const int* ids, int k);           // the fields ids[0..k)
     FieldTable work_table(double* base, int k) const;        // components of a work array

@@ -63,6 +63,8 @@ constexpr int kMacCompactMax = 200;
 // line scaling, FP16 one term with line scaling, four signed 7-bit integer
 // digits with exact INT32 accumulation.
 constexpr int kPrecFp64 = 0, kPrecMixed = 1, kPrecBf16x3 = 2, kPrecF16x3 = 3, kPrecF16 = 4, kPrecI8x4 = 5;
+// Solver precision code: FP16 three-term forward legs, FP64 inverse legs.
+constexpr int kPrecF16x3Forward = 6;
 constexpr bool prec_mfma(int p) { return p >= kPrecBf16x3 && p <= kPrecI8x4; }
 
 namespace launch {

@@ -36,6 +36,7 @@ PREC_BF16X3 = 2
 PREC_FP16X3 = 3
 PREC_FP16 = 4
 PREC_INT8X4 = 5
+PREC_FP16X3_FORWARD = 6
 
 FLAG_LEAF = 1
 FLAG_CORRECTION = 2
@@ -338,7 +339,8 @@ class PoissonSolver:
     def set_precision(self, precision):
         """PREC_FP64 (plain); PREC_MIXED (FP32 spectra and MAC products, FP64 DFT legs,
         fields and level sums); PREC_BF16X3, PREC_FP16X3, PREC_FP16, PREC_INT8X4 (as
-        PREC_MIXED with the DFT legs on the matrix cores in that arithmetic)."""
+        PREC_MIXED with the DFT legs on the matrix cores in that arithmetic);
+        PREC_FP16X3_FORWARD (forward legs as PREC_FP16X3, inverse legs FP64)."""
         _check(lib().iblgf_gpu_set_precision(self._h, int(precision)), self._h)
 
     def prepare(self, kernel=KERNEL_LAPLACE, alpha_base=0.0):

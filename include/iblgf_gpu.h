@@ -214,8 +214,13 @@ enum iblgf_precision
     IBLGF_PREC_FP16X3 = 3,  /* FP16 hi/lo split of power-of-two scaled lines, three products,
                                FP32 accumulation */
     IBLGF_PREC_FP16 = 4,    /* FP16 of power-of-two scaled lines, one product */
-    IBLGF_PREC_INT8X4 = 5   /* four signed 7-bit digits per value, digit products summed
+    IBLGF_PREC_INT8X4 = 5,  /* four signed 7-bit digits per value, digit products summed
                                exactly in INT32, combined in FP64 */
+    /* The forward transforms' legs as IBLGF_PREC_FP16X3 on the matrix cores,
+     * the inverse transforms' legs in FP64 as IBLGF_PREC_MIXED (the inverse
+     * output, a sum of many contributions, is where reduced-precision legs
+     * cost accuracy); spectra and MAC as IBLGF_PREC_MIXED. */
+    IBLGF_PREC_FP16X3_FORWARD = 6
 };
 
 /* Arithmetic of the subsequent solves (no reference counterpart: the

@@ -14,7 +14,7 @@ import time
 import numpy as np
 import pytest
 
-from iblgf_amd import FLAG_LEAF, KERNEL_IF, PREC_MIXED, PoissonSolver, setups
+from iblgf_amd import FLAG_LEAF, KERNEL_IF, PREC_FP16X3_FORWARD, PREC_MIXED, PoissonSolver, setups
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +79,28 @@ def test_full_size_mixed_batched_matches_oracle(oracle_mod):
     for c in range(3):
         l2, mx = _rel(got[c][leaves], _ref(st, c)[leaves])
         print(f"mixed batched component {c}: relL2={l2:.3e} relmax={mx:.3e}", flush=True)
+        assert l2 < 1e-6
+
+
+def test_full_size_fp16x3_forward_batched_matches_oracle(oracle_mod):
+    """The matrix-core forward legs (FP16 three-term) with FP64 inverse legs,
+    three components batched: within relative L2 1e-6 of the oracle on every
+    component of the full tree."""
+    st = _workload(oracle_mod)
+    v, srcs = st["v"], st["srcs"]
+    print("case: full vortex-ring tree, fp16x3 forward legs, batched, 3 components", flush=True)
+    solver = PoissonSolver(v.tree)
+    fin = [solver.field() for _ in range(3)]
+    fout = [solver.field() for _ in range(3)]
+    for f, x in zip(fin, srcs):
+        solver.upload(f, x)
+    solver.set_precision(PREC_FP16X3_FORWARD)
+    solver.apply_lgf_batch(fin, fout)
+    got = [solver.download(f) for f in fout]
+    leaves = (v.arrays["flags"] & FLAG_LEAF) != 0
+    for c in range(3):
+        l2, mx = _rel(got[c][leaves], _ref(st, c)[leaves])
+        print(f"fp16x3 forward batched component {c}: relL2={l2:.3e} relmax={mx:.3e}", flush=True)
         assert l2 < 1e-6
 
 

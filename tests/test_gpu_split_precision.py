@@ -10,19 +10,22 @@ FP32 accumulation follows, so the legs err at ~1e-5 (bound 2e-4); FP16 one
 term carries 11 bits (~1e-3, bound 1e-2); the FP16 hi/lo pair of a scaled line
 carries 22 bits relative to the line's maximum (bound 5e-6); the integer
 digits carry 27 bits and sum exactly, so only the FP32 storage of leg outputs
-remains (bound 2e-6).  Batched solves equal single solves bit for bit (same
+remains (bound 2e-6).  With FP16 three-term legs in the forward transforms
+only (the inverse legs FP64, as the mixed path) the solve meets the 1e-6
+target (bound 1e-6).  Batched solves equal single solves bit for bit (same
 kernels, same order), and the solves are linear to FP32 storage precision.
 """
 import numpy as np
 import pytest
 
-from iblgf_amd import (FLAG_LEAF, PREC_BF16X3, PREC_FP16, PREC_FP16X3, PREC_INT8X4, PoissonSolver,
-                       setups)
+from iblgf_amd import (FLAG_LEAF, PREC_BF16X3, PREC_FP16, PREC_FP16X3, PREC_FP16X3_FORWARD, PREC_INT8X4,
+                       PoissonSolver, setups)
 
 pytestmark = pytest.mark.gpu
 
-BOUND = {PREC_BF16X3: 2e-4, PREC_FP16X3: 5e-6, PREC_FP16: 1e-2, PREC_INT8X4: 2e-6}
-NAME = {PREC_BF16X3: "bf16x3", PREC_FP16X3: "fp16x3", PREC_FP16: "fp16", PREC_INT8X4: "int8x4"}
+BOUND = {PREC_BF16X3: 2e-4, PREC_FP16X3: 5e-6, PREC_FP16: 1e-2, PREC_INT8X4: 2e-6, PREC_FP16X3_FORWARD: 1e-6}
+NAME = {PREC_BF16X3: "bf16x3", PREC_FP16X3: "fp16x3", PREC_FP16: "fp16", PREC_INT8X4: "int8x4",
+        PREC_FP16X3_FORWARD: "fp16x3 forward"}
 
 
 def _rel(a, b):
@@ -49,7 +52,7 @@ def _case(oracle_mod, extent):
     return _CACHE[extent]
 
 
-@pytest.mark.parametrize("prec", [PREC_INT8X4, PREC_FP16X3, PREC_BF16X3, PREC_FP16])
+@pytest.mark.parametrize("prec", [PREC_INT8X4, PREC_FP16X3, PREC_BF16X3, PREC_FP16, PREC_FP16X3_FORWARD])
 @pytest.mark.parametrize("extent", [8, 16])
 def test_split_precision_error_per_component(oracle_mod, extent, prec):
     print(f"case: {NAME[prec]} legs, adaptive rings, block extent {extent}", flush=True)
