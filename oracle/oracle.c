@@ -20,10 +20,37 @@
 #include "oracle.h"
 
 #include <math.h>
-#include <omp.h>
 #include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
+
+/* Workers of the rank-split loops: the OpenMP team, or one worker when the
+ * library is built without OpenMP (every range then runs on it in turn). */
+static int worker_id(void)
+{
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
+static int worker_count(void)
+{
+#ifdef _OPENMP
+    return omp_get_num_threads();
+#else
+    return 1;
+#endif
+}
+static double wall_seconds(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 #define O_PI 3.14159265358979323846264338327950288
 
@@ -887,9 +914,9 @@ static void prepare_spectra(ostate* o, const int* tg, int nt, int L, int idx, in
         split_ranges(ld, nl, o->ranks, b);
 #pragma omp parallel num_threads(o->ranks)
         {
-            const int w = omp_get_thread_num();
-            int       k;
-            for (k = b[w]; k < b[w + 1]; ++k) source_spectrum(o, list[k]);
+            int w, k;
+            for (w = worker_id(); w < o->ranks; w += worker_count())
+                for (k = b[w]; k < b[w + 1]; ++k) source_spectrum(o, list[k]);
         }
         free(b);
         free(ld);
@@ -972,10 +999,10 @@ static void fmm_apply_idx(ostate* o, const double* Source, double* Target, int L
             split_ranges(ld, nt, o->ranks, b);
 #pragma omp parallel num_threads(o->ranks)
             {
-                const int w = omp_get_thread_num();
-                int       k;
-                for (k = b[w]; k < b[w + 1]; ++k)
-                    influence_field(o, tg[k], L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
+                int w, k;
+                for (w = worker_id(); w < o->ranks; w += worker_count())
+                    for (k = b[w]; k < b[w + 1]; ++k)
+                        influence_field(o, tg[k], L, idx, neighbor_only, neighbor_only ? 1.0 : dx_level * dx_level);
             }
             free(b);
             free(ld);
@@ -1164,7 +1191,7 @@ int oracle_lgf_baseline(const oracle_tree* t, const double* g, int nmax, const d
     for (i = 0; i <= nsolves; ++i)
     {
         const size_t all = o->n3 * (size_t)t->nb * sizeof(double);
-        const double t0 = omp_get_wtime();
+        const double t0 = wall_seconds();
         /* every solve starts from cleared work fields, as clean_field does
          * (poisson.hpp:597-600); the kernel spectra stay cached */
         memset(o->source_tmp, 0, all);
@@ -1173,7 +1200,7 @@ int oracle_lgf_baseline(const oracle_tree* t, const double* g, int nmax, const d
         memset(o->fmm_s, 0, all);
         memset(o->fmm_t, 0, all);
         lgf_solve(o, src, dst);
-        if (i > 0) seconds[i - 1] = omp_get_wtime() - t0;
+        if (i > 0) seconds[i - 1] = wall_seconds() - t0;
     }
     ostate_free(o);
     return 0;
@@ -1209,13 +1236,6 @@ int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double
     return 0;
 }
 
-static double now_seconds(void)
-{
-    struct timespec ts;
-    clock_gettime(CLOCK_MONOTONIC, &ts);
-    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
-}
-
 long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const double* fmm_s, int L, int nl,
     int ntargets, const int* targets, int faithful, double* fmm_t, double* seconds)
 {
@@ -1244,7 +1264,7 @@ long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const dou
             lgf_spectrum(&o->cache, &o->dft, &o->ker, j < 27 ? 0 : L - lvl, base);
         }
     }
-    t0 = now_seconds();
+    t0 = wall_seconds();
     for (i = 0; i < ntargets; ++i)
     {
         int s = targets[i];
@@ -1252,7 +1272,7 @@ long oracle_bx_sample(const oracle_tree* t, const double* g, int nmax, const dou
         pairs += influence_field(o, s, L, idx, 0, dx_level * dx_level);
         memcpy(fmm_t + (size_t)i * o->n3, BLK(o->fmm_t, s), sizeof(double) * o->n3);
     }
-    if (seconds) *seconds = now_seconds() - t0;
+    if (seconds) *seconds = wall_seconds() - t0;
     ostate_free(o);
     return pairs;
 }
