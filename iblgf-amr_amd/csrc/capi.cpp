@@ -6,6 +6,7 @@
 #include "gpu_solver.hpp"
 #include "ib_solver.hpp"
 #include "learned_precond.hpp"
+#include "noalloc.hpp"
 #include "ops.hpp"
 
 #include <memory>
@@ -45,7 +46,10 @@ int iblgf_gpu_create(int device, iblgf_gpu_ctx** out)
 {
     if (!out) return fail(IBLGF_ERR_ARG, "iblgf_gpu_create: null output");
     auto ctx = std::make_unique<iblgf_gpu_ctx>();
-    const int rc = guarded(ctx.get(), [&] { ctx->solver = std::make_unique<GpuSolver>(device); });
+    const int rc = guarded(ctx.get(), [&] {
+        ctx->solver = std::make_unique<GpuSolver>(device);  // makes `device` current
+        noalloc_scratch_init(device);
+    });
     if (rc == IBLGF_OK) *out = ctx.release();
     return rc;
 }
@@ -322,6 +326,29 @@ int iblgf_gpu_apply_lgf_typed(iblgf_gpu_ctx* ctx, int src_field, int dst_field, 
     }
     if (fmm_type == IBLGF_FMM_AMR2AMR) return iblgf_gpu_apply_lgf(ctx, src_field, dst_field, kernel, alpha_base);
     return iblgf_gpu_apply_lgf_masked(ctx, src_field, dst_field, kernel, alpha_base, fmm_type);
+}
+
+int iblgf_gpu_apply_lgf_noalloc(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base)
+{
+    NEED_CTX("iblgf_gpu_apply_lgf_noalloc");
+    if (kernel != IBLGF_KERNEL_LAPLACE && kernel != IBLGF_KERNEL_IF)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_noalloc: unknown kernel", ctx);
+    GpuSolver& s = *ctx->solver;
+    const bool laplace = kernel == IBLGF_KERNEL_LAPLACE;
+    const double alpha = laplace ? 0.0 : alpha_base;
+    return guarded(ctx, [&] {
+        const size_t need = s.region_bytes();
+        if (need > kNoallocScratchBytes)
+            throw capi::Unsupported("iblgf_gpu_apply_lgf_noalloc: the solve needs " + std::to_string(need) +
+                                    " bytes of work memory; the fixed scratch holds " +
+                                    std::to_string(kNoallocScratchBytes));
+        if (!s.prepared(laplace, alpha))
+            throw capi::StateError("iblgf_gpu_apply_lgf_noalloc: call iblgf_gpu_prepare(ctx, kernel, alpha_base) "
+                                   "in the current precision first (plans and kernel spectra are allocated there)");
+        void* region = noalloc_scratch(s.device());
+        if (!region) throw capi::StateError("iblgf_gpu_apply_lgf_noalloc: no scratch address for this device");
+        s.solve_in_region(src_field, dst_field, laplace, alpha, region, kNoallocScratchBytes);
+    });
 }
 
 int iblgf_gpu_ib_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t)

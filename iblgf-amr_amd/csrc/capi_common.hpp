@@ -22,6 +22,17 @@ struct iblgf_tree
 namespace iblgf_amd {
 namespace capi {
 
+// Thrown by entry points for a case the device path does not build
+// (IBLGF_ERR_UNSUPPORTED) and for a call in the wrong state (IBLGF_ERR_STATE).
+struct Unsupported : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+struct StateError : std::runtime_error
+{
+    using std::runtime_error::runtime_error;
+};
+
 // Records msg as the thread's last error (and in *ctx_error when given).
 int fail(int code, const std::string& msg, std::string* ctx_error = nullptr);
 // The calling thread's last error message.
@@ -35,6 +46,14 @@ int guarded(std::string* ctx_error, F&& f)
     {
         f();
         return IBLGF_OK;
+    }
+    catch (const Unsupported& e)
+    {
+        return fail(IBLGF_ERR_UNSUPPORTED, e.what(), ctx_error);
+    }
+    catch (const StateError& e)
+    {
+        return fail(IBLGF_ERR_STATE, e.what(), ctx_error);
     }
     catch (const std::invalid_argument& e)
     {

@@ -2,6 +2,8 @@
 // gpu_solver.hpp.
 #include "gpu_solver.hpp"
 
+#include "noalloc.hpp"
+
 #include "ops.hpp"
 
 #include "host/tables.hpp"
@@ -690,7 +692,7 @@ PassPlan& GpuSolver::pass(int L, int nl, bool nbonly)
 
 PassPlan& GpuSolver::pass_idx(int L, int idx, bool nbonly)
 {
-    const int key = (L * 64 + idx) * 2 + (nbonly ? 1 : 0);
+    const int key = pass_key(L, idx, nbonly);
     auto      it = passes_.find(key);
     if (it != passes_.end()) return *it->second;
     auto p = std::make_unique<PassPlan>();
@@ -1054,6 +1056,73 @@ void GpuSolver::apply_if_batch(const int* src, const int* dst, int k, double alp
     run_captured(
         batch_key(1, src, dst, k, alpha_base), [this, alpha_base] { prepare_if(alpha_base); },
         [this, src, dst, k, alpha_base] { if_batch_body(src, dst, k, alpha_base); });
+}
+
+bool GpuSolver::has_ghat(bool laplace, int d, double alpha) const
+{
+    if (precision_ == 0) return laplace ? ghat_lap_.count(d) > 0 : ghat_if_.count(alpha) > 0;
+    return laplace ? ghat_lap32_.count(d) > 0 : ghat_if32_.count(alpha) > 0;
+}
+
+bool GpuSolver::prepared(bool laplace, double alpha_base) const
+{
+    if (!uploaded_) return false;
+    for (int l = base_level_; l < depth_; ++l)
+    {
+        const double alpha = alpha_base * std::pow(4.0, l - base_level_);
+        for (int nl = 0; nl < 2; ++nl)
+        {
+            const auto it = passes_.find(pass_key(l, fmm_mask_idx_gen(MASK_AMR2AMR, l - base_level_, nl), !laplace));
+            if (it == passes_.end()) return false;
+            for (const auto& m : it->second->mac)
+                if (!has_ghat(laplace, m.d, alpha)) return false;
+        }
+    }
+    return true;
+}
+
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t GpuSolver::region_bytes() const
+{
+    const size_t fb = round256(field_size() * sizeof(double));
+    const size_t sb = round256(static_cast<size_t>(nb_) * nf_ * (precision_ == 0 ? sizeof(double2) : sizeof(float2)));
+    return 4 * fb + 2 * sb;
+}
+
+void GpuSolver::solve_in_region(int src, int dst, bool laplace, double alpha_base, void* region, size_t bytes)
+{
+    if (!uploaded_) throw std::runtime_error("tree not uploaded");
+    const size_t need = region_bytes();
+    if (!region || bytes < need)
+        throw std::length_error("solve needs " + std::to_string(need) + " bytes of work memory, the region has " +
+                                std::to_string(bytes));
+    if (!prepared(laplace, alpha_base))
+        throw std::runtime_error("plans or kernel spectra of this solve are not built (prepare it first in this precision)");
+    if (timer_.enabled()) throw std::runtime_error("kernel timing is on (it records events)");
+    field_ptr(src);
+    field_ptr(dst);
+    IBLGF_HIP(hipSetDevice(device_));
+    const size_t fb = round256(field_size() * sizeof(double));
+    const size_t sb = round256(static_cast<size_t>(nb_) * nf_ * (precision_ == 0 ? sizeof(double2) : sizeof(float2)));
+    char*        p = static_cast<char*>(region);
+    // The region keeps whatever the last user left in it.
+    noalloc_clear(stream_, region, need);
+    const WorkPtrs saved = wp_;
+    wp_ = WorkPtrs{reinterpret_cast<double*>(p), reinterpret_cast<double*>(p + fb),
+        reinterpret_cast<double*>(p + 2 * fb), reinterpret_cast<double*>(p + 3 * fb), p + 4 * fb, p + 4 * fb + sb};
+    try
+    {
+        if (laplace) lgf_batch_body(&src, &dst, 1);
+        else
+            if_batch_body(&src, &dst, 1, alpha_base);
+    }
+    catch (...)
+    {
+        wp_ = saved;
+        throw;
+    }
+    wp_ = saved;
 }
 
 void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alpha_base)

@@ -155,6 +155,14 @@ class GpuSolver
     void apply_if_batch(const int* src, const int* dst, int k, double alpha_base);
     // Size the work arrays for k components ahead of a batched solve.
     void reserve_components(int k);
+    // A single-component solve that allocates nothing: its four work fields
+    // and two spectrum arrays are carved from a caller's device region
+    // (region_bytes() bytes, each part 256-byte aligned), which the solve
+    // clears by kernel first.  Needs prepared(): every pass plan and kernel
+    // spectrum of the solve built beforehand in the current precision.
+    size_t region_bytes() const;
+    bool   prepared(bool laplace, double alpha_base) const;
+    void   solve_in_region(int src, int dst, bool laplace, double alpha_base, void* region, size_t bytes);
     static constexpr int kMaxComponents = 64;
     // Batched solves replayed from captured HIP graphs: the first call with a
     // given (kernel, fields, k, precision, alpha) prepares its plans and
@@ -205,6 +213,7 @@ class GpuSolver
     int  precision() const { return precision_; }
 
     void        sync();
+    int         device() const { return device_; }
     hipStream_t stream() const { return stream_; }
     int         nblocks() const { return nb_; }
     int         block_n() const { return n_; }
@@ -216,6 +225,8 @@ class GpuSolver
     void      build_pass(PassPlan& p, int L, int idx, bool neighbor_only);
     PassPlan& pass(int L, int nl, bool neighbor_only);
     PassPlan& pass_idx(int L, int idx, bool neighbor_only);
+    static int pass_key(int L, int idx, bool neighbor_only) { return (L * 64 + idx) * 2 + (neighbor_only ? 1 : 0); }
+    bool      has_ghat(bool laplace, int d, double alpha) const;
     const DevArray<int>& face_clean_list(int kind, int level, bool leaf_only);
     void      run_pass(PassPlan& p, double scale, bool laplace, double alpha, const Batch& b);
     void      lgf_batch_body(const int* src, const int* dst, int k);

@@ -117,6 +117,7 @@ SYMBOLS = {
     "iblgf_gpu_clean_boundary": (_I, [_P, _I, _I, _I, _I, _I]),
     "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_apply_lgf_typed": (_I, [_P, _I, _I, _I, _D, _I]),
+    "iblgf_gpu_apply_lgf_noalloc": (_I, [_P, _I, _I, _I, _D]),
     "iblgf_gpu_ib_create": (_I, [_P, _P]),
     "iblgf_gpu_ib_info": (_I, [_P, _IP, _IP, ctypes.POINTER(ctypes.c_long)]),
     "iblgf_gpu_ib_smear": (_I, [_P, _DP, _IP]),
@@ -434,6 +435,13 @@ class PoissonSolver:
         IblgfError with code IBLGF_ERR_UNSUPPORTED (4)."""
         _check(lib().iblgf_gpu_apply_lgf_typed(self._h, int(src), int(dst), int(kernel), float(alpha_base),
                                                int(fmm_type)), self._h)
+
+    def apply_lgf_noalloc(self, src, dst, kernel=KERNEL_LAPLACE, alpha_base=0.0):
+        """apply_lgf / apply_lgf_IF of one component with its work memory in the
+        library's fixed per-GPU scratch (no device allocation during the call);
+        prepare(kernel, alpha_base) first, in the current precision."""
+        _check(lib().iblgf_gpu_apply_lgf_noalloc(self._h, int(src), int(dst), int(kernel), float(alpha_base)),
+               self._h)
 
     def apply_lgf_masked(self, src, dst, mask_type, kernel=KERNEL_LAPLACE, alpha_base=0.0):
         """apply_lgf / apply_lgf_IF with fmm_type IB2xIB or xIB2IB (finest level, one pass)."""

@@ -330,6 +330,26 @@ enum iblgf_fmm_type
 int iblgf_gpu_apply_lgf_typed(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
     int fmm_type);
 
+/* No counterpart in the reference.  iblgf_gpu_apply_lgf for callers that
+ * cannot allocate device memory during the call (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc): same solve, same
+ * kernels and result, but its work fields and spectra live in a fixed device
+ * array of IBLGF_NOALLOC_SCRATCH_BYTES that each GPU holds for the life of the
+ * process, and the call makes no hipMalloc/hipFree.
+ *  - iblgf_gpu_prepare(ctx, kernel, alpha_base) must have run in the current
+ *    precision (plans and kernel spectra are allocated there); otherwise
+ *    IBLGF_ERR_STATE.
+ *  - A tree whose single-component work exceeds the scratch
+ *    (4 n^3 FP64 fields + 2 spectra of H P^2 complex values per block, each
+ *    array rounded up to 256 bytes: e.g. about 455 blocks at n = 10 in FP64)
+ *    is refused with IBLGF_ERR_UNSUPPORTED.
+ *  - All calls on one GPU share its scratch, which keeps the last call's
+ *    contents (each call clears the part it uses first): calls that overlap
+ *    on one GPU, from any context or stream, race.
+ *  - Kernel timing must be off. */
+#define IBLGF_NOALLOC_SCRATCH_BYTES (64L << 20)
+int iblgf_gpu_apply_lgf_noalloc(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base);
+
 /* ---- immersed-boundary force system (device) ---------------------------- */
 
 /* Sparse delta weights of the tree's immersed-boundary points on the finest

@@ -110,3 +110,9 @@ def test_subtract_non_leaf_refused_from_python():
     L = P.lib()
     assert L.iblgf_gpu_tree_upload(None, t._h) == 4  # IBLGF_ERR_UNSUPPORTED
     assert b"subtract_non_leaf" in L.iblgf_last_error()
+
+
+def test_noalloc_entry_refuses_null_context():
+    L = P.lib()
+    assert L.iblgf_gpu_apply_lgf_noalloc(None, 0, 1, 0, 0.0) == 1
+    assert b"null context" in L.iblgf_last_error()
