@@ -4,6 +4,7 @@
 // citations are in include/iblgf_gpu.h.
 #include "capi_common.hpp"
 
+#include "host/synth.hpp"
 #include "host/tables.hpp"
 
 #include <algorithm>
@@ -279,6 +280,35 @@ int iblgf_lagrange_matrix(int n, int pts_cap, int relative_pos, double* out)
     const auto A = relative_pos == 0 && pts_cap == 10 ? nli_matrix(n, 10) : cc_interp_matrix(n, pts_cap, relative_pos);
     std::copy(A.begin(), A.end(), out);
     return IBLGF_OK;
+}
+
+int iblgf_vortex_rings_refine(void* user, const int block_base[3], int block_extent, int refinement_level,
+    int diff_level)
+{
+    // A refinement condition cannot report an error through its return value:
+    // bad parameters refine nothing (iblgf_vortex_rings_vorticity rejects them).
+    try
+    {
+        const RingSet rs(static_cast<const double*>(user));
+        return rs.refine(block_base, block_extent, refinement_level, diff_level) ? 1 : 0;
+    }
+    catch (...)
+    {
+        return 0;
+    }
+}
+
+int iblgf_vortex_rings_vorticity(const iblgf_tree* t, const double* params, int comp_mask, double* out_x,
+    double* out_y, double* out_z)
+{
+    if (!t || !params || comp_mask <= 0 || comp_mask > 7 || ((comp_mask & 1) && !out_x) ||
+        ((comp_mask & 2) && !out_y) || ((comp_mask & 4) && !out_z))
+        return capi::fail(IBLGF_ERR_ARG, "iblgf_vortex_rings_vorticity: invalid argument");
+    return capi::guarded(nullptr, [&] {
+        const RingSet rs(params);
+        double* const out[3] = {out_x, out_y, out_z};
+        rs.vorticity(t->tree, comp_mask, out);
+    });
 }
 
 }  // extern "C"

@@ -73,6 +73,8 @@ SYMBOLS = {
     "iblgf_tree_import": (_I, [_IP, _I, _I, _D, _I, ctypes.POINTER(_U64), ctypes.POINTER(ctypes.c_uint32),
                                ctypes.POINTER(_P)]),
     "iblgf_tree_init_refine": (_I, [_P, _I, _I, REFINE_FN, _P]),
+    "iblgf_vortex_rings_refine": (_I, [_P, _IP, _I, _I, _I]),
+    "iblgf_vortex_rings_vorticity": (_I, [_P, _DP, _I, _DP, _DP, _DP]),
     "iblgf_tree_build_masks": (_I, [_P, _I]),
     "iblgf_tree_info": (_I, [_P, _IP, _IP, _IP, _IP, _DP, _IP]),
     "iblgf_tree_export": (_I, [_P] + [_P] * 12),
@@ -262,6 +264,25 @@ class Tree:
         if err:
             raise err[0]
         return self
+
+    def init_refine_rings(self, nref, params, level_up_max=0):
+        """Domain::init_refine with the native vortex-ring refinement condition
+        (iblgf_vortex_rings_refine; params as in include/iblgf_gpu.h)."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        fn = ctypes.cast(lib().iblgf_vortex_rings_refine, REFINE_FN)
+        _check(lib().iblgf_tree_init_refine(self._h, int(nref), int(level_up_max), fn, p.ctypes.data_as(_P)))
+        return self
+
+    def vortex_rings_vorticity(self, params, components=(0, 1, 2)):
+        """The listed components of the rings' vorticity on the interior cells of
+        every leaf, [len(components), nblocks, n, n, n] (iblgf_vortex_rings_vorticity)."""
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        nb = self.info()["nblocks"]
+        outs = {c: np.zeros((nb, self.n, self.n, self.n)) for c in set(components)}
+        mask = sum(1 << c for c in outs)
+        ptrs = [outs[c].ctypes.data_as(_DP) if c in outs else None for c in range(3)]
+        _check(lib().iblgf_vortex_rings_vorticity(self._h, p.ctypes.data_as(_DP), int(mask), *ptrs))
+        return np.stack([outs[c] for c in components])
 
     def build_masks(self, subtract_non_leaf=False):
         _check(lib().iblgf_tree_build_masks(self._h, 1 if subtract_non_leaf else 0))

@@ -150,9 +150,11 @@ class VortexRings:
             rings.append((c * self.dx_base, R * self.dx_base, a * self.dx_base, axis, gam))
         self.rings = rings
         self.source_max = max(g / (math.pi * a * a) for (_, _, a, _, g) in rings)
+        self.params = np.array([len(rings), self.source_max, refinement_factor, self.dx_base]
+                               + [v for (c, R, a, axis, gam) in rings for v in (*c, R, a, *axis, gam)])
         self.tree = Tree((-region,) * 3, (2 * region,) * 3, e, self.dx_base,
                          [(-half, -half, -half, region, region, region)])
-        self.tree.init_refine(nref, self._refine)
+        self.tree.init_refine_rings(nref, self.params)
         self.tree.build_masks()
         self.arrays = self.tree.export()
         self.info = self.tree.info()
@@ -212,8 +214,13 @@ class VortexRings:
 
     def sources(self, components=(0, 1, 2)):
         """The listed vorticity components on the interior cells of every leaf,
-        [len(components), nblocks, n, n, n]; the vorticity is evaluated once
-        per leaf, over chunks of leaves at a time."""
+        [len(components), nblocks, n, n, n] (evaluated natively,
+        iblgf_vortex_rings_vorticity)."""
+        return self.tree.vortex_rings_vorticity(self.params, tuple(components))
+
+    def sources_numpy(self, components=(0, 1, 2)):
+        """sources() evaluated with numpy, chunks of leaves at a time (the
+        restatement the native evaluation is checked against)."""
         a = self.arrays
         n = self.tree.n
         nb = self.info["nblocks"]

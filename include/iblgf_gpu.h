@@ -141,6 +141,25 @@ double iblgf_if_kernel(int x, int y, int z, double alpha);
  * (include/iblgf/fmm/fmm_nli.hpp:319-374; cell_center_nli_intrp.hpp:677-787). */
 int iblgf_lagrange_matrix(int n, int pts_cap, int relative_pos, double* out);
 
+/* ---- synthetic vortex-ring workload (host) --------------------------------- */
+
+/* The seeded Gaussian-core vortex rings of the benchmark workload, evaluated
+ * natively; they play the roles of the reference's test-setup source and
+ * refinement condition (tests/poisson/vortexrings.hpp:165-178, 471-520).
+ * params = {nrings, source_max, refinement_factor, dx_base, then per ring:
+ * centre xyz, radius, core radius, unit axis xyz, circulation}, physical
+ * units; omega = circulation / (pi core^2) exp(-rho^2 / core^2) along the
+ * circle's tangent, rho the distance to the core circle.
+ * iblgf_vortex_rings_refine is an iblgf_refine_fn (user = params): refine
+ * where |omega| over the block grown by two cells exceeds
+ * source_max * refinement_factor^diff_level. */
+int iblgf_vortex_rings_refine(void* user, const int block_base[3], int block_extent, int refinement_level,
+    int diff_level);
+/* The components in comp_mask (bit c = component c) of omega on the interior
+ * cells of every leaf block of t, [nb][n^3] in slot order, zeros elsewhere. */
+int iblgf_vortex_rings_vorticity(const iblgf_tree* t, const double* params, int comp_mask, double* out_x,
+    double* out_y, double* out_z);
+
 /* ---- device context ------------------------------------------------------ */
 
 /* Replaces the per-rank cudaSetDevice + Convolution_GPU resources
