@@ -229,6 +229,83 @@ def learned_fcg(seed=1004, alpha=0.5, tol=1e-6):
     return out
 
 
+def forest_step(steps, log):
+    """The batched time-step workload: 64 independent vortex-ring trees of
+    uneven sizes (seed 1003) mirrored into one context; one step = the three
+    IF-HERK stages, each an integrating-factor solve of the 3 vorticity
+    components with the stage's alpha and a Poisson solve of the 3 results,
+    issued as one solve sequence (one HIP graph per step once captured).  The
+    launch shapes are autotuned on the Poisson batch first."""
+    import numpy as np
+
+    from iblgf_amd import KERNEL_IF, KERNEL_LAPLACE, PREC_FP16X3_FORWARD, PoissonSolver, setups
+
+    t0 = time.perf_counter()
+    f = setups.RingForest()
+    srcs = [m.sources() for m in f.members]
+    t_setup = time.perf_counter() - t0
+    log(f"forest: {len(f.trees)} trees, {sum(f.blocks)} blocks ({min(f.blocks)}-{max(f.blocks)} per tree), "
+        f"set-up {t_setup:.1f} s")
+    fs = PoissonSolver(f.trees)
+    fs.set_precision(PREC_FP16X3_FORWARD)
+    w = [fs.field() for _ in range(3)]
+    u = [fs.field() for _ in range(3)]
+    p = [fs.field() for _ in range(3)]
+    for c in range(3):
+        fs.upload(w[c], fs.to_forest([s[c] for s in srcs]))
+    del srcs
+    seq = []
+    for a in f.alphas:
+        seq.append((KERNEL_IF, a, w, u))
+        seq.append((KERNEL_LAPLACE, 0.0, u, p))
+    fs.solve_sequence(seq)  # plans and kernel spectra (cold)
+    fs.sync()
+    report, choice = fs.autotune(u, p, reps=2)
+    log(f"forest autotune: {choice}")
+
+    def run(graphs):
+        fs.set_graphs(graphs)
+        fs.solve_sequence(seq)  # warm-up (and capture)
+        fs.sync()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fs.solve_sequence(seq)
+        fs.sync()
+        dt = (time.perf_counter() - t) / steps
+        fs.set_graphs(False)
+        return dt
+
+    t_eager = run(False)
+    out_eager = fs.download(p[2])
+    t_graph = run(True)
+    same = bool(np.array_equal(fs.download(p[2]), out_eager))
+    solves = len(f.trees) * len(seq) * 3
+    log(f"forest step: {t_eager * 1e3:.1f} ms eager, {t_graph * 1e3:.1f} ms from one graph "
+        f"({solves / t_graph:.0f} scalar solves/s)")
+    return {
+        "workload": ("64 vortex-ring trees (seed 1003; tree i seeded by the i-th draw, sizes from "
+                     "floor(exp(U[ln 200, ln 4000])) targets) in one context; per step 3 IF-HERK stages x "
+                     "(integrating factor of 3 components at the stage's alpha + Poisson of 3 components); "
+                     "precision fp16x3forward; one solve sequence per step"),
+        "trees": len(f.trees),
+        "blocks_total": int(sum(f.blocks)),
+        "blocks_per_tree": [int(b) for b in f.blocks],
+        "target_blocks_per_tree": [int(b) for b in f.targets],
+        "leaves_total": int(sum(f.leaves)),
+        "stage_alphas": f.alphas,
+        "scalar_solves_per_step": solves,
+        "ms_per_step_eager": t_eager * 1e3,
+        "ms_per_step_graph": t_graph * 1e3,
+        "scalar_solves_per_s_graph": solves / t_graph,
+        "leaf_points_per_s_graph": sum(f.leaves) * 16 ** 3 * solves / len(f.trees) / t_graph,
+        "graph_launches_per_step": 1,
+        "graph_bitwise_equal_to_eager": same,
+        "autotune_choice": choice,
+        "autotune_report": report,
+        "setup_s": t_setup,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +315,8 @@ def main():
                     help="budget of the one-thread per-pair-transform sample of the CPU baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ib", action="store_true", help="skip the learned-FCG immersed-boundary comparison")
+    ap.add_argument("--no-forest", action="store_true", help="skip the 64-tree batched time-step workload")
+    ap.add_argument("--forest-only", action="store_true", help="run only the 64-tree batched time-step workload")
     ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph",
                                        "fp16x3forward_batched", "int8x4_batched", "fp16x3_batched", "bf16x3_batched",
                                        "fp16_batched"],
@@ -250,6 +329,12 @@ def main():
               f"(the multi-GPU sharding is a design in DESIGN.md, not built)", file=sys.stderr)
         return 2
     rank = int(os.environ.get("RANK", "0"))
+    if args.forest_only:
+        def flog(msg):
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+        print(json.dumps({"forest_step": forest_step(args.steps, flog)}))
+        return 0
 
     import numpy as np
 
@@ -503,6 +588,9 @@ def main():
     }
     if not args.no_ib:
         result["config"]["learned_fcg"] = learned_fcg()
+    if not args.no_forest:
+        del solver
+        result["config"]["forest_step"] = forest_step(max(2, args.steps // 5), log)
     if rank == 0 and not args.no_cpu_baseline:
         # counts cover the FMM passes of one Laplace solve (each pass plan is built once)
         result["cpu_baseline"] = cpu_baseline(wl, srcs[0], counts, args.cpu_seconds)

@@ -9,10 +9,12 @@
 #include "noalloc.hpp"
 #include "ops.hpp"
 
+#include <algorithm>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 using namespace iblgf_amd;
 
@@ -91,6 +93,46 @@ int iblgf_gpu_tree_upload(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
     });
 }
 
+int iblgf_gpu_forest_upload(iblgf_gpu_ctx* ctx, int ntrees, const iblgf_tree* const* trees)
+{
+    NEED_CTX("iblgf_gpu_forest_upload");
+    if (ntrees < 1 || !trees) return fail(IBLGF_ERR_ARG, "iblgf_gpu_forest_upload: no trees", ctx);
+    std::vector<const Octree*> ts;
+    for (int i = 0; i < ntrees; ++i)
+    {
+        const iblgf_tree* t = trees[i];
+        if (!t) return fail(IBLGF_ERR_ARG, "iblgf_gpu_forest_upload: null tree", ctx);
+        if (!t->masks_built) return fail(IBLGF_ERR_STATE, "iblgf_gpu_forest_upload: masks not built", ctx);
+        if (t->subtract_non_leaf)
+            return fail(IBLGF_ERR_UNSUPPORTED, "iblgf_gpu_forest_upload: masks built with subtract_non_leaf = 1", ctx);
+        ts.push_back(&t->tree);
+    }
+    return guarded(ctx, [&] {
+        ctx->pc.reset();
+        ctx->ib.reset();
+        ctx->solver->upload_forest(ts);
+    });
+}
+
+int iblgf_gpu_forest_slots(iblgf_gpu_ctx* ctx, int tree, int* slots)
+{
+    NEED_CTX("iblgf_gpu_forest_slots");
+    if (!slots) return fail(IBLGF_ERR_ARG, "iblgf_gpu_forest_slots: null output", ctx);
+    return guarded(ctx, [&] {
+        GpuSolver& s = *ctx->solver;
+        if (tree < 0 || tree >= s.num_trees()) throw std::invalid_argument("iblgf_gpu_forest_slots: tree index");
+        // the tree's own slot order: level-major, key-sorted (as iblgf_tree_export)
+        const std::vector<int>& so = s.forest_slot_of(tree);
+        std::vector<std::pair<int, int>> own;  // (combined slot, ...) of the tree's octants with data
+        for (int o = 0; o < (int)so.size(); ++o)
+            if (so[o] >= 0) own.emplace_back(so[o], o);
+        // combined slots keep each tree's level-major key order, so sorting them
+        // gives the tree's own order
+        std::sort(own.begin(), own.end());
+        for (size_t k = 0; k < own.size(); ++k) slots[k] = own[k].first;
+    });
+}
+
 int iblgf_gpu_field_create(iblgf_gpu_ctx* ctx, int* field_id)
 {
     NEED_CTX("iblgf_gpu_field_create");
@@ -155,6 +197,62 @@ int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const i
         else
             ctx->solver->apply_if_batch(src_fields, dst_fields, k, alpha_base);
     });
+}
+
+int iblgf_gpu_solve_sequence(iblgf_gpu_ctx* ctx, int nitems, const iblgf_solve_item* items)
+{
+    NEED_CTX("iblgf_gpu_solve_sequence");
+    if (nitems < 1 || !items) return fail(IBLGF_ERR_ARG, "iblgf_gpu_solve_sequence: no items", ctx);
+    std::vector<GpuSolver::SolveItem> v(nitems);
+    for (int i = 0; i < nitems; ++i)
+    {
+        const iblgf_solve_item& it = items[i];
+        if ((it.kernel != IBLGF_KERNEL_LAPLACE && it.kernel != IBLGF_KERNEL_IF) || it.k < 1 ||
+            it.k > GpuSolver::kMaxComponents || !it.src_fields || !it.dst_fields)
+            return fail(IBLGF_ERR_ARG, "iblgf_gpu_solve_sequence: item " + std::to_string(i) + " is invalid", ctx);
+        v[i].laplace = it.kernel == IBLGF_KERNEL_LAPLACE;
+        v[i].alpha = it.alpha_base;
+        v[i].src.assign(it.src_fields, it.src_fields + it.k);
+        v[i].dst.assign(it.dst_fields, it.dst_fields + it.k);
+    }
+    return guarded(ctx, [&] { ctx->solver->solve_sequence(v); });
+}
+
+int iblgf_gpu_autotune(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
+    double alpha_base, int reps, char* report, int report_len)
+{
+    NEED_CTX("iblgf_gpu_autotune");
+    if (!src_fields || !dst_fields || k < 1 || k > GpuSolver::kMaxComponents ||
+        (kernel != IBLGF_KERNEL_LAPLACE && kernel != IBLGF_KERNEL_IF) || reps < 1 || (report_len > 0 && !report))
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_autotune: invalid argument", ctx);
+    return guarded(ctx, [&] {
+        const std::string r =
+            ctx->solver->autotune(src_fields, dst_fields, k, kernel == IBLGF_KERNEL_LAPLACE, alpha_base, reps);
+        if (report_len > 0)
+        {
+            const size_t n = std::min(r.size(), static_cast<size_t>(report_len - 1));
+            std::copy(r.begin(), r.begin() + static_cast<long>(n), report);
+            report[n] = '\0';
+        }
+    });
+}
+
+int iblgf_gpu_set_launch_shapes(iblgf_gpu_ctx* ctx, int mac_policy, int dft_grid)
+{
+    NEED_CTX("iblgf_gpu_set_launch_shapes");
+    if (mac_policy < 0 || mac_policy > 1 || dft_grid < 0 || dft_grid > 1)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_set_launch_shapes: shape out of range", ctx);
+    return guarded(ctx, [&] { ctx->solver->set_launch_shapes(mac_policy, dft_grid); });
+}
+
+int iblgf_gpu_launch_shapes(iblgf_gpu_ctx* ctx, int* mac_policy, int* dft_grid, int* streams)
+{
+    NEED_CTX("iblgf_gpu_launch_shapes");
+    if (!mac_policy || !dft_grid || !streams) return fail(IBLGF_ERR_ARG, "iblgf_gpu_launch_shapes: null output", ctx);
+    *mac_policy = ctx->solver->mac_policy();
+    *dft_grid = ctx->solver->dft_grid();
+    *streams = ctx->solver->concurrency() ? 2 : 1;
+    return IBLGF_OK;
 }
 
 int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on)
@@ -355,6 +453,8 @@ int iblgf_gpu_ib_create(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
 {
     NEED_CTX("iblgf_gpu_ib_create");
     if (!t || !t->has_ib) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_create: tree without immersed-boundary points", ctx);
+    if (ctx->solver->num_trees() != 1)
+        return fail(IBLGF_ERR_UNSUPPORTED, "iblgf_gpu_ib_create: the context holds a forest (one tree only)", ctx);
     return guarded(ctx, [&] {
         ctx->pc.reset();
         ctx->ib.reset();

@@ -719,7 +719,7 @@ void fwd(hipStream_t s, const double* f, const double* fin, int split, const int
     const Batch& b)
 {
     const long items = static_cast<long>(count) * b.k;
-    hipLaunchKernelGGL((k_forward_mfma<N, NUM>), dim3(resident(k_forward_mfma<N, NUM>, items)),
+    hipLaunchKernelGGL((k_forward_mfma<N, NUM>), dim3(b.dft_grid ? items : resident(k_forward_mfma<N, NUM>, items)),
         dim3(kMfmaThreads), 0, s, f, fin, split, slots, count, b.k, spec, b.fstride, b.sstride);
 }
 template<int N, class NUM>
@@ -727,7 +727,7 @@ void inv(hipStream_t s, const float2* spec, const int* slots, int count, double*
     const Batch& b)
 {
     const long items = static_cast<long>(count) * b.k;
-    hipLaunchKernelGGL((k_inverse_mfma<N, NUM>), dim3(resident(k_inverse_mfma<N, NUM>, items)),
+    hipLaunchKernelGGL((k_inverse_mfma<N, NUM>), dim3(b.dft_grid ? items : resident(k_inverse_mfma<N, NUM>, items)),
         dim3(kMfmaThreads), 0, s, spec, slots, count, b.k, f, scale, acc, b.sstride, b.fstride);
 }
 

@@ -151,10 +151,21 @@ void GpuSolver::sync() { IBLGF_HIP(hipStreamSynchronize(stream_)); }
 // ---------------------------------------------------------------------------
 // Tree upload: slot table, plans, matrices, LGF table.
 // ---------------------------------------------------------------------------
-void GpuSolver::upload_tree(const Octree& t)
+void GpuSolver::upload_tree(const Octree& t) { upload_forest({&t}); }
+
+// Several independent trees in one slot table: slots are level-major, then by
+// tree, then key-sorted, so every block list of a solve (and every launch)
+// covers the blocks of all trees; lists and masks stay per tree.
+void GpuSolver::upload_forest(const std::vector<const Octree*>& trees)
 {
+    if (trees.empty()) throw std::invalid_argument("no tree to upload");
+    const Octree& t = *trees[0];
+    for (const Octree* u : trees)
+        if (!u || u->block_extent != t.block_extent || u->base_level != t.base_level || u->dx_base != t.dx_base)
+            throw std::invalid_argument("forest trees must share block extent, base level and dx_base");
     IBLGF_HIP(hipSetDevice(device_));
     tree_ = t;
+    ntrees_ = static_cast<int>(trees.size());
     n_ = t.block_extent + 2;
     if (!launch::supported_n(n_)) throw std::runtime_error("unsupported block extent for the device path");
     e_ = t.block_extent;
@@ -162,23 +173,31 @@ void GpuSolver::upload_tree(const Octree& t)
     const int P = 2 * n_ - 1, H = P / 2 + 1;
     nf_ = H * P * P;
     base_level_ = t.base_level;
-    depth_ = t.depth;
+    depth_ = 0;
+    for (const Octree* u : trees) depth_ = std::max(depth_, u->depth);
     dx_base_ = t.dx_base;
 
-    // slots: octants with data, level-major, key-sorted within the level
+    // slots: octants with data, level-major, then tree, key-sorted within
     oct_of_.clear();
+    tree_of_.clear();
     lvl_begin_.assign(depth_ + 2, 0);
     for (int l = 0; l < depth_; ++l)
     {
         lvl_begin_[l] = static_cast<int>(oct_of_.size());
-        if (l < (int)t.level_map.size())
-            for (int o : t.level_map[l]) oct_of_.push_back(o);
+        for (int i = 0; i < ntrees_; ++i)
+            if (l < (int)trees[i]->level_map.size())
+                for (int o : trees[i]->level_map[l])
+                {
+                    oct_of_.push_back(o);
+                    tree_of_.push_back(i);
+                }
     }
     lvl_begin_[depth_] = lvl_begin_[depth_ + 1] = static_cast<int>(oct_of_.size());
     nb_ = static_cast<int>(oct_of_.size());
-    slot_of_.assign(t.oct.size(), -1);
-    for (int s = 0; s < nb_; ++s) slot_of_[oct_of_[s]] = s;
-    auto sl = [&](int o) { return o < 0 ? -1 : slot_of_[o]; };
+    forest_slot_of_.assign(ntrees_, {});
+    for (int i = 0; i < ntrees_; ++i) forest_slot_of_[i].assign(trees[i]->oct.size(), -1);
+    for (int s = 0; s < nb_; ++s) forest_slot_of_[tree_of_[s]][oct_of_[s]] = s;
+    slot_of_ = forest_slot_of_[0];
 
     key_.resize(nb_);
     level_.resize(nb_);
@@ -195,11 +214,14 @@ void GpuSolver::upload_tree(const Octree& t)
     n_leaf_ = 0;
     for (int s = 0; s < nb_; ++s)
     {
-        const Octant& o = t.oct[oct_of_[s]];
+        const Octree&           u = *trees[tree_of_[s]];
+        const std::vector<int>& so = forest_slot_of_[tree_of_[s]];
+        auto                    sl = [&](int o) { return o < 0 ? -1 : so[o]; };
+        const Octant&           o = u.oct[oct_of_[s]];
         key_[s] = o.key;
         level_[s] = o.level;
         int b[3];
-        t.block_base(oct_of_[s], b);
+        u.block_base(oct_of_[s], b);
         for (int d = 0; d < 3; ++d)
         {
             coord_[3 * s + d] = o.coord[d];
@@ -207,11 +229,11 @@ void GpuSolver::upload_tree(const Octree& t)
         }
         flags_[s] = (o.leaf ? 1u : 0u) | (o.correction ? 2u : 0u) | (o.is_ib ? 4u : 0u) | (o.extended_ib ? 8u : 0u);
         if (o.leaf) ++n_leaf_;
-        parent_[s] = (o.parent >= 0 && t.oct[o.parent].has_data) ? sl(o.parent) : -1;
+        parent_[s] = (o.parent >= 0 && u.oct[o.parent].has_data) ? sl(o.parent) : -1;
         for (int c = 0; c < 8; ++c)
         {
             const int ch = o.children[c];
-            children_[8 * s + c] = (ch >= 0 && t.oct[ch].has_data) ? sl(ch) : -1;
+            children_[8 * s + c] = (ch >= 0 && u.oct[ch].has_data) ? sl(ch) : -1;
         }
         for (int i = 0; i < 27; ++i) neighbor_[27 * s + i] = sl(o.neighbor[i]);
         infl_num_[s] = o.influence_num;
@@ -392,6 +414,8 @@ Batch GpuSolver::batch(int k) const
 {
     Batch b;
     b.k = k;
+    b.mac_policy = mac_policy_;
+    b.dft_grid = dft_grid_;
     b.fstride = static_cast<long>(field_size());
     b.sstride = static_cast<long>(nb_) * nf_;
     return b;
@@ -487,7 +511,7 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
             std::unordered_map<int, std::vector<int>> by_slot;
         };
         std::vector<GroupBuild>                gb;
-        std::unordered_map<uint64_t, int>      gidx;
+        std::map<std::pair<int, uint64_t>, int> gidx;
         long                                   pairs_here = 0;
         for (int t = lvl_begin_[lam]; t < lvl_begin_[lam + 1]; ++t)
         {
@@ -507,8 +531,9 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
                 }
             if (srcs.empty()) continue;
             inv.push_back(t);
-            const uint64_t pk = Morton::parent(key_[t]);
-            auto           it = gidx.find(pk);
+            // sibling groups are keyed by (tree, parent key): keys repeat across a forest
+            const std::pair<int, uint64_t> pk(tree_of_[t], Morton::parent(key_[t]));
+            auto                           it = gidx.find(pk);
             int            gi;
             if (it == gidx.end())
             {
@@ -1045,6 +1070,124 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
         timer_.end(stream_, KC_OTHER, 0);
     }
     IBLGF_HIP(hipGetLastError());
+}
+
+void GpuSolver::solve_sequence(const std::vector<SolveItem>& items)
+{
+    if (items.empty()) return;
+    int kmax = 1;
+    for (const SolveItem& it : items)
+    {
+        if (it.src.size() != it.dst.size() || it.src.empty()) throw std::invalid_argument("solve item fields");
+        check_batch(it.src.data(), it.dst.data(), static_cast<int>(it.src.size()));
+        kmax = std::max(kmax, static_cast<int>(it.src.size()));
+    }
+    reserve_components(kmax);
+    std::vector<int64_t> key = {2, static_cast<int64_t>(items.size())};
+    for (const SolveItem& it : items)
+    {
+        const auto k = batch_key(it.laplace ? 0 : 1, it.src.data(), it.dst.data(), static_cast<int>(it.src.size()),
+            it.laplace ? 0.0 : it.alpha);
+        key.insert(key.end(), k.begin(), k.end());
+    }
+    run_captured(
+        key,
+        [this, &items] {
+            for (const SolveItem& it : items)
+            {
+                if (it.laplace) prepare_laplace();
+                else
+                    prepare_if(it.alpha);
+            }
+        },
+        [this, &items] {
+            for (const SolveItem& it : items)
+            {
+                const int k = static_cast<int>(it.src.size());
+                if (it.laplace) lgf_batch_body(it.src.data(), it.dst.data(), k);
+                else
+                    if_batch_body(it.src.data(), it.dst.data(), k, it.alpha);
+            }
+        });
+}
+
+void GpuSolver::set_launch_shapes(int mac_policy, int dft_grid)
+{
+    if (mac_policy < 0 || mac_policy > 1 || dft_grid < 0 || dft_grid > 1)
+        throw std::invalid_argument("launch shape out of range");
+    if (mac_policy == mac_policy_ && dft_grid == dft_grid_) return;
+    sync();
+    drop_graphs();  // captured graphs hold the old launch shapes
+    mac_policy_ = mac_policy;
+    dft_grid_ = dft_grid;
+}
+
+std::string GpuSolver::autotune(const int* src, const int* dst, int k, bool laplace, double alpha_base, int reps)
+{
+    if (reps < 1) throw std::invalid_argument("autotune: reps must be positive");
+    check_batch(src, dst, k);
+    if (laplace) prepare_laplace();
+    else
+        prepare_if(alpha_base);
+    const bool graphs = graphs_, timing = timer_.enabled();
+    graphs_ = false;
+    timer_.enable(false);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    IBLGF_HIP(hipEventCreate(&e0));
+    IBLGF_HIP(hipEventCreate(&e1));
+    auto solve = [&] {
+        if (laplace) lgf_batch_body(src, dst, k);
+        else
+            if_batch_body(src, dst, k, alpha_base);
+    };
+    std::string report;
+    double      best = -1.0;
+    int         bm = mac_policy_, bd = dft_grid_;
+    bool        bc = concurrent_;
+    try
+    {
+        for (int conc = 1; conc >= 0; --conc)
+            for (int mp = 0; mp < 2; ++mp)
+                for (int dg = 0; dg < 2; ++dg)
+                {
+                    set_concurrency(conc != 0);
+                    set_launch_shapes(mp, dg);
+                    solve();  // warm-up
+                    IBLGF_HIP(hipEventRecord(e0, stream_));
+                    for (int r = 0; r < reps; ++r) solve();
+                    IBLGF_HIP(hipEventRecord(e1, stream_));
+                    IBLGF_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    IBLGF_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    const double per = ms / reps;
+                    report += "mac_policy=" + std::to_string(mp) + " dft_grid=" + std::to_string(dg) +
+                              " streams=" + std::to_string(conc ? 2 : 1) + " ms_per_call=" + std::to_string(per) + "\n";
+                    if (best < 0 || per < best)
+                    {
+                        best = per;
+                        bm = mp;
+                        bd = dg;
+                        bc = conc != 0;
+                    }
+                }
+    }
+    catch (...)
+    {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        graphs_ = graphs;
+        timer_.enable(timing);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_concurrency(bc);
+    set_launch_shapes(bm, bd);
+    graphs_ = graphs;
+    timer_.enable(timing);
+    report += "chosen: mac_policy=" + std::to_string(bm) + " dft_grid=" + std::to_string(bd) +
+              " streams=" + std::to_string(bc ? 2 : 1) + " ms_per_call=" + std::to_string(best) + "\n";
+    return report;
 }
 
 // PoissonSolver::apply_if (include/iblgf/solver/poisson/poisson.hpp:435-494)

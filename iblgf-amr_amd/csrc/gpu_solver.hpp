@@ -135,6 +135,13 @@ class GpuSolver
     ~GpuSolver();
 
     void upload_tree(const Octree& t);
+    // Independent trees of one block extent, base level and dx_base in one
+    // slot table (level-major, then tree, then key): every solve covers all
+    // trees in each launch, each tree's result that of its own solve.
+    void upload_forest(const std::vector<const Octree*>& trees);
+    int  num_trees() const { return ntrees_; }
+    // Combined slot of each octant of tree i (-1 without data).
+    const std::vector<int>& forest_slot_of(int i) const { return forest_slot_of_.at(i); }
 
     int     field_create();
     double* field_ptr(int id);
@@ -155,6 +162,26 @@ class GpuSolver
     void apply_if_batch(const int* src, const int* dst, int k, double alpha_base);
     // Size the work arrays for k components ahead of a batched solve.
     void reserve_components(int k);
+    // Batched solves issued in order (each as apply_lgf_batch / apply_if_batch);
+    // with graphs on, the whole sequence is one captured graph, e.g. every
+    // solve of one time step.
+    struct SolveItem
+    {
+        bool             laplace = true;
+        double           alpha = 0.0;
+        std::vector<int> src, dst;
+    };
+    void solve_sequence(const std::vector<SolveItem>& items);
+    // Times apply_lgf_batch(src, dst, k) (Laplace, or IF with alpha_base)
+    // under every combination of the launch shapes that do not change results
+    // (multiply-accumulate grid policy, transform grid policy, one or two
+    // streams per FMM pass), `reps` eager solves each after a warm-up, keeps
+    // the fastest for later solves and returns a report of every timing.
+    std::string autotune(const int* src, const int* dst, int k, bool laplace, double alpha_base, int reps);
+    void        set_launch_shapes(int mac_policy, int dft_grid);
+    int         mac_policy() const { return mac_policy_; }
+    int         dft_grid() const { return dft_grid_; }
+    bool        concurrency() const { return concurrent_; }
     // A single-component solve that allocates nothing: its four work fields
     // and two spectrum arrays are carved from a caller's device region
     // (region_bytes() bytes, each part 256-byte aligned), which the solve
@@ -252,6 +279,7 @@ class GpuSolver
     hipStream_t side_ = nullptr;  // the coarse chain of an FMM pass
     hipEvent_t  ev_fork_ = nullptr, ev_join_ = nullptr;
     bool        concurrent_ = true;
+    int         mac_policy_ = 0, dft_grid_ = 0;  // launch shapes (Batch)
     bool        uploaded_ = false;
     KernelTimer timer_;
 
@@ -260,7 +288,10 @@ class GpuSolver
     int                     nb_ = 0, n_ = 0, e_ = 0, n3_ = 0, nf_ = 0;
     int                     base_level_ = 0, depth_ = 0, n_leaf_ = 0;
     double                  dx_base_ = 1.0;
-    std::vector<int>        slot_of_;   // octant index -> slot
+    int                     ntrees_ = 1;
+    std::vector<int>        slot_of_;   // octant index -> slot (first tree)
+    std::vector<std::vector<int>> forest_slot_of_;  // per tree: octant index -> slot
+    std::vector<int>        tree_of_;   // slot -> tree
     std::vector<int>        oct_of_;    // slot -> octant index
     std::vector<int>        lvl_begin_;
     std::vector<uint64_t>   key_;

@@ -1205,12 +1205,12 @@ template<int N> struct RunFwd
         const long items = static_cast<long>(count) * b.k;
         if (prec == 0)
             hipLaunchKernelGGL((k_forward_dft<N, double>),
-                dim3(resident_grid(k_forward_dft<N, double>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, fin,
-                split, slots, count, b.k, (double2*)spec, b.fstride, b.sstride);
+                dim3(b.dft_grid ? items : resident_grid(k_forward_dft<N, double>, kFwdThreads, items)),
+                dim3(kFwdThreads), 0, s, f, fin, split, slots, count, b.k, (double2*)spec, b.fstride, b.sstride);
         else
             hipLaunchKernelGGL((k_forward_dft<N, float>),
-                dim3(resident_grid(k_forward_dft<N, float>, kFwdThreads, items)), dim3(kFwdThreads), 0, s, f, fin,
-                split, slots, count, b.k, (float2*)spec, b.fstride, b.sstride);
+                dim3(b.dft_grid ? items : resident_grid(k_forward_dft<N, float>, kFwdThreads, items)),
+                dim3(kFwdThreads), 0, s, f, fin, split, slots, count, b.k, (float2*)spec, b.fstride, b.sstride);
     }
 };
 template<int N> struct RunInv
@@ -1241,9 +1241,9 @@ inline int mac_threads(long nvg, int kf)
 // many (each takes whole chunks in rounds and an equal share of the chunks
 // left over); every other launch gives each workgroup one chunk.
 template<class F>
-inline int mac_grid(F kernel, int nt, int nchunk)
+inline int mac_grid(F kernel, int nt, int nchunk, int policy)
 {
-    return nt < kMacMaxThreads ? nchunk : resident_grid(kernel, nt, nchunk);
+    return (nt < kMacMaxThreads || policy == 1) ? nchunk : resident_grid(kernel, nt, nchunk);
 }
 
 template<int N> struct RunMac
@@ -1255,7 +1255,7 @@ template<int N> struct RunMac
     do                                                                                                           \
     {                                                                                                            \
         const int nt = mac_threads((long)ng * b.k, KF);                                                          \
-        const int nwg = mac_grid(k_spectral_mac<N, KF, CP, T>, nt, (Geo<N>::NF + KF - 1) / KF);                 \
+        const int nwg = mac_grid(k_spectral_mac<N, KF, CP, T>, nt, (Geo<N>::NF + KF - 1) / KF, b.mac_policy);   \
         hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3(nwg), dim3(nt), 0, s, g, ng, src, rowoff, olist,  \
             nused, (const C*)ghat, (const C*)ss, (C*)ts, b.k, b.sstride, b.sstride);                      \
     } while (0)

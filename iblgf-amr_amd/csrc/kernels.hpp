@@ -42,6 +42,14 @@ struct Batch
     int  k = 1;
     long fstride = 0;
     long sstride = 0;
+    // Launch shapes chosen by the solver's autotuner (results do not depend
+    // on them): mac_policy 0 spreads a full-size multiply-accumulate launch
+    // over the resident workgroups in rounds, 1 gives every frequency chunk a
+    // workgroup; dft_grid 0 runs the block transforms on the resident
+    // workgroups looping over blocks, 1 gives every (block, component) a
+    // workgroup.
+    int mac_policy = 0;
+    int dft_grid = 0;
 };
 
 // Per-component field base pointers, passed by value (component c in p[c]).

@@ -119,6 +119,12 @@ SYMBOLS = {
     "iblgf_gpu_clean_boundary": (_I, [_P, _I, _I, _I, _I, _I]),
     "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_apply_lgf_typed": (_I, [_P, _I, _I, _I, _D, _I]),
+    "iblgf_gpu_forest_upload": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "iblgf_gpu_solve_sequence": (_I, [_P, _I, _P]),
+    "iblgf_gpu_autotune": (_I, [_P, _IP, _IP, _I, _I, _D, _I, ctypes.c_char_p, _I]),
+    "iblgf_gpu_launch_shapes": (_I, [_P, _IP, _IP, _IP]),
+    "iblgf_gpu_set_launch_shapes": (_I, [_P, _I, _I]),
+    "iblgf_gpu_forest_slots": (_I, [_P, _I, _IP]),
     "iblgf_gpu_apply_lgf_noalloc": (_I, [_P, _I, _I, _I, _D]),
     "iblgf_gpu_ib_create": (_I, [_P, _P]),
     "iblgf_gpu_ib_info": (_I, [_P, _IP, _IP, ctypes.POINTER(ctypes.c_long)]),
@@ -314,20 +320,53 @@ class Tree:
 # ---------------------------------------------------------------------------
 # Device solver
 # ---------------------------------------------------------------------------
+class SolveItem(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int), ("alpha_base", ctypes.c_double), ("k", ctypes.c_int),
+                ("src_fields", _IP), ("dst_fields", _IP)]
+
+
 class PoissonSolver:
-    """PoissonSolver on one GPU; fields are [nblocks, n, n, n] FP64 (x fastest)."""
+    """PoissonSolver on one GPU; fields are [nblocks, n, n, n] FP64 (x fastest).
+
+    Given a list of trees, the context mirrors them as one forest
+    (iblgf_gpu_forest_upload): fields then hold the blocks of every tree in the
+    combined slot order, and to_forest / from_forest convert between per-tree
+    arrays and forest fields."""
 
     def __init__(self, tree, device=0):
         L = lib()
         h = ctypes.c_void_p()
         _check(L.iblgf_gpu_create(int(device), ctypes.byref(h)))
         self._h = h
-        self.tree = tree
-        _check(L.iblgf_gpu_tree_upload(self._h, tree._h), self._h)
-        info = tree.info()
-        self.nblocks = info["nblocks"]
-        self.n = tree.n
+        trees = list(tree) if isinstance(tree, (list, tuple)) else [tree]
+        self.trees = trees
+        self.tree = trees[0]
+        self.n = trees[0].n
+        if len(trees) == 1 and not isinstance(tree, (list, tuple)):
+            _check(L.iblgf_gpu_tree_upload(self._h, trees[0]._h), self._h)
+            self.nblocks = trees[0].info()["nblocks"]
+            self.slots = [np.arange(self.nblocks)]
+        else:
+            arr = (_P * len(trees))(*[t._h for t in trees])
+            _check(L.iblgf_gpu_forest_upload(self._h, len(trees), arr), self._h)
+            self.slots = []
+            for i, t in enumerate(trees):
+                sl = np.empty(t.info()["nblocks"], np.int32)
+                _check(L.iblgf_gpu_forest_slots(self._h, i, sl.ctypes.data_as(_IP)), self._h)
+                self.slots.append(sl)
+            self.nblocks = int(sum(len(s) for s in self.slots))
         self.shape = (self.nblocks, self.n, self.n, self.n)
+
+    def to_forest(self, arrays):
+        """Per-tree arrays [nb_i, n, n, n] -> one forest field array."""
+        out = np.zeros(self.shape)
+        for sl, a in zip(self.slots, arrays):
+            out[sl] = a
+        return out
+
+    def from_forest(self, arr):
+        """A forest field array -> per-tree arrays in each tree's own slot order."""
+        return [arr[sl] for sl in self.slots]
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:
@@ -456,6 +495,36 @@ class PoissonSolver:
         IblgfError with code IBLGF_ERR_UNSUPPORTED (4)."""
         _check(lib().iblgf_gpu_apply_lgf_typed(self._h, int(src), int(dst), int(kernel), float(alpha_base),
                                                int(fmm_type)), self._h)
+
+    def solve_sequence(self, items):
+        """items: (kernel, alpha_base, srcs, dsts) tuples, issued in order as
+        apply_lgf_batch calls; with graphs on, one captured graph for all."""
+        keep = []
+        arr = (SolveItem * len(items))()
+        for i, (kernel, alpha, srcs, dsts) in enumerate(items):
+            a = (ctypes.c_int * len(srcs))(*srcs)
+            b = (ctypes.c_int * len(dsts))(*dsts)
+            keep += [a, b]
+            arr[i] = SolveItem(int(kernel), float(alpha), len(srcs), ctypes.cast(a, _IP), ctypes.cast(b, _IP))
+        _check(lib().iblgf_gpu_solve_sequence(self._h, len(items), ctypes.cast(arr, _P)), self._h)
+
+    def autotune(self, srcs, dsts, kernel=KERNEL_LAPLACE, alpha_base=0.0, reps=3):
+        """Time the batched solve under every result-preserving launch shape and
+        keep the fastest (iblgf_gpu_autotune); returns (report, choice)."""
+        a = (ctypes.c_int * len(srcs))(*srcs)
+        b = (ctypes.c_int * len(dsts))(*dsts)
+        buf = ctypes.create_string_buffer(4096)
+        _check(lib().iblgf_gpu_autotune(self._h, ctypes.cast(a, _IP), ctypes.cast(b, _IP), len(srcs), int(kernel),
+                                        float(alpha_base), int(reps), buf, len(buf)), self._h)
+        return buf.value.decode(), self.launch_shapes()
+
+    def set_launch_shapes(self, mac_policy, dft_grid):
+        _check(lib().iblgf_gpu_set_launch_shapes(self._h, int(mac_policy), int(dft_grid)), self._h)
+
+    def launch_shapes(self):
+        m, d, s = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().iblgf_gpu_launch_shapes(self._h, ctypes.byref(m), ctypes.byref(d), ctypes.byref(s)), self._h)
+        return {"mac_policy": m.value, "dft_grid": d.value, "streams": s.value}
 
     def apply_lgf_noalloc(self, src, dst, kernel=KERNEL_LAPLACE, alpha_base=0.0):
         """apply_lgf / apply_lgf_IF of one component with its work memory in the

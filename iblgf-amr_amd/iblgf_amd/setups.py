@@ -128,12 +128,14 @@ class VortexRings:
     """
 
     def __init__(self, seed=1001, nrings=16, block_extent=16, base_blocks=8, nref=3, refinement_factor=0.9,
-                 centre_range=40.0, radius=(8.0, 24.0), core=(2.0, 4.0), circulation=(0.5, 1.5)):
+                 centre_range=40.0, radius=(8.0, 24.0), core=(2.0, 4.0), circulation=(0.5, 1.5), box_blocks=None,
+                 dx_base=None):
         e = block_extent
         region = base_blocks * e
         half = region // 2
+        box = (box_blocks if box_blocks is not None else base_blocks) * e
         self.block_extent = e
-        self.dx_base = 1.0 / region
+        self.dx_base = float(dx_base) if dx_base is not None else 1.0 / region
         self.nref = nref
         self.refinement_factor = refinement_factor
         rng = SplitMix64(seed)
@@ -152,7 +154,7 @@ class VortexRings:
         self.source_max = max(g / (math.pi * a * a) for (_, _, a, _, g) in rings)
         self.params = np.array([len(rings), self.source_max, refinement_factor, self.dx_base]
                                + [v for (c, R, a, axis, gam) in rings for v in (*c, R, a, *axis, gam)])
-        self.tree = Tree((-region,) * 3, (2 * region,) * 3, e, self.dx_base,
+        self.tree = Tree((-box,) * 3, (2 * box,) * 3, e, self.dx_base,
                          [(-half, -half, -half, region, region, region)])
         self.tree.init_refine_rings(nref, self.params)
         self.tree.build_masks()
@@ -250,6 +252,77 @@ class VortexRings:
     def source(self, component):
         """Component `component` of the vorticity on the interior cells of every leaf."""
         return self.sources((component,))[0]
+
+
+# ---------------------------------------------------------------------------
+# A forest of independent vortex-ring trees (the batched time-step workload)
+# ---------------------------------------------------------------------------
+class RingForest:
+    """Independent adaptive vortex-ring trees of uneven sizes, stepped together.
+
+    From one SplitMix64 stream (seed 1003 for the 64-tree workload), tree i
+    draws its own seed (the i-th 64-bit draw) and then a target block count
+    floor(exp(U[ln lo, ln hi])); its ring count scales the 16 rings of the
+    4290-block single-tree workload to that target (the actual count is
+    recorded).  After the trees, the stream draws one integrating-factor
+    alpha = c dt / dx^2 ~ U[0.1, 2] per IF-HERK stage (three): the trees step
+    with one time step, so a stage's alpha is shared by all of them.
+    Every tree has the single-tree workload's box, base level and dx_base.
+    """
+
+    def __init__(self, ntrees=64, seed=1003, blocks=(200, 4000), block_extent=16, box_blocks=8, search_steps=8):
+        rng = SplitMix64(seed)
+        self.members, self.targets, self.seeds = [], [], []
+        for _ in range(ntrees):
+            tseed = rng.next()
+            target = int(math.floor(math.exp(rng.uniform(math.log(blocks[0]), math.log(blocks[1])))))
+            self.seeds.append(tseed)
+            self.targets.append(target)
+            self.members.append(self._sized(tseed, target, block_extent, box_blocks, search_steps))
+        self.alphas = [rng.uniform(0.1, 2.0) for _ in range(3)]
+        self.trees = [m.tree for m in self.members]
+        self.blocks = [m.info["nblocks"] for m in self.members]
+        self.leaves = [m.info["nleaves"] for m in self.members]
+
+    @staticmethod
+    def _sized(tseed, target, e, box_blocks, steps):
+        """A ring tree near `target` blocks.  The base region (4, 6 or 8 blocks
+        a side, centred in the shared box) and the ring count (one per 400
+        target blocks, 1 to 8) follow the target; then, from 3 refinement levels
+        down to 1 (fewer while even the least refinement overshoots), a
+        bisection on the refinement threshold source_max * rf^diff over
+        ln(-ln rf).  The closest tree is kept."""
+        b = 4 if target < 900 else (6 if target < 2500 else 8)
+        nrings = min(8, max(1, int(round(target / 400.0))))
+
+        def build(nref, q):
+            return VortexRings(seed=tseed, nrings=nrings, block_extent=e, base_blocks=b, box_blocks=box_blocks,
+                               dx_base=1.0 / (box_blocks * e), nref=nref, refinement_factor=math.exp(-q),
+                               centre_range=40.0 * b / 8, radius=(8.0 * b / 8, 24.0 * b / 8))
+
+        best = None
+
+        def keep(v):
+            nonlocal best
+            if best is None or abs(v.info["nblocks"] - target) < abs(best.info["nblocks"] - target):
+                best = v
+
+        for nref in (3, 2, 1):
+            lo, hi = math.log(1e-5), math.log(4.0)  # ln q: fewer blocks at lo
+            v = build(nref, math.exp(lo))
+            keep(v)
+            if v.info["nblocks"] > 1.3 * target and nref > 1:
+                continue
+            for _ in range(steps):
+                mid = 0.5 * (lo + hi)
+                v = build(nref, math.exp(mid))
+                keep(v)
+                if v.info["nblocks"] < target:
+                    lo = mid
+                else:
+                    hi = mid
+            break
+        return best
 
 
 # ---------------------------------------------------------------------------

@@ -175,6 +175,17 @@ const char* iblgf_last_error(void);
 int iblgf_gpu_tree_upload(iblgf_gpu_ctx* ctx, const iblgf_tree* t);
 
 /* Fields ([nblocks][n^3] FP64 on the device). */
+/* No counterpart in the reference (it runs one tree per process).  Several
+ * independent trees of one block extent, base level and dx_base mirrored into
+ * one context: the slot table is level-major, then by tree (in the order
+ * given), then key-sorted, so every launch of a solve covers the blocks of all
+ * trees, and each tree's result is that of its own solve.  Fields are
+ * [total blocks][n^3] in that order; iblgf_gpu_forest_slots gives, for tree i,
+ * the combined slot of each of its blocks in the tree's own slot order
+ * (iblgf_tree_export's).  Immersed-boundary systems need a single tree. */
+int iblgf_gpu_forest_upload(iblgf_gpu_ctx* ctx, int ntrees, const iblgf_tree* const* trees);
+int iblgf_gpu_forest_slots(iblgf_gpu_ctx* ctx, int tree, int* slots);
+
 int iblgf_gpu_field_create(iblgf_gpu_ctx* ctx, int* field_id);
 int iblgf_gpu_field_device_ptr(iblgf_gpu_ctx* ctx, int field_id, void** dptr);
 int iblgf_gpu_field_upload(iblgf_gpu_ctx* ctx, int field_id, const double* host);
@@ -212,6 +223,41 @@ int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const i
  * reference (it has no device path); it serves time-step loops that repeat the
  * same solves.  Off by default. */
 int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on);
+
+/* No counterpart in the reference.  A sequence of solves issued in order, each
+ * as iblgf_gpu_apply_lgf_batch (kernel, alpha_base, k components
+ * src_fields[c] -> dst_fields[c]); with graphs on, the whole sequence is one
+ * captured HIP graph, so a caller that repeats the same sequence every time
+ * step (e.g. the integrating-factor and Poisson solves of the three IF-HERK
+ * stages, ifherk.hpp:1078-1200) launches one graph per step.  Results equal
+ * the items' own calls bit for bit. */
+/* No counterpart in the reference.  Autotuner: times
+ * iblgf_gpu_apply_lgf_batch(src_fields, dst_fields, k, kernel, alpha_base)
+ * under every combination of the launch shapes that leave results unchanged
+ * bit for bit — the multiply-accumulate grid (resident workgroups in rounds,
+ * or one workgroup per frequency chunk), the block-transform grid (resident
+ * workgroups looping over blocks, or one per block and component) and one or
+ * two streams per FMM pass — with `reps` eager calls each after a warm-up,
+ * keeps the fastest for the context's later solves and writes one line per
+ * timing plus the choice into report (report_len bytes, NUL-terminated;
+ * report may be NULL with report_len 0).  The destination fields are
+ * overwritten.  iblgf_gpu_launch_shapes reads the current choice. */
+int iblgf_gpu_autotune(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
+    double alpha_base, int reps, char* report, int report_len);
+int iblgf_gpu_launch_shapes(iblgf_gpu_ctx* ctx, int* mac_policy, int* dft_grid, int* streams);
+/* Sets the two grid shapes directly (0 or 1 each, as above; streams through
+ * iblgf_gpu_set_concurrency). */
+int iblgf_gpu_set_launch_shapes(iblgf_gpu_ctx* ctx, int mac_policy, int dft_grid);
+
+typedef struct iblgf_solve_item
+{
+    int         kernel;
+    double      alpha_base;
+    int         k;
+    const int*  src_fields;
+    const int*  dst_fields;
+} iblgf_solve_item;
+int iblgf_gpu_solve_sequence(iblgf_gpu_ctx* ctx, int nitems, const iblgf_solve_item* items);
 
 /* on = 1: each FMM pass runs its coarse chain (anterpolation, coarser
  * transforms and multiply-accumulates, interpolation above the finest level)
