@@ -132,6 +132,7 @@ int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf)
         if (t->has_ib) update_ib_flags(t->tree, t->ib);
         t->tree.fmm_lgf_mask_build(subtract_non_leaf != 0);
         t->tree.fmm_ib_masks();
+        t->tree.fmm_stream_masks();
         t->masks_built = true;
         t->subtract_non_leaf = subtract_non_leaf != 0;
     });

@@ -1302,11 +1302,13 @@ void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alph
 // ---------------------------------------------------------------------------
 void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
 {
-    if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB)
-        throw std::invalid_argument("mask type must be IB2xIB or xIB2IB");
+    if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB && mask_type != MASK_STREAM)
+        throw std::invalid_argument("mask type must be STREAM, IB2xIB or xIB2IB");
     check_batch(&src, &dst, 1);
     const Batch b = batch(1);
-    const int L = depth_ - 1;
+    // STREAM: one pass at the base level (poisson.hpp:609-611, 645-647); the
+    // immersed-boundary types: one pass at the finest level.
+    const int L = mask_type == MASK_STREAM ? base_level_ : depth_ - 1;
     double*   st = wp_.src;
     double*   tt = wp_.tgt;
     // The pass reads the source's level-L interiors only and adds into the

@@ -629,6 +629,26 @@ void Octree::fmm_clean_no_inf_masks(int base, int idx)
     }
 }
 
+void Octree::fmm_stream_masks()
+{
+    // STREAM (FmmMaskBuilder::fmm_vortex_streamfun_mask, include/iblgf/fmm/fmm.hpp:199-233):
+    // at the base level every block with data is a target and every
+    // non-correction block a source; parents are marked upward.
+    const int idx = fmm_mask_idx_gen(MASK_STREAM);
+    for (auto& o : oct)
+    {
+        o.set_mask(idx, 0, false);
+        o.set_mask(idx, 1, false);
+    }
+    if (base_level < (int)level_map.size())
+        for (int o : level_map[base_level])
+        {
+            oct[o].set_mask(idx, 0, !oct[o].correction);
+            oct[o].set_mask(idx, 1, true);
+        }
+    fmm_upward_pass_masks(base_level, idx);
+}
+
 void Octree::fmm_ib_masks()
 {
     // IB2xIB / xIB2IB: include/iblgf/fmm/fmm.hpp:70-144.  Finest level only.

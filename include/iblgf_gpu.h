@@ -388,10 +388,15 @@ enum iblgf_fmm_type
 };
 /* apply_lgf / apply_lgf_IF with the reference's fmm_type: AMR2AMR runs
  * iblgf_gpu_apply_lgf, IB2xIB and xIB2IB run iblgf_gpu_apply_lgf_masked;
- * STREAM, IB2AMR, Laplacian_BC and the Fourier-mode types return
- * IBLGF_ERR_UNSUPPORTED (checked before anything else, so a caller can route
- * them to its own path), any other value IBLGF_ERR_ARG.  Call sites:
- * ifherk.hpp:1471, 1555; linsys.hpp:142, 277, 332. */
+ * STREAM (Laplace kernel) runs one FMM pass at the base level with the
+ * stream-function masks (FmmMaskBuilder::fmm_vortex_streamfun_mask,
+ * fmm.hpp:199-233: base-level blocks are targets, the non-correction ones
+ * sources), the source's base-level interiors as input, no coarsening and no
+ * correction, and writes dst on the base-level leaves (poisson.hpp:609-611,
+ * 645-647, 760-768); IB2AMR, Laplacian_BC, the Fourier-mode types and STREAM
+ * with the IF kernel return IBLGF_ERR_UNSUPPORTED (checked before anything
+ * else, so a caller can route them to its own path), any other value
+ * IBLGF_ERR_ARG.  Call sites: ifherk.hpp:1471, 1555; linsys.hpp:142, 277, 332. */
 int iblgf_gpu_apply_lgf_typed(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
     int fmm_type);
 

@@ -1291,8 +1291,11 @@ int oracle_apply_lgf_masked(const oracle_tree* t, const double* g, int nmax, con
 {
     ostate o_;
     ostate* o = &o_;
-    const int L = t->depth - 1;
+    /* STREAM (0): the base level (poisson.hpp:609-611, 645-647); the IB types: the finest */
+    const int L = mask_type == 0 ? t->base_level : t->depth - 1;
+    const int idx = mask_type == 0 ? 0 : (mask_type == 2 ? 1 : 2);
     int       s, i, j, k, n = t->n;
+    if (mask_type != 0 && mask_type != 2 && mask_type != 3) return 1;
     ostate_init(o, t, faithful, intrp_order);
     o->ker.is_if = 0;
     o->ker.g = g;
@@ -1305,7 +1308,7 @@ int oracle_apply_lgf_masked(const oracle_tree* t, const double* g, int nmax, con
                     size_t q = ((size_t)k * n + j) * n + i;
                     BLK(o->source_tmp, s)[q] = BLK(src, s)[q];
                 }
-    fmm_apply_idx(o, o->source_tmp, o->target_tmp, L, mask_type == 2 ? 1 : 2, 1.0);
+    fmm_apply_idx(o, o->source_tmp, o->target_tmp, L, idx, 1.0);
     for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
         if (IS_LEAF(s)) memcpy(BLK(dst, s), BLK(o->target_tmp, s), sizeof(double) * o->n3);
     ostate_free(o);

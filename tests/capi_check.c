@@ -39,7 +39,8 @@ int main(int argc, char** argv)
     for (f = 0; f <= 7; ++f)
     {
         const int rc = iblgf_gpu_apply_lgf_typed(NULL, 0, 1, IBLGF_KERNEL_LAPLACE, 0.0, f);
-        const int built = f == IBLGF_FMM_AMR2AMR || f == IBLGF_FMM_IB2xIB || f == IBLGF_FMM_xIB2IB;
+        const int built = f == IBLGF_FMM_AMR2AMR || f == IBLGF_FMM_IB2xIB || f == IBLGF_FMM_xIB2IB ||
+                          f == IBLGF_FMM_STREAM;
         char      what[96];
         snprintf(what, sizeof what, "fmm_type %d without context -> %s", f, built ? "ERR_ARG" : "ERR_UNSUPPORTED");
         check(rc == (built ? IBLGF_ERR_ARG : IBLGF_ERR_UNSUPPORTED), what);
@@ -73,9 +74,11 @@ int main(int argc, char** argv)
         iblgf_gpu_field_download(ctx, d1, r1);
         iblgf_gpu_field_download(ctx, d2, r2);
         check(memcmp(r1, r2, npts * sizeof(double)) == 0, "typed AMR2AMR equals apply_lgf bit for bit");
-        check(iblgf_gpu_apply_lgf_typed(ctx, fs, d2, IBLGF_KERNEL_LAPLACE, 0.0, IBLGF_FMM_STREAM) ==
+        check(iblgf_gpu_apply_lgf_typed(ctx, fs, d2, IBLGF_KERNEL_IF, 0.5, IBLGF_FMM_STREAM) ==
                   IBLGF_ERR_UNSUPPORTED,
-            "STREAM refused with a context");
+            "STREAM with the IF kernel refused with a context");
+        check(iblgf_gpu_apply_lgf_typed(ctx, fs, d2, IBLGF_KERNEL_LAPLACE, 0.0, IBLGF_FMM_STREAM) == IBLGF_OK,
+            "STREAM Laplace solve");
         check(iblgf_gpu_apply_lgf_typed(ctx, fs, d2, IBLGF_KERNEL_IF, 0.5, IBLGF_FMM_IB2AMR) == IBLGF_ERR_UNSUPPORTED,
             "IB2AMR refused with a context");
         free(src);

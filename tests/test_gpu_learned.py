@@ -116,3 +116,26 @@ def test_krylov_recycling_cuts_iterations(small):
     f2, it2, res2, _ = sysm.fcg(s.uc, 0.3, tol=tol, maxit=1500, use_mlp=False)
     assert sysm.stats()[3] == "converged" and np.all(np.isfinite(f2))
     sysm.recycle(0)
+
+
+def test_non_finite_network_falls_back_to_the_identity(small):
+    """A network whose parameters are NaN (set from the host, so every kernel
+    still runs to completion) makes every preconditioned direction
+    non-finite; the flexible solve must take the residual itself instead,
+    report the fallback, and converge exactly as the identity solve does."""
+    s, solver, sysm = small
+    alpha, tol = 0.5, 1e-5
+    pc = LearnedPreconditioner(sysm, K=16, hidden=128, seed=9)
+    p = pc.params()
+    p[:] = np.nan
+    pc.set_params(p)
+    print("case: NaN network parameters, flexible solve on the 600-point sphere", flush=True)
+    sysm.recycle(0)
+    f_id, it_id, res_id, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500, use_mlp=False)
+    f_l, it_l, res_l, _ = sysm.fcg(s.uc, alpha, tol=tol, maxit=1500, use_mlp=True)
+    st = sysm.stats()
+    print(f"identity {it_id} iterations; NaN network {it_l} iterations, fell back {st[2]}, stop {st[3]}", flush=True)
+    assert st[2] and st[3] == "converged"
+    assert np.all(np.isfinite(f_l))
+    assert it_l == it_id
+    assert np.allclose(f_l, f_id, rtol=1e-9, atol=1e-12 * np.max(np.abs(f_id)))

@@ -355,3 +355,33 @@ def test_full_size_vortex_rings_properties():
     l2, _ = _rel(r2[leaves], lin[leaves])
     print(f"linearity relL2={l2:.3e}")
     assert l2 < 1e-12
+
+
+@pytest.mark.parametrize("extent", [8, 16])
+def test_stream_type_matches_oracle(oracle_mod, extent):
+    """apply_lgf with fmm_type STREAM (the stream-function solve of
+    Ifherk::pad_velocity, ifherk.hpp:1555) through the typed entry point:
+    base-level pass only, written on the base-level leaves; relative L2 <= 1e-10
+    against the oracle, whose STREAM solve equals the direct LGF sum over the
+    non-correction base-level cells."""
+    from iblgf_amd import FMM_STREAM
+
+    v = setups.VortexRings(seed=7, block_extent=extent, base_blocks=4, nref=2, nrings=3, centre_range=4.0,
+                           radius=(4.0, 8.0), core=(1.5, 2.5), refinement_factor=0.5)
+    print(f"case: STREAM solve, block extent {extent}, {v.info['nblocks']} blocks", flush=True)
+    rng = np.random.default_rng(17)
+    n = v.tree.n
+    src = rng.standard_normal((v.info["nblocks"], n, n, n))
+    solver = PoissonSolver(v.tree)
+    fs, fd = solver.field(), solver.field()
+    solver.upload(fs, src)
+    solver.upload(fd, np.full_like(src, 7.0))
+    solver.apply_lgf_typed(fs, fd, fmm_type=FMM_STREAM)
+    got = solver.download(fd)
+    ref = oracle_mod.Tree(v.arrays, v.info, n).apply_lgf_masked(src, 0)
+    a = v.arrays
+    sel = (a["level"] == v.info["base_level"]) & ((a["flags"] & FLAG_LEAF) != 0)
+    rel = np.linalg.norm(got[sel] - ref[sel]) / np.linalg.norm(ref[sel])
+    print(f"STREAM vs oracle relL2={rel:.3e}", flush=True)
+    assert rel < 1e-10
+    assert np.all(got[~sel] == 7.0)  # nothing else is written

@@ -107,3 +107,53 @@ def test_rank_split_baseline_equals_single_solve(ranks):
     sec, got = ot.lgf_baseline(src, ranks=ranks, nsolves=2)
     assert len(sec) == 2 and np.all(sec > 0)
     assert np.array_equal(got, ref)
+
+
+def test_stream_solve_is_the_base_level_direct_sum():
+    """fmm_type STREAM (poisson.hpp:609-611, 645-647; masks fmm.hpp:199-233):
+    one FMM pass at the base level from the interiors of the non-correction
+    base-level blocks (base blocks missing a neighbour are correction blocks,
+    domain.hpp:617-735), written on the base-level leaves, equals the exact
+    all-pairs LGF sum over those cells (the base level as a uniform mesh),
+    whatever the finer levels hold.  The STREAM masks follow the rule:
+    base-level blocks targets, non-correction base-level blocks sources,
+    parents marked upward, nothing finer."""
+    v = setups.VortexRings(seed=7, block_extent=8, base_blocks=4, nref=2, nrings=3, centre_range=4.0,
+                           radius=(4.0, 8.0), core=(1.5, 2.5), refinement_factor=0.5)
+    a, n, info = v.arrays, v.tree.n, v.info
+    bl = info["base_level"]
+    one = np.uint64(1)
+    ms, mt = a["mask_src"] & one, a["mask_tgt"] & one
+    lvl = a["level"]
+    corr = (a["flags"] & 2) != 0
+    assert np.array_equal(mt[lvl == bl], np.ones(np.sum(lvl == bl), np.uint64))
+    assert np.array_equal(ms[lvl == bl] == 1, ~corr[lvl == bl])
+    assert not ms[lvl > bl].any() and not mt[lvl > bl].any()
+    for s in np.nonzero(lvl < bl)[0]:
+        kids = [c for c in a["children"][s] if c >= 0]
+        assert ms[s] == (1 if any(ms[c] for c in kids) else 0)
+    rng = np.random.default_rng(3)
+    src = rng.standard_normal((info["nblocks"], n, n, n))
+    phi = O.Tree(a, info, n).apply_lgf_masked(src, 0)
+    base = np.nonzero(lvl == bl)[0]
+    sx, q = [], []
+    kk, jj, ii = np.meshgrid(np.arange(1, n - 1), np.arange(1, n - 1), np.arange(1, n - 1), indexing="ij")
+    for s in base:
+        if corr[s]:
+            continue
+        b = a["bbase"][s]
+        sx.append(np.stack([b[0] + ii - 1, b[1] + jj - 1, b[2] + kk - 1], -1).reshape(-1, 3))
+        q.append(src[s, 1:-1, 1:-1, 1:-1].reshape(-1))
+    sx, q = np.concatenate(sx), np.concatenate(q)
+    leaves = [s for s in base if a["flags"][s] & FLAG_LEAF]
+    assert leaves
+    K, J, I = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    for s in leaves[:3]:
+        b = a["bbase"][s]
+        tx = np.stack([b[0] + I - 1, b[1] + J - 1, b[2] + K - 1], -1).reshape(-1, 3)
+        direct = O.direct_sum(sx, q, tx, info["dx_base"] ** 2)
+        assert np.max(np.abs(phi[s].reshape(-1) - direct)) < 1e-11 * np.max(np.abs(direct))
+    # nothing outside the base-level leaves is written
+    other = np.ones(info["nblocks"], bool)
+    other[leaves] = False
+    assert not phi[other].any()
