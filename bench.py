@@ -248,11 +248,12 @@ def forest_step(steps, log):
         f"set-up {t_setup:.1f} s")
     fs = PoissonSolver(f.trees)
     fs.set_precision(PREC_FP16X3_FORWARD)
-    w = [fs.field() for _ in range(3)]
-    u = [fs.field() for _ in range(3)]
-    p = [fs.field() for _ in range(3)]
+    # FP32 user fields (the solves convert to and from their FP64 work arrays)
+    w = [fs.field_f32() for _ in range(3)]
+    u = [fs.field_f32() for _ in range(3)]
+    p = [fs.field_f32() for _ in range(3)]
     for c in range(3):
-        fs.upload(w[c], fs.to_forest([s[c] for s in srcs]))
+        fs.upload(w[c], fs.to_forest([s[c] for s in srcs]).astype(np.float32))
     del srcs
     seq = []
     for a in f.alphas:
@@ -286,7 +287,8 @@ def forest_step(steps, log):
         "workload": ("64 vortex-ring trees (seed 1003; tree i seeded by the i-th draw, sizes from "
                      "floor(exp(U[ln 200, ln 4000])) targets) in one context; per step 3 IF-HERK stages x "
                      "(integrating factor of 3 components at the stage's alpha + Poisson of 3 components); "
-                     "precision fp16x3forward; one solve sequence per step"),
+                     "precision fp16x3forward, FP32 user fields; one solve sequence per step"),
+        "fields": "fp32",
         "trees": len(f.trees),
         "blocks_total": int(sum(f.blocks)),
         "blocks_per_tree": [int(b) for b in f.blocks],

@@ -147,7 +147,31 @@ int iblgf_gpu_field_device_ptr(iblgf_gpu_ctx* ctx, int field_id, void** dptr)
 {
     NEED_CTX("iblgf_gpu_field_device_ptr");
     if (!dptr) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_device_ptr: null output", ctx);
-    return guarded(ctx, [&] { *dptr = ctx->solver->field_ptr(field_id); });
+    return guarded(ctx, [&] { *dptr = ctx->solver->field_data(field_id); });
+}
+
+int iblgf_gpu_field_create_f32(iblgf_gpu_ctx* ctx, int* field_id)
+{
+    NEED_CTX("iblgf_gpu_field_create_f32");
+    if (!field_id) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_create_f32: null output", ctx);
+    return guarded(ctx, [&] {
+        if (ctx->solver->nblocks() == 0) throw std::runtime_error("tree not uploaded");
+        *field_id = ctx->solver->field_create_f32();
+    });
+}
+
+int iblgf_gpu_field_upload_f32(iblgf_gpu_ctx* ctx, int field_id, const float* host)
+{
+    NEED_CTX("iblgf_gpu_field_upload_f32");
+    if (!host) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_upload_f32: null host pointer", ctx);
+    return guarded(ctx, [&] { ctx->solver->field_upload_f32(field_id, host); });
+}
+
+int iblgf_gpu_field_download_f32(iblgf_gpu_ctx* ctx, int field_id, float* host)
+{
+    NEED_CTX("iblgf_gpu_field_download_f32");
+    if (!host) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_download_f32: null host pointer", ctx);
+    return guarded(ctx, [&] { ctx->solver->field_download_f32(field_id, host); });
 }
 
 int iblgf_gpu_field_upload(iblgf_gpu_ctx* ctx, int field_id, const double* host)

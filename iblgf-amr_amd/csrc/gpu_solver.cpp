@@ -350,13 +350,50 @@ int GpuSolver::field_create()
     f.alloc(field_size());
     IBLGF_HIP(hipMemset(f.p, 0, f.n * sizeof(double)));
     fields_.push_back(std::move(f));
+    field_f32_.push_back(0);
     return static_cast<int>(fields_.size()) - 1;
+}
+
+int GpuSolver::field_create_f32()
+{
+    DevArray<double> f;
+    f.alloc((field_size() + 1) / 2);
+    IBLGF_HIP(hipMemset(f.p, 0, f.n * sizeof(double)));
+    fields_.push_back(std::move(f));
+    field_f32_.push_back(1);
+    return static_cast<int>(fields_.size()) - 1;
+}
+
+bool GpuSolver::field_is_f32(int id) const
+{
+    if (id < 0 || id >= (int)fields_.size()) throw std::runtime_error("invalid field id");
+    return field_f32_[id] != 0;
 }
 
 double* GpuSolver::field_ptr(int id)
 {
-    if (id < 0 || id >= (int)fields_.size()) throw std::runtime_error("invalid field id");
+    if (field_is_f32(id)) throw std::invalid_argument("FP32 field: this operation takes FP64 fields");
     return fields_[id].p;
+}
+
+void* GpuSolver::field_data(int id)
+{
+    field_is_f32(id);  // id check
+    return fields_[id].p;
+}
+
+void GpuSolver::field_upload_f32(int id, const float* host)
+{
+    if (!field_is_f32(id)) throw std::invalid_argument("FP64 field: upload FP64 values");
+    IBLGF_HIP(hipMemcpyAsync(fields_[id].p, host, field_size() * sizeof(float), hipMemcpyHostToDevice, stream_));
+    sync();
+}
+
+void GpuSolver::field_download_f32(int id, float* host)
+{
+    if (!field_is_f32(id)) throw std::invalid_argument("FP64 field: download FP64 values");
+    IBLGF_HIP(hipMemcpyAsync(host, fields_[id].p, field_size() * sizeof(float), hipMemcpyDeviceToHost, stream_));
+    sync();
 }
 
 void GpuSolver::field_upload(int id, const double* host)
@@ -377,7 +414,11 @@ FieldTable GpuSolver::field_table(const int* ids, int k)
 {
     static_assert(kMaxComponents <= kMaxFieldTable, "field tables hold every component");
     FieldTable t{};
-    for (int c = 0; c < k; ++c) t.p[c] = F(ids[c]);
+    for (int c = 0; c < k; ++c)
+    {
+        t.p[c] = static_cast<double*>(field_data(ids[c]));
+        if (field_f32_[ids[c]]) t.f32 |= 1ull << c;
+    }
     return t;
 }
 
@@ -933,8 +974,8 @@ void GpuSolver::check_batch(const int* src, const int* dst, int k)
     if (k < 1 || k > kMaxComponents) throw std::invalid_argument("component count out of range");
     for (int c = 0; c < k; ++c)
     {
-        field_ptr(src[c]);
-        field_ptr(dst[c]);
+        field_data(src[c]);
+        field_data(dst[c]);
     }
     IBLGF_HIP(hipSetDevice(device_));
     reserve_components(k);
@@ -1243,8 +1284,8 @@ void GpuSolver::solve_in_region(int src, int dst, bool laplace, double alpha_bas
     if (!prepared(laplace, alpha_base))
         throw std::runtime_error("plans or kernel spectra of this solve are not built (prepare it first in this precision)");
     if (timer_.enabled()) throw std::runtime_error("kernel timing is on (it records events)");
-    field_ptr(src);
-    field_ptr(dst);
+    field_data(src);
+    field_data(dst);
     IBLGF_HIP(hipSetDevice(device_));
     const size_t fb = round256(field_size() * sizeof(double));
     const size_t sb = round256(static_cast<size_t>(nb_) * nf_ * (precision_ == 0 ? sizeof(double2) : sizeof(float2)));

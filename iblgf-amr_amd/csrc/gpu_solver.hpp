@@ -144,9 +144,18 @@ class GpuSolver
     const std::vector<int>& forest_slot_of(int i) const { return forest_slot_of_.at(i); }
 
     int     field_create();
-    double* field_ptr(int id);
+    // An FP32 field ([nslots][n^3] float): accepted by the AMR2AMR solves
+    // (apply_lgf / apply_if, batched, sequences, allocation-free), which
+    // convert it to and from their FP64 work arrays; every other operation
+    // takes FP64 fields only.
+    int     field_create_f32();
+    bool    field_is_f32(int id) const;
+    double* field_ptr(int id);        // FP64 fields only
+    void*   field_data(int id);       // either precision
     void    field_upload(int id, const double* host);
     void    field_download(int id, double* host);
+    void    field_upload_f32(int id, const float* host);
+    void    field_download_f32(int id, float* host);
     size_t  field_size() const { return static_cast<size_t>(nb_) * n3_; }
 
     // Laplace solve of one scalar component (AMR2AMR, use_correction,
@@ -305,8 +314,9 @@ class GpuSolver
     DevArray<double> m_nli_anterp_, m_nli_interp_, m_cc_restrict_, m_cc_interp_;
     int              intrp_order_ = 5;
 
-    // fields
+    // fields (an FP32 field keeps its floats in a DevArray<double> of half the count)
     std::vector<DevArray<double>> fields_;
+    std::vector<char>             field_f32_;
     // Work arrays of the solves, k components each, component-major
     // (component c of a field array at c * field_size(), of a spectrum array
     // at c * nb * NF): source and target temporaries, correction, the FMM

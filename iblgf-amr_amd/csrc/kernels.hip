@@ -59,6 +59,21 @@ __global__ __launch_bounds__(kThreads) void k_zero(double* f, const int* slots, 
     for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads) p[q] = 0.0;
 }
 
+template<int N, class TD, class TS>
+__device__ inline void copy_block_as(TD* d, const TS* s, int interior)
+{
+    for (int q = threadIdx.x; q < Geo<N>::N3; q += kThreads)
+    {
+        bool ring = false;
+        if (interior)
+        {
+            const int i = q % N, j = (q / N) % N, k = q / (N * N);
+            ring = i == 0 || j == 0 || k == 0 || i == N - 1 || j == N - 1 || k == N - 1;
+        }
+        d[q] = ring ? TD(0) : static_cast<TD>(s[q]);
+    }
+}
+
 template<int N>
 __device__ inline void copy_block(double* d, const double* s, int interior)
 {
@@ -91,7 +106,16 @@ __global__ __launch_bounds__(kThreads) void k_copy_table(
     const int b = blockIdx.x;
     if (b >= count) return;
     const size_t off = (size_t)slots[b] * Geo<N>::N3;
-    copy_block<N>(dst.p[blockIdx.y] + off, src.p[blockIdx.y] + off, interior);
+    const int    c = blockIdx.y;
+    const bool   df = (dst.f32 >> c) & 1ull, sf = (src.f32 >> c) & 1ull;  // uniform per workgroup
+    if (!df && !sf) copy_block<N>(dst.p[c] + off, src.p[c] + off, interior);
+    else if (df && !sf)
+        copy_block_as<N>(reinterpret_cast<float*>(dst.p[c]) + off, src.p[c] + off, interior);
+    else if (!df && sf)
+        copy_block_as<N>(dst.p[c] + off, reinterpret_cast<const float*>(src.p[c]) + off, interior);
+    else
+        copy_block_as<N>(reinterpret_cast<float*>(dst.p[c]) + off, reinterpret_cast<const float*>(src.p[c]) + off,
+            interior);
 }
 
 template<int N>

@@ -53,10 +53,14 @@ struct Batch
 };
 
 // Per-component field base pointers, passed by value (component c in p[c]).
+// Bit c of f32 set: component c is an FP32 field and p[c] holds its float
+// address (read and written as float, converted to and from FP64 by the
+// copies between user fields and a solve's work arrays).
 constexpr int kMaxFieldTable = 64;
 struct FieldTable
 {
-    double* p[kMaxFieldTable];
+    double*            p[kMaxFieldTable];
+    unsigned long long f32 = 0;
 };
 
 // Bytes per staged kernel row of the multiply-accumulate (KF frequencies of

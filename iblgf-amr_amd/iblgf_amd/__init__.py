@@ -120,6 +120,9 @@ SYMBOLS = {
     "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_apply_lgf_typed": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_forest_upload": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "iblgf_gpu_field_create_f32": (_I, [_P, _IP]),
+    "iblgf_gpu_field_upload_f32": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
+    "iblgf_gpu_field_download_f32": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
     "iblgf_gpu_solve_sequence": (_I, [_P, _I, _P]),
     "iblgf_gpu_autotune": (_I, [_P, _IP, _IP, _I, _I, _D, _I, ctypes.c_char_p, _I]),
     "iblgf_gpu_launch_shapes": (_I, [_P, _IP, _IP, _IP]),
@@ -381,18 +384,37 @@ class PoissonSolver:
         _check(lib().iblgf_gpu_field_create(self._h, ctypes.byref(fid)), self._h)
         return fid.value
 
+    def field_f32(self):
+        """An FP32 field (iblgf_gpu_field_create_f32); upload/download take float32 arrays."""
+        fid = ctypes.c_int()
+        _check(lib().iblgf_gpu_field_create_f32(self._h, ctypes.byref(fid)), self._h)
+        self._f32 = getattr(self, "_f32", set()) | {fid.value}
+        return fid.value
+
     def device_ptr(self, fid):
         p = ctypes.c_void_p()
         _check(lib().iblgf_gpu_field_device_ptr(self._h, int(fid), ctypes.byref(p)), self._h)
         return p.value
 
     def upload(self, fid, arr):
+        if fid in getattr(self, "_f32", ()):
+            a = np.ascontiguousarray(arr, dtype=np.float32)
+            if a.size != int(np.prod(self.shape)):
+                raise ValueError(f"field has {a.size} values, expected {self.shape}")
+            _check(lib().iblgf_gpu_field_upload_f32(self._h, int(fid), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                   self._h)
+            return
         a = np.ascontiguousarray(arr, dtype=np.float64)
         if a.size != int(np.prod(self.shape)):
             raise ValueError(f"field has {a.size} values, expected {self.shape}")
         _check(lib().iblgf_gpu_field_upload(self._h, int(fid), a.ctypes.data_as(_DP)), self._h)
 
     def download(self, fid):
+        if fid in getattr(self, "_f32", ()):
+            a = np.empty(self.shape, np.float32)
+            _check(lib().iblgf_gpu_field_download_f32(self._h, int(fid), a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                   self._h)
+            return a
         a = np.empty(self.shape, np.float64)
         _check(lib().iblgf_gpu_field_download(self._h, int(fid), a.ctypes.data_as(_DP)), self._h)
         return a

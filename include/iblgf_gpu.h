@@ -190,6 +190,14 @@ int iblgf_gpu_field_create(iblgf_gpu_ctx* ctx, int* field_id);
 int iblgf_gpu_field_device_ptr(iblgf_gpu_ctx* ctx, int field_id, void** dptr);
 int iblgf_gpu_field_upload(iblgf_gpu_ctx* ctx, int field_id, const double* host);
 int iblgf_gpu_field_download(iblgf_gpu_ctx* ctx, int field_id, double* host);
+/* No counterpart in the reference (its fields are FP64).  FP32 fields of the
+ * same layout, for callers that keep their fields in single precision: the
+ * AMR2AMR solves (iblgf_gpu_apply_lgf, _batch, _noalloc and
+ * iblgf_gpu_solve_sequence) read and write them, converting to and from their
+ * FP64 work arrays; every other operation refuses them with IBLGF_ERR_ARG. */
+int iblgf_gpu_field_create_f32(iblgf_gpu_ctx* ctx, int* field_id);
+int iblgf_gpu_field_upload_f32(iblgf_gpu_ctx* ctx, int field_id, const float* host);
+int iblgf_gpu_field_download_f32(iblgf_gpu_ctx* ctx, int field_id, float* host);
 
 enum iblgf_kernel
 {

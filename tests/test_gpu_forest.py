@@ -144,3 +144,38 @@ def test_autotuner_keeps_results_and_records_its_choice():
         report.splitlines()[-1]
     fs.apply_lgf_batch(fi, fo)
     assert all(np.array_equal(fs.download(f), r) for f, r in zip(fo, ref))
+
+
+def test_fp32_fields_round_the_fp64_field_solve():
+    """FP32 user fields: the solve reads them into its FP64 work arrays and
+    rounds its result once on the way out, so an FP32-field solve equals the
+    FP64-field solve of the same (float-valued) sources rounded to FP32, bit
+    for bit; operations outside the AMR2AMR solves refuse FP32 fields."""
+    from iblgf_amd import IBLGF_ERR_ARG, IblgfError
+
+    ws, srcs = _workload()
+    print("case: FP32 fields, forest, mixed precision, Laplace + IF sequence", flush=True)
+    fs = PoissonSolver([w.tree for w in ws])
+    fs.set_precision(PREC_MIXED)
+    x = [fs.to_forest([s[c] for s in srcs]).astype(np.float32) for c in range(3)]
+    w64 = [fs.field() for _ in range(3)]
+    o64 = [fs.field() for _ in range(3)]
+    w32 = [fs.field_f32() for _ in range(3)]
+    o32 = [fs.field_f32() for _ in range(3)]
+    for c in range(3):
+        fs.upload(w64[c], x[c].astype(np.float64))
+        fs.upload(w32[c], x[c])
+    for kernel, alpha in ((KERNEL_LAPLACE, 0.0), (KERNEL_IF, 0.56)):
+        fs.apply_lgf_batch(w64, o64, kernel=kernel, alpha_base=alpha)
+        fs.apply_lgf_batch(w32, o32, kernel=kernel, alpha_base=alpha)
+        for c in range(3):
+            ref = fs.download(o64[c]).astype(np.float32)
+            got = fs.download(o32[c])
+            assert got.dtype == np.float32 and np.array_equal(got, ref), (kernel, c)
+    with pytest.raises(IblgfError) as e:
+        fs.upload(w32[0], np.zeros(fs.shape, np.float64))  # converted to float32: allowed
+        from iblgf_amd import lib
+        rc = lib().iblgf_gpu_buffer_exchange(fs._h, w32[0], ws[0].info["base_level"])
+        if rc:
+            raise IblgfError(rc, "buffer exchange")
+    assert e.value.code == IBLGF_ERR_ARG
