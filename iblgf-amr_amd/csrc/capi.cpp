@@ -442,24 +442,32 @@ int iblgf_gpu_apply_lgf_typed(iblgf_gpu_ctx* ctx, int src_field, int dst_field, 
                     ctx);
             break;
         case IBLGF_FMM_IB2AMR:
+            if (kernel == IBLGF_KERNEL_IF)
+                return fail(IBLGF_ERR_UNSUPPORTED,
+                    "iblgf_gpu_apply_lgf_typed: IB2AMR is built for the Laplace kernel only (the reference calls "
+                    "apply_lgf with it, linsys.hpp:142)",
+                    ctx);
+            break;
         case IBLGF_FMM_LAPLACIAN_BC:
         case IBLGF_FMM_REF_FOURIER_STREAM:
         case IBLGF_FMM_REF_FOURIER_LGF:
             return fail(IBLGF_ERR_UNSUPPORTED,
                 "iblgf_gpu_apply_lgf_typed: fmm_type " + std::to_string(fmm_type) +
-                    " has no device path (built: AMR2AMR, IB2xIB, xIB2IB, STREAM)",
+                    " has no device path (built: AMR2AMR, IB2xIB, xIB2IB, STREAM, IB2AMR)",
                 ctx);
         default:
             return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_typed: unknown fmm_type", ctx);
     }
     if (fmm_type == IBLGF_FMM_AMR2AMR) return iblgf_gpu_apply_lgf(ctx, src_field, dst_field, kernel, alpha_base);
-    if (fmm_type == IBLGF_FMM_STREAM)
+    if (fmm_type == IBLGF_FMM_STREAM || fmm_type == IBLGF_FMM_IB2AMR)
     {
         NEED_CTX("iblgf_gpu_apply_lgf_typed");
         return guarded(ctx, [&] {
             ctx->solver->field_ptr(src_field);
             ctx->solver->field_ptr(dst_field);
-            ctx->solver->apply_lgf_masked(src_field, dst_field, MASK_STREAM);
+            if (fmm_type == IBLGF_FMM_STREAM) ctx->solver->apply_lgf_masked(src_field, dst_field, MASK_STREAM);
+            else
+                ctx->solver->apply_lgf_ib2amr(src_field, dst_field);
         });
     }
     return iblgf_gpu_apply_lgf_masked(ctx, src_field, dst_field, kernel, alpha_base, fmm_type);

@@ -133,6 +133,7 @@ int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf)
         t->tree.fmm_lgf_mask_build(subtract_non_leaf != 0);
         t->tree.fmm_ib_masks();
         t->tree.fmm_stream_masks();
+        t->tree.fmm_ib2amr_masks();
         t->masks_built = true;
         t->subtract_non_leaf = subtract_non_leaf != 0;
     });

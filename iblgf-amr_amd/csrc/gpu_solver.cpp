@@ -1362,6 +1362,32 @@ void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
     IBLGF_HIP(hipGetLastError());
 }
 
+// apply_lgf with fmm_type IB2AMR (poisson.hpp:613-616, 632-633, 645-650): the
+// source's finest-level interiors coarsened to every level, then one FMM pass
+// per level with the IB2AMR masks of that level (immersed-boundary blocks as
+// sources, every block of the level a target), each level's leaves written.
+void GpuSolver::apply_lgf_ib2amr(int src, int dst)
+{
+    check_batch(&src, &dst, 1);
+    const Batch b = batch(1);
+    const int   Lf = depth_ - 1;
+    double*     st = wp_.src;
+    double*     tt = wp_.tgt;
+    fill_zero(st, b);
+    fill_zero(tt, b);
+    launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[Lf].p, p_level_slots_[Lf].size(), true);
+    for (int ls = depth_ - 2; ls >= base_level_; --ls)
+        launch::transfer_gather(
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b);
+    for (int l = base_level_; l < depth_; ++l)
+    {
+        const double dx = dx_base_ / std::pow(2.0, l - base_level_);
+        run_pass(pass_idx(l, fmm_mask_idx_gen(MASK_IB2AMR, l - base_level_), false), dx * dx, true, 0.0, b);
+        launch::copy_blocks(stream_, n_, F(dst), tt, p_leaf_level_[l].p, p_leaf_level_[l].size(), false);
+    }
+    IBLGF_HIP(hipGetLastError());
+}
+
 void GpuSolver::apply_if_masked(int src, int dst, double alpha_base, int mask_type)
 {
     apply_if_masked_batch(&src, &dst, 1, alpha_base, mask_type);

@@ -222,6 +222,10 @@ class GpuSolver
     // apply_lgf_masked writes the finest leaves, apply_if_masked every
     // finest-level block (buffers included).
     void apply_lgf_masked(int src, int dst, int mask_type);
+    // apply_lgf with fmm_type IB2AMR (LinSysSolver::pressure_correction,
+    // linsys.hpp:131-145): finest interiors coarsened, one pass per level with
+    // the immersed-boundary blocks as sources, every level's leaves written.
+    void apply_lgf_ib2amr(int src, int dst);
     void apply_if_masked(int src, int dst, double alpha_base, int mask_type);
     // The same for k components through every launch (src[c] -> dst[c]; a
     // component may be solved in place, src[c] == dst[c]).

@@ -629,6 +629,44 @@ void Octree::fmm_clean_no_inf_masks(int base, int idx)
     }
 }
 
+void Octree::fmm_ib2amr_masks()
+{
+    // IB2AMR (FmmMaskBuilder::fmm_IB2AMR_mask, include/iblgf/fmm/fmm.hpp:146-195):
+    // per refinement level, every block of the level is a target and the
+    // blocks holding immersed-boundary points (or having a descendant that
+    // does) are sources; parents are marked upward.  The reference marks the
+    // ancestors is_ib in place; here the marking stays local to the masks.
+    std::vector<char> ib(oct.size(), 0);
+    for (size_t o = 0; o < oct.size(); ++o) ib[o] = oct[o].is_ib ? 1 : 0;
+    for (int l = depth - 2; l >= base_level; --l)
+    {
+        if (l >= (int)level_map.size()) continue;
+        for (int o : level_map[l])
+            for (int c = 0; c < 8; ++c)
+            {
+                const int ch = oct[o].children[c];
+                if (ch >= 0 && oct[ch].has_data && ib[ch]) ib[o] = 1;
+            }
+    }
+    for (int l = depth - 1; l >= base_level; --l)
+    {
+        const int idx = fmm_mask_idx_gen(MASK_IB2AMR, l - base_level);
+        if (idx >= fmm_mask_idx_gen(MASK_AMR2AMR, 0, 0)) throw std::runtime_error("too many levels for IB2AMR masks");
+        for (auto& o : oct)
+        {
+            o.set_mask(idx, 0, false);
+            o.set_mask(idx, 1, false);
+        }
+        if (l >= (int)level_map.size()) continue;
+        for (int o : level_map[l])
+        {
+            oct[o].set_mask(idx, 1, true);
+            if (ib[o]) oct[o].set_mask(idx, 0, true);
+        }
+        fmm_upward_pass_masks(l, idx);
+    }
+}
+
 void Octree::fmm_stream_masks()
 {
     // STREAM (FmmMaskBuilder::fmm_vortex_streamfun_mask, include/iblgf/fmm/fmm.hpp:199-233):

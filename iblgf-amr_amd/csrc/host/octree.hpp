@@ -291,6 +291,7 @@ struct Octree
     void fmm_clean_no_inf_masks(int base, int idx);
     void fmm_ib_masks();
     void fmm_stream_masks();
+    void fmm_ib2amr_masks();
 
     // -- geometry ------------------------------------------------------------
     // Block base (cells at the octant's refinement level, clamped at 0).

@@ -401,10 +401,16 @@ enum iblgf_fmm_type
  * fmm.hpp:199-233: base-level blocks are targets, the non-correction ones
  * sources), the source's base-level interiors as input, no coarsening and no
  * correction, and writes dst on the base-level leaves (poisson.hpp:609-611,
- * 645-647, 760-768); IB2AMR, Laplacian_BC, the Fourier-mode types and STREAM
- * with the IF kernel return IBLGF_ERR_UNSUPPORTED (checked before anything
- * else, so a caller can route them to its own path), any other value
- * IBLGF_ERR_ARG.  Call sites: ifherk.hpp:1471, 1555; linsys.hpp:142, 277, 332. */
+ * 645-647, 760-768); IB2AMR (Laplace kernel) coarsens the source's finest-level
+ * interiors to every level and runs one pass per level with the IB2AMR masks
+ * (FmmMaskBuilder::fmm_IB2AMR_mask, fmm.hpp:146-195: blocks holding
+ * immersed-boundary points, or with descendants that do, are sources, every
+ * block of the level a target), writing each level's leaves
+ * (poisson.hpp:613-616, 632-633, 645-650); Laplacian_BC, the Fourier-mode
+ * types and STREAM or IB2AMR with the IF kernel return IBLGF_ERR_UNSUPPORTED
+ * (checked before anything else, so a caller can route them to its own path),
+ * any other value IBLGF_ERR_ARG.  Call sites: ifherk.hpp:1471, 1555;
+ * linsys.hpp:142, 277, 332. */
 int iblgf_gpu_apply_lgf_typed(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base,
     int fmm_type);
 

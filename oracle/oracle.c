@@ -1295,11 +1295,35 @@ int oracle_apply_lgf_masked(const oracle_tree* t, const double* g, int nmax, con
     const int L = mask_type == 0 ? t->base_level : t->depth - 1;
     const int idx = mask_type == 0 ? 0 : (mask_type == 2 ? 1 : 2);
     int       s, i, j, k, n = t->n;
-    if (mask_type != 0 && mask_type != 2 && mask_type != 3) return 1;
+    if (mask_type != 0 && mask_type != 2 && mask_type != 3 && mask_type != 4) return 1;
     ostate_init(o, t, faithful, intrp_order);
     o->ker.is_if = 0;
     o->ker.g = g;
     o->ker.nmax = nmax;
+    if (mask_type == 4)
+    {
+        /* IB2AMR (poisson.hpp:613-616, 632-633, 645-650): the finest level's
+         * interiors coarsened to every level, then per level l one pass with
+         * the IB2AMR masks of l (mask index 4 + l - base), its leaves written */
+        int l;
+        for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
+            for (k = 1; k < n - 1; ++k)
+                for (j = 1; j < n - 1; ++j)
+                    for (i = 1; i < n - 1; ++i)
+                    {
+                        size_t q = ((size_t)k * n + j) * n + i;
+                        BLK(o->source_tmp, s)[q] = BLK(src, s)[q];
+                    }
+        source_coarsify(o, o->source_tmp);
+        for (l = t->base_level; l < t->depth; ++l)
+        {
+            fmm_apply_idx(o, o->source_tmp, o->target_tmp, l, 4 + l - t->base_level, 1.0);
+            for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s)
+                if (IS_LEAF(s)) memcpy(BLK(dst, s), BLK(o->target_tmp, s), sizeof(double) * o->n3);
+        }
+        ostate_free(o);
+        return 0;
+    }
     for (s = o->lvl_begin[L]; s < o->lvl_begin[L + 1]; ++s)
         for (k = 1; k < n - 1; ++k)
             for (j = 1; j < n - 1; ++j)

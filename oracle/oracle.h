@@ -108,8 +108,9 @@ typedef struct oracle_ib
 
 /* apply_lgf with fmm_type STREAM (0: one pass at the base level, targets
  * written on the base-level leaves), IB2xIB (2) or xIB2IB (3) (one pass at
- * the finest level), poisson.hpp:587-894; apply_if with IB2xIB / xIB2IB,
- * poisson.hpp:435-494. */
+ * the finest level), IB2AMR (4: the finest interiors coarsened, one pass per
+ * level, every level's leaves written), poisson.hpp:587-894; apply_if with
+ * IB2xIB / xIB2IB, poisson.hpp:435-494. */
 int oracle_apply_lgf_masked(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
     int mask_type, int faithful, int intrp_order);
 int oracle_apply_if_masked(const oracle_tree* t, const double* src, double* dst, double alpha_base, int mask_type,

@@ -40,7 +40,7 @@ int main(int argc, char** argv)
     {
         const int rc = iblgf_gpu_apply_lgf_typed(NULL, 0, 1, IBLGF_KERNEL_LAPLACE, 0.0, f);
         const int built = f == IBLGF_FMM_AMR2AMR || f == IBLGF_FMM_IB2xIB || f == IBLGF_FMM_xIB2IB ||
-                          f == IBLGF_FMM_STREAM;
+                          f == IBLGF_FMM_STREAM || f == IBLGF_FMM_IB2AMR;
         char      what[96];
         snprintf(what, sizeof what, "fmm_type %d without context -> %s", f, built ? "ERR_ARG" : "ERR_UNSUPPORTED");
         check(rc == (built ? IBLGF_ERR_ARG : IBLGF_ERR_UNSUPPORTED), what);
@@ -80,7 +80,10 @@ int main(int argc, char** argv)
         check(iblgf_gpu_apply_lgf_typed(ctx, fs, d2, IBLGF_KERNEL_LAPLACE, 0.0, IBLGF_FMM_STREAM) == IBLGF_OK,
             "STREAM Laplace solve");
         check(iblgf_gpu_apply_lgf_typed(ctx, fs, d2, IBLGF_KERNEL_IF, 0.5, IBLGF_FMM_IB2AMR) == IBLGF_ERR_UNSUPPORTED,
-            "IB2AMR refused with a context");
+            "IB2AMR with the IF kernel refused with a context");
+        check(iblgf_gpu_apply_lgf_typed(ctx, fs, d2, IBLGF_KERNEL_LAPLACE, 0.0, IBLGF_FMM_LAPLACIAN_BC) ==
+                  IBLGF_ERR_UNSUPPORTED,
+            "Laplacian_BC refused with a context");
         free(src);
         free(r1);
         free(r2);

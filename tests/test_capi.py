@@ -93,7 +93,7 @@ def build_c_caller(tmp_path):
 
 def test_c_caller_status_codes(tmp_path):
     """A C caller sees ERR_UNSUPPORTED for the fmm types without a device
-    path (IB2AMR, Laplacian_BC, Fourier modes, STREAM with the IF kernel) and for a tree whose
+    path (Laplacian_BC, Fourier modes, STREAM or IB2AMR with the IF kernel) and for a tree whose
     masks were built with subtract_non_leaf = 1, ERR_ARG otherwise."""
     import subprocess
 

@@ -157,3 +157,64 @@ def test_stream_solve_is_the_base_level_direct_sum():
     other = np.ones(info["nblocks"], bool)
     other[leaves] = False
     assert not phi[other].any()
+
+
+def test_ib2amr_single_level_is_the_direct_sum_from_ib_blocks():
+    """fmm_type IB2AMR (pressure correction, linsys.hpp:131-145; masks
+    fmm.hpp:146-195) on a one-level mesh: the blocks that hold
+    immersed-boundary points are the only sources and every block a target,
+    so the solve on the leaves equals the direct LGF sum over the interiors of
+    the is_ib blocks.  The masks follow the rule per level (mask index 4 + rl)."""
+    from iblgf_amd import Tree, setups as S
+
+    # 4^3 base blocks: every pair of blocks is in each other's neighbour or
+    # influence lists, so the pass is exact (no far field through coarser levels)
+    t = Tree((-16, -16, -16), (32, 32, 32), 8, 1.0 / 32, [(-16, -16, -16, 32, 32, 32)])
+    pts = S.fibonacci_sphere(40, 0.03) - 0.3  # near a corner: only some blocks hold (or border) points
+    t.set_ib(pts, 0, 2, 2.0)
+    t.init_refine(0)
+    t.build_masks()
+    a, info, n = t.export(), t.info(), t.n
+    isib = (a["flags"] & 16) != 0
+    bl = info["base_level"]
+    one = np.uint64(1)
+    ms, mt = (a["mask_src"] >> np.uint64(4)) & one, (a["mask_tgt"] >> np.uint64(4)) & one
+    base = a["level"] == bl
+    assert isib[base].any() and not isib[base].all()
+    assert np.array_equal(ms[base] == 1, isib[base]) and np.all(mt[base] == 1)
+    rng = np.random.default_rng(5)
+    src = rng.standard_normal((info["nblocks"], n, n, n))
+    phi = O.Tree(a, info, n).apply_lgf_masked(src, 4)
+    sx, q = [], []
+    kk, jj, ii = np.meshgrid(np.arange(1, n - 1), np.arange(1, n - 1), np.arange(1, n - 1), indexing="ij")
+    for s in np.nonzero(base & isib)[0]:
+        b = a["bbase"][s]
+        sx.append(np.stack([b[0] + ii - 1, b[1] + jj - 1, b[2] + kk - 1], -1).reshape(-1, 3))
+        q.append(src[s, 1:-1, 1:-1, 1:-1].reshape(-1))
+    sx, q = np.concatenate(sx), np.concatenate(q)
+    K, J, I = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    leaves = np.nonzero(base & ((a["flags"] & FLAG_LEAF) != 0))[0]
+    for s in leaves[[0, len(leaves) // 2, -1]]:
+        b = a["bbase"][s]
+        tx = np.stack([b[0] + I - 1, b[1] + J - 1, b[2] + K - 1], -1).reshape(-1, 3)
+        direct = O.direct_sum(sx, q, tx, info["dx_base"] ** 2)
+        assert np.max(np.abs(phi[s].reshape(-1) - direct)) < 1e-11 * np.max(np.abs(direct))
+
+
+def test_ib2amr_masks_mark_ib_ancestors_per_level():
+    s = setups.IbSphere(npoints=600, block_extent=8, nref=2, half_width=1.0)
+    a, info = s.arrays if hasattr(s, "arrays") else s.tree.export(), s.tree.info()
+    bl, depth = info["base_level"], info["depth"]
+    isib = (a["flags"] & 16) != 0
+    sub = isib.copy()  # is_ib or a descendant that is
+    for l in range(depth - 2, bl - 1, -1):
+        for o in np.nonzero(a["level"] == l)[0]:
+            if any(c >= 0 and sub[c] for c in a["children"][o]):
+                sub[o] = True
+    one = np.uint64(1)
+    for l in range(bl, depth):
+        idx = np.uint64(4 + l - bl)
+        ms, mt = (a["mask_src"] >> idx) & one, (a["mask_tgt"] >> idx) & one
+        at = a["level"] == l
+        assert np.all(mt[at] == 1) and np.array_equal(ms[at] == 1, sub[at])
+        assert not ms[a["level"] > l].any() and not mt[a["level"] > l].any()
