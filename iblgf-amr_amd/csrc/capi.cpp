@@ -206,7 +206,15 @@ int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int ke
 int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
     double alpha_base)
 {
+    return iblgf_gpu_apply_lgf_mesh(ctx, src_fields, dst_fields, k, kernel, alpha_base, IBLGF_MESH_CELL);
+}
+
+int iblgf_gpu_apply_lgf_mesh(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
+    double alpha_base, int mesh_type)
+{
     NEED_CTX("iblgf_gpu_apply_lgf_batch");
+    if (mesh_type != IBLGF_MESH_CELL && mesh_type != IBLGF_MESH_FACE && mesh_type != IBLGF_MESH_EDGE)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_mesh: unknown mesh type", ctx);
     if (kernel != IBLGF_KERNEL_LAPLACE && kernel != IBLGF_KERNEL_IF)
         return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_batch: unknown kernel", ctx);
     if (!src_fields || !dst_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_batch: null field list", ctx);
@@ -217,9 +225,9 @@ int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const i
             if (src_fields[c] == dst_fields[e] || (c != e && dst_fields[c] == dst_fields[e]))
                 return fail(IBLGF_ERR_ARG, "iblgf_gpu_apply_lgf_batch: fields overlap", ctx);
     return guarded(ctx, [&] {
-        if (kernel == IBLGF_KERNEL_LAPLACE) ctx->solver->apply_lgf_batch(src_fields, dst_fields, k);
+        if (kernel == IBLGF_KERNEL_LAPLACE) ctx->solver->apply_lgf_batch(src_fields, dst_fields, k, mesh_type);
         else
-            ctx->solver->apply_if_batch(src_fields, dst_fields, k, alpha_base);
+            ctx->solver->apply_if_batch(src_fields, dst_fields, k, alpha_base, mesh_type);
     });
 }
 
@@ -234,8 +242,11 @@ int iblgf_gpu_solve_sequence(iblgf_gpu_ctx* ctx, int nitems, const iblgf_solve_i
         if ((it.kernel != IBLGF_KERNEL_LAPLACE && it.kernel != IBLGF_KERNEL_IF) || it.k < 1 ||
             it.k > GpuSolver::kMaxComponents || !it.src_fields || !it.dst_fields)
             return fail(IBLGF_ERR_ARG, "iblgf_gpu_solve_sequence: item " + std::to_string(i) + " is invalid", ctx);
+        if (it.mesh_type != IBLGF_MESH_CELL && it.mesh_type != IBLGF_MESH_FACE && it.mesh_type != IBLGF_MESH_EDGE)
+            return fail(IBLGF_ERR_ARG, "iblgf_gpu_solve_sequence: item " + std::to_string(i) + " mesh type", ctx);
         v[i].laplace = it.kernel == IBLGF_KERNEL_LAPLACE;
         v[i].alpha = it.alpha_base;
+        v[i].mesh = it.mesh_type;
         v[i].src.assign(it.src_fields, it.src_fields + it.k);
         v[i].dst.assign(it.dst_fields, it.dst_fields + it.k);
     }

@@ -243,24 +243,31 @@ void GpuSolver::upload_forest(const std::vector<const Octree*>& trees)
     }
     d_neighbor_.upload(neighbor_);
 
-    // transfer matrices
+    // transfer matrices: the FMM's Lagrange operators [2][n*n]; the
+    // cell-centred restriction and interpolation [2 relative positions][2][n*n]
     {
         const auto A = nli_matrix(n_, 10);
-        const auto C = cc_interp_matrix(n_, intrp_order_, 1);
-        std::vector<double> an(2 * n_ * n_), in(2 * n_ * n_), cr(2 * n_ * n_), ci(2 * n_ * n_);
-        for (int b = 0; b < 2; ++b)
+        std::vector<double> an(2 * n_ * n_), in(2 * n_ * n_), cr(4 * n_ * n_), ci(4 * n_ * n_);
+        for (int rel = 0; rel < 2; ++rel)
         {
-            const auto R = cc_restrict_matrix(n_, b, 1);
-            for (int p = 0; p < n_; ++p)
-                for (int j = 0; j < n_; ++j)
-                {
-                    const double a = A[(size_t)p * 2 * n_ + b * n_ + j];
-                    const double c = C[(size_t)p * 2 * n_ + b * n_ + j];
-                    an[b * n_ * n_ + p * n_ + j] = a;  // [parent][child]
-                    in[b * n_ * n_ + j * n_ + p] = a;  // [child][parent]
-                    ci[b * n_ * n_ + j * n_ + p] = c;
-                    cr[b * n_ * n_ + p * n_ + j] = R[(size_t)p * n_ + j];
-                }
+            const auto C = cc_interp_matrix(n_, intrp_order_, rel);
+            for (int b = 0; b < 2; ++b)
+            {
+                const auto   R = cc_restrict_matrix(n_, b, rel);
+                const size_t o = static_cast<size_t>(rel * 2 + b) * n_ * n_;
+                for (int p = 0; p < n_; ++p)
+                    for (int j = 0; j < n_; ++j)
+                    {
+                        if (rel == 0)
+                        {
+                            const double a = A[(size_t)p * 2 * n_ + b * n_ + j];
+                            an[b * n_ * n_ + p * n_ + j] = a;  // [parent][child]
+                            in[b * n_ * n_ + j * n_ + p] = a;  // [child][parent]
+                        }
+                        ci[o + j * n_ + p] = C[(size_t)p * 2 * n_ + b * n_ + j];
+                        cr[o + p * n_ + j] = R[(size_t)p * n_ + j];
+                    }
+            }
         }
         m_nli_anterp_.upload(an);
         m_nli_interp_.upload(in);
@@ -1050,12 +1057,28 @@ std::vector<int64_t> GpuSolver::batch_key(int kind, const int* src, const int* d
     return key;
 }
 
-void GpuSolver::apply_lgf_batch(const int* src, const int* dst, int k)
+static void check_mesh(int mesh)
 {
+    if (mesh != kMeshCell && mesh != kMeshFace && mesh != kMeshEdge) throw std::invalid_argument("unknown mesh type");
+}
+
+void GpuSolver::apply_lgf_batch(const int* src, const int* dst, int k, int mesh)
+{
+    check_mesh(mesh);
     check_batch(src, dst, k);
-    run_captured(
-        batch_key(0, src, dst, k, 0.0), [this] { prepare_laplace(); },
-        [this, src, dst, k] { lgf_batch_body(src, dst, k); });
+    auto key = batch_key(0, src, dst, k, 0.0);
+    key.push_back(mesh);
+    mesh_ = mesh;
+    try
+    {
+        run_captured(key, [this] { prepare_laplace(); }, [this, src, dst, k] { lgf_batch_body(src, dst, k); });
+    }
+    catch (...)
+    {
+        mesh_ = kMeshCell;
+        throw;
+    }
+    mesh_ = kMeshCell;
 }
 
 void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
@@ -1080,7 +1103,7 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
     {
         launch::transfer_gather(
-            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b);
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b, false, mesh_);
         ncoarse += p_kids_[ls].size();
     }
     timer_.end(stream_, KC_TRANSFER, ncoarse * k);
@@ -1089,7 +1112,8 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
         const double dx = dx_base_ / std::pow(2.0, l - base_level_);
         run_pass(pass(l, 1, false), dx * dx, true, 0.0, b);
         timer_.begin(stream_);
-        launch::transfer_scatter(stream_, n_, tt, tt, p_ccint_[l].p, p_ccint_[l].size() / 3, m_cc_interp_.p, false, b);
+        launch::transfer_scatter(
+            stream_, n_, tt, tt, p_ccint_[l].p, p_ccint_[l].size() / 3, m_cc_interp_.p, false, b, mesh_);
         timer_.end(stream_, KC_TRANSFER, static_cast<long>(p_ccint_[l].size() / 3) * k);
         run_pass(pass(l, 0, false), dx * dx, true, 0.0, b);
         timer_.begin(stream_);
@@ -1103,7 +1127,7 @@ void GpuSolver::lgf_batch_body(const int* src, const int* dst, int k)
             launch::buffer_exchange(
                 stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p, b);
             launch::transfer_scatter(
-                stream_, n_, st, st, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, true, b);
+                stream_, n_, st, st, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, true, b, mesh_);
             const double dxc = dx / 2.0;
             launch::correction_stencil(
                 stream_, n_, tt, st, p_kids_[l].p, p_kids_[l].size(), 1.0 / (dxc * dxc), 0.0, b);
@@ -1127,9 +1151,11 @@ void GpuSolver::solve_sequence(const std::vector<SolveItem>& items)
     std::vector<int64_t> key = {2, static_cast<int64_t>(items.size())};
     for (const SolveItem& it : items)
     {
+        check_mesh(it.mesh);
         const auto k = batch_key(it.laplace ? 0 : 1, it.src.data(), it.dst.data(), static_cast<int>(it.src.size()),
             it.laplace ? 0.0 : it.alpha);
         key.insert(key.end(), k.begin(), k.end());
+        key.push_back(it.mesh);
     }
     run_captured(
         key,
@@ -1145,11 +1171,14 @@ void GpuSolver::solve_sequence(const std::vector<SolveItem>& items)
             for (const SolveItem& it : items)
             {
                 const int k = static_cast<int>(it.src.size());
+                mesh_ = it.mesh;
                 if (it.laplace) lgf_batch_body(it.src.data(), it.dst.data(), k);
                 else
                     if_batch_body(it.src.data(), it.dst.data(), k, it.alpha);
             }
+            mesh_ = kMeshCell;
         });
+    mesh_ = kMeshCell;
 }
 
 void GpuSolver::set_launch_shapes(int mac_policy, int dft_grid)
@@ -1234,12 +1263,24 @@ std::string GpuSolver::autotune(const int* src, const int* dst, int k, bool lapl
 // PoissonSolver::apply_if (include/iblgf/solver/poisson/poisson.hpp:435-494)
 void GpuSolver::apply_if(int src, int dst, double alpha_base) { apply_if_batch(&src, &dst, 1, alpha_base); }
 
-void GpuSolver::apply_if_batch(const int* src, const int* dst, int k, double alpha_base)
+void GpuSolver::apply_if_batch(const int* src, const int* dst, int k, double alpha_base, int mesh)
 {
+    check_mesh(mesh);
     check_batch(src, dst, k);
-    run_captured(
-        batch_key(1, src, dst, k, alpha_base), [this, alpha_base] { prepare_if(alpha_base); },
-        [this, src, dst, k, alpha_base] { if_batch_body(src, dst, k, alpha_base); });
+    auto key = batch_key(1, src, dst, k, alpha_base);
+    key.push_back(mesh);
+    mesh_ = mesh;
+    try
+    {
+        run_captured(key, [this, alpha_base] { prepare_if(alpha_base); },
+            [this, src, dst, k, alpha_base] { if_batch_body(src, dst, k, alpha_base); });
+    }
+    catch (...)
+    {
+        mesh_ = kMeshCell;
+        throw;
+    }
+    mesh_ = kMeshCell;
 }
 
 bool GpuSolver::has_ghat(bool laplace, int d, double alpha) const
@@ -1320,12 +1361,12 @@ void GpuSolver::if_batch_body(const int* src, const int* dst, int k, double alph
         stream_, n_, work_table(st, k), field_table(src, k), p_leaves_.p, p_leaves_.size(), false, k);
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
         launch::transfer_gather(
-            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b);
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b, false, mesh_);
     for (int l = depth_ - 2; l >= base_level_; --l)
     {
         launch::buffer_exchange(stream_, n_, st, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p, b);
         launch::transfer_scatter(
-            stream_, n_, st, st, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false, b);
+            stream_, n_, st, st, p_ccint_corr_[l].p, p_ccint_corr_[l].size() / 3, m_cc_interp_.p, false, b, mesh_);
     }
     for (int l = base_level_; l < depth_; ++l)
     {
@@ -1378,7 +1419,7 @@ void GpuSolver::apply_lgf_ib2amr(int src, int dst)
     launch::copy_blocks(stream_, n_, st, F(src), p_level_slots_[Lf].p, p_level_slots_[Lf].size(), true);
     for (int ls = depth_ - 2; ls >= base_level_; --ls)
         launch::transfer_gather(
-            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b);
+            stream_, n_, st, st, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9, m_cc_restrict_.p, true, b, false, mesh_);
     for (int l = base_level_; l < depth_; ++l)
     {
         const double dx = dx_base_ / std::pow(2.0, l - base_level_);

@@ -167,8 +167,12 @@ class GpuSolver
     // The same solves of k components (src[c] -> dst[c], e.g. the three
     // vorticity or velocity components) with every launch covering all k;
     // each component's result equals its single-component solve.
-    void apply_lgf_batch(const int* src, const int* dst, int k);
-    void apply_if_batch(const int* src, const int* dst, int k, double alpha_base);
+    // mesh: kMeshCell (scalar cell fields), kMeshFace or kMeshEdge (component
+    // c is field index c % 3): the mesh type the level transfers of the solve
+    // (coarsening, interpolation, correction) take their relative positions
+    // from, as Source::mesh_type() in the reference (poisson.hpp:632, 1177).
+    void apply_lgf_batch(const int* src, const int* dst, int k, int mesh = kMeshCell);
+    void apply_if_batch(const int* src, const int* dst, int k, double alpha_base, int mesh = kMeshCell);
     // Size the work arrays for k components ahead of a batched solve.
     void reserve_components(int k);
     // Batched solves issued in order (each as apply_lgf_batch / apply_if_batch);
@@ -178,6 +182,7 @@ class GpuSolver
     {
         bool             laplace = true;
         double           alpha = 0.0;
+        int              mesh = kMeshCell;
         std::vector<int> src, dst;
     };
     void solve_sequence(const std::vector<SolveItem>& items);
@@ -293,6 +298,7 @@ class GpuSolver
     hipEvent_t  ev_fork_ = nullptr, ev_join_ = nullptr;
     bool        concurrent_ = true;
     int         mac_policy_ = 0, dft_grid_ = 0;  // launch shapes (Batch)
+    int         mesh_ = kMeshCell;               // mesh type of the solve in progress
     bool        uploaded_ = false;
     KernelTimer timer_;
 

@@ -224,13 +224,28 @@ __device__ inline void leg_z_acc(const double* t2, const double* __restrict__ Kz
     }
 }
 
+// Side matrix of one leg.  mesh < 0: mats is [2][N*N] (by the child's bit on
+// the leg's axis), the FMM's Lagrange operators.  mesh >= 0: mats is
+// [2 relative positions][2][N*N], the cell-centred level transfers, with the
+// relative position on each axis set by the field's mesh type
+// (cell_center_nli_intrp.hpp:599-618): cell 1 on every axis; face component
+// fi 0 on axis fi, 1 elsewhere; edge component fi 1 on axis fi, 0 elsewhere.
+template<int N>
+__device__ inline const double* side_mat(const double* mats, int mesh, int fi, int axis, int bit)
+{
+    constexpr int NN = N * N;
+    if (mesh < 0) return mats + bit * NN;
+    const int rel = mesh == 1 ? (axis == fi ? 0 : 1) : (mesh == 2 ? (axis == fi ? 1 : 0) : 1);
+    return mats + (rel * 2 + bit) * NN;
+}
+
 // dst[parent] (=|+=) sum over listed children of K(child); one workgroup per
 // parent, the children's contributions summed in registers in child order and
 // written to (added into) the parent once, every point of it (deterministic).
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
-    const double* __restrict__ mats, int accumulate, int ring0, long cs)
+    const double* __restrict__ mats, int accumulate, int ring0, long cs, int mesh, int comp0)
 {
     constexpr int N3 = Geo<N>::N3;
     __shared__ TransferLds<N> L;
@@ -238,6 +253,7 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
     if (e >= count) return;
     src += blockIdx.y * cs;
     dst += blockIdx.y * cs;
+    const int  fi = (comp0 + (int)blockIdx.y) % 3;
     const int* ent = entries + 9 * e;
     double     acc[N];
 #pragma unroll
@@ -252,9 +268,10 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
         first = false;
         int nx = -1;  // the next listed child, touched ahead of its x leg
         for (int d = c + 1; d < 8 && nx < 0; ++d) nx = ent[1 + d];
-        transfer_xy<N>(src + (size_t)ch * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N, ring0,
-            nx >= 0 ? src + (size_t)nx * N3 : nullptr, &pend, &touch);
-        if (threadIdx.x < N * N) leg_z_acc<N>(L.buf, mats + ((c >> 2) & 1) * N * N, acc);
+        transfer_xy<N>(src + (size_t)ch * N3, L, side_mat<N>(mats, mesh, fi, 0, c & 1),
+            side_mat<N>(mats, mesh, fi, 1, (c >> 1) & 1), ring0, nx >= 0 ? src + (size_t)nx * N3 : nullptr, &pend,
+            &touch);
+        if (threadIdx.x < N * N) leg_z_acc<N>(L.buf, side_mat<N>(mats, mesh, fi, 2, (c >> 2) & 1), acc);
     }
     asm volatile("" ::"v"(touch + pend));  // the touch loads are kept, their value unused
     if (threadIdx.x < N * N)
@@ -269,7 +286,7 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_gather(
 template<int N>
 __global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
     const double* __restrict__ src, double* __restrict__ dst, const int* __restrict__ entries, int count,
-    const double* __restrict__ mats, int accumulate, long cs)
+    const double* __restrict__ mats, int accumulate, long cs, int mesh, int comp0)
 {
     constexpr int N3 = Geo<N>::N3;
     __shared__ TransferLds<N> L;
@@ -277,14 +294,16 @@ __global__ __launch_bounds__(kTransferThreads) void k_transfer_scatter(
     if (e >= count) return;
     src += blockIdx.y * cs;
     dst += blockIdx.y * cs;
+    const int fi = (comp0 + (int)blockIdx.y) % 3;
     const int child = entries[3 * e], parent = entries[3 * e + 1], c = entries[3 * e + 2];
-    transfer_xy<N>(src + (size_t)parent * N3, L, mats + (c & 1) * N * N, mats + ((c >> 1) & 1) * N * N);
+    transfer_xy<N>(src + (size_t)parent * N3, L, side_mat<N>(mats, mesh, fi, 0, c & 1),
+        side_mat<N>(mats, mesh, fi, 1, (c >> 1) & 1));
     if (threadIdx.x < N * N)
     {
         double acc[N];
 #pragma unroll
         for (int k = 0; k < N; ++k) acc[k] = 0.0;
-        leg_z_acc<N>(L.buf, mats + ((c >> 2) & 1) * N * N, acc);
+        leg_z_acc<N>(L.buf, side_mat<N>(mats, mesh, fi, 2, (c >> 2) & 1), acc);
         double* out = dst + (size_t)child * N3 + threadIdx.x;
 #pragma unroll
         for (int k = 0; k < N; ++k)
@@ -1157,19 +1176,19 @@ template<int N> struct RunAxpy
 template<int N> struct RunGather
 {
     static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m,
-        int accumulate, int ring0, const Batch& b)
+        int accumulate, int ring0, const Batch& b, int mesh)
     {
         hipLaunchKernelGGL(k_transfer_gather<N>, dim3(count, b.k), dim3(kTransferThreads), 0, s, src, dst, e, count,
-            m, accumulate, ring0, b.fstride);
+            m, accumulate, ring0, b.fstride, mesh, b.comp0);
     }
 };
 template<int N> struct RunScatter
 {
     static void run(hipStream_t s, const double* src, double* dst, const int* e, int count, const double* m, int acc,
-        const Batch& b)
+        const Batch& b, int mesh)
     {
         hipLaunchKernelGGL(k_transfer_scatter<N>, dim3(count, b.k), dim3(kTransferThreads), 0, s, src, dst, e, count,
-            m, acc, b.fstride);
+            m, acc, b.fstride, mesh, b.comp0);
     }
 };
 template<int N> struct RunHalo
@@ -1339,15 +1358,16 @@ void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int
     if (count > 0) dispatch_n<RunAxpy>(n, s, dst, src, slots, count, a, b);
 }
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate, const Batch& b, bool children_interior)
+    const double* mats, bool accumulate, const Batch& b, bool children_interior, int mesh)
 {
     if (count > 0)
-        dispatch_n<RunGather>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, children_interior ? 1 : 0, b);
+        dispatch_n<RunGather>(
+            n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, children_interior ? 1 : 0, b, mesh);
 }
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate, const Batch& b)
+    const double* mats, bool accumulate, const Batch& b, int mesh)
 {
-    if (count > 0) dispatch_n<RunScatter>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, b);
+    if (count > 0) dispatch_n<RunScatter>(n, s, src, dst, entries, count, mats, accumulate ? 1 : 0, b, mesh);
 }
 void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor,
     const Batch& b)

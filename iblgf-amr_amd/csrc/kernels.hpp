@@ -50,7 +50,13 @@ struct Batch
     // workgroup.
     int mac_policy = 0;
     int dft_grid = 0;
+    // Field index of component 0 (vector fields: component c is field index
+    // (comp0 + c) % 3), for the mesh-type level transfers.
+    int comp0 = 0;
 };
+
+// Mesh types of the level transfers (MeshObject of the reference).
+constexpr int kMeshCell = 0, kMeshFace = 1, kMeshEdge = 2;
 
 // Per-component field base pointers, passed by value (component c in p[c]).
 // Bit c of f32 set: component c is an FP32 field and p[c] holds its float
@@ -95,14 +101,17 @@ void axpy_blocks(hipStream_t s, int n, double* dst, const double* src, const int
     double a, const Batch& b = {});
 
 // dst[entry.parent] (=|+=) sum_c K(child) for entries {parent, child[8]}; K
-// built from the side matrices mats[2][n*n] (row = output point, col = input
-// point).
+// built from side matrices (row = output point, col = input point):
+// mesh < 0, mats[2][n*n] by the child's bit on each axis (the FMM's Lagrange
+// operators); mesh = kMeshCell / kMeshFace / kMeshEdge, mats[2 relative
+// positions][2][n*n] with each axis's position set by the mesh type and the
+// component's field index (b.comp0 + component) % 3.
 // children_interior: the children's boundary rings are read as zero.
 void transfer_gather(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate, const Batch& b = {}, bool children_interior = false);
+    const double* mats, bool accumulate, const Batch& b = {}, bool children_interior = false, int mesh = -1);
 // dst[entry.child] (=|+=) K(src[entry.parent]) for entries {child, parent, cidx}.
 void transfer_scatter(hipStream_t s, int n, const double* src, double* dst, const int* entries, int count,
-    const double* mats, bool accumulate, const Batch& b = {});
+    const double* mats, bool accumulate, const Batch& b = {}, int mesh = -1);
 
 void buffer_exchange(hipStream_t s, int n, double* f, const int* slots, int count, const int* neighbor,
     const Batch& b = {});

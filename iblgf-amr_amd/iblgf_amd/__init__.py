@@ -37,6 +37,7 @@ PREC_FP16X3 = 3
 PREC_FP16 = 4
 PREC_INT8X4 = 5
 PREC_FP16X3_FORWARD = 6
+MESH_CELL, MESH_FACE, MESH_EDGE = range(3)
 
 FLAG_LEAF = 1
 FLAG_CORRECTION = 2
@@ -124,6 +125,7 @@ SYMBOLS = {
     "iblgf_gpu_field_upload_f32": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
     "iblgf_gpu_field_download_f32": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
     "iblgf_gpu_solve_sequence": (_I, [_P, _I, _P]),
+    "iblgf_gpu_apply_lgf_mesh": (_I, [_P, _IP, _IP, _I, _I, _D, _I]),
     "iblgf_gpu_autotune": (_I, [_P, _IP, _IP, _I, _I, _D, _I, ctypes.c_char_p, _I]),
     "iblgf_gpu_launch_shapes": (_I, [_P, _IP, _IP, _IP]),
     "iblgf_gpu_set_launch_shapes": (_I, [_P, _I, _I]),
@@ -325,7 +327,7 @@ class Tree:
 # ---------------------------------------------------------------------------
 class SolveItem(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int), ("alpha_base", ctypes.c_double), ("k", ctypes.c_int),
-                ("src_fields", _IP), ("dst_fields", _IP)]
+                ("src_fields", _IP), ("dst_fields", _IP), ("mesh_type", ctypes.c_int)]
 
 
 class PoissonSolver:
@@ -437,15 +439,20 @@ class PoissonSolver:
         """PoissonSolver::apply_lgf_IF for one component (asynchronous)."""
         _check(lib().iblgf_gpu_apply_lgf(self._h, int(src), int(dst), KERNEL_IF, float(alpha_base)), self._h)
 
-    def apply_lgf_batch(self, srcs, dsts, kernel=KERNEL_LAPLACE, alpha_base=0.0):
+    def apply_lgf_batch(self, srcs, dsts, kernel=KERNEL_LAPLACE, alpha_base=0.0, mesh=MESH_CELL):
         """The component loop of apply_lgf / apply_lgf_IF (poisson.hpp:130-145,
-        419-434) with all components in every launch (asynchronous)."""
+        419-434) with all components in every launch (asynchronous); mesh is the
+        fields' mesh type (MESH_FACE / MESH_EDGE: component c is field index c % 3)."""
         k = len(srcs)
         if len(dsts) != k:
             raise ValueError("srcs and dsts differ in length")
         a = (ctypes.c_int * k)(*[int(x) for x in srcs])
         b = (ctypes.c_int * k)(*[int(x) for x in dsts])
-        _check(lib().iblgf_gpu_apply_lgf_batch(self._h, a, b, k, int(kernel), float(alpha_base)), self._h)
+        if mesh == MESH_CELL:
+            _check(lib().iblgf_gpu_apply_lgf_batch(self._h, a, b, k, int(kernel), float(alpha_base)), self._h)
+        else:
+            _check(lib().iblgf_gpu_apply_lgf_mesh(self._h, a, b, k, int(kernel), float(alpha_base), int(mesh)),
+                   self._h)
 
     def set_graphs(self, on):
         """Replay repeated solves from captured HIP graphs (iblgf_gpu_set_graphs)."""
@@ -523,11 +530,14 @@ class PoissonSolver:
         apply_lgf_batch calls; with graphs on, one captured graph for all."""
         keep = []
         arr = (SolveItem * len(items))()
-        for i, (kernel, alpha, srcs, dsts) in enumerate(items):
+        for i, item in enumerate(items):
+            kernel, alpha, srcs, dsts = item[:4]
+            mesh = item[4] if len(item) > 4 else MESH_CELL
             a = (ctypes.c_int * len(srcs))(*srcs)
             b = (ctypes.c_int * len(dsts))(*dsts)
             keep += [a, b]
-            arr[i] = SolveItem(int(kernel), float(alpha), len(srcs), ctypes.cast(a, _IP), ctypes.cast(b, _IP))
+            arr[i] = SolveItem(int(kernel), float(alpha), len(srcs), ctypes.cast(a, _IP), ctypes.cast(b, _IP),
+                               int(mesh))
         _check(lib().iblgf_gpu_solve_sequence(self._h, len(items), ctypes.cast(arr, _P)), self._h)
 
     def autotune(self, srcs, dsts, kernel=KERNEL_LAPLACE, alpha_base=0.0, reps=3):

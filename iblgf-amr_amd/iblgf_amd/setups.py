@@ -405,16 +405,27 @@ class AnalyticVortexRing:
         thr = self.source_max * self.refinement_factor ** diff
         return bool(np.max(np.abs(self.laplacian(X, Y, Z))) > thr)
 
-    def _points(self, s):
-        return cell_centres(self.arrays, s, self.tree.n, self.dx_base, self.info["base_level"])
+    def _points(self, s, mesh=0, comp=0):
+        """Sample positions of block s for a field of mesh type mesh (0 cell,
+        1 face, 2 edge) and field index comp: cell centres, moved half a cell
+        down on axis comp (face) or on the other two axes (edge)."""
+        P = cell_centres(self.arrays, s, self.tree.n, self.dx_base, self.info["base_level"])
+        if mesh:
+            rl = max(int(self.arrays["level"][s]) - self.info["base_level"], 0)
+            dx = self.dx_base / 2 ** rl
+            for ax in range(3):
+                if (mesh == 1 and ax == comp) or (mesh == 2 and ax != comp):
+                    P[..., ax] -= 0.5 * dx
+        return P
 
-    def source(self):
-        """The source on the interior cells of every leaf (vortexrings.hpp:384-410)."""
+    def source(self, mesh=0, comp=0):
+        """The source on the interior cells of every leaf (vortexrings.hpp:384-410),
+        sampled where a field of mesh type `mesh` and field index `comp` sits."""
         n = self.tree.n
         f = np.zeros((self.info["nblocks"], n, n, n))
         inner = _interior_mask(n)
         for s in np.nonzero(self.arrays["flags"] & FLAG_LEAF)[0]:
-            P = self._points(s)
+            P = self._points(s, mesh, comp)
             f[s][inner] = self.laplacian(P[..., 0], P[..., 1], P[..., 2])[inner]
         return f
 
@@ -436,10 +447,10 @@ class AnalyticVortexRing:
         L = self.info["depth"] - 1
         return np.nonzero((a["level"] == L) & ((a["flags"] & FLAG_LEAF) != 0) & ((a["flags"] & FLAG_CORRECTION) == 0))[0]
 
-    def linf_error(self, phi):
-        """max |phi - psi| over the interiors of finest_blocks()."""
+    def linf_error(self, phi, mesh=0, comp=0):
+        """max |phi - psi| over the interiors of finest_blocks() (psi sampled as source(mesh, comp))."""
         sel = self.finest_blocks()
-        ex = np.stack([self.psi(*np.moveaxis(self._points(s), -1, 0)) for s in sel])
+        ex = np.stack([self.psi(*np.moveaxis(self._points(s, mesh, comp), -1, 0)) for s in sel])
         return float(np.max(np.abs(phi[sel][:, 1:-1, 1:-1, 1:-1] - ex[:, 1:-1, 1:-1, 1:-1])))
 
 

@@ -221,6 +221,26 @@ int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int ke
 int iblgf_gpu_apply_lgf_batch(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
     double alpha_base);
 
+/* Mesh type of a field (MeshObject, include/iblgf/domain/dataFields/datafield.hpp):
+ * where its samples sit in a cell, which sets the relative positions of the
+ * level transfers on each axis (cell_center_nli_intrp.hpp:599-618): cell 1 on
+ * every axis; face component c 0 on axis c, 1 elsewhere; edge component c 1
+ * on axis c, 0 elsewhere (1 = half a cell off the child's samples). */
+enum iblgf_mesh
+{
+    IBLGF_MESH_CELL = 0,
+    IBLGF_MESH_FACE = 1,
+    IBLGF_MESH_EDGE = 2
+};
+/* iblgf_gpu_apply_lgf_batch for fields of a mesh type, as apply_lgf /
+ * apply_lgf_IF coarsen, interpolate and correct with Source::mesh_type()
+ * (poisson.hpp:632, 456-460, 1177; the integrating factor of the IF-HERK
+ * stages acts on face fields, ifherk.hpp:1155-1156, 1184): component c of the
+ * batch is field index c % 3 (so a batch of two face vectors is {x, y, z, x,
+ * y, z}).  IBLGF_MESH_CELL is iblgf_gpu_apply_lgf_batch. */
+int iblgf_gpu_apply_lgf_mesh(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
+    double alpha_base, int mesh_type);
+
 /* Replay batched solves (iblgf_gpu_apply_lgf, iblgf_gpu_apply_lgf_batch) from
  * captured HIP graphs: the first call with a given kernel, field set, k,
  * precision and alpha prepares its plans and kernel spectra and records its
@@ -264,6 +284,7 @@ typedef struct iblgf_solve_item
     int         k;
     const int*  src_fields;
     const int*  dst_fields;
+    int         mesh_type; /* IBLGF_MESH_*, as iblgf_gpu_apply_lgf_mesh */
 } iblgf_solve_item;
 int iblgf_gpu_solve_sequence(iblgf_gpu_ctx* ctx, int nitems, const iblgf_solve_item* items);
 

@@ -648,6 +648,12 @@ typedef struct
     int                ranks; /* > 0: Bx and source transforms on `ranks` workers, contiguous load split */
     /* transfer matrices: nli (anterp/interp) and cell-centre (restrict/interp) */
     double *nli_sub[2], *nli_subT[2], *cc_restrict[2], *cc_interpT[2];
+    /* the same at relative position 0 (samples coincide with the child's) */
+    double *cc_restrict0[2], *cc_interpT0[2];
+    /* mesh type of the solved field (0 cell, 1 face, 2 edge) and its field
+     * index: per axis, the relative position of the level transfers
+     * (cell_center_nli_intrp.hpp:599-618) */
+    int mesh, comp;
     /* working fields [nb][n^3] */
     double *source_tmp, *target_tmp, *correction_tmp, *fmm_s, *fmm_t;
     /* per-pass source spectra (non-faithful mode) */
@@ -690,6 +696,15 @@ static void ostate_init(ostate* o, const oracle_tree* t, int faithful, int intrp
         sub_matrix(A, n, b, S);
         transpose(S, n, o->cc_interpT[b]);
     }
+    oracle_lagrange_matrix(n, intrp_order, 0, A);
+    for (b = 0; b < 2; ++b)
+    {
+        o->cc_restrict0[b] = (double*)malloc(sizeof(double) * n * n);
+        o->cc_interpT0[b] = (double*)malloc(sizeof(double) * n * n);
+        restrict_matrix(n, b, 0, o->cc_restrict0[b]);
+        sub_matrix(A, n, b, S);
+        transpose(S, n, o->cc_interpT0[b]);
+    }
     free(A);
     free(S);
     o->source_tmp = (double*)calloc(o->n3 * t->nb, sizeof(double));
@@ -717,6 +732,8 @@ static void ostate_free(ostate* o)
         free(o->nli_subT[b]);
         free(o->cc_restrict[b]);
         free(o->cc_interpT[b]);
+        free(o->cc_restrict0[b]);
+        free(o->cc_interpT0[b]);
     }
     for (s = 0; s < o->t->nb; ++s)
     {
@@ -1070,6 +1087,22 @@ static void buffer_exchange(ostate* o, double* f, int l)
                 }
 }
 
+/* relative position (0 or 1) of the solved field's samples on one axis */
+static int rel_pos(const ostate* o, int axis)
+{
+    if (o->mesh == 1) return axis == o->comp ? 0 : 1; /* face */
+    if (o->mesh == 2) return axis == o->comp ? 1 : 0; /* edge */
+    return 1;                                          /* cell */
+}
+static const double* restrict_of(const ostate* o, int axis, int bit)
+{
+    return rel_pos(o, axis) ? o->cc_restrict[bit] : o->cc_restrict0[bit];
+}
+static const double* interp_of(const ostate* o, int axis, int bit)
+{
+    return rel_pos(o, axis) ? o->cc_interpT[bit] : o->cc_interpT0[bit];
+}
+
 /* cell-centred restriction of all children of octants at level ls into the parent */
 static void source_coarsify(ostate* o, double* f)
 {
@@ -1082,7 +1115,7 @@ static void source_coarsify(ostate* o, double* f)
                 int ch = t->children[8 * s + c], bx, by, bz;
                 if (ch < 0) continue;
                 child_bits(c, &bx, &by, &bz);
-                transfer3(o->n, o->cc_restrict[bx], o->cc_restrict[by], o->cc_restrict[bz], BLK(f, ch),
+                transfer3(o->n, restrict_of(o, 0, bx), restrict_of(o, 1, by), restrict_of(o, 2, bz), BLK(f, ch),
                     BLK(f, s), 1);
             }
 }
@@ -1098,7 +1131,7 @@ static void cc_intrp_node(ostate* o, const double* from, double* to, int s, int 
         if (ch < 0) continue;
         if (correction_only && !IS_CORR(ch)) continue;
         child_bits(c, &bx, &by, &bz);
-        transfer3(o->n, o->cc_interpT[bx], o->cc_interpT[by], o->cc_interpT[bz], BLK(from, s), BLK(to, ch), 0);
+        transfer3(o->n, interp_of(o, 0, bx), interp_of(o, 1, by), interp_of(o, 2, bz), BLK(from, s), BLK(to, ch), 0);
     }
 }
 
@@ -1163,6 +1196,23 @@ static void lgf_solve(ostate* o, const double* src, double* dst)
     }
 }
 
+int oracle_apply_lgf_mesh(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst, int mesh,
+    int comp)
+{
+    ostate  o_;
+    ostate* o = &o_;
+    if (mesh < 0 || mesh > 2 || comp < 0 || comp > 2) return 1;
+    ostate_init(o, t, 0, 5);
+    o->mesh = mesh;
+    o->comp = comp;
+    o->ker.is_if = 0;
+    o->ker.g = g;
+    o->ker.nmax = nmax;
+    lgf_solve(o, src, dst);
+    ostate_free(o);
+    return 0;
+}
+
 int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
     int faithful, int intrp_order)
 {
@@ -1206,13 +1256,55 @@ int oracle_lgf_baseline(const oracle_tree* t, const double* g, int nmax, const d
     return 0;
 }
 
+static int if_solve(const oracle_tree* t, const double* src, double* dst, double alpha_base, int faithful,
+    int intrp_order, int mesh, int comp);
+
 int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double alpha_base, int faithful,
     int intrp_order)
+{
+    return if_solve(t, src, dst, alpha_base, faithful, intrp_order, 0, 0);
+}
+
+int oracle_apply_if_mesh(const oracle_tree* t, const double* src, double* dst, double alpha_base, int mesh, int comp)
+{
+    if (mesh < 0 || mesh > 2 || comp < 0 || comp > 2) return 1;
+    return if_solve(t, src, dst, alpha_base, 0, 5, mesh, comp);
+}
+
+int oracle_mesh_transfer(int n, int mesh, int comp, int op, int child, const double* in, double* out)
+{
+    oracle_tree t;
+    ostate      o_;
+    ostate*     o = &o_;
+    if (n < 4 || mesh < 0 || mesh > 2 || comp < 0 || comp > 2 || op < 0 || op > 1 || child < 0 || child > 7)
+        return 1;
+    memset(&t, 0, sizeof(t));
+    t.n = n; /* no blocks: only the transfer matrices are built */
+    ostate_init(o, &t, 0, 5);
+    o->mesh = mesh;
+    o->comp = comp;
+    {
+        int bx, by, bz;
+        child_bits(child, &bx, &by, &bz);
+        if (op == 0)
+            transfer3(n, restrict_of(o, 0, bx), restrict_of(o, 1, by), restrict_of(o, 2, bz), in, out, 1);
+        else
+            transfer3(n, interp_of(o, 0, bx), interp_of(o, 1, by), interp_of(o, 2, bz), in, out, 0);
+    }
+    ostate_free(o);
+    return 0;
+}
+
+/* apply_if (poisson.hpp:435-494) */
+static int if_solve(const oracle_tree* t, const double* src, double* dst, double alpha_base, int faithful,
+    int intrp_order, int mesh, int comp)
 {
     ostate o_;
     ostate* o = &o_;
     int     l, s;
     ostate_init(o, t, faithful, intrp_order);
+    o->mesh = mesh;
+    o->comp = comp;
     o->ker.is_if = 1;
     for (s = 0; s < t->nb; ++s)
         if (IS_LEAF(s)) memcpy(BLK(o->source_tmp, s), BLK(src, s), sizeof(double) * o->n3);

@@ -81,6 +81,21 @@ int oracle_apply_lgf(const oracle_tree* t, const double* g, int nmax, const doub
 int oracle_lgf_baseline(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst,
     int faithful, int ranks, int nsolves, double* seconds);
 
+/* apply_lgf / apply_lgf_IF (faithful 0, intrp_order 5) of component comp of a
+ * field of mesh type mesh (0 cell, 1 face, 2 edge): the level transfers take
+ * per axis the relative position of cell_center_nli_intrp.hpp:599-618 (cell 1;
+ * face 0 on axis comp, 1 elsewhere; edge 1 on axis comp, 0 elsewhere). */
+int oracle_apply_lgf_mesh(const oracle_tree* t, const double* g, int nmax, const double* src, double* dst, int mesh,
+    int comp);
+int oracle_apply_if_mesh(const oracle_tree* t, const double* src, double* dst, double alpha_base, int mesh, int comp);
+
+/* One level transfer of one block for a field of mesh type mesh, field index
+ * comp, with the solves' operators (intrp_order 5): op 0 restricts child
+ * block `child` (child id, bit a = side on axis a) into the parent block
+ * (out += R in), op 1 interpolates the parent block into that child (out =
+ * I in). */
+int oracle_mesh_transfer(int n, int mesh, int comp, int op, int child, const double* in, double* out);
+
 /* PoissonSolver::apply_lgf_IF (poisson.hpp:427-494), integrating-factor kernel
  * E_a with a_l = a_base * 4^(l-base). */
 int oracle_apply_if(const oracle_tree* t, const double* src, double* dst, double alpha_base,

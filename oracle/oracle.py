@@ -55,6 +55,9 @@ def lib():
         L.oracle_apply_lgf.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _P, _I, _I]
         L.oracle_apply_if.argtypes = [ctypes.POINTER(OracleTree), _P, _P, _D, _I, _I]
         L.oracle_lgf_baseline.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _P, _I, _I, _I, _P]
+        L.oracle_apply_lgf_mesh.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _P, _I, _I]
+        L.oracle_apply_if_mesh.argtypes = [ctypes.POINTER(OracleTree), _P, _P, _D, _I, _I]
+        L.oracle_mesh_transfer.argtypes = [_I, _I, _I, _I, _I, _P, _P]
         L.oracle_bx_sample.argtypes = [ctypes.POINTER(OracleTree), _P, _I, _P, _I, _I, _I, _P, _I, _P, _P]
         L.oracle_bx_sample.restype = ctypes.c_long
         T, B = ctypes.POINTER(OracleTree), ctypes.POINTER(OracleIb)
@@ -109,6 +112,16 @@ def lagrange_matrix(n, cap, rp):
     return A
 
 
+def mesh_transfer(block, mesh, comp, op, child, out=None):
+    """One block's level transfer for a field of mesh type `mesh` and field
+    index `comp`: op 0 restricts child block `child` into `out` (accumulating),
+    op 1 interpolates the parent block into child `child`."""
+    b = np.ascontiguousarray(block, np.float64)
+    o = np.zeros_like(b) if out is None else out
+    assert lib().oracle_mesh_transfer(b.shape[0], int(mesh), int(comp), int(op), int(child), _p(b), _p(o)) == 0
+    return o
+
+
 def direct_sum(src_xyz, q, tgt_xyz, scale, nmax=100):
     sx = np.ascontiguousarray(src_xyz, np.int32)
     tx = np.ascontiguousarray(tgt_xyz, np.int32)
@@ -160,6 +173,22 @@ class Tree:
                                        1 if faithful else 0, int(ranks), int(nsolves), _p(sec))
         assert rc == 0
         return sec[:int(nsolves)], dst
+
+    def apply_lgf_mesh(self, src, mesh, comp, nmax=100):
+        """apply_lgf of component `comp` of a field of mesh type `mesh` (0 cell, 1 face, 2 edge)."""
+        s = np.ascontiguousarray(src, np.float64)
+        dst = np.zeros_like(s)
+        rc = lib().oracle_apply_lgf_mesh(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, _p(s), _p(dst), int(mesh),
+                                         int(comp))
+        assert rc == 0
+        return dst
+
+    def apply_if_mesh(self, src, alpha_base, mesh, comp):
+        s = np.ascontiguousarray(src, np.float64)
+        dst = np.zeros_like(s)
+        rc = lib().oracle_apply_if_mesh(ctypes.byref(self.s), _p(s), _p(dst), float(alpha_base), int(mesh), int(comp))
+        assert rc == 0
+        return dst
 
     def apply_if(self, src, alpha_base, faithful=False, intrp_order=5):
         s = np.ascontiguousarray(src, np.float64)

@@ -385,3 +385,54 @@ def test_stream_type_matches_oracle(oracle_mod, extent):
     print(f"STREAM vs oracle relL2={rel:.3e}", flush=True)
     assert rel < 1e-10
     assert np.all(got[~sel] == 7.0)  # nothing else is written
+
+
+@pytest.mark.parametrize("kernel", ["laplace", "if"])
+def test_face_and_edge_fields_match_oracle(oracle_mod, kernel):
+    """Face and edge fields (iblgf_gpu_apply_lgf_mesh): the level transfers
+    take each axis's relative position from the mesh type and the component's
+    field index (cell_center_nli_intrp.hpp:599-618).  Every component matches
+    the oracle's solve of that mesh type and field index (relative L2 <=
+    1e-10); the cell mesh through the same entry equals the plain batched
+    solve bit for bit; a sequence item with a mesh type equals the call."""
+    from iblgf_amd import KERNEL_IF, KERNEL_LAPLACE, MESH_CELL, MESH_EDGE, MESH_FACE, lib
+
+    kern, alpha = (KERNEL_LAPLACE, 0.0) if kernel == "laplace" else (KERNEL_IF, 0.7)
+    print(f"case: face/edge fields, small adaptive rings, kernel {kernel}", flush=True)
+    v = _small_rings()
+    srcs = [v.source(c) for c in range(3)]
+    solver = PoissonSolver(v.tree)
+    fs = [solver.field() for _ in range(3)]
+    fd = [solver.field() for _ in range(3)]
+    fc = [solver.field() for _ in range(3)]
+    for f, a in zip(fs, srcs):
+        solver.upload(f, a)
+    ot = oracle_mod.Tree(v.arrays, v.info, v.tree.n)
+    sel = _leaf_mask(v.arrays) if kern == KERNEL_LAPLACE else slice(None)
+    results = {}
+    for mesh in (MESH_FACE, MESH_EDGE):
+        solver.apply_lgf_batch(fs, fd, kern, alpha, mesh=mesh)
+        got = [solver.download(f) for f in fd]
+        results[mesh] = got
+        for c in range(3):
+            ref = ot.apply_lgf_mesh(srcs[c], mesh, c) if kern == KERNEL_LAPLACE else \
+                ot.apply_if_mesh(srcs[c], alpha, mesh, c)
+            l2, mx = _rel(got[c][sel], ref[sel])
+            print(f"mesh {mesh} component {c}: relL2={l2:.3e} relmax={mx:.3e}", flush=True)
+            assert l2 < REL_L2 and mx < REL_MAX, (mesh, c)
+    # the two mesh types differ (the transfers moved), and neither is the cell solve
+    solver.apply_lgf_batch(fs, fc, kern, alpha)
+    cell = [solver.download(f) for f in fc]
+    assert not np.array_equal(results[MESH_FACE][0], cell[0])
+    assert not np.array_equal(results[MESH_FACE][0], results[MESH_EDGE][0])
+    # the cell mesh through the mesh entry is the batched solve
+    import ctypes
+    a = (ctypes.c_int * 3)(*fs)
+    b = (ctypes.c_int * 3)(*fd)
+    assert lib().iblgf_gpu_apply_lgf_mesh(solver._h, a, b, 3, kern, alpha, MESH_CELL) == 0
+    assert all(np.array_equal(solver.download(f), r) for f, r in zip(fd, cell))
+    # sequence items carry their mesh type
+    solver.solve_sequence([(kern, alpha, fs, fd, MESH_EDGE), (kern, alpha, fs, fc, MESH_FACE)])
+    assert all(np.array_equal(solver.download(f), r) for f, r in zip(fd, results[MESH_EDGE]))
+    assert all(np.array_equal(solver.download(f), r) for f, r in zip(fc, results[MESH_FACE]))
+    assert lib().iblgf_gpu_apply_lgf_mesh(solver._h, a, b, 3, kern, alpha, 3) != 0

@@ -218,3 +218,59 @@ def test_ib2amr_masks_mark_ib_ancestors_per_level():
         at = a["level"] == l
         assert np.all(mt[at] == 1) and np.array_equal(ms[at] == 1, sub[at])
         assert not ms[a["level"] > l].any() and not mt[a["level"] > l].any()
+
+
+def _mesh_points(n, mesh, comp, fine, child):
+    """Sample positions of the n^3 points of a parent block (fine False) or of
+    its child `child` (fine True) in parent-cell units for a field of mesh type
+    mesh and field index comp: cell centres on axes of relative position 1,
+    cell nodes on axes of relative position 0.  Face fields (normal comp) sit
+    on nodes along comp, edge fields (along comp) on nodes across comp."""
+    idx = np.arange(n) - 1.0
+    axes = []
+    for a in range(3):
+        node = (mesh == 1 and a == comp) or (mesh == 2 and a != comp)
+        if fine:
+            b = (child >> a) & 1
+            x = (b * (n - 2) + idx + (0.0 if node else 0.5)) / 2.0
+        else:
+            x = idx + (0.0 if node else 0.5)
+        axes.append(x)
+    Z, Y, X = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return X, Y, Z
+
+
+@pytest.mark.parametrize("mesh,comp", [(0, 0), (1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2)])
+def test_mesh_transfers_reproduce_polynomials_at_the_mesh_positions(mesh, comp):
+    """The mesh-type level transfers (cell_center_nli_intrp.hpp:599-618) are
+    pinned by geometry: the interpolation reproduces a quadratic sampled at the
+    field's own positions on the parent, at the child's interior positions; the
+    restriction of all eight children reproduces a linear function at the
+    parent's interior positions.  The same transfers with any other mesh type
+    or field index miss by a quarter of a fine cell times the gradient."""
+    n = 18
+    f = lambda X, Y, Z: 0.3 + 0.7 * X - 0.4 * Y + 0.9 * Z + 0.05 * X * Y - 0.03 * Z * Z + 0.02 * X * X
+    lin = lambda X, Y, Z: 0.3 + 0.7 * X - 0.4 * Y + 0.9 * Z
+    inner = (slice(1, -1),) * 3
+    # every other placement (a cell mesh has no field index)
+    wrong = [(m, c) for m, c in [(0, 0)] + [(m, c) for m in (1, 2) for c in range(3)]
+             if (m, c) != (mesh, comp) and not (mesh == 0 and m == 0)]
+    for child in range(8):
+        parent = f(*_mesh_points(n, mesh, comp, False, 0))
+        got = O.mesh_transfer(parent, mesh, comp, 1, child)
+        ref = f(*_mesh_points(n, mesh, comp, True, child))
+        assert np.max(np.abs(got[inner] - ref[inner])) < 1e-11, child
+        for m, c in wrong:
+            bad = O.mesh_transfer(parent, m, c, 1, child)
+            assert np.max(np.abs(bad[inner] - ref[inner])) > 0.05, (m, c, child)
+    out = np.zeros((n, n, n))
+    for child in range(8):
+        fine = lin(*_mesh_points(n, mesh, comp, True, child))
+        O.mesh_transfer(fine, mesh, comp, 0, child, out)
+    ref = lin(*_mesh_points(n, mesh, comp, False, 0))
+    assert np.max(np.abs(out[inner] - ref[inner])) < 1e-12
+    for m, c in wrong:
+        out = np.zeros((n, n, n))
+        for child in range(8):
+            O.mesh_transfer(lin(*_mesh_points(n, mesh, comp, True, child)), m, c, 0, child, out)
+        assert np.max(np.abs(out[inner] - ref[inner])) > 0.05, (m, c)
