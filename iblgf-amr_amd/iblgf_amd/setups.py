@@ -457,6 +457,54 @@ class AnalyticVortexRing:
 # ---------------------------------------------------------------------------
 # Immersed-boundary sphere
 # ---------------------------------------------------------------------------
+def mesh_points(tree_arrays, slot, n, dx_base, base_level, mesh=0, comp=0):
+    """Physical positions [n,n,n,3] of block `slot` for a field of mesh type
+    `mesh` (0 cell, 1 face, 2 edge) and field index `comp`: cell centres moved
+    half a cell down on axis comp (face) or on the two other axes (edge)."""
+    P = cell_centres(tree_arrays, slot, n, dx_base, base_level)
+    if mesh:
+        rl = max(int(tree_arrays["level"][slot]) - base_level, 0)
+        dx = dx_base / 2 ** rl
+        for ax in range(3):
+            if (mesh == 1 and ax == comp) or (mesh == 2 and ax != comp):
+                P[..., ax] -= 0.5 * dx
+    return P
+
+
+class GaussianFlow:
+    """A face velocity u = curl A with A = sum_k a_k exp(-|x - c_k|^2 / s_k^2) e_k
+    (Gaussian vector potentials along unit axes e_k), sampled on every point
+    of every block where the face components sit: u = sum_k grad(psi_k) x e_k.
+    Compactly supported to round-off when the blobs sit well inside the
+    domain; the initial condition of the IF-HERK step workloads."""
+
+    def __init__(self, tree, blobs):
+        self.tree = tree
+        self.arrays = tree.export()
+        self.info = tree.info()
+        self.blobs = [(np.asarray(c, float), float(s), np.asarray(e, float) / np.linalg.norm(e), float(a))
+                      for c, s, e, a in blobs]
+
+    def _at(self, P):
+        u = np.zeros(P.shape)
+        for c, s, e, a in self.blobs:
+            d = P - c
+            psi = a * np.exp(-np.sum(d * d, axis=-1) / (s * s))
+            grad = d * (-2.0 / (s * s) * psi)[..., None]
+            u += np.cross(grad, e)
+        return u
+
+    def velocity(self):
+        """[3][nb, n, n, n]: component c sampled at the faces normal to axis c."""
+        n, nb = self.tree.n, self.info["nblocks"]
+        out = [np.zeros((nb, n, n, n)) for _ in range(3)]
+        for s in range(nb):
+            for c in range(3):
+                P = mesh_points(self.arrays, s, n, self.info["dx_base"], self.info["base_level"], 1, c)
+                out[c][s] = self._at(P)[..., c]
+        return out
+
+
 def fibonacci_sphere(npoints, R=0.5):
     """IB::read_points, geometry "sphere" (include/iblgf/domain/ib.hpp:179-200)."""
     lam = (1.0 + math.sqrt(5)) / 2.0

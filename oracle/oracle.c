@@ -1810,3 +1810,366 @@ int oracle_ib_cg(const oracle_tree* t, const oracle_ib* ib, const double* g, int
     free(own);
     return iters;
 }
+
+/* ---- IF-HERK time step (ifherk.hpp:1078-1200) ------------------------------
+ * Without the immersed boundary (no IB points: lin_sys_with_ib_solve,
+ * 1451-1517, reduces to the projection of lin_sys_solve, 1425-1449, with the
+ * IF threshold 1e-14).  Fields are [nb][n^3]; face and edge vectors are three
+ * arrays (component c = field index c). */
+
+/* Ifherk::clean (1306-1350): every block zero, or with non_leaf_only the
+ * one-point ring of the leaves and every non-leaf block */
+static void hk_clean(const oracle_tree* t, double* f, int non_leaf_only)
+{
+    const int    n = t->n;
+    const size_t n3 = (size_t)n * n * n;
+    int          s, i, j, k;
+    for (s = 0; s < t->nb; ++s)
+    {
+        double* b = f + s * n3;
+        if (non_leaf_only && (t->flags[s] & 1u))
+        {
+            for (k = 0; k < n; ++k)
+                for (j = 0; j < n; ++j)
+                    for (i = 0; i < n; ++i)
+                        if (i == 0 || j == 0 || k == 0 || i == n - 1 || j == n - 1 || k == n - 1)
+                            b[((size_t)k * n + j) * n + i] = 0.0;
+        }
+        else
+            memset(b, 0, n3 * sizeof(double));
+    }
+}
+
+/* Ifherk::copy (1944-1960): to = from * scale on every node */
+static void hk_copy(const oracle_tree* t, const double* from, double* to, double scale)
+{
+    const size_t len = (size_t)t->nb * t->n * t->n * t->n;
+    size_t       q;
+    for (q = 0; q < len; ++q) to[q] = from[q] * scale;
+}
+
+/* Ifherk::add (1877-1893): to += from * scale */
+static void hk_add(const oracle_tree* t, const double* from, double* to, double scale)
+{
+    const size_t len = (size_t)t->nb * t->n * t->n * t->n;
+    size_t       q;
+    for (q = 0; q < len; ++q) to[q] = to[q] + from[q] * scale;
+}
+
+/* smooth2zero's ramp (operators.hpp:668-675) */
+static double hk_ramp(double x)
+{
+    const double fac = 10.0, shift = 0.2;
+    const double c = 1 - (0.5 + 0.5 * tanh(fac * (1 - shift)));
+    return ((0.5 + 0.5 * tanh(fac * (x - shift))) + c);
+}
+
+/* Operator::smooth2zero (operators.hpp:653-760): every node of the block is
+ * multiplied by the largest ramp value over the axes on which neighbour
+ * ngb (x fastest in 3x3x3) lies off the block; the node's position on an
+ * axis is its storage index / (n - 1). */
+static void hk_smooth2zero(double* blk, int n, int ngb)
+{
+    const int x = ngb % 3, y = (ngb / 3) % 3, z = (ngb / 9) % 3;
+    int       i, j, k;
+    for (k = 0; k < n; ++k)
+        for (j = 0; j < n; ++j)
+            for (i = 0; i < n; ++i)
+            {
+                const double px = (double)i / (double)(n - 1), py = (double)j / (double)(n - 1),
+                             pz = (double)k / (double)(n - 1);
+                double square = 0.0, r;
+                int    c = 0;
+                if (z == 0) { r = hk_ramp(pz); square = square > r ? square : r; ++c; }
+                else if (z == 2) { r = hk_ramp(1 - pz); square = square > r ? square : r; ++c; }
+                if (y == 0) { r = hk_ramp(py); square = square > r ? square : r; ++c; }
+                else if (y == 2) { r = hk_ramp(1 - py); square = square > r ? square : r; ++c; }
+                if (x == 0) { r = hk_ramp(px); square = square > r ? square : r; ++c; }
+                else if (x == 2) { r = hk_ramp(1 - px); square = square > r ? square : r; ++c; }
+                if (c > 0) blk[((size_t)k * n + j) * n + i] = blk[((size_t)k * n + j) * n + i] * square;
+            }
+}
+
+/* Ifherk::clean_leaf_correction_boundary (1352-1412), one rank: with
+ * leaf_only the correction blocks of level l are zeroed; at the base level
+ * every block is ramped towards each neighbour that is missing (or, with
+ * leaf_only, a correction block), neighbours in ascending index. */
+void oracle_smooth_boundary(const oracle_tree* t, double* f, int l, int leaf_only)
+{
+    const int    n = t->n;
+    const size_t n3 = (size_t)n * n * n;
+    int          s, i;
+    if (leaf_only)
+        for (s = level_first(t, l); s < t->nb && t->level[s] == l; ++s)
+            if (t->flags[s] & 2u) memset(f + s * n3, 0, n3 * sizeof(double));
+    if (l != t->base_level) return;
+    for (s = level_first(t, l); s < t->nb && t->level[s] == l; ++s)
+        for (i = 0; i < 27; ++i)
+        {
+            const int nb = t->neighbor[27 * s + i];
+            if (nb < 0 || (leaf_only && (t->flags[nb] & 2u))) hk_smooth2zero(f + s * n3, n, i);
+        }
+}
+
+/* Operator::nonlinear (operators.hpp:2455-2500): dst = 0.25 (omega x u) on
+ * the interior nodes of every block at level l, face velocity u, edge
+ * vorticity omega, face result; then every node of the block times scale
+ * (Ifherk::nonlinear, ifherk.hpp:1676-1686). */
+void oracle_nonlinear(const oracle_tree* t, const double* u0, const double* u1, const double* u2, const double* w0,
+    const double* w1, const double* w2, double* d0, double* d1, double* d2, int l, double scale)
+{
+    const int    n = t->n;
+    const size_t n3 = (size_t)n * n * n, X = 1, Y = n, Z = (size_t)n * n;
+    int          b, i, j, k;
+    size_t       q;
+    for (b = level_first(t, l); b < t->nb && t->level[b] == l; ++b)
+    {
+        for (k = 1; k < n - 1; ++k)
+            for (j = 1; j < n - 1; ++j)
+                for (i = 1; i < n - 1; ++i)
+                {
+                    q = (size_t)b * n3 + (k * Z + j * Y + i * X);
+                    d0[q] = 0.25 * (+w1[q] * (+u2[q] + u2[q - X]) + w1[q + Z] * (+u2[q + Z] + u2[q + Z - X]) -
+                                       w2[q] * (+u1[q] + u1[q - X]) - w2[q + Y] * (+u1[q + Y] + u1[q + Y - X]));
+                    d1[q] = 0.25 * (+w2[q] * (+u0[q] + u0[q - Y]) + w2[q + X] * (+u0[q + X] + u0[q + X - Y]) -
+                                       w0[q] * (+u2[q] + u2[q - Y]) - w0[q + Z] * (+u2[q + Z] + u2[q + Z - Y]));
+                    d2[q] = 0.25 * (+w0[q] * (+u1[q] + u1[q - Z]) + w0[q + Y] * (+u1[q + Y] + u1[q + Y - Z]) -
+                                       w1[q] * (+u0[q] + u0[q - Z]) - w1[q + X] * (+u0[q + X] + u0[q + X - Z]));
+                }
+        for (q = (size_t)b * n3; q < (size_t)(b + 1) * n3; ++q)
+        {
+            d0[q] *= scale;
+            d1[q] *= scale;
+            d2[q] *= scale;
+        }
+    }
+}
+
+typedef struct hk_state
+{
+    const oracle_tree* t;
+    const double*      g;
+    int                nmax;
+    double *q[3], *r[3], *gi[3], *w1[3], *w2[3], *ui[3], *fa[3], *ea[3], *di, *ca;
+} hk_state;
+
+/* Ifherk::up_and_down (991-997): clean<Field>(true), then per component the
+ * face-mesh coarsening (up, 999-1004) and the interpolation into the
+ * correction blocks (down_to_correction, 1006-1013). */
+static void hk_up_and_down(const hk_state* h, double* const f[3])
+{
+    const oracle_tree* t = h->t;
+    int                c, l, s;
+    for (c = 0; c < 3; ++c) hk_clean(t, f[c], 1);
+    for (c = 0; c < 3; ++c)
+    {
+        ostate  o_;
+        ostate* o = &o_;
+        ostate_init(o, t, 0, 5);
+        o->mesh = 1;
+        o->comp = c;
+        source_coarsify(o, f[c]);
+        for (l = t->depth - 2; l >= t->base_level; --l)
+        {
+            int e;
+            for (e = 0; e < 3; ++e) buffer_exchange(o, f[e], l);
+            for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s) cc_intrp_node(o, f[c], f[c], s, 1);
+        }
+        ostate_free(o);
+    }
+}
+
+/* Ifherk::nonlinear (1637-1703) with a frame velocity that is constant over
+ * the domain (add_field_expression, operators.hpp:2985-3002) */
+static void hk_nonlinear(const hk_state* h, double* const src[3], double* const dst[3], double scale,
+    const double frame[3])
+{
+    const oracle_tree* t = h->t;
+    int                c, l;
+    const size_t       len = (size_t)t->nb * t->n * t->n * t->n;
+    for (c = 0; c < 3; ++c)
+    {
+        hk_clean(t, h->ea[c], 0);
+        hk_clean(t, dst[c], 0);
+        hk_clean(t, h->fa[c], 0);
+    }
+    for (l = t->base_level; l < t->depth; ++l)
+    {
+        for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, src[c], l);
+        oracle_block_op(t, 3, src[0], src[1], src[2], h->ea[0], h->ea[1], h->ea[2], l, 1.0);
+    }
+    for (c = 0; c < 3; ++c)
+    {
+        size_t q;
+        hk_copy(t, src[c], h->fa[c], 1.0);
+        for (q = 0; q < len; ++q) h->fa[c][q] += frame[c] * -1.0;
+    }
+    for (l = t->base_level; l < t->depth; ++l)
+    {
+        for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, h->ea[c], l);
+        for (c = 0; c < 3; ++c) oracle_smooth_boundary(t, h->ea[c], l, 0);
+        oracle_nonlinear(t, h->fa[0], h->fa[1], h->fa[2], h->ea[0], h->ea[1], h->ea[2], dst[0], dst[1], dst[2], l,
+            scale);
+    }
+}
+
+/* Ifherk::divergence (1738-1764) */
+static void hk_divergence(const hk_state* h, double* const src[3], double* dst)
+{
+    const oracle_tree* t = h->t;
+    int                c, l;
+    hk_up_and_down(h, src);
+    for (l = t->base_level; l < t->depth; ++l)
+    {
+        for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, src[c], l);
+        oracle_block_op(t, 2, src[0], src[1], src[2], dst, NULL, NULL, l, 1.0);
+        oracle_smooth_boundary(t, dst, l, 1);
+    }
+}
+
+/* Ifherk::gradient (1766-1793), scale 1 */
+static void hk_gradient(const hk_state* h, const double* src, double* const dst[3])
+{
+    const oracle_tree* t = h->t;
+    int                c, l;
+    for (c = 0; c < 3; ++c) hk_clean(t, dst[c], 0);
+    for (l = t->base_level; l < t->depth; ++l)
+    {
+        oracle_block_op(t, 1, src, NULL, NULL, dst[0], dst[1], dst[2], l, 1.0);
+        for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, dst[c], l);
+    }
+}
+
+static void hk_if(const hk_state* h, double* const src[3], double* const dst[3], double alpha)
+{
+    int c;
+    for (c = 0; c < 3; ++c) if_solve(h->t, src[c], dst[c], alpha, 0, 5, 1, c);
+}
+
+/* lin_sys_with_ib_solve (1451-1517) with no immersed-boundary points: the
+ * IB2xIB solve, the force solve and the pressure correction act on zero
+ * forces and leave d_i, face_aux and r_i as they are. */
+static void hk_lin_sys(const hk_state* h, double alpha)
+{
+    const oracle_tree* t = h->t;
+    int                c;
+    hk_divergence(h, h->r, h->ca);
+    {
+        ostate  o_;
+        ostate* o = &o_;
+        ostate_init(o, t, 0, 5);
+        o->ker.is_if = 0;
+        o->ker.g = h->g;
+        o->ker.nmax = h->nmax;
+        lgf_solve(o, h->ca, h->di);
+        ostate_free(o);
+    }
+    hk_gradient(h, h->di, h->fa);
+    for (c = 0; c < 3; ++c) hk_add(t, h->fa[c], h->r[c], -1.0);
+    if (fabs(alpha) > 1e-14) hk_if(h, h->r, h->ui, alpha);
+    else
+        for (c = 0; c < 3; ++c) hk_copy(t, h->r[c], h->ui[c], 1.0);
+}
+
+int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* u0, double* u1, double* u2,
+    double* p, double dt, double Re, const double* frame)
+{
+    static const double a_[6] = {1.0 / 3, -1.0, 2.0, 0.0, 0.75, 0.25};
+    static const double c_[4] = {0.0, 1.0 / 3, 1.0, 1.0};
+#define COEFF_A(i, j) a_[(i) * ((i)-1) / 2 + (j)-1]
+    const size_t len = (size_t)t->nb * t->n * t->n * t->n;
+    double* const u[3] = {u0, u1, u2};
+    double        zero[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, alpha[3], tmp;
+    hk_state      h;
+    int           c;
+    double*       pool;
+    if (!frame) frame = zero;
+    memset(&h, 0, sizeof(h));
+    h.t = t;
+    h.g = g;
+    h.nmax = nmax;
+    pool = (double*)calloc(len * 26, sizeof(double));
+    if (!pool) return 1;
+    for (c = 0; c < 3; ++c)
+    {
+        h.q[c] = pool + len * (0 + c);
+        h.r[c] = pool + len * (3 + c);
+        h.gi[c] = pool + len * (6 + c);
+        h.w1[c] = pool + len * (9 + c);
+        h.w2[c] = pool + len * (12 + c);
+        h.ui[c] = pool + len * (15 + c);
+        h.fa[c] = pool + len * (18 + c);
+        h.ea[c] = pool + len * (21 + c);
+    }
+    h.di = pool + len * 24;
+    h.ca = pool + len * 25;
+    /* update_marching_parameters (ifherk.hpp:164-175) */
+    tmp = Re * t->dx_base * t->dx_base / dt;
+    alpha[0] = (c_[1] - c_[0]) / tmp;
+    alpha[1] = (c_[2] - c_[1]) / tmp;
+    alpha[2] = (c_[3] - c_[2]) / tmp;
+
+    hk_up_and_down(&h, u);
+    for (c = 0; c < 3; ++c) hk_copy(t, u[c], h.q[c], 1.0);
+
+    /* stage 1 */
+    for (c = 0; c < 3; ++c)
+    {
+        hk_clean(t, h.gi[c], 0);
+        hk_clean(t, h.fa[c], 0);
+    }
+    hk_clean(t, h.di, 0);
+    hk_clean(t, h.ca, 0);
+    hk_nonlinear(&h, u, h.gi, COEFF_A(1, 1) * (-dt), frame);
+    for (c = 0; c < 3; ++c)
+    {
+        hk_copy(t, h.q[c], h.r[c], 1.0);
+        hk_add(t, h.gi[c], h.r[c], 1.0);
+    }
+    hk_lin_sys(&h, alpha[0]);
+
+    /* stage 2 */
+    for (c = 0; c < 3; ++c) hk_clean(t, h.r[c], 0);
+    hk_clean(t, h.di, 0);
+    hk_clean(t, h.ca, 0);
+    for (c = 0; c < 3; ++c)
+    {
+        hk_add(t, h.gi[c], h.fa[c], -1.0);
+        hk_copy(t, h.fa[c], h.w1[c], -1.0 / dt / COEFF_A(1, 1));
+    }
+    hk_if(&h, h.q, h.q, alpha[0]);
+    hk_if(&h, h.w1, h.w1, alpha[0]);
+    for (c = 0; c < 3; ++c)
+    {
+        hk_add(t, h.q[c], h.r[c], 1.0);
+        hk_add(t, h.w1[c], h.r[c], dt * COEFF_A(2, 1));
+    }
+    hk_up_and_down(&h, h.ui);
+    hk_nonlinear(&h, h.ui, h.gi, COEFF_A(2, 2) * (-dt), frame + 3);
+    for (c = 0; c < 3; ++c) hk_add(t, h.gi[c], h.r[c], 1.0);
+    hk_lin_sys(&h, alpha[1]);
+
+    /* stage 3 */
+    hk_clean(t, h.di, 0);
+    hk_clean(t, h.ca, 0);
+    for (c = 0; c < 3; ++c) hk_clean(t, h.w2[c], 0);
+    for (c = 0; c < 3; ++c)
+    {
+        hk_add(t, h.gi[c], h.fa[c], -1.0);
+        hk_copy(t, h.fa[c], h.w2[c], -1.0 / dt / COEFF_A(2, 2));
+        hk_copy(t, h.q[c], h.r[c], 1.0);
+        hk_add(t, h.w1[c], h.r[c], dt * COEFF_A(3, 1));
+        hk_add(t, h.w2[c], h.r[c], dt * COEFF_A(3, 2));
+    }
+    hk_if(&h, h.r, h.r, alpha[1]);
+    hk_up_and_down(&h, h.ui);
+    hk_nonlinear(&h, h.ui, h.gi, COEFF_A(3, 3) * (-dt), frame + 6);
+    for (c = 0; c < 3; ++c) hk_add(t, h.gi[c], h.r[c], 1.0);
+    hk_lin_sys(&h, alpha[2]);
+
+    for (c = 0; c < 3; ++c) hk_copy(t, h.ui[c], u[c], 1.0);
+    if (p) hk_copy(t, h.di, p, 1.0 / COEFF_A(3, 3) / dt);
+#undef COEFF_A
+    free(pool);
+    return 0;
+}

@@ -137,6 +137,22 @@ void oracle_block_op(const oracle_tree* t, int op, const double* s0, const doubl
 void oracle_buffer_exchange(const oracle_tree* t, double* f, int level);
 /* kind 0 clean_leaf_correction_boundary, 1 clean_ib_region_boundary (operators.hpp:329-513, 73-187) */
 void oracle_clean_boundary(const oracle_tree* t, double* f, int level, int kind, int leaf_only, int width);
+
+/* Ifherk::clean_leaf_correction_boundary (ifherk.hpp:1352-1412): correction
+ * blocks of the level zeroed when leaf_only, base-level blocks ramped to zero
+ * towards missing neighbours (Operator::smooth2zero, operators.hpp:653-760). */
+void oracle_smooth_boundary(const oracle_tree* t, double* f, int level, int leaf_only);
+/* Operator::nonlinear (operators.hpp:2455-2500) on the blocks of one level,
+ * face velocity u, edge vorticity w, face result d, then d *= scale. */
+void oracle_nonlinear(const oracle_tree* t, const double* u0, const double* u1, const double* u2, const double* w0,
+    const double* w1, const double* w2, double* d0, double* d1, double* d2, int level, double scale);
+/* Ifherk::time_step (ifherk.hpp:1078-1200) without immersed-boundary points:
+ * u (face velocity, three arrays) advanced by one step of dt in place, p the
+ * pressure (may be NULL); Re the Reynolds number; frame[9] the frame
+ * velocity per stage (x, y, z of stages 1..3), constant over the domain, or
+ * NULL for none.  Returns 0. */
+int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* u0, double* u1, double* u2,
+    double* p, double dt, double Re, const double* frame);
 double oracle_delta_1d(int kind, double x);
 /* LinSysSolver::smearing / projection (linsys.hpp:350-445) with Operator::ib_smearing /
  * ib_projection (operators.hpp:996-1015, 844-861) over the influence lists of

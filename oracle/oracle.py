@@ -66,6 +66,9 @@ def lib():
         L.oracle_block_op.argtypes = [T, _I, _P, _P, _P, _P, _P, _P, _I, _D]
         L.oracle_buffer_exchange.argtypes = [T, _P, _I]
         L.oracle_clean_boundary.argtypes = [T, _P, _I, _I, _I, _I]
+        L.oracle_smooth_boundary.argtypes = [T, _P, _I, _I]
+        L.oracle_nonlinear.argtypes = [T] + [_P] * 9 + [_I, _D]
+        L.oracle_ifherk_step.argtypes = [T, _P, _I, _P, _P, _P, _P, _D, _D, _P]
         L.oracle_delta_1d.argtypes = [_I, _D]
         L.oracle_delta_1d.restype = _D
         L.oracle_ib_smear.argtypes = [T, B, _P, _P, _P, _P]
@@ -231,6 +234,27 @@ class Tree:
         dp = [_p(x) for x in outs] + [None] * (3 - len(outs))
         lib().oracle_block_op(ctypes.byref(self.s), int(op), *sp, *dp, int(level), float(scale))
         return outs[0] if nd == 1 else outs
+
+    def smooth_boundary(self, f, level, leaf_only):
+        lib().oracle_smooth_boundary(ctypes.byref(self.s), _p(f), int(level), 1 if leaf_only else 0)
+
+    def nonlinear(self, u, w, level, scale=1.0, dst=None):
+        """Operator::nonlinear on the blocks of one level (face u, edge w)."""
+        u = [np.ascontiguousarray(x, np.float64) for x in u]
+        w = [np.ascontiguousarray(x, np.float64) for x in w]
+        d = [np.zeros_like(u[0]) for _ in range(3)] if dst is None else dst
+        lib().oracle_nonlinear(ctypes.byref(self.s), *[_p(x) for x in u + w + d], int(level), float(scale))
+        return d
+
+    def ifherk_step(self, u, dt, Re, frame=None, nmax=100):
+        """One IF-HERK time step of the face velocity u (three arrays); returns (u_new, p)."""
+        u = [np.array(x, np.float64, order="C", copy=True) for x in u]
+        p = np.zeros_like(u[0])
+        fr = None if frame is None else np.ascontiguousarray(np.broadcast_to(np.asarray(frame, np.float64), (3, 3)))
+        rc = lib().oracle_ifherk_step(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, *[_p(x) for x in u], _p(p),
+                                      float(dt), float(Re), None if fr is None else _p(fr))
+        assert rc == 0
+        return u, p
 
     def buffer_exchange(self, f, level):
         lib().oracle_buffer_exchange(ctypes.byref(self.s), _p(f), int(level))
