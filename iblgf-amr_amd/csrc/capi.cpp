@@ -404,6 +404,63 @@ int iblgf_gpu_level_gradient(iblgf_gpu_ctx* ctx, int src_field, const int dst_fi
     return guarded(ctx, [&] { ctx->solver->level_gradient(src_field, dst_fields, level); });
 }
 
+int iblgf_gpu_smooth_boundary(iblgf_gpu_ctx* ctx, int field, int level, int leaf_only)
+{
+    NEED_CTX("iblgf_gpu_smooth_boundary");
+    return guarded(ctx, [&] {
+        if (level < 0 || level >= ctx->solver->tree().depth) throw std::invalid_argument("level out of range");
+        ctx->solver->field_ptr(field);
+        ctx->solver->smooth_boundary(field, level, leaf_only != 0);
+    });
+}
+
+int iblgf_gpu_up_and_down(iblgf_gpu_ctx* ctx, const int* fields, int k, int mesh_type)
+{
+    NEED_CTX("iblgf_gpu_up_and_down");
+    if (!fields || k < 1 || k > GpuSolver::kMaxComponents)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_up_and_down: bad field list", ctx);
+    if (mesh_type != IBLGF_MESH_CELL && mesh_type != IBLGF_MESH_FACE && mesh_type != IBLGF_MESH_EDGE)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_up_and_down: unknown mesh type", ctx);
+    return guarded(ctx, [&] {
+        for (int c = 0; c < k; ++c) ctx->solver->field_ptr(fields[c]);
+        ctx->solver->up_and_down(fields, k, mesh_type);
+    });
+}
+
+int iblgf_gpu_nonlinear(iblgf_gpu_ctx* ctx, const int u_fields[3], const int w_fields[3], const int dst_fields[3],
+    int level, double scale)
+{
+    NEED_CTX("iblgf_gpu_nonlinear");
+    if (!u_fields || !w_fields || !dst_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_nonlinear: null fields", ctx);
+    for (int c = 0; c < 3; ++c)
+        for (int e = 0; e < 3; ++e)
+            if (dst_fields[c] == u_fields[e] || dst_fields[c] == w_fields[e] || (c != e && dst_fields[c] == dst_fields[e]))
+                return fail(IBLGF_ERR_ARG, "iblgf_gpu_nonlinear: fields overlap", ctx);
+    return guarded(ctx, [&] {
+        if (level < 0 || level >= ctx->solver->tree().depth) throw std::invalid_argument("level out of range");
+        for (int c = 0; c < 3; ++c)
+        {
+            ctx->solver->field_ptr(u_fields[c]);
+            ctx->solver->field_ptr(w_fields[c]);
+            ctx->solver->field_ptr(dst_fields[c]);
+        }
+        ctx->solver->nonlinear(u_fields, w_fields, dst_fields, level, scale);
+    });
+}
+
+int iblgf_gpu_ifherk_step(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field, double dt, double Re,
+    const double* frame_vel)
+{
+    NEED_CTX("iblgf_gpu_ifherk_step");
+    if (!u_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step: null velocity fields", ctx);
+    if (!(dt > 0.0) || !(Re > 0.0)) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step: dt and Re must be positive", ctx);
+    return guarded(ctx, [&] {
+        for (int c = 0; c < 3; ++c) ctx->solver->field_data(u_fields[c]);
+        if (p_field >= 0) ctx->solver->field_data(p_field);
+        ctx->solver->ifherk_step(u_fields, p_field, dt, Re, frame_vel);
+    });
+}
+
 int iblgf_gpu_clean_boundary(iblgf_gpu_ctx* ctx, int field, int level, int kind, int leaf_only, int width)
 {
     NEED_CTX("iblgf_gpu_clean_boundary");

@@ -344,6 +344,11 @@ void GpuSolver::upload_forest(const std::vector<const Octree*>& trees)
     passes_.clear();
     clean_lists_.clear();
     corr_level_.clear();
+    smooth_lists_.clear();
+    p_nonleaf_all_ = DevArray<int>{};
+    p_leaf_ring_ = DevArray<int>{};
+    d_ramp_ = DevArray<double>{};
+    step_ready_ = false;
     ghat_lap_.clear();
     ghat_if_.clear();
     ghat_lap32_.clear();
@@ -1528,23 +1533,29 @@ void GpuSolver::clean_leaf_correction_boundary(int f, int level, bool leaf_only,
 {
     if (leaf_only)
     {
-        auto it = corr_level_.find(level);
-        if (it == corr_level_.end())
-        {
-            std::vector<int> cs;
-            for (int s = lvl_begin_[level]; s < lvl_begin_[level + 1]; ++s)
-                if (flags_[s] & 2u) cs.push_back(s);
-            DevArray<int> a;
-            a.upload(cs);
-            it = corr_level_.emplace(level, std::move(a)).first;
-        }
-        launch::zero_blocks(stream_, n_, F(f), it->second.p, it->second.size());
+        const auto& c = corr_list(level);
+        launch::zero_blocks(stream_, n_, F(f), c.p, c.size());
     }
     if (level == base_level_)
     {
         const auto& l = face_clean_list(0, level, leaf_only);
         launch::clean_faces(stream_, n_, F(f), l.p, l.size() / 2, width);
     }
+}
+
+const DevArray<int>& GpuSolver::corr_list(int level)
+{
+    auto it = corr_level_.find(level);
+    if (it == corr_level_.end())
+    {
+        std::vector<int> cs;
+        for (int s = lvl_begin_[level]; s < lvl_begin_[level + 1]; ++s)
+            if (flags_[s] & 2u) cs.push_back(s);
+        DevArray<int> a;
+        a.upload(cs);
+        it = corr_level_.emplace(level, std::move(a)).first;
+    }
+    return it->second;
 }
 
 void GpuSolver::clean_ib_region_boundary(int f, int level, int width)

@@ -250,6 +250,26 @@ class GpuSolver
     // Operator::domainClean (42-68) and Operator::add (3136-3153)
     void domain_clean(int f);
     void add(int from, int to, double scale);
+    // Field operations of the IF-HERK integrator (Ifherk, ifherk.hpp) over
+    // every block: to = from * scale (copy, 1944-1960), f = f + v on every
+    // node (add_field_expression with a constant, operators.hpp:2985-3002),
+    // clean<F>(true) (1306-1350: the one-point ring of the leaves and every
+    // non-leaf block zeroed), the ramped boundary clean of
+    // Ifherk::clean_leaf_correction_boundary (1352-1412), up_and_down
+    // (991-1013: clean, coarsen, interpolate into the correction blocks) of k
+    // components of mesh type `mesh` (component c is field index c % 3), and
+    // Operator::nonlinear on one level (operators.hpp:2455-2500).
+    void copy_scaled(int from, int to, double scale);
+    void add_constant(int f, double v);
+    void clean_nonleaf(int f);
+    void smooth_boundary(int f, int level, bool leaf_only);
+    void up_and_down(const int* f, int k, int mesh);
+    void nonlinear(const int u[3], const int w[3], const int d[3], int level, double scale);
+    // Ifherk::time_step (ifherk.hpp:1078-1200) without immersed-boundary
+    // points, in place on the face velocity u[3]; p (or -1) receives the
+    // pressure; frame[9] (or null) the frame velocity of each stage, constant
+    // over the domain.  With graphs on, the whole step is one captured graph.
+    void ifherk_step(const int u[3], int p, double dt, double Re, const double* frame);
     const Octree& tree() const { return tree_; }
     const std::vector<int>& slot_of() const { return slot_of_; }
     // 0: plain FP64; 1: mixed (source and kernel spectra stored FP32 and
@@ -379,6 +399,25 @@ class GpuSolver
     DevArray<int>                            p_all_;
     std::map<int, DevArray<int>>             clean_lists_;   // {slot, face mask} entries
     std::map<int, DevArray<int>>             corr_level_;    // correction blocks per level
+
+    // IF-HERK step state
+    struct StepFields
+    {
+        int q[3], r[3], g[3], w1[3], w2[3], ui[3], fa[3], ea[3], di, ca;
+    };
+    void                 step_prepare(const double alpha[3]);
+    void                 step_body(const int u[3], int p, double dt, const double alpha[3], const double* frame);
+    void                 step_nonlinear(const int src[3], const int dst[3], double scale, const double frame[3]);
+    void                 step_lin_sys(double alpha);
+    const DevArray<int>& corr_list(int level);
+    const DevArray<int>& smooth_list(bool leaf_only);
+    void                 step_lists();
+    bool                 step_ready_ = false;
+    StepFields           sf_{};
+    DevArray<int>        p_nonleaf_all_;  // non-leaf blocks at every level
+    DevArray<int>        p_leaf_ring_;    // {leaf slot, all six sides}
+    DevArray<double>     d_ramp_;         // smooth2zero ramp [2][n]
+    std::map<int, DevArray<int>> smooth_lists_;  // {base-level slot, 27-bit mask}
 };
 
 }  // namespace iblgf_amd

@@ -122,6 +122,7 @@ template<int N>
 __global__ __launch_bounds__(kThreads) void k_axpy(
     double* dst, const double* src, const int* slots, int count, double a, long cs)
 {
+#pragma clang fp contract(off)  // product and sum rounded separately (Operator::add)
     const int b = blockIdx.x;
     if (b >= count) return;
     const size_t off = blockIdx.y * cs + (size_t)slots[b] * Geo<N>::N3;

@@ -228,6 +228,105 @@ __global__ __launch_bounds__(kThreads) void k_mask_zero(long len, const double* 
     if (i < len && mask[i] == 0.0) y[i] = 0.0;
 }
 
+// Operator::nonlinear (operators.hpp:2455-2500): d = 0.25 (omega x u) on the
+// interior nodes, face velocity u, edge vorticity w; then every node of the
+// block times scale (Ifherk::nonlinear, ifherk.hpp:1676-1686).
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_nonlinear(const double* __restrict__ u0,
+    const double* __restrict__ u1, const double* __restrict__ u2, const double* __restrict__ w0,
+    const double* __restrict__ w1, const double* __restrict__ w2, double* __restrict__ d0, double* __restrict__ d1,
+    double* __restrict__ d2, const int* __restrict__ slots, int count, double scale)
+{
+#pragma clang fp contract(off)
+    constexpr int X = 1, Y = N, Z = N * N, N3 = N * N * N;
+    const int     b = blockIdx.x;
+    if (b >= count) return;
+    const size_t off = (size_t)slots[b] * N3;
+    for (int t = threadIdx.x; t < N3; t += kThreads)
+    {
+        const int    i = t % N, j = (t / N) % N, k = t / (N * N);
+        const size_t q = off + t;
+        if (i == 0 || j == 0 || k == 0 || i == N - 1 || j == N - 1 || k == N - 1)
+        {
+            d0[q] = d0[q] * scale;
+            d1[q] = d1[q] * scale;
+            d2[q] = d2[q] * scale;
+            continue;
+        }
+        double a = w1[q] * (u2[q] + u2[q - X]);
+        a = a + w1[q + Z] * (u2[q + Z] + u2[q + Z - X]);
+        a = a - w2[q] * (u1[q] + u1[q - X]);
+        a = a - w2[q + Y] * (u1[q + Y] + u1[q + Y - X]);
+        double c = w2[q] * (u0[q] + u0[q - Y]);
+        c = c + w2[q + X] * (u0[q + X] + u0[q + X - Y]);
+        c = c - w0[q] * (u2[q] + u2[q - Y]);
+        c = c - w0[q + Z] * (u2[q + Z] + u2[q + Z - Y]);
+        double e = w0[q] * (u1[q] + u1[q - Z]);
+        e = e + w0[q + Y] * (u1[q + Y] + u1[q + Y - Z]);
+        e = e - w1[q] * (u0[q] + u0[q - Z]);
+        e = e - w1[q + X] * (u0[q + X] + u0[q + X - Z]);
+        d0[q] = (0.25 * a) * scale;
+        d1[q] = (0.25 * c) * scale;
+        d2[q] = (0.25 * e) * scale;
+    }
+}
+
+// dst = src * a, every node of the listed blocks.
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_scale_copy(
+    double* __restrict__ dst, const double* __restrict__ src, const int* __restrict__ slots, int count, double a)
+{
+    constexpr int N3 = N * N * N;
+    const int     b = blockIdx.x;
+    if (b >= count) return;
+    const size_t off = (size_t)slots[b] * N3;
+    for (int t = threadIdx.x; t < N3; t += kThreads) dst[off + t] = src[off + t] * a;
+}
+
+// f = f + v, every node of the listed blocks.
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_add_const(double* __restrict__ f, const int* __restrict__ slots,
+    int count, double v)
+{
+    constexpr int N3 = N * N * N;
+    const int     b = blockIdx.x;
+    if (b >= count) return;
+    const size_t off = (size_t)slots[b] * N3;
+    for (int t = threadIdx.x; t < N3; t += kThreads) f[off + t] = f[off + t] + v;
+}
+
+// Operator::smooth2zero (operators.hpp:653-760) for every neighbour i set in
+// the entry's 27-bit mask {slot, mask}, ascending i: each node times the
+// largest ramp value over the axes on which neighbour i lies off the block
+// (ramp[0][idx] towards the low side, ramp[1][idx] towards the high side).
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_smooth_ramp(
+    double* __restrict__ f, const int* __restrict__ entries, int count, const double* __restrict__ ramp)
+{
+#pragma clang fp contract(off)
+    constexpr int N3 = N * N * N;
+    const int     b = blockIdx.x;
+    if (b >= count) return;
+    const int mask = entries[2 * b + 1];
+    double*   blk = f + (size_t)entries[2 * b] * N3;
+    for (int t = threadIdx.x; t < N3; t += kThreads)
+    {
+        const int i = t % N, j = (t / N) % N, k = t / (N * N);
+        double    v = blk[t];
+        for (int g = 0; g < 27; ++g)
+        {
+            if (!((mask >> g) & 1)) continue;
+            const int gx = g % 3, gy = (g / 3) % 3, gz = g / 9;
+            double    square = 0.0;
+            if (gz != 1) square = fmax(square, ramp[(gz == 0 ? 0 : N) + k]);
+            if (gy != 1) square = fmax(square, ramp[(gy == 0 ? 0 : N) + j]);
+            if (gx != 1) square = fmax(square, ramp[(gx == 0 ? 0 : N) + i]);
+            if (gx != 1 || gy != 1 || gz != 1) v = v * square;
+        }
+        blk[t] = v;
+    }
+}
+
 template<template<int> class F, typename... Args>
 void dispatch_n(int n, Args... args)
 {
@@ -274,9 +373,65 @@ struct RunClean
     }
 };
 
+template<int N>
+struct RunNonlinear
+{
+    static void run(hipStream_t s, const double* const* u, const double* const* w, double* const* d, const int* slots,
+        int count, double scale)
+    {
+        hipLaunchKernelGGL((k_nonlinear<N>), dim3(count), dim3(kThreads), 0, s, u[0], u[1], u[2], w[0], w[1], w[2],
+            d[0], d[1], d[2], slots, count, scale);
+    }
+};
+template<int N>
+struct RunScaleCopy
+{
+    static void run(hipStream_t s, double* dst, const double* src, const int* slots, int count, double a)
+    {
+        hipLaunchKernelGGL((k_scale_copy<N>), dim3(count), dim3(kThreads), 0, s, dst, src, slots, count, a);
+    }
+};
+template<int N>
+struct RunAddConst
+{
+    static void run(hipStream_t s, double* f, const int* slots, int count, double v)
+    {
+        hipLaunchKernelGGL((k_add_const<N>), dim3(count), dim3(kThreads), 0, s, f, slots, count, v);
+    }
+};
+template<int N>
+struct RunSmooth
+{
+    static void run(hipStream_t s, double* f, const int* entries, int count, const double* ramp)
+    {
+        hipLaunchKernelGGL((k_smooth_ramp<N>), dim3(count), dim3(kThreads), 0, s, f, entries, count, ramp);
+    }
+};
+
 }  // namespace
 
 namespace launch {
+
+void nonlinear(hipStream_t s, int n, const double* const u[3], const double* const w[3], double* const d[3],
+    const int* slots, int count, double scale)
+{
+    if (count > 0) dispatch_n<RunNonlinear>(n, s, u, w, d, slots, count, scale);
+}
+
+void scale_copy(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a)
+{
+    if (count > 0) dispatch_n<RunScaleCopy>(n, s, dst, src, slots, count, a);
+}
+
+void add_const(hipStream_t s, int n, double* f, const int* slots, int count, double v)
+{
+    if (count > 0) dispatch_n<RunAddConst>(n, s, f, slots, count, v);
+}
+
+void smooth_ramp(hipStream_t s, int n, double* f, const int* entries, int count, const double* ramp)
+{
+    if (count > 0) dispatch_n<RunSmooth>(n, s, f, entries, count, ramp);
+}
 
 void block_op(hipStream_t s, int n, int op, const double* const src[3], double* const dst[3], const int* slots,
     int count, double fac)

@@ -33,6 +33,18 @@ void block_op(hipStream_t s, int n, int op, const double* const src[3], double* 
 // (z-, y-, x-, x+, y+, z+).
 void clean_faces(hipStream_t s, int n, double* f, const int* entries, int count, int width);
 
+// Operator::nonlinear (operators.hpp:2455-2500) on the listed blocks: d =
+// 0.25 (w x u) on the interior nodes from face u and edge w (one-cell
+// buffers read), then every node of d times scale.
+void nonlinear(hipStream_t s, int n, const double* const u[3], const double* const w[3], double* const d[3],
+    const int* slots, int count, double scale);
+// dst = src * a and f = f + v on every node of the listed blocks.
+void scale_copy(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a);
+void add_const(hipStream_t s, int n, double* f, const int* slots, int count, double v);
+// Operator::smooth2zero for entries {slot, 27-bit neighbour mask}; ramp[2][n]
+// holds the ramp at node index idx towards the low and the high side.
+void smooth_ramp(hipStream_t s, int n, double* f, const int* entries, int count, const double* ramp);
+
 // Immersed-boundary transfer with precomputed delta weights.
 // spread: field_c[node] = sum_k force[col_k] * w_k over the rows of node,
 //         in stored order (rows list {comp, slot*n^3 + q}).

@@ -387,6 +387,50 @@ int iblgf_gpu_level_gradient(iblgf_gpu_ctx* ctx, int src_field, const int dst_fi
  * neighbour is missing or not IB). */
 int iblgf_gpu_clean_boundary(iblgf_gpu_ctx* ctx, int field, int level, int kind, int leaf_only, int width);
 
+/* ---- IF-HERK time integration (Ifherk, ifherk.hpp) ------------------------
+ * Velocity fields are face vectors (three fields, component c the faces
+ * normal to axis c), vorticity edge vectors.  All asynchronous. */
+
+/* Ifherk::clean_leaf_correction_boundary (ifherk.hpp:1352-1412), the ramped
+ * clean of the integrator (not Operator::clean_leaf_correction_boundary):
+ * with leaf_only the level's correction blocks are zeroed; at the base level
+ * each block is multiplied, node by node, by Operator::smooth2zero's ramp
+ * towards every neighbour that is missing (or a correction block, with
+ * leaf_only), operators.hpp:653-760. */
+int iblgf_gpu_smooth_boundary(iblgf_gpu_ctx* ctx, int field, int level, int leaf_only);
+/* Ifherk::up_and_down (ifherk.hpp:991-1013) of k fields of one mesh type
+ * (field c is field index c % 3): the leaves' one-point ring and every
+ * non-leaf block zeroed (clean<F>(true)), the children coarsened into their
+ * parents (source_coarsify, poisson.hpp:1136-1194), the parents interpolated
+ * into their correction children (intrp_to_correction_buffer, 1015-1060);
+ * as the reference's halo fills cover every field of the list, each field
+ * but the last has its buffers of levels base..depth-2 filled once more from
+ * its final values. */
+int iblgf_gpu_up_and_down(iblgf_gpu_ctx* ctx, const int* fields, int k, int mesh_type);
+/* Operator::nonlinear (operators.hpp:2455-2500) on the blocks of one level:
+ * dst = 0.25 (w x u) on the interior nodes from face u and edge w (their
+ * one-cell buffers read), then every node of dst times scale (the
+ * Ifherk::nonlinear scaling, ifherk.hpp:1676-1686).  dst must not alias u
+ * or w. */
+int iblgf_gpu_nonlinear(iblgf_gpu_ctx* ctx, const int u_fields[3], const int w_fields[3], const int dst_fields[3],
+    int level, double scale);
+/* Ifherk::time_step (ifherk.hpp:1078-1200) for a tree without
+ * immersed-boundary points, in place on the face velocity u: up_and_down of
+ * u, then three stages of nonlinear term, integrating-factor solves and
+ * projection (lin_sys_with_ib_solve, 1451-1517, whose force solve and
+ * pressure correction vanish without IB points), the stage alphas
+ * (c_{i+1} - c_i) dt / (Re dx_base^2) (164-175).  dt is the step of the
+ * finest level (dt_base / 2^nref); Re may be infinite (every integrating
+ * factor the identity).  p_field (or -1) receives the pressure d_i / (a33 dt).
+ * frame_vel (or NULL) holds the frame velocity x, y, z of stages 1, 2, 3,
+ * constant over the domain (simulation frame_vel at T + c_i dt; a
+ * position-dependent frame is not supported).  The solves run in the
+ * context's precision.  With graphs on, the whole step is captured as one
+ * graph per (fields, dt, Re, frame) and replayed.  The context holds 26 work
+ * fields from the first call. */
+int iblgf_gpu_ifherk_step(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field, double dt, double Re,
+    const double* frame_vel);
+
 /* PoissonSolver::apply_lgf / apply_lgf_IF with an immersed-boundary FMM type
  * (poisson.hpp:587-894, 435-494): mask_type IBLGF_MASK_IB2xIB or
  * IBLGF_MASK_xIB2IB; one FMM pass at the finest level from the interior of

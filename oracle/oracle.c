@@ -1956,27 +1956,41 @@ typedef struct hk_state
 /* Ifherk::up_and_down (991-997): clean<Field>(true), then per component the
  * face-mesh coarsening (up, 999-1004) and the interpolation into the
  * correction blocks (down_to_correction, 1006-1013). */
-static void hk_up_and_down(const hk_state* h, double* const f[3])
+static void hk_up_and_down_mesh(const oracle_tree* t, double* const f[3], int mesh)
 {
-    const oracle_tree* t = h->t;
-    int                c, l, s;
+    ostate  o_;
+    ostate* o = &o_;
+    int     c, l, s;
+    ostate_init(o, t, 0, 5);
+    o->mesh = mesh;
     for (c = 0; c < 3; ++c) hk_clean(t, f[c], 1);
+    /* up: every component coarsened first */
     for (c = 0; c < 3; ++c)
     {
-        ostate  o_;
-        ostate* o = &o_;
-        ostate_init(o, t, 0, 5);
-        o->mesh = 1;
         o->comp = c;
         source_coarsify(o, f[c]);
+    }
+    /* down_to_correction: per component, each level's halo fill (of every
+     * component, as buffer_exchange<Field>) before its interpolation */
+    for (c = 0; c < 3; ++c)
+    {
+        o->comp = c;
         for (l = t->depth - 2; l >= t->base_level; --l)
         {
             int e;
             for (e = 0; e < 3; ++e) buffer_exchange(o, f[e], l);
             for (s = o->lvl_begin[l]; s < o->lvl_begin[l + 1]; ++s) cc_intrp_node(o, f[c], f[c], s, 1);
         }
-        ostate_free(o);
     }
+    ostate_free(o);
+}
+
+static void hk_up_and_down(const hk_state* h, double* const f[3]) { hk_up_and_down_mesh(h->t, f, 1); }
+
+void oracle_up_and_down(const oracle_tree* t, double* f0, double* f1, double* f2, int mesh)
+{
+    double* const f[3] = {f0, f1, f2};
+    hk_up_and_down_mesh(t, f, mesh);
 }
 
 /* Ifherk::nonlinear (1637-1703) with a frame velocity that is constant over

@@ -146,6 +146,9 @@ void oracle_smooth_boundary(const oracle_tree* t, double* f, int level, int leaf
  * face velocity u, edge vorticity w, face result d, then d *= scale. */
 void oracle_nonlinear(const oracle_tree* t, const double* u0, const double* u1, const double* u2, const double* w0,
     const double* w1, const double* w2, double* d0, double* d1, double* d2, int level, double scale);
+/* Ifherk::up_and_down (ifherk.hpp:991-1013) of a three-component field of
+ * mesh type mesh (component c is field index c). */
+void oracle_up_and_down(const oracle_tree* t, double* f0, double* f1, double* f2, int mesh);
 /* Ifherk::time_step (ifherk.hpp:1078-1200) without immersed-boundary points:
  * u (face velocity, three arrays) advanced by one step of dt in place, p the
  * pressure (may be NULL); Re the Reynolds number; frame[9] the frame

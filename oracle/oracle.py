@@ -68,6 +68,7 @@ def lib():
         L.oracle_clean_boundary.argtypes = [T, _P, _I, _I, _I, _I]
         L.oracle_smooth_boundary.argtypes = [T, _P, _I, _I]
         L.oracle_nonlinear.argtypes = [T] + [_P] * 9 + [_I, _D]
+        L.oracle_up_and_down.argtypes = [T, _P, _P, _P, _I]
         L.oracle_ifherk_step.argtypes = [T, _P, _I, _P, _P, _P, _P, _D, _D, _P]
         L.oracle_delta_1d.argtypes = [_I, _D]
         L.oracle_delta_1d.restype = _D
@@ -245,6 +246,12 @@ class Tree:
         d = [np.zeros_like(u[0]) for _ in range(3)] if dst is None else dst
         lib().oracle_nonlinear(ctypes.byref(self.s), *[_p(x) for x in u + w + d], int(level), float(scale))
         return d
+
+    def up_and_down(self, f, mesh=1):
+        """Ifherk::up_and_down of three components (returns new arrays)."""
+        f = [np.array(x, np.float64, order="C", copy=True) for x in f]
+        lib().oracle_up_and_down(ctypes.byref(self.s), *[_p(x) for x in f], int(mesh))
+        return f
 
     def ifherk_step(self, u, dt, Re, frame=None, nmax=100):
         """One IF-HERK time step of the face velocity u (three arrays); returns (u_new, p)."""
