@@ -308,6 +308,87 @@ def forest_step(steps, log):
     }
 
 
+def ifherk_bench(steps, log):
+    """The IF-HERK time step (iblgf_gpu_ifherk_step) on the ~4096-block
+    vortex-ring tree (seed 1001, n = 18): face velocity from Gaussian vector
+    potentials on the three rings (core 0.05), Re 1000, dt at a CFL number of
+    0.2 on the finest level.  One step = up-and-down, 3 stages of nonlinear
+    term, projection (3 Poisson solves) and 18 integrating-factor component
+    solves.  Timed eager and as one graph per step, in FP64 and in the
+    fp16x3-forward precision; the oracle's step and the device's are also
+    timed on the 514-block adaptive test tree (n = 10)."""
+    import numpy as np
+
+    from iblgf_amd import PREC_FP16X3_FORWARD, PREC_FP64, PoissonSolver, setups
+
+    wl = setups.VortexRings(seed=1001)
+    blobs = [(tuple(c), 0.05, tuple(ax), 0.2) for c, R, a, ax, g in wl.rings]
+    flow = setups.GaussianFlow(wl.tree, blobs)
+    u0 = flow.velocity()
+    umax = max(float(np.abs(x).max()) for x in u0)
+    dx_fine = wl.info["dx_base"] / 2 ** (wl.info["depth"] - 1 - wl.info["base_level"])
+    dt, Re = 0.2 * dx_fine / umax, 1000.0
+    s = PoissonSolver(wl.tree)
+    fu = [s.field() for _ in range(3)]
+    fp = s.field()
+    out = {"workload": ("IF-HERK step (Ifherk::time_step without IB points) on the vortex-ring tree, seed 1001, "
+                        "Gaussian vector-potential velocity on the rings"),
+           "blocks": int(wl.info["nblocks"]), "n": int(wl.tree.n), "dt": dt, "Re": Re,
+           "scalar_solves_per_step": {"laplace": 3, "integrating_factor": 18}}
+    for name, prec in (("fp64", PREC_FP64), ("fp16x3forward", PREC_FP16X3_FORWARD)):
+        s.set_precision(prec)
+        for graphs in (False, True):
+            for f, x in zip(fu, u0):
+                s.upload(f, x)
+            s.set_graphs(graphs)
+            s.ifherk_step(fu, dt, Re, p=fp)  # plans, spectra, work fields (and capture)
+            s.sync()
+            t = time.perf_counter()
+            for _ in range(steps):
+                s.ifherk_step(fu, dt, Re, p=fp)
+            s.sync()
+            ms = (time.perf_counter() - t) / steps * 1e3
+            s.set_graphs(False)
+            key = f"ms_per_step_{name}_{'graph' if graphs else 'eager'}"
+            out[key] = ms
+            log(f"IF-HERK step, {name}, {'one graph' if graphs else 'eager'}: {ms:.1f} ms")
+        if name == "fp64":
+            final = [s.download(f) for f in fu]
+            out["velocity_change_relL2_over_steps"] = float(
+                np.sqrt(sum(np.sum((a - b) ** 2) for a, b in zip(final, u0))) /
+                np.sqrt(sum(np.sum(b ** 2) for b in u0)))
+    del s
+    # like for like on a tree the oracle steps in seconds
+    try:
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        from oracle import oracle as O
+        v = setups.VortexRings(seed=7, block_extent=8, base_blocks=4, nref=2, nrings=3, centre_range=4.0,
+                               radius=(4.0, 8.0), core=(1.5, 2.5), refinement_factor=0.5)
+        sflow = setups.GaussianFlow(v.tree, [((0.02, -0.03, 0.01), 0.09, (0, 0, 1), 1.0)])
+        su = sflow.velocity()
+        ot = O.Tree(v.arrays, v.info, v.tree.n)
+        t = time.perf_counter()
+        ot.ifherk_step(su, 2e-4, 500.0)
+        t_cpu = time.perf_counter() - t
+        g = PoissonSolver(v.tree)
+        gu = [g.field() for _ in range(3)]
+        for f, x in zip(gu, su):
+            g.upload(f, x)
+        g.ifherk_step(gu, 2e-4, 500.0)
+        g.sync()
+        t = time.perf_counter()
+        for _ in range(5):
+            g.ifherk_step(gu, 2e-4, 500.0)
+        g.sync()
+        t_gpu = (time.perf_counter() - t) / 5
+        out["small_tree"] = {"blocks": int(v.info["nblocks"]), "n": int(v.tree.n), "oracle_step_s": t_cpu,
+                             "oracle_threads": int(os.environ.get("OMP_NUM_THREADS", host_cpu()["affinity"])), "gpu_step_ms": t_gpu * 1e3}
+        log(f"IF-HERK step on the 514-block tree: oracle {t_cpu:.2f} s, device {t_gpu * 1e3:.1f} ms")
+    except Exception as e:  # the oracle is a reported comparison only
+        out["small_tree"] = {"error": str(e)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -319,6 +400,8 @@ def main():
     ap.add_argument("--no-ib", action="store_true", help="skip the learned-FCG immersed-boundary comparison")
     ap.add_argument("--no-forest", action="store_true", help="skip the 64-tree batched time-step workload")
     ap.add_argument("--forest-only", action="store_true", help="run only the 64-tree batched time-step workload")
+    ap.add_argument("--no-ifherk", action="store_true", help="skip the IF-HERK time-step workload")
+    ap.add_argument("--ifherk-only", action="store_true", help="run only the IF-HERK time-step workload")
     ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph",
                                        "fp16x3forward_batched", "int8x4_batched", "fp16x3_batched", "bf16x3_batched",
                                        "fp16_batched"],
@@ -331,6 +414,12 @@ def main():
               f"(the multi-GPU sharding is a design in DESIGN.md, not built)", file=sys.stderr)
         return 2
     rank = int(os.environ.get("RANK", "0"))
+    if args.ifherk_only:
+        def hlog(msg):
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+        print(json.dumps({"ifherk_step": ifherk_bench(args.steps, hlog)}))
+        return 0
     if args.forest_only:
         def flog(msg):
             print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -593,6 +682,8 @@ def main():
     if not args.no_forest:
         del solver
         result["config"]["forest_step"] = forest_step(max(2, args.steps // 5), log)
+    if not args.no_ifherk:
+        result["config"]["ifherk_step"] = ifherk_bench(max(2, args.steps // 5), log)
     if rank == 0 and not args.no_cpu_baseline:
         # counts cover the FMM passes of one Laplace solve (each pass plan is built once)
         result["cpu_baseline"] = cpu_baseline(wl, srcs[0], counts, args.cpu_seconds)

@@ -449,15 +449,16 @@ int iblgf_gpu_nonlinear(iblgf_gpu_ctx* ctx, const int u_fields[3], const int w_f
 }
 
 int iblgf_gpu_ifherk_step(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field, double dt, double Re,
-    const double* frame_vel)
+    const double* frame_vel, int refresh)
 {
     NEED_CTX("iblgf_gpu_ifherk_step");
     if (!u_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step: null velocity fields", ctx);
+    if (refresh < 0 || refresh > 2) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step: refresh must be 0, 1 or 2", ctx);
     if (!(dt > 0.0) || !(Re > 0.0)) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step: dt and Re must be positive", ctx);
     return guarded(ctx, [&] {
         for (int c = 0; c < 3; ++c) ctx->solver->field_data(u_fields[c]);
         if (p_field >= 0) ctx->solver->field_data(p_field);
-        ctx->solver->ifherk_step(u_fields, p_field, dt, Re, frame_vel);
+        ctx->solver->ifherk_step(u_fields, p_field, dt, Re, frame_vel, refresh);
     });
 }
 

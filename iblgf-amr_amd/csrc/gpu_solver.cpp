@@ -345,6 +345,7 @@ void GpuSolver::upload_forest(const std::vector<const Octree*>& trees)
     clean_lists_.clear();
     corr_level_.clear();
     smooth_lists_.clear();
+    noncorr_lists_.clear();
     p_nonleaf_all_ = DevArray<int>{};
     p_leaf_ring_ = DevArray<int>{};
     d_ramp_ = DevArray<double>{};
@@ -1392,6 +1393,11 @@ void GpuSolver::apply_lgf_masked(int src, int dst, int mask_type)
     if (mask_type != MASK_IB2xIB && mask_type != MASK_xIB2IB && mask_type != MASK_STREAM)
         throw std::invalid_argument("mask type must be STREAM, IB2xIB or xIB2IB");
     check_batch(&src, &dst, 1);
+    masked_body(src, dst, mask_type);
+}
+
+void GpuSolver::masked_body(int src, int dst, int mask_type)
+{
     const Batch b = batch(1);
     // STREAM: one pass at the base level (poisson.hpp:609-611, 645-647); the
     // immersed-boundary types: one pass at the finest level.

@@ -269,7 +269,9 @@ class GpuSolver
     // points, in place on the face velocity u[3]; p (or -1) receives the
     // pressure; frame[9] (or null) the frame velocity of each stage, constant
     // over the domain.  With graphs on, the whole step is one captured graph.
-    void ifherk_step(const int u[3], int p, double dt, double Re, const double* frame);
+    // refresh 0: the step opens with up_and_down of u; 1: with pad_velocity
+    // (1521-1575) and adapt_corr_time_step (1206-1304); 2: pad_velocity only.
+    void ifherk_step(const int u[3], int p, double dt, double Re, const double* frame, int refresh = 0);
     const Octree& tree() const { return tree_; }
     const std::vector<int>& slot_of() const { return slot_of_; }
     // 0: plain FP64; 1: mixed (source and kernel spectra stored FP32 and
@@ -403,10 +405,18 @@ class GpuSolver
     // IF-HERK step state
     struct StepFields
     {
-        int q[3], r[3], g[3], w1[3], w2[3], ui[3], fa[3], ea[3], di, ca;
+        int q[3], r[3], g[3], w1[3], w2[3], ui[3], fa[3], ea[3], sf[3], di, ca;
     };
-    void                 step_prepare(const double alpha[3]);
-    void                 step_body(const int u[3], int p, double dt, const double alpha[3], const double* frame);
+    void                 masked_body(int src, int dst, int mask_type);
+    void                 step_prepare(const double alpha[3], int refresh);
+    void                 step_body(const int u[3], int p, double dt, const double alpha[3], const double* frame,
+                        int refresh);
+    void                 step_up(const int* f, int k, int mesh);
+    void                 step_down(const int* f, int k, int mesh);
+    void                 step_pad_velocity(const int u[3]);
+    void                 step_adapt_corr(const int u[3]);
+    const DevArray<int>& noncorr_list(int level);
+    std::map<int, DevArray<int>> noncorr_lists_;  // non-correction blocks per level
     void                 step_nonlinear(const int src[3], const int dst[3], double scale, const double frame[3]);
     void                 step_lin_sys(double alpha);
     const DevArray<int>& corr_list(int level);

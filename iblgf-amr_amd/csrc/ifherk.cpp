@@ -117,17 +117,11 @@ void GpuSolver::smooth_boundary(int f, int level, bool leaf_only)
     launch::smooth_ramp(stream_, n_, F(f), l.p, l.size() / 2, d_ramp_.p);
 }
 
-// Ifherk::up_and_down (ifherk.hpp:991-1013): clean<F>(true), then per
-// component the coarsening of source_coarsify and the interpolation into the
-// correction blocks of intrp_to_correction_buffer (poisson.hpp:1015-1060,
-// 1136-1194), with the component's mesh-type transfers.  Components are
-// independent, so each runs its level loop on its own; but the reference's
-// halo fill in that loop (buffer_exchange<From>) fills every component, so a
-// component's buffers are filled once more, from its final values, during
-// the loops of the components after it: here one more fill of each level.
-void GpuSolver::up_and_down(const int* f, int k, int mesh)
+// Ifherk::up (ifherk.hpp:999-1004) after clean<F>(true): every component
+// coarsened (source_coarsify, poisson.hpp:1136-1194) with its mesh-type
+// transfers.
+void GpuSolver::step_up(const int* f, int k, int mesh)
 {
-    if (mesh != kMeshCell && mesh != kMeshFace && mesh != kMeshEdge) throw std::invalid_argument("unknown mesh type");
     for (int c = 0; c < k; ++c) clean_nonleaf(f[c]);
     for (int c = 0; c < k; ++c)
     {
@@ -137,6 +131,23 @@ void GpuSolver::up_and_down(const int* f, int k, int mesh)
         for (int ls = depth_ - 2; ls >= base_level_; --ls)
             launch::transfer_gather(stream_, n_, p, p, p_coarsify_[ls].p, p_coarsify_[ls].size() / 9,
                 m_cc_restrict_.p, true, b, false, mesh);
+    }
+}
+
+// Ifherk::down_to_correction (ifherk.hpp:1006-1013): per component, each
+// level's halo fill, then the interpolation into the correction children
+// (intrp_to_correction_buffer, poisson.hpp:1015-1060).  Components are
+// independent, so each runs its level loop on its own; but the reference's
+// halo fill in that loop (buffer_exchange<From>) fills every component, so a
+// component's buffers are filled once more, from its final values, during
+// the loops of the components after it: here one more fill of each level.
+void GpuSolver::step_down(const int* f, int k, int mesh)
+{
+    for (int c = 0; c < k; ++c)
+    {
+        Batch b;
+        b.comp0 = c % 3;
+        double* p = F(f[c]);
         for (int l = depth_ - 2; l >= base_level_; --l)
         {
             launch::buffer_exchange(stream_, n_, p, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p);
@@ -147,7 +158,93 @@ void GpuSolver::up_and_down(const int* f, int k, int mesh)
             for (int l = depth_ - 2; l >= base_level_; --l)
                 launch::buffer_exchange(stream_, n_, p, p_level_slots_[l].p, p_level_slots_[l].size(), d_neighbor_.p);
     }
+}
+
+// Ifherk::up_and_down (ifherk.hpp:991-997)
+void GpuSolver::up_and_down(const int* f, int k, int mesh)
+{
+    if (mesh != kMeshCell && mesh != kMeshFace && mesh != kMeshEdge) throw std::invalid_argument("unknown mesh type");
+    step_up(f, k, mesh);
+    step_down(f, k, mesh);
     IBLGF_HIP_HK(hipGetLastError());
+}
+
+const DevArray<int>& GpuSolver::noncorr_list(int level)
+{
+    auto it = noncorr_lists_.find(level);
+    if (it != noncorr_lists_.end()) return it->second;
+    std::vector<int> v;
+    for (int s = lvl_begin_[level]; s < lvl_begin_[level + 1]; ++s)
+        if (!(flags_[s] & 2u)) v.push_back(s);
+    DevArray<int> a;
+    a.upload(v);
+    return noncorr_lists_.emplace(level, std::move(a)).first->second;
+}
+
+// Ifherk::pad_velocity<u, u>(true) (ifherk.hpp:1521-1575): the velocity
+// coarsened; its curl on the non-correction blocks of every level, ramped
+// at the base level; the base-level stream function from the STREAM solve
+// of each component (fmm.hpp:199-233); the base-level velocity from its curl
+// transpose times -1; then interpolated into the correction blocks.
+void GpuSolver::step_pad_velocity(const int u[3])
+{
+    const StepFields& h = sf_;
+    step_up(u, 3, kMeshFace);
+    for (int c = 0; c < 3; ++c)
+    {
+        domain_clean(h.ea[c]);
+        domain_clean(h.sf[c]);
+    }
+    for (int l = base_level_; l < depth_; ++l)
+    {
+        for (int c = 0; c < 3; ++c) buffer_exchange(u[c], l);
+        const double  dx = dx_base_ / std::pow(2.0, l - base_level_);
+        const double* sp[3] = {F(u[0]), F(u[1]), F(u[2])};
+        double*       dp[3] = {F(h.ea[0]), F(h.ea[1]), F(h.ea[2])};
+        const auto&   list = noncorr_list(l);
+        launch::block_op(stream_, n_, OP_CURL, sp, dp, list.p, list.size(), 1.0 / dx);
+    }
+    for (int c = 0; c < 3; ++c) smooth_boundary(h.ea[c], base_level_, true);
+    for (int c = 0; c < 3; ++c) masked_body(h.ea[c], h.sf[c], MASK_STREAM);
+    block_op(OP_CURL_TRANSPOSE, h.sf, u, base_level_, -1.0);
+    step_down(u, 3, kMeshFace);
+}
+
+// Ifherk::adapt_corr_time_step (ifherk.hpp:1206-1304): the three stages with
+// alpha 0, without the nonlinear term.
+void GpuSolver::step_adapt_corr(const int u[3])
+{
+    const StepFields& h = sf_;
+    up_and_down(u, 3, kMeshFace);
+    for (int c = 0; c < 3; ++c) copy_scaled(u[c], h.q[c], 1.0);
+    for (int c = 0; c < 3; ++c)
+    {
+        domain_clean(h.g[c]);
+        domain_clean(h.fa[c]);
+    }
+    domain_clean(h.di);
+    domain_clean(h.ca);
+    for (int c = 0; c < 3; ++c) copy_scaled(h.q[c], h.r[c], 1.0);
+    step_lin_sys(0.0);
+    for (int c = 0; c < 3; ++c) domain_clean(h.r[c]);
+    domain_clean(h.di);
+    domain_clean(h.ca);
+    for (int c = 0; c < 3; ++c) add(h.q[c], h.r[c], 1.0);
+    up_and_down(h.ui, 3, kMeshFace);
+    step_lin_sys(0.0);
+    domain_clean(h.di);
+    domain_clean(h.ca);
+    for (int c = 0; c < 3; ++c)
+    {
+        domain_clean(h.w2[c]);
+        copy_scaled(h.q[c], h.r[c], 1.0);
+    }
+    mesh_ = kMeshFace;
+    if_batch_body(h.r, h.r, 3, 0.0);
+    mesh_ = kMeshCell;
+    up_and_down(h.ui, 3, kMeshFace);
+    step_lin_sys(0.0);
+    for (int c = 0; c < 3; ++c) copy_scaled(h.ui[c], u[c], 1.0);
 }
 
 void GpuSolver::nonlinear(const int u[3], const int w[3], const int d[3], int level, double scale)
@@ -220,7 +317,7 @@ void GpuSolver::step_lin_sys(double alpha)
         for (int c = 0; c < 3; ++c) copy_scaled(h.r[c], h.ui[c], 1.0);
 }
 
-void GpuSolver::step_prepare(const double alpha[3])
+void GpuSolver::step_prepare(const double alpha[3], int refresh)
 {
     if (!step_ready_)
     {
@@ -235,6 +332,7 @@ void GpuSolver::step_prepare(const double alpha[3])
             h.ui[c] = field_create();
             h.fa[c] = field_create();
             h.ea[c] = field_create();
+            h.sf[c] = field_create();
         }
         h.di = field_create();
         h.ca = field_create();
@@ -245,17 +343,38 @@ void GpuSolver::step_prepare(const double alpha[3])
     step_lists();
     smooth_list(false);
     smooth_list(true);
-    for (int l = 0; l < depth_; ++l) corr_list(l);
+    for (int l = 0; l < depth_; ++l)
+    {
+        corr_list(l);
+        noncorr_list(l);
+    }
     prepare_laplace();
     prepare_if(alpha[0]);
     prepare_if(alpha[1]);
     if (std::fabs(alpha[2]) > 1e-14) prepare_if(alpha[2]);
+    if (refresh)
+    {
+        // the STREAM pass and its kernel spectra
+        PassPlan& sp = pass_idx(base_level_, fmm_mask_idx_gen(MASK_STREAM), false);
+        for (auto& m : sp.mac) ghat(true, m.d, 0.0);
+        if (refresh == 1) prepare_if(0.0);
+        sync();
+    }
 }
 
-void GpuSolver::step_body(const int u[3], int p, double dt, const double alpha[3], const double* frame)
+void GpuSolver::step_body(const int u[3], int p, double dt, const double alpha[3], const double* frame, int refresh)
 {
     const StepFields& h = sf_;
-    up_and_down(u, 3, kMeshFace);
+    // time_step (ifherk.hpp:1093-1116): the velocity refreshed from the
+    // stream function (then, with use_adaptation_correction, the alpha-0
+    // correction step), or up_and_down
+    if (refresh)
+    {
+        step_pad_velocity(u);
+        if (refresh == 1) step_adapt_corr(u);
+    }
+    else
+        up_and_down(u, 3, kMeshFace);
     for (int c = 0; c < 3; ++c) copy_scaled(u[c], h.q[c], 1.0);
 
     // Stage 1
@@ -326,8 +445,9 @@ void GpuSolver::step_body(const int u[3], int p, double dt, const double alpha[3
     IBLGF_HIP_HK(hipGetLastError());
 }
 
-void GpuSolver::ifherk_step(const int u[3], int p, double dt, double Re, const double* frame)
+void GpuSolver::ifherk_step(const int u[3], int p, double dt, double Re, const double* frame, int refresh)
 {
+    if (refresh < 0 || refresh > 2) throw std::invalid_argument("refresh must be 0, 1 or 2");
     if (!uploaded_) throw std::runtime_error("tree not uploaded");
     if (ntrees_ != 1) throw std::invalid_argument("the time step runs on a single tree");
     if (!(dt > 0.0) || !(Re > 0.0)) throw std::invalid_argument("dt and Re must be positive");
@@ -346,8 +466,8 @@ void GpuSolver::ifherk_step(const int u[3], int p, double dt, double Re, const d
     const double tmp = Re * dx_base_ * dx_base_ / dt;
     const double alpha[3] = {(kC[1] - kC[0]) / tmp, (kC[2] - kC[1]) / tmp, (kC[3] - kC[2]) / tmp};
     // the work fields exist before the key names them
-    if (!step_ready_) step_prepare(alpha);
-    std::vector<int64_t> key = {4, u[0], u[1], u[2], p, static_cast<int64_t>(precision_)};
+    if (!step_ready_) step_prepare(alpha, refresh);
+    std::vector<int64_t> key = {4, u[0], u[1], u[2], p, static_cast<int64_t>(precision_), refresh};
     auto bits = [](double x) {
         int64_t b;
         std::memcpy(&b, &x, sizeof(b));
@@ -358,8 +478,8 @@ void GpuSolver::ifherk_step(const int u[3], int p, double dt, double Re, const d
     for (double x : fr) key.push_back(bits(x));
     try
     {
-        run_captured(key, [this, &alpha] { step_prepare(alpha); },
-            [this, u, p, dt, &alpha, &fr] { step_body(u, p, dt, alpha, fr); });
+        run_captured(key, [this, &alpha, refresh] { step_prepare(alpha, refresh); },
+            [this, u, p, dt, &alpha, &fr, refresh] { step_body(u, p, dt, alpha, fr, refresh); });
     }
     catch (...)
     {

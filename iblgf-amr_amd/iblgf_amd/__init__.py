@@ -121,7 +121,7 @@ SYMBOLS = {
     "iblgf_gpu_smooth_boundary": (_I, [_P, _I, _I, _I]),
     "iblgf_gpu_up_and_down": (_I, [_P, _IP, _I, _I]),
     "iblgf_gpu_nonlinear": (_I, [_P, _IP, _IP, _IP, _I, _D]),
-    "iblgf_gpu_ifherk_step": (_I, [_P, _IP, _I, _D, _D, _DP]),
+    "iblgf_gpu_ifherk_step": (_I, [_P, _IP, _I, _D, _D, _DP, _I]),
     "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_apply_lgf_typed": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_forest_upload": (_I, [_P, _I, ctypes.POINTER(_P)]),
@@ -534,15 +534,17 @@ class PoissonSolver:
         """Operator::nonlinear on one level: dst = 0.25 (w x u), then times scale."""
         _check(lib().iblgf_gpu_nonlinear(self._h, _ip(u3), _ip(w3), _ip(dst3), int(level), float(scale)), self._h)
 
-    def ifherk_step(self, u3, dt, Re, p=-1, frame=None):
+    def ifherk_step(self, u3, dt, Re, p=-1, frame=None, refresh=0):
         """One IF-HERK time step (Ifherk::time_step) of the face velocity
         fields u3 in place; p (a field id, or -1) receives the pressure; frame
-        is the frame velocity (3 values, or 3 x 3 per stage)."""
+        is the frame velocity (3 values, or 3 x 3 per stage); refresh 1 opens
+        the step with pad_velocity and the adaptation-correction step, 2 with
+        pad_velocity only."""
         fr = None
         if frame is not None:
             fr = np.ascontiguousarray(np.broadcast_to(np.asarray(frame, np.float64), (3, 3)))
         _check(lib().iblgf_gpu_ifherk_step(self._h, _ip(u3), int(p), float(dt), float(Re),
-                                           None if fr is None else fr.ctypes.data_as(_DP)), self._h)
+                                           None if fr is None else fr.ctypes.data_as(_DP), int(refresh)), self._h)
 
     def apply_lgf_typed(self, src, dst, fmm_type=FMM_AMR2AMR, kernel=KERNEL_LAPLACE, alpha_base=0.0):
         """apply_lgf / apply_lgf_IF with the reference's fmm_type

@@ -424,12 +424,18 @@ int iblgf_gpu_nonlinear(iblgf_gpu_ctx* ctx, const int u_fields[3], const int w_f
  * factor the identity).  p_field (or -1) receives the pressure d_i / (a33 dt).
  * frame_vel (or NULL) holds the frame velocity x, y, z of stages 1, 2, 3,
  * constant over the domain (simulation frame_vel at T + c_i dt; a
- * position-dependent frame is not supported).  The solves run in the
- * context's precision.  With graphs on, the whole step is captured as one
- * graph per (fields, dt, Re, frame) and replayed.  The context holds 26 work
+ * position-dependent frame is not supported).  refresh selects how the step
+ * opens (time_step, 1093-1116): 0 with up_and_down of u; 1 with
+ * pad_velocity<u, u>(true) (the base-level velocity rebuilt from the STREAM
+ * solve of its curl, 1521-1575) and adapt_corr_time_step (three alpha-0
+ * projection stages, 1206-1304), as the reference does when the base mesh
+ * changed or the velocity-refresh interval has passed; 2 with pad_velocity
+ * only (use_adaptation_correction off).  The solves run in the context's
+ * precision.  With graphs on, the whole step is captured as one graph per
+ * (fields, dt, Re, frame, refresh) and replayed.  The context holds 29 work
  * fields from the first call. */
 int iblgf_gpu_ifherk_step(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field, double dt, double Re,
-    const double* frame_vel);
+    const double* frame_vel, int refresh);
 
 /* PoissonSolver::apply_lgf / apply_lgf_IF with an immersed-boundary FMM type
  * (poisson.hpp:587-894, 435-494): mask_type IBLGF_MASK_IB2xIB or

@@ -1950,28 +1950,39 @@ typedef struct hk_state
     const oracle_tree* t;
     const double*      g;
     int                nmax;
-    double *q[3], *r[3], *gi[3], *w1[3], *w2[3], *ui[3], *fa[3], *ea[3], *di, *ca;
+    double *q[3], *r[3], *gi[3], *w1[3], *w2[3], *ui[3], *fa[3], *ea[3], *sf[3], *di, *ca;
 } hk_state;
 
 /* Ifherk::up_and_down (991-997): clean<Field>(true), then per component the
  * face-mesh coarsening (up, 999-1004) and the interpolation into the
  * correction blocks (down_to_correction, 1006-1013). */
-static void hk_up_and_down_mesh(const oracle_tree* t, double* const f[3], int mesh)
+/* Ifherk::up (999-1004) after clean<Field>(true): every component coarsened */
+static void hk_up_mesh(const oracle_tree* t, double* const f[3], int mesh)
+{
+    ostate  o_;
+    ostate* o = &o_;
+    int     c;
+    ostate_init(o, t, 0, 5);
+    o->mesh = mesh;
+    for (c = 0; c < 3; ++c) hk_clean(t, f[c], 1);
+    for (c = 0; c < 3; ++c)
+    {
+        o->comp = c;
+        source_coarsify(o, f[c]);
+    }
+    ostate_free(o);
+}
+
+/* Ifherk::down_to_correction (1006-1013): per component, each level's halo
+ * fill (of every component, as buffer_exchange<Field>) before its
+ * interpolation into the correction children */
+static void hk_down_mesh(const oracle_tree* t, double* const f[3], int mesh)
 {
     ostate  o_;
     ostate* o = &o_;
     int     c, l, s;
     ostate_init(o, t, 0, 5);
     o->mesh = mesh;
-    for (c = 0; c < 3; ++c) hk_clean(t, f[c], 1);
-    /* up: every component coarsened first */
-    for (c = 0; c < 3; ++c)
-    {
-        o->comp = c;
-        source_coarsify(o, f[c]);
-    }
-    /* down_to_correction: per component, each level's halo fill (of every
-     * component, as buffer_exchange<Field>) before its interpolation */
     for (c = 0; c < 3; ++c)
     {
         o->comp = c;
@@ -1983,6 +1994,12 @@ static void hk_up_and_down_mesh(const oracle_tree* t, double* const f[3], int me
         }
     }
     ostate_free(o);
+}
+
+static void hk_up_and_down_mesh(const oracle_tree* t, double* const f[3], int mesh)
+{
+    hk_up_mesh(t, f, mesh);
+    hk_down_mesh(t, f, mesh);
 }
 
 static void hk_up_and_down(const hk_state* h, double* const f[3]) { hk_up_and_down_mesh(h->t, f, 1); }
@@ -2085,8 +2102,81 @@ static void hk_lin_sys(const hk_state* h, double alpha)
         for (c = 0; c < 3; ++c) hk_copy(t, h->r[c], h->ui[c], 1.0);
 }
 
+/* Ifherk::pad_velocity<u, u>(true) (1521-1575): the velocity coarsened, its
+ * curl on every non-correction block of every level, ramped at the base
+ * level, the base-level stream function by the STREAM solve of each
+ * component, the velocity on the base level from its curl transpose (scale
+ * -1), then interpolated into the correction blocks. */
+static void hk_pad_velocity(const hk_state* h, double* const u[3])
+{
+    const oracle_tree* t = h->t;
+    const int          n = t->n;
+    const size_t       n3 = (size_t)n * n * n, len = (size_t)t->nb * n3;
+    double*            tmp[3];
+    int                c, l, s;
+    hk_up_mesh(t, u, 1);
+    for (c = 0; c < 3; ++c)
+    {
+        hk_clean(t, h->ea[c], 0);
+        hk_clean(t, h->sf[c], 0);
+        tmp[c] = (double*)calloc(len, sizeof(double));
+    }
+    for (l = t->base_level; l < t->depth; ++l)
+    {
+        for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, u[c], l);
+        oracle_block_op(t, 3, u[0], u[1], u[2], tmp[0], tmp[1], tmp[2], l, 1.0);
+        for (s = level_first(t, l); s < t->nb && t->level[s] == l; ++s)
+            if (!(t->flags[s] & 2u))
+                for (c = 0; c < 3; ++c) memcpy(h->ea[c] + s * n3, tmp[c] + s * n3, n3 * sizeof(double));
+    }
+    for (c = 0; c < 3; ++c)
+    {
+        free(tmp[c]);
+        oracle_smooth_boundary(t, h->ea[c], t->base_level, 1);
+    }
+    for (c = 0; c < 3; ++c) oracle_apply_lgf_masked(t, h->g, h->nmax, h->ea[c], h->sf[c], 0, 0, 5);
+    oracle_block_op(t, 4, h->sf[0], h->sf[1], h->sf[2], u[0], u[1], u[2], t->base_level, -1.0);
+    hk_down_mesh(t, u, 1);
+}
+
+/* Ifherk::adapt_corr_time_step (1206-1304): the three stages with alpha 0,
+ * no nonlinear term and the stage integrating factor of alpha 0 */
+static void hk_adapt_corr(const hk_state* h, double* const u[3])
+{
+    const oracle_tree* t = h->t;
+    int                c;
+    hk_up_and_down(h, u);
+    for (c = 0; c < 3; ++c) hk_copy(t, u[c], h->q[c], 1.0);
+    for (c = 0; c < 3; ++c)
+    {
+        hk_clean(t, h->gi[c], 0);
+        hk_clean(t, h->fa[c], 0);
+    }
+    hk_clean(t, h->di, 0);
+    hk_clean(t, h->ca, 0);
+    for (c = 0; c < 3; ++c) hk_copy(t, h->q[c], h->r[c], 1.0);
+    hk_lin_sys(h, 0.0);
+    for (c = 0; c < 3; ++c) hk_clean(t, h->r[c], 0);
+    hk_clean(t, h->di, 0);
+    hk_clean(t, h->ca, 0);
+    for (c = 0; c < 3; ++c) hk_add(t, h->q[c], h->r[c], 1.0);
+    hk_up_and_down(h, h->ui);
+    hk_lin_sys(h, 0.0);
+    hk_clean(t, h->di, 0);
+    hk_clean(t, h->ca, 0);
+    for (c = 0; c < 3; ++c)
+    {
+        hk_clean(t, h->w2[c], 0);
+        hk_copy(t, h->q[c], h->r[c], 1.0);
+    }
+    hk_if(h, h->r, h->r, 0.0);
+    hk_up_and_down(h, h->ui);
+    hk_lin_sys(h, 0.0);
+    for (c = 0; c < 3; ++c) hk_copy(t, h->ui[c], u[c], 1.0);
+}
+
 int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* u0, double* u1, double* u2,
-    double* p, double dt, double Re, const double* frame)
+    double* p, double dt, double Re, const double* frame, int refresh)
 {
     static const double a_[6] = {1.0 / 3, -1.0, 2.0, 0.0, 0.75, 0.25};
     static const double c_[4] = {0.0, 1.0 / 3, 1.0, 1.0};
@@ -2102,7 +2192,7 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
     h.t = t;
     h.g = g;
     h.nmax = nmax;
-    pool = (double*)calloc(len * 26, sizeof(double));
+    pool = (double*)calloc(len * 29, sizeof(double));
     if (!pool) return 1;
     for (c = 0; c < 3; ++c)
     {
@@ -2114,6 +2204,7 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
         h.ui[c] = pool + len * (15 + c);
         h.fa[c] = pool + len * (18 + c);
         h.ea[c] = pool + len * (21 + c);
+        h.sf[c] = pool + len * (26 + c);
     }
     h.di = pool + len * 24;
     h.ca = pool + len * 25;
@@ -2123,7 +2214,16 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
     alpha[1] = (c_[2] - c_[1]) / tmp;
     alpha[2] = (c_[3] - c_[2]) / tmp;
 
-    hk_up_and_down(&h, u);
+    /* time_step (1093-1116): refresh the velocity from the stream function
+     * (then, with use_adaptation_correction, the alpha-0 correction step),
+     * or up_and_down */
+    if (refresh)
+    {
+        hk_pad_velocity(&h, u);
+        if (refresh == 1) hk_adapt_corr(&h, u);
+    }
+    else
+        hk_up_and_down(&h, u);
     for (c = 0; c < 3; ++c) hk_copy(t, u[c], h.q[c], 1.0);
 
     /* stage 1 */

@@ -153,9 +153,12 @@ void oracle_up_and_down(const oracle_tree* t, double* f0, double* f1, double* f2
  * u (face velocity, three arrays) advanced by one step of dt in place, p the
  * pressure (may be NULL); Re the Reynolds number; frame[9] the frame
  * velocity per stage (x, y, z of stages 1..3), constant over the domain, or
- * NULL for none.  Returns 0. */
+ * NULL for none.  refresh 0: the step opens with up_and_down of u; 1: with
+ * pad_velocity (the velocity refreshed from the STREAM solve of its curl,
+ * 1521-1575) and adapt_corr_time_step (1206-1304); 2: with pad_velocity
+ * only (use_adaptation_correction off).  Returns 0. */
 int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* u0, double* u1, double* u2,
-    double* p, double dt, double Re, const double* frame);
+    double* p, double dt, double Re, const double* frame, int refresh);
 double oracle_delta_1d(int kind, double x);
 /* LinSysSolver::smearing / projection (linsys.hpp:350-445) with Operator::ib_smearing /
  * ib_projection (operators.hpp:996-1015, 844-861) over the influence lists of

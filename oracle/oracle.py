@@ -69,7 +69,7 @@ def lib():
         L.oracle_smooth_boundary.argtypes = [T, _P, _I, _I]
         L.oracle_nonlinear.argtypes = [T] + [_P] * 9 + [_I, _D]
         L.oracle_up_and_down.argtypes = [T, _P, _P, _P, _I]
-        L.oracle_ifherk_step.argtypes = [T, _P, _I, _P, _P, _P, _P, _D, _D, _P]
+        L.oracle_ifherk_step.argtypes = [T, _P, _I, _P, _P, _P, _P, _D, _D, _P, _I]
         L.oracle_delta_1d.argtypes = [_I, _D]
         L.oracle_delta_1d.restype = _D
         L.oracle_ib_smear.argtypes = [T, B, _P, _P, _P, _P]
@@ -253,13 +253,14 @@ class Tree:
         lib().oracle_up_and_down(ctypes.byref(self.s), *[_p(x) for x in f], int(mesh))
         return f
 
-    def ifherk_step(self, u, dt, Re, frame=None, nmax=100):
-        """One IF-HERK time step of the face velocity u (three arrays); returns (u_new, p)."""
+    def ifherk_step(self, u, dt, Re, frame=None, refresh=0, nmax=100):
+        """One IF-HERK time step of the face velocity u (three arrays); returns (u_new, p).
+        refresh 1 opens it with pad_velocity and the adaptation-correction step, 2 with pad_velocity only."""
         u = [np.array(x, np.float64, order="C", copy=True) for x in u]
         p = np.zeros_like(u[0])
         fr = None if frame is None else np.ascontiguousarray(np.broadcast_to(np.asarray(frame, np.float64), (3, 3)))
         rc = lib().oracle_ifherk_step(ctypes.byref(self.s), _p(lgf_table(nmax)), nmax, *[_p(x) for x in u], _p(p),
-                                      float(dt), float(Re), None if fr is None else _p(fr))
+                                      float(dt), float(Re), None if fr is None else _p(fr), int(refresh))
         assert rc == 0
         return u, p
 

@@ -70,7 +70,8 @@ def test_field_operations_match_oracle(oracle_mod):
         assert err < 1e-13
 
 
-@pytest.mark.parametrize("case", ["adaptive", "adaptive_frame_inviscid", "uniform"])
+@pytest.mark.parametrize("case", ["adaptive", "adaptive_frame_inviscid", "uniform", "adaptive_refresh",
+                                  "adaptive_refresh_pad_only"])
 def test_step_matches_oracle(oracle_mod, case):
     print(f"case: IF-HERK step, {case}", flush=True)
     if case == "uniform":
@@ -81,17 +82,19 @@ def test_step_matches_oracle(oracle_mod, case):
     flow = setups.GaussianFlow(tree, BLOBS)
     u0 = flow.velocity()
     info, n = flow.info, tree.n
-    dt, Re, frame = 2e-4, 500.0, None
+    dt, Re, frame, refresh = 2e-4, 500.0, None, 0
     if case == "adaptive_frame_inviscid":
         Re, frame = np.inf, ((0.3, -0.1, 0.2), (0.31, -0.1, 0.2), (0.33, -0.1, 0.2))
+    if case.startswith("adaptive_refresh"):
+        refresh = 2 if case.endswith("pad_only") else 1
     ot = oracle_mod.Tree(flow.arrays, info, n)
-    ref_u, ref_p = ot.ifherk_step(u0, dt, Re, frame)
+    ref_u, ref_p = ot.ifherk_step(u0, dt, Re, frame, refresh=refresh)
     s = PoissonSolver(tree)
     fu = [s.field() for _ in range(3)]
     fp = s.field()
     for f, x in zip(fu, u0):
         s.upload(f, x)
-    s.ifherk_step(fu, dt, Re, p=fp, frame=frame)
+    s.ifherk_step(fu, dt, Re, p=fp, frame=frame, refresh=refresh)
     got_u = [s.download(f) for f in fu]
     got_p = s.download(fp)
     leaves = (flow.arrays["flags"] & FLAG_LEAF) != 0
@@ -106,7 +109,8 @@ def test_step_matches_oracle(oracle_mod, case):
 
 
 def test_step_graph_replays_equal_eager():
-    print("case: IF-HERK step captured as one graph, replayed three times", flush=True)
+    print("case: IF-HERK steps (one opening with the velocity refresh) captured as graphs, replayed three times",
+          flush=True)
     tree = _rings().tree
     flow = setups.GaussianFlow(tree, BLOBS)
     u0 = flow.velocity()
@@ -115,14 +119,14 @@ def test_step_graph_replays_equal_eager():
     fp = s.field()
     for f, x in zip(fu, u0):
         s.upload(f, x)
-    s.ifherk_step(fu, 2e-4, 500.0, p=fp)
+    s.ifherk_step(fu, 2e-4, 500.0, p=fp, refresh=1)
     s.ifherk_step(fu, 2e-4, 500.0, p=fp)
     ref = [s.download(f) for f in fu] + [s.download(fp)]
     s.set_graphs(True)
     for rep in range(3):
         for f, x in zip(fu, u0):
             s.upload(f, x)
-        s.ifherk_step(fu, 2e-4, 500.0, p=fp)
+        s.ifherk_step(fu, 2e-4, 500.0, p=fp, refresh=1)
         s.ifherk_step(fu, 2e-4, 500.0, p=fp)
         s.sync()
         got = [s.download(f) for f in fu] + [s.download(fp)]
@@ -145,3 +149,6 @@ def test_step_refuses_bad_arguments():
         s.ifherk_step([fu[0], fu[0], fu[1]], 1e-3, 1.0)
     with pytest.raises(IblgfError):
         s.ifherk_step(fu, 1e-3, 1.0, p=fu[1])
+    with pytest.raises(IblgfError) as e:
+        s.ifherk_step(fu, 1e-3, 1.0, refresh=3)
+    assert e.value.code == IBLGF_ERR_ARG
