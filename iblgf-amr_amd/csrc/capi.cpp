@@ -462,6 +462,40 @@ int iblgf_gpu_ifherk_step(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field
     });
 }
 
+int iblgf_gpu_ifherk_step_ib(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field, double dt, double Re,
+    const double* frame_vel, int refresh, const double* bc_vel, double cg_tol, int cg_maxit, int* cg_iterations)
+{
+    NEED_CTX("iblgf_gpu_ifherk_step_ib");
+    if (!ctx->ib) return fail(IBLGF_ERR_STATE, "iblgf_gpu_ifherk_step_ib: no immersed-boundary system", ctx);
+    if (!u_fields) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step_ib: null velocity fields", ctx);
+    if (!(dt > 0.0) || !(Re > 0.0)) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step_ib: dt and Re must be positive", ctx);
+    if (refresh < 0 || refresh > 2) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step_ib: refresh must be 0, 1 or 2", ctx);
+    if (!(cg_tol > 0.0) || cg_maxit < 1) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ifherk_step_ib: bad CG limits", ctx);
+    return guarded(ctx, [&] {
+        for (int c = 0; c < 3; ++c) ctx->solver->field_data(u_fields[c]);
+        if (p_field >= 0) ctx->solver->field_data(p_field);
+        ctx->ib->set_step_params(bc_vel, cg_tol, cg_maxit);
+        ctx->solver->ifherk_step(u_fields, p_field, dt, Re, frame_vel, refresh, ctx->ib.get());
+        if (cg_iterations)
+        {
+            ctx->solver->sync();
+            for (int k = 0; k < 3; ++k) cg_iterations[k] = ctx->ib->stage_stats(k + 1).iterations;
+        }
+    });
+}
+
+int iblgf_gpu_ib_stage_forces(iblgf_gpu_ctx* ctx, double* forces, int write)
+{
+    NEED_CTX("iblgf_gpu_ib_stage_forces");
+    if (!ctx->ib) return fail(IBLGF_ERR_STATE, "iblgf_gpu_ib_stage_forces: no immersed-boundary system", ctx);
+    if (!forces) return fail(IBLGF_ERR_ARG, "iblgf_gpu_ib_stage_forces: null array", ctx);
+    return guarded(ctx, [&] {
+        if (write) ctx->ib->write_stage_forces(forces);
+        else
+            ctx->ib->read_stage_forces(forces);
+    });
+}
+
 int iblgf_gpu_clean_boundary(iblgf_gpu_ctx* ctx, int field, int level, int kind, int leaf_only, int width)
 {
     NEED_CTX("iblgf_gpu_clean_boundary");

@@ -128,6 +128,21 @@ class KernelTimer
     std::vector<Rec> recs_;
 };
 
+// The immersed-boundary part of lin_sys_with_ib_solve
+// (ifherk.hpp:1451-1517), supplied to the IF-HERK step by the IB system.
+struct StepIb
+{
+    virtual ~StepIb() = default;
+    // LinSysSolver::ib_solve (linsys.hpp:113-129): the stage's force from the
+    // velocity fields (the IB2xIB integrating factor applied): their
+    // projection less the boundary velocity, the fields cleaned, CG_solve
+    // from the stage's previous force, kept as that when write_prev.
+    virtual void force_solve(const int fields[3], double alpha, int stage, bool write_prev) = 0;
+    // smearing (linsys.hpp:350-376) of the current force, added into the
+    // fields without cleaning them.
+    virtual void smear_add(const int fields[3]) = 0;
+};
+
 class GpuSolver
 {
   public:
@@ -271,7 +286,10 @@ class GpuSolver
     // over the domain.  With graphs on, the whole step is one captured graph.
     // refresh 0: the step opens with up_and_down of u; 1: with pad_velocity
     // (1521-1575) and adapt_corr_time_step (1206-1304); 2: pad_velocity only.
-    void ifherk_step(const int u[3], int p, double dt, double Re, const double* frame, int refresh = 0);
+    // With ib, the step runs the force solves of lin_sys_with_ib_solve
+    // (host-checked CG, so the step runs eagerly, graphs or not).
+    void ifherk_step(const int u[3], int p, double dt, double Re, const double* frame, int refresh = 0,
+        StepIb* ib = nullptr);
     const Octree& tree() const { return tree_; }
     const std::vector<int>& slot_of() const { return slot_of_; }
     // 0: plain FP64; 1: mixed (source and kernel spectra stored FP32 and
@@ -405,8 +423,9 @@ class GpuSolver
     // IF-HERK step state
     struct StepFields
     {
-        int q[3], r[3], g[3], w1[3], w2[3], ui[3], fa[3], ea[3], sf[3], di, ca;
+        int q[3], r[3], g[3], w1[3], w2[3], ui[3], fa[3], ea[3], sf[3], fa2[3], di, ca, ca2;
     };
+    StepIb* step_ib_ = nullptr;  // during an IB step
     void                 masked_body(int src, int dst, int mask_type);
     void                 step_prepare(const double alpha[3], int refresh);
     void                 step_body(const int u[3], int p, double dt, const double alpha[3], const double* frame,
@@ -418,7 +437,7 @@ class GpuSolver
     const DevArray<int>& noncorr_list(int level);
     std::map<int, DevArray<int>> noncorr_lists_;  // non-correction blocks per level
     void                 step_nonlinear(const int src[3], const int dst[3], double scale, const double frame[3]);
-    void                 step_lin_sys(double alpha);
+    void                 step_lin_sys(double alpha, int stage, bool write_prev);
     const DevArray<int>& corr_list(int level);
     const DevArray<int>& smooth_list(bool leaf_only);
     void                 step_lists();

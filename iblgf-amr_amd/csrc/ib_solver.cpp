@@ -54,6 +54,67 @@ void IbSystem::smear(const double* d_force, const int fields[3])
     IBLGF_HIP_IB(hipGetLastError());
 }
 
+void IbSystem::smear_add(const int fields[3])
+{
+    double* f[3] = {s_.field_ptr(fields[0]), s_.field_ptr(fields[1]), s_.field_ptr(fields[2])};
+    launch::ib_spread(s_.stream(), d_force_.p, d_sp_ptr_.p, d_sp_dst_.p, d_sp_col_.p, d_sp_w_.p,
+        static_cast<int>(st_.spread_ptr.size()) - 1, f, true);
+    IBLGF_HIP_IB(hipGetLastError());
+}
+
+void IbSystem::init_stage_forces()
+{
+    if (d_fprev_.p) return;
+    const size_t n3 = 3 * static_cast<size_t>(npoints_);
+    d_fprev_.alloc(3 * n3);
+    IBLGF_HIP_IB(hipMemset(d_fprev_.p, 0, 3 * n3 * sizeof(double)));
+    d_force_.alloc(n3);
+    d_uc_.alloc(n3);
+    IBLGF_HIP_IB(hipMemset(d_force_.p, 0, n3 * sizeof(double)));
+}
+
+void IbSystem::read_stage_forces(double* host)
+{
+    init_stage_forces();
+    IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
+    IBLGF_HIP_IB(hipMemcpy(host, d_fprev_.p, 9 * static_cast<size_t>(npoints_) * sizeof(double),
+        hipMemcpyDeviceToHost));
+}
+
+void IbSystem::write_stage_forces(const double* host)
+{
+    init_stage_forces();
+    IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
+    IBLGF_HIP_IB(hipMemcpy(d_fprev_.p, host, 9 * static_cast<size_t>(npoints_) * sizeof(double),
+        hipMemcpyHostToDevice));
+}
+
+void IbSystem::set_step_params(const double* bc, double tol, int maxit)
+{
+    if (!(tol > 0.0) || maxit < 1) throw std::invalid_argument("CG tolerance and iteration limit must be positive");
+    for (int i = 0; i < 9; ++i) bc_[i] = bc ? bc[i] : 0.0;
+    step_tol_ = tol;
+    step_maxit_ = maxit;
+}
+
+// LinSysSolver::ib_solve (linsys.hpp:113-129) inside the stage, with the
+// stage's previous force as the initial guess (ifherk.hpp:1477-1484).
+void IbSystem::force_solve(const int fields[3], double alpha, int stage, bool write_prev)
+{
+    if (stage < 1 || stage > 3) throw std::invalid_argument("stage out of range");
+    const size_t n3 = 3 * static_cast<size_t>(npoints_);
+    init_stage_forces();
+    hipStream_t st = s_.stream();
+    double*     prev = d_fprev_.p + (stage - 1) * n3;
+    IBLGF_HIP_IB(hipMemcpyAsync(d_force_.p, prev, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    project(fields, d_uc_.p);
+    launch::force_shift(st, d_uc_.p, npoints_, bc_ + 3 * (stage - 1));
+    for (int c = 0; c < 3; ++c) s_.domain_clean(fields[c]);
+    stage_stats_[stage - 1] = cg(d_uc_.p, d_force_.p, alpha, step_tol_, step_maxit_);
+    if (write_prev)
+        IBLGF_HIP_IB(hipMemcpyAsync(prev, d_force_.p, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+}
+
 void IbSystem::project(const int fields[3], double* d_force)
 {
     const double* f[3] = {s_.field_ptr(fields[0]), s_.field_ptr(fields[1]), s_.field_ptr(fields[2])};

@@ -45,7 +45,7 @@ struct FcgMonitor
 // Flexible preconditioner: z = M(r) on device vectors of length 3N.
 using Preconditioner = std::function<void(hipStream_t, const double* r, double* z)>;
 
-class IbSystem
+class IbSystem : public StepIb
 {
   public:
     IbSystem(GpuSolver& solver, const IbGeometry& g);
@@ -89,6 +89,18 @@ class IbSystem
     // a = a*s1 + b*s2 over owned entries (LinSysSolver::add).
     void add(double* a, const double* b, double s1, double s2);
 
+    // The IF-HERK step's force solves (StepIb): the boundary velocity of
+    // each stage (bc[9], x y z of stages 1..3, constant over the points), the
+    // CG stopping rule, and the stage forces kept across steps (the
+    // reference's ib().force_prev(stage), zero at first).
+    void set_step_params(const double* bc, double tol, int maxit);
+    void force_solve(const int fields[3], double alpha, int stage, bool write_prev) override;
+    void smear_add(const int fields[3]) override;
+    // the stage forces [3 stages][N][3] to or from the host
+    void           read_stage_forces(double* host);
+    void           write_stage_forces(const double* host);
+    const CgStats& stage_stats(int stage) const { return stage_stats_[stage - 1]; }
+
     // Called after each CG/FCG iteration with (k, r, z or p, f) device
     // vectors; used to collect training pairs.
     std::function<void(int, const double*, const double*, const double*)> on_iteration;
@@ -107,6 +119,12 @@ class IbSystem
 
     void recycle_push(const double* p, const double* q, double qq);
     void recycle_sync_qq();
+    DevArray<double>    d_fprev_, d_force_, d_uc_;
+    void                init_stage_forces();
+    double              bc_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double              step_tol_ = 1e-3;
+    int                 step_maxit_ = 100;
+    CgStats             stage_stats_[3];
     DevArray<double>    rec_P_, rec_Q_, rec_qq_, rec_coef_;
     std::vector<double> rec_qq_host_;
     bool                rec_dirty_ = false;

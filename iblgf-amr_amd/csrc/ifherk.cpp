@@ -225,13 +225,13 @@ void GpuSolver::step_adapt_corr(const int u[3])
     domain_clean(h.di);
     domain_clean(h.ca);
     for (int c = 0; c < 3; ++c) copy_scaled(h.q[c], h.r[c], 1.0);
-    step_lin_sys(0.0);
+    step_lin_sys(0.0, 1, false);
     for (int c = 0; c < 3; ++c) domain_clean(h.r[c]);
     domain_clean(h.di);
     domain_clean(h.ca);
     for (int c = 0; c < 3; ++c) add(h.q[c], h.r[c], 1.0);
     up_and_down(h.ui, 3, kMeshFace);
-    step_lin_sys(0.0);
+    step_lin_sys(0.0, 2, false);
     domain_clean(h.di);
     domain_clean(h.ca);
     for (int c = 0; c < 3; ++c)
@@ -243,7 +243,7 @@ void GpuSolver::step_adapt_corr(const int u[3])
     if_batch_body(h.r, h.r, 3, 0.0);
     mesh_ = kMeshCell;
     up_and_down(h.ui, 3, kMeshFace);
-    step_lin_sys(0.0);
+    step_lin_sys(0.0, 3, false);
     for (int c = 0; c < 3; ++c) copy_scaled(h.ui[c], u[c], 1.0);
 }
 
@@ -284,11 +284,13 @@ void GpuSolver::step_nonlinear(const int src[3], const int dst[3], double scale,
     }
 }
 
-// lin_sys_with_ib_solve (ifherk.hpp:1451-1517) with no immersed-boundary
-// points (the IB2xIB solve, the force solve and the pressure correction act
-// on zero forces): divergence (1738-1764), Poisson solve, gradient
-// (1766-1793), subtraction, then the integrating factor (or a copy).
-void GpuSolver::step_lin_sys(double alpha)
+// lin_sys_with_ib_solve (ifherk.hpp:1451-1517): divergence (1738-1764),
+// Poisson solve, gradient (1766-1793); with IB points the IB2xIB
+// integrating factor of r_i - grad d_i, the force solve, the pressure
+// correction (linsys.hpp:131-145) and the gradient again with the smeared
+// force added (without IB points these act on zero forces and drop out);
+// then the subtraction and the integrating factor (or a copy).
+void GpuSolver::step_lin_sys(double alpha, int stage, bool write_prev)
 {
     const StepFields& h = sf_;
     up_and_down(h.r, 3, kMeshFace);
@@ -305,6 +307,30 @@ void GpuSolver::step_lin_sys(double alpha)
     {
         block_op(OP_GRADIENT, &h.di, h.fa, l, 1.0);
         for (int c = 0; c < 3; ++c) buffer_exchange(h.fa[c], l);
+    }
+    if (step_ib_)
+    {
+        for (int c = 0; c < 3; ++c)
+        {
+            copy_scaled(h.r[c], h.fa2[c], 1.0);
+            add(h.fa[c], h.fa2[c], -1.0);
+        }
+        if (std::fabs(alpha) > 1e-14) apply_if_masked_batch(h.fa2, h.fa2, 3, alpha, MASK_IB2xIB);
+        step_ib_->force_solve(h.fa2, alpha, stage, write_prev);
+        // pressure_correction<d_i>
+        for (int c = 0; c < 3; ++c) domain_clean(h.fa2[c]);
+        domain_clean(h.ca2);
+        step_ib_->smear_add(h.fa2);
+        level_divergence(h.fa2, h.ca2, depth_ - 1);
+        apply_lgf_ib2amr(h.ca2, h.ca2);
+        add(h.ca2, h.di, -1.0);
+        for (int c = 0; c < 3; ++c) domain_clean(h.fa[c]);
+        for (int l = base_level_; l < depth_; ++l)
+        {
+            block_op(OP_GRADIENT, &h.di, h.fa, l, 1.0);
+            for (int c = 0; c < 3; ++c) buffer_exchange(h.fa[c], l);
+        }
+        step_ib_->smear_add(h.fa);
     }
     for (int c = 0; c < 3; ++c) add(h.fa[c], h.r[c], -1.0);
     if (std::fabs(alpha) > 1e-14)
@@ -333,9 +359,11 @@ void GpuSolver::step_prepare(const double alpha[3], int refresh)
             h.fa[c] = field_create();
             h.ea[c] = field_create();
             h.sf[c] = field_create();
+            h.fa2[c] = field_create();
         }
         h.di = field_create();
         h.ca = field_create();
+        h.ca2 = field_create();
         sf_ = h;
         step_ready_ = true;
     }
@@ -391,7 +419,7 @@ void GpuSolver::step_body(const int u[3], int p, double dt, const double alpha[3
         copy_scaled(h.q[c], h.r[c], 1.0);
         add(h.g[c], h.r[c], 1.0);
     }
-    step_lin_sys(alpha[0]);
+    step_lin_sys(alpha[0], 1, true);
 
     // Stage 2
     for (int c = 0; c < 3; ++c) domain_clean(h.r[c]);
@@ -418,7 +446,7 @@ void GpuSolver::step_body(const int u[3], int p, double dt, const double alpha[3
     up_and_down(h.ui, 3, kMeshFace);
     step_nonlinear(h.ui, h.g, coeff_a(2, 2) * (-dt), frame + 3);
     for (int c = 0; c < 3; ++c) add(h.g[c], h.r[c], 1.0);
-    step_lin_sys(alpha[1]);
+    step_lin_sys(alpha[1], 2, true);
 
     // Stage 3
     domain_clean(h.di);
@@ -438,14 +466,15 @@ void GpuSolver::step_body(const int u[3], int p, double dt, const double alpha[3
     up_and_down(h.ui, 3, kMeshFace);
     step_nonlinear(h.ui, h.g, coeff_a(3, 3) * (-dt), frame + 6);
     for (int c = 0; c < 3; ++c) add(h.g[c], h.r[c], 1.0);
-    step_lin_sys(alpha[2]);
+    step_lin_sys(alpha[2], 3, true);
 
     for (int c = 0; c < 3; ++c) copy_scaled(h.ui[c], u[c], 1.0);
     if (p >= 0) copy_scaled(h.di, p, 1.0 / coeff_a(3, 3) / dt);
     IBLGF_HIP_HK(hipGetLastError());
 }
 
-void GpuSolver::ifherk_step(const int u[3], int p, double dt, double Re, const double* frame, int refresh)
+void GpuSolver::ifherk_step(const int u[3], int p, double dt, double Re, const double* frame, int refresh,
+    StepIb* ib)
 {
     if (refresh < 0 || refresh > 2) throw std::invalid_argument("refresh must be 0, 1 or 2");
     if (!uploaded_) throw std::runtime_error("tree not uploaded");
@@ -478,11 +507,20 @@ void GpuSolver::ifherk_step(const int u[3], int p, double dt, double Re, const d
     for (double x : fr) key.push_back(bits(x));
     try
     {
-        run_captured(key, [this, &alpha, refresh] { step_prepare(alpha, refresh); },
-            [this, u, p, dt, &alpha, &fr, refresh] { step_body(u, p, dt, alpha, fr, refresh); });
+        if (ib)
+        {
+            // the force solves check convergence on the host: eager
+            step_ib_ = ib;
+            step_body(u, p, dt, alpha, fr, refresh);
+            step_ib_ = nullptr;
+        }
+        else
+            run_captured(key, [this, &alpha, refresh] { step_prepare(alpha, refresh); },
+                [this, u, p, dt, &alpha, &fr, refresh] { step_body(u, p, dt, alpha, fr, refresh); });
     }
     catch (...)
     {
+        step_ib_ = nullptr;
         mesh_ = kMeshCell;
         throw;
     }

@@ -119,16 +119,27 @@ __global__ __launch_bounds__(kThreads) void k_clean_faces(double* f, const int* 
 
 __global__ __launch_bounds__(kThreads) void k_ib_spread(const double* __restrict__ force,
     const int* __restrict__ row_ptr, const int* __restrict__ row_dst, const int* __restrict__ col,
-    const double* __restrict__ w, int nrows, double* f0, double* f1, double* f2)
+    const double* __restrict__ w, int nrows, double* f0, double* f1, double* f2, int add)
 {
 #pragma clang fp contract(off)
     const int r = blockIdx.x * kThreads + threadIdx.x;
     if (r >= nrows) return;
-    double s = 0.0;
-    for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) s = s + force[col[k]] * w[k];
     const int comp = row_dst[2 * r], q = row_dst[2 * r + 1];
     double*   f = comp == 0 ? f0 : (comp == 1 ? f1 : f2);
+    double    s = add ? f[q] : 0.0;
+    for (int k = row_ptr[r]; k < row_ptr[r + 1]; ++k) s = s + force[col[k]] * w[k];
     f[q] = s;
+}
+
+// force[3 i + c] = force[3 i + c] - v[c] for every point i.
+__global__ __launch_bounds__(kThreads) void k_force_shift(double* __restrict__ force, int npoints, double v0,
+    double v1, double v2)
+{
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= npoints) return;
+    force[3 * i] = force[3 * i] - v0;
+    force[3 * i + 1] = force[3 * i + 1] - v1;
+    force[3 * i + 2] = force[3 * i + 2] - v2;
 }
 
 __global__ __launch_bounds__(kThreads) void k_ib_interp(const double* __restrict__ f0,
@@ -446,11 +457,18 @@ void clean_faces(hipStream_t s, int n, double* f, const int* entries, int count,
 }
 
 void ib_spread(hipStream_t s, const double* force, const int* row_ptr, const int* row_dst, const int* col,
-    const double* w, int nrows, double* const fields[3])
+    const double* w, int nrows, double* const fields[3], bool add)
 {
     if (nrows <= 0) return;
     hipLaunchKernelGGL(k_ib_spread, dim3((nrows + kThreads - 1) / kThreads), dim3(kThreads), 0, s, force, row_ptr,
-        row_dst, col, w, nrows, fields[0], fields[1], fields[2]);
+        row_dst, col, w, nrows, fields[0], fields[1], fields[2], add ? 1 : 0);
+}
+
+void force_shift(hipStream_t s, double* force, int npoints, const double v[3])
+{
+    if (npoints <= 0) return;
+    hipLaunchKernelGGL(k_force_shift, dim3((npoints + kThreads - 1) / kThreads), dim3(kThreads), 0, s, force,
+        npoints, v[0], v[1], v[2]);
 }
 
 void ib_interp(hipStream_t s, const double* const fields[3], const int* row_ptr, const int* node, const double* w,

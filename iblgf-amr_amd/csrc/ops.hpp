@@ -49,8 +49,11 @@ void smooth_ramp(hipStream_t s, int n, double* f, const int* entries, int count,
 // spread: field_c[node] = sum_k force[col_k] * w_k over the rows of node,
 //         in stored order (rows list {comp, slot*n^3 + q}).
 // interp: force[row] = sum_k field_c[node_k] * w_k, in stored order.
+// add: the rows' sums start from the node's value (smearing without cleaning).
 void ib_spread(hipStream_t s, const double* force, const int* row_ptr, const int* row_dst, const int* col,
-    const double* w, int nrows, double* const fields[3]);
+    const double* w, int nrows, double* const fields[3], bool add = false);
+// force[3 i + c] -= v[c] for every point (the boundary velocity subtracted).
+void force_shift(hipStream_t s, double* force, int npoints, const double v[3]);
 void ib_interp(hipStream_t s, const double* const fields[3], const int* row_ptr, const int* node, const double* w,
     int nrows, double* force);
 

@@ -122,6 +122,8 @@ SYMBOLS = {
     "iblgf_gpu_up_and_down": (_I, [_P, _IP, _I, _I]),
     "iblgf_gpu_nonlinear": (_I, [_P, _IP, _IP, _IP, _I, _D]),
     "iblgf_gpu_ifherk_step": (_I, [_P, _IP, _I, _D, _D, _DP, _I]),
+    "iblgf_gpu_ifherk_step_ib": (_I, [_P, _IP, _I, _D, _D, _DP, _I, _DP, _D, _I, _IP]),
+    "iblgf_gpu_ib_stage_forces": (_I, [_P, _DP, _I]),
     "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_apply_lgf_typed": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_forest_upload": (_I, [_P, _I, ctypes.POINTER(_P)]),
@@ -633,6 +635,30 @@ class IbSystem:
         out = np.zeros((self.npoints, 3), np.float64)
         _check(lib().iblgf_gpu_ib_apply(self.solver._h, self._vec(fin).ctypes.data_as(_DP),
                                         out.ctypes.data_as(_DP), float(alpha)), self.solver._h)
+        return out
+
+    def ifherk_step(self, u3, dt, Re, p=-1, frame=None, refresh=0, bc=None, cg_tol=1e-3, cg_maxit=100):
+        """One IF-HERK step with these IB points (iblgf_gpu_ifherk_step_ib);
+        returns the three main stages' CG iteration counts."""
+        def arr9(x):
+            return None if x is None else np.ascontiguousarray(
+                np.broadcast_to(np.asarray(x, np.float64), (3, 3))).ctypes.data_as(_DP)
+
+        fr, b = arr9(frame), arr9(bc)
+        its = (ctypes.c_int * 3)()
+        _check(lib().iblgf_gpu_ifherk_step_ib(self.solver._h, _ip(u3), int(p), float(dt), float(Re), fr,
+                                              int(refresh), b, float(cg_tol), int(cg_maxit), its), self.solver._h)
+        return list(its)
+
+    def stage_forces(self, value=None):
+        """The IF-HERK stage forces [3, N, 3] (set them when value is given)."""
+        if value is not None:
+            a = np.ascontiguousarray(value, np.float64).reshape(-1)
+            if a.size != 9 * self.npoints:
+                raise ValueError("stage forces must hold 3 x N x 3 values")
+            _check(lib().iblgf_gpu_ib_stage_forces(self.solver._h, a.ctypes.data_as(_DP), 1), self.solver._h)
+        out = np.zeros((3, self.npoints, 3))
+        _check(lib().iblgf_gpu_ib_stage_forces(self.solver._h, out.ctypes.data_as(_DP), 0), self.solver._h)
         return out
 
     def cg(self, uc, alpha, tol=1e-8, maxit=200, f0=None):

@@ -436,6 +436,24 @@ int iblgf_gpu_nonlinear(iblgf_gpu_ctx* ctx, const int u_fields[3], const int w_f
  * fields from the first call. */
 int iblgf_gpu_ifherk_step(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field, double dt, double Re,
     const double* frame_vel, int refresh);
+/* The same step with the context's immersed-boundary points
+ * (iblgf_gpu_ib_create first): lin_sys_with_ib_solve (ifherk.hpp:1451-1517)
+ * in full - per stage the IB2xIB integrating factor of r_i - grad d_i, the
+ * force solve (LinSysSolver::ib_solve, linsys.hpp:113-129: the projected
+ * velocity less the boundary velocity bc_vel, CG_solve to cg_tol or cg_maxit
+ * iterations from the stage's previous force), the pressure correction
+ * (linsys.hpp:131-145) and the smeared force removed with the pressure
+ * gradient.  The stage forces persist in the context across steps (the
+ * reference's force_prev, zero at first; the main stages update them, the
+ * adaptation-correction stages do not).  bc_vel (or NULL) holds the boundary
+ * velocity x, y, z of stages 1..3, constant over the points.  cg_iterations
+ * (or NULL) receives the three main stages' CG iteration counts.  The force
+ * solves check convergence on the host, so this step runs eagerly. */
+int iblgf_gpu_ifherk_step_ib(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_field, double dt, double Re,
+    const double* frame_vel, int refresh, const double* bc_vel, double cg_tol, int cg_maxit, int* cg_iterations);
+/* Reads (write 0) or sets (write 1) the stage forces of the IF-HERK step,
+ * forces[3 stages][npoints][3]. */
+int iblgf_gpu_ib_stage_forces(iblgf_gpu_ctx* ctx, double* forces, int write);
 
 /* PoissonSolver::apply_lgf / apply_lgf_IF with an immersed-boundary FMM type
  * (poisson.hpp:587-894, 435-494): mask_type IBLGF_MASK_IB2xIB or

@@ -1951,6 +1951,13 @@ typedef struct hk_state
     const double*      g;
     int                nmax;
     double *q[3], *r[3], *gi[3], *w1[3], *w2[3], *ui[3], *fa[3], *ea[3], *sf[3], *di, *ca;
+    /* immersed boundary (NULL: none) */
+    const oracle_ib* ib;
+    double *         fa2[3], *ca2, *force, *uc;
+    double*          force_prev; /* [3 stages][3N] */
+    const double*    bc;         /* [3 stages][3] */
+    double           cg_tol;
+    int              cg_maxit;
 } hk_state;
 
 /* Ifherk::up_and_down (991-997): clean<Field>(true), then per component the
@@ -2080,7 +2087,56 @@ static void hk_if(const hk_state* h, double* const src[3], double* const dst[3],
 /* lin_sys_with_ib_solve (1451-1517) with no immersed-boundary points: the
  * IB2xIB solve, the force solve and the pressure correction act on zero
  * forces and leave d_i, face_aux and r_i as they are. */
-static void hk_lin_sys(const hk_state* h, double alpha)
+/* Operator::levelDivergence (operators.hpp:522-538) at level l */
+static void hk_level_divergence(const oracle_tree* t, double* const src[3], double* dst, int l)
+{
+    int c;
+    for (c = 0; c < 3; ++c) oracle_buffer_exchange(t, src[c], l);
+    oracle_block_op(t, 2, src[0], src[1], src[2], dst, NULL, NULL, l, 1.0);
+    oracle_clean_boundary(t, dst, l, 0, 1, 2);
+}
+
+/* smearing (linsys.hpp:350-376) of the IB force, adding into the fields */
+static void hk_smear_add(const hk_state* h, double* const f[3])
+{
+    double* fl[3] = {f[0], f[1], f[2]};
+    ib_transfer(h->t, h->ib, 0, h->force, fl, NULL);
+}
+
+/* The immersed-boundary part of lin_sys_with_ib_solve (ifherk.hpp:1465-1494):
+ * face_aux2 = r_i - grad d_i through the IB2xIB integrating factor, the
+ * force solve (LinSysSolver::ib_solve, linsys.hpp:113-129: the velocity at
+ * the points less the boundary velocity, CG_solve from the stage's previous
+ * force), then pressure_correction (131-145). */
+static void hk_ib_solve(const hk_state* h, double alpha, int stage, int write_prev)
+{
+    const oracle_tree* t = h->t;
+    const int          N = h->ib->npoints, L = t->depth - 1;
+    int                c, i;
+    for (c = 0; c < 3; ++c)
+    {
+        hk_copy(t, h->r[c], h->fa2[c], 1.0);
+        hk_add(t, h->fa[c], h->fa2[c], -1.0);
+    }
+    if (fabs(alpha) > 1e-14)
+        for (c = 0; c < 3; ++c) oracle_apply_if_masked(t, h->fa2[c], h->fa2[c], alpha, 2, 0, 5);
+    memcpy(h->force, h->force_prev + (size_t)3 * N * (stage - 1), 3 * (size_t)N * sizeof(double));
+    oracle_ib_project(t, h->ib, h->fa2[0], h->fa2[1], h->fa2[2], h->uc);
+    for (i = 0; i < N; ++i)
+        for (c = 0; c < 3; ++c) h->uc[3 * i + c] -= h->bc[3 * (stage - 1) + c];
+    for (c = 0; c < 3; ++c) hk_clean(t, h->fa2[c], 0);
+    oracle_ib_cg(t, h->ib, h->g, h->nmax, h->uc, h->force, alpha, h->cg_tol, h->cg_maxit, NULL);
+    if (write_prev) memcpy(h->force_prev + (size_t)3 * N * (stage - 1), h->force, 3 * (size_t)N * sizeof(double));
+    /* pressure_correction<d_i> */
+    for (c = 0; c < 3; ++c) hk_clean(t, h->fa2[c], 0);
+    hk_clean(t, h->ca2, 0);
+    hk_smear_add(h, h->fa2);
+    hk_level_divergence(t, h->fa2, h->ca2, L);
+    oracle_apply_lgf_masked(t, h->g, h->nmax, h->ca2, h->ca2, 4, 0, 5);
+    hk_add(t, h->ca2, h->di, -1.0);
+}
+
+static void hk_lin_sys(const hk_state* h, double alpha, int stage, int write_prev)
 {
     const oracle_tree* t = h->t;
     int                c;
@@ -2096,6 +2152,12 @@ static void hk_lin_sys(const hk_state* h, double alpha)
         ostate_free(o);
     }
     hk_gradient(h, h->di, h->fa);
+    if (h->ib)
+    {
+        hk_ib_solve(h, alpha, stage, write_prev);
+        hk_gradient(h, h->di, h->fa);
+        hk_smear_add(h, h->fa);
+    }
     for (c = 0; c < 3; ++c) hk_add(t, h->fa[c], h->r[c], -1.0);
     if (fabs(alpha) > 1e-14) hk_if(h, h->r, h->ui, alpha);
     else
@@ -2155,13 +2217,13 @@ static void hk_adapt_corr(const hk_state* h, double* const u[3])
     hk_clean(t, h->di, 0);
     hk_clean(t, h->ca, 0);
     for (c = 0; c < 3; ++c) hk_copy(t, h->q[c], h->r[c], 1.0);
-    hk_lin_sys(h, 0.0);
+    hk_lin_sys(h, 0.0, 1, 0);
     for (c = 0; c < 3; ++c) hk_clean(t, h->r[c], 0);
     hk_clean(t, h->di, 0);
     hk_clean(t, h->ca, 0);
     for (c = 0; c < 3; ++c) hk_add(t, h->q[c], h->r[c], 1.0);
     hk_up_and_down(h, h->ui);
-    hk_lin_sys(h, 0.0);
+    hk_lin_sys(h, 0.0, 2, 0);
     hk_clean(t, h->di, 0);
     hk_clean(t, h->ca, 0);
     for (c = 0; c < 3; ++c)
@@ -2171,12 +2233,31 @@ static void hk_adapt_corr(const hk_state* h, double* const u[3])
     }
     hk_if(h, h->r, h->r, 0.0);
     hk_up_and_down(h, h->ui);
-    hk_lin_sys(h, 0.0);
+    hk_lin_sys(h, 0.0, 3, 0);
     for (c = 0; c < 3; ++c) hk_copy(t, h->ui[c], u[c], 1.0);
 }
 
+static int hk_step(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0, double* u1,
+    double* u2, double* p, double dt, double Re, const double* frame, int refresh, const double* bc,
+    double* force_prev, double cg_tol, int cg_maxit);
+
 int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* u0, double* u1, double* u2,
     double* p, double dt, double Re, const double* frame, int refresh)
+{
+    return hk_step(t, NULL, g, nmax, u0, u1, u2, p, dt, Re, frame, refresh, NULL, NULL, 0.0, 0);
+}
+
+int oracle_ifherk_step_ib(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0,
+    double* u1, double* u2, double* p, double dt, double Re, const double* frame, int refresh, const double* bc,
+    double* force_prev, double cg_tol, int cg_maxit)
+{
+    if (!ib || !force_prev) return 1;
+    return hk_step(t, ib, g, nmax, u0, u1, u2, p, dt, Re, frame, refresh, bc, force_prev, cg_tol, cg_maxit);
+}
+
+static int hk_step(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0, double* u1,
+    double* u2, double* p, double dt, double Re, const double* frame, int refresh, const double* bc,
+    double* force_prev, double cg_tol, int cg_maxit)
 {
     static const double a_[6] = {1.0 / 3, -1.0, 2.0, 0.0, 0.75, 0.25};
     static const double c_[4] = {0.0, 1.0 / 3, 1.0, 1.0};
@@ -2192,8 +2273,20 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
     h.t = t;
     h.g = g;
     h.nmax = nmax;
-    pool = (double*)calloc(len * 29, sizeof(double));
+    pool = (double*)calloc(len * 33, sizeof(double));
     if (!pool) return 1;
+    if (ib)
+    {
+        h.ib = ib;
+        for (c = 0; c < 3; ++c) h.fa2[c] = pool + len * (29 + c);
+        h.ca2 = pool + len * 32;
+        h.force = (double*)calloc(3 * (size_t)ib->npoints, sizeof(double));
+        h.uc = (double*)calloc(3 * (size_t)ib->npoints, sizeof(double));
+        h.force_prev = force_prev;
+        h.bc = bc ? bc : zero;
+        h.cg_tol = cg_tol;
+        h.cg_maxit = cg_maxit;
+    }
     for (c = 0; c < 3; ++c)
     {
         h.q[c] = pool + len * (0 + c);
@@ -2240,7 +2333,7 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
         hk_copy(t, h.q[c], h.r[c], 1.0);
         hk_add(t, h.gi[c], h.r[c], 1.0);
     }
-    hk_lin_sys(&h, alpha[0]);
+    hk_lin_sys(&h, alpha[0], 1, 1);
 
     /* stage 2 */
     for (c = 0; c < 3; ++c) hk_clean(t, h.r[c], 0);
@@ -2261,7 +2354,7 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
     hk_up_and_down(&h, h.ui);
     hk_nonlinear(&h, h.ui, h.gi, COEFF_A(2, 2) * (-dt), frame + 3);
     for (c = 0; c < 3; ++c) hk_add(t, h.gi[c], h.r[c], 1.0);
-    hk_lin_sys(&h, alpha[1]);
+    hk_lin_sys(&h, alpha[1], 2, 1);
 
     /* stage 3 */
     hk_clean(t, h.di, 0);
@@ -2279,11 +2372,13 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
     hk_up_and_down(&h, h.ui);
     hk_nonlinear(&h, h.ui, h.gi, COEFF_A(3, 3) * (-dt), frame + 6);
     for (c = 0; c < 3; ++c) hk_add(t, h.gi[c], h.r[c], 1.0);
-    hk_lin_sys(&h, alpha[2]);
+    hk_lin_sys(&h, alpha[2], 3, 1);
 
     for (c = 0; c < 3; ++c) hk_copy(t, h.ui[c], u[c], 1.0);
     if (p) hk_copy(t, h.di, p, 1.0 / COEFF_A(3, 3) / dt);
 #undef COEFF_A
     free(pool);
+    free(h.force);
+    free(h.uc);
     return 0;
 }

@@ -159,6 +159,15 @@ void oracle_up_and_down(const oracle_tree* t, double* f0, double* f1, double* f2
  * only (use_adaptation_correction off).  Returns 0. */
 int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* u0, double* u1, double* u2,
     double* p, double dt, double Re, const double* frame, int refresh);
+/* The same with immersed-boundary points (lin_sys_with_ib_solve,
+ * ifherk.hpp:1451-1517): per stage the IB2xIB integrating factor of
+ * r_i - grad d_i, the force solve (CG_solve to cg_tol or cg_maxit from the
+ * stage's previous force force_prev[stage][3N], updated in place by the main
+ * stages; the boundary velocity bc[stage][3] is constant over the points),
+ * the pressure correction and the smeared force's removal from r_i. */
+int oracle_ifherk_step_ib(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0,
+    double* u1, double* u2, double* p, double dt, double Re, const double* frame, int refresh, const double* bc,
+    double* force_prev, double cg_tol, int cg_maxit);
 double oracle_delta_1d(int kind, double x);
 /* LinSysSolver::smearing / projection (linsys.hpp:350-445) with Operator::ib_smearing /
  * ib_projection (operators.hpp:996-1015, 844-861) over the influence lists of

@@ -152,3 +152,57 @@ def test_step_refuses_bad_arguments():
     with pytest.raises(IblgfError) as e:
         s.ifherk_step(fu, 1e-3, 1.0, refresh=3)
     assert e.value.code == IBLGF_ERR_ARG
+
+
+def test_ib_step_matches_oracle_and_enforces_the_boundary_velocity(oracle_mod):
+    """iblgf_gpu_ifherk_step_ib against the oracle's step with the same IB
+    points: the force solves converge (tolerance 1e-10) to the same forces and
+    velocities (relative 1e-7: the CG iterates differ in rounding, not in the
+    solution), the velocity at the owned points is the boundary velocity, and
+    the stage forces persist into the next step (its initial guesses)."""
+    from iblgf_amd import IbSystem
+
+    print("case: IF-HERK step with 100 IB points on the sphere mesh (n = 10)", flush=True)
+    sph = setups.IbSphere(npoints=100, block_extent=8, nref=1, half_width=0.5)
+    flow = setups.GaussianFlow(sph.tree, [((0.1, 0.0, 0.0), 0.3, (0, 0, 1), 0.5)])
+    u0 = flow.velocity()
+    dt = 0.1 * sph.dx_base / max(np.abs(x).max() for x in u0)
+    bc = np.array([0.2, -0.1, 0.05])
+    ot = oracle_mod.Tree(sph.arrays, sph.info, sph.tree.n)
+    oib = oracle_mod.Ib(sph.points, sph.nref, sph.delta, 4.5, sph.safety_dis)
+    fp = np.zeros((3, sph.npoints, 3))
+    ref_u, ref_p = oib.ifherk_step(ot, u0, dt, 200.0, fp, bc=bc, cg_tol=1e-10, cg_maxit=400)
+    s = PoissonSolver(sph.tree)
+    ib = IbSystem(s)
+    fu = [s.field() for _ in range(3)]
+    fpr = s.field()
+    for f, x in zip(fu, u0):
+        s.upload(f, x)
+    its1 = ib.ifherk_step(fu, dt, 200.0, p=fpr, bc=bc, cg_tol=1e-10, cg_maxit=400)
+    got_u = [s.download(f) for f in fu]
+    forces = ib.stage_forces()
+    leaves = (sph.arrays["flags"] & FLAG_LEAF) != 0
+    eu = _rel([x[leaves] for x in got_u], [x[leaves] for x in ref_u])
+    ef = np.linalg.norm(forces - fp) / np.linalg.norm(fp)
+    own = oib.owned(ot).astype(bool)
+    slip = np.abs(ib.project(fu)[own] - bc).max()
+    print(f"velocity relL2 {eu:.2e}, stage forces relL2 {ef:.2e}, slip {slip:.2e}, CG iterations {its1}", flush=True)
+    assert eu < 1e-7 and ef < 1e-6 and slip < 1e-8
+    # the stage forces persist: a second step from them equals the oracle's
+    # second step from its kept forces
+    ref_u2, _ = oib.ifherk_step(ot, ref_u, dt, 200.0, fp, bc=bc, cg_tol=1e-10, cg_maxit=400)
+    its2 = ib.ifherk_step(fu, dt, 200.0, p=fpr, bc=bc, cg_tol=1e-10, cg_maxit=400)
+    eu2 = _rel([s.download(f)[leaves] for f in fu], [x[leaves] for x in ref_u2])
+    ef2 = np.linalg.norm(ib.stage_forces() - fp) / np.linalg.norm(fp)
+    print(f"second step: CG iterations {its2}, velocity relL2 {eu2:.2e}, stage forces relL2 {ef2:.2e}", flush=True)
+    assert eu2 < 1e-7 and ef2 < 1e-6
+    # set forces round-trip
+    ib.stage_forces(np.zeros((3, sph.npoints, 3)))
+    assert not ib.stage_forces().any()
+    from iblgf_amd import IBLGF_ERR_STATE, IblgfError
+    s2 = PoissonSolver(sph.tree)
+    g = [s2.field() for _ in range(3)]
+    from iblgf_amd import lib
+    import ctypes
+    a = (ctypes.c_int * 3)(*g)
+    assert lib().iblgf_gpu_ifherk_step_ib(s2._h, a, -1, dt, 200.0, None, 0, None, 1e-6, 10, None) == IBLGF_ERR_STATE

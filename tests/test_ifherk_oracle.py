@@ -249,3 +249,27 @@ def test_velocity_refresh_is_identity_on_solenoidal_fields(oracle_mod, refresh):
     # the pressure, d_i / (a33 dt), magnifies the velocity's difference by
     # about dx / dt and is only reported
     assert d / nrm < 1e-4
+
+
+def test_ib_step_enforces_the_boundary_velocity(oracle_mod):
+    """With immersed-boundary points each stage's force solve makes the
+    projected velocity E u_i equal the boundary velocity (lin_sys_with_ib_solve,
+    ifherk.hpp:1451-1517: the IB2xIB integrating factor of the stage acts at the
+    finest level, where the points are), so after a step the velocity at the
+    owned points is bc to the CG tolerance, from an O(1) slip before; the main
+    stages keep their forces for the next step."""
+    s = setups.IbSphere(npoints=100, block_extent=8, nref=1, half_width=0.5)
+    ot = oracle_mod.Tree(s.arrays, s.info, s.tree.n)
+    ib = oracle_mod.Ib(s.points, s.nref, s.delta, 4.5, s.safety_dis)
+    flow = setups.GaussianFlow(s.tree, [((0.1, 0.0, 0.0), 0.3, (0, 0, 1), 0.5)])
+    u0 = flow.velocity()
+    own = ib.owned(ot).astype(bool)
+    dt = 0.1 * s.dx_base / max(np.abs(x).max() for x in u0)
+    bc = np.array([0.2, -0.1, 0.05])
+    fp = np.zeros((3, s.npoints, 3))
+    u1, _p = ib.ifherk_step(ot, u0, dt, 200.0, fp, bc=bc, cg_tol=1e-6, cg_maxit=300)
+    slip0 = np.abs(ib.project(ot, u0)[own] - bc).max()
+    slip1 = np.abs(ib.project(ot, u1)[own] - bc).max()
+    print(f"slip before {slip0:.3e}, after {slip1:.3e}; stage force max {np.abs(fp).max(axis=(1, 2))}")
+    assert slip0 > 0.1 and slip1 < 1e-5
+    assert np.all(np.abs(fp).max(axis=(1, 2)) > 0.1)
