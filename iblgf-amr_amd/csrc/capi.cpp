@@ -93,6 +93,35 @@ int iblgf_gpu_tree_upload(iblgf_gpu_ctx* ctx, const iblgf_tree* t)
     });
 }
 
+int iblgf_gpu_tree_readapt(iblgf_gpu_ctx* ctx, const iblgf_tree* t, int nf, const int* fields, const int* mesh,
+    const int* comp, int* nparents)
+{
+    NEED_CTX("iblgf_gpu_tree_readapt");
+    if (!t) return fail(IBLGF_ERR_ARG, "iblgf_gpu_tree_readapt: null tree", ctx);
+    if (!t->masks_built) return fail(IBLGF_ERR_STATE, "iblgf_gpu_tree_readapt: masks not built", ctx);
+    if (t->subtract_non_leaf)
+        return fail(IBLGF_ERR_UNSUPPORTED, "iblgf_gpu_tree_readapt: masks built with subtract_non_leaf = 1", ctx);
+    if (nf < 0 || (nf > 0 && (!fields || !mesh || !comp)))
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_tree_readapt: bad field list", ctx);
+    return guarded(ctx, [&] {
+        ctx->pc.reset();
+        ctx->ib.reset();
+        const int np = ctx->solver->readapt(t->tree, fields, mesh, comp, nf);
+        if (nparents) *nparents = np;
+    });
+}
+
+int iblgf_gpu_coarsen(iblgf_gpu_ctx* ctx, const int* fields, int k, int mesh_type)
+{
+    NEED_CTX("iblgf_gpu_coarsen");
+    if (!fields || k < 1 || k > GpuSolver::kMaxComponents)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_coarsen: bad field list", ctx);
+    return guarded(ctx, [&] {
+        for (int c = 0; c < k; ++c) ctx->solver->field_ptr(fields[c]);
+        ctx->solver->coarsen(fields, k, mesh_type);
+    });
+}
+
 int iblgf_gpu_forest_upload(iblgf_gpu_ctx* ctx, int ntrees, const iblgf_tree* const* trees)
 {
     NEED_CTX("iblgf_gpu_forest_upload");

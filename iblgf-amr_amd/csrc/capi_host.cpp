@@ -139,6 +139,24 @@ int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf)
     });
 }
 
+int iblgf_tree_adapt(iblgf_tree* t, int n, const uint64_t* keys, const int* level_change, int* base_mesh_update)
+{
+    if (!t || n < 0 || (n > 0 && (!keys || !level_change)))
+        return capi::fail(IBLGF_ERR_ARG, "iblgf_tree_adapt: invalid argument");
+    return capi::guarded(nullptr, [&] {
+        std::unordered_map<uint64_t, int> lc;
+        for (int i = 0; i < n; ++i)
+        {
+            if (level_change[i] < -1 || level_change[i] > 1)
+                throw std::invalid_argument("iblgf_tree_adapt: level changes are -1, 0 or +1");
+            lc[keys[i]] = level_change[i];
+        }
+        const bool upd = t->tree.adapt(lc);
+        if (base_mesh_update) *base_mesh_update = upd ? 1 : 0;
+        t->masks_built = false;
+    });
+}
+
 int iblgf_tree_set_ib(iblgf_tree* t, const double* points, int npoints, int ib_level, int delta, double safety_dis)
 {
     if (!t || !points || npoints <= 0 || ib_level < 0 || delta < IBLGF_DELTA_YANG3 || delta > IBLGF_DELTA_ROMA)

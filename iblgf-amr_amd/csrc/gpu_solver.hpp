@@ -290,6 +290,19 @@ class GpuSolver
     // (host-checked CG, so the step runs eagerly, graphs or not).
     void ifherk_step(const int u[3], int p, double dt, double Re, const double* frame, int refresh = 0,
         StepIb* ib = nullptr);
+    // Mirror refresh after the host tree was adapted (Octree::adapt): the
+    // tree re-uploaded (plans and lists rebuilt, kernel spectra kept), every
+    // field id kept; the fields fields[0..nf) (FP64, one field group such as
+    // the components of u, field index comp[i], mesh type mesh) carried over
+    // block by block by key and the new leaves filled as Ifherk::adapt does
+    // (ifherk.hpp:1047-1074): per field, halo fills of the group at every
+    // level, then each parent of a new leaf (refinement level > 0)
+    // interpolated into all its children, in new-leaf key order.  Other
+    // fields restart at zero.  Returns the number of parents interpolated.
+    int readapt(const Octree& t, const int* fields, const int* mesh, const int* comp, int nf);
+    // Ifherk::up (ifherk.hpp:999-1004) after clean<F>(true): k fields of mesh
+    // type mesh coarsened into their parents.
+    void coarsen(const int* f, int k, int mesh);
     const Octree& tree() const { return tree_; }
     const std::vector<int>& slot_of() const { return slot_of_; }
     // 0: plain FP64; 1: mixed (source and kernel spectra stored FP32 and

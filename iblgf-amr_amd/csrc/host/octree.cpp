@@ -289,6 +289,214 @@ void Octree::init_refine(int nRef, int level_up_max, const RefineCond& cond)
     mark_leaf_boundary();
 }
 
+// Tree::delete_oct (tree.hpp:455-467): the block dropped, and the octant
+// unlinked from its parent when it has no children.
+void Octree::delete_oct(int o)
+{
+    oct[o].has_data = false;
+    bool leaf_search = true;
+    for (int i = 0; i < 8; ++i)
+        if (oct[o].children[i] >= 0) leaf_search = false;
+    if (!leaf_search) return;
+    const int pa = oct[o].parent;
+    if (pa < 0) return;
+    for (int i = 0; i < 8; ++i)
+        if (oct[pa].children[i] == o) oct[pa].children[i] = -1;
+    index.erase(oct[o].key);
+    oct[o].parent = -1;
+}
+
+// Tree::deletionReset_2to1 (tree.hpp:304-328)
+void Octree::deletion_reset_2to1(int o, std::unordered_set<int>& checklist)
+{
+    if (checklist.count(o)) return;
+    if (refinement_level(o) < 0) return;
+    uint64_t nk[27];
+    Morton::neighbor_keys(oct[o].key, nk);
+    for (int i = 0; i < 27; ++i)
+    {
+        const int n = find(nk[i]);
+        if (n < 0 || !oct[n].has_data) continue;
+        oct[n].aim_deletion = false;
+        if (oct[n].parent >= 0) deletion_reset_2to1(oct[n].parent, checklist);
+    }
+    checklist.insert(o);
+}
+
+// Domain::delete_all_children(deletion, true) (domain.hpp:512-568): a parent
+// all of whose children with data are leaves or corrections aimed for
+// deletion becomes the leaf, its children stay aimed for deletion; otherwise
+// its children's deletion is cancelled.
+void Octree::delete_all_children()
+{
+    for (int l = depth - 2; l >= base_level; --l)
+    {
+        if (l >= (int)level_map.size()) continue;
+        for (int o : level_map[l])
+        {
+            bool leaf_search_data = true;
+            for (int i = 0; i < 8; ++i)
+            {
+                const int c = oct[o].children[i];
+                if (c >= 0 && oct[c].has_data) leaf_search_data = false;
+            }
+            if (leaf_search_data) continue;
+            bool all = true;
+            for (int i = 0; i < 8; ++i)
+            {
+                const int c = oct[o].children[i];
+                if (c < 0 || !oct[c].has_data) continue;
+                if (!oct[c].aim_deletion || (!oct[c].leaf && !oct[c].correction))
+                {
+                    all = false;
+                    break;
+                }
+            }
+            if (all)
+            {
+                for (int i = 0; i < 8; ++i)
+                {
+                    const int c = oct[o].children[i];
+                    if (c < 0) continue;
+                    oct[c].leaf = false;
+                    oct[c].correction = false;
+                    oct[c].leaf_boundary = false;
+                    oct[c].aim_deletion = true;
+                }
+                oct[o].leaf = true;
+                oct[o].correction = false;
+                oct[o].leaf_boundary = false;
+                oct[o].aim_deletion = false;
+            }
+            else
+                for (int i = 0; i < 8; ++i)
+                {
+                    const int c = oct[o].children[i];
+                    if (c >= 0) oct[c].aim_deletion = false;
+                }
+        }
+    }
+}
+
+bool Octree::adapt(const std::unordered_map<uint64_t, int>& level_change)
+{
+    bool base_mesh_update = false;
+    // corrections (not immersed-boundary blocks) are to be deleted
+    for (int o : dfs_snapshot()) oct[o].aim_deletion = oct[o].correction && !oct[o].is_ib;
+    std::unordered_map<int, int> change;
+    for (const auto& kv : level_change)
+    {
+        const int o = find(kv.first);
+        if (o < 0 || !oct[o].has_data) throw std::runtime_error("adapt: no block with key " + std::to_string(kv.first));
+        change[o] = kv.second;
+    }
+    // leaves in key order (Tree::leaves_, a key-sorted map)
+    std::vector<int> leaves;
+    for (int o : dfs_snapshot())
+        if (oct[o].has_data && oct[o].leaf) leaves.push_back(o);
+    std::sort(leaves.begin(), leaves.end(), [&](int a, int b) { return oct[a].key < oct[b].key; });
+    std::unordered_map<uint64_t, bool> checklist;
+    std::vector<int>                   to_refine;
+    for (int o : leaves)
+    {
+        auto it = change.find(o);
+        const int lc = it == change.end() ? 0 : it->second;
+        if (lc == 0) continue;
+        if (lc < 0) oct[o].aim_deletion = true;
+        else
+        {
+            if (!oct[o].is_ib && !try_2to1(oct[o].key, checklist)) continue;
+            to_refine.push_back(o);
+        }
+    }
+    for (int o : leaves)
+        if (oct[o].is_ib || oct[o].extended_ib) oct[o].aim_deletion = false;
+    // base-level blocks within baseBlockBufferNumber (2) of a kept block
+    std::unordered_set<uint64_t> near;
+    construct_level_maps();
+    for (int o : level_map[base_level])
+    {
+        if (oct[o].aim_deletion || oct[o].correction) continue;
+        int c[3];
+        Morton::coord(oct[o].key, c);
+        for (int dz = -2; dz <= 2; ++dz)
+            for (int dy = -2; dy <= 2; ++dy)
+                for (int dx = -2; dx <= 2; ++dx)
+                {
+                    const uint64_t nk = Morton::make(c[0] + dx, c[1] + dy, c[2] + dz, base_level);
+                    if (!Morton::is_end(nk)) near.insert(nk);
+                }
+    }
+    std::vector<uint64_t> near_sorted(near.begin(), near.end());
+    std::sort(near_sorted.begin(), near_sorted.end());
+    for (uint64_t nk : near_sorted)
+    {
+        int c[3];
+        Morton::coord(nk, c);
+        bool inside = true;
+        for (int d = 0; d < 3; ++d)
+            if (c[d] < key_box_base[d] - 2 || c[d] > key_box_base[d] + key_box_extent[d] + 1) inside = false;
+        if (!inside) continue;
+        int n = find(nk);
+        if (n < 0 || !oct[n].has_data)
+        {
+            n = insert(nk);
+            give_data(n);
+            oct[n].leaf = true;
+            oct[n].correction = false;
+            oct[n].leaf_boundary = false;
+            oct[n].aim_deletion = false;
+            base_mesh_update = true;
+        }
+        oct[n].aim_deletion = false;
+    }
+    construct_level_maps();
+    for (int o : to_refine)
+        if (oct[o].leaf) refine(o, true);
+    {
+        std::unordered_set<int> reset;
+        std::vector<int>        now;
+        for (int o : dfs_snapshot())
+            if (oct[o].has_data && oct[o].leaf) now.push_back(o);
+        std::sort(now.begin(), now.end(), [&](int a, int b) { return oct[a].key < oct[b].key; });
+        for (int o : now)
+            if (refinement_level(o) > 0 && oct[o].parent >= 0) deletion_reset_2to1(oct[o].parent, reset);
+    }
+    construct_level_maps();
+    // delete base-level leaves aimed for deletion
+    for (int o : std::vector<int>(level_map[base_level]))
+        if (oct[o].aim_deletion && oct[o].leaf) delete_oct(o);
+    construct_level_maps();
+    delete_all_children();
+    construct_level_maps();
+    construct_lists();
+    mark_correction();
+    mark_leaf_boundary();
+    for (int o : dfs_snapshot())
+        if (oct[o].has_data && oct[o].aim_deletion && refinement_level(o) > 0) delete_oct(o);
+    construct_level_maps();
+    for (int l = base_level - 1; l > 0; --l)
+        for (int o : std::vector<int>(level_map[l]))
+        {
+            bool leaf_search = true;
+            for (int i = 0; i < 8; ++i)
+                if (oct[o].children[i] >= 0) leaf_search = false;
+            if (leaf_search) delete_oct(o);
+        }
+    // every ancestor of a block holds a block
+    for (int o : dfs_snapshot())
+    {
+        if (!oct[o].has_data) continue;
+        for (int pa = oct[o].parent; pa >= 0 && !oct[pa].has_data; pa = oct[pa].parent) give_data(pa);
+    }
+    depth = 0;
+    for (int o : dfs_snapshot())
+        if (oct[o].has_data && oct[o].level + 1 > depth) depth = oct[o].level + 1;
+    construct_level_maps();
+    construct_lists();
+    return base_mesh_update;
+}
+
 void Octree::mark_correction()
 {
     // include/iblgf/domain/domain.hpp:617-735

@@ -21,6 +21,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 namespace iblgf_amd {
@@ -272,6 +273,14 @@ struct Octree
     void refine(int o, bool ratio_2to1 = true);
     std::vector<uint64_t> unfound_neighbors(int o, bool correction_as_neighbors) const;
     void init_refine(int nRef, int level_up_max, const RefineCond& cond);
+    // Decomposition::adapt_decoposition (decomposition.hpp:182-470) on one
+    // rank: level_change (+1 refine, -1 coarsen) by leaf key; returns
+    // base_mesh_update (base-level buffer blocks were added).  Lists are
+    // rebuilt; masks are not (build them next).
+    bool adapt(const std::unordered_map<uint64_t, int>& level_change);
+    void delete_oct(int o);
+    void deletion_reset_2to1(int o, std::unordered_set<int>& checklist);
+    void delete_all_children();
     void mark_correction();
     void mark_leaf_boundary();
 

@@ -131,6 +131,19 @@ __global__ __launch_bounds__(kThreads) void k_ib_spread(const double* __restrict
     f[q] = s;
 }
 
+// dst[pairs[2e+1]] = src[pairs[2e]], whole blocks (old slot -> new slot).
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_remap(double* __restrict__ dst, const double* __restrict__ src,
+    const int* __restrict__ pairs, int count)
+{
+    constexpr int N3 = N * N * N;
+    const int     e = blockIdx.x;
+    if (e >= count) return;
+    const double* a = src + (size_t)pairs[2 * e] * N3;
+    double*       b = dst + (size_t)pairs[2 * e + 1] * N3;
+    for (int t = threadIdx.x; t < N3; t += kThreads) b[t] = a[t];
+}
+
 // force[3 i + c] = force[3 i + c] - v[c] for every point i.
 __global__ __launch_bounds__(kThreads) void k_force_shift(double* __restrict__ force, int npoints, double v0,
     double v1, double v2)
@@ -385,6 +398,14 @@ struct RunClean
 };
 
 template<int N>
+struct RunRemap
+{
+    static void run(hipStream_t s, double* dst, const double* src, const int* pairs, int count)
+    {
+        hipLaunchKernelGGL((k_remap<N>), dim3(count), dim3(kThreads), 0, s, dst, src, pairs, count);
+    }
+};
+template<int N>
 struct RunNonlinear
 {
     static void run(hipStream_t s, const double* const* u, const double* const* w, double* const* d, const int* slots,
@@ -427,6 +448,11 @@ void nonlinear(hipStream_t s, int n, const double* const u[3], const double* con
     const int* slots, int count, double scale)
 {
     if (count > 0) dispatch_n<RunNonlinear>(n, s, u, w, d, slots, count, scale);
+}
+
+void remap_blocks(hipStream_t s, int n, double* dst, const double* src, const int* pairs, int count)
+{
+    if (count > 0) dispatch_n<RunRemap>(n, s, dst, src, pairs, count);
 }
 
 void scale_copy(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a)

@@ -38,6 +38,8 @@ void clean_faces(hipStream_t s, int n, double* f, const int* entries, int count,
 // buffers read), then every node of d times scale.
 void nonlinear(hipStream_t s, int n, const double* const u[3], const double* const w[3], double* const d[3],
     const int* slots, int count, double scale);
+// dst[new slot] = src[old slot] for pairs {old, new}, whole blocks.
+void remap_blocks(hipStream_t s, int n, double* dst, const double* src, const int* pairs, int count);
 // dst = src * a and f = f + v on every node of the listed blocks.
 void scale_copy(hipStream_t s, int n, double* dst, const double* src, const int* slots, int count, double a);
 void add_const(hipStream_t s, int n, double* f, const int* slots, int count, double v);

@@ -77,6 +77,7 @@ SYMBOLS = {
     "iblgf_vortex_rings_refine": (_I, [_P, _IP, _I, _I, _I]),
     "iblgf_vortex_rings_vorticity": (_I, [_P, _DP, _I, _DP, _DP, _DP]),
     "iblgf_tree_build_masks": (_I, [_P, _I]),
+    "iblgf_tree_adapt": (_I, [_P, _I, ctypes.POINTER(_U64), _IP, _IP]),
     "iblgf_tree_info": (_I, [_P, _IP, _IP, _IP, _IP, _DP, _IP]),
     "iblgf_tree_export": (_I, [_P] + [_P] * 12),
     "iblgf_tree_destroy": (None, [_P]),
@@ -127,6 +128,8 @@ SYMBOLS = {
     "iblgf_gpu_apply_lgf_masked": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_apply_lgf_typed": (_I, [_P, _I, _I, _I, _D, _I]),
     "iblgf_gpu_forest_upload": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "iblgf_gpu_tree_readapt": (_I, [_P, _P, _I, _IP, _IP, _IP, _IP]),
+    "iblgf_gpu_coarsen": (_I, [_P, _IP, _I, _I]),
     "iblgf_gpu_field_create_f32": (_I, [_P, _IP]),
     "iblgf_gpu_field_upload_f32": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
     "iblgf_gpu_field_download_f32": (_I, [_P, _I, ctypes.POINTER(ctypes.c_float)]),
@@ -305,6 +308,17 @@ class Tree:
         _check(lib().iblgf_tree_build_masks(self._h, 1 if subtract_non_leaf else 0))
         return self
 
+    def adapt(self, changes):
+        """Decomposition::adapt_decoposition on one rank (iblgf_tree_adapt):
+        changes maps leaf keys to +1 (refine) or -1 (coarsen); returns
+        base_mesh_update.  Build the masks again before uploading."""
+        keys = np.ascontiguousarray(list(changes.keys()), np.uint64)
+        lc = np.ascontiguousarray(list(changes.values()), np.int32)
+        upd = ctypes.c_int()
+        _check(lib().iblgf_tree_adapt(self._h, len(keys), keys.ctypes.data_as(ctypes.POINTER(_U64)),
+                                      lc.ctypes.data_as(_IP), ctypes.byref(upd)))
+        return bool(upd.value)
+
     def info(self):
         nb, bl, dp, be, nl = (ctypes.c_int() for _ in range(5))
         dx = ctypes.c_double()
@@ -367,6 +381,30 @@ class PoissonSolver:
                 self.slots.append(sl)
             self.nblocks = int(sum(len(s) for s in self.slots))
         self.shape = (self.nblocks, self.n, self.n, self.n)
+
+    def readapt(self, tree, fields=(), mesh=MESH_FACE, comp=None):
+        """Mirror refresh after tree.adapt + tree.build_masks
+        (iblgf_gpu_tree_readapt): fields (one group, field index comp[i],
+        default i % 3) carried over by key, new leaves interpolated from their
+        parents; returns the number of parents interpolated."""
+        fields = [int(f) for f in fields]
+        k = len(fields)
+        comp = [i % 3 for i in range(k)] if comp is None else list(comp)
+        meshes = [int(mesh)] * k if np.isscalar(mesh) else [int(m) for m in mesh]
+        npar = ctypes.c_int()
+        _check(lib().iblgf_gpu_tree_readapt(self._h, tree._h, k, _ip(fields) if k else None,
+                                            _ip(meshes) if k else None, _ip(comp) if k else None,
+                                            ctypes.byref(npar)), self._h)
+        self.trees = [tree]
+        self.tree = tree
+        self.nblocks = tree.info()["nblocks"]
+        self.slots = [np.arange(self.nblocks)]
+        self.shape = (self.nblocks, self.n, self.n, self.n)
+        return npar.value
+
+    def coarsen(self, fields, mesh=MESH_FACE):
+        """Ifherk::up after clean<F>(true) (iblgf_gpu_coarsen)."""
+        _check(lib().iblgf_gpu_coarsen(self._h, _ip(list(fields)), len(fields), int(mesh)), self._h)
 
     def to_forest(self, arrays):
         """Per-tree arrays [nb_i, n, n, n] -> one forest field array."""

@@ -77,6 +77,17 @@ int iblgf_tree_init_refine(iblgf_tree* t, int nref, int level_up_max, iblgf_refi
  * device solves implement the reference default subtract_non_leaf = false. */
 int iblgf_tree_build_masks(iblgf_tree* t, int subtract_non_leaf);
 
+/* Decomposition::adapt_decoposition (include/iblgf/domain/decomposition/
+ * decomposition.hpp:182-470) on one rank: n (key, level change) pairs name
+ * leaves to refine (+1; done where the 2:1 balance allows, Tree::try_2to1,
+ * tree.hpp:330-377, always for immersed-boundary blocks) or to coarsen (-1:
+ * a parent whose children with data are all leaves or corrections aimed for
+ * deletion becomes the leaf, Domain::delete_all_children, domain.hpp:
+ * 512-568); corrections are re-marked (mark_correction, 617-735), base-level
+ * blocks within two blocks of a kept one are added (base_mesh_update 1 when
+ * any was).  The tree's masks must be built again before an upload. */
+int iblgf_tree_adapt(iblgf_tree* t, int n, const uint64_t* keys, const int* level_change, int* base_mesh_update);
+
 int iblgf_tree_info(const iblgf_tree* t, int* nblocks, int* base_level, int* depth, int* block_extent,
     double* dx_base, int* nleaves);
 
@@ -386,6 +397,25 @@ int iblgf_gpu_level_gradient(iblgf_gpu_ctx* ctx, int src_field, const int dst_fi
  * kind 1 Operator::clean_ib_region_boundary (operators.hpp:73-187; sides whose
  * neighbour is missing or not IB). */
 int iblgf_gpu_clean_boundary(iblgf_gpu_ctx* ctx, int field, int level, int kind, int leaf_only, int width);
+
+/* ---- adaptation: device mirror refresh -------------------------------------
+ * After iblgf_tree_adapt and iblgf_tree_build_masks on the context's tree:
+ * the tree re-uploaded (plans and lists rebuilt; kernel spectra, which do
+ * not depend on the tree, kept), every field id kept at the new size; the nf
+ * fields listed (FP64; one field group such as the components of a vector,
+ * comp[i] its field index, mesh[i] its mesh type) carried over block by
+ * block by key on the device, and the new leaves filled as Ifherk::adapt does
+ * (ifherk.hpp:1047-1074): per field, halo fills of the group at every level,
+ * then every parent of a new leaf (refinement level > 0, in new-leaf key
+ * order) interpolated into all its children.  Unlisted fields restart at
+ * zero; an immersed-boundary system of the context is dropped (create it
+ * again).  nparents (or NULL) receives the parents interpolated. */
+int iblgf_gpu_tree_readapt(iblgf_gpu_ctx* ctx, const iblgf_tree* t, int nf, const int* fields, const int* mesh,
+    const int* comp, int* nparents);
+/* Ifherk::up (ifherk.hpp:999-1004) after clean<F>(true): the leaves' ring and
+ * the non-leaf blocks of k fields zeroed, children coarsened into parents
+ * (field c is field index c % 3) - what Ifherk::adapt does to u first. */
+int iblgf_gpu_coarsen(iblgf_gpu_ctx* ctx, const int* fields, int k, int mesh_type);
 
 /* ---- IF-HERK time integration (Ifherk, ifherk.hpp) ------------------------
  * Velocity fields are face vectors (three fields, component c the faces

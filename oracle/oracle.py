@@ -127,6 +127,47 @@ def mesh_transfer(block, mesh, comp, op, child, out=None):
     return o
 
 
+def remap(old_arrays, new_tree, fields, mesh, comp):
+    """The fields after an adaptation, as Ifherk::adapt fills them
+    (ifherk.hpp:1047-1074): every block kept by key, the others zero; then per
+    field the halo fills of every field at levels depth-2 .. base, and every
+    parent of a new leaf (refinement level > 0, not a leaf before; in key order
+    of the new leaves, repeats dropped) interpolated into all its children
+    (nli_intrp_node, cell_center_nli_intrp.hpp:310-420).  new_tree is an
+    oracle Tree of the adapted tree; returns the new field arrays and the
+    slots of the parents interpolated."""
+    a, info, n = new_tree.a, new_tree.info, new_tree.n
+    old_slot = {int(k): s for s, k in enumerate(old_arrays["key"])}
+    old_leaves = {int(k) for k, f in zip(old_arrays["key"], old_arrays["flags"]) if f & 1}
+    out = []
+    for f in fields:
+        g = np.zeros((info["nblocks"], n, n, n))
+        for s, k in enumerate(a["key"]):
+            o = old_slot.get(int(k))
+            if o is not None:
+                g[s] = f[o]
+        out.append(g)
+    bl = info["base_level"]
+    new = [s for s in range(info["nblocks"]) if (a["flags"][s] & 1) and a["level"][s] > bl
+           and int(a["key"][s]) not in old_leaves]
+    new.sort(key=lambda s: int(a["key"][s]))
+    parents = []
+    for s in new:
+        pa = int(a["parent"][s])
+        if pa >= 0 and (not parents or parents[-1] != pa):
+            parents.append(pa)
+    for i in range(len(out)):
+        for l in range(info["depth"] - 2, bl - 1, -1):
+            for g in out:
+                new_tree.buffer_exchange(g, l)
+        for pa in parents:
+            for c in range(8):
+                ch = int(a["children"][pa][c])
+                if ch >= 0:
+                    out[i][ch] = mesh_transfer(out[i][pa], mesh[i], comp[i], 1, c)
+    return out, parents
+
+
 def direct_sum(src_xyz, q, tgt_xyz, scale, nmax=100):
     sx = np.ascontiguousarray(src_xyz, np.int32)
     tx = np.ascontiguousarray(tgt_xyz, np.int32)
