@@ -505,6 +505,104 @@ class GaussianFlow:
         return out
 
 
+class NsAmrLgfRing:
+    """The reference's IF-HERK known-answer test (tests/ns_amr_lgf,
+    configs/configFile_0 and configFile_1): a fat vortex ring
+    (vortex_ring_vor_fat_ic, ns_amr_lgf.hpp:596-619: w_theta = alpha / R^2
+    exp(-4 s^2 / (R^2 - s^2)), alpha = 0.54857674, R = 0.5, s the distance to
+    the core circle) in the box [-24, 24)^3 base cells with 6^3-cell blocks;
+    config 0: dx_base 0.25, no refinement, 4 base steps; config 1: dx_base
+    0.5, one refinement level where the vorticity over the block grown by two
+    cells exceeds w_max 0.125^diff (refinement, ns_amr_lgf.hpp:757-818), 2
+    base steps of 2 finest steps.  Re 1000, cfl 0.35: dt_base = dx_base cfl
+    (ifherk.hpp:125), dt = dt_base / 2^nref, so both run 4 steps of 0.0875 to
+    T = 0.35.  The initial velocity is -curl^T of the LGF solve of the
+    vorticity (ns_amr_lgf.hpp:505-560); after the steps the reference cleans
+    the base level (clean_leaf_correction_boundary(base, true)) and compares
+    u_0 on the non-correction leaves of the finest level with ref.hdf5
+    (cells [-12, 11]^3 at dx 0.25, zero outside) to L_inf EXP_LInf (1e-2 for
+    config 0, 3e-4 for config 1)."""
+
+    ALPHA, R = 0.54857674, 0.5
+    DT, RE, STEPS = 0.0875, 1000.0, 4
+
+    def __init__(self, config=0):
+        self.config = int(config)
+        self.dx_base = 0.25 if self.config == 0 else 0.5
+        self.EXP_LINF = 1e-2 if self.config == 0 else 3e-4
+        self.tree = Tree((-24,) * 3, (48,) * 3, 6, self.dx_base, [(-24, -24, -24, 48, 48, 48)])
+        if self.config == 0:
+            self.tree.init_refine(0)
+        else:
+            w_max = abs(self.ALPHA / (self.R * self.R))
+
+            def refine(b, ext, rl, diff):
+                dxl = self.dx_base / 2 ** rl
+                i = np.arange(b[0] - 2, b[0] + ext + 2)
+                j = np.arange(b[1] - 2, b[1] + ext + 2)
+                k = np.arange(b[2] - 2, b[2] + ext + 2)
+                K, J, I = np.meshgrid(k, j, i, indexing="ij")
+                thr = w_max * 0.125 ** diff
+                for c in range(3):
+                    x = (I + (0.5 if c == 0 else 0.0)) * dxl
+                    y = (J + (0.5 if c == 1 else 0.0)) * dxl
+                    z = (K + (0.5 if c == 2 else 0.0)) * dxl
+                    if np.any(np.abs(self.vorticity_at(x, y, z, c)) > thr):
+                        return True
+                return False
+
+            self.tree.init_refine(1, refine)
+        self.tree.build_masks()
+        self.arrays = self.tree.export()
+        self.info = self.tree.info()
+
+    @classmethod
+    def vorticity_at(cls, x, y, z, comp):
+        R2 = cls.R * cls.R
+        r = np.sqrt(x * x + y * y)
+        s2 = z * z + (r - cls.R) ** 2
+        theta = np.arctan2(y, x)
+        out = np.zeros(np.shape(x))
+        m = s2 < R2
+        w = cls.ALPHA / R2 * np.exp(-4.0 * s2[m] / (R2 - s2[m]))
+        if comp == 0:
+            out[m] = -w * np.sin(theta[m])
+        elif comp == 1:
+            out[m] = w * np.cos(theta[m])
+        return out
+
+    def vorticity(self):
+        """[3][nb, n, n, n]: the edge components on every block's interior."""
+        n, nb = self.tree.n, self.info["nblocks"]
+        inner = _interior_mask(n)
+        out = [np.zeros((nb, n, n, n)) for _ in range(3)]
+        for s in range(nb):
+            for c in range(3):
+                P = mesh_points(self.arrays, s, n, self.dx_base, self.info["base_level"], 2, c)
+                out[c][s][inner] = self.vorticity_at(P[..., 0], P[..., 1], P[..., 2], c)[inner]
+        return out
+
+    def reference_error(self, u0, ref):
+        """L_inf of u_0 minus the reference's u_0 (ref dict from
+        tests/golden/ns_amr_lgf_ref.npz) over the interiors of the
+        non-correction leaves of the finest level."""
+        a, n = self.arrays, self.tree.n
+        lo, m = int(ref["lo"]), ref["u"].shape[-1]
+        L = self.info["depth"] - 1
+        err = 0.0
+        for s in range(self.info["nblocks"]):
+            if a["level"][s] != L or not (a["flags"][s] & FLAG_LEAF) or (a["flags"][s] & FLAG_CORRECTION):
+                continue
+            b = a["bbase"][s]
+            idx = [np.arange(b[d], b[d] + n - 2) - lo for d in range(3)]
+            refblk = np.zeros((n - 2,) * 3)
+            ok = [(i >= 0) & (i < m) for i in idx]
+            sub = ref["u"][0][np.ix_(idx[2][ok[2]], idx[1][ok[1]], idx[0][ok[0]])]
+            refblk[np.ix_(ok[2], ok[1], ok[0])] = sub
+            err = max(err, float(np.max(np.abs(u0[s][1:-1, 1:-1, 1:-1] - refblk))))
+        return err
+
+
 def fibonacci_sphere(npoints, R=0.5):
     """IB::read_points, geometry "sphere" (include/iblgf/domain/ib.hpp:179-200)."""
     lam = (1.0 + math.sqrt(5)) / 2.0

@@ -206,3 +206,53 @@ def test_ib_step_matches_oracle_and_enforces_the_boundary_velocity(oracle_mod):
     import ctypes
     a = (ctypes.c_int * 3)(*g)
     assert lib().iblgf_gpu_ifherk_step_ib(s2._h, a, -1, dt, 200.0, None, 0, None, 1e-6, 10, None) == IBLGF_ERR_STATE
+
+
+@pytest.mark.parametrize("config", [0, 1])
+def test_device_ifherk_matches_the_reference_vortex_ring_run(oracle_mod, config):
+    """The reference's known answer (tests/ns_amr_lgf with ref.hdf5, fixture
+    tests/golden/ns_amr_lgf_ref.npz) on the device: the ring initialised by
+    the batched edge-mesh LGF solve and -curl^T per level, four
+    iblgf_gpu_ifherk_step calls, the base level cleaned; u_0 on the finest
+    non-correction leaves within the reference's EXP_LInf of its u_0, and the
+    whole velocity within 1e-10 of the oracle's run."""
+    import os
+
+    from iblgf_amd import MESH_EDGE, OP_CURL_TRANSPOSE
+
+    print(f"case: ns_amr_lgf configFile_{config} on the device", flush=True)
+    ns = setups.NsAmrLgfRing(config)
+    ref = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", "ns_amr_lgf_ref.npz")))
+    info = ns.info
+    L = info["base_level"]
+    s = PoissonSolver(ns.tree)
+    w = [s.field() for _ in range(3)]
+    psi = [s.field() for _ in range(3)]
+    u = [s.field() for _ in range(3)]
+    for f, x in zip(w, ns.vorticity()):
+        s.upload(f, x)
+    s.apply_lgf_batch(w, psi, mesh=MESH_EDGE)
+    for l in range(L, info["depth"]):
+        s.block_op(OP_CURL_TRANSPOSE, psi, u, l, scale=-1.0)
+        for c in range(3):
+            s.buffer_exchange(u[c], l)
+    for _ in range(ns.STEPS):
+        s.ifherk_step(u, ns.DT, ns.RE)
+    for c in range(3):
+        s.smooth_boundary(u[c], L, True)
+    got = [s.download(f) for f in u]
+    err = ns.reference_error(got[0], ref)
+    # the oracle's run
+    import test_ifherk_oracle as T
+
+    ot = oracle_mod.Tree(ns.arrays, info, ns.tree.n)
+    v = T._ns_init_oracle(ot, ns)
+    for _ in range(ns.STEPS):
+        v, _p = ot.ifherk_step(v, ns.DT, ns.RE)
+    for c in range(3):
+        ot.smooth_boundary(v[c], L, True)
+    leaves = (ns.arrays["flags"] & FLAG_LEAF) != 0
+    e_or = _rel([x[leaves] for x in got], [x[leaves] for x in v])
+    print(f"config {config}: L_inf vs ref.hdf5 {err:.3e} (EXP_LInf {ns.EXP_LINF}), relL2 vs oracle {e_or:.2e}",
+          flush=True)
+    assert err < ns.EXP_LINF and e_or < 1e-10
