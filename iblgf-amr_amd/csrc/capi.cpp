@@ -4,6 +4,7 @@
 #include "capi_common.hpp"
 
 #include "gpu_solver.hpp"
+#include "host/chombo.hpp"
 #include "ib_solver.hpp"
 #include "learned_precond.hpp"
 #include "noalloc.hpp"
@@ -215,6 +216,47 @@ int iblgf_gpu_field_download(iblgf_gpu_ctx* ctx, int field_id, double* host)
     NEED_CTX("iblgf_gpu_field_download");
     if (!host) return fail(IBLGF_ERR_ARG, "iblgf_gpu_field_download: null host pointer", ctx);
     return guarded(ctx, [&] { ctx->solver->field_download(field_id, host); });
+}
+
+int iblgf_gpu_chombo_write(iblgf_gpu_ctx* ctx, const char* path, int nfields, const char* const* names,
+    const int* ncomp, const int* field_ids, int include_correction, int base_correction_only, double time, double dt)
+{
+    NEED_CTX("iblgf_gpu_chombo_write");
+    if (!path || nfields < 1 || !names || !ncomp || !field_ids)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_chombo_write: invalid argument", ctx);
+    return guarded(ctx, [&] {
+        GpuSolver& s = *ctx->solver;
+        if (s.num_trees() != 1) throw capi::Unsupported("iblgf_gpu_chombo_write: a forest context holds several trees");
+        const size_t per = s.field_size();
+        std::vector<std::vector<double>> host(nfields);
+        std::vector<ChomboField>         f;
+        int                              id = 0;
+        for (int i = 0; i < nfields; ++i)
+        {
+            if (!names[i] || ncomp[i] < 1) throw std::invalid_argument("iblgf_gpu_chombo_write: bad field");
+            host[i].resize(per * ncomp[i]);
+            for (int c = 0; c < ncomp[i]; ++c) s.field_download(field_ids[id++], host[i].data() + per * c);
+            f.push_back(ChomboField{names[i], ncomp[i], host[i].data()});
+        }
+        chombo_write(s.tree(), path, f, include_correction != 0, base_correction_only != 0, time, dt);
+    });
+}
+
+int iblgf_gpu_chombo_read(iblgf_gpu_ctx* ctx, const char* path, const char* name, int ncomp, int mesh_type,
+    const int* field_ids)
+{
+    NEED_CTX("iblgf_gpu_chombo_read");
+    if (!path || !name || ncomp < 1 || mesh_type < 0 || mesh_type > 3 || !field_ids)
+        return fail(IBLGF_ERR_ARG, "iblgf_gpu_chombo_read: invalid argument", ctx);
+    return guarded(ctx, [&] {
+        GpuSolver& s = *ctx->solver;
+        if (s.num_trees() != 1) throw capi::Unsupported("iblgf_gpu_chombo_read: a forest context holds several trees");
+        const size_t        per = s.field_size();
+        std::vector<double> host(per * ncomp);
+        for (int c = 0; c < ncomp; ++c) s.field_download(field_ids[c], host.data() + per * c);
+        chombo_read(s.tree(), path, name, ncomp, mesh_type, host.data());
+        for (int c = 0; c < ncomp; ++c) s.field_upload(field_ids[c], host.data() + per * c);
+    });
 }
 
 int iblgf_gpu_apply_lgf(iblgf_gpu_ctx* ctx, int src_field, int dst_field, int kernel, double alpha_base)

@@ -4,6 +4,7 @@
 // citations are in include/iblgf_gpu.h.
 #include "capi_common.hpp"
 
+#include "host/chombo.hpp"
 #include "host/synth.hpp"
 #include "host/tables.hpp"
 
@@ -268,6 +269,84 @@ int iblgf_tree_export(const iblgf_tree* t, uint64_t* key, int* level, int* coord
 }
 
 void iblgf_tree_destroy(iblgf_tree* t) { delete t; }
+
+// ---- output and restart files ------------------------------------------------
+
+int iblgf_tree_write_restart(const iblgf_tree* t, const char* path)
+{
+    if (!t || !path) return capi::fail(IBLGF_ERR_ARG, "iblgf_tree_write_restart: invalid argument");
+    return capi::guarded(nullptr, [&] { tree_write_restart(t->tree, path); });
+}
+
+int iblgf_tree_read_restart(const char* path, const int bd_base[3], const int bd_extent[3], int block_extent,
+    double dx_base, iblgf_tree** out)
+{
+    if (!path || !bd_base || !bd_extent || block_extent <= 0 || !(dx_base > 0) || !out)
+        return capi::fail(IBLGF_ERR_ARG, "iblgf_tree_read_restart: invalid argument");
+    return capi::guarded(nullptr, [&] {
+        std::vector<uint64_t> keys;
+        std::vector<uint8_t>  leafs;
+        tree_read_restart(path, keys, leafs);
+        auto    t = std::make_unique<iblgf_tree>();
+        Octree& tr = t->tree;
+        tr.block_extent = block_extent;
+        tr.dx_base = dx_base;
+        int maxext = 0;
+        for (int d = 0; d < 3; ++d)
+        {
+            if (bd_extent[d] % block_extent || bd_base[d] % block_extent)
+                throw std::invalid_argument("iblgf_tree_read_restart: domain is not a whole number of blocks");
+            tr.base_shift[d] = bd_base[d] / block_extent;
+            maxext = std::max(maxext, bd_extent[d] / block_extent);
+            tr.key_box_base[d] = 2;  // baseBlockBufferNumber
+            tr.key_box_extent[d] = bd_extent[d] / block_extent - 4;
+        }
+        tr.base_level = Morton::minimum_level(maxext);
+        tr.depth = tr.base_level + 1;
+        tr.insert(Morton::make(0, 0, 0, 0));
+        for (size_t i = 0; i < keys.size(); ++i)
+        {
+            if ((keys[i] & 1ULL) == 0 || Morton::level(keys[i]) > Morton::max_level)
+                throw std::invalid_argument("iblgf_tree_read_restart: malformed key");
+            const int o = tr.insert(keys[i]);
+            if (tr.oct[o].has_data) throw std::invalid_argument("iblgf_tree_read_restart: duplicate key");
+            tr.oct[o].has_data = true;
+            tr.oct[o].leaf = leafs[i] != 0;
+        }
+        // Domain::restart_list_construct (domain.hpp:839-849).
+        tr.construct_level_maps();
+        tr.construct_lists();
+        tr.construct_leaf_maps(true);
+        tr.mark_correction();
+        tr.mark_leaf_boundary();
+        *out = t.release();
+    });
+}
+
+int iblgf_chombo_write(const iblgf_tree* t, const char* path, int nfields, const char* const* names,
+    const int* ncomp, const double* const* data, int include_correction, int base_correction_only, double time,
+    double dt)
+{
+    if (!t || !path || nfields < 1 || !names || !ncomp || !data)
+        return capi::fail(IBLGF_ERR_ARG, "iblgf_chombo_write: invalid argument");
+    return capi::guarded(nullptr, [&] {
+        std::vector<ChomboField> f;
+        for (int i = 0; i < nfields; ++i)
+        {
+            if (!names[i] || ncomp[i] < 1 || !data[i]) throw std::invalid_argument("iblgf_chombo_write: bad field");
+            f.push_back(ChomboField{names[i], ncomp[i], data[i]});
+        }
+        chombo_write(t->tree, path, f, include_correction != 0, base_correction_only != 0, time, dt);
+    });
+}
+
+int iblgf_chombo_read(const iblgf_tree* t, const char* path, const char* name, int ncomp, int mesh_type,
+    double* data)
+{
+    if (!t || !path || !name || ncomp < 1 || mesh_type < 0 || mesh_type > 3 || !data)
+        return capi::fail(IBLGF_ERR_ARG, "iblgf_chombo_read: invalid argument");
+    return capi::guarded(nullptr, [&] { chombo_read(t->tree, path, name, ncomp, mesh_type, data); });
+}
 
 uint64_t iblgf_key_make(int x, int y, int z, int level) { return Morton::make(x, y, z, level); }
 int      iblgf_key_coord(uint64_t key, int* xyz)

@@ -37,7 +37,7 @@ PREC_FP16X3 = 3
 PREC_FP16 = 4
 PREC_INT8X4 = 5
 PREC_FP16X3_FORWARD = 6
-MESH_CELL, MESH_FACE, MESH_EDGE = range(3)
+MESH_CELL, MESH_FACE, MESH_EDGE, MESH_VERTEX = range(4)
 
 FLAG_LEAF = 1
 FLAG_CORRECTION = 2
@@ -158,6 +158,14 @@ SYMBOLS = {
     "iblgf_gpu_ib_fcg": (_I, [_P, _DP, _DP, _D, _D, _I, _I, _IP, _DP, _DP]),
     "iblgf_gpu_ib_switch_iteration": (_I, [_P, _IP]),
     "iblgf_gpu_ib_recycle": (_I, [_P, _I]),
+    "iblgf_tree_write_restart": (_I, [_P, ctypes.c_char_p]),
+    "iblgf_tree_read_restart": (_I, [ctypes.c_char_p, _IP, _IP, _I, _D, ctypes.POINTER(_P)]),
+    "iblgf_chombo_write": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(ctypes.c_char_p), _IP,
+                                ctypes.POINTER(_DP), _I, _I, _D, _D]),
+    "iblgf_chombo_read": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I, _DP]),
+    "iblgf_gpu_chombo_write": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(ctypes.c_char_p), _IP, _IP, _I, _I,
+                                    _D, _D]),
+    "iblgf_gpu_chombo_read": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I, _IP]),
 }
 
 KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
@@ -327,6 +335,53 @@ class Tree:
         return dict(nblocks=nb.value, base_level=bl.value, depth=dp.value, block_extent=be.value,
                     dx_base=dx.value, nleaves=nl.value)
 
+    def write_restart(self, path):
+        """Tree::write: keys of the octants with data (depth-first) and their leaf flags."""
+        _check(lib().iblgf_tree_write_restart(self._h, os.fsencode(path)))
+
+    @classmethod
+    def read_restart(cls, path, bd_base, bd_extent, block_extent, dx_base):
+        """Domain::initialize_with_keys + restart_list_construct from a Tree::write file."""
+        obj = cls.__new__(cls)
+        h = ctypes.c_void_p()
+        _check(lib().iblgf_tree_read_restart(os.fsencode(path), _ip(bd_base), _ip(bd_extent), int(block_extent),
+                                             float(dx_base), ctypes.byref(h)))
+        obj._h = h
+        obj.block_extent = int(block_extent)
+        obj.n = obj.block_extent + 2
+        obj.dx_base = float(dx_base)
+        return obj
+
+    def write_chombo(self, path, fields, include_correction=True, base_correction_only=True, time=0.0, dt=1.0):
+        """H5io::write_h5: fields is a list of (name, array [ncomp, nblocks, n, n, n]) (or [nblocks, n, n, n]
+        for one component), written in the Chombo layout."""
+        names, ncomp, arrs = [], [], []
+        for name, a in fields:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.ndim == 4:
+                a = a[None]
+            names.append(name.encode())
+            ncomp.append(a.shape[0])
+            arrs.append(a)
+        nm = (ctypes.c_char_p * len(names))(*names)
+        nc = np.ascontiguousarray(ncomp, np.int32)
+        dp = (_DP * len(arrs))(*[a.ctypes.data_as(_DP) for a in arrs])
+        _check(lib().iblgf_chombo_write(self._h, os.fsencode(path), len(names), nm, nc.ctypes.data_as(_IP), dp,
+                                        int(bool(include_correction)), int(bool(base_correction_only)), float(time),
+                                        float(dt)))
+
+    def read_chombo(self, path, name, ncomp=1, mesh=0, out=None):
+        """H5io::read_h5 (Chombo::read_u): field `name` into [ncomp, nblocks, n, n, n]; blocks outside
+        blocks_list_build(true, true) keep the values of `out` (zeros when out is None)."""
+        nb = self.info()["nblocks"]
+        a = np.zeros((ncomp, nb, self.n, self.n, self.n)) if out is None else np.array(out, dtype=np.float64,
+                                                                                        copy=True).reshape(
+            ncomp, nb, self.n, self.n, self.n)
+        a = np.ascontiguousarray(a)
+        _check(lib().iblgf_chombo_read(self._h, os.fsencode(path), name.encode(), int(ncomp), int(mesh),
+                                       a.ctypes.data_as(_DP)))
+        return a
+
     def export(self):
         nb = self.info()["nblocks"]
         a = dict(
@@ -464,6 +519,22 @@ class PoissonSolver:
         a = np.empty(self.shape, np.float64)
         _check(lib().iblgf_gpu_field_download(self._h, int(fid), a.ctypes.data_as(_DP)), self._h)
         return a
+
+    def write_chombo(self, path, fields, include_correction=True, base_correction_only=True, time=0.0, dt=1.0):
+        """Simulation::write on device fields: fields is a list of (name, [field ids])."""
+        names = [n.encode() for n, _ in fields]
+        ncomp = np.ascontiguousarray([len(ids) for _, ids in fields], np.int32)
+        ids = np.ascontiguousarray([i for _, f in fields for i in f], np.int32)
+        nm = (ctypes.c_char_p * len(names))(*names)
+        _check(lib().iblgf_gpu_chombo_write(self._h, os.fsencode(path), len(names), nm, ncomp.ctypes.data_as(_IP),
+                                            ids.ctypes.data_as(_IP), int(bool(include_correction)),
+                                            int(bool(base_correction_only)), float(time), float(dt)), self._h)
+
+    def read_chombo(self, path, name, fields, mesh=0):
+        """Simulation::read_h5 into device fields (one id per component)."""
+        ids = np.ascontiguousarray(fields, np.int32)
+        _check(lib().iblgf_gpu_chombo_read(self._h, os.fsencode(path), name.encode(), len(ids), int(mesh),
+                                           ids.ctypes.data_as(_IP)), self._h)
 
     def set_precision(self, precision):
         """PREC_FP64 (plain); PREC_MIXED (FP32 spectra and MAC products, FP64 DFT legs,

@@ -130,6 +130,46 @@ int iblgf_tree_export(const iblgf_tree* t, uint64_t* key, int* level, int* coord
 
 void iblgf_tree_destroy(iblgf_tree* t);
 
+/* ---- output and restart files (host) ------------------------------------ */
+
+/* Tree::write (include/iblgf/domain/octree/tree.hpp:832-853): the keys of the
+ * octants with data in depth-first order (uint64 each), then one bool leaf
+ * flag per key; Simulation::write_tree (include/iblgf/simulation.hpp:89-97). */
+int iblgf_tree_write_restart(const iblgf_tree* t, const char* path);
+
+/* Domain::initialize_with_keys + restart_list_construct (include/iblgf/domain/
+ * domain.hpp:212-283, 839-849): the tree of a Tree::write file on the domain
+ * {bd_base, bd_extent} (cells of the base level); correction and leaf-boundary
+ * flags are recomputed (mark_correction, mark_leaf_boundary). */
+int iblgf_tree_read_restart(const char* path, const int bd_base[3], const int bd_extent[3], int block_extent,
+    double dx_base, iblgf_tree** out);
+
+/* H5io::write_h5 (include/iblgf/IO/chombo/h5_io.hpp:161-173) with
+ * Chombo::write_global_metaData / write_level_info (include/iblgf/IO/chombo/
+ * chombo_impl.hpp:1755-2026, 2336-2598): a Chombo-layout HDF5 file of the
+ * blocks of blocks_list_build (h5_io.hpp:232-263) — refinement level >= 0,
+ * corrections only with include_correction, and then only on the base level
+ * with base_correction_only (the reference's flow and restart files pass 1, 1,
+ * simulation.hpp:99-121).  Field i is named names[i] (one component) or
+ * names[i]_c, has ncomp[i] components and is the host array data[i] of
+ * [ncomp[i]][nslots][n^3] in slot order.  time and dt are the level
+ * attributes (write_h5 passes 0 and 1). */
+int iblgf_chombo_write(const iblgf_tree* t, const char* path, int nfields, const char* const* names,
+    const int* ncomp, const double* const* data, int include_correction, int base_correction_only, double time,
+    double dt);
+
+/* H5io::read_h5 + Chombo::read_u (h5_io.hpp:96-108, chombo_impl.hpp:217-626):
+ * field `name` with ncomp components of mesh type mesh_type (IBLGF_MESH_*, or
+ * 3 for vertex fields) into data ([ncomp][nslots][n^3]): zero on the blocks of
+ * blocks_list_build(true, true), then every leaf receives the file's cells at
+ * its level; a coarser leaf averages, per axis, the one or two middle file
+ * cells of the 2^d it covers (one along a face field's own axis, one on the
+ * other axes of an edge field).  The file's base dx may be the tree's
+ * divided by a power of two.  A name the file does not hold is an error
+ * (IBLGF_ERR_ARG; the reference reads component 0 then). */
+int iblgf_chombo_read(const iblgf_tree* t, const char* path, const char* name, int ncomp, int mesh_type,
+    double* data);
+
 /* Morton key helpers: Key<3>::compute_index / coordinate / parent / child
  * (include/iblgf/domain/octree/key.hpp:64-75, 124-132, 254-275). */
 uint64_t iblgf_key_make(int x, int y, int z, int level);
@@ -209,6 +249,17 @@ int iblgf_gpu_field_download(iblgf_gpu_ctx* ctx, int field_id, double* host);
 int iblgf_gpu_field_create_f32(iblgf_gpu_ctx* ctx, int* field_id);
 int iblgf_gpu_field_upload_f32(iblgf_gpu_ctx* ctx, int field_id, const float* host);
 int iblgf_gpu_field_download_f32(iblgf_gpu_ctx* ctx, int field_id, float* host);
+
+/* Simulation::write / read_h5 (include/iblgf/simulation.hpp:99-121, 196-200)
+ * on device fields: iblgf_chombo_write / iblgf_chombo_read with the
+ * context's tree, field i taking ncomp[i] consecutive ids of field_ids.
+ * The fields go through host memory; a forest context is refused
+ * (IBLGF_ERR_UNSUPPORTED). */
+int iblgf_gpu_chombo_write(iblgf_gpu_ctx* ctx, const char* path, int nfields, const char* const* names,
+    const int* ncomp, const int* field_ids, int include_correction, int base_correction_only, double time,
+    double dt);
+int iblgf_gpu_chombo_read(iblgf_gpu_ctx* ctx, const char* path, const char* name, int ncomp, int mesh_type,
+    const int* field_ids);
 
 enum iblgf_kernel
 {

@@ -2,7 +2,8 @@
  * half of the C ABI (octree, keys, masks, immersed-boundary weights, kernel
  * tables) and the FP64 oracle, linked into one executable by
  * tests/test_sanitizers.py.  Exercises tree construction with refinement,
- * correction blocks, masks and export, the immersed-boundary stencil, and
+ * correction blocks, masks and export, the output and restart files, the
+ * immersed-boundary stencil, and
  * the oracle's Laplace and integrating-factor solves and IB transfers on the
  * exported trees.  Exit status 0 when every call succeeds; the sanitizers
  * abort on the first error they find. */
@@ -178,6 +179,30 @@ int main(void)
         export_table(t, &a, &o);
         CHECK(o.depth > o.base_level + 1);
         solve_both(&o, g, nmax);
+        /* output and restart files: Chombo HDF5 and Tree::write, read back */
+        {
+            const size_t  nv = (size_t)o.nb * o.n * o.n * o.n;
+            double*       u = (double*)malloc(3 * nv * sizeof(double));
+            double*       back = (double*)calloc(3 * nv, sizeof(double));
+            const char*   names[1] = {"u"};
+            const int     nc[1] = {3};
+            const double* data[1] = {u};
+            iblgf_tree*   t2 = NULL;
+            int           nb2 = 0, b2, d2, e2, nl2;
+            double        dx2;
+            for (size_t i = 0; i < 3 * nv; ++i) u[i] = (double)(i % 97) - 48.0;
+            CHECK(iblgf_chombo_write(t, "sanitize_flow.hdf5", 1, names, nc, data, 1, 1, 0.0, 1.0) == IBLGF_OK);
+            CHECK(iblgf_chombo_read(t, "sanitize_flow.hdf5", "u", 3, IBLGF_MESH_FACE, back) == IBLGF_OK);
+            CHECK(iblgf_chombo_read(t, "sanitize_flow.hdf5", "w", 3, IBLGF_MESH_FACE, back) == IBLGF_ERR_ARG);
+            CHECK(iblgf_tree_write_restart(t, "sanitize_tree.bin") == IBLGF_OK);
+            CHECK(iblgf_tree_read_restart("sanitize_tree.bin", base, ext, 8, 1.0 / 32, &t2) == IBLGF_OK);
+            CHECK(iblgf_tree_info(t2, &nb2, &b2, &d2, &e2, &dx2, &nl2) == IBLGF_OK && nb2 == o.nb);
+            iblgf_tree_destroy(t2);
+            remove("sanitize_flow.hdf5");
+            remove("sanitize_tree.bin");
+            free(u);
+            free(back);
+        }
         free_table(&a);
         iblgf_tree_destroy(t);
     }

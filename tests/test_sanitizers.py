@@ -33,7 +33,7 @@ def test_host_code_and_oracle_are_sanitizer_clean(tmp_path):
     subprocess.run(["g++", "-o", exe] + objs + SAN + ["-lm"], check=True, capture_output=True)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1:verify_asan_link_order=0",
                UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
-    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=600)
+    r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=600, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
     assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr
     assert "sanitized host checks passed" in r.stdout
