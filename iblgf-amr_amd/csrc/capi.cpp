@@ -346,7 +346,7 @@ int iblgf_gpu_autotune(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst
 int iblgf_gpu_set_launch_shapes(iblgf_gpu_ctx* ctx, int mac_policy, int dft_grid)
 {
     NEED_CTX("iblgf_gpu_set_launch_shapes");
-    if (mac_policy < 0 || mac_policy > 1 || dft_grid < 0 || dft_grid > 1)
+    if (mac_policy < 0 || mac_policy > 2 || dft_grid < 0 || dft_grid > 1)
         return fail(IBLGF_ERR_ARG, "iblgf_gpu_set_launch_shapes: shape out of range", ctx);
     return guarded(ctx, [&] { ctx->solver->set_launch_shapes(mac_policy, dft_grid); });
 }

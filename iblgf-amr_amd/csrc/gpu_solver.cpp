@@ -732,6 +732,7 @@ void GpuSolver::build_pass(PassPlan& p, int L, int idx, bool nbonly)
             }
             m.olist.upload(ol);
             m.rowoff.upload(ro);
+            m.queue.upload(std::vector<int>(9, 0));
         }
         p.npairs += pairs_here;
         p.mac.push_back(std::move(m));
@@ -936,7 +937,7 @@ void GpuSolver::run_pass(PassPlan& p, double scale, bool laplace, double alpha, 
         hipStream_t         s = m.lambda == L ? A : B;
         timer_.begin(s);
         launch::spectral_mac(s, n_, precision_, m.groups.p, m.groups.size(), m.srcs.p, m.rowoff.p, m.olist.p,
-            m.nused, m.compact, gh[i], wp_.sspec, wp_.tspec, b);
+            m.nused, m.compact, gh[i], wp_.sspec, wp_.tspec, b, m.queue.p);
         timer_.end(s, KC_MAC, m.npairs * b.k);
     }
     // Level-L targets (first in inv) add into the target array, coarser ones
@@ -1189,7 +1190,7 @@ void GpuSolver::solve_sequence(const std::vector<SolveItem>& items)
 
 void GpuSolver::set_launch_shapes(int mac_policy, int dft_grid)
 {
-    if (mac_policy < 0 || mac_policy > 1 || dft_grid < 0 || dft_grid > 1)
+    if (mac_policy < 0 || mac_policy > 2 || dft_grid < 0 || dft_grid > 1)
         throw std::invalid_argument("launch shape out of range");
     if (mac_policy == mac_policy_ && dft_grid == dft_grid_) return;
     sync();
@@ -1223,7 +1224,7 @@ std::string GpuSolver::autotune(const int* src, const int* dst, int k, bool lapl
     try
     {
         for (int conc = 1; conc >= 0; --conc)
-            for (int mp = 0; mp < 2; ++mp)
+            for (int mp = 0; mp < 3; ++mp)
                 for (int dg = 0; dg < 2; ++dg)
                 {
                     set_concurrency(conc != 0);

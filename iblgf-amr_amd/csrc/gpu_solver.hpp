@@ -58,6 +58,7 @@ struct MacLevelPlan
     DevArray<MacSrc>   srcs;
     DevArray<int>      olist;   // kernel-spectrum rows (offsets) the entries use
     DevArray<uint4>    rowoff;  // per entry: LDS byte offsets of its 8 rows
+    DevArray<int>      queue;   // work-queue counters of the launch (mac_policy 2: 8 XCD ranges + done), zero between launches
     int                nused = 0;
     bool               compact = false;  // stage only the olist rows
     long               npairs = 0;

@@ -45,7 +45,9 @@ struct Batch
     // Launch shapes chosen by the solver's autotuner (results do not depend
     // on them): mac_policy 0 spreads a full-size multiply-accumulate launch
     // over the resident workgroups in rounds, 1 gives every frequency chunk a
-    // workgroup; dft_grid 0 runs the block transforms on the resident
+    // workgroup, 2 runs the resident workgroups as a persistent grid that
+    // claims (chunk, group run) units from a device work queue; dft_grid 0
+    // runs the block transforms on the resident
     // workgroups looping over blocks, 1 gives every (block, component) a
     // workgroup.
     int mac_policy = 0;
@@ -137,9 +139,11 @@ void inverse_dft(hipStream_t s, int n, int prec, const void* spec, const int* sl
 // (nused or 343) is all zeros.  rowoff[e] holds entry e's 8 LDS byte offsets
 // row * kMacRowBytes (target child t in 16-bit field t; children outside the
 // entry's mask point at the zero row).
+// queue: the launch's nine work-queue counters (zero between launches),
+// used when b.mac_policy is 2.
 void spectral_mac(hipStream_t s, int n, int prec, const MacGroup* groups, int ngroups, const MacSrc* srcs,
     const uint4* rowoff, const int* olist, int nused, bool compact, const void* ghat, const void* sspec,
-    void* tspec, const Batch& b = {});
+    void* tspec, const Batch& b = {}, int* queue = nullptr);
 void to_float2(hipStream_t s, const double2* in, float2* out, long n);
 
 // The forward and inverse transforms with every 1-D leg on the matrix cores

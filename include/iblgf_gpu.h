@@ -324,8 +324,10 @@ int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on);
 /* No counterpart in the reference.  Autotuner: times
  * iblgf_gpu_apply_lgf_batch(src_fields, dst_fields, k, kernel, alpha_base)
  * under every combination of the launch shapes that leave results unchanged
- * bit for bit — the multiply-accumulate grid (resident workgroups in rounds,
- * or one workgroup per frequency chunk), the block-transform grid (resident
+ * bit for bit — the multiply-accumulate grid (mac_policy 0: resident
+ * workgroups in rounds; 1: one workgroup per frequency chunk; 2: resident
+ * workgroups as a persistent grid claiming (frequency chunk, run of sibling
+ * groups) units from a device work queue), the block-transform grid (resident
  * workgroups looping over blocks, or one per block and component) and one or
  * two streams per FMM pass — with `reps` eager calls each after a warm-up,
  * keeps the fastest for the context's later solves and writes one line per
@@ -335,8 +337,8 @@ int iblgf_gpu_set_graphs(iblgf_gpu_ctx* ctx, int on);
 int iblgf_gpu_autotune(iblgf_gpu_ctx* ctx, const int* src_fields, const int* dst_fields, int k, int kernel,
     double alpha_base, int reps, char* report, int report_len);
 int iblgf_gpu_launch_shapes(iblgf_gpu_ctx* ctx, int* mac_policy, int* dft_grid, int* streams);
-/* Sets the two grid shapes directly (0 or 1 each, as above; streams through
- * iblgf_gpu_set_concurrency). */
+/* Sets the two grid shapes directly (mac_policy 0, 1 or 2, dft_grid 0 or 1,
+ * as above; streams through iblgf_gpu_set_concurrency). */
 int iblgf_gpu_set_launch_shapes(iblgf_gpu_ctx* ctx, int mac_policy, int dft_grid);
 
 typedef struct iblgf_solve_item

@@ -11,7 +11,8 @@ graph replays equal the eager run.
 import numpy as np
 import pytest
 
-from iblgf_amd import FLAG_LEAF, KERNEL_IF, KERNEL_LAPLACE, PREC_FP16X3_FORWARD, PREC_MIXED, PoissonSolver, setups
+from iblgf_amd import (FLAG_LEAF, KERNEL_IF, KERNEL_LAPLACE, PREC_FP16X3_FORWARD, PREC_FP64, PREC_MIXED, PoissonSolver,
+                       setups)
 
 pytestmark = pytest.mark.gpu
 
@@ -118,7 +119,7 @@ def test_solve_sequence_equals_calls_and_replays():
 
 def test_autotuner_keeps_results_and_records_its_choice():
     """Every launch shape the autotuner tries gives the default shape's result
-    bit for bit; it keeps the fastest and reports all eight timings."""
+    bit for bit; it keeps the fastest and reports all twelve timings."""
     ws, srcs = _workload()
     print("case: autotune on the forest, mixed precision, Laplace, 3 components", flush=True)
     fs = PoissonSolver([w.tree for w in ws])
@@ -129,7 +130,7 @@ def test_autotuner_keeps_results_and_records_its_choice():
         fs.upload(fi[c], fs.to_forest([s[c] for s in srcs]))
     fs.apply_lgf_batch(fi, fo)
     ref = [fs.download(f) for f in fo]
-    for mp in (0, 1):
+    for mp in (0, 1, 2):
         for dg in (0, 1):
             for conc in (True, False):
                 fs.set_launch_shapes(mp, dg)
@@ -139,11 +140,48 @@ def test_autotuner_keeps_results_and_records_its_choice():
     report, choice = fs.autotune(fi, fo, reps=2)
     print(report, flush=True)
     lines = [l for l in report.splitlines() if l.startswith("mac_policy=")]
-    assert len(lines) == 8 and report.splitlines()[-1].startswith("chosen:")
+    assert len(lines) == 12 and report.splitlines()[-1].startswith("chosen:")
     assert f"mac_policy={choice['mac_policy']} dft_grid={choice['dft_grid']} streams={choice['streams']}" in \
         report.splitlines()[-1]
     fs.apply_lgf_batch(fi, fo)
     assert all(np.array_equal(fs.download(f), r) for f, r in zip(fo, ref))
+
+
+@pytest.mark.parametrize("prec", [PREC_FP64, PREC_MIXED])
+def test_work_queue_launches_repeat_and_replay(prec):
+    """The persistent work-queue multiply-accumulate (mac_policy 2): its
+    counters reset themselves at the end of every launch, so repeated
+    Laplace and IF solves, eager and replayed from graphs, equal the default
+    grid's results bit for bit."""
+    ws, srcs = _workload()
+    print(f"case: work-queue MAC on the forest, precision {prec}, Laplace and IF, eager and graph", flush=True)
+    fs = PoissonSolver([w.tree for w in ws])
+    fs.set_precision(prec)
+    fi = [fs.field() for _ in range(3)]
+    fo = [fs.field() for _ in range(3)]
+    for c in range(3):
+        fs.upload(fi[c], fs.to_forest([s[c] for s in srcs]))
+
+    def run():
+        for f in fo:  # Laplace writes leaves only: start every sequence from zeros
+            fs.upload(f, np.zeros(fs.shape))
+        out = []
+        for kern, alpha in ((KERNEL_LAPLACE, 0.0), (KERNEL_IF, 0.4), (KERNEL_LAPLACE, 0.0)):
+            fs.apply_lgf_batch(fi, fo, kernel=kern, alpha_base=alpha)
+            fs.sync()
+            out.append([fs.download(f) for f in fo])
+        return out
+
+    ref = run()
+    fs.set_launch_shapes(2, 0)
+    for graphs in (False, True):
+        fs.set_graphs(graphs)
+        for rep in range(3):
+            got = run()
+            for i, (g, r) in enumerate(zip(got, ref)):
+                assert all(np.array_equal(a, b) for a, b in zip(g, r)), (graphs, rep, i)
+            fs.apply_lgf(fi[0], fo[0])  # other work between sequences
+    fs.set_graphs(False)
 
 
 def test_fp32_fields_round_the_fp64_field_solve():
