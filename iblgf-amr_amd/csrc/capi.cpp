@@ -485,6 +485,16 @@ int iblgf_gpu_smooth_boundary(iblgf_gpu_ctx* ctx, int field, int level, int leaf
     });
 }
 
+int iblgf_gpu_block_norms(iblgf_gpu_ctx* ctx, const int* fields, int k, double* maxnorm, double* rms)
+{
+    NEED_CTX("iblgf_gpu_block_norms");
+    if (!fields || k < 1 || k > 3) return fail(IBLGF_ERR_ARG, "iblgf_gpu_block_norms: 1 to 3 fields", ctx);
+    return guarded(ctx, [&] {
+        if (ctx->solver->nblocks() == 0) throw std::runtime_error("tree not uploaded");
+        ctx->solver->block_norms(fields, k, maxnorm, rms);
+    });
+}
+
 int iblgf_gpu_up_and_down(iblgf_gpu_ctx* ctx, const int* fields, int k, int mesh_type)
 {
     NEED_CTX("iblgf_gpu_up_and_down");

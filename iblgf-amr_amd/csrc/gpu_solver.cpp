@@ -421,6 +421,28 @@ void GpuSolver::field_download(int id, double* host)
     sync();
 }
 
+// Operator::maxnorm / blockRootMeanSquare (operators.hpp:1149-1167,
+// 1351-1367) per block: the device reduces the sums of squares, the host takes
+// the square roots (so the largest norm is the reference's bit for bit).
+void GpuSolver::block_norms(const int* f, int k, double* maxnorm, double* rms)
+{
+    if (k < 1 || k > 3) throw std::invalid_argument("block norms of 1 to 3 components");
+    const double* c[3] = {nullptr, nullptr, nullptr};
+    for (int i = 0; i < k; ++i) c[i] = field_ptr(f[i]);
+    DevArray<double> out;
+    out.alloc(2 * static_cast<size_t>(nb_));
+    launch::block_norms(stream_, n_, c, k, p_all_.p, nb_, out.p);
+    std::vector<double> h(2 * static_cast<size_t>(nb_));
+    IBLGF_HIP(hipMemcpyAsync(h.data(), out.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    sync();
+    const double e3 = static_cast<double>(e_) * e_ * e_;
+    for (int b = 0; b < nb_; ++b)
+    {
+        if (maxnorm) maxnorm[b] = std::sqrt(h[2 * b]);
+        if (rms) rms[b] = std::sqrt(h[2 * b + 1] / e3);
+    }
+}
+
 void GpuSolver::fill_zero(double* f, const Batch& b) { launch::zero_blocks(stream_, n_, f, p_all_.p, nb_, b); }
 
 FieldTable GpuSolver::field_table(const int* ids, int k)

@@ -279,6 +279,9 @@ class GpuSolver
     void add_constant(int f, double v);
     void clean_nonleaf(int f);
     void smooth_boundary(int f, int level, bool leaf_only);
+    // Operator::maxnorm and blockRootMeanSquare of the k <= 3 component field
+    // f[0..k) on every block with data, in slot order (either output may be null).
+    void block_norms(const int* f, int k, double* maxnorm, double* rms);
     void up_and_down(const int* f, int k, int mesh);
     void nonlinear(const int u[3], const int w[3], const int d[3], int level, double scale);
     // Ifherk::time_step (ifherk.hpp:1078-1200) without immersed-boundary

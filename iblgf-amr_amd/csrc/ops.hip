@@ -117,6 +117,62 @@ __global__ __launch_bounds__(kThreads) void k_clean_faces(double* f, const int* 
     }
 }
 
+// Operator::maxnorm and Operator::blockRootMeanSquare (include/iblgf/
+// operators/operators.hpp:1149-1167, 1351-1367) of a field of k <= 3
+// components on the listed blocks: per interior point the sum of the
+// components' squares in component order; per block the largest such sum
+// (out[2b]) and their total (out[2b+1]).  The square roots are the host's,
+// so the largest norm is the reference's bit for bit.
+template<int N>
+__global__ __launch_bounds__(kThreads) void k_block_norms(const double* __restrict__ c0,
+    const double* __restrict__ c1, const double* __restrict__ c2, int k, const int* __restrict__ slots, int count,
+    double* __restrict__ out)
+{
+#pragma clang fp contract(off)
+    constexpr int E = N - 2, E3 = E * E * E, N3 = N * N * N;
+    __shared__ double ssum[kThreads], smax[kThreads];
+    const int         b = blockIdx.x;
+    if (b >= count) return;
+    const size_t base = (size_t)slots[b] * N3;
+    double       sum = 0.0, mx = 0.0;
+    for (int t = threadIdx.x; t < E3; t += kThreads)
+    {
+        const int    i = t % E + 1, j = (t / E) % E + 1, kk = t / (E * E) + 1;
+        const size_t q = base + i + (size_t)N * (j + (size_t)N * kk);
+        double       v = c0[q];
+        double       tmp = v * v;
+        if (k > 1)
+        {
+            v = c1[q];
+            tmp = tmp + v * v;
+        }
+        if (k > 2)
+        {
+            v = c2[q];
+            tmp = tmp + v * v;
+        }
+        sum += tmp;
+        mx = fmax(mx, tmp);
+    }
+    ssum[threadIdx.x] = sum;
+    smax[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w)
+        {
+            ssum[threadIdx.x] += ssum[threadIdx.x + w];
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+    {
+        out[2 * b] = smax[0];
+        out[2 * b + 1] = ssum[0];
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void k_ib_spread(const double* __restrict__ force,
     const int* __restrict__ row_ptr, const int* __restrict__ row_dst, const int* __restrict__ col,
     const double* __restrict__ w, int nrows, double* f0, double* f1, double* f2, int add)
@@ -389,6 +445,16 @@ struct RunOp
 };
 
 template<int N>
+struct RunNorms
+{
+    static void run(hipStream_t s, const double* const c[3], int k, const int* slots, int count, double* out)
+    {
+        hipLaunchKernelGGL((k_block_norms<N>), dim3(count), dim3(kThreads), 0, s, c[0], c[1], c[2], k, slots, count,
+            out);
+    }
+};
+
+template<int N>
 struct RunClean
 {
     static void run(hipStream_t s, double* f, const int* entries, int count, int width)
@@ -474,6 +540,12 @@ void block_op(hipStream_t s, int n, int op, const double* const src[3], double* 
     int count, double fac)
 {
     if (count > 0) dispatch_n<RunOp>(n, s, op, src, dst, slots, count, fac);
+}
+
+void block_norms(hipStream_t s, int n, const double* const c[3], int k, const int* slots, int count, double* out)
+{
+    if (k < 1 || k > 3) throw std::runtime_error("block norms of 1 to 3 components");
+    if (count > 0) dispatch_n<RunNorms>(n, s, c, k, slots, count, out);
 }
 
 void clean_faces(hipStream_t s, int n, double* f, const int* entries, int count, int width)

@@ -32,6 +32,9 @@ void block_op(hipStream_t s, int n, int op, const double* const src[3], double* 
 // entry {slot, mask}; bit b is neighbour slot {4,10,12,14,16,22}[b]
 // (z-, y-, x-, x+, y+, z+).
 void clean_faces(hipStream_t s, int n, double* f, const int* entries, int count, int width);
+// Per listed block: the largest sum of squares of the k <= 3 components over
+// its interior points (out[2b]) and the total of those sums (out[2b+1]).
+void block_norms(hipStream_t s, int n, const double* const c[3], int k, const int* slots, int count, double* out);
 
 // Operator::nonlinear (operators.hpp:2455-2500) on the listed blocks: d =
 // 0.25 (w x u) on the interior nodes from face u and edge w (one-cell

@@ -166,6 +166,7 @@ SYMBOLS = {
     "iblgf_gpu_chombo_write": (_I, [_P, ctypes.c_char_p, _I, ctypes.POINTER(ctypes.c_char_p), _IP, _IP, _I, _I,
                                     _D, _D]),
     "iblgf_gpu_chombo_read": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I, _IP]),
+    "iblgf_gpu_block_norms": (_I, [_P, _IP, _I, _DP, _DP]),
 }
 
 KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
@@ -529,6 +530,16 @@ class PoissonSolver:
         _check(lib().iblgf_gpu_chombo_write(self._h, os.fsencode(path), len(names), nm, ncomp.ctypes.data_as(_IP),
                                             ids.ctypes.data_as(_IP), int(bool(include_correction)),
                                             int(bool(base_correction_only)), float(time), float(dt)), self._h)
+
+    def block_norms(self, fields):
+        """Operator::maxnorm and blockRootMeanSquare of the 1-3 component field on every block:
+        (maxnorm[nblocks], rms[nblocks])."""
+        ids = np.ascontiguousarray(fields, np.int32)
+        nb = self.shape[0]
+        mx, rms = np.zeros(nb), np.zeros(nb)
+        _check(lib().iblgf_gpu_block_norms(self._h, ids.ctypes.data_as(_IP), len(ids), mx.ctypes.data_as(_DP),
+                                           rms.ctypes.data_as(_DP)), self._h)
+        return mx, rms
 
     def read_chombo(self, path, name, fields, mesh=0):
         """Simulation::read_h5 into device fields (one id per component)."""

@@ -658,3 +658,64 @@ class IbSphere:
         self.info = self.tree.info()
         self.arrays = self.tree.export()
         self.uc = np.tile(np.array([1.0, 0.0, 0.0]), (self.npoints, 1))
+
+
+def adapt_level_changes(arrays, base_level, maxnorm, source_max, nref, refinement_factor=0.125,
+                        base_threshold=1e-4, hard_max_level=None, ib_add_level=0):
+    """The adaptation criterion of the reference's IF-HERK vortex-ring setup
+    (NS_AMR_LGF::adapt_level_change and adapt_levle_change_for_field,
+    tests/ns_amr_lgf/ns_amr_lgf.hpp:402-495), from per-block maximum norms.
+
+    arrays: the tree's export; maxnorm: dict of per-slot Operator::maxnorm
+    arrays for "cell_aux", "correction_tmp" and "edge_aux" (e.g. from
+    PoissonSolver.block_norms); source_max: (cell_aux, edge_aux) maxima
+    (Ifherk::update_source_max).  Returns {key: level change} for
+    Tree.adapt: a leaf's own change, or for a correction block whose field
+    asks for refinement, +1 on its parent."""
+    hard_max = nref if hard_max_level is None else hard_max_level
+    deletion_factor = 0.7
+
+    def field_change(slot, fmax, smax, use_base):
+        rl = int(arrays["level"][slot]) - base_level
+        flags = int(arrays["flags"][slot])
+        is_ib = bool(flags & 16)
+        if is_ib and (flags & FLAG_LEAF):
+            return 1 if rl < nref + ib_add_level else 0
+        smax *= 1.05
+        if fmax < 1e-10:
+            return -1
+        l_aim = int(math.ceil(nref - math.log(fmax / smax) / math.log(refinement_factor)))
+        l_del = int(math.ceil(nref - (math.log(fmax / smax) - math.log(deletion_factor)) / math.log(refinement_factor)))
+        l_aim = min(l_aim, hard_max)
+        if rl == 0 and use_base:
+            if fmax > smax * base_threshold:
+                l_aim = max(l_aim, 0)
+            if fmax > smax * base_threshold * deletion_factor:
+                l_del = max(l_del, 0)
+        if is_ib:
+            return 0
+        if l_aim - rl > 0:
+            return 1
+        if l_del - rl < 0:
+            return -1
+        return 0
+
+    out = {}
+    for s in range(len(arrays["key"])):
+        flags = int(arrays["flags"][s])
+        leaf, corr = bool(flags & FLAG_LEAF), bool(flags & FLAG_CORRECTION)
+        if int(arrays["level"][s]) < base_level or (not leaf and not corr) or (leaf and corr):
+            continue
+        l1 = l2 = l3 = -1
+        if leaf:
+            l1 = field_change(s, maxnorm["cell_aux"][s], source_max[0], True)
+            l3 = field_change(s, maxnorm["edge_aux"][s], source_max[1], False)
+        else:
+            l2 = field_change(s, maxnorm["correction_tmp"][s], source_max[0], False)
+        l = max(l1, l2, l3)
+        if l != 0:
+            if leaf:
+                out[int(arrays["key"][s])] = l
+            elif l2 > 0:
+                out[int(arrays["key"][arrays["parent"][s]])] = l2
+    return out

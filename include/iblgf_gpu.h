@@ -470,6 +470,16 @@ int iblgf_gpu_tree_readapt(iblgf_gpu_ctx* ctx, const iblgf_tree* t, int nf, cons
  * (field c is field index c % 3) - what Ifherk::adapt does to u first. */
 int iblgf_gpu_coarsen(iblgf_gpu_ctx* ctx, const int* fields, int k, int mesh_type);
 
+/* Operator::maxnorm and Operator::blockRootMeanSquare (include/iblgf/
+ * operators/operators.hpp:1351-1367, 1149-1167) of the field with components
+ * fields[0..k) (1 <= k <= 3) on every block with data, in slot order: per
+ * interior point the Euclidean norm over the components; maxnorm[b] its
+ * largest value (the reference's bit for bit), rms[b] its root mean square.
+ * Either output may be NULL.  The inputs of the setups' adaptation criteria
+ * (tests/ns_amr_lgf/ns_amr_lgf.hpp:448-495, Ifherk::update_source_max,
+ * ifherk.hpp:401-420) without a download of the fields. */
+int iblgf_gpu_block_norms(iblgf_gpu_ctx* ctx, const int* fields, int k, double* maxnorm, double* rms);
+
 /* ---- IF-HERK time integration (Ifherk, ifherk.hpp) ------------------------
  * Velocity fields are face vectors (three fields, component c the faces
  * normal to axis c), vorticity edge vectors.  All asynchronous. */
