@@ -683,8 +683,8 @@ int iblgf_gpu_ib_precond_features(iblgf_gpu_ctx* ctx, const double* r, float* X)
 int iblgf_gpu_ib_precond_grad(iblgf_gpu_ctx* ctx, const float* X, const float* T, int rows, float* grads, double* loss);
 /* Flexible CG in conjugate-residual form (flexible GCR with 30 kept
  * directions, restarted; minimises |r| each step, so it converges although
- * the FMM-approximated operator is only symmetric to ~1e-4, where plain CG
- * stalls), preconditioned by the learned network when use_mlp is nonzero
+ * the FMM-approximated operator is only symmetric to ~1e-4, on which plain
+ * CG converges slowly), preconditioned by the learned network when use_mlp is nonzero
  * (the identity otherwise, and wherever the network's direction is not a
  * finite descent direction).  A learned solve is monitored: every 10
  * iterations its mean residual reduction is compared with 0.8 times that of
