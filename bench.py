@@ -669,7 +669,10 @@ def main():
         "config": {
             "workload": ("adaptive vortex-ring octree Poisson: 8^3 base blocks of 16^3 cells + 3 refinement levels, "
                          "16 seeded Gaussian-core rings (seed 1001); one step = 3 vorticity components, "
-                         "batched through every launch"),
+                         "batched through every launch; bounding box 256^3 base cells (twice the block region, "
+                         "as the reference's ring configurations), refined where |w| > source_max 0.9^diff_level "
+                         "(the reference's rule with refinement factor 0.9, which lands near 4096 blocks; "
+                         "the literal 1e-3 max threshold gives 81292 blocks)"),
             "headline_path": head_name or "fp64_batched",
             "precision": {
                 None: "plain FP64 (no reduced-precision path met relative L2 1e-6)",
