@@ -577,6 +577,13 @@ int iblgf_gpu_ib_stage_forces(iblgf_gpu_ctx* ctx, double* forces, int write)
     });
 }
 
+int iblgf_gpu_ib_set_boundary_velocity(iblgf_gpu_ctx* ctx, const double* velocity)
+{
+    NEED_CTX("iblgf_gpu_ib_set_boundary_velocity");
+    if (!ctx->ib) return fail(IBLGF_ERR_STATE, "iblgf_gpu_ib_set_boundary_velocity: no immersed-boundary system", ctx);
+    return guarded(ctx, [&] { ctx->ib->set_point_velocities(velocity); });
+}
+
 int iblgf_gpu_clean_boundary(iblgf_gpu_ctx* ctx, int field, int level, int kind, int leaf_only, int width)
 {
     NEED_CTX("iblgf_gpu_clean_boundary");

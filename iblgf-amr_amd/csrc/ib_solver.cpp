@@ -97,6 +97,20 @@ void IbSystem::set_step_params(const double* bc, double tol, int maxit)
     step_maxit_ = maxit;
 }
 
+// LinSysSolver::subtract_boundary_vel (linsys.hpp:148-158) with bc_vel(idx,
+// t, coordinate(i)) evaluated per point by the caller.
+void IbSystem::set_point_velocities(const double* host)
+{
+    if (!host)
+    {
+        d_bcpts_.release();
+        return;
+    }
+    std::vector<double> v(host, host + 9 * static_cast<size_t>(npoints_));
+    IBLGF_HIP_IB(hipStreamSynchronize(s_.stream()));
+    d_bcpts_.upload(v);
+}
+
 // LinSysSolver::ib_solve (linsys.hpp:113-129) inside the stage, with the
 // stage's previous force as the initial guess (ifherk.hpp:1477-1484).
 void IbSystem::force_solve(const int fields[3], double alpha, int stage, bool write_prev)
@@ -108,7 +122,10 @@ void IbSystem::force_solve(const int fields[3], double alpha, int stage, bool wr
     double*     prev = d_fprev_.p + (stage - 1) * n3;
     IBLGF_HIP_IB(hipMemcpyAsync(d_force_.p, prev, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
     project(fields, d_uc_.p);
-    launch::force_shift(st, d_uc_.p, npoints_, bc_ + 3 * (stage - 1));
+    if (d_bcpts_.p)  // uc - v, as the reference subtracts bc_vel point by point
+        launch::axpby(st, static_cast<long>(n3), -1.0, d_bcpts_.p + (stage - 1) * n3, 1.0, d_uc_.p, nullptr);
+    else
+        launch::force_shift(st, d_uc_.p, npoints_, bc_ + 3 * (stage - 1));
     for (int c = 0; c < 3; ++c) s_.domain_clean(fields[c]);
     stage_stats_[stage - 1] = cg(d_uc_.p, d_force_.p, alpha, step_tol_, step_maxit_);
     if (write_prev)

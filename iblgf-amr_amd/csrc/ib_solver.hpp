@@ -94,6 +94,9 @@ class IbSystem : public StepIb
     // CG stopping rule, and the stage forces kept across steps (the
     // reference's ib().force_prev(stage), zero at first).
     void set_step_params(const double* bc, double tol, int maxit);
+    // Per-point boundary velocities [3 stages][npoints][3] for the following
+    // steps (null: back to the constant bc of set_step_params).
+    void set_point_velocities(const double* host);
     void force_solve(const int fields[3], double alpha, int stage, bool write_prev) override;
     void smear_add(const int fields[3]) override;
     // the stage forces [3 stages][N][3] to or from the host
@@ -120,6 +123,7 @@ class IbSystem : public StepIb
     void recycle_push(const double* p, const double* q, double qq);
     void recycle_sync_qq();
     DevArray<double>    d_fprev_, d_force_, d_uc_;
+    DevArray<double>    d_bcpts_;  // per-point boundary velocities, empty when the constant bc_ holds
     void                init_stage_forces();
     double              bc_[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     double              step_tol_ = 1e-3;

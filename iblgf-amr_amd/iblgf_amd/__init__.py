@@ -167,6 +167,7 @@ SYMBOLS = {
                                     _D, _D]),
     "iblgf_gpu_chombo_read": (_I, [_P, ctypes.c_char_p, ctypes.c_char_p, _I, _I, _IP]),
     "iblgf_gpu_block_norms": (_I, [_P, _IP, _I, _DP, _DP]),
+    "iblgf_gpu_ib_set_boundary_velocity": (_I, [_P, _DP]),
 }
 
 KERNEL_CLASSES = ("forward_dft", "spectral_mac", "inverse_dft", "transfer", "other")
@@ -769,6 +770,17 @@ class IbSystem:
         _check(lib().iblgf_gpu_ifherk_step_ib(self.solver._h, _ip(u3), int(p), float(dt), float(Re), fr,
                                               int(refresh), b, float(cg_tol), int(cg_maxit), its), self.solver._h)
         return list(its)
+
+    def set_boundary_velocity(self, velocity):
+        """Per-point boundary velocities [3 stages, N, 3] for the following IF-HERK steps
+        (LinSysSolver::subtract_boundary_vel); None returns to the steps' constant bc."""
+        if velocity is None:
+            _check(lib().iblgf_gpu_ib_set_boundary_velocity(self.solver._h, None), self.solver._h)
+            return
+        a = np.ascontiguousarray(velocity, np.float64).reshape(-1)
+        if a.size != 9 * self.npoints:
+            raise ValueError("boundary velocities must hold 3 x N x 3 values")
+        _check(lib().iblgf_gpu_ib_set_boundary_velocity(self.solver._h, a.ctypes.data_as(_DP)), self.solver._h)
 
     def stage_forces(self, value=None):
         """The IF-HERK stage forces [3, N, 3] (set them when value is given)."""

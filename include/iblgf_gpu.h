@@ -547,6 +547,14 @@ int iblgf_gpu_ifherk_step_ib(iblgf_gpu_ctx* ctx, const int u_fields[3], int p_fi
 /* Reads (write 0) or sets (write 1) the stage forces of the IF-HERK step,
  * forces[3 stages][npoints][3]. */
 int iblgf_gpu_ib_stage_forces(iblgf_gpu_ctx* ctx, double* forces, int write);
+/* Boundary velocity per point and stage for the following
+ * iblgf_gpu_ifherk_step_ib calls, velocity[3 stages][npoints][3]: the
+ * reference's LinSysSolver::subtract_boundary_vel (linsys.hpp:148-158)
+ * subtracts bc_vel(idx, t, coordinate(i)) point by point, a function of the
+ * stage's time and the point's position (moving or deforming bodies); the
+ * caller evaluates it.  It replaces the step's constant bc_vel until called
+ * with NULL. */
+int iblgf_gpu_ib_set_boundary_velocity(iblgf_gpu_ctx* ctx, const double* velocity);
 
 /* PoissonSolver::apply_lgf / apply_lgf_IF with an immersed-boundary FMM type
  * (poisson.hpp:587-894, 435-494): mask_type IBLGF_MASK_IB2xIB or

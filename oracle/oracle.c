@@ -1956,6 +1956,7 @@ typedef struct hk_state
     double *         fa2[3], *ca2, *force, *uc;
     double*          force_prev; /* [3 stages][3N] */
     const double*    bc;         /* [3 stages][3] */
+    const double*    bc_pts;     /* [3 stages][N][3] or NULL */
     double           cg_tol;
     int              cg_maxit;
 } hk_state;
@@ -2123,7 +2124,9 @@ static void hk_ib_solve(const hk_state* h, double alpha, int stage, int write_pr
     memcpy(h->force, h->force_prev + (size_t)3 * N * (stage - 1), 3 * (size_t)N * sizeof(double));
     oracle_ib_project(t, h->ib, h->fa2[0], h->fa2[1], h->fa2[2], h->uc);
     for (i = 0; i < N; ++i)
-        for (c = 0; c < 3; ++c) h->uc[3 * i + c] -= h->bc[3 * (stage - 1) + c];
+        for (c = 0; c < 3; ++c)
+            h->uc[3 * i + c] -= h->bc_pts ? h->bc_pts[(size_t)3 * N * (stage - 1) + 3 * i + c]
+                                          : h->bc[3 * (stage - 1) + c];
     for (c = 0; c < 3; ++c) hk_clean(t, h->fa2[c], 0);
     oracle_ib_cg(t, h->ib, h->g, h->nmax, h->uc, h->force, alpha, h->cg_tol, h->cg_maxit, NULL);
     if (write_prev) memcpy(h->force_prev + (size_t)3 * N * (stage - 1), h->force, 3 * (size_t)N * sizeof(double));
@@ -2239,12 +2242,12 @@ static void hk_adapt_corr(const hk_state* h, double* const u[3])
 
 static int hk_step(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0, double* u1,
     double* u2, double* p, double dt, double Re, const double* frame, int refresh, const double* bc,
-    double* force_prev, double cg_tol, int cg_maxit);
+    const double* bc_pts, double* force_prev, double cg_tol, int cg_maxit);
 
 int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* u0, double* u1, double* u2,
     double* p, double dt, double Re, const double* frame, int refresh)
 {
-    return hk_step(t, NULL, g, nmax, u0, u1, u2, p, dt, Re, frame, refresh, NULL, NULL, 0.0, 0);
+    return hk_step(t, NULL, g, nmax, u0, u1, u2, p, dt, Re, frame, refresh, NULL, NULL, NULL, 0.0, 0);
 }
 
 int oracle_ifherk_step_ib(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0,
@@ -2252,12 +2255,21 @@ int oracle_ifherk_step_ib(const oracle_tree* t, const oracle_ib* ib, const doubl
     double* force_prev, double cg_tol, int cg_maxit)
 {
     if (!ib || !force_prev) return 1;
-    return hk_step(t, ib, g, nmax, u0, u1, u2, p, dt, Re, frame, refresh, bc, force_prev, cg_tol, cg_maxit);
+    return hk_step(t, ib, g, nmax, u0, u1, u2, p, dt, Re, frame, refresh, bc, NULL, force_prev, cg_tol, cg_maxit);
+}
+
+int oracle_ifherk_step_ib_points(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0,
+    double* u1, double* u2, double* p, double dt, double Re, const double* frame, int refresh,
+    const double* bc_points, double* force_prev, double cg_tol, int cg_maxit)
+{
+    if (!ib || !force_prev || !bc_points) return 1;
+    return hk_step(t, ib, g, nmax, u0, u1, u2, p, dt, Re, frame, refresh, NULL, bc_points, force_prev, cg_tol,
+        cg_maxit);
 }
 
 static int hk_step(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0, double* u1,
     double* u2, double* p, double dt, double Re, const double* frame, int refresh, const double* bc,
-    double* force_prev, double cg_tol, int cg_maxit)
+    const double* bc_pts, double* force_prev, double cg_tol, int cg_maxit)
 {
     static const double a_[6] = {1.0 / 3, -1.0, 2.0, 0.0, 0.75, 0.25};
     static const double c_[4] = {0.0, 1.0 / 3, 1.0, 1.0};
@@ -2284,6 +2296,7 @@ static int hk_step(const oracle_tree* t, const oracle_ib* ib, const double* g, i
         h.uc = (double*)calloc(3 * (size_t)ib->npoints, sizeof(double));
         h.force_prev = force_prev;
         h.bc = bc ? bc : zero;
+        h.bc_pts = bc_pts;
         h.cg_tol = cg_tol;
         h.cg_maxit = cg_maxit;
     }

@@ -168,6 +168,12 @@ int oracle_ifherk_step(const oracle_tree* t, const double* g, int nmax, double* 
 int oracle_ifherk_step_ib(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0,
     double* u1, double* u2, double* p, double dt, double Re, const double* frame, int refresh, const double* bc,
     double* force_prev, double cg_tol, int cg_maxit);
+/* The same step with a boundary velocity per point and stage, bc_points
+ * [3 stages][N][3]: LinSysSolver::subtract_boundary_vel (linsys.hpp:148-158)
+ * with bc_vel(idx, t, coordinate(i)) evaluated by the caller. */
+int oracle_ifherk_step_ib_points(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, double* u0,
+    double* u1, double* u2, double* p, double dt, double Re, const double* frame, int refresh,
+    const double* bc_points, double* force_prev, double cg_tol, int cg_maxit);
 double oracle_delta_1d(int kind, double x);
 /* LinSysSolver::smearing / projection (linsys.hpp:350-445) with Operator::ib_smearing /
  * ib_projection (operators.hpp:996-1015, 844-861) over the influence lists of

@@ -70,6 +70,7 @@ def lib():
         L.oracle_nonlinear.argtypes = [T] + [_P] * 9 + [_I, _D]
         L.oracle_up_and_down.argtypes = [T, _P, _P, _P, _I]
         L.oracle_ifherk_step_ib.argtypes = [T, B, _P, _I, _P, _P, _P, _P, _D, _D, _P, _I, _P, _P, _D, _I]
+        L.oracle_ifherk_step_ib_points.argtypes = [T, B, _P, _I, _P, _P, _P, _P, _D, _D, _P, _I, _P, _P, _D, _I]
         L.oracle_ifherk_step.argtypes = [T, _P, _I, _P, _P, _P, _P, _D, _D, _P, _I]
         L.oracle_delta_1d.argtypes = [_I, _D]
         L.oracle_delta_1d.restype = _D
@@ -350,14 +351,24 @@ class Ib:
         return out
 
     def ifherk_step(self, tree, u, dt, Re, force_prev, frame=None, refresh=0, bc=None, cg_tol=1e-3, cg_maxit=100,
-                    nmax=100):
+                    nmax=100, bc_points=None):
         """One IF-HERK step with these IB points; force_prev [3 stages, N, 3]
-        is updated in place; returns (u_new, p)."""
+        is updated in place; bc [3 stages, 3] (constant) or bc_points
+        [3 stages, N, 3] (per point) is the boundary velocity; returns (u_new, p)."""
         u = [np.array(x, np.float64, order="C", copy=True) for x in u]
         p = np.zeros_like(u[0])
         assert force_prev.dtype == np.float64 and force_prev.flags["C_CONTIGUOUS"]
         assert force_prev.size == 9 * self.x.shape[0]
         fr = None if frame is None else np.ascontiguousarray(np.broadcast_to(np.asarray(frame, np.float64), (3, 3)))
+        if bc_points is not None:
+            bp = np.ascontiguousarray(bc_points, np.float64).reshape(3, -1, 3)
+            assert bp.shape[1] == self.x.shape[0]
+            rc = lib().oracle_ifherk_step_ib_points(ctypes.byref(tree.s), ctypes.byref(self.s), _p(lgf_table(nmax)),
+                                                    nmax, *[_p(x) for x in u], _p(p), float(dt), float(Re),
+                                                    None if fr is None else _p(fr), int(refresh), _p(bp),
+                                                    _p(force_prev), float(cg_tol), int(cg_maxit))
+            assert rc == 0
+            return u, p
         b = None if bc is None else np.ascontiguousarray(np.broadcast_to(np.asarray(bc, np.float64), (3, 3)))
         rc = lib().oracle_ifherk_step_ib(ctypes.byref(tree.s), ctypes.byref(self.s), _p(lgf_table(nmax)), nmax,
                                          *[_p(x) for x in u], _p(p), float(dt), float(Re),

@@ -208,6 +208,58 @@ def test_ib_step_matches_oracle_and_enforces_the_boundary_velocity(oracle_mod):
     assert lib().iblgf_gpu_ifherk_step_ib(s2._h, a, -1, dt, 200.0, None, 0, None, 1e-6, 10, None) == IBLGF_ERR_STATE
 
 
+def test_ib_step_with_per_point_boundary_velocities(oracle_mod):
+    """iblgf_gpu_ib_set_boundary_velocity (subtract_boundary_vel point by
+    point, linsys.hpp:148-158) for a translating, spinning body: the device
+    step equals the oracle's (velocity relative 1e-7, stage forces 1e-6), the
+    velocity at the owned points is the last stage's prescribed velocity, and
+    the same velocity at every point gives the constant-bc step bit for bit."""
+    from iblgf_amd import IbSystem
+
+    print("case: IF-HERK step with 100 IB points, per-point boundary velocities of a spinning body", flush=True)
+    sph = setups.IbSphere(npoints=100, block_extent=8, nref=1, half_width=0.5)
+    flow = setups.GaussianFlow(sph.tree, [((0.1, 0.0, 0.0), 0.3, (0, 0, 1), 0.5)])
+    u0 = flow.velocity()
+    dt = 0.1 * sph.dx_base / max(np.abs(x).max() for x in u0)
+    vp = []
+    for t in (dt / 3, dt, dt):
+        om = 0.8 + 0.3 * t
+        v = np.zeros_like(sph.points)
+        v[:, 0], v[:, 1], v[:, 2] = 0.1 - om * sph.points[:, 1], -0.05 + om * sph.points[:, 0], 0.02
+        vp.append(v)
+    vp = np.stack(vp)
+    ot = oracle_mod.Tree(sph.arrays, sph.info, sph.tree.n)
+    oib = oracle_mod.Ib(sph.points, sph.nref, sph.delta, 4.5, sph.safety_dis)
+    fp = np.zeros((3, sph.npoints, 3))
+    ref_u, _ = oib.ifherk_step(ot, u0, dt, 200.0, fp, cg_tol=1e-10, cg_maxit=400, bc_points=vp)
+    s = PoissonSolver(sph.tree)
+    ib = IbSystem(s)
+    fu = [s.field() for _ in range(3)]
+    fpr = s.field()
+    for f, x in zip(fu, u0):
+        s.upload(f, x)
+    ib.set_boundary_velocity(vp)
+    ib.ifherk_step(fu, dt, 200.0, p=fpr, cg_tol=1e-10, cg_maxit=400)
+    leaves = (sph.arrays["flags"] & FLAG_LEAF) != 0
+    eu = _rel([s.download(f)[leaves] for f in fu], [x[leaves] for x in ref_u])
+    ef = np.linalg.norm(ib.stage_forces() - fp) / np.linalg.norm(fp)
+    own = oib.owned(ot).astype(bool)
+    slip = np.abs(ib.project(fu)[own] - vp[2][own]).max()
+    print(f"velocity relL2 {eu:.2e}, stage forces relL2 {ef:.2e}, slip {slip:.2e}", flush=True)
+    assert eu < 1e-7 and ef < 1e-6 and slip < 1e-8
+    # the same velocity at every point is the constant-bc step
+    bc = np.array([[0.2, -0.1, 0.05], [0.1, 0.0, -0.2], [0.3, 0.1, 0.0]])
+    outs = []
+    for per_point in (False, True):
+        for f, x in zip(fu, u0):
+            s.upload(f, x)
+        ib.stage_forces(np.zeros((3, sph.npoints, 3)))
+        ib.set_boundary_velocity(np.repeat(bc[:, None, :], sph.npoints, axis=1) if per_point else None)
+        ib.ifherk_step(fu, dt, 200.0, p=fpr, bc=None if per_point else bc, cg_tol=1e-10, cg_maxit=400)
+        outs.append([s.download(f) for f in fu] + [s.download(fpr), ib.stage_forces()])
+    assert all(np.array_equal(a, b) for a, b in zip(*outs))
+
+
 @pytest.mark.parametrize("config", [0, 1])
 def test_device_ifherk_matches_the_reference_vortex_ring_run(oracle_mod, config):
     """The reference's known answer (tests/ns_amr_lgf with ref.hdf5, fixture
