@@ -902,8 +902,7 @@ template<int N, int KF, bool COMPACT, class T>
 __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup* __restrict__ groups,
     int ngroups, const MacSrc* __restrict__ srcs, const uint4* __restrict__ rowoff, const int* __restrict__ olist,
     int nused, const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
-    typename Cx<T>::type* __restrict__ tspec, int k, long scs, long tcs, int* __restrict__ queue, int upc,
-    int tail)
+    typename Cx<T>::type* __restrict__ tspec, int k, long scs, long tcs)
 {
     using C = typename Cx<T>::type;
     static_assert(KF * sizeof(C) == kMacRowBytes, "row offsets assume 128-byte kernel rows");
@@ -919,6 +918,89 @@ __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup*
     // its L2.
     const int nwg = gridDim.x, xcd = blockIdx.x & 7, q = nwg >> 3, rem = nwg & 7;
     const int w = xcd * q + min(xcd, rem) + (int)(blockIdx.x >> 3);
+    const int nvg = ngroups * k;
+    // Whole chunks first, chunk r W + w in round r (the workgroups of a round
+    // walk the same groups together); the T = NCHUNK mod W chunks left over
+    // are split as units, workgroup w taking units [Ut w / W, Ut (w+1) / W)
+    // of their Ut = T nvg.
+    const int  R = NCHUNK / nwg, T0 = R * nwg;
+    const long Ut = (long)nvg * (NCHUNK - T0);
+    const long ua = Ut * w / nwg, ub = Ut * (w + 1) / nwg;
+    const int  ca = T0 + (int)(ua / nvg), cb = ub > ua ? T0 + (int)((ub - 1) / nvg) : ca - 1;
+    const int  va = (int)(ua - (long)(ca - T0) * nvg), vb = (int)(ub - (long)(cb - T0) * nvg);
+    const int  nseg = R + (cb - ca + 1);
+    const int  rows = COMPACT ? nused : 343;
+    const int  f = threadIdx.x % KF;
+    for (int sg = 0; sg < nseg; ++sg)
+    {
+        const bool tail = sg >= R;
+        const int  chunk = tail ? ca + (sg - R) : sg * nwg + w;
+        const int  vlo = tail && chunk == ca ? va : 0, vhi = tail && chunk == cb ? vb : nvg;
+        const int  f0 = chunk * KF;
+        __syncthreads();  // the previous chunk's rows are no longer read
+        for (int i = threadIdx.x; i < (rows + 1) * KF; i += nt)
+        {
+            const int c = i / KF, fi = i - c * KF;
+            const int o = COMPACT ? olist[min(c, nused - 1)] : min(c, 342);
+            Gs[i] = (c < rows && f0 + fi < NF) ? ghat[(size_t)o * NF + f0 + fi] : Cx<T>::make(0, 0);
+        }
+        __syncthreads();
+        const int fg = min(f0 + f, NF - 1);  // tail lanes compute a valid frequency, never store
+        // The workgroup walks the (Morton-ordered) groups in tiles, so a
+        // source shared by neighbouring groups is re-read while its line is
+        // still in L2.
+        for (int vg = vlo + threadIdx.x / KF; vg < vhi; vg += GPW)
+        {
+            const int      g = vg / k, c = vg - g * k;
+            const C*       sp = sspec + c * scs + fg;
+            C*             tp = tspec + c * tcs + fg;
+            const MacGroup* gp = groups + g;
+            MacAcc<C>       acc;
+            acc.zero();
+            const int e0 = gp->begin, e2 = e0 + gp->count;
+            int       e1 = e0;
+            if constexpr (!COMPACT)
+            {
+                e1 = e0 + gp->nfull;
+                mac_range<true, NF, KF, C>(srcs, rowoff, sp, Gs, f, e0, e1, acc);
+            }
+            mac_range<false, NF, KF, C>(srcs, rowoff, sp, Gs, f, e1, e2, acc);
+            if (f0 + f < NF)
+            {
+                // target slots read only now: not held in registers over the entries
+                int ts[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) ts[t] = gp->tslot[t];
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    if (ts[t] >= 0)
+                    {
+                        const double2 v = acc.get(t);
+                        tp[(size_t)ts[t] * NF] = Cx<T>::make((T)v.x, (T)v.y);
+                    }
+            }
+        }
+    }
+}
+
+// The same launch as a persistent grid claiming work units from a device
+// work queue (the autotuner's mac_policy 2): a kernel of its own, so the
+// static grid's code is not changed by it.
+template<int N, int KF, bool COMPACT, class T>
+__global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac_queue(const MacGroup* __restrict__ groups,
+    int ngroups, const MacSrc* __restrict__ srcs, const uint4* __restrict__ rowoff, const int* __restrict__ olist,
+    int nused, const typename Cx<T>::type* __restrict__ ghat, const typename Cx<T>::type* __restrict__ sspec,
+    typename Cx<T>::type* __restrict__ tspec, int k, long scs, long tcs, int* __restrict__ queue, int upc,
+    int tail)
+{
+    using C = typename Cx<T>::type;
+    static_assert(KF * sizeof(C) == kMacRowBytes, "row offsets assume 128-byte kernel rows");
+    constexpr int NF = Geo<N>::NF;
+    constexpr int NCHUNK = (NF + KF - 1) / KF;
+    const int     nt = blockDim.x;  // a multiple of KF, sized to the launch's group count
+    const int     GPW = nt / KF;
+    __shared__ C  Gs[344 * KF];  // the chunk's rows, then one all-zero row
+    const int nwg = gridDim.x, xcd = blockIdx.x & 7;  // workgroup i runs on XCD i mod 8
     const int nvg = ngroups * k;
     const int rows = COMPACT ? nused : 343;
     const int f = threadIdx.x % KF;
@@ -970,80 +1052,56 @@ __global__ __launch_bounds__(kMacMaxThreads) void k_spectral_mac(const MacGroup*
             }
         }
     };
-    if (queue)
-    {
-        // Work queue: the launch's units are claimed one at a time from a
-        // device counter, so a workgroup that drew cheap units (groups with
-        // short source lists, or CUs left free by the other stream's
-        // kernels) takes more of them.  The first NCHUNK - tail units are
-        // whole chunks (all of the launch's (group, component) pairs); each
-        // of the last `tail` chunks is split into upc runs of
-        // ceil(nvg / upc) consecutive pairs, so the launch ends on short
-        // units.  The units are dealt to the 8 XCDs as 8 contiguous ranges,
-        // one counter each (queue[0..7]): a workgroup claims from its own
-        // XCD's range, neighbouring chunks staying in one L2, and then helps
-        // the other XCDs in turn.  A workgroup restages LDS only when its
-        // next unit is in another chunk.  The last workgroup to find every
-        // range empty (counter queue[8]) resets the counters for the next
-        // launch.
-        __shared__ int unit;
-        const int      ug = (nvg + upc - 1) / upc, nwhole = NCHUNK - tail, nunits = nwhole + tail * upc;
-        int            staged = -1, r = 0;
-        auto           claim = [&]() {  // thread 0 only
-            for (; r < 8; ++r)
-            {
-                const int x = (xcd + r) & 7;
-                const int lo = (int)((long)nunits * x / 8), hi = (int)((long)nunits * (x + 1) / 8);
-                const int c = lo + atomicAdd(queue + x, 1);
-                if (c < hi) return c;
-            }
-            return nunits;
-        };
-        for (;;)
+    // Work queue: the launch's units are claimed one at a time from a
+    // device counter, so a workgroup that drew cheap units (groups with
+    // short source lists, or CUs left free by the other stream's
+    // kernels) takes more of them.  The first NCHUNK - tail units are
+    // whole chunks (all of the launch's (group, component) pairs); each
+    // of the last `tail` chunks is split into upc runs of
+    // ceil(nvg / upc) consecutive pairs, so the launch ends on short
+    // units.  The units are dealt to the 8 XCDs as 8 contiguous ranges,
+    // one counter each (queue[0..7]): a workgroup claims from its own
+    // XCD's range, neighbouring chunks staying in one L2, and then helps
+    // the other XCDs in turn.  A workgroup restages LDS only when its
+    // next unit is in another chunk.  The last workgroup to find every
+    // range empty (counter queue[8]) resets the counters for the next
+    // launch.
+    __shared__ int unit;
+    const int      ug = (nvg + upc - 1) / upc, nwhole = NCHUNK - tail, nunits = nwhole + tail * upc;
+    int            staged = -1, r = 0;
+    auto           claim = [&]() {  // thread 0 only
+        for (; r < 8; ++r)
         {
-            if (threadIdx.x == 0) unit = claim();
-            __syncthreads();
-            const int u = unit;
-            __syncthreads();  // every thread holds u before thread 0 claims again
-            if (u >= nunits) break;
-            const int v = u - nwhole;
-            const int chunk = v < 0 ? u : nwhole + v / upc;
-            const int vlo = v < 0 ? 0 : (v % upc) * ug, vhi = v < 0 ? nvg : min(nvg, vlo + ug);
-            if (chunk != staged)
-            {
-                stage(chunk);
-                __syncthreads();
-                staged = chunk;
-            }
-            walk(chunk, vlo, vhi);
+            const int x = (xcd + r) & 7;
+            const int lo = (int)((long)nunits * x / 8), hi = (int)((long)nunits * (x + 1) / 8);
+            const int c = lo + atomicAdd(queue + x, 1);
+            if (c < hi) return c;
         }
-        if (threadIdx.x == 0)
-        {
-            __threadfence();
-            if (atomicAdd(queue + 8, 1) == nwg - 1)
-                for (int x = 0; x < 9; ++x) atomicExch(queue + x, 0);
-        }
-        return;
-    }
-    // Whole chunks first, chunk r W + w in round r (the workgroups of a round
-    // walk the same groups together); the T = NCHUNK mod W chunks left over
-    // are split as units, workgroup w taking units [Ut w / W, Ut (w+1) / W)
-    // of their Ut = T nvg.
-    const int  R = NCHUNK / nwg, T0 = R * nwg;
-    const long Ut = (long)nvg * (NCHUNK - T0);
-    const long ua = Ut * w / nwg, ub = Ut * (w + 1) / nwg;
-    const int  ca = T0 + (int)(ua / nvg), cb = ub > ua ? T0 + (int)((ub - 1) / nvg) : ca - 1;
-    const int  va = (int)(ua - (long)(ca - T0) * nvg), vb = (int)(ub - (long)(cb - T0) * nvg);
-    const int  nseg = R + (cb - ca + 1);
-    for (int sg = 0; sg < nseg; ++sg)
+        return nunits;
+    };
+    for (;;)
     {
-        const bool tail = sg >= R;
-        const int  chunk = tail ? ca + (sg - R) : sg * nwg + w;
-        const int  vlo = tail && chunk == ca ? va : 0, vhi = tail && chunk == cb ? vb : nvg;
-        __syncthreads();  // the previous chunk's rows are no longer read
-        stage(chunk);
+        if (threadIdx.x == 0) unit = claim();
         __syncthreads();
+        const int u = unit;
+        __syncthreads();  // every thread holds u before thread 0 claims again
+        if (u >= nunits) break;
+        const int v = u - nwhole;
+        const int chunk = v < 0 ? u : nwhole + v / upc;
+        const int vlo = v < 0 ? 0 : (v % upc) * ug, vhi = v < 0 ? nvg : min(nvg, vlo + ug);
+        if (chunk != staged)
+        {
+            stage(chunk);
+            __syncthreads();
+            staged = chunk;
+        }
         walk(chunk, vlo, vhi);
+    }
+    if (threadIdx.x == 0)
+    {
+        __threadfence();
+        if (atomicAdd(queue + 8, 1) == nwg - 1)
+            for (int x = 0; x < 9; ++x) atomicExch(queue + x, 0);
     }
 }
 
@@ -1374,11 +1432,20 @@ template<int N> struct RunMac
         const int nt = mac_threads((long)ng * b.k, KF);                                                          \
         constexpr int nch = (Geo<N>::NF + KF - 1) / KF;                                                          \
         const int upc = mac_units_per_chunk((long)ng * b.k, nt, KF);                                             \
-        const int nwg = queue ? resident_grid(k_spectral_mac<N, KF, CP, T>, nt, (long)nch * upc)                  \
-                              : mac_grid(k_spectral_mac<N, KF, CP, T>, nt, nch, b.mac_policy);                   \
-        const int tail = nwg < nch ? nwg : nch;                                                                  \
-        hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3(nwg), dim3(nt), 0, s, g, ng, src, rowoff, olist,  \
-            nused, (const C*)ghat, (const C*)ss, (C*)ts, b.k, b.sstride, b.sstride, queue, upc, tail);           \
+        if (queue)                                                                                               \
+        {                                                                                                        \
+            const int nwg = resident_grid(k_spectral_mac_queue<N, KF, CP, T>, nt, (long)nch * upc);              \
+            const int tail = nwg < nch ? nwg : nch;                                                              \
+            hipLaunchKernelGGL((k_spectral_mac_queue<N, KF, CP, T>), dim3(nwg), dim3(nt), 0, s, g, ng, src,      \
+                rowoff, olist, nused, (const C*)ghat, (const C*)ss, (C*)ts, b.k, b.sstride, b.sstride, queue,    \
+                upc, tail);                                                                                      \
+        }                                                                                                        \
+        else                                                                                                     \
+        {                                                                                                        \
+            const int nwg = mac_grid(k_spectral_mac<N, KF, CP, T>, nt, nch, b.mac_policy);                       \
+            hipLaunchKernelGGL((k_spectral_mac<N, KF, CP, T>), dim3(nwg), dim3(nt), 0, s, g, ng, src, rowoff,    \
+                olist, nused, (const C*)ghat, (const C*)ss, (C*)ts, b.k, b.sstride, b.sstride);                  \
+        }                                                                                                        \
     } while (0)
         if (prec == 0)
         {
