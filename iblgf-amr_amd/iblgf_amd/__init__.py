@@ -140,6 +140,8 @@ SYMBOLS = {
     "iblgf_gpu_set_launch_shapes": (_I, [_P, _I, _I]),
     "iblgf_gpu_forest_slots": (_I, [_P, _I, _IP]),
     "iblgf_gpu_apply_lgf_noalloc": (_I, [_P, _I, _I, _I, _D]),
+    "iblgf_gpu_helm2d_table": (_I, [_DP, _I, _I, _I, _P, _P]),
+    "iblgf_gpu_helm2d_convolve": (_I, [_P, _I, _I, _I, _P, _P, _D, _P]),
     "iblgf_gpu_ib_create": (_I, [_P, _P]),
     "iblgf_gpu_ib_info": (_I, [_P, _IP, _IP, ctypes.POINTER(ctypes.c_long)]),
     "iblgf_gpu_ib_smear": (_I, [_P, _DP, _IP]),
@@ -905,3 +907,39 @@ def lagrange_matrix(n, pts_cap, relative_pos):
     A = np.zeros((n, 2 * n), np.float64)
     _check(lib().iblgf_lagrange_matrix(int(n), int(pts_cap), int(relative_pos), A.ctypes.data_as(_DP)))
     return A
+
+
+# ---------------------------------------------------------------------------
+# Screened 2-D LGFs of the spanwise Fourier modes (device arrays from torch)
+# ---------------------------------------------------------------------------
+def _device_f64(t, numel, what):
+    if not (t.is_cuda and t.dtype.is_floating_point and t.element_size() == 8 and t.is_contiguous()):
+        raise ValueError(f"{what}: a contiguous float64 tensor on the GPU")
+    if t.numel() < numel:
+        raise ValueError(f"{what}: needs {numel} values, has {t.numel()}")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def helm2d_table(c, nx, ny, table):
+    """Fill table (torch float64 [nmodes, ny, nx] on the GPU) with the screened
+    2-D LGF values of screenings c (one per Fourier mode) on torch's current
+    stream (Helmholtz::get for Dim = 2, include/iblgf/lgf/helmholtz.hpp:143-153)."""
+    c = np.ascontiguousarray(np.atleast_1d(c), np.float64)
+    tp = _device_f64(table, c.size * nx * ny, "helm2d_table")
+    _check(lib().iblgf_gpu_helm2d_table(c.ctypes.data_as(_DP), int(c.size), int(nx), int(ny), tp, _stream()))
+
+
+def helm2d_convolve(table, src, dst, scale=1.0):
+    """dst = scale * (free-space convolution of src with table) per mode, all
+    torch float64 [nmodes, ny, nx] on the GPU, on torch's current stream."""
+    nmodes, ny, nx = src.shape
+    n = nmodes * ny * nx
+    _check(lib().iblgf_gpu_helm2d_convolve(_device_f64(table, n, "table"), int(nmodes), int(nx), int(ny),
+                                           _device_f64(src, n, "src"), _device_f64(dst, n, "dst"), float(scale),
+                                           _stream()))

@@ -702,6 +702,32 @@ int iblgf_gpu_ib_precond_grad(iblgf_gpu_ctx* ctx, const float* X, const float* T
 int iblgf_gpu_ib_fcg(iblgf_gpu_ctx* ctx, const double* uc, double* f, double alpha, double tol, int maxit,
     int use_mlp, int* iterations, double* residual, double* history);
 
+/* ---- Screened 2-D LGFs of the spanwise Fourier modes ------------------------
+ * The kernels of the reference's 2-D + spanwise-Fourier solvers: per Fourier
+ * mode the lattice Green's function of (Lap_h - c^2) on a 2-D grid, c = k dx.
+ * Both calls take device pointers from the caller, run on `stream` (NULL: the
+ * null stream), allocate nothing and return before the work ends. */
+
+/* Helmholtz::get / Helmholtz_Lookup::get for Dim = 2 (include/iblgf/lgf/
+ * helmholtz.hpp:143-153, helmholtz_lookup.hpp:60-68, 146-181): table (nmodes
+ * x ny x nx doubles, x fastest) receives, for mode k with screening c[k] (host
+ * array, 1e-4 <= c[k]) and 0 <= i < nx, 0 <= j < ny, the value at lattice
+ * offset (i, j), which equals -G with (-Lap_h + c^2) G = delta.  The trapezoid
+ * rule on the integral's half period uses enough nodes for 1e-19 relative
+ * error (about nx + ny + 45 / c).  1 <= nmodes <= 256, nx * ny <= 65536. */
+int iblgf_gpu_helm2d_table(const double* c, int nmodes, int nx, int ny, double* table, void* stream);
+
+/* The free-space convolution of each mode's source over a uniform nx x ny
+ * patch with that mode's table (Convolution<2> over every source/target
+ * block pair of one level, include/iblgf/utilities/convolution.hpp:259-260
+ * with the Helmholtz kernel):
+ *   dst[k][t] = scale * sum_s table[k][|tj - sj|][|ti - si|] * src[k][s],
+ * summed directly over every cell s of the patch (so dst satisfies
+ * (Lap_h - c^2) dst = scale * src at interior cells).  src and dst (nmodes x
+ * ny x nx doubles) must differ; same limits as iblgf_gpu_helm2d_table. */
+int iblgf_gpu_helm2d_convolve(
+    const double* table, int nmodes, int nx, int ny, const double* src, double* dst, double scale, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,8 @@
  *   cell_center_nli          include/iblgf/interpolation/cell_center_nli_intrp.hpp:171-238, 310-420, 557-787
  *   halo exchange            include/iblgf/domain/mpi/haloCommunicator.hpp:67-128
  *   PoissonSolver            include/iblgf/solver/poisson/poisson.hpp:427-494, 587-894, 896-1194
+ *   2-D Helmholtz LGF        include/iblgf/lgf/helmholtz.hpp:143-153,
+ *                            include/iblgf/lgf/helmholtz_lookup.hpp:60-68, 146-181
  */
 #define _POSIX_C_SOURCE 199309L
 #include "oracle.h"
@@ -2394,4 +2396,60 @@ static int hk_step(const oracle_tree* t, const oracle_ib* ib, const double* g, i
     free(h.force);
     free(h.uc);
     return 0;
+}
+
+/* ---- Screened 2-D LGF of one spanwise Fourier mode --------------------------
+ * Helmholtz::get for Dim = 2 (include/iblgf/lgf/helmholtz.hpp:143-153): the
+ * trapezoid integral of (1 - cos(m x) K^-n) / (K - 1/K) / (2 pi) over a period
+ * (helmholtz_lookup.hpp:60-68, 170-181) minus origin_val
+ * (helmholtz_lookup.hpp:114-118), n >= m the larger offset.  The first term's
+ * integral is origin_val itself, so the value is -G(n, m) with
+ *   G(n, m) = 1/pi int_0^pi cos(m x) K^-n / sqrt(a^2 - 4) dx,
+ * a = 4 + c^2 - 2 cos x, K = (a + sqrt(a^2 - 4)) / 2, evaluated here by the
+ * trapezoid rule with `nodes` intervals on the half period (the integrand is
+ * even).  Pinned in tests against origin_val's hypergeometric series, an
+ * adaptive quadrature of the same integral and the lattice identity
+ * (Lap_h - c^2) value = delta. */
+double oracle_helm2d_lgf(double c, int i, int j, int nodes)
+{
+    int n = abs(i), m = abs(j);
+    if (n < m)
+    {
+        int t = n;
+        n = m;
+        m = t;
+    }
+    const double h = O_PI / nodes;
+    double acc = 0.0;
+    for (int p = 0; p <= nodes; ++p)
+    {
+        const double x = p * h;
+        const double sh = sin(0.5 * x);
+        const double e = c * c + 4.0 * sh * sh;
+        const double root = sqrt(e * (e + 4.0));
+        double f = cos(m * x) * exp(-n * log1p(0.5 * (e + root))) / root;
+        if (p == 0 || p == nodes) f *= 0.5;
+        acc += f;
+    }
+    return -acc / nodes;
+}
+
+/* Convolution<2> of one mode over a uniform nx x ny patch
+ * (include/iblgf/utilities/convolution.hpp:259-260 summed over every block
+ * pair of one level): dst[t] = scale * sum_s table[|tj-sj|][|ti-si|] src[s],
+ * sources in index order. */
+void oracle_helm2d_convolve(const double* table, int nx, int ny, const double* src, double* dst, double scale)
+{
+    const int per = nx * ny;
+    for (int t = 0; t < per; ++t)
+    {
+        const int ti = t % nx, tj = t / nx;
+        double acc = 0.0;
+        for (int s = 0; s < per; ++s)
+        {
+            const int si = s % nx, sj = s / nx;
+            acc += table[abs(tj - sj) * nx + abs(ti - si)] * src[s];
+        }
+        dst[t] = scale * acc;
+    }
 }

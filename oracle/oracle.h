@@ -189,6 +189,11 @@ int oracle_ib_apply(const oracle_tree* t, const oracle_ib* ib, const double* g, 
 int oracle_ib_cg(const oracle_tree* t, const oracle_ib* ib, const double* g, int nmax, const double* uc, double* f,
     double alpha, double tol, int maxit, double* residual);
 
+/* Screened 2-D LGF value (-G) at offset (i, j), trapezoid with `nodes`
+ * intervals on [0, pi]; and the direct convolution of one mode's patch. */
+double oracle_helm2d_lgf(double c, int i, int j, int nodes);
+void oracle_helm2d_convolve(const double* table, int nx, int ny, const double* src, double* dst, double scale);
+
 #ifdef __cplusplus
 }
 #endif

@@ -44,6 +44,9 @@ def lib():
             build()
         L = ctypes.CDLL(LIB_PATH)
         L.oracle_lgf_table.argtypes = [_I, _P]
+        L.oracle_helm2d_lgf.argtypes = [_D, _I, _I, _I]
+        L.oracle_helm2d_lgf.restype = _D
+        L.oracle_helm2d_convolve.argtypes = [_P, _I, _I, _P, _P, _D]
         L.oracle_lgf_get.argtypes = [_P, _I, _I, _I, _I]
         L.oracle_lgf_get.restype = _D
         L.oracle_lgf_asym.argtypes = [_I, _I, _I]
@@ -110,6 +113,30 @@ def ive(x, kmax):
     out = np.zeros(kmax + 1)
     lib().oracle_ive(float(x), int(kmax), _p(out))
     return out
+
+
+def helm2d_nodes(c, span):
+    """Trapezoid intervals the device table uses for screening c and offsets
+    n + m <= span: ceil(span + 45 / acosh(1 + c^2/2)) rounded up to 256."""
+    q = int(np.ceil(span + 45.0 / np.arccosh(1.0 + 0.5 * c * c)))
+    return (q + 255) & ~255
+
+
+def helm2d_table(c, nx, ny, nodes=None):
+    """Table [ny, nx] of the screened 2-D LGF values (-G) of screening c."""
+    q = helm2d_nodes(c, nx + ny) if nodes is None else int(nodes)
+    L = lib()
+    return np.array([[L.oracle_helm2d_lgf(float(c), i, j, q) for i in range(nx)] for j in range(ny)])
+
+
+def helm2d_convolve(table, src, scale=1.0):
+    """Direct convolution of one mode's patch src [ny, nx] with its table."""
+    table = np.ascontiguousarray(table, np.float64)
+    src = np.ascontiguousarray(src, np.float64)
+    ny, nx = src.shape
+    dst = np.zeros_like(src)
+    lib().oracle_helm2d_convolve(_p(table), nx, ny, _p(src), _p(dst), float(scale))
+    return dst
 
 
 def lagrange_matrix(n, cap, rp):
