@@ -393,6 +393,43 @@ def ifherk_bench(steps, log):
     return out
 
 
+def helm2d_bench(steps, log, n=256, nmodes=8):
+    """The screened 2-D LGF path of the spanwise-Fourier solvers: per mode the
+    table (iblgf_gpu_helm2d_table) and the direct free-space convolution
+    (iblgf_gpu_helm2d_convolve) over an n x n patch, nmodes modes with
+    screening c = 2 pi m / 64 / 8 (modes m = 1..nmodes of a 64-cell span at
+    dx = 1/8), each launch timed with HIP events on torch's stream."""
+    import numpy as np
+    import torch
+
+    from iblgf_amd import helm2d_convolve, helm2d_table
+
+    cs = 2.0 * np.pi * np.arange(1, nmodes + 1) / 64.0 / 8.0
+    g = torch.Generator(device="cuda").manual_seed(1005)
+    src = torch.randn((nmodes, n, n), dtype=torch.float64, device="cuda", generator=g)
+    tab, dst = torch.empty_like(src), torch.empty_like(src)
+    helm2d_table(cs, n, n, tab)
+    helm2d_convolve(tab, src, dst)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    for _ in range(steps):
+        helm2d_table(cs, n, n, tab)
+    ev[1].record()
+    for _ in range(steps):
+        helm2d_convolve(tab, src, dst)
+    ev[2].record()
+    torch.cuda.synchronize()
+    t_tab, t_conv = ev[0].elapsed_time(ev[1]) / steps, ev[1].elapsed_time(ev[2]) / steps
+    cells = n * n
+    flops = 2.0 * cells * cells * nmodes
+    log(f"helm2d {n}x{n} x {nmodes} modes: table {t_tab:.3f} ms, convolution {t_conv:.3f} ms")
+    return {"patch": [n, n], "modes": nmodes, "screenings": cs.tolist(), "table_ms": t_tab,
+            "convolve_ms": t_conv, "convolve_tflops": flops / t_conv / 1e9,
+            "convolve_frac_fp64_peak": flops / t_conv / 1e9 / 78.6,
+            "target_cells_per_s": cells * nmodes / (t_conv / 1e3)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -406,6 +443,8 @@ def main():
     ap.add_argument("--forest-only", action="store_true", help="run only the 64-tree batched time-step workload")
     ap.add_argument("--no-ifherk", action="store_true", help="skip the IF-HERK time-step workload")
     ap.add_argument("--ifherk-only", action="store_true", help="run only the IF-HERK time-step workload")
+    ap.add_argument("--helm2d-only", action="store_true",
+                    help="run only the screened 2-D LGF table and convolution of the Fourier modes")
     ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph",
                                        "fp16x3forward_batched", "int8x4_batched", "fp16x3_batched", "bf16x3_batched",
                                        "fp16_batched"],
@@ -423,6 +462,12 @@ def main():
             print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
         print(json.dumps({"ifherk_step": ifherk_bench(args.steps, hlog)}))
+        return 0
+    if args.helm2d_only:
+        def h2log(msg):
+            print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+        print(json.dumps({"helm2d": helm2d_bench(args.steps, h2log)}))
         return 0
     if args.forest_only:
         def flog(msg):
