@@ -108,6 +108,57 @@ __global__ __launch_bounds__(kTile) void k_helm2d_convolve(const double* __restr
     if (t < per) dst[off + t] = scale * acc;
 }
 
+// The same sums for patches 64 to 1024 cells wide: one workgroup per target
+// row and mode, each thread up to four targets of the row; per source row the
+// table row |tj - sj| and the source row are staged in LDS, so every
+// multiply-add reads LDS only (the source value as a broadcast).  Sources are
+// summed in index order, as above.
+constexpr int kRowMin = 64, kRowMax = 4 * kTile;
+
+__global__ __launch_bounds__(kTile) void k_helm2d_convolve_rows(const double* __restrict__ table, int nx, int ny,
+    const double* __restrict__ src, double* __restrict__ dst, double scale)
+{
+    __shared__ double trow[kRowMax];
+    __shared__ double srow[kRowMax];
+    const long        off = (long)blockIdx.y * nx * ny;
+    const int         tj = blockIdx.x;
+    const double*     tab = table + off;
+    const double*     f = src + off;
+    double            acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int sj = 0; sj < ny; ++sj)
+    {
+        const int dy = tj > sj ? tj - sj : sj - tj;
+        for (int x = threadIdx.x; x < nx; x += kTile)
+        {
+            trow[x] = tab[(long)dy * nx + x];
+            srow[x] = f[(long)sj * nx + x];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+        {
+            const int ti = threadIdx.x + r * kTile;
+            if (ti < nx)
+            {
+                double a = acc[r];
+                for (int si = 0; si < nx; ++si)
+                {
+                    const int dx = ti > si ? ti - si : si - ti;
+                    a += trow[dx] * srow[si];
+                }
+                acc[r] = a;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+    {
+        const int ti = threadIdx.x + r * kTile;
+        if (ti < nx) dst[off + (long)tj * nx + ti] = scale * acc[r];
+    }
+}
+
 void check_shape(const char* who, int nmodes, int nx, int ny)
 {
     if (nmodes < 1 || nmodes > kHelmMaxModes)
@@ -157,8 +208,12 @@ int iblgf_gpu_helm2d_convolve(
         if (!table || !src || !dst) throw std::invalid_argument("iblgf_gpu_helm2d_convolve: null pointer");
         if (src == dst) throw std::invalid_argument("iblgf_gpu_helm2d_convolve: source and target must differ");
         const int per = nx * ny;
-        hipLaunchKernelGGL(k_helm2d_convolve, dim3((unsigned)((per + kTile - 1) / kTile), (unsigned)nmodes),
-            dim3(kTile), 0, (hipStream_t)stream, table, nx, ny, src, dst, scale);
+        if (nx >= kRowMin && nx <= kRowMax)
+            hipLaunchKernelGGL(k_helm2d_convolve_rows, dim3((unsigned)ny, (unsigned)nmodes), dim3(kTile), 0,
+                (hipStream_t)stream, table, nx, ny, src, dst, scale);
+        else
+            hipLaunchKernelGGL(k_helm2d_convolve, dim3((unsigned)((per + kTile - 1) / kTile), (unsigned)nmodes),
+                dim3(kTile), 0, (hipStream_t)stream, table, nx, ny, src, dst, scale);
         check_launch("iblgf_gpu_helm2d_convolve");
     });
 }

@@ -45,6 +45,23 @@ def test_convolution_matches_oracle_per_mode():
         assert np.linalg.norm(got[k] - ref) <= 1e-12 * np.linalg.norm(ref), k
 
 
+@pytest.mark.parametrize("nx,ny", [(64, 3), (72, 5), (300, 2), (1024, 1), (1025, 1)])
+def test_row_staged_widths_match_oracle(nx, ny):
+    print(f"case: convolution of 2 modes on a {nx} x {ny} patch (row-staged widths and the edges)", flush=True)
+    cs = np.array([0.08, 1.5])
+    rng = np.random.default_rng(nx + ny)
+    src = rng.standard_normal((2, ny, nx))
+    t = torch.empty((2, ny, nx), dtype=torch.float64, device="cuda")
+    helm2d_table(cs, nx, ny, t)
+    s = torch.from_numpy(src).cuda()
+    d = torch.empty_like(s)
+    helm2d_convolve(t, s, d, -1.25)
+    got, tab = d.cpu().numpy(), t.cpu().numpy()
+    for k in range(2):
+        ref = O.helm2d_convolve(tab[k], src[k], -1.25)
+        assert np.linalg.norm(got[k] - ref) <= 1e-12 * np.linalg.norm(ref), k
+
+
 def test_full_patch_satisfies_screened_poisson():
     print("case: 8 modes on a 256 x 256 patch, (Lap_h - c^2) u = src", flush=True)
     nx = ny = 256
