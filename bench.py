@@ -443,6 +443,8 @@ def main():
     ap.add_argument("--forest-only", action="store_true", help="run only the 64-tree batched time-step workload")
     ap.add_argument("--no-ifherk", action="store_true", help="skip the IF-HERK time-step workload")
     ap.add_argument("--ifherk-only", action="store_true", help="run only the IF-HERK time-step workload")
+    ap.add_argument("--no-helm2d", action="store_true",
+                    help="skip the screened 2-D LGF table and convolution of the Fourier modes")
     ap.add_argument("--helm2d-only", action="store_true",
                     help="run only the screened 2-D LGF table and convolution of the Fourier modes")
     ap.add_argument("--only", choices=["fp64", "fp64_batched", "mixed", "mixed_batched", "mixed_batched_graph",
@@ -775,6 +777,8 @@ def main():
         result["config"]["forest_step"] = forest_step(max(2, args.steps // 5), log)
     if not args.no_ifherk:
         result["config"]["ifherk_step"] = ifherk_bench(max(2, args.steps // 5), log)
+    if not args.no_helm2d:
+        result["config"]["helm2d"] = helm2d_bench(5, log)
     if rank == 0 and not args.no_cpu_baseline:
         # counts cover the FMM passes of one Laplace solve (each pass plan is built once)
         result["cpu_baseline"] = cpu_baseline(wl, srcs[0], counts, args.cpu_seconds)
